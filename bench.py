@@ -1,0 +1,182 @@
+"""Benchmark of the wavefront hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload sorting_stress]
+
+A step is one full frame of the plain renderer (WavefrontRenderer::renderScene): every sample
+pass of every pixel through every bounce (ray sort, closest hit + split, shading-queue policy
+sort, shade with NEE shadow rays, misses, adaptive bookkeeping). The default workload is the
+material/texture sorting stress scene: a 1.31M-triangle noise-displaced icosphere with 64
+materials and 32 1024^2 checker textures, 1920x1080, 64 spp, depth 12, rendered with each of
+the four scheduling policies. Scene upload and BVH build happen before timing; the timed region
+is bracketed by device synchronisation. Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def workload(name: str, wf, scenes, quick: bool):
+    if name == "sorting_stress":
+        data = scenes.sorting_stress(subdiv=6 if quick else 8, tex_size=256 if quick else 1024)
+        W, H = (480, 270) if quick else (1920, 1080)
+        spp, depth = (4 if quick else 64), 12
+        desc = (f"sorting_stress: noise-displaced icosphere subdiv {8 if not quick else 6} "
+                f"({data.triangle_count} tris), 64 materials (40% Lambertian / 40% GGX / 20% "
+                f"dielectric), 32 checker textures, 2 area lights, {W}x{H}, {spp} spp, depth {depth}")
+        return data, W, H, spp, depth, desc
+    if name == "cornell_box":
+        data = scenes.cornell_box()
+        return data, 256, 256, 16, 8, "cornell_box: 5132 tris, 256x256, 16 spp, depth 8"
+    raise SystemExit(f"unknown workload {name}")
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="sorting_stress")
+    ap.add_argument("--policy", default="none")
+    ap.add_argument("--quick", action="store_true", help="reduced sizes (CPU-side checks)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-policies", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    wf = importlib.import_module("wavefront-renderer_amd")
+    scenes = importlib.import_module("wavefront-renderer_amd.scenes")
+    t0 = time.time()
+    data, W, H, spp, depth, desc = workload(args.workload, wf, scenes, args.quick)
+    log(f"[bench] scene generated in {time.time() - t0:.1f}s: {desc}")
+    ctx = wf.Context(0)
+    t0 = time.time()
+    scene = wf.Scene(ctx, data)
+    log(f"[bench] uploaded + BVH built in {time.time() - t0:.1f}s "
+        f"(device build {scene.stats.build_ms:.1f} ms, depth {scene.stats.max_depth})")
+    physics = wf.PHYSICS_UNBIASED if args.workload == "sorting_stress" else wf.PHYSICS_REFERENCE
+
+    def params(policy, seed=1):
+        return wf.render_params(width=W, height=H, spp=spp, max_depth=depth,
+                                camera=data.camera, policy=policy, physics=physics,
+                                firefly=1e30, seed=seed)
+
+    def frame(policy, seed=1):
+        img, cnt, st = wf.render(scene, params(policy, seed), keep_on_device=True)
+        return st
+
+    for w in range(args.warmup):
+        st = frame(args.policy, seed=100 + w)
+        log(f"[bench] warmup {w}: {st.frame_ms:.1f} ms")
+    ctx.synchronize()
+    rays = 0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        st = frame(args.policy, seed=1 + k)
+        rays += st.rays_traced + st.shadow_rays
+        log(f"[bench] step {k}: {st.frame_ms:.1f} ms device, {st.rays_traced} rays + "
+            f"{st.shadow_rays} shadow rays")
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    mrays = rays / wall / 1e6
+    ms_per_step = wall * 1e3 / args.steps
+
+    policies = {}
+    if not args.no_policies:
+        for pol in ("none", "material", "texture", "costBenefit"):
+            t1 = time.perf_counter()
+            st = frame(pol, seed=7)
+            ctx.synchronize()
+            dt = time.perf_counter() - t1
+            policies[pol] = {"mrays_s": round((st.rays_traced + st.shadow_rays) / dt / 1e6, 2),
+                             "ms": round(dt * 1e3, 1), "sort_ms": round(st.sort_ms, 1),
+                             "intersect_ms": round(st.intersect_ms, 1),
+                             "schedule_ms": round(st.schedule_ms, 1),
+                             "shade_ms": round(st.shade_ms, 1)}
+            log(f"[bench] policy {pol}: {policies[pol]}")
+
+    # per-kernel device time and algorithmic bytes over one profiled frame
+    ctx.profile(True)
+    st = frame(args.policy, seed=3)
+    kernels = {}
+    for name in ctx.profile_names():
+        ms, launches, nbytes = ctx.profile_query(name)
+        if launches == 0:
+            continue
+        gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        kernels[name] = {"ms": round(ms, 3), "launches": launches,
+                         "avg_us": round(ms * 1e3 / launches, 2), "GB_s": round(gbs, 1),
+                         "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    ctx.profile(False)
+    dom = max(kernels, key=lambda k: kernels[k]["ms"]) if kernels else None
+    roofline = None
+    if dom:
+        kd = kernels[dom]
+        roofline = {"kernel": dom, "bound": "hbm", "achieved": kd["GB_s"], "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": kd["frac"], "traffic": None}
+    log(f"[bench] kernels: {json.dumps(kernels)}")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, str(ROOT / "tests"))
+        import oracle_lib  # CPU baseline leg only
+
+        threads = min(16, os.cpu_count() or 1)
+        osc = oracle_lib.Oracle().scene(data)
+        cw, ch = max(W // 4, 2), max(H // 4, 2)
+        cp = wf.render_params(width=cw, height=ch, spp=1, max_depth=depth, camera=data.camera,
+                              policy=args.policy, physics=physics, firefly=1e30, seed=1)
+        t1 = time.perf_counter()
+        _, _, _, cst = osc.render(cp, threads=threads)
+        dt = time.perf_counter() - t1
+        cpu = {"value": round((cst.rays_traced + cst.shadow_rays) / dt / 1e6, 3),
+               "unit": "Mrays/s", "cores": threads, "kind": "port",
+               "sample": f"same scene at {cw}x{ch}, 1 spp, depth {depth}, policy {args.policy}, "
+                         f"{cst.rays_traced + cst.shadow_rays} rays in {dt:.1f}s (oracle restatement)"}
+        log(f"[bench] cpu baseline: {cpu}")
+
+    out = {
+        "metric": "Mrays/s @1080p depth 12",
+        "value": round(mrays, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": desc, "policy": args.policy, "physics":
+                   "unbiased" if physics else "reference", "triangles": data.triangle_count,
+                   "bvh_build_ms": round(scene.stats.build_ms, 1), "policies": policies},
+        "roofline": roofline,
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,341 @@
+/*
+ * wavefront_hip.h -- C-ABI of the MI355X (gfx950) wavefront path-tracing hot path.
+ *
+ * The reference renderer (fhidargue/wavefront-renderer) has no FFI: its seam is the set of C++
+ * class surfaces that WavefrontRenderer::renderScene drives once per bounce. Every entry point
+ * below replaces one of those surfaces; the comment on each names the reference interface
+ * (file:line in the reference tree) it stands in for.
+ *
+ * Conventions
+ *  - Every entry point returns wf_status (WF_OK == 0). On failure wf_last_error() returns a
+ *    thread-local message describing the failing call.
+ *  - Queue buffers (wf_ray_soa, wf_shade_soa, index arrays, key arrays) are caller-owned DEVICE
+ *    pointers. The library never frees them. Scene/context handles are library-owned and have an
+ *    explicit destroy, like the reference's move-only Embree handles (include/accelerator/Embree.h:16-22).
+ *  - Calls are stream-ordered on the context's HIP stream unless stated otherwise; functions
+ *    that return a count to the host synchronise that stream.
+ *  - Scene data is copied on wf_scene_create, so there is no back-pointer hazard
+ *    (the reference keeps a raw Scene* inside the accelerator, Embree.h:29).
+ *  - Calls that share one context must not overlap (one in-flight frame per context), as the
+ *    reference renderer is not re-entrant (include/render/WavefrontRenderer.h:46-48).
+ */
+#ifndef WAVEFRONT_HIP_H
+#define WAVEFRONT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t wf_status;
+#define WF_OK 0
+#define WF_ERR_INVALID_ARGUMENT 1
+#define WF_ERR_HIP 2
+#define WF_ERR_OUT_OF_MEMORY 3
+#define WF_ERR_CAPACITY 4
+#define WF_ERR_NOT_BUILT 5
+#define WF_ERR_UNSUPPORTED 6
+#define WF_ERR_ADMISSION 7
+
+/* ---- material / policy enums (values are the reference's enum order) -------------------- */
+
+/* MaterialType, include/shading/Material.h:9-17. The CostBenefit policy orders by this value. */
+#define WF_MAT_DIFFUSE 0
+#define WF_MAT_METAL 1
+#define WF_MAT_EMISSIVE 2
+#define WF_MAT_SPOTLIGHT 3
+#define WF_MAT_GLASS 4
+#define WF_MAT_PLASTIC 5
+
+/* BSDF model of a material: the reference's own BSDFs (Material.cpp:165-266) or the microfacet
+ * models (GGX conductor for METAL, exact-Fresnel smooth dielectric for GLASS). */
+#define WF_MODEL_REFERENCE 0
+#define WF_MODEL_MICROFACET 1
+
+/* SchedulingPolicy, include/scheduling/ShadingQueue.h:18-24. */
+#define WF_POLICY_NONE 0
+#define WF_POLICY_MATERIAL 1
+#define WF_POLICY_TEXTURE 2
+#define WF_POLICY_COST_BENEFIT 3
+
+/* Light transport track.
+ * REFERENCE: balance-heuristic MIS, emitter mesh then triangle picked uniformly with pdf 1/A_total
+ *            (src/core/Scene.cpp:192-237, include/shading/MIS.h:14-31), nearest-texel environment.
+ * UNBIASED:  power-heuristic MIS, area*power alias table over emissive triangles, environment map
+ *            importance-sampled through a luminance alias table with environment MIS, NEE for
+ *            every non-delta BSDF. */
+#define WF_PHYSICS_REFERENCE 0
+#define WF_PHYSICS_UNBIASED 1
+
+/* Texture texel formats. RGBA8_SRGB texels are decoded with the reference's sRGB curve
+ * (src/shading/Texture.cpp:20-28) through a 256-entry table; RGB_F32 texels are used as is. */
+#define WF_TEX_RGBA8_SRGB 0
+#define WF_TEX_RGB_F32 1
+
+/* ---- scene description (mirrors Scene::addMaterial/addTexture/addMesh, Scene.cpp:5-56) ---- */
+
+typedef struct wf_material {
+    int32_t type;              /* WF_MAT_* */
+    float albedo[3];
+    float roughness;
+    float emission;
+    int32_t texture_id;        /* -1 = none */
+    float spot_outer_deg;
+    float spot_falloff_deg;
+    float ior;
+    int32_t use_spatial_checker;
+    float checker_cell_size;
+    int32_t model;             /* WF_MODEL_* */
+} wf_material;
+
+typedef struct wf_texture_desc {
+    int32_t width;
+    int32_t height;
+    int32_t format;            /* WF_TEX_* */
+    const void* data;          /* host pointer: width*height*4 bytes (RGBA8) or *3 floats */
+} wf_texture_desc;
+
+typedef struct wf_mesh_desc {
+    const float* positions;    /* host, xyz interleaved, vertex_count*3 */
+    int32_t vertex_count;
+    const int32_t* indices;    /* host, 3 per triangle, mesh-local vertex indices */
+    int32_t triangle_count;
+    const float* uv_u;         /* host, face-varying, 3 per triangle, or NULL (Mesh.h:8-20) */
+    const float* uv_v;
+    int32_t material_id;
+} wf_mesh_desc;
+
+typedef struct wf_directional_light {   /* DirectionalLight, include/core/Scene.h:57-62 */
+    float direction[3];
+    float color[3];
+    float intensity;           /* 0 = absent */
+} wf_directional_light;
+
+typedef struct wf_environment_desc {    /* EnvironmentMap, include/core/EnvironmentMap.h */
+    int32_t width;             /* 0 = no map: the reference's sky gradient is used */
+    int32_t height;
+    const float* rgb;          /* host, width*height*3 floats, equirectangular */
+    float intensity;
+    float tint[3];
+} wf_environment_desc;
+
+typedef struct wf_scene_desc {
+    const wf_mesh_desc* meshes;           /* geomID == array position (attach order, Embree.cpp:77-108) */
+    int32_t mesh_count;
+    const wf_material* materials;
+    int32_t material_count;
+    const wf_texture_desc* textures;
+    int32_t texture_count;
+    wf_directional_light directional;
+    wf_environment_desc environment;
+} wf_scene_desc;
+
+/* ---- queues (structure of arrays, field order of RayQueue.h:66-90 and ShadingQueue.h:29-53) - */
+
+typedef struct wf_ray_soa {       /* RayQueue */
+    float* ox; float* oy; float* oz;
+    float* dx; float* dy; float* dz;
+    float* tr; float* tg; float* tb;   /* throughput */
+    int32_t* pixel;
+    int32_t* depth;
+    float* pdf;
+} wf_ray_soa;
+
+typedef struct wf_shade_soa {     /* ShadingQueue, plus the geomID and front-face bit it drops */
+    float* px; float* py; float* pz;   /* hit point */
+    float* nx; float* ny; float* nz;   /* normal facing the ray */
+    float* t;
+    float* u; float* v;                /* interpolated UV, or barycentrics without UVs */
+    int32_t* tri;                      /* primID within its mesh */
+    int32_t* mat;
+    int32_t* tex;
+    float* ox; float* oy; float* oz;
+    float* dx; float* dy; float* dz;
+    float* tr; float* tg; float* tb;
+    int32_t* pixel;
+    int32_t* depth;
+    float* pdf;
+    int32_t* geom;                     /* geomID (mesh index) */
+    uint8_t* front;                    /* HitRecord::frontFace (HitRecord.h:15) */
+} wf_shade_soa;
+
+/* Per-ray closest-hit result (Embree RTCHit in the reference). prim < 0 means miss. */
+typedef struct wf_hit_soa {
+    float* t;
+    int32_t* geom;                     /* geomID */
+    int32_t* prim;                     /* primID within the mesh */
+    float* bu; float* bv;              /* barycentrics of vertex 1 and 2 */
+} wf_hit_soa;
+
+/* ---- context and device memory ------------------------------------------------------------ */
+
+const char* wf_last_error(void);
+const char* wf_version(void);
+
+typedef struct wf_context wf_context;
+typedef struct wf_scene wf_scene;
+
+wf_status wf_context_create(int32_t device, wf_context** out);
+wf_status wf_context_destroy(wf_context* ctx);
+/* The context's HIP stream (hipStream_t) every stream-ordered call below is issued on. */
+wf_status wf_context_stream(wf_context* ctx, void** out_stream);
+wf_status wf_device_alloc(wf_context* ctx, size_t bytes, void** out);
+wf_status wf_device_free(wf_context* ctx, void* ptr);
+wf_status wf_memcpy_htod(wf_context* ctx, void* dst, const void* src, size_t bytes);
+wf_status wf_memcpy_dtoh(wf_context* ctx, void* dst, const void* src, size_t bytes);
+wf_status wf_memcpy_dtod(wf_context* ctx, void* dst, const void* src, size_t bytes);
+wf_status wf_fill_u32(wf_context* ctx, void* dst, uint32_t value, size_t count);
+wf_status wf_synchronize(wf_context* ctx);
+
+/* ---- scene and accelerator ------------------------------------------------------------------- */
+
+typedef struct wf_build_stats {   /* EmbreeAccelerator m_meshCount/m_totalTriangles/... (Embree.h:31-35) */
+    int32_t mesh_count;
+    int32_t triangle_count;
+    int32_t vertex_count;
+    int32_t node_count;
+    int32_t max_depth;
+    double build_ms;              /* device time of the BVH build kernels */
+    float bounds_min[3];          /* Scene::computeBounds over all vertices (Scene.cpp:59-77) */
+    float bounds_max[3];
+} wf_build_stats;
+
+/* Scene::addMesh/addMaterial/addTexture + upload. Copies everything to the device. */
+wf_status wf_scene_create(wf_context* ctx, const wf_scene_desc* desc, wf_scene** out);
+wf_status wf_scene_destroy(wf_scene* scene);
+/* Scene::buildAccelerator + EmbreeAccelerator::build (Scene.cpp:79-113, Embree.cpp:66-119):
+ * GPU LBVH over the flattened triangle soup, emitter tables and bounds. */
+wf_status wf_accel_build(wf_scene* scene, wf_build_stats* stats);
+/* Device bytes the scene and its accelerator hold (admission accounting). */
+wf_status wf_scene_device_bytes(const wf_scene* scene, size_t* out);
+
+/* EmbreeAccelerator::intersect4 / intersect over a whole batch (Embree.cpp:122-256): closest hit
+ * for rays [0,n) with tnear/tfar; writes the raw hit per ray (prim < 0 = miss). */
+wf_status wf_intersect(const wf_scene* scene, const wf_ray_soa* rays, int32_t n, float tnear,
+                       float tfar, wf_hit_soa* hits);
+/* EmbreeAccelerator::occluded (Embree.cpp:258-279) for n shadow rays: occluded[i] = 1 when any
+ * triangle lies in [0.001, max_dist[i]-0.001] (and, as in the reference, whenever that upper
+ * bound is negative). Inputs are device arrays. */
+wf_status wf_occluded(const wf_scene* scene, const float* ox, const float* oy, const float* oz,
+                      const float* dx, const float* dy, const float* dz, const float* max_dist,
+                      int32_t n, uint8_t* occluded);
+/* WavefrontRenderer::intersectAll (src/render/WavefrontRenderer.cpp:459-567): closest hit, then a
+ * stable split into a ShadingQueue (HitRecord mapping of Embree.cpp:240-252) and a miss RayQueue,
+ * both in ascending input order. Counts are returned to the host. */
+wf_status wf_intersect_split(const wf_scene* scene, const wf_ray_soa* rays, int32_t n,
+                             wf_shade_soa* hits_out, int32_t* n_hits, wf_ray_soa* misses_out,
+                             int32_t* n_misses);
+
+/* ---- scheduling ---------------------------------------------------------------------------- */
+
+/* RayQueue::sortKey + schedule (include/scheduling/RayQueue.h:135-179): 33-bit octant+Morton
+ * key per ray against the given bounds, then a stable sort; sorted_indices[i] is the input index
+ * at sorted position i. keys_out (optional, may be NULL) receives the unsorted keys. */
+wf_status wf_ray_schedule(wf_context* ctx, const wf_ray_soa* rays, int32_t n,
+                          const float bounds_min[3], const float bounds_max[3],
+                          int32_t* sorted_indices, uint64_t* keys_out);
+/* RayQueue::materialize (RayQueue.h:182-215), out of place: dst[i] = src[idx[i]]. */
+wf_status wf_materialize_rays(wf_context* ctx, const wf_ray_soa* src, const int32_t* idx,
+                              int32_t n, wf_ray_soa* dst);
+
+/* Cost tables for the Material/CostBenefit policies and cost-aware RR: relativeCost per material
+ * and per texture (CostTracker::relativeCost, include/render/CostTracker.h:44-54). NULL restores
+ * the neutral tables (every cost 1.0), which is what a CostTracker with no samples returns. */
+wf_status wf_scene_set_costs(wf_scene* scene, const double* material_rel_cost,
+                             const double* texture_rel_cost);
+
+/* ShadingQueue::schedule (ShadingQueue.h:90-141, 277-329, 437-550) for the four policies.
+ * sorted_indices[i] = queue index shaded at position i. Ties the reference leaves to an unstable
+ * sort are broken by ascending queue index. */
+wf_status wf_shading_schedule(const wf_scene* scene, const wf_shade_soa* queue, int32_t n,
+                              int32_t policy, int32_t* sorted_indices);
+/* ShadingQueue::materialize (ShadingQueue.h:233-275), out of place. */
+wf_status wf_materialize_shading(wf_context* ctx, const wf_shade_soa* src, const int32_t* idx,
+                                 int32_t n, wf_shade_soa* dst);
+
+/* ---- shading --------------------------------------------------------------------------------- */
+
+typedef struct wf_shade_params {
+    int32_t physics;           /* WF_PHYSICS_* */
+    int32_t rr_min_depth;      /* reference: always 3 (constructor argument ignored) */
+    int32_t cost_aware_rr;     /* enableCostAwareRR */
+    float firefly_threshold;   /* reference default 10 */
+    uint32_t seed;
+    int32_t sample_index;      /* RNG counter: sample pass */
+    int32_t job;               /* RNG counter: job tag (0 in the plain form) */
+} wf_shade_params;
+
+/* WavefrontRenderer::shadeAll (cpp:569-877) for one bounce: emission+MIS and NEE into the
+ * per-pixel accumulator (float RGB, pixel-indexed), shadow rays resolved with any-hit, scatter and
+ * Russian roulette, and the next RayQueue appended in shading order. n_next returned. */
+wf_status wf_shade(const wf_scene* scene, const wf_shade_soa* queue, int32_t n,
+                   const wf_shade_params* params, float* accum_rgb, wf_ray_soa* next,
+                   int32_t* n_next);
+/* The miss loop (cpp:256-265) with getSkyColor (cpp:879-889): accum[pixel] += T * sky(dir)
+ * (WF_PHYSICS_UNBIASED adds environment MIS for rays leaving non-delta bounces). */
+wf_status wf_shade_misses(const wf_scene* scene, const wf_ray_soa* misses, int32_t n,
+                          int32_t physics, float* accum_rgb);
+
+/* ---- whole-frame render (WavefrontRenderer::renderScene, cpp:94-399) --------------------- */
+
+typedef struct wf_camera {        /* Camera(lookFrom, lookAt, up, vfov, w, h), include/core/Camera.h:13-33 */
+    float eye[3];
+    float target[3];
+    float up[3];
+    float vfov_deg;
+} wf_camera;
+
+typedef struct wf_render_params {
+    int32_t width;
+    int32_t height;
+    int32_t spp;               /* samplesPerPixel (a budget when adaptive) */
+    int32_t max_depth;
+    int32_t policy;            /* WF_POLICY_* */
+    int32_t ray_sort;          /* enableRaySort */
+    int32_t adaptive;          /* enableAdaptiveSampling */
+    float adaptive_threshold;  /* AdaptiveSampler relative threshold (reference default 0.05) */
+    int32_t adaptive_min_samples;   /* reference: 32 */
+    int32_t adaptive_interval;      /* reference: 16 */
+    wf_shade_params shade;
+    wf_camera camera;
+} wf_render_params;
+
+typedef struct wf_render_stats {  /* the reference's stats blocks (cpp:321-370) plus ray counts */
+    int64_t rays_traced;       /* sum over bounces of rays entering the bounce */
+    int64_t shadow_rays;
+    int64_t shaded_hits;       /* "Total shaded hits" */
+    int64_t misses;
+    int32_t samples_run;       /* "Samples actually run" */
+    int32_t active_pixels_end; /* "Pixels still active at end" */
+    double frame_ms;           /* device time raygen .. final resolve */
+    double sort_ms;            /* ray sort + materialize */
+    double intersect_ms;       /* traversal + hit/miss split */
+    double schedule_ms;        /* shading-queue policy sort + materialize */
+    double shade_ms;           /* shade + shadow resolve */
+    double miss_ms;
+} wf_render_stats;
+
+/* Renders one image. image_rgb: caller-owned device float buffer width*height*3 receiving
+ * accumulator / per-pixel sample count (writeAveragedImage, cpp:127-137). sample_counts
+ * (optional device int32[width*height]) receives the per-pixel sample counts. */
+wf_status wf_render(wf_scene* scene, const wf_render_params* params, float* image_rgb,
+                    int32_t* sample_counts, wf_render_stats* stats);
+
+/* ---- per-kernel device timing (HIP events on the context stream) ------------------------ */
+
+/* Enables/disables event timing around every library kernel launch and clears the totals. */
+wf_status wf_profile_enable(wf_context* ctx, int32_t enable);
+/* Totals for a kernel name ("ray_keys", "radix_pass", "traverse_closest", "split", ...):
+ * summed device milliseconds, launch count and algorithmic HBM bytes accounted to it. */
+wf_status wf_profile_query(wf_context* ctx, const char* kernel, double* total_ms,
+                           int64_t* launches, double* algorithmic_bytes);
+/* Newline-separated list of profiled kernel names into buf (truncated to cap). */
+wf_status wf_profile_names(wf_context* ctx, char* buf, int32_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WAVEFRONT_HIP_H */

@@ -1,0 +1,206 @@
+"""ctypes mirror of include/wavefront_hip.h (structures, constants and prototypes).
+
+The structures are plain data mirrors of the C header; `load_library()` binds every entry point
+of libwavefront_hip.so and raises if the library is missing -- there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "lib" / "libwavefront_hip.so"
+HEADER = HERE.parent / "include" / "wavefront_hip.h"
+
+WF_OK = 0
+WF_ERR_INVALID_ARGUMENT = 1
+WF_ERR_HIP = 2
+WF_ERR_OUT_OF_MEMORY = 3
+WF_ERR_CAPACITY = 4
+WF_ERR_NOT_BUILT = 5
+WF_ERR_UNSUPPORTED = 6
+WF_ERR_ADMISSION = 7
+
+MAT_DIFFUSE, MAT_METAL, MAT_EMISSIVE, MAT_SPOTLIGHT, MAT_GLASS, MAT_PLASTIC = range(6)
+MODEL_REFERENCE, MODEL_MICROFACET = 0, 1
+POLICY_NONE, POLICY_MATERIAL, POLICY_TEXTURE, POLICY_COST_BENEFIT = range(4)
+POLICIES = {"none": POLICY_NONE, "material": POLICY_MATERIAL, "texture": POLICY_TEXTURE,
+            "costBenefit": POLICY_COST_BENEFIT}
+PHYSICS_REFERENCE, PHYSICS_UNBIASED = 0, 1
+TEX_RGBA8_SRGB, TEX_RGB_F32 = 0, 1
+
+f3 = C.c_float * 3
+P = C.c_void_p
+
+
+class Material(C.Structure):
+    _fields_ = [("type", C.c_int32), ("albedo", f3), ("roughness", C.c_float),
+                ("emission", C.c_float), ("texture_id", C.c_int32),
+                ("spot_outer_deg", C.c_float), ("spot_falloff_deg", C.c_float),
+                ("ior", C.c_float), ("use_spatial_checker", C.c_int32),
+                ("checker_cell_size", C.c_float), ("model", C.c_int32)]
+
+
+class TextureDesc(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("format", C.c_int32),
+                ("data", P)]
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("positions", P), ("vertex_count", C.c_int32), ("indices", P),
+                ("triangle_count", C.c_int32), ("uv_u", P), ("uv_v", P),
+                ("material_id", C.c_int32)]
+
+
+class DirectionalLight(C.Structure):
+    _fields_ = [("direction", f3), ("color", f3), ("intensity", C.c_float)]
+
+
+class EnvironmentDesc(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("rgb", P),
+                ("intensity", C.c_float), ("tint", f3)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("meshes", C.POINTER(MeshDesc)), ("mesh_count", C.c_int32),
+                ("materials", C.POINTER(Material)), ("material_count", C.c_int32),
+                ("textures", C.POINTER(TextureDesc)), ("texture_count", C.c_int32),
+                ("directional", DirectionalLight), ("environment", EnvironmentDesc)]
+
+
+RAY_FIELDS = ["ox", "oy", "oz", "dx", "dy", "dz", "tr", "tg", "tb", "pixel", "depth", "pdf"]
+RAY_DTYPES = {k: ("i4" if k in ("pixel", "depth") else "f4") for k in RAY_FIELDS}
+SHADE_FIELDS = ["px", "py", "pz", "nx", "ny", "nz", "t", "u", "v", "tri", "mat", "tex",
+                "ox", "oy", "oz", "dx", "dy", "dz", "tr", "tg", "tb", "pixel", "depth", "pdf",
+                "geom", "front"]
+SHADE_DTYPES = {k: ("i4" if k in ("tri", "mat", "tex", "pixel", "depth", "geom") else
+                    ("u1" if k == "front" else "f4")) for k in SHADE_FIELDS}
+HIT_FIELDS = ["t", "geom", "prim", "bu", "bv"]
+HIT_DTYPES = {"t": "f4", "geom": "i4", "prim": "i4", "bu": "f4", "bv": "f4"}
+
+
+class RaySoA(C.Structure):
+    _fields_ = [(k, P) for k in RAY_FIELDS]
+
+
+class ShadeSoA(C.Structure):
+    _fields_ = [(k, P) for k in SHADE_FIELDS]
+
+
+class HitSoA(C.Structure):
+    _fields_ = [(k, P) for k in HIT_FIELDS]
+
+
+class BuildStats(C.Structure):
+    _fields_ = [("mesh_count", C.c_int32), ("triangle_count", C.c_int32),
+                ("vertex_count", C.c_int32), ("node_count", C.c_int32),
+                ("max_depth", C.c_int32), ("build_ms", C.c_double),
+                ("bounds_min", f3), ("bounds_max", f3)]
+
+
+class ShadeParams(C.Structure):
+    _fields_ = [("physics", C.c_int32), ("rr_min_depth", C.c_int32),
+                ("cost_aware_rr", C.c_int32), ("firefly_threshold", C.c_float),
+                ("seed", C.c_uint32), ("sample_index", C.c_int32), ("job", C.c_int32)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("eye", f3), ("target", f3), ("up", f3), ("vfov_deg", C.c_float)]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32),
+                ("max_depth", C.c_int32), ("policy", C.c_int32), ("ray_sort", C.c_int32),
+                ("adaptive", C.c_int32), ("adaptive_threshold", C.c_float),
+                ("adaptive_min_samples", C.c_int32), ("adaptive_interval", C.c_int32),
+                ("shade", ShadeParams), ("camera", Camera)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("rays_traced", C.c_int64), ("shadow_rays", C.c_int64),
+                ("shaded_hits", C.c_int64), ("misses", C.c_int64),
+                ("samples_run", C.c_int32), ("active_pixels_end", C.c_int32),
+                ("frame_ms", C.c_double), ("sort_ms", C.c_double),
+                ("intersect_ms", C.c_double), ("schedule_ms", C.c_double),
+                ("shade_ms", C.c_double), ("miss_ms", C.c_double)]
+
+
+_S = C.c_int32
+_PROTOS = {
+    "wf_last_error": (C.c_char_p, []),
+    "wf_version": (C.c_char_p, []),
+    "wf_context_create": (_S, [C.c_int32, C.POINTER(P)]),
+    "wf_context_destroy": (_S, [P]),
+    "wf_context_stream": (_S, [P, C.POINTER(P)]),
+    "wf_device_alloc": (_S, [P, C.c_size_t, C.POINTER(P)]),
+    "wf_device_free": (_S, [P, P]),
+    "wf_memcpy_htod": (_S, [P, P, P, C.c_size_t]),
+    "wf_memcpy_dtoh": (_S, [P, P, P, C.c_size_t]),
+    "wf_memcpy_dtod": (_S, [P, P, P, C.c_size_t]),
+    "wf_fill_u32": (_S, [P, P, C.c_uint32, C.c_size_t]),
+    "wf_synchronize": (_S, [P]),
+    "wf_scene_create": (_S, [P, C.POINTER(SceneDesc), C.POINTER(P)]),
+    "wf_scene_destroy": (_S, [P]),
+    "wf_accel_build": (_S, [P, C.POINTER(BuildStats)]),
+    "wf_scene_device_bytes": (_S, [P, C.POINTER(C.c_size_t)]),
+    "wf_intersect": (_S, [P, C.POINTER(RaySoA), C.c_int32, C.c_float, C.c_float,
+                          C.POINTER(HitSoA)]),
+    "wf_occluded": (_S, [P, P, P, P, P, P, P, P, C.c_int32, P]),
+    "wf_intersect_split": (_S, [P, C.POINTER(RaySoA), C.c_int32, C.POINTER(ShadeSoA),
+                                C.POINTER(C.c_int32), C.POINTER(RaySoA), C.POINTER(C.c_int32)]),
+    "wf_ray_schedule": (_S, [P, C.POINTER(RaySoA), C.c_int32, C.POINTER(C.c_float),
+                             C.POINTER(C.c_float), P, P]),
+    "wf_materialize_rays": (_S, [P, C.POINTER(RaySoA), P, C.c_int32, C.POINTER(RaySoA)]),
+    "wf_scene_set_costs": (_S, [P, P, P]),
+    "wf_shading_schedule": (_S, [P, C.POINTER(ShadeSoA), C.c_int32, C.c_int32, P]),
+    "wf_materialize_shading": (_S, [P, C.POINTER(ShadeSoA), P, C.c_int32, C.POINTER(ShadeSoA)]),
+    "wf_shade": (_S, [P, C.POINTER(ShadeSoA), C.c_int32, C.POINTER(ShadeParams), P,
+                      C.POINTER(RaySoA), C.POINTER(C.c_int32)]),
+    "wf_shade_misses": (_S, [P, C.POINTER(RaySoA), C.c_int32, C.c_int32, P]),
+    "wf_render": (_S, [P, C.POINTER(RenderParams), P, P, C.POINTER(RenderStats)]),
+    "wf_profile_enable": (_S, [P, C.c_int32]),
+    "wf_profile_query": (_S, [P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64),
+                              C.POINTER(C.c_double)]),
+    "wf_profile_names": (_S, [P, C.c_char_p, C.c_int32]),
+}
+
+_lib = None
+
+
+def header_functions() -> list[str]:
+    """Names of every function the public header declares."""
+    import re
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"^\s*(?:wf_status|const char\*)\s+(wf_\w+)\s*\(", text, re.M)))
+
+
+def load_library(path: Path | None = None):
+    """Loads libwavefront_hip.so and binds all prototypes. Raises if it is not built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(f"{p} is not built: run `python wavefront-renderer_amd/build.py` "
+                           "(the HIP path has no fallback)")
+    lib = C.CDLL(str(p))
+    for name, (res, args) in _PROTOS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+class WavefrontError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"status {status}: {message}")
+        self.status = status
+
+
+def check(status: int) -> None:
+    if status != WF_OK:
+        lib = load_library()
+        msg = lib.wf_last_error()
+        raise WavefrontError(status, msg.decode() if msg else "")

@@ -1,0 +1,124 @@
+// Device-side helpers shared by the kernels: the reference's Vec3 arithmetic (one IEEE op per
+// operator, evaluated in the written order; the library is compiled with -ffp-contract=off so
+// no multiply-add is fused), the Philox4x32-10 counter RNG and small wave utilities.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace wf {
+
+struct V3 {
+    float x, y, z;
+    __device__ __forceinline__ V3() : x(0.0f), y(0.0f), z(0.0f) {}
+    __device__ __forceinline__ V3(float a, float b, float c) : x(a), y(b), z(c) {}
+    __device__ __forceinline__ V3 operator+(const V3& o) const { return V3(x + o.x, y + o.y, z + o.z); }
+    __device__ __forceinline__ V3 operator-(const V3& o) const { return V3(x - o.x, y - o.y, z - o.z); }
+    __device__ __forceinline__ V3 operator*(float s) const { return V3(x * s, y * s, z * s); }
+    __device__ __forceinline__ V3 operator/(float s) const { return V3(x / s, y / s, z / s); }
+    __device__ __forceinline__ V3 operator*(const V3& o) const { return V3(x * o.x, y * o.y, z * o.z); }
+    __device__ __forceinline__ float dot(const V3& o) const { return x * o.x + y * o.y + z * o.z; }
+    __device__ __forceinline__ V3 cross(const V3& o) const {
+        return V3(y * o.z - z * o.y, z * o.x - x * o.z, x * o.y - y * o.x);
+    }
+    __device__ __forceinline__ float length() const { return __builtin_sqrtf(x * x + y * y + z * z); }
+    __device__ __forceinline__ V3 normalized() const { return *this / length(); }
+};
+
+__device__ __forceinline__ V3 ld3(const float* x, const float* y, const float* z, int i) {
+    return V3(x[i], y[i], z[i]);
+}
+__device__ __forceinline__ void st3(float* x, float* y, float* z, int i, const V3& v) {
+    x[i] = v.x;
+    y[i] = v.y;
+    z[i] = v.z;
+}
+__device__ __forceinline__ V3 xyz(const float4& f) { return V3(f.x, f.y, f.z); }
+
+// BT.709 luminance with the reference's float evaluation order.
+__device__ __forceinline__ float luminance(const V3& c) {
+    return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z;
+}
+
+// std::min/std::max semantics for non-NaN operands: min(a,b) = (b < a) ? b : a.
+__device__ __forceinline__ float smin(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float smax(float a, float b) { return (a < b) ? b : a; }
+
+// ---- Philox4x32-10 counter-based generator -------------------------------------------------
+// Counter {pixel, sample, depth<<10 | dim>>2, job}, key {seed, 0}; output lane dim&3;
+// uniform = (x >> 8) * 2^-24 in [0,1).
+struct Philox {
+    uint32_t v[4];
+    __device__ __forceinline__ Philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                      uint32_t k0, uint32_t k1) {
+#pragma unroll
+        for (int r = 0; r < 10; ++r) {
+            uint32_t lo0 = 0xD2511F53u * c0;
+            uint32_t hi0 = __umulhi(0xD2511F53u, c0);
+            uint32_t lo1 = 0xCD9E8D57u * c2;
+            uint32_t hi1 = __umulhi(0xCD9E8D57u, c2);
+            uint32_t n0 = hi1 ^ c1 ^ k0;
+            uint32_t n2 = hi0 ^ c3 ^ k1;
+            c0 = n0;
+            c1 = lo1;
+            c2 = n2;
+            c3 = lo0;
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        v[0] = c0;
+        v[1] = c1;
+        v[2] = c2;
+        v[3] = c3;
+    }
+};
+
+constexpr uint32_t kRaygenDepth = 0x3FFFFFu;
+
+struct PathRng {
+    uint32_t seed, job, pixel, sample, depth;
+    // one Philox block (4 dims) cached
+    uint32_t cached_block = 0xFFFFFFFFu;
+    uint32_t cache[4];
+    __device__ __forceinline__ float operator()(uint32_t dim) {
+        uint32_t blk = dim >> 2;
+        if (blk != cached_block) {
+            Philox p(pixel, sample, (depth << 10) | blk, job, seed, 0u);
+            cache[0] = p.v[0];
+            cache[1] = p.v[1];
+            cache[2] = p.v[2];
+            cache[3] = p.v[3];
+            cached_block = blk;
+        }
+        const uint32_t lane = dim & 3u;
+        uint32_t x = lane == 0 ? cache[0] : (lane == 1 ? cache[1] : (lane == 2 ? cache[2] : cache[3]));
+        return (float)(x >> 8) * (1.0f / 16777216.0f);
+    }
+};
+
+enum : uint32_t {
+    D_LIGHT_PICK = 1,
+    D_LIGHT_TRI = 2,
+    D_LIGHT_R1 = 3,
+    D_LIGHT_R2 = 4,
+    D_RR = 5,
+    D_ENV_CHOICE = 6,
+    D_LOBE = 8,
+    D_DIR_U1 = 9,
+    D_DIR_U2 = 10,
+    D_ENV_SLOT = 12,
+    D_ENV_COIN = 13,
+    D_ENV_JX = 14,
+    D_ENV_JY = 15,
+    D_SPHERE = 16,
+};
+constexpr int kSphereTries = 256;
+
+// ---- wave utilities (wave64) ----------------------------------------------------------------
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    const unsigned lane = __lane_id();
+    return lane == 0 ? 0ull : ((~0ull) >> (64u - lane));
+}
+
+}  // namespace wf

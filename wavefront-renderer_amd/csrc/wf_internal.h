@@ -1,0 +1,259 @@
+// Host-side internals shared by the HIP translation units of libwavefront_hip.so: the context
+// (stream, grow-only workspace pool, event timing), the scene's device layout, and the internal
+// launch entry points each .hip file provides.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/wavefront_hip.h"
+
+namespace wf {
+
+// ---- errors ----------------------------------------------------------------------------------
+wf_status set_error(wf_status code, const char* fmt, ...);
+void clear_error();
+
+#define WF_HIP(expr)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return ::wf::set_error(WF_ERR_HIP, "%s failed: %s (%s:%d)", #expr,          \
+                                   hipGetErrorString(e_), __FILE__, __LINE__);          \
+    } while (0)
+
+#define WF_TRY(expr)                      \
+    do {                                  \
+        wf_status s_ = (expr);            \
+        if (s_ != WF_OK) return s_;       \
+    } while (0)
+
+#define WF_REQUIRE(cond, ...)                                                 \
+    do {                                                                      \
+        if (!(cond)) return ::wf::set_error(WF_ERR_INVALID_ARGUMENT, __VA_ARGS__); \
+    } while (0)
+
+// ---- context ---------------------------------------------------------------------------------
+struct ProfEntry {
+    double ms = 0.0;
+    int64_t launches = 0;
+    double bytes = 0.0;
+};
+
+struct PendingTiming {
+    std::string name;
+    hipEvent_t start, stop;
+    double bytes;
+};
+
+}  // namespace wf
+
+struct wf_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // grow-only named scratch buffers (freed with the context)
+    struct Buf {
+        void* ptr = nullptr;
+        size_t bytes = 0;
+    };
+    std::map<std::string, Buf> workspace;
+    // pinned host staging for small device->host readbacks
+    void* pinned = nullptr;
+    size_t pinned_bytes = 0;
+    // timing
+    bool profile = false;
+    std::map<std::string, wf::ProfEntry> prof;
+    std::vector<wf::PendingTiming> pending;
+    std::vector<hipEvent_t> event_pool;
+    int cu_count = 256;
+};
+
+namespace wf {
+
+// Named scratch buffer of at least `bytes` on the context's device (grow-only).
+wf_status workspace(wf_context* ctx, const char* name, size_t bytes, void** out);
+// Pinned host scratch of at least `bytes`.
+wf_status pinned(wf_context* ctx, size_t bytes, void** out);
+// Copies `bytes` from device to host through pinned memory and synchronises the stream.
+wf_status readback(wf_context* ctx, void* host, const void* dev, size_t bytes);
+
+// Brackets a kernel launch with events when profiling is on.
+struct Timed {
+    wf_context* ctx;
+    const char* name;
+    double bytes;
+    hipEvent_t a = nullptr, b = nullptr;
+    Timed(wf_context* c, const char* n, double algorithmic_bytes);
+    ~Timed();
+};
+wf_status harvest_timings(wf_context* ctx);
+
+inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+// ---- scene layout on the device ---------------------------------------------------------------
+struct DevMaterial {   // wf_material packed for the device
+    int32_t type;
+    int32_t model;
+    int32_t texture_id;
+    int32_t use_spatial_checker;
+    float albedo[3];
+    float roughness;
+    float emission;
+    float spot_outer_deg;
+    float spot_falloff_deg;
+    float ior;
+    float checker_cell_size;
+    float light_pdf_area;      // unbiased track: selection pdf per unit area of this emitter
+    float pad[2];
+};
+
+struct DevTexture {
+    int32_t width, height, format, pad;
+    int64_t offset;            // texel offset (RGBA8: u32 units; float RGB: float4 units)
+};
+
+struct BvhNode {               // two child boxes; child index >= 0 internal node, < 0 ~leaf
+    float4 a;                  // c0.lo.xyz, c0 index
+    float4 b;                  // c0.hi.xyz, c1 index
+    float4 c;                  // c1.lo.xyz, unused
+    float4 d;                  // c1.hi.xyz, unused
+};
+
+struct SceneDev {
+    int32_t n_tris = 0, n_meshes = 0, n_materials = 0, n_textures = 0, n_vertices = 0;
+    // triangles in BVH leaf order: v0 (w = global index bits), e1, e2
+    float4* tri = nullptr;
+    // per global triangle
+    int32_t* tri_geom = nullptr;
+    float* tri_uv = nullptr;           // 6 per triangle (zeros without UVs)
+    float4* tri_v0 = nullptr;          // unsorted v0/e1/e2 for light sampling (global index)
+    float4* tri_e1 = nullptr;
+    float4* tri_e2 = nullptr;
+    // per mesh
+    int32_t* mesh_tri_offset = nullptr;
+    int32_t* mesh_material = nullptr;
+    int32_t* mesh_has_uv = nullptr;
+    // materials / textures
+    DevMaterial* materials = nullptr;
+    DevTexture* textures = nullptr;
+    uint32_t* texels8 = nullptr;
+    float4* texels32 = nullptr;
+    float* srgb_lut = nullptr;         // 256 entries
+    uint64_t* tex_mem_bytes = nullptr; // Texture::memoryBytes per texture (w*h*12)
+    // policy tables
+    int32_t* mat_cost_rank = nullptr;  // dense rank of relativeCost (descending)
+    int32_t cost_rank_bits = 0;
+    double* mat_rel_cost = nullptr;
+    double* tex_rel_cost = nullptr;
+    int32_t* tex_size_rank = nullptr;  // dense rank of memoryBytes (descending), n_textures+1 entries
+    int32_t size_rank_bits = 0;
+    // lights
+    int32_t n_emissive_meshes = 0;
+    int32_t* emissive_meshes = nullptr;
+    int32_t* mesh_tri_count = nullptr;
+    float total_emissive_area = 0.0f;
+    int32_t n_light_tris = 0;
+    int32_t* light_tris = nullptr;         // global triangle id per emitter slot
+    float4* light_v0 = nullptr;            // original vertices per emitter slot
+    float4* light_v1 = nullptr;
+    float4* light_v2 = nullptr;
+    int32_t* light_mat = nullptr;          // material per emitter slot
+    int32_t* emissive_light_base = nullptr;  // first emitter slot of each emissive mesh
+    float* light_prob = nullptr;
+    int32_t* light_alias = nullptr;
+    wf_directional_light directional{};
+    int32_t env_w = 0, env_h = 0;
+    float4* env = nullptr;             // raw RGB (w unused)
+    float* env_prob = nullptr;
+    int32_t* env_alias = nullptr;
+    float* env_pdf = nullptr;
+    float env_intensity = 1.0f;
+    float env_tint[3] = {1.0f, 1.0f, 1.0f};
+    float env_choice_prob = 0.0f;
+    // BVH
+    BvhNode* nodes = nullptr;
+    int32_t n_nodes = 0;
+    int32_t max_depth = 0;
+    bool built = false;
+    float bounds_min[3] = {0, 0, 0}, bounds_max[3] = {0, 0, 0};
+};
+
+}  // namespace wf
+
+struct wf_scene {
+    wf_context* ctx = nullptr;
+    wf::SceneDev dev;
+    std::vector<void*> allocations;
+    size_t device_bytes = 0;
+    // host copies needed for table rebuilds
+    std::vector<wf_material> materials;
+    std::vector<uint64_t> tex_mem;
+};
+
+namespace wf {
+
+wf_status scene_alloc(wf_scene* s, size_t bytes, void** out);
+template <typename T>
+wf_status scene_upload(wf_scene* s, const T* host, size_t count, T** out) {
+    if (count == 0) {
+        *out = nullptr;
+        return WF_OK;
+    }
+    void* p = nullptr;
+    WF_TRY(scene_alloc(s, count * sizeof(T), &p));
+    WF_HIP(hipMemcpyAsync(p, host, count * sizeof(T), hipMemcpyHostToDevice, s->ctx->stream));
+    *out = (T*)p;
+    return WF_OK;
+}
+
+// ---- primitives (wf_sort.hip) ------------------------------------------------------------------
+// Stable LSD radix sort of (key, value) pairs on key bits [0, key_bits). When vals_in is NULL
+// the values start as 0..n-1. The sorted pairs end in keys_out/vals_out (which must not alias
+// the inputs); keys_in is not modified.
+wf_status radix_sort_pairs(wf_context* ctx, const uint64_t* keys_in, const int32_t* vals_in,
+                           uint64_t* keys_out, int32_t* vals_out, int32_t n, int32_t key_bits,
+                           const char* tag);
+// Exclusive prefix sum of n u32 flags/counts into out (may alias in); total (optional) is
+// written to a device int32.
+wf_status exclusive_scan_u32(wf_context* ctx, const uint32_t* in, uint32_t* out, int32_t n,
+                             int32_t* total_dev, const char* tag);
+// dst[i] = i
+wf_status iota_i32(wf_context* ctx, int32_t* dst, int32_t n);
+
+// ---- stage entry points (used by the C-ABI and the render loop) -------------------------------
+wf_status scene_ready(const wf_scene* s);
+wf_status trace_closest(const wf_scene* s, const wf_ray_soa* rays, int32_t n, float tnear,
+                        float tfar, int32_t* hit_leaf, float* hit_t, float* hit_bu, float* hit_bv,
+                        uint32_t* flags);
+wf_status trace_occluded(const wf_scene* s, const float* ox, const float* oy, const float* oz,
+                         const float* dx, const float* dy, const float* dz, const float* maxd,
+                         int32_t n, uint8_t* out);
+// Closest hit + stable split; counts_dev[0] receives the hit count (device int32).
+wf_status intersect_split(const wf_scene* s, const wf_ray_soa* rays, int32_t n,
+                          wf_shade_soa* hits_out, wf_ray_soa* misses_out, int32_t* counts_dev);
+wf_status ray_schedule(wf_context* ctx, const wf_ray_soa* rays, int32_t n, const float bmin[3],
+                       const float bmax[3], int32_t* sorted, uint64_t* keys_out);
+wf_status shading_schedule(const wf_scene* s, const wf_shade_soa* q, int32_t n, int32_t policy,
+                           int32_t* sorted);
+// One bounce of shadeAll: emission into accum, shadow rays resolved, next rays compacted into
+// `next`; counts_dev[0] = next rays, counts_dev[1] = shadow rays traced (device int32).
+wf_status shade_bounce(const wf_scene* s, const wf_shade_soa* q, int32_t n,
+                       const wf_shade_params* p, float* accum, wf_ray_soa* next,
+                       int32_t* counts_dev);
+wf_status shade_misses(const wf_scene* s, const wf_ray_soa* q, int32_t n, int32_t physics,
+                       float* accum);
+wf_status ray_keys(wf_context* ctx, const wf_ray_soa* rays, int32_t n, const float bmin[3],
+                   const float bmax[3], uint64_t* keys);
+wf_status shading_keys(const wf_scene* s, const wf_shade_soa* q, int32_t n, int32_t policy,
+                       uint64_t* keys, int32_t* key_bits);
+wf_status gather_rays(wf_context* ctx, const wf_ray_soa* src, const int32_t* idx, int32_t n,
+                      wf_ray_soa* dst);
+wf_status gather_shading(wf_context* ctx, const wf_shade_soa* src, const int32_t* idx, int32_t n,
+                         wf_shade_soa* dst);
+
+}  // namespace wf

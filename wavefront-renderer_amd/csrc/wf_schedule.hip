@@ -1,0 +1,493 @@
+// Queue scheduling: RayQueue::sortKey/schedule/materialize (include/scheduling/RayQueue.h:135-215)
+// and ShadingQueue::schedule/materialize for the four policies (include/scheduling/ShadingQueue.h:
+// 90-141, 233-329, 437-550). Every policy becomes an integer key plus one stable radix sort:
+//   none        : identity
+//   material    : typeRank(3 bits) | costRank(material)          (ShadingQueue.h:112-140,306-312)
+//   texture     : typeRank(3 bits) | sizeRank(texture)           (ShadingQueue.h:122-134,315-316)
+//   costBenefit : type(3 bits) | priorityRank | hit-point Morton (ShadingQueue.h:437-510)
+// typeRank is computed on the device from per-type hit counts / texture bytes; cost and size
+// ranks are dense descending ranks of the host tables; ties keep queue order.
+#include "wf_device.h"
+#include "wf_internal.h"
+
+namespace wf {
+
+// spreadBits10 (RayQueue.h:45-55)
+__device__ __forceinline__ uint32_t spread_bits10(uint32_t v) {
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x30000FFu;
+    v = (v | (v << 8)) & 0x300F00Fu;
+    v = (v | (v << 4)) & 0x30C30C3u;
+    v = (v | (v << 2)) & 0x9249249u;
+    return v;
+}
+
+// std::clamp(v, 0, 1) for non-NaN v
+__device__ __forceinline__ float clamp01(float v) { return v < 0.0f ? 0.0f : (1.0f < v ? 1.0f : v); }
+
+__global__ void k_ray_keys(const float* __restrict__ ox, const float* __restrict__ oy,
+                           const float* __restrict__ oz, const float* __restrict__ dx,
+                           const float* __restrict__ dy, const float* __restrict__ dz, int n,
+                           float minx, float miny, float minz, float ex, float ey, float ez,
+                           uint64_t* __restrict__ keys) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t octant = (dx[i] < 0.0f ? 1u : 0u) | (dy[i] < 0.0f ? 2u : 0u) |
+                            (dz[i] < 0.0f ? 4u : 0u);
+    const uint32_t gx = (uint32_t)(clamp01((ox[i] - minx) / ex) * 1023.0f);
+    const uint32_t gy = (uint32_t)(clamp01((oy[i] - miny) / ey) * 1023.0f);
+    const uint32_t gz = (uint32_t)(clamp01((oz[i] - minz) / ez) * 1023.0f);
+    const uint32_t morton = spread_bits10(gx) | (spread_bits10(gy) << 1) | (spread_bits10(gz) << 2);
+    keys[i] = (octant << 30) | (uint64_t)morton;
+}
+
+wf_status ray_keys(wf_context* ctx, const wf_ray_soa* r, int32_t n, const float bmin[3],
+                   const float bmax[3], uint64_t* keys) {
+    if (n <= 0) return WF_OK;
+    // extent = max(max - min, 1e-6) per axis (RayQueue.h:143-145), host float math
+    auto ext = [](float hi, float lo) {
+        float e = hi - lo;
+        return e < 1e-6f ? 1e-6f : e;
+    };
+    Timed tm(ctx, "ray_keys", 32.0 * n);
+    k_ray_keys<<<grid_for(n, 256), 256, 0, ctx->stream>>>(
+        r->ox, r->oy, r->oz, r->dx, r->dy, r->dz, n, bmin[0], bmin[1], bmin[2],
+        ext(bmax[0], bmin[0]), ext(bmax[1], bmin[1]), ext(bmax[2], bmin[2]), keys);
+    WF_HIP(hipGetLastError());
+    return WF_OK;
+}
+
+// ---- materialize ---------------------------------------------------------------------------------
+__global__ void k_gather_rays(wf_ray_soa s, const int32_t* __restrict__ idx, int n, wf_ray_soa d) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int j = idx[i];
+    d.ox[i] = s.ox[j];
+    d.oy[i] = s.oy[j];
+    d.oz[i] = s.oz[j];
+    d.dx[i] = s.dx[j];
+    d.dy[i] = s.dy[j];
+    d.dz[i] = s.dz[j];
+    d.tr[i] = s.tr[j];
+    d.tg[i] = s.tg[j];
+    d.tb[i] = s.tb[j];
+    d.pixel[i] = s.pixel[j];
+    d.depth[i] = s.depth[j];
+    d.pdf[i] = s.pdf[j];
+}
+
+__global__ void k_gather_shade(wf_shade_soa s, const int32_t* __restrict__ idx, int n,
+                               wf_shade_soa d) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int j = idx[i];
+    d.px[i] = s.px[j];
+    d.py[i] = s.py[j];
+    d.pz[i] = s.pz[j];
+    d.nx[i] = s.nx[j];
+    d.ny[i] = s.ny[j];
+    d.nz[i] = s.nz[j];
+    d.t[i] = s.t[j];
+    d.u[i] = s.u[j];
+    d.v[i] = s.v[j];
+    d.tri[i] = s.tri[j];
+    d.mat[i] = s.mat[j];
+    d.tex[i] = s.tex[j];
+    d.ox[i] = s.ox[j];
+    d.oy[i] = s.oy[j];
+    d.oz[i] = s.oz[j];
+    d.dx[i] = s.dx[j];
+    d.dy[i] = s.dy[j];
+    d.dz[i] = s.dz[j];
+    d.tr[i] = s.tr[j];
+    d.tg[i] = s.tg[j];
+    d.tb[i] = s.tb[j];
+    d.pixel[i] = s.pixel[j];
+    d.depth[i] = s.depth[j];
+    d.pdf[i] = s.pdf[j];
+    if (d.geom) d.geom[i] = s.geom ? s.geom[j] : 0;
+    if (d.front) d.front[i] = s.front ? s.front[j] : 1;
+}
+
+wf_status gather_rays(wf_context* ctx, const wf_ray_soa* src, const int32_t* idx, int32_t n,
+                      wf_ray_soa* dst) {
+    if (n <= 0) return WF_OK;
+    Timed tm(ctx, "gather_rays", 100.0 * n);
+    k_gather_rays<<<grid_for(n, 256), 256, 0, ctx->stream>>>(*src, idx, n, *dst);
+    WF_HIP(hipGetLastError());
+    return WF_OK;
+}
+
+wf_status gather_shading(wf_context* ctx, const wf_shade_soa* src, const int32_t* idx, int32_t n,
+                         wf_shade_soa* dst) {
+    if (n <= 0) return WF_OK;
+    Timed tm(ctx, "gather_shading", 204.0 * n);
+    k_gather_shade<<<grid_for(n, 256), 256, 0, ctx->stream>>>(*src, idx, n, *dst);
+    WF_HIP(hipGetLastError());
+    return WF_OK;
+}
+
+// ---- material / texture policies ---------------------------------------------------------------
+struct TypeStats {
+    uint32_t count[8];
+    unsigned long long bytes[8];
+    int32_t rank[8];
+};
+
+__global__ void k_type_stats(const int32_t* __restrict__ mat, const int32_t* __restrict__ tex,
+                             int n, const DevMaterial* __restrict__ mats,
+                             const uint64_t* __restrict__ tex_mem, int n_tex,
+                             TypeStats* __restrict__ st) {
+    __shared__ uint32_t cnt[8];
+    __shared__ unsigned long long bytes[8];
+    if (threadIdx.x < 8) {
+        cnt[threadIdx.x] = 0;
+        bytes[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int ty = mats[mat[i]].type & 7;
+        atomicAdd(&cnt[ty], 1u);
+        const int t = tex[i];
+        if (t >= 0 && t < n_tex) atomicAdd(&bytes[ty], (unsigned long long)tex_mem[t]);
+    }
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        atomicAdd(&st->count[threadIdx.x], cnt[threadIdx.x]);
+        atomicAdd(&st->bytes[threadIdx.x], bytes[threadIdx.x]);
+    }
+}
+
+// computeTypeOrder (ShadingQueue.h:277-329): material = count desc, type desc;
+// texture = bytes desc (equal sums: type desc).
+__global__ void k_type_rank(int policy, TypeStats* st) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int order[8];
+    int m = 0;
+    for (int t = 0; t < 8; ++t)
+        if (st->count[t] > 0) order[m++] = t;
+    for (int a = 1; a < m; ++a) {
+        int x = order[a];
+        int b = a - 1;
+        while (b >= 0) {
+            int y = order[b];
+            bool x_first;
+            if (policy == WF_POLICY_MATERIAL)
+                x_first = st->count[x] != st->count[y] ? st->count[x] > st->count[y] : x > y;
+            else
+                x_first = st->bytes[x] != st->bytes[y] ? st->bytes[x] > st->bytes[y] : x > y;
+            if (!x_first) break;
+            order[b + 1] = order[b];
+            --b;
+        }
+        order[b + 1] = x;
+    }
+    for (int t = 0; t < 8; ++t) st->rank[t] = 7;
+    for (int r = 0; r < m; ++r) st->rank[order[r]] = r;
+}
+
+__global__ void k_policy_keys(int policy, const int32_t* __restrict__ mat,
+                              const int32_t* __restrict__ tex, int n,
+                              const DevMaterial* __restrict__ mats,
+                              const TypeStats* __restrict__ st,
+                              const int32_t* __restrict__ cost_rank, int cost_bits,
+                              const int32_t* __restrict__ size_rank, int size_bits, int n_tex,
+                              uint64_t* __restrict__ keys) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int m = mat[i];
+    const uint64_t tr = (uint64_t)st->rank[mats[m].type & 7];
+    uint64_t sub;
+    int bits;
+    if (policy == WF_POLICY_MATERIAL) {
+        sub = (uint64_t)cost_rank[m];
+        bits = cost_bits;
+    } else {
+        const int t = tex[i];
+        sub = (uint64_t)size_rank[(t >= 0 && t < n_tex) ? t : n_tex];
+        bits = size_bits;
+    }
+    keys[i] = (tr << bits) | sub;
+}
+
+// ---- cost-benefit policy ---------------------------------------------------------------------------
+constexpr int kCbBlock = 256;
+constexpr int kCbItems = 8;
+constexpr int kCbTile = kCbBlock * kCbItems;
+constexpr int kCbMaxMaterials = 1024;
+
+__device__ __forceinline__ uint32_t f2o(float f) {
+    uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float o2f(uint32_t u) {
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+}
+
+// Per-tile, per-material double sums of luminance(throughput) in a fixed order (wave chunks of 64
+// in queue order, peers summed by their lowest lane in lane order), plus hit-point bounds.
+__global__ __launch_bounds__(kCbBlock) void k_cb_partials(
+    const int32_t* __restrict__ mat, const float* __restrict__ tr, const float* __restrict__ tg,
+    const float* __restrict__ tb, const float* __restrict__ px, const float* __restrict__ py,
+    const float* __restrict__ pz, int n, int n_mat, double* __restrict__ partials,
+    uint32_t* __restrict__ present, uint32_t* __restrict__ bounds) {
+    __shared__ double wsum[kCbBlock / 64][kCbMaxMaterials];
+    __shared__ double vals[kCbBlock];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int k = threadIdx.x; k < (kCbBlock / 64) * n_mat; k += kCbBlock)
+        wsum[k / n_mat][k % n_mat] = 0.0;
+    __syncthreads();
+    const uint64_t lt = lanemask_lt();
+    float mn[3] = {3.0e38f, 3.0e38f, 3.0e38f}, mx[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+    const int chunk = blockIdx.x * kCbTile + wave * 64 * kCbItems;
+    for (int it = 0; it < kCbItems; ++it) {
+        const int i = chunk + it * 64 + lane;
+        const bool valid = i < n;
+        const int m = valid ? mat[i] : 0;
+        double x = 0.0;
+        if (valid) {
+            x = (double)luminance(V3(tr[i], tg[i], tb[i]));
+            mn[0] = fminf(mn[0], px[i]); mx[0] = fmaxf(mx[0], px[i]);
+            mn[1] = fminf(mn[1], py[i]); mx[1] = fmaxf(mx[1], py[i]);
+            mn[2] = fminf(mn[2], pz[i]); mx[2] = fmaxf(mx[2], pz[i]);
+        }
+        vals[threadIdx.x] = x;
+        uint64_t peers = __ballot(valid);
+        for (int b = 0; b < 10; ++b) {
+            const bool bit = (m >> b) & 1;
+            const uint64_t msk = __ballot(bit);
+            peers &= bit ? msk : ~msk;
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (valid && (peers & lt) == 0) {
+            double s = wsum[wave][m];
+            uint64_t p = peers;
+            while (p) {
+                const int l = __ffsll((unsigned long long)p) - 1;
+                s += vals[wave * 64 + l];
+                p &= p - 1;
+            }
+            wsum[wave][m] = s;
+            atomicOr(&present[m], 1u);
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    for (int m = threadIdx.x; m < n_mat; m += kCbBlock) {
+        double s = 0.0;
+        for (int w = 0; w < kCbBlock / 64; ++w) s += wsum[w][m];
+        partials[(size_t)blockIdx.x * n_mat + m] = s;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float a = mn[k], b = mx[k];
+        for (int o = 32; o >= 1; o >>= 1) {
+            a = fminf(a, __shfl_xor(a, o, 64));
+            b = fmaxf(b, __shfl_xor(b, o, 64));
+        }
+        if (lane == 0) {
+            atomicMin(&bounds[k], f2o(a));
+            atomicMax(&bounds[3 + k], f2o(b));
+        }
+    }
+}
+
+// priority = benefit / max(relCost(mat) + relCost(tex of mat), 1e-6)   (ShadingQueue.h:458-471)
+__global__ void k_cb_priority(const double* __restrict__ partials, int tiles, int n_mat,
+                              const DevMaterial* __restrict__ mats,
+                              const double* __restrict__ mat_cost,
+                              const double* __restrict__ tex_cost, double* __restrict__ prio) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n_mat) return;
+    double s = 0.0;
+    for (int t = 0; t < tiles; ++t) s += partials[(size_t)t * n_mat + m];
+    const double sc = mat_cost[m];
+    const int tid = mats[m].texture_id;
+    const double tc = tid >= 0 ? tex_cost[tid] : 0.0;
+    const double c = sc + tc;
+    prio[m] = s / (c > 1e-6 ? c : 1e-6);
+}
+
+// Competition rank by descending priority among materials present in the queue.
+__global__ void k_cb_rank(const double* __restrict__ prio, const uint32_t* __restrict__ present,
+                          int n_mat, int32_t* __restrict__ rank) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n_mat) return;
+    const double p = prio[m];
+    int r = 0;
+    for (int k = 0; k < n_mat; ++k)
+        if (present[k] && prio[k] > p) ++r;
+    rank[m] = r;
+}
+
+// computeMortonCode (ShadingQueue.h:513-550)
+__device__ __forceinline__ uint32_t quantise_hit(float v, float lo, float hi) {
+    const float t = (hi > lo) ? (v - lo) / (hi - lo) : 0.0f;
+    return (uint32_t)(clamp01(t) * 1023.0f);
+}
+
+__global__ void k_cb_keys(const int32_t* __restrict__ mat, const float* __restrict__ px,
+                          const float* __restrict__ py, const float* __restrict__ pz, int n,
+                          const DevMaterial* __restrict__ mats, const int32_t* __restrict__ rank,
+                          int rank_bits, const uint32_t* __restrict__ bounds,
+                          uint64_t* __restrict__ keys) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float lx = o2f(bounds[0]), ly = o2f(bounds[1]), lz = o2f(bounds[2]);
+    const float hx = o2f(bounds[3]), hy = o2f(bounds[4]), hz = o2f(bounds[5]);
+    const uint32_t morton = spread_bits10(quantise_hit(px[i], lx, hx)) |
+                            (spread_bits10(quantise_hit(py[i], ly, hy)) << 1) |
+                            (spread_bits10(quantise_hit(pz[i], lz, hz)) << 2);
+    const int m = mat[i];
+    keys[i] = ((uint64_t)(mats[m].type & 7) << (rank_bits + 30)) | ((uint64_t)rank[m] << 30) |
+              (uint64_t)morton;
+}
+
+static int bits_for(int max_value) {
+    int b = 0;
+    while ((1 << b) <= max_value) ++b;
+    return b;
+}
+
+wf_status shading_keys(const wf_scene* s, const wf_shade_soa* q, int32_t n, int32_t policy,
+                       uint64_t* keys, int32_t* key_bits) {
+    wf_context* ctx = s->ctx;
+    const SceneDev& d = s->dev;
+    if (policy == WF_POLICY_MATERIAL || policy == WF_POLICY_TEXTURE) {
+        void* p = nullptr;
+        WF_TRY(workspace(ctx, "type_stats", sizeof(TypeStats), &p));
+        TypeStats* st = (TypeStats*)p;
+        WF_TRY(wf_fill_u32(ctx, st, 0u, sizeof(TypeStats) / 4));
+        {
+            Timed tm(ctx, "policy_keys", (policy == WF_POLICY_MATERIAL ? 4.0 : 8.0) * n);
+            int blocks = (int)std::min<int64_t>(grid_for(n, 256), 1024);
+            k_type_stats<<<blocks, 256, 0, ctx->stream>>>(q->mat, q->tex, n, d.materials,
+                                                          d.tex_mem_bytes, d.n_textures, st);
+            WF_HIP(hipGetLastError());
+            k_type_rank<<<1, 1, 0, ctx->stream>>>(policy, st);
+            WF_HIP(hipGetLastError());
+            k_policy_keys<<<grid_for(n, 256), 256, 0, ctx->stream>>>(
+                policy, q->mat, q->tex, n, d.materials, st, d.mat_cost_rank, d.cost_rank_bits,
+                d.tex_size_rank, d.size_rank_bits, d.n_textures, keys);
+            WF_HIP(hipGetLastError());
+        }
+        *key_bits = 3 + (policy == WF_POLICY_MATERIAL ? d.cost_rank_bits : d.size_rank_bits);
+        return WF_OK;
+    }
+    if (policy == WF_POLICY_COST_BENEFIT) {
+        const int nm = d.n_materials;
+        if (nm > kCbMaxMaterials)
+            return set_error(WF_ERR_UNSUPPORTED,
+                             "costBenefit policy supports at most %d materials (scene has %d)",
+                             kCbMaxMaterials, nm);
+        const int tiles = (n + kCbTile - 1) / kCbTile;
+        void *pp, *ppres, *pb, *pprio, *prank;
+        WF_TRY(workspace(ctx, "cb_partials", sizeof(double) * (size_t)tiles * nm, &pp));
+        WF_TRY(workspace(ctx, "cb_present", sizeof(uint32_t) * nm, &ppres));
+        WF_TRY(workspace(ctx, "cb_bounds", sizeof(uint32_t) * 8, &pb));
+        WF_TRY(workspace(ctx, "cb_prio", sizeof(double) * nm, &pprio));
+        WF_TRY(workspace(ctx, "cb_rank", sizeof(int32_t) * nm, &prank));
+        WF_TRY(wf_fill_u32(ctx, ppres, 0u, (size_t)nm));
+        WF_TRY(wf_fill_u32(ctx, pb, 0xFFFFFFFFu, 3));
+        WF_TRY(wf_fill_u32(ctx, (uint32_t*)pb + 3, 0u, 5));
+        Timed tm(ctx, "policy_keys", 32.0 * n);
+        k_cb_partials<<<tiles, kCbBlock, 0, ctx->stream>>>(q->mat, q->tr, q->tg, q->tb, q->px,
+                                                           q->py, q->pz, n, nm, (double*)pp,
+                                                           (uint32_t*)ppres, (uint32_t*)pb);
+        WF_HIP(hipGetLastError());
+        k_cb_priority<<<grid_for(nm, 256), 256, 0, ctx->stream>>>(
+            (double*)pp, tiles, nm, d.materials, d.mat_rel_cost, d.tex_rel_cost, (double*)pprio);
+        WF_HIP(hipGetLastError());
+        k_cb_rank<<<grid_for(nm, 256), 256, 0, ctx->stream>>>((double*)pprio, (uint32_t*)ppres,
+                                                              nm, (int32_t*)prank);
+        WF_HIP(hipGetLastError());
+        const int rb = nm <= 1 ? 0 : bits_for(nm - 1);
+        k_cb_keys<<<grid_for(n, 256), 256, 0, ctx->stream>>>(q->mat, q->px, q->py, q->pz, n,
+                                                             d.materials, (int32_t*)prank, rb,
+                                                             (uint32_t*)pb, keys);
+        WF_HIP(hipGetLastError());
+        *key_bits = 3 + rb + 30;
+        return WF_OK;
+    }
+    return set_error(WF_ERR_INVALID_ARGUMENT, "unknown scheduling policy %d", policy);
+}
+
+wf_status shading_schedule(const wf_scene* s, const wf_shade_soa* q, int32_t n, int32_t policy,
+                           int32_t* sorted) {
+    wf_context* ctx = s->ctx;
+    if (n <= 0) return WF_OK;
+    if (policy == WF_POLICY_NONE) return iota_i32(ctx, sorted, n);
+    void *pk, *pk2;
+    WF_TRY(workspace(ctx, "sched_keys", sizeof(uint64_t) * n, &pk));
+    WF_TRY(workspace(ctx, "sched_keys2", sizeof(uint64_t) * n, &pk2));
+    int32_t bits = 0;
+    WF_TRY(shading_keys(s, q, n, policy, (uint64_t*)pk, &bits));
+    return radix_sort_pairs(ctx, (uint64_t*)pk, nullptr, (uint64_t*)pk2, sorted, n, bits,
+                            "shading_sort");
+}
+
+wf_status ray_schedule(wf_context* ctx, const wf_ray_soa* rays, int32_t n, const float bmin[3],
+                       const float bmax[3], int32_t* sorted, uint64_t* keys_out) {
+    if (n <= 0) return WF_OK;
+    void *pk, *pk2;
+    WF_TRY(workspace(ctx, "ray_keys", sizeof(uint64_t) * n, &pk));
+    WF_TRY(workspace(ctx, "ray_keys2", sizeof(uint64_t) * n, &pk2));
+    WF_TRY(ray_keys(ctx, rays, n, bmin, bmax, (uint64_t*)pk));
+    if (keys_out)
+        WF_HIP(hipMemcpyAsync(keys_out, pk, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice,
+                              ctx->stream));
+    return radix_sort_pairs(ctx, (uint64_t*)pk, nullptr, (uint64_t*)pk2, sorted, n, 33,
+                            "ray_sort");
+}
+
+}  // namespace wf
+
+using namespace wf;
+
+extern "C" {
+
+wf_status wf_ray_schedule(wf_context* ctx, const wf_ray_soa* rays, int32_t n,
+                          const float bounds_min[3], const float bounds_max[3],
+                          int32_t* sorted_indices, uint64_t* keys_out) {
+    clear_error();
+    WF_REQUIRE(ctx && rays && bounds_min && bounds_max && sorted_indices,
+               "wf_ray_schedule: NULL argument");
+    WF_REQUIRE(n >= 0, "wf_ray_schedule: negative ray count");
+    WF_TRY(ray_schedule(ctx, rays, n, bounds_min, bounds_max, sorted_indices, keys_out));
+    WF_HIP(hipStreamSynchronize(ctx->stream));
+    return WF_OK;
+}
+
+wf_status wf_materialize_rays(wf_context* ctx, const wf_ray_soa* src, const int32_t* idx,
+                              int32_t n, wf_ray_soa* dst) {
+    clear_error();
+    WF_REQUIRE(ctx && src && idx && dst, "wf_materialize_rays: NULL argument");
+    WF_REQUIRE(n >= 0, "wf_materialize_rays: negative count");
+    WF_TRY(gather_rays(ctx, src, idx, n, dst));
+    WF_HIP(hipStreamSynchronize(ctx->stream));
+    return WF_OK;
+}
+
+wf_status wf_shading_schedule(const wf_scene* s, const wf_shade_soa* q, int32_t n, int32_t policy,
+                              int32_t* sorted_indices) {
+    clear_error();
+    WF_REQUIRE(s && q && sorted_indices, "wf_shading_schedule: NULL argument");
+    WF_REQUIRE(n >= 0, "wf_shading_schedule: negative count");
+    WF_REQUIRE(policy >= WF_POLICY_NONE && policy <= WF_POLICY_COST_BENEFIT,
+               "wf_shading_schedule: unknown policy %d", policy);
+    WF_TRY(shading_schedule(s, q, n, policy, sorted_indices));
+    WF_HIP(hipStreamSynchronize(s->ctx->stream));
+    return WF_OK;
+}
+
+wf_status wf_materialize_shading(wf_context* ctx, const wf_shade_soa* src, const int32_t* idx,
+                                 int32_t n, wf_shade_soa* dst) {
+    clear_error();
+    WF_REQUIRE(ctx && src && idx && dst, "wf_materialize_shading: NULL argument");
+    WF_REQUIRE(n >= 0, "wf_materialize_shading: negative count");
+    WF_TRY(gather_shading(ctx, src, idx, n, dst));
+    WF_HIP(hipStreamSynchronize(ctx->stream));
+    return WF_OK;
+}
+
+}  // extern "C"

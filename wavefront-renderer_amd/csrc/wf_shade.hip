@@ -1,0 +1,764 @@
+// Shading kernels replacing WavefrontRenderer::shadeAll (src/render/WavefrontRenderer.cpp:569-877),
+// the miss loop and getSkyColor (cpp:256-265, 879-889), generatePrimaryRays (cpp:401-457),
+// the AdaptiveSampler updates (include/render/AdaptiveSampler.h:40-80) and the final resolve
+// (cpp:127-137).
+//
+// One lane per shading-queue entry, in shading order. A path owns exactly one queue entry per
+// bounce, so every accumulator update is a plain read-modify-write of its own pixel: no float
+// atomics, and the per-pixel order is the reference's (emission, area NEE, directional NEE,
+// miss). Shadow rays are resolved by a second kernel over the compacted list of entries that
+// emitted one; surviving next rays are compacted in shading order.
+#include "wf_traverse.h"
+
+namespace wf {
+
+constexpr float kPi = 3.14159265f;
+
+struct ShadeArgs {
+    int32_t physics;
+    int32_t rr_min_depth;
+    int32_t cost_aware_rr;
+    float firefly;
+    uint32_t seed;
+    int32_t sample;
+    int32_t job;
+};
+
+// Per-entry outputs of k_shade consumed by the shadow and compaction kernels.
+struct ShadeOut {
+    float* ldx; float* ldy; float* ldz; float* lmax;   // area/environment shadow ray
+    float* lr; float* lg; float* lb;                   // its contribution
+    float* dr; float* dg; float* db;                   // directional-light contribution
+    uint32_t* shadow_flag;                             // bit0 area/env ray, bit1 directional
+    float* ndx; float* ndy; float* ndz;                // scattered direction
+    float* ntr; float* ntg; float* ntb;                // next throughput
+    float* npdf;
+    uint32_t* alive;
+};
+
+__device__ __forceinline__ V3 texel(const SceneDev& s, int tid, float u, float v) {
+    const DevTexture T = s.textures[tid];
+    u = u - floorf(u);
+    v = v - floorf(v);
+    int x = (int)(u * (float)T.width);
+    int y = (int)(v * (float)T.height);
+    x = x < 0 ? 0 : (x > T.width - 1 ? T.width - 1 : x);
+    y = y < 0 ? 0 : (y > T.height - 1 ? T.height - 1 : y);
+    const size_t idx = (size_t)y * T.width + x;
+    if (T.format == WF_TEX_RGBA8_SRGB) {
+        const uint32_t p = s.texels8[T.offset + idx];
+        return V3(s.srgb_lut[p & 0xFFu], s.srgb_lut[(p >> 8) & 0xFFu], s.srgb_lut[(p >> 16) & 0xFFu]);
+    }
+    const float4 f = s.texels32[T.offset + idx];
+    return V3(f.x, f.y, f.z);
+}
+
+// Material::getSurfaceColor (Material.cpp:143-163) with hashGridCell/hashToUV (SpatialHash.h:18-44).
+__device__ __forceinline__ V3 surface_color(const SceneDev& s, const DevMaterial& m, const V3& p,
+                                            float u, float v) {
+    const int t = m.texture_id;
+    if (m.use_spatial_checker && t >= 0 && t < s.n_textures) {
+        const int cx = (int)floorf(p.x / m.checker_cell_size);
+        const int cy = (int)floorf(p.y / m.checker_cell_size);
+        const int cz = (int)floorf(p.z / m.checker_cell_size);
+        const uint32_t h = ((uint32_t)cx * 73856093u ^ (uint32_t)cy * 19349663u ^
+                            (uint32_t)cz * 83492791u) * 2654435761u;
+        const float pu = ((h >> 16) & 0xFFFFu) / 65535.0f;
+        const float pv = (h & 0xFFFFu) / 65535.0f;
+        return texel(s, t, pu, pv);
+    }
+    if (t >= 0 && t < s.n_textures) return texel(s, t, u, v);
+    return V3(m.albedo[0], m.albedo[1], m.albedo[2]);
+}
+
+// Material::emitted (Material.cpp:268-297)
+__device__ __forceinline__ V3 emitted(const DevMaterial& m, const V3& ln, const V3& dfl) {
+    const V3 alb(m.albedo[0], m.albedo[1], m.albedo[2]);
+    if (m.type == WF_MAT_EMISSIVE) return alb * m.emission;
+    if (m.type == WF_MAT_SPOTLIGHT) {
+        V3 radiance = alb * m.emission;
+        const float c = ln.dot(dfl);
+        const float deg = acosf(smax(-1.0f, smin(1.0f, c))) * (180.0f / kPi);
+        if (deg >= m.spot_outer_deg) return V3(0.0f, 0.0f, 0.0f);
+        if (deg > m.spot_falloff_deg) {
+            const float t = (deg - m.spot_falloff_deg) / (m.spot_outer_deg - m.spot_falloff_deg);
+            float f = 1.0f - t;
+            f = f * f * (3.0f - 2.0f * f);
+            radiance = radiance * f;
+        }
+        return radiance;
+    }
+    return V3(0.0f, 0.0f, 0.0f);
+}
+
+__device__ __forceinline__ float area_to_angle(float area, float dist, float c) {
+    const float safe = smax(fabsf(c), 1e-4f);
+    return area * (dist * dist) / safe;
+}
+__device__ __forceinline__ float mis_balance(float a, float b) {
+    const float den = a + b;
+    return den <= 0.0f ? 0.0f : a / den;
+}
+__device__ __forceinline__ float mis_power(float a, float b) {
+    const float a2 = a * a, b2 = b * b;
+    const float den = a2 + b2;
+    return den <= 0.0f ? 0.0f : a2 / den;
+}
+__device__ __forceinline__ V3 vmin(const V3& a, float c) { return V3(smin(a.x, c), smin(a.y, c), smin(a.z, c)); }
+
+__device__ __forceinline__ V3 reflect(const V3& in, const V3& n) { return in - n * 2.0f * in.dot(n); }
+__device__ __forceinline__ V3 refract(const V3& d, const V3& n, float eta) {
+    const float ci = smin(1.0f, (d * -1.0f).dot(n));
+    const V3 perp = (d + n * ci) * eta;
+    const V3 par = n * -__builtin_sqrtf(fabsf(1.0f - perp.dot(perp)));
+    return perp + par;
+}
+__device__ __forceinline__ float schlick(float ci, float eta) {
+    float r0 = (1.0f - eta) / (1.0f + eta);
+    r0 = r0 * r0;
+    return r0 + (1.0f - r0) * powf((1.0f - ci), 5.0f);
+}
+__device__ __forceinline__ void frame(const V3& n, V3& t, V3& b) {
+    const V3 up = fabsf(n.x) > 0.9f ? V3(0.0f, 1.0f, 0.0f) : V3(1.0f, 0.0f, 0.0f);
+    t = n.cross(up).normalized();
+    b = n.cross(t);
+}
+__device__ __forceinline__ V3 cosine_hemisphere(const V3& n, float u1, float u2) {
+    const float r = __builtin_sqrtf(u1);
+    const float ang = 2.0f * kPi * u2;
+    const float x = r * cosf(ang);
+    const float z = r * sinf(ang);
+    const float y = __builtin_sqrtf(smax(0.0f, 1.0f - r * r));
+    V3 t, b;
+    frame(n, t, b);
+    return (t * x + n * y + b * z).normalized();
+}
+__device__ __forceinline__ V3 in_unit_sphere(PathRng& rng) {
+    V3 c;
+    for (int k = 0; k < kSphereTries; ++k) {
+        const uint32_t base = D_SPHERE + 3u * (uint32_t)k;
+        const float x = rng(base) * 2.0f - 1.0f;
+        const float y = rng(base + 1) * 2.0f - 1.0f;
+        const float z = rng(base + 2) * 2.0f - 1.0f;
+        c = V3(x, y, z);
+        if (c.dot(c) < 1.0f) return c;
+    }
+    return c;
+}
+
+// GGX conductor: Heitz 2018 visible-normal sampling, Smith separable G, Schlick Fresnel (F0 =
+// surface colour), alpha = roughness^2.
+__device__ __forceinline__ float ggx_alpha(float r) { return smax(r * r, 1e-4f); }
+__device__ __forceinline__ float ggx_d(const V3& m, float a) {
+    const float a2 = a * a;
+    const float k = (m.x * m.x + m.y * m.y) / a2 + m.z * m.z;
+    return 1.0f / (kPi * a2 * k * k);
+}
+__device__ __forceinline__ float ggx_g1(const V3& w, float a) {
+    const float s2 = a * a * (w.x * w.x + w.y * w.y);
+    return 2.0f * w.z / (w.z + __builtin_sqrtf(s2 + w.z * w.z));
+}
+__device__ __forceinline__ V3 schlick_rgb(const V3& f0, float c) {
+    const float k = powf(1.0f - smax(0.0f, smin(1.0f, c)), 5.0f);
+    return V3(f0.x + (1.0f - f0.x) * k, f0.y + (1.0f - f0.y) * k, f0.z + (1.0f - f0.z) * k);
+}
+__device__ __forceinline__ V3 to_local(const V3& w, const V3& t, const V3& b, const V3& n) {
+    return V3(w.dot(t), w.dot(b), w.dot(n));
+}
+__device__ __forceinline__ void ggx_eval(const V3& f0, float a, const V3& n, const V3& wo,
+                                         const V3& wi, V3& fcos, float& pdf) {
+    V3 t, b;
+    frame(n, t, b);
+    const V3 lo = to_local(wo, t, b, n), li = to_local(wi, t, b, n);
+    fcos = V3(0.0f, 0.0f, 0.0f);
+    pdf = 0.0f;
+    if (lo.z <= 0.0f || li.z <= 0.0f) return;
+    const V3 m = (lo + li).normalized();
+    const float d = ggx_d(m, a);
+    const float g1o = ggx_g1(lo, a);
+    const float g1i = ggx_g1(li, a);
+    const V3 F = schlick_rgb(f0, lo.dot(m));
+    const float k = d * g1o * g1i / (4.0f * lo.z);
+    fcos = F * k;
+    pdf = d * g1o / (4.0f * lo.z);
+}
+__device__ __forceinline__ bool ggx_sample(const V3& f0, float a, const V3& n, const V3& wo,
+                                           float u1, float u2, V3& wi, V3& weight, float& pdf) {
+    V3 t, b;
+    frame(n, t, b);
+    const V3 lo = to_local(wo, t, b, n);
+    if (lo.z <= 0.0f) return false;
+    const V3 vh = V3(a * lo.x, a * lo.y, lo.z).normalized();
+    const float lensq = vh.x * vh.x + vh.y * vh.y;
+    const V3 t1 = lensq > 0.0f ? V3(-vh.y, vh.x, 0.0f) / __builtin_sqrtf(lensq) : V3(1.0f, 0.0f, 0.0f);
+    const V3 t2 = vh.cross(t1);
+    const float r = __builtin_sqrtf(u1);
+    const float phi = 2.0f * kPi * u2;
+    const float p1 = r * cosf(phi);
+    float p2 = r * sinf(phi);
+    const float sh = 0.5f * (1.0f + vh.z);
+    p2 = (1.0f - sh) * __builtin_sqrtf(smax(0.0f, 1.0f - p1 * p1)) + sh * p2;
+    const V3 nh = t1 * p1 + t2 * p2 + vh * __builtin_sqrtf(smax(0.0f, 1.0f - p1 * p1 - p2 * p2));
+    const V3 m = V3(a * nh.x, a * nh.y, smax(1e-6f, nh.z)).normalized();
+    const float om = lo.dot(m);
+    const V3 li = m * (2.0f * om) - lo;
+    if (li.z <= 0.0f) return false;
+    weight = schlick_rgb(f0, om) * ggx_g1(li, a);
+    pdf = ggx_d(m, a) * ggx_g1(lo, a) / (4.0f * lo.z);
+    wi = (t * li.x + b * li.y + n * li.z).normalized();
+    return true;
+}
+__device__ __forceinline__ float fresnel_dielectric(float ci, float eta, float& ct) {
+    const float s2 = eta * eta * (1.0f - ci * ci);
+    if (s2 >= 1.0f) {
+        ct = 0.0f;
+        return 1.0f;
+    }
+    ct = __builtin_sqrtf(1.0f - s2);
+    const float rs = (eta * ci - ct) / (eta * ci + ct);
+    const float rp = (ci - eta * ct) / (ci + eta * ct);
+    return 0.5f * (rs * rs + rp * rp);
+}
+
+// Material::scatter (Material.cpp:165-266) + the microfacet models.
+__device__ __forceinline__ bool scatter(const SceneDev& s, const DevMaterial& m, const V3& d,
+                                        const V3& n, const V3& p, float u, float v, bool front,
+                                        PathRng& rng, V3& att, V3& dir, float& pdf) {
+    const V3 sc = surface_color(s, m, p, u, v);
+    switch (m.type) {
+    case WF_MAT_DIFFUSE: {
+        const float u1 = rng(D_DIR_U1);
+        const float u2 = rng(D_DIR_U2);
+        dir = cosine_hemisphere(n, u1, u2);
+        att = sc;
+        pdf = smax(0.0f, n.dot(dir)) / kPi;
+        return true;
+    }
+    case WF_MAT_METAL: {
+        if (m.model == WF_MODEL_MICROFACET) {
+            const V3 wo = (d * -1.0f).normalized();
+            const float u1 = rng(D_DIR_U1);
+            const float u2 = rng(D_DIR_U2);
+            return ggx_sample(sc, ggx_alpha(m.roughness), n, wo, u1, u2, dir, att, pdf);
+        }
+        const V3 reflected = reflect(d, n);
+        const V3 fuzz = in_unit_sphere(rng) * m.roughness;
+        dir = (reflected + fuzz).normalized();
+        att = sc;
+        pdf = -1.0f;
+        return dir.dot(n) > 0.0f;
+    }
+    case WF_MAT_EMISSIVE:
+    case WF_MAT_SPOTLIGHT:
+        att = sc * m.emission;
+        pdf = -1.0f;
+        return false;
+    case WF_MAT_GLASS: {
+        att = sc;
+        pdf = -1.0f;
+        const float eta = front ? (1.0f / m.ior) : m.ior;
+        const V3 id = d.normalized();
+        const float ci = smin(1.0f, (id * -1.0f).dot(n));
+        if (m.model == WF_MODEL_MICROFACET) {
+            float ct;
+            const float F = fresnel_dielectric(ci, eta, ct);
+            if (rng(D_LOBE) < F)
+                dir = reflect(id, n);
+            else
+                dir = id * eta + n * (eta * ci - ct);
+            return true;
+        }
+        const float si = __builtin_sqrtf(1.0f - ci * ci);
+        const bool tir = eta * si > 1.0f;
+        if (tir || schlick(ci, eta) > rng(D_LOBE))
+            dir = reflect(id, n);
+        else
+            dir = refract(id, n, eta);
+        return true;
+    }
+    case WF_MAT_PLASTIC: {
+        const float ci = smax(0.0f, (d * -1.0f).dot(n));
+        const float R = schlick(ci, 1.5f);
+        if (rng(D_LOBE) < R) {
+            const V3 reflected = reflect(d.normalized(), n);
+            const V3 fuzz = in_unit_sphere(rng) * m.roughness;
+            dir = (reflected + fuzz).normalized();
+            att = V3(1.0f, 1.0f, 1.0f);
+            pdf = -1.0f;
+        } else {
+            const float u1 = rng(D_DIR_U1);
+            const float u2 = rng(D_DIR_U2);
+            dir = cosine_hemisphere(n, u1, u2);
+            att = sc;
+            pdf = smax(0.0f, n.dot(dir)) / kPi;
+        }
+        return true;
+    }
+    default:
+        pdf = -1.0f;
+        return false;
+    }
+}
+
+// EnvironmentMap::sample mapping (EnvironmentMap.cpp:30-48): nearest texel of the direction.
+__device__ __forceinline__ int env_texel(const SceneDev& s, const V3& dir, float& lat_cos) {
+    const V3 d = dir.normalized();
+    const float u = 0.5f + atan2f(d.z, d.x) / (2.0f * 3.14159274f);
+    const float v = 0.5f - asinf(smax(-1.0f, smin(1.0f, d.y))) / 3.14159274f;
+    int x = (int)(u * s.env_w);
+    int y = (int)(v * s.env_h);
+    x = x < 0 ? 0 : (x > s.env_w - 1 ? s.env_w - 1 : x);
+    y = y < 0 ? 0 : (y > s.env_h - 1 ? s.env_h - 1 : y);
+    lat_cos = __builtin_sqrtf(smax(0.0f, 1.0f - d.y * d.y));
+    return y * s.env_w + x;
+}
+__device__ __forceinline__ V3 env_radiance(const SceneDev& s, int texel_i) {
+    const float4 r = s.env[texel_i];
+    return V3(r.x, r.y, r.z) * s.env_intensity * V3(s.env_tint[0], s.env_tint[1], s.env_tint[2]);
+}
+__device__ __forceinline__ float env_pdf_sa(const SceneDev& s, int texel_i, float lc) {
+    if (s.env_choice_prob <= 0.0f || lc <= 0.0f) return 0.0f;
+    const float p = s.env_choice_prob * s.env_pdf[texel_i] * (float)s.env_w * (float)s.env_h;
+    return p / (2.0f * kPi * kPi * lc);
+}
+
+__global__ void k_shade(const SceneDev s, wf_shade_soa q, int n, ShadeArgs a,
+                        const double* __restrict__ mat_rel_cost, float* __restrict__ accum,
+                        ShadeOut o) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t pixel = q.pixel[i];
+    const int32_t depth = q.depth[i];
+    const float pdf_in = q.pdf[i];
+    const V3 P = ld3(q.px, q.py, q.pz, i);
+    const V3 N = ld3(q.nx, q.ny, q.nz, i);
+    const V3 D = ld3(q.dx, q.dy, q.dz, i);
+    const V3 T(q.tr[i], q.tg[i], q.tb[i]);
+    const float t = q.t[i];
+    const float u = q.u[i], v = q.v[i];
+    const int mat_id = q.mat[i];
+    const bool front = q.front ? q.front[i] != 0 : true;
+    const DevMaterial m = s.materials[mat_id];
+    const bool unbiased = a.physics == WF_PHYSICS_UNBIASED;
+    PathRng rng{a.seed, (uint32_t)a.job, (uint32_t)pixel, (uint32_t)a.sample, (uint32_t)depth};
+    float* acc = accum + (size_t)pixel * 3;
+    uint32_t sflag = 0;
+
+    // emission + MIS (cpp:629-658)
+    V3 e = emitted(m, N, D * -1.0f);
+    if (e.x > 0.0f || e.y > 0.0f || e.z > 0.0f) {
+        float w = 1.0f;
+        if (depth > 0 && pdf_in > 0.0f) {
+            const float cl = smax(0.0f, (D * -1.0f).dot(N));
+            if (cl > 0.0f) {
+                if (unbiased) {
+                    const float pa = m.light_pdf_area * (1.0f - s.env_choice_prob);
+                    w = mis_power(pdf_in, area_to_angle(pa, t, cl));
+                } else {
+                    const float lpa = s.total_emissive_area <= 0.0f ? 0.0f : 1.0f / s.total_emissive_area;
+                    w = mis_balance(pdf_in, area_to_angle(lpa, t, cl));
+                }
+            }
+        }
+        e = vmin(e, a.firefly);
+        const V3 c = T * e * w;
+        acc[0] = acc[0] + c.x;
+        acc[1] = acc[1] + c.y;
+        acc[2] = acc[2] + c.z;
+    }
+
+    // next-event estimation
+    const bool ggx = m.type == WF_MAT_METAL && m.model == WF_MODEL_MICROFACET;
+    const bool nee = m.type == WF_MAT_DIFFUSE || m.type == WF_MAT_PLASTIC || (unbiased && ggx);
+    if (nee && !unbiased) {
+        // Scene::sampleLight (Scene.cpp:192-237)
+        const int ne = s.n_emissive_meshes;
+        if (ne > 0) {
+            const int ei = (int)(rng(D_LIGHT_PICK) * (float)ne) % ne;
+            const int mi = s.emissive_meshes[ei];
+            const int tc = s.mesh_tri_count[mi];
+            if (tc > 0) {
+                const int k = (int)(rng(D_LIGHT_TRI) * (float)tc) % tc;
+                const int slot = s.emissive_light_base[ei] + k;
+                float r1 = rng(D_LIGHT_R1);
+                float r2 = rng(D_LIGHT_R2);
+                if (r1 + r2 > 1.0f) {
+                    r1 = 1.0f - r1;
+                    r2 = 1.0f - r2;
+                }
+                const float r3 = 1.0f - r1 - r2;
+                const V3 v0 = xyz(s.light_v0[slot]);
+                const V3 v1 = xyz(s.light_v1[slot]), v2 = xyz(s.light_v2[slot]);
+                const V3 lp = v0 * r3 + v1 * r1 + v2 * r2;
+                const V3 ln = (v1 - v0).cross(v2 - v0).normalized();
+                const V3 dfl = (P - lp).normalized();
+                const V3 le = emitted(s.materials[s.mesh_material[mi]], ln, dfl);
+                const V3 tl = lp - P;
+                const float dist = tl.length();
+                const V3 dl = tl / dist;
+                const float cs = N.dot(dl);
+                const float cl = (dl * -1.0f).dot(ln);
+                if (cs > 0.0f && cl > 0.0f) {
+                    const V3 sc = surface_color(s, m, P, u, v);
+                    const float geometry = (cs * cl) / (dist * dist);
+                    const V3 brdf = sc * (1.0f / kPi);
+                    const float lpa = s.total_emissive_area <= 0.0f ? 0.0f : 1.0f / s.total_emissive_area;
+                    const float lps = area_to_angle(lpa, dist, cl);
+                    const float bps = cs / kPi;
+                    const float w = mis_balance(lps, bps);
+                    const V3 direct = vmin(T * brdf * le * geometry * s.total_emissive_area * w, a.firefly);
+                    st3(o.ldx, o.ldy, o.ldz, i, dl);
+                    o.lmax[i] = dist;
+                    o.lr[i] = direct.x;
+                    o.lg[i] = direct.y;
+                    o.lb[i] = direct.z;
+                    sflag |= 1u;
+                }
+            }
+        }
+    } else if (nee && unbiased) {
+        const bool use_env = s.env_choice_prob > 0.0f &&
+                             (s.n_light_tris == 0 || rng(D_ENV_CHOICE) < s.env_choice_prob);
+        const V3 wo = (D * -1.0f).normalized();
+        const V3 sc = surface_color(s, m, P, u, v);
+        V3 wi, le;
+        float lpdf = 0.0f, maxd = 0.0f;
+        bool ok = false;
+        if (use_env) {
+            const int nt = s.env_w * s.env_h;
+            int slot = (int)(rng(D_ENV_SLOT) * (float)nt);
+            slot = slot < nt - 1 ? slot : nt - 1;
+            if (rng(D_ENV_COIN) >= s.env_prob[slot]) slot = s.env_alias[slot];
+            const int tx = slot % s.env_w, ty = slot / s.env_w;
+            const float eu = ((float)tx + rng(D_ENV_JX)) / (float)s.env_w;
+            const float ev = ((float)ty + rng(D_ENV_JY)) / (float)s.env_h;
+            const float phi = (eu - 0.5f) * (2.0f * kPi);
+            const float lat = (0.5f - ev) * kPi;
+            const float cl = cosf(lat);
+            wi = V3(cl * cosf(phi), sinf(lat), cl * sinf(phi));
+            if (N.dot(wi) > 0.0f) {
+                float lc;
+                const int tx2 = env_texel(s, wi, lc);
+                lpdf = env_pdf_sa(s, tx2, lc);
+                le = env_radiance(s, tx2);
+                maxd = 1e6f;
+                ok = lpdf > 0.0f;
+            }
+        } else if (s.n_light_tris > 0) {
+            const int nl = s.n_light_tris;
+            int slot = (int)(rng(D_LIGHT_PICK) * (float)nl);
+            slot = slot < nl - 1 ? slot : nl - 1;
+            if (rng(D_LIGHT_TRI) >= s.light_prob[slot]) slot = s.light_alias[slot];
+            float r1 = rng(D_LIGHT_R1);
+            float r2 = rng(D_LIGHT_R2);
+            if (r1 + r2 > 1.0f) {
+                r1 = 1.0f - r1;
+                r2 = 1.0f - r2;
+            }
+            const float r3 = 1.0f - r1 - r2;
+            const V3 v0 = xyz(s.light_v0[slot]);
+            const V3 v1 = xyz(s.light_v1[slot]), v2 = xyz(s.light_v2[slot]);
+            const V3 lp = v0 * r3 + v1 * r1 + v2 * r2;
+            const V3 ln = (v1 - v0).cross(v2 - v0).normalized();
+            const int lmat = s.light_mat[slot];
+            const V3 tl = lp - P;
+            const float dist = tl.length();
+            const V3 dl = tl / dist;
+            const float cs = N.dot(dl);
+            const float cl = fabsf((dl * -1.0f).dot(ln));
+            if (cs > 0.0f && cl > 0.0f) {
+                const DevMaterial lm = s.materials[lmat];
+                le = emitted(lm, ln, dl * -1.0f);
+                const float pa = lm.light_pdf_area * (1.0f - s.env_choice_prob);
+                lpdf = area_to_angle(pa, dist, cl);
+                wi = dl;
+                maxd = dist;
+                ok = lpdf > 0.0f;
+            }
+        }
+        if (ok) {
+            V3 fcos;
+            float bpdf;
+            if (ggx) {
+                ggx_eval(sc, ggx_alpha(m.roughness), N, wo, wi, fcos, bpdf);
+            } else {
+                const float c = N.dot(wi);
+                fcos = sc * (1.0f / kPi) * smax(0.0f, c);
+                bpdf = smax(0.0f, c) / kPi;
+            }
+            const float w = mis_power(lpdf, bpdf);
+            const V3 direct = vmin(T * fcos * le * (w / lpdf), a.firefly);
+            st3(o.ldx, o.ldy, o.ldz, i, wi);
+            o.lmax[i] = maxd;
+            o.lr[i] = direct.x;
+            o.lg[i] = direct.y;
+            o.lb[i] = direct.z;
+            sflag |= 1u;
+        }
+    }
+    // directional light (cpp:712-737), every material type
+    if (s.directional.intensity > 0.0f) {
+        const V3 ldir(s.directional.direction[0], s.directional.direction[1], s.directional.direction[2]);
+        const V3 tl = ldir * -1.0f;
+        const float cs = N.dot(tl);
+        if (cs > 0.0f) {
+            const V3 sc = surface_color(s, m, P, u, v);
+            const V3 brdf = sc * (1.0f / kPi);
+            const V3 col(s.directional.color[0], s.directional.color[1], s.directional.color[2]);
+            const V3 direct = vmin(T * brdf * col * s.directional.intensity * cs, a.firefly);
+            o.dr[i] = direct.x;
+            o.dg[i] = direct.y;
+            o.db[i] = direct.z;
+            sflag |= 2u;
+        }
+    }
+    o.shadow_flag[i] = sflag;
+
+    // scatter + Russian roulette (cpp:739-796)
+    V3 att, dir;
+    float spdf = -1.0f;
+    bool alive = scatter(s, m, D, N, P, u, v, front, rng, att, dir, spdf);
+    V3 nt = T * att;
+    if (alive && depth + 1 >= a.rr_min_depth) {
+        const float base = smin(0.95f, smax(smax(nt.x, nt.y), nt.z));
+        float surv = base;
+        if (a.cost_aware_rr) {
+            const float rc = (float)mat_rel_cost[mat_id];
+            const float x = base / rc;
+            surv = x < 0.05f ? 0.05f : (0.95f < x ? 0.95f : x);
+        }
+        if (rng(D_RR) > surv)
+            alive = false;
+        else
+            nt = nt / surv;
+    }
+    o.alive[i] = alive ? 1u : 0u;
+    if (alive) {
+        st3(o.ndx, o.ndy, o.ndz, i, dir);
+        o.ntr[i] = nt.x;
+        o.ntg[i] = nt.y;
+        o.ntb[i] = nt.z;
+        o.npdf[i] = spdf;
+    }
+}
+
+__global__ void k_flags_nonzero(const uint32_t* __restrict__ in, int n, uint32_t* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[i] != 0u ? 1u : 0u;
+}
+
+__global__ void k_compact_index(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ prefix,
+                                int n, int32_t* __restrict__ list) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && flag[i]) list[prefix[i]] = i;
+}
+
+__global__ void k_next_rays(wf_shade_soa q, ShadeOut o, const uint32_t* __restrict__ prefix, int n,
+                            wf_ray_soa next) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !o.alive[i]) return;
+    const int k = (int)prefix[i];
+    next.ox[k] = q.px[i];
+    next.oy[k] = q.py[i];
+    next.oz[k] = q.pz[i];
+    next.dx[k] = o.ndx[i];
+    next.dy[k] = o.ndy[i];
+    next.dz[k] = o.ndz[i];
+    next.tr[k] = o.ntr[i];
+    next.tg[k] = o.ntg[i];
+    next.tb[k] = o.ntb[i];
+    next.pixel[k] = q.pixel[i];
+    next.depth[k] = q.depth[i] + 1;
+    next.pdf[k] = o.npdf[i];
+}
+
+// Miss loop (cpp:256-265): accum += T * getSkyColor(dir); the unbiased track weights environment
+// hits of non-delta bounces by the power heuristic against environment NEE.
+__global__ void k_miss(const SceneDev s, wf_ray_soa q, int n, int physics, float* __restrict__ accum) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const V3 d(q.dx[i], q.dy[i], q.dz[i]);
+    const V3 T(q.tr[i], q.tg[i], q.tb[i]);
+    V3 c;
+    if (s.env_w > 0) {
+        float lc;
+        const int tx = env_texel(s, d, lc);
+        float w = 1.0f;
+        if (physics == WF_PHYSICS_UNBIASED && q.depth[i] > 0 && q.pdf[i] > 0.0f)
+            w = mis_power(q.pdf[i], env_pdf_sa(s, tx, lc));
+        c = physics == WF_PHYSICS_UNBIASED ? T * env_radiance(s, tx) * w : T * env_radiance(s, tx);
+    } else {
+        const float blend = 0.5f * (d.normalized().y + 1.0f);
+        const V3 dark(0.02f, 0.02f, 0.05f), horizon(0.05f, 0.04f, 0.03f);
+        c = T * (horizon * (1.0f - blend) + dark * blend);
+    }
+    float* acc = accum + (size_t)q.pixel[i] * 3;
+    acc[0] = acc[0] + c.x;
+    acc[1] = acc[1] + c.y;
+    acc[2] = acc[2] + c.z;
+}
+
+// EmbreeAccelerator::occluded for the NEE rays of each listed entry, then the contributions in
+// the reference order: area/environment light first, directional light second (cpp:678, 720).
+__global__ __launch_bounds__(kTraceBlock) void k_shadow(const SceneDev s, wf_shade_soa q, ShadeOut o,
+                                                        const int32_t* __restrict__ list, int count,
+                                                        float* __restrict__ accum) {
+    __shared__ int32_t stack[kStack * kTraceBlock];
+    const int k = blockIdx.x * kTraceBlock + threadIdx.x;
+    if (k >= count) return;
+    const int i = list[k];
+    const uint32_t f = o.shadow_flag[i];
+    const V3 P = ld3(q.px, q.py, q.pz, i);
+    float* acc = accum + (size_t)q.pixel[i] * 3;
+    if (f & 1u) {
+        const float tfar = o.lmax[i] - 0.001f;
+        TraceHit h;
+        const bool occ = tfar < 0.0f ||
+                         traverse<true>(s.nodes, s.tri, s.n_tris, P, ld3(o.ldx, o.ldy, o.ldz, i),
+                                        0.001f, tfar, stack, h);
+        if (!occ) {
+            acc[0] = acc[0] + o.lr[i];
+            acc[1] = acc[1] + o.lg[i];
+            acc[2] = acc[2] + o.lb[i];
+        }
+    }
+    if (f & 2u) {
+        const V3 tl = V3(s.directional.direction[0], s.directional.direction[1],
+                         s.directional.direction[2]) * -1.0f;
+        TraceHit h;
+        const bool occ = traverse<true>(s.nodes, s.tri, s.n_tris, P, tl, 0.001f, 1e6f - 0.001f, stack, h);
+        if (!occ) {
+            acc[0] = acc[0] + o.dr[i];
+            acc[1] = acc[1] + o.dg[i];
+            acc[2] = acc[2] + o.db[i];
+        }
+    }
+}
+
+__global__ void k_count_shadow_rays(const uint32_t* __restrict__ flag, const int32_t* __restrict__ list,
+                                    int count, int32_t* __restrict__ total) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = 0;
+    if (k < count) {
+        const uint32_t f = flag[list[k]];
+        c = (int)(f & 1u) + (int)((f >> 1) & 1u);
+    }
+    for (int off = 32; off >= 1; off >>= 1) c += __shfl_xor(c, off, 64);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(total, c);
+}
+
+static wf_status shade_outputs(wf_context* ctx, int n, ShadeOut& o) {
+    void* p = nullptr;
+    const size_t nn = (size_t)(n > 0 ? n : 1);
+    WF_TRY(workspace(ctx, "shade_out", nn * 4 * 20, &p));
+    float* f = (float*)p;
+    o.ldx = f + 0 * nn; o.ldy = f + 1 * nn; o.ldz = f + 2 * nn; o.lmax = f + 3 * nn;
+    o.lr = f + 4 * nn; o.lg = f + 5 * nn; o.lb = f + 6 * nn;
+    o.dr = f + 7 * nn; o.dg = f + 8 * nn; o.db = f + 9 * nn;
+    o.shadow_flag = (uint32_t*)(f + 10 * nn);
+    o.ndx = f + 11 * nn; o.ndy = f + 12 * nn; o.ndz = f + 13 * nn;
+    o.ntr = f + 14 * nn; o.ntg = f + 15 * nn; o.ntb = f + 16 * nn;
+    o.npdf = f + 17 * nn;
+    o.alive = (uint32_t*)(f + 18 * nn);
+    return WF_OK;
+}
+
+wf_status shade_bounce(const wf_scene* s, const wf_shade_soa* q, int32_t n,
+                       const wf_shade_params* p, float* accum, wf_ray_soa* next,
+                       int32_t* counts_dev) {
+    WF_TRY(scene_ready(s));
+    wf_context* ctx = s->ctx;
+    if (n <= 0) {
+        WF_TRY(wf_fill_u32(ctx, counts_dev, 0u, 2));
+        return WF_OK;
+    }
+    ShadeOut o;
+    WF_TRY(shade_outputs(ctx, n, o));
+    void *pprefix, *plist, *pshflag;
+    const size_t nn = (size_t)n;
+    WF_TRY(workspace(ctx, "shade_prefix", nn * 4, &pprefix));
+    WF_TRY(workspace(ctx, "shade_list", nn * 4, &plist));
+    WF_TRY(workspace(ctx, "shade_sflag", nn * 4, &pshflag));
+    ShadeArgs a{p->physics, p->rr_min_depth, p->cost_aware_rr, p->firefly_threshold, p->seed,
+                p->sample_index, p->job};
+    {
+        Timed tm(ctx, "shade", 120.0 * n);
+        k_shade<<<grid_for(n, 128), 128, 0, ctx->stream>>>(s->dev, *q, n, a, s->dev.mat_rel_cost,
+                                                           accum, o);
+        WF_HIP(hipGetLastError());
+    }
+    // shadow rays: compact the entries that emitted one, trace, accumulate
+    k_flags_nonzero<<<grid_for(n, 256), 256, 0, ctx->stream>>>(o.shadow_flag, n, (uint32_t*)pshflag);
+    WF_HIP(hipGetLastError());
+    int32_t* counts = counts_dev;
+    WF_TRY(exclusive_scan_u32(ctx, (uint32_t*)pshflag, (uint32_t*)pprefix, n, counts + 2, "compact"));
+    k_compact_index<<<grid_for(n, 256), 256, 0, ctx->stream>>>((uint32_t*)pshflag,
+                                                              (uint32_t*)pprefix, n, (int32_t*)plist);
+    WF_HIP(hipGetLastError());
+    // the shadow kernel runs over n entries worst case; entries past the count exit at once
+    int32_t n_shadow = 0;
+    WF_TRY(readback(ctx, &n_shadow, counts + 2, sizeof(int32_t)));
+    WF_TRY(wf_fill_u32(ctx, counts + 1, 0u, 1));
+    if (n_shadow > 0) {
+        k_count_shadow_rays<<<grid_for(n_shadow, 256), 256, 0, ctx->stream>>>(
+            o.shadow_flag, (int32_t*)plist, n_shadow, counts + 1);
+        WF_HIP(hipGetLastError());
+        Timed tm(ctx, "shadow", 41.0 * n_shadow);
+        k_shadow<<<grid_for(n_shadow, kTraceBlock), kTraceBlock, 0, ctx->stream>>>(
+            s->dev, *q, o, (int32_t*)plist, n_shadow, accum);
+        WF_HIP(hipGetLastError());
+    }
+    // next rays: stable compaction in shading order
+    WF_TRY(exclusive_scan_u32(ctx, o.alive, (uint32_t*)pprefix, n, counts, "compact"));
+    {
+        Timed tm(ctx, "next_rays", 76.0 * n);
+        k_next_rays<<<grid_for(n, 256), 256, 0, ctx->stream>>>(*q, o, (uint32_t*)pprefix, n, *next);
+        WF_HIP(hipGetLastError());
+    }
+    return WF_OK;
+}
+
+wf_status shade_misses(const wf_scene* s, const wf_ray_soa* q, int32_t n, int32_t physics,
+                       float* accum) {
+    if (n <= 0) return WF_OK;
+    wf_context* ctx = s->ctx;
+    Timed tm(ctx, "miss", 52.0 * n);
+    k_miss<<<grid_for(n, 256), 256, 0, ctx->stream>>>(s->dev, *q, n, physics, accum);
+    WF_HIP(hipGetLastError());
+    return WF_OK;
+}
+
+}  // namespace wf
+
+using namespace wf;
+
+extern "C" {
+
+wf_status wf_shade(const wf_scene* s, const wf_shade_soa* queue, int32_t n,
+                   const wf_shade_params* params, float* accum_rgb, wf_ray_soa* next,
+                   int32_t* n_next) {
+    clear_error();
+    WF_REQUIRE(s && queue && params && accum_rgb && next && n_next, "wf_shade: NULL argument");
+    WF_REQUIRE(n >= 0, "wf_shade: negative count");
+    WF_REQUIRE(params->physics == WF_PHYSICS_REFERENCE || params->physics == WF_PHYSICS_UNBIASED,
+               "wf_shade: unknown physics %d", params->physics);
+    void* pc = nullptr;
+    WF_TRY(workspace(s->ctx, "shade_counts", sizeof(int32_t) * 4, &pc));
+    WF_TRY(shade_bounce(s, queue, n, params, accum_rgb, next, (int32_t*)pc));
+    int32_t c[2] = {0, 0};
+    WF_TRY(readback(s->ctx, c, pc, sizeof(c)));
+    *n_next = c[0];
+    return WF_OK;
+}
+
+wf_status wf_shade_misses(const wf_scene* s, const wf_ray_soa* misses, int32_t n,
+                          int32_t physics, float* accum_rgb) {
+    clear_error();
+    WF_REQUIRE(s && misses && accum_rgb, "wf_shade_misses: NULL argument");
+    WF_REQUIRE(n >= 0, "wf_shade_misses: negative count");
+    WF_TRY(shade_misses(s, misses, n, physics, accum_rgb));
+    WF_HIP(hipStreamSynchronize(s->ctx->stream));
+    return WF_OK;
+}
+
+}  // extern "C"

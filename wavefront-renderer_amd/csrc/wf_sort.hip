@@ -1,0 +1,301 @@
+// Device-wide primitives: stable LSD radix sort of (u64 key, i32 value) pairs and an exclusive
+// prefix sum over u32 counts. Both are deterministic: ranks inside a tile are computed with
+// wave-level ballots in tile order, never with atomics whose order could vary.
+//
+// Radix sort pass (8-bit digit, 2048-key tiles, 256 threads = 4 waves of 64):
+//   1. k_radix_hist    per-tile digit histogram -> hist[digit * tiles + tile]
+//   2. exclusive scan  over hist (digit-major), giving each (digit, tile) its output base
+//   3. k_radix_scatter each wave ranks its 512 keys round by round: lanes holding the same digit
+//      are matched with 8 ballots, the rank is the popcount of matching lower lanes plus the
+//      wave's running count for that digit; waves are then offset by an exclusive prefix over
+//      waves, so equal digits keep input order (stability).
+#include "wf_device.h"
+#include "wf_internal.h"
+
+namespace wf {
+
+constexpr int kSortBlock = 256;
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kSortBlock * kSortItems;
+constexpr int kSortWaves = kSortBlock / 64;
+
+__global__ __launch_bounds__(kSortBlock) void k_radix_hist(const uint64_t* __restrict__ keys,
+                                                           int n, int shift, uint32_t mask,
+                                                           uint32_t* __restrict__ hist,
+                                                           int tiles) {
+    __shared__ uint32_t cnt[256];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * kSortTile;
+#pragma unroll
+    for (int k = 0; k < kSortItems; ++k) {
+        int i = base + k * kSortBlock + threadIdx.x;
+        if (i < n) atomicAdd(&cnt[(uint32_t)(keys[i] >> shift) & mask], 1u);
+    }
+    __syncthreads();
+    hist[(size_t)threadIdx.x * tiles + blockIdx.x] = cnt[threadIdx.x];
+}
+
+__global__ __launch_bounds__(kSortBlock) void k_radix_scatter(
+    const uint64_t* __restrict__ keys_in, const int32_t* __restrict__ vals_in,
+    uint64_t* __restrict__ keys_out, int32_t* __restrict__ vals_out, int n, int shift,
+    uint32_t mask, int nbits, const uint32_t* __restrict__ offsets, int tiles) {
+    __shared__ uint32_t wave_cnt[kSortWaves][256];
+    __shared__ uint32_t digit_base[256];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int w = 0; w < kSortWaves; ++w) wave_cnt[w][threadIdx.x] = 0;
+    digit_base[threadIdx.x] = offsets[(size_t)threadIdx.x * tiles + blockIdx.x];
+    __syncthreads();
+
+    const int chunk = blockIdx.x * kSortTile + wave * 64 * kSortItems;
+    uint64_t k[kSortItems];
+    int32_t v[kSortItems];
+    uint32_t d[kSortItems];
+    uint32_t r[kSortItems];
+    const uint64_t lt = lanemask_lt();
+#pragma unroll
+    for (int it = 0; it < kSortItems; ++it) {
+        const int i = chunk + it * 64 + lane;
+        const bool valid = i < n;
+        k[it] = valid ? keys_in[i] : 0ull;
+        v[it] = valid ? (vals_in ? vals_in[i] : i) : 0;
+        const uint32_t dig = valid ? ((uint32_t)(k[it] >> shift) & mask) : 0u;
+        uint64_t peers = __ballot(valid);
+        for (int b = 0; b < nbits; ++b) {
+            const bool bit = (dig >> b) & 1u;
+            const uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const uint32_t before = wave_cnt[wave][dig];
+        const uint32_t rank_in = (uint32_t)__popcll(peers & lt);
+        r[it] = before + rank_in;
+        d[it] = dig;
+        if (valid && rank_in == 0) wave_cnt[wave][dig] = before + (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    {
+        uint32_t run = 0;
+#pragma unroll
+        for (int w = 0; w < kSortWaves; ++w) {
+            uint32_t c = wave_cnt[w][threadIdx.x];
+            wave_cnt[w][threadIdx.x] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < kSortItems; ++it) {
+        const int i = chunk + it * 64 + lane;
+        if (i < n) {
+            const uint32_t pos = digit_base[d[it]] + wave_cnt[wave][d[it]] + r[it];
+            keys_out[pos] = k[it];
+            vals_out[pos] = v[it];
+        }
+    }
+}
+
+// ---- exclusive scan ----------------------------------------------------------------------------
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t x, uint32_t* lds,
+                                                         uint32_t* block_total) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    uint32_t v = x;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(v, o, 64);
+        if (lane >= o) v += y;
+    }
+    if (lane == 63) lds[wave] = v;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kScanBlock / 64; ++w) {
+        uint32_t t = lds[w];
+        if (w < wave) off += t;
+        tot += t;
+    }
+    __syncthreads();
+    *block_total = tot;
+    return off + v - x;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const uint32_t* __restrict__ in,
+                                                            int n,
+                                                            uint32_t* __restrict__ sums) {
+    __shared__ uint32_t lds[kScanBlock / 64];
+    const int base = blockIdx.x * kScanTile;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        int i = base + k * kScanBlock + threadIdx.x;
+        if (i < n) s += in[i];
+    }
+    uint32_t tot;
+    block_exclusive_scan(s, lds, &tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_down(const uint32_t* in, uint32_t* out,
+                                                          int n,
+                                                          const uint32_t* __restrict__ offsets,
+                                                          int32_t* __restrict__ total) {
+    __shared__ uint32_t tile[kScanTile];
+    __shared__ uint32_t lds[kScanBlock / 64];
+    const int base = blockIdx.x * kScanTile;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        int i = base + k * kScanBlock + threadIdx.x;
+        tile[k * kScanBlock + threadIdx.x] = i < n ? in[i] : 0u;
+    }
+    __syncthreads();
+    uint32_t x[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        x[k] = tile[threadIdx.x * kScanItems + k];
+        s += x[k];
+    }
+    uint32_t tot;
+    uint32_t run = block_exclusive_scan(s, lds, &tot) + (offsets ? offsets[blockIdx.x] : 0u);
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        tile[threadIdx.x * kScanItems + k] = run;
+        run += x[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        int i = base + k * kScanBlock + threadIdx.x;
+        if (i < n) {
+            uint32_t e = tile[k * kScanBlock + threadIdx.x];
+            if (total && i == n - 1) *total = (int32_t)(e + in[i]);
+            out[i] = e;
+        }
+    }
+}
+
+__global__ void k_iota(int32_t* dst, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = i;
+}
+
+__global__ void k_fill_u32(uint32_t* dst, uint32_t value, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = value;
+}
+
+__global__ void k_set_i32(int32_t* dst, int32_t value) { *dst = value; }
+
+static wf_status scan_level(wf_context* ctx, const uint32_t* in, uint32_t* out, int n,
+                            int32_t* total, int level, const char* tag) {
+    const int tiles = (n + kScanTile - 1) / kScanTile;
+    if (tiles <= 1) {
+        Timed t(ctx, tag, 8.0 * n);
+        k_scan_down<<<1, kScanBlock, 0, ctx->stream>>>(in, out, n, nullptr, total);
+        WF_HIP(hipGetLastError());
+        return WF_OK;
+    }
+    char name[32];
+    snprintf(name, sizeof(name), "scan_sums_%d", level);
+    void* p = nullptr;
+    WF_TRY(workspace(ctx, name, sizeof(uint32_t) * (size_t)tiles, &p));
+    uint32_t* sums = (uint32_t*)p;
+    {
+        Timed t(ctx, tag, 4.0 * n);
+        k_scan_reduce<<<tiles, kScanBlock, 0, ctx->stream>>>(in, n, sums);
+        WF_HIP(hipGetLastError());
+    }
+    WF_TRY(scan_level(ctx, sums, sums, tiles, nullptr, level + 1, tag));
+    {
+        Timed t(ctx, tag, 8.0 * n);
+        k_scan_down<<<tiles, kScanBlock, 0, ctx->stream>>>(in, out, n, sums, total);
+        WF_HIP(hipGetLastError());
+    }
+    return WF_OK;
+}
+
+wf_status exclusive_scan_u32(wf_context* ctx, const uint32_t* in, uint32_t* out, int32_t n,
+                             int32_t* total_dev, const char* tag) {
+    if (n <= 0) {
+        if (total_dev) {
+            k_set_i32<<<1, 1, 0, ctx->stream>>>(total_dev, 0);
+            WF_HIP(hipGetLastError());
+        }
+        return WF_OK;
+    }
+    return scan_level(ctx, in, out, n, total_dev, 0, tag);
+}
+
+wf_status iota_i32(wf_context* ctx, int32_t* dst, int32_t n) {
+    if (n <= 0) return WF_OK;
+    k_iota<<<grid_for(n, 256), 256, 0, ctx->stream>>>(dst, n);
+    WF_HIP(hipGetLastError());
+    return WF_OK;
+}
+
+wf_status radix_sort_pairs(wf_context* ctx, const uint64_t* keys_in, const int32_t* vals_in,
+                           uint64_t* keys_out, int32_t* vals_out, int32_t n, int32_t key_bits,
+                           const char* tag) {
+    if (n <= 0) return WF_OK;
+    if (key_bits <= 0) {
+        // nothing to order by: identity permutation of the values
+        if (vals_in)
+            WF_HIP(hipMemcpyAsync(vals_out, vals_in, sizeof(int32_t) * n, hipMemcpyDeviceToDevice,
+                                  ctx->stream));
+        else
+            WF_TRY(iota_i32(ctx, vals_out, n));
+        WF_HIP(hipMemcpyAsync(keys_out, keys_in, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice,
+                              ctx->stream));
+        return WF_OK;
+    }
+    const int tiles = (n + kSortTile - 1) / kSortTile;
+    const int passes = (key_bits + 7) / 8;
+    void *pk = nullptr, *pv = nullptr, *ph = nullptr;
+    WF_TRY(workspace(ctx, "sort_keys_alt", sizeof(uint64_t) * (size_t)n, &pk));
+    WF_TRY(workspace(ctx, "sort_vals_alt", sizeof(int32_t) * (size_t)n, &pv));
+    WF_TRY(workspace(ctx, "sort_hist", sizeof(uint32_t) * 256 * (size_t)tiles, &ph));
+    uint32_t* hist = (uint32_t*)ph;
+    // ping-pong so that the last pass writes keys_out/vals_out
+    const uint64_t* ksrc = keys_in;
+    const int32_t* vsrc = vals_in;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = p * 8;
+        const int nbits = (key_bits - shift) < 8 ? (key_bits - shift) : 8;
+        const uint32_t mask = (1u << nbits) - 1u;
+        const bool to_out = ((passes - 1 - p) % 2) == 0;
+        uint64_t* kdst = to_out ? keys_out : (uint64_t*)pk;
+        int32_t* vdst = to_out ? vals_out : (int32_t*)pv;
+        {
+            Timed t(ctx, tag, 8.0 * n);
+            k_radix_hist<<<tiles, kSortBlock, 0, ctx->stream>>>(ksrc, n, shift, mask, hist, tiles);
+            WF_HIP(hipGetLastError());
+        }
+        WF_TRY(exclusive_scan_u32(ctx, hist, hist, 256 * tiles, nullptr, tag));
+        {
+            Timed t(ctx, tag, 24.0 * n);
+            k_radix_scatter<<<tiles, kSortBlock, 0, ctx->stream>>>(
+                ksrc, vsrc, kdst, vdst, n, shift, mask, nbits, hist, tiles);
+            WF_HIP(hipGetLastError());
+        }
+        ksrc = kdst;
+        vsrc = vdst;
+    }
+    return WF_OK;
+}
+
+}  // namespace wf
+
+extern "C" wf_status wf_fill_u32(wf_context* ctx, void* dst, uint32_t value, size_t count) {
+    WF_REQUIRE(ctx, "wf_fill_u32: NULL context");
+    if (count == 0) return WF_OK;
+    wf::k_fill_u32<<<(unsigned)((count + 255) / 256), 256, 0, ctx->stream>>>((uint32_t*)dst, value,
+                                                                           count);
+    WF_HIP(hipGetLastError());
+    return WF_OK;
+}

@@ -1,0 +1,255 @@
+// Ray traversal replacing EmbreeAccelerator::intersect4/intersect/occluded
+// (src/accelerator/Embree.cpp:122-279) and the hit/miss split of WavefrontRenderer::intersectAll
+// (src/render/WavefrontRenderer.cpp:459-567).
+//
+// One lane per ray. The traversal stack lives in LDS, one 64-entry column per lane
+// (stack[depth][lane] so that a wave's pushes and pops hit 64 different banks). The triangle test
+// is the reference's Moller-Trumbore (include/geometry/Triangle.h:6-42) with the same float
+// operations; closest hit = lexicographic minimum of (t, global triangle index) over every
+// triangle the test accepts in [tnear, tfar], which makes the result independent of traversal
+// order. Box tests are conservative (boxes are padded at build time).
+#include "wf_traverse.h"
+
+namespace wf {
+
+__global__ __launch_bounds__(kTraceBlock) void k_trace_closest(
+    const BvhNode* __restrict__ nodes, const float4* __restrict__ tri, int n_tris,
+    const float* __restrict__ ox, const float* __restrict__ oy, const float* __restrict__ oz,
+    const float* __restrict__ dx, const float* __restrict__ dy, const float* __restrict__ dz,
+    int n, float tnear, float tfar, int32_t* __restrict__ out_leaf, float* __restrict__ out_t,
+    float* __restrict__ out_u, float* __restrict__ out_v, uint32_t* __restrict__ out_flag) {
+    __shared__ int32_t stack[kStack * kTraceBlock];
+    const int i = blockIdx.x * kTraceBlock + threadIdx.x;
+    if (i >= n) return;
+    TraceHit h;
+    traverse<false>(nodes, tri, n_tris, V3(ox[i], oy[i], oz[i]), V3(dx[i], dy[i], dz[i]), tnear,
+                    tfar, stack, h);
+    out_leaf[i] = h.leaf;
+    out_t[i] = h.t;
+    out_u[i] = h.u;
+    out_v[i] = h.v;
+    if (out_flag) out_flag[i] = h.leaf >= 0 ? 1u : 0u;
+}
+
+// occluded(): tnear 0.001, tfar = maxDistance - 0.001; a negative tfar reports occluded
+// (Embree.cpp:268-278).
+__global__ __launch_bounds__(kTraceBlock) void k_trace_occluded(
+    const BvhNode* __restrict__ nodes, const float4* __restrict__ tri, int n_tris,
+    const float* __restrict__ ox, const float* __restrict__ oy, const float* __restrict__ oz,
+    const float* __restrict__ dx, const float* __restrict__ dy, const float* __restrict__ dz,
+    const float* __restrict__ maxd, int n, uint8_t* __restrict__ out) {
+    __shared__ int32_t stack[kStack * kTraceBlock];
+    const int i = blockIdx.x * kTraceBlock + threadIdx.x;
+    if (i >= n) return;
+    const float tfar = maxd[i] - 0.001f;
+    bool occ = tfar < 0.0f;
+    if (!occ) {
+        TraceHit h;
+        occ = traverse<true>(nodes, tri, n_tris, V3(ox[i], oy[i], oz[i]),
+                             V3(dx[i], dy[i], dz[i]), 0.001f, tfar, stack, h);
+    }
+    out[i] = occ ? 1 : 0;
+}
+
+// HitRecord mapping (Embree.cpp:240-252) + stable split by the exclusive prefix of hit flags.
+__global__ void k_split(const SceneDev s, wf_ray_soa in, int n, const int32_t* __restrict__ leaf,
+                        const float* __restrict__ ht, const float* __restrict__ hu,
+                        const float* __restrict__ hv, const uint32_t* __restrict__ prefix,
+                        wf_shade_soa out, wf_ray_soa miss) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int L = leaf[i];
+    const uint32_t pos = prefix[i];
+    const V3 o(in.ox[i], in.oy[i], in.oz[i]);
+    const V3 d(in.dx[i], in.dy[i], in.dz[i]);
+    if (L >= 0) {
+        const int j = (int)pos;
+        const float4 A = s.tri[3 * (size_t)L];
+        const float4 B = s.tri[3 * (size_t)L + 1];
+        const float4 C = s.tri[3 * (size_t)L + 2];
+        const int32_t g = __float_as_int(A.w);
+        const float t = ht[i];
+        const float bu = hu[i], bv = hv[i];
+        const V3 P = o + d * t;
+        const int32_t geom = s.tri_geom[g];
+        const int32_t mat = s.mesh_material[geom];
+        float u = bu, v = bv;
+        if (s.mesh_has_uv[geom]) {
+            const float* uv = s.tri_uv + 6 * (size_t)g;
+            const float w0 = 1.0f - bu - bv;
+            u = w0 * uv[0] + bu * uv[2] + bv * uv[4];
+            v = w0 * uv[1] + bu * uv[3] + bv * uv[5];
+        }
+        const V3 outward = xyz(B).cross(xyz(C)).normalized();
+        const bool ff = d.dot(outward) < 0.0f;
+        const V3 N = ff ? outward : outward * -1.0f;
+        st3(out.px, out.py, out.pz, j, P);
+        st3(out.nx, out.ny, out.nz, j, N);
+        out.t[j] = t;
+        out.u[j] = u;
+        out.v[j] = v;
+        out.tri[j] = g - s.mesh_tri_offset[geom];
+        out.mat[j] = mat;
+        out.tex[j] = s.materials[mat].texture_id;
+        st3(out.ox, out.oy, out.oz, j, o);
+        st3(out.dx, out.dy, out.dz, j, d);
+        out.tr[j] = in.tr[i];
+        out.tg[j] = in.tg[i];
+        out.tb[j] = in.tb[i];
+        out.pixel[j] = in.pixel[i];
+        out.depth[j] = in.depth[i];
+        out.pdf[j] = in.pdf[i];
+        if (out.geom) out.geom[j] = geom;
+        if (out.front) out.front[j] = ff ? 1 : 0;
+    } else {
+        const int m = i - (int)pos;
+        st3(miss.ox, miss.oy, miss.oz, m, o);
+        st3(miss.dx, miss.dy, miss.dz, m, d);
+        miss.tr[m] = in.tr[i];
+        miss.tg[m] = in.tg[i];
+        miss.tb[m] = in.tb[i];
+        miss.pixel[m] = in.pixel[i];
+        miss.depth[m] = in.depth[i];
+        miss.pdf[m] = in.pdf[i];
+    }
+}
+
+__global__ void k_hits_to_soa(const SceneDev s, int n, const int32_t* __restrict__ leaf,
+                              const float* __restrict__ ht, const float* __restrict__ hu,
+                              const float* __restrict__ hv, wf_hit_soa out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int L = leaf[i];
+    if (L >= 0) {
+        const int32_t g = __float_as_int(s.tri[3 * (size_t)L].w);
+        const int32_t geom = s.tri_geom[g];
+        out.geom[i] = geom;
+        out.prim[i] = g - s.mesh_tri_offset[geom];
+    } else {
+        out.geom[i] = -1;
+        out.prim[i] = -1;
+    }
+    out.t[i] = ht[i];
+    out.bu[i] = hu[i];
+    out.bv[i] = hv[i];
+}
+
+wf_status scene_ready(const wf_scene* s) {
+    if (!s) return set_error(WF_ERR_INVALID_ARGUMENT, "NULL scene");
+    if (!s->dev.built) return set_error(WF_ERR_NOT_BUILT, "accelerator not built (call wf_accel_build)");
+    if (s->dev.max_depth > kStack)
+        return set_error(WF_ERR_UNSUPPORTED, "BVH depth %d exceeds the traversal stack (%d)",
+                         s->dev.max_depth, kStack);
+    return WF_OK;
+}
+
+wf_status trace_closest(const wf_scene* s, const wf_ray_soa* r, int32_t n, float tnear,
+                        float tfar, int32_t* hit_leaf, float* hit_t, float* hit_bu, float* hit_bv,
+                        uint32_t* flags) {
+    WF_TRY(scene_ready(s));
+    if (n <= 0) return WF_OK;
+    wf_context* ctx = s->ctx;
+    Timed tm(ctx, "traverse_closest", 40.0 * n);
+    k_trace_closest<<<grid_for(n, kTraceBlock), kTraceBlock, 0, ctx->stream>>>(
+        s->dev.nodes, s->dev.tri, s->dev.n_tris, r->ox, r->oy, r->oz, r->dx, r->dy, r->dz, n,
+        tnear, tfar, hit_leaf, hit_t, hit_bu, hit_bv, flags);
+    WF_HIP(hipGetLastError());
+    return WF_OK;
+}
+
+wf_status trace_occluded(const wf_scene* s, const float* ox, const float* oy, const float* oz,
+                         const float* dx, const float* dy, const float* dz, const float* maxd,
+                         int32_t n, uint8_t* out) {
+    WF_TRY(scene_ready(s));
+    if (n <= 0) return WF_OK;
+    wf_context* ctx = s->ctx;
+    Timed tm(ctx, "traverse_any", 29.0 * n);
+    k_trace_occluded<<<grid_for(n, kTraceBlock), kTraceBlock, 0, ctx->stream>>>(
+        s->dev.nodes, s->dev.tri, s->dev.n_tris, ox, oy, oz, dx, dy, dz, maxd, n, out);
+    WF_HIP(hipGetLastError());
+    return WF_OK;
+}
+
+// Closest hit + stable split; counts_dev[0] = hits, counts_dev[1] = misses (device int32).
+wf_status intersect_split(const wf_scene* s, const wf_ray_soa* rays, int32_t n,
+                          wf_shade_soa* hits_out, wf_ray_soa* misses_out, int32_t* counts_dev) {
+    wf_context* ctx = s->ctx;
+    void *pl, *pt, *pu, *pv, *pf;
+    size_t nn = (size_t)(n > 0 ? n : 1);
+    WF_TRY(workspace(ctx, "hit_leaf", sizeof(int32_t) * nn, &pl));
+    WF_TRY(workspace(ctx, "hit_t", sizeof(float) * nn, &pt));
+    WF_TRY(workspace(ctx, "hit_u", sizeof(float) * nn, &pu));
+    WF_TRY(workspace(ctx, "hit_v", sizeof(float) * nn, &pv));
+    WF_TRY(workspace(ctx, "hit_flag", sizeof(uint32_t) * nn, &pf));
+    WF_TRY(trace_closest(s, rays, n, 0.001f, 1e9f, (int32_t*)pl, (float*)pt, (float*)pu,
+                         (float*)pv, (uint32_t*)pf));
+    WF_TRY(exclusive_scan_u32(ctx, (uint32_t*)pf, (uint32_t*)pf, n, counts_dev, "split"));
+    if (n > 0) {
+        Timed tm(ctx, "split", 97.0 * n);
+        k_split<<<grid_for(n, 256), 256, 0, ctx->stream>>>(s->dev, *rays, n, (int32_t*)pl,
+                                                           (float*)pt, (float*)pu, (float*)pv,
+                                                           (uint32_t*)pf, *hits_out, *misses_out);
+        WF_HIP(hipGetLastError());
+    }
+    return WF_OK;
+}
+
+}  // namespace wf
+
+using namespace wf;
+
+extern "C" {
+
+wf_status wf_intersect(const wf_scene* s, const wf_ray_soa* rays, int32_t n, float tnear,
+                       float tfar, wf_hit_soa* hits) {
+    clear_error();
+    WF_REQUIRE(s && rays && hits, "wf_intersect: NULL argument");
+    WF_REQUIRE(n >= 0, "wf_intersect: negative ray count");
+    WF_TRY(scene_ready(s));
+    if (n == 0) return WF_OK;
+    wf_context* ctx = s->ctx;
+    void *pl, *pt, *pu, *pv;
+    WF_TRY(workspace(ctx, "hit_leaf", sizeof(int32_t) * n, &pl));
+    WF_TRY(workspace(ctx, "hit_t", sizeof(float) * n, &pt));
+    WF_TRY(workspace(ctx, "hit_u", sizeof(float) * n, &pu));
+    WF_TRY(workspace(ctx, "hit_v", sizeof(float) * n, &pv));
+    WF_TRY(trace_closest(s, rays, n, tnear, tfar, (int32_t*)pl, (float*)pt, (float*)pu,
+                         (float*)pv, nullptr));
+    k_hits_to_soa<<<grid_for(n, 256), 256, 0, ctx->stream>>>(s->dev, n, (int32_t*)pl, (float*)pt,
+                                                             (float*)pu, (float*)pv, *hits);
+    WF_HIP(hipGetLastError());
+    WF_HIP(hipStreamSynchronize(ctx->stream));
+    return WF_OK;
+}
+
+wf_status wf_occluded(const wf_scene* s, const float* ox, const float* oy, const float* oz,
+                      const float* dx, const float* dy, const float* dz, const float* max_dist,
+                      int32_t n, uint8_t* occluded) {
+    clear_error();
+    WF_REQUIRE(s && ox && oy && oz && dx && dy && dz && max_dist && occluded,
+               "wf_occluded: NULL argument");
+    WF_REQUIRE(n >= 0, "wf_occluded: negative ray count");
+    WF_TRY(trace_occluded(s, ox, oy, oz, dx, dy, dz, max_dist, n, occluded));
+    WF_HIP(hipStreamSynchronize(s->ctx->stream));
+    return WF_OK;
+}
+
+wf_status wf_intersect_split(const wf_scene* s, const wf_ray_soa* rays, int32_t n,
+                             wf_shade_soa* hits_out, int32_t* n_hits, wf_ray_soa* misses_out,
+                             int32_t* n_misses) {
+    clear_error();
+    WF_REQUIRE(s && rays && hits_out && misses_out && n_hits && n_misses,
+               "wf_intersect_split: NULL argument");
+    WF_REQUIRE(n >= 0, "wf_intersect_split: negative ray count");
+    WF_TRY(scene_ready(s));
+    void* pc = nullptr;
+    WF_TRY(workspace(s->ctx, "split_count", sizeof(int32_t) * 2, &pc));
+    WF_TRY(intersect_split(s, rays, n, hits_out, misses_out, (int32_t*)pc));
+    int32_t h = 0;
+    WF_TRY(readback(s->ctx, &h, pc, sizeof(int32_t)));
+    *n_hits = h;
+    *n_misses = n - h;
+    return WF_OK;
+}
+
+}  // extern "C"
