@@ -215,6 +215,12 @@ wf_status wf_scene_device_bytes(const wf_scene* scene, size_t* out);
  * for rays [0,n) with tnear/tfar; writes the raw hit per ray (prim < 0 = miss). */
 wf_status wf_intersect(const wf_scene* scene, const wf_ray_soa* rays, int32_t n, float tnear,
                        float tfar, wf_hit_soa* hits);
+/* EmbreeAccelerator::intersect4 / intersect with the HitRecord mapping (Embree.cpp:164-174,
+ * 240-252) for rays [0,n): records slot i receives ray i's HitRecord (plus the ray state) when
+ * hit_flags[i] == 1. Any tnear/tfar (Scene::hit passes its own range, Scene.cpp:120-160). */
+wf_status wf_intersect_records(const wf_scene* scene, const wf_ray_soa* rays, int32_t n,
+                               float tnear, float tfar, wf_shade_soa* records,
+                               uint8_t* hit_flags);
 /* EmbreeAccelerator::occluded (Embree.cpp:258-279) for n shadow rays: occluded[i] = 1 when any
  * triangle lies in [0.001, max_dist[i]-0.001] (and, as in the reference, whenever that upper
  * bound is negative). Inputs are device arrays. */

@@ -216,6 +216,15 @@ class Scene:
         check(self.ctx._lib.wf_intersect(self.handle, C.byref(r), n, tnear, tfar, C.byref(h)))
         return hits.download(n)
 
+    # EmbreeAccelerator::intersect4 with the HitRecord mapping, one record per input ray
+    def intersect_records(self, rays: RayQueue, n: int, tnear: float = 0.001, tfar: float = 1e9):
+        rec = ShadingQueue(self.ctx, n)
+        flags = DeviceBuffer(self.ctx, max(n, 1), np.uint8)
+        r, q = rays.soa(), rec.soa()
+        check(self.ctx._lib.wf_intersect_records(self.handle, C.byref(r), n, tnear, tfar,
+                                                 C.byref(q), C.c_void_p(flags.ptr)))
+        return rec.download(n), flags.download(n).astype(bool)
+
     # EmbreeAccelerator::occluded over a batch
     def occluded(self, origins: np.ndarray, dirs: np.ndarray, max_dist: np.ndarray) -> np.ndarray:
         n = len(max_dist)

@@ -145,6 +145,8 @@ _PROTOS = {
     "wf_scene_device_bytes": (_S, [P, C.POINTER(C.c_size_t)]),
     "wf_intersect": (_S, [P, C.POINTER(RaySoA), C.c_int32, C.c_float, C.c_float,
                           C.POINTER(HitSoA)]),
+    "wf_intersect_records": (_S, [P, C.POINTER(RaySoA), C.c_int32, C.c_float, C.c_float,
+                                  C.POINTER(ShadeSoA), P]),
     "wf_occluded": (_S, [P, P, P, P, P, P, P, P, C.c_int32, P]),
     "wf_intersect_split": (_S, [P, C.POINTER(RaySoA), C.c_int32, C.POINTER(ShadeSoA),
                                 C.POINTER(C.c_int32), C.POINTER(RaySoA), C.POINTER(C.c_int32)]),

@@ -51,7 +51,49 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_occluded(
     out[i] = occ ? 1 : 0;
 }
 
-// HitRecord mapping (Embree.cpp:240-252) + stable split by the exclusive prefix of hit flags.
+// HitRecord mapping of EmbreeAccelerator::intersect4 (Embree.cpp:240-252) for ray i into slot j.
+__device__ __forceinline__ void map_hit(const SceneDev& s, const wf_ray_soa& in, int i, int L,
+                                        float t, float bu, float bv, wf_shade_soa& out, int j) {
+    const V3 o(in.ox[i], in.oy[i], in.oz[i]);
+    const V3 d(in.dx[i], in.dy[i], in.dz[i]);
+    const float4 A = s.tri[3 * (size_t)L];
+    const float4 B = s.tri[3 * (size_t)L + 1];
+    const float4 C = s.tri[3 * (size_t)L + 2];
+    const int32_t g = __float_as_int(A.w);
+    const V3 P = o + d * t;
+    const int32_t geom = s.tri_geom[g];
+    const int32_t mat = s.mesh_material[geom];
+    float u = bu, v = bv;
+    if (s.mesh_has_uv[geom]) {
+        const float* uv = s.tri_uv + 6 * (size_t)g;
+        const float w0 = 1.0f - bu - bv;
+        u = w0 * uv[0] + bu * uv[2] + bv * uv[4];
+        v = w0 * uv[1] + bu * uv[3] + bv * uv[5];
+    }
+    const V3 outward = xyz(B).cross(xyz(C)).normalized();
+    const bool ff = d.dot(outward) < 0.0f;
+    const V3 N = ff ? outward : outward * -1.0f;
+    st3(out.px, out.py, out.pz, j, P);
+    st3(out.nx, out.ny, out.nz, j, N);
+    out.t[j] = t;
+    out.u[j] = u;
+    out.v[j] = v;
+    out.tri[j] = g - s.mesh_tri_offset[geom];
+    out.mat[j] = mat;
+    out.tex[j] = s.materials[mat].texture_id;
+    st3(out.ox, out.oy, out.oz, j, o);
+    st3(out.dx, out.dy, out.dz, j, d);
+    out.tr[j] = in.tr[i];
+    out.tg[j] = in.tg[i];
+    out.tb[j] = in.tb[i];
+    out.pixel[j] = in.pixel[i];
+    out.depth[j] = in.depth[i];
+    out.pdf[j] = in.pdf[i];
+    if (out.geom) out.geom[j] = geom;
+    if (out.front) out.front[j] = ff ? 1 : 0;
+}
+
+// Stable split by the exclusive prefix of hit flags (intersectAll's hit and miss queues).
 __global__ void k_split(const SceneDev s, wf_ray_soa in, int n, const int32_t* __restrict__ leaf,
                         const float* __restrict__ ht, const float* __restrict__ hu,
                         const float* __restrict__ hv, const uint32_t* __restrict__ prefix,
@@ -60,51 +102,16 @@ __global__ void k_split(const SceneDev s, wf_ray_soa in, int n, const int32_t* _
     if (i >= n) return;
     const int L = leaf[i];
     const uint32_t pos = prefix[i];
-    const V3 o(in.ox[i], in.oy[i], in.oz[i]);
-    const V3 d(in.dx[i], in.dy[i], in.dz[i]);
     if (L >= 0) {
-        const int j = (int)pos;
-        const float4 A = s.tri[3 * (size_t)L];
-        const float4 B = s.tri[3 * (size_t)L + 1];
-        const float4 C = s.tri[3 * (size_t)L + 2];
-        const int32_t g = __float_as_int(A.w);
-        const float t = ht[i];
-        const float bu = hu[i], bv = hv[i];
-        const V3 P = o + d * t;
-        const int32_t geom = s.tri_geom[g];
-        const int32_t mat = s.mesh_material[geom];
-        float u = bu, v = bv;
-        if (s.mesh_has_uv[geom]) {
-            const float* uv = s.tri_uv + 6 * (size_t)g;
-            const float w0 = 1.0f - bu - bv;
-            u = w0 * uv[0] + bu * uv[2] + bv * uv[4];
-            v = w0 * uv[1] + bu * uv[3] + bv * uv[5];
-        }
-        const V3 outward = xyz(B).cross(xyz(C)).normalized();
-        const bool ff = d.dot(outward) < 0.0f;
-        const V3 N = ff ? outward : outward * -1.0f;
-        st3(out.px, out.py, out.pz, j, P);
-        st3(out.nx, out.ny, out.nz, j, N);
-        out.t[j] = t;
-        out.u[j] = u;
-        out.v[j] = v;
-        out.tri[j] = g - s.mesh_tri_offset[geom];
-        out.mat[j] = mat;
-        out.tex[j] = s.materials[mat].texture_id;
-        st3(out.ox, out.oy, out.oz, j, o);
-        st3(out.dx, out.dy, out.dz, j, d);
-        out.tr[j] = in.tr[i];
-        out.tg[j] = in.tg[i];
-        out.tb[j] = in.tb[i];
-        out.pixel[j] = in.pixel[i];
-        out.depth[j] = in.depth[i];
-        out.pdf[j] = in.pdf[i];
-        if (out.geom) out.geom[j] = geom;
-        if (out.front) out.front[j] = ff ? 1 : 0;
+        map_hit(s, in, i, L, ht[i], hu[i], hv[i], out, (int)pos);
     } else {
         const int m = i - (int)pos;
-        st3(miss.ox, miss.oy, miss.oz, m, o);
-        st3(miss.dx, miss.dy, miss.dz, m, d);
+        miss.ox[m] = in.ox[i];
+        miss.oy[m] = in.oy[i];
+        miss.oz[m] = in.oz[i];
+        miss.dx[m] = in.dx[i];
+        miss.dy[m] = in.dy[i];
+        miss.dz[m] = in.dz[i];
         miss.tr[m] = in.tr[i];
         miss.tg[m] = in.tg[i];
         miss.tb[m] = in.tb[i];
@@ -112,6 +119,18 @@ __global__ void k_split(const SceneDev s, wf_ray_soa in, int n, const int32_t* _
         miss.depth[m] = in.depth[i];
         miss.pdf[m] = in.pdf[i];
     }
+}
+
+// One HitRecord per input ray (slot i), hit flag per ray.
+__global__ void k_records(const SceneDev s, wf_ray_soa in, int n, const int32_t* __restrict__ leaf,
+                          const float* __restrict__ ht, const float* __restrict__ hu,
+                          const float* __restrict__ hv, wf_shade_soa out,
+                          uint8_t* __restrict__ flags) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int L = leaf[i];
+    flags[i] = L >= 0 ? 1 : 0;
+    if (L >= 0) map_hit(s, in, i, L, ht[i], hu[i], hv[i], out, i);
 }
 
 __global__ void k_hits_to_soa(const SceneDev s, int n, const int32_t* __restrict__ leaf,
@@ -217,6 +236,29 @@ wf_status wf_intersect(const wf_scene* s, const wf_ray_soa* rays, int32_t n, flo
                          (float*)pv, nullptr));
     k_hits_to_soa<<<grid_for(n, 256), 256, 0, ctx->stream>>>(s->dev, n, (int32_t*)pl, (float*)pt,
                                                              (float*)pu, (float*)pv, *hits);
+    WF_HIP(hipGetLastError());
+    WF_HIP(hipStreamSynchronize(ctx->stream));
+    return WF_OK;
+}
+
+wf_status wf_intersect_records(const wf_scene* s, const wf_ray_soa* rays, int32_t n, float tnear,
+                               float tfar, wf_shade_soa* records, uint8_t* hit_flags) {
+    clear_error();
+    WF_REQUIRE(s && rays && records && hit_flags, "wf_intersect_records: NULL argument");
+    WF_REQUIRE(n >= 0, "wf_intersect_records: negative ray count");
+    WF_TRY(scene_ready(s));
+    if (n == 0) return WF_OK;
+    wf_context* ctx = s->ctx;
+    void *pl, *pt, *pu, *pv;
+    WF_TRY(workspace(ctx, "hit_leaf", sizeof(int32_t) * n, &pl));
+    WF_TRY(workspace(ctx, "hit_t", sizeof(float) * n, &pt));
+    WF_TRY(workspace(ctx, "hit_u", sizeof(float) * n, &pu));
+    WF_TRY(workspace(ctx, "hit_v", sizeof(float) * n, &pv));
+    WF_TRY(trace_closest(s, rays, n, tnear, tfar, (int32_t*)pl, (float*)pt, (float*)pu,
+                         (float*)pv, nullptr));
+    k_records<<<grid_for(n, 256), 256, 0, ctx->stream>>>(s->dev, *rays, n, (int32_t*)pl,
+                                                         (float*)pt, (float*)pu, (float*)pv,
+                                                         *records, hit_flags);
     WF_HIP(hipGetLastError());
     WF_HIP(hipStreamSynchronize(ctx->stream));
     return WF_OK;
