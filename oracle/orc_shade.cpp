@@ -657,6 +657,68 @@ int32_t orc_scatter_replay(void* sp, const wf_material* m, const float* d, const
     return ok ? 1 : 0;
 }
 
+// Unbiased-track building blocks, exported for closed-form tests.
+int32_t orc_ggx_sample(const float* f0, float roughness, const float* n, const float* wo, float u1,
+                       float u2, float* wi, float* weight, float* pdf) {
+    V3 w, wt;
+    float p = 0.0f;
+    bool ok = ggx_sample(V3(f0[0], f0[1], f0[2]), ggx_alpha(roughness), V3(n[0], n[1], n[2]),
+                         V3(wo[0], wo[1], wo[2]), u1, u2, w, wt, p);
+    wi[0] = w.x; wi[1] = w.y; wi[2] = w.z;
+    weight[0] = wt.x; weight[1] = wt.y; weight[2] = wt.z;
+    *pdf = p;
+    return ok ? 1 : 0;
+}
+
+void orc_ggx_eval(const float* f0, float roughness, const float* n, const float* wo,
+                  const float* wi, float* fcos, float* pdf) {
+    V3 f;
+    float p;
+    ggx_eval(V3(f0[0], f0[1], f0[2]), ggx_alpha(roughness), V3(n[0], n[1], n[2]),
+             V3(wo[0], wo[1], wo[2]), V3(wi[0], wi[1], wi[2]), f, p);
+    fcos[0] = f.x; fcos[1] = f.y; fcos[2] = f.z;
+    *pdf = p;
+}
+
+float orc_fresnel_dielectric(float cos_i, float eta) {
+    float ct;
+    return fresnel_dielectric(cos_i, eta, ct);
+}
+
+float orc_mis_power(float a, float b) { return mis_power(a, b); }
+float orc_mis_balance(float a, float b) { return mis_balance(a, b); }
+
+void orc_cosine_hemisphere(const float* n, float u1, float u2, float* out) {
+    V3 d = cosine_hemisphere(V3(n[0], n[1], n[2]), u1, u2);
+    out[0] = d.x; out[1] = d.y; out[2] = d.z;
+}
+
+// Light tables of the unbiased track: emitter alias table and per-material area pdf.
+int32_t orc_light_tables(void* sp, float* prob, int32_t* alias, int32_t* tris, float* mat_pdf,
+                         int32_t cap) {
+    const Scene& s = *(Scene*)sp;
+    int n = (int)s.light_tris.size();
+    for (int i = 0; i < n && i < cap; ++i) {
+        prob[i] = s.light_prob[i];
+        alias[i] = s.light_alias[i];
+        tris[i] = s.light_tris[i];
+    }
+    for (size_t m = 0; m < s.mat_light_pdf_area.size(); ++m) mat_pdf[m] = s.mat_light_pdf_area[m];
+    return n;
+}
+
+int32_t orc_env_tables(void* sp, float* prob, int32_t* alias, float* pdf, float* choice) {
+    const Scene& s = *(Scene*)sp;
+    int n = (int)s.env_prob.size();
+    for (int i = 0; i < n; ++i) {
+        prob[i] = s.env_prob[i];
+        alias[i] = s.env_alias[i];
+        pdf[i] = s.env_pdf[i];
+    }
+    *choice = s.env_choice_prob;
+    return n;
+}
+
 // Material::emitted (Material.cpp:268-297).
 void orc_emitted(const wf_material* m, const float* ln, const float* dfl, float* out) {
     V3 e = emitted(*m, V3(ln[0], ln[1], ln[2]), V3(dfl[0], dfl[1], dfl[2]));
