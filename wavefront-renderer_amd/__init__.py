@@ -8,6 +8,7 @@ moves data and calls it. Names follow the reference renderer's surfaces: `RayQue
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 from typing import Dict, Optional
 
 import numpy as np
@@ -35,9 +36,12 @@ class Context:
         check(self._lib.wf_context_create(device, C.byref(h)))
         self.handle = h
         self.device = device
+        self._scenes = weakref.WeakSet()
 
     def close(self):
         if self.handle:
+            for sc in list(self._scenes):   # scenes first: they were created on this context
+                sc.close()
             self._lib.wf_context_destroy(self.handle)
             self.handle = None
 
@@ -176,6 +180,7 @@ class Scene:
         h = C.c_void_p()
         check(ctx._lib.wf_scene_create(ctx.handle, C.byref(desc), C.byref(h)))
         self.handle = h
+        ctx._scenes.add(self)
         self.stats = None
         if build:
             self.build()

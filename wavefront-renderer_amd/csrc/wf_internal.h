@@ -187,6 +187,7 @@ struct SceneDev {
 
 struct wf_scene {
     wf_context* ctx = nullptr;
+    int device = 0;            // kept so that destroy does not depend on the context
     wf::SceneDev dev;
     std::vector<void*> allocations;
     size_t device_bytes = 0;

@@ -362,6 +362,7 @@ wf_status wf_scene_create(wf_context* ctx, const wf_scene_desc* desc, wf_scene**
     WF_HIP(hipSetDevice(ctx->device));
     auto* s = new wf_scene();
     s->ctx = ctx;
+    s->device = ctx->device;
     auto fail = [&](wf_status st) {
         for (void* p : s->allocations) (void)hipFree(p);
         delete s;
@@ -600,8 +601,9 @@ wf_status wf_scene_create(wf_context* ctx, const wf_scene_desc* desc, wf_scene**
 
 wf_status wf_scene_destroy(wf_scene* s) {
     if (!s) return WF_OK;
-    (void)hipSetDevice(s->ctx->device);
-    (void)hipStreamSynchronize(s->ctx->stream);
+    // the owning context may already be gone: synchronise the device, not the context stream
+    (void)hipSetDevice(s->device);
+    (void)hipDeviceSynchronize();
     for (void* p : s->allocations) (void)hipFree(p);
     delete s;
     return WF_OK;
