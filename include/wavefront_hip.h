@@ -187,6 +187,10 @@ wf_status wf_memcpy_htod(wf_context* ctx, void* dst, const void* src, size_t byt
 wf_status wf_memcpy_dtoh(wf_context* ctx, void* dst, const void* src, size_t bytes);
 wf_status wf_memcpy_dtod(wf_context* ctx, void* dst, const void* src, size_t bytes);
 wf_status wf_fill_u32(wf_context* ctx, void* dst, uint32_t value, size_t count);
+/* Streaming device copy (16-byte loads and stores, grid-stride) of `bytes` (a multiple of 16).
+ * No reference counterpart: it measures the achievable HBM bandwidth beside the hot path and
+ * calibrates hardware byte counters against a kernel of known traffic. */
+wf_status wf_stream_copy(wf_context* ctx, void* dst, const void* src, size_t bytes);
 wf_status wf_synchronize(wf_context* ctx);
 
 /* ---- scene and accelerator ------------------------------------------------------------------- */
@@ -195,8 +199,10 @@ typedef struct wf_build_stats {   /* EmbreeAccelerator m_meshCount/m_totalTriang
     int32_t mesh_count;
     int32_t triangle_count;
     int32_t vertex_count;
-    int32_t node_count;
-    int32_t max_depth;
+    int32_t node_count;           /* four-wide BVH nodes */
+    int32_t max_depth;            /* depth of the binary LBVH the four-wide BVH is collapsed from */
+    int32_t stack_bound;          /* most entries the traversal stack can hold (<= 64 supported) */
+    int32_t reserved;
     double build_ms;              /* device time of the BVH build kernels */
     float bounds_min[3];          /* Scene::computeBounds over all vertices (Scene.cpp:59-77) */
     float bounds_max[3];
@@ -206,7 +212,8 @@ typedef struct wf_build_stats {   /* EmbreeAccelerator m_meshCount/m_totalTriang
 wf_status wf_scene_create(wf_context* ctx, const wf_scene_desc* desc, wf_scene** out);
 wf_status wf_scene_destroy(wf_scene* scene);
 /* Scene::buildAccelerator + EmbreeAccelerator::build (Scene.cpp:79-113, Embree.cpp:66-119):
- * GPU LBVH over the flattened triangle soup, emitter tables and bounds. */
+ * GPU LBVH over the flattened triangle soup collapsed to four-wide nodes, emitter tables and
+ * bounds. */
 wf_status wf_accel_build(wf_scene* scene, wf_build_stats* stats);
 /* Device bytes the scene and its accelerator hold (admission accounting). */
 wf_status wf_scene_device_bytes(const wf_scene* scene, size_t* out);

@@ -81,7 +81,70 @@ def test_build_stats(cornell):
     assert st.triangle_count == 5132 and st.mesh_count == 7
     bmin, bmax, area, ntri = osc.info()
     assert list(st.bounds_min) == bmin.tolist() and list(st.bounds_max) == bmax.tolist()
-    assert 0 < st.max_depth <= 64
+    assert 0 < st.max_depth
+    # four-wide collapse: at most 3 pushes per wide node on a root-to-leaf path
+    assert 0 < st.stack_bound <= min(64, 3 * ((st.max_depth + 1) // 2))
+    assert st.node_count < st.triangle_count
+
+
+def chain_scene(scenes):
+    """Triangles centred on (2^-k, 0, 0), (0, 2^-k, 0) and (0, 0, 2^-k), k = 1..15, inside the
+    centroid bounds [0, 1]^3: each splits the 48-bit Morton code one bit further from the
+    origin's, so the binary LBVH is about 45 levels deep; a dense cluster adds a balanced
+    subtree. The four-wide collapse sees both one-sided and balanced parts."""
+    a = 0.01
+    centres = [(0.0, 0.0, 0.0), (1.0, 1.0, 1.0)]
+    for k in range(1, 16):
+        x = 2.0 ** -k
+        centres += [(x, 0.0, 0.0), (0.0, x, 0.0), (0.0, 0.0, x)]
+    rng = np.random.default_rng(7)
+    centres += [tuple(p) for p in rng.uniform(0.2, 0.3, (600, 3))]
+    c = np.array(centres, np.float64)
+    tris = np.stack([c + [-a, -a, 0], c + [a, -a, 0], c + [0, 2 * a, 0]], 1)
+    pos = tris.astype(f32).reshape(-1, 3)
+    idx = np.arange(len(pos), dtype=np.int32).reshape(-1, 3)
+    return scenes.SceneData([scenes.Mesh(pos, idx, 0)], [{"type": 0}]), c
+
+
+def test_deep_tree_closest_hit_matches_bruteforce(wf, ctx, scenes, orc, rng):
+    data, centres = chain_scene(scenes)
+    sc = wf.Scene(ctx, data)
+    print("chain scene: binary depth", sc.stats.max_depth, "stack bound", sc.stats.stack_bound)
+    assert sc.stats.max_depth > 32 and sc.stats.stack_bound <= 64
+    osc = orc.scene(data)
+    n = 6000
+    o = rng.uniform(-0.2, 1.2, (n, 3))
+    o[:, 2] = rng.uniform(0.5, 1.5, n)             # in front of the z-facing triangles
+    tgt = centres[rng.integers(0, len(centres), n)]
+    d = (tgt - o).astype(f32)
+    o = o.astype(f32)
+    rays = {"ox": o[:, 0].copy(), "oy": o[:, 1].copy(), "oz": o[:, 2].copy(),
+            "dx": d[:, 0].copy(), "dy": d[:, 1].copy(), "dz": d[:, 2].copy()}
+    hits = sc.intersect(upload_rays(wf, ctx, rays), n)
+    ref = osc.closest_hit(rays, use_bvh=False)
+    assert (ref["prim"] >= 0).sum() > n // 2
+    assert np.array_equal(hits["prim"], ref["prim"])
+    h = ref["prim"] >= 0
+    assert np.array_equal(hits["t"][h], ref["t"][h])
+
+
+def test_large_scene_closest_hit_matches_oracle(wf, ctx, scenes, orc, rng):
+    """~330 k triangles: the oracle's SAH BVH (itself pinned against brute force) against the
+    device LBVH, over camera rays and random rays."""
+    data = scenes.sorting_stress(subdiv=6, tex_size=8)
+    sc = wf.Scene(ctx, data)
+    osc = orc.scene(data)
+    rays = camera_rays(orc, data, 128, 96)
+    extra = random_rays(rng, 20000)
+    for k in ("ox", "oy", "oz", "dx", "dy", "dz"):
+        rays[k] = np.concatenate([rays[k], extra[k]])
+    n = len(rays["ox"])
+    hits = sc.intersect(upload_rays(wf, ctx, rays), n)
+    ref = osc.closest_hit(rays, use_bvh=True)
+    assert np.array_equal(hits["geom"], ref["geom"])
+    assert np.array_equal(hits["prim"], ref["prim"])
+    h = ref["prim"] >= 0
+    assert np.array_equal(hits["t"][h], ref["t"][h])
 
 
 def test_closest_hit_matches_bruteforce(wf, ctx, cornell, orc, rng):

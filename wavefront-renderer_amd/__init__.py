@@ -74,6 +74,9 @@ class Context:
     def fill_u32(self, ptr: int, value: int, count: int):
         check(self._lib.wf_fill_u32(self.handle, C.c_void_p(ptr), value, count))
 
+    def stream_copy(self, dst: int, src: int, nbytes: int):
+        check(self._lib.wf_stream_copy(self.handle, C.c_void_p(dst), C.c_void_p(src), nbytes))
+
     def synchronize(self):
         check(self._lib.wf_synchronize(self.handle))
 

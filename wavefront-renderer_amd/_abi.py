@@ -94,7 +94,8 @@ class HitSoA(C.Structure):
 class BuildStats(C.Structure):
     _fields_ = [("mesh_count", C.c_int32), ("triangle_count", C.c_int32),
                 ("vertex_count", C.c_int32), ("node_count", C.c_int32),
-                ("max_depth", C.c_int32), ("build_ms", C.c_double),
+                ("max_depth", C.c_int32), ("stack_bound", C.c_int32),
+                ("reserved", C.c_int32), ("build_ms", C.c_double),
                 ("bounds_min", f3), ("bounds_max", f3)]
 
 
@@ -138,6 +139,7 @@ _PROTOS = {
     "wf_memcpy_dtoh": (_S, [P, P, P, C.c_size_t]),
     "wf_memcpy_dtod": (_S, [P, P, P, C.c_size_t]),
     "wf_fill_u32": (_S, [P, P, C.c_uint32, C.c_size_t]),
+    "wf_stream_copy": (_S, [P, P, P, C.c_size_t]),
     "wf_synchronize": (_S, [P]),
     "wf_scene_create": (_S, [P, C.POINTER(SceneDesc), C.POINTER(P)]),
     "wf_scene_destroy": (_S, [P]),

@@ -117,12 +117,15 @@ struct DevTexture {
     int64_t offset;            // texel offset (RGBA8: u32 units; float RGB: float4 units)
 };
 
-struct BvhNode {               // two child boxes; child index >= 0 internal node, < 0 ~leaf
-    float4 a;                  // c0.lo.xyz, c0 index
-    float4 b;                  // c0.hi.xyz, c1 index
-    float4 c;                  // c1.lo.xyz, unused
-    float4 d;                  // c1.hi.xyz, unused
+// Four-wide node (128 B): child boxes per axis in SoA form, child index >= 0 = wide node,
+// < 0 and != kEmptyChild = ~leaf triangle.
+struct Bvh4Node {
+    float4 lox, loy, loz;
+    float4 hix, hiy, hiz;
+    int4 child;
+    int4 pad;
 };
+constexpr int32_t kEmptyChild = (int32_t)0x80000000;
 
 struct SceneDev {
     int32_t n_tris = 0, n_meshes = 0, n_materials = 0, n_textures = 0, n_vertices = 0;
@@ -175,10 +178,12 @@ struct SceneDev {
     float env_intensity = 1.0f;
     float env_tint[3] = {1.0f, 1.0f, 1.0f};
     float env_choice_prob = 0.0f;
-    // BVH
-    BvhNode* nodes = nullptr;
-    int32_t n_nodes = 0;
-    int32_t max_depth = 0;
+    // BVH (two-child LBVH collapsed to four-wide nodes)
+    Bvh4Node* nodes4 = nullptr;
+    int32_t n_nodes = 0;               // two-child internal nodes of the build
+    int32_t n_nodes4 = 0;
+    int32_t max_depth = 0;             // two-child depth (deepest leaf's ancestor count)
+    int32_t stack_bound = 0;           // max entries the four-wide traversal stack can hold
     bool built = false;
     float bounds_min[3] = {0, 0, 0}, bounds_max[3] = {0, 0, 0};
 };

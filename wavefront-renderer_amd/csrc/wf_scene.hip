@@ -6,8 +6,10 @@
 // Build: padded triangle boxes + centroid bounds -> 48-bit Morton codes of the centroids ->
 // stable radix sort -> Karras hierarchy (one triangle per leaf, ties of equal codes split by
 // index) -> bottom-up box refit (second thread to arrive at a node merges its children) ->
-// packed two-child nodes (64 B) -> depth. Triangles are stored in leaf order as v0/e1/e2
-// (48 B, v0.w = the global triangle index), exactly the edges Moller-Trumbore recomputes.
+// collapse into four-wide nodes (every even-depth node adopts its grandchildren; 128 B, child
+// boxes in SoA form) -> the exact bound on the traversal stack. Triangles are stored in leaf
+// order as v0/e1/e2 (48 B, v0.w = the global triangle index), exactly the edges
+// Moller-Trumbore recomputes.
 #include <algorithm>
 #include <cfloat>
 #include <chrono>
@@ -285,34 +287,6 @@ __global__ void k_bvh_refit(int n, const int32_t* __restrict__ left,
     }
 }
 
-__global__ void k_bvh_pack(int n, const int32_t* __restrict__ left,
-                           const int32_t* __restrict__ right, const float4* __restrict__ slo,
-                           const float4* __restrict__ shi, const float4* __restrict__ nlo,
-                           const float4* __restrict__ nhi, BvhNode* __restrict__ nodes) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n - 1) return;
-    int c0 = left[i], c1 = right[i];
-    float4 l0 = c0 < 0 ? slo[~c0] : nlo[c0], h0 = c0 < 0 ? shi[~c0] : nhi[c0];
-    float4 l1 = c1 < 0 ? slo[~c1] : nlo[c1], h1 = c1 < 0 ? shi[~c1] : nhi[c1];
-    BvhNode nd;
-    nd.a = make_float4(l0.x, l0.y, l0.z, __int_as_float(c0));
-    nd.b = make_float4(h0.x, h0.y, h0.z, __int_as_float(c1));
-    nd.c = make_float4(l1.x, l1.y, l1.z, 0.0f);
-    nd.d = make_float4(h1.x, h1.y, h1.z, 0.0f);
-    nodes[i] = nd;
-}
-
-__global__ void k_bvh_single(const float4* __restrict__ slo, const float4* __restrict__ shi,
-                             BvhNode* __restrict__ nodes) {
-    float4 l = slo[0], h = shi[0];
-    BvhNode nd;
-    nd.a = make_float4(l.x, l.y, l.z, __int_as_float(~0));
-    nd.b = make_float4(h.x, h.y, h.z, __int_as_float(~0));
-    nd.c = make_float4(l.x, l.y, l.z, 0.0f);
-    nd.d = make_float4(h.x, h.y, h.z, 0.0f);
-    nodes[0] = nd;
-}
-
 // Depth of each leaf = number of internal ancestors; max over leaves.
 __global__ void k_bvh_depth(int n, const int32_t* __restrict__ iparent,
                             const int32_t* __restrict__ lparent, int32_t* max_depth) {
@@ -325,6 +299,105 @@ __global__ void k_bvh_depth(int n, const int32_t* __restrict__ iparent,
         p = iparent[p];
     }
     atomicMax(max_depth, d);
+}
+
+// Depth of every two-child internal node (root = 0).
+__global__ void k_bvh_node_depth(int n_internal, const int32_t* __restrict__ iparent,
+                                 int32_t* __restrict__ depth, uint32_t* __restrict__ even) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_internal) return;
+    int d = 0;
+    for (int p = iparent[i]; p >= 0; p = iparent[p]) ++d;
+    depth[i] = d;
+    even[i] = (d & 1) == 0 ? 1u : 0u;
+}
+
+// Collapse: every even-depth two-child node becomes a four-wide node whose children are its
+// grandchildren (or its children where those are leaves).
+__global__ void k_bvh4_collapse(int n_internal, const int32_t* __restrict__ left,
+                                const int32_t* __restrict__ right,
+                                const uint32_t* __restrict__ even,
+                                const uint32_t* __restrict__ map4,
+                                const float4* __restrict__ slo, const float4* __restrict__ shi,
+                                const float4* __restrict__ nlo, const float4* __restrict__ nhi,
+                                Bvh4Node* __restrict__ out, int32_t* __restrict__ kint) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_internal || !even[i]) return;
+    int32_t ch[4];
+    float4 lo[4], hi[4];
+    int m = 0, internal = 0;
+    const int c2[2] = {left[i], right[i]};
+    for (int a = 0; a < 2; ++a) {
+        const int c = c2[a];
+        if (c < 0) {
+            ch[m] = c;
+            lo[m] = slo[~c];
+            hi[m] = shi[~c];
+            ++m;
+            continue;
+        }
+        const int g2[2] = {left[c], right[c]};
+        for (int b = 0; b < 2; ++b) {
+            const int g = g2[b];
+            if (g < 0) {
+                ch[m] = g;
+                lo[m] = slo[~g];
+                hi[m] = shi[~g];
+            } else {
+                ch[m] = (int32_t)map4[g];
+                lo[m] = nlo[g];
+                hi[m] = nhi[g];
+                ++internal;
+            }
+            ++m;
+        }
+    }
+    for (int k = m; k < 4; ++k) {
+        ch[k] = kEmptyChild;
+        lo[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        hi[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    Bvh4Node nd;
+    nd.lox = make_float4(lo[0].x, lo[1].x, lo[2].x, lo[3].x);
+    nd.loy = make_float4(lo[0].y, lo[1].y, lo[2].y, lo[3].y);
+    nd.loz = make_float4(lo[0].z, lo[1].z, lo[2].z, lo[3].z);
+    nd.hix = make_float4(hi[0].x, hi[1].x, hi[2].x, hi[3].x);
+    nd.hiy = make_float4(hi[0].y, hi[1].y, hi[2].y, hi[3].y);
+    nd.hiz = make_float4(hi[0].z, hi[1].z, hi[2].z, hi[3].z);
+    nd.child = make_int4(ch[0], ch[1], ch[2], ch[3]);
+    nd.pad = make_int4(0, 0, 0, 0);
+    const uint32_t j = map4[i];
+    out[j] = nd;
+    kint[j] = internal;
+}
+
+// Traversal stack bound: at a four-wide node the stack holds at most the unvisited internal
+// siblings pushed at each four-wide ancestor, plus the ones pushed at the node itself.
+__global__ void k_bvh4_stack_bound(int n_internal, const int32_t* __restrict__ iparent,
+                                   const uint32_t* __restrict__ even,
+                                   const uint32_t* __restrict__ map4,
+                                   const int32_t* __restrict__ kint, int32_t* __restrict__ bound) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_internal || !even[i]) return;
+    int s = 0;
+    for (int p = i; p >= 0; p = iparent[p])
+        if (even[p]) s += kint[map4[p]] > 0 ? kint[map4[p]] - 1 : 0;
+    atomicMax(bound, s);
+}
+
+__global__ void k_bvh4_single(const float4* __restrict__ slo, const float4* __restrict__ shi,
+                              Bvh4Node* __restrict__ out) {
+    float4 l = slo[0], h = shi[0];
+    Bvh4Node nd;
+    nd.lox = make_float4(l.x, 0, 0, 0);
+    nd.loy = make_float4(l.y, 0, 0, 0);
+    nd.loz = make_float4(l.z, 0, 0, 0);
+    nd.hix = make_float4(h.x, 0, 0, 0);
+    nd.hiy = make_float4(h.y, 0, 0, 0);
+    nd.hiz = make_float4(h.z, 0, 0, 0);
+    nd.child = make_int4(~0, kEmptyChild, kEmptyChild, kEmptyChild);
+    nd.pad = make_int4(0, 0, 0, 0);
+    out[0] = nd;
 }
 
 }  // namespace wf
@@ -656,19 +729,23 @@ wf_status wf_accel_build(wf_scene* s, wf_build_stats* stats) {
             void* pt = nullptr;
             WF_TRY(scene_alloc(s, sizeof(float4) * 3 * (size_t)n, &pt));
             d.tri = (float4*)pt;
-            void* pn = nullptr;
-            WF_TRY(scene_alloc(s, sizeof(BvhNode) * (size_t)std::max(1, n - 1), &pn));
-            d.nodes = (BvhNode*)pn;
         }
         k_bvh_gather_tris<<<grid_for(n, 256), 256, 0, ctx->stream>>>(
             (int32_t*)porder, n, d.tri_v0, d.tri_e1, d.tri_e2, (float4*)plo, (float4*)phi, d.tri,
             (float4*)pslo, (float4*)pshi);
         WF_HIP(hipGetLastError());
         int32_t depth = 0;
+        if (!d.nodes4) {
+            void* pn4 = nullptr;
+            WF_TRY(scene_alloc(s, sizeof(Bvh4Node) * (size_t)std::max(1, n - 1), &pn4));
+            d.nodes4 = (Bvh4Node*)pn4;
+        }
         if (n == 1) {
-            k_bvh_single<<<1, 1, 0, ctx->stream>>>((float4*)pslo, (float4*)pshi, d.nodes);
+            k_bvh4_single<<<1, 1, 0, ctx->stream>>>((float4*)pslo, (float4*)pshi, d.nodes4);
             WF_HIP(hipGetLastError());
             d.n_nodes = 1;
+            d.n_nodes4 = 1;
+            d.stack_bound = 0;
             depth = 1;
         } else {
             void *pl, *pr, *pip, *plp, *pnlo, *pnhi, *pvis, *pdep;
@@ -689,19 +766,43 @@ wf_status wf_accel_build(wf_scene* s, wf_build_stats* stats) {
                 n, (int32_t*)pl, (int32_t*)pr, (int32_t*)pip, (int32_t*)plp, (float4*)pslo,
                 (float4*)pshi, (float4*)pnlo, (float4*)pnhi, (uint32_t*)pvis);
             WF_HIP(hipGetLastError());
-            k_bvh_pack<<<grid_for(n - 1, 256), 256, 0, ctx->stream>>>(
-                n, (int32_t*)pl, (int32_t*)pr, (float4*)pslo, (float4*)pshi, (float4*)pnlo,
-                (float4*)pnhi, d.nodes);
-            WF_HIP(hipGetLastError());
             k_bvh_depth<<<grid_for(n, 256), 256, 0, ctx->stream>>>(n, (int32_t*)pip,
                                                                    (int32_t*)plp, (int32_t*)pdep);
             WF_HIP(hipGetLastError());
             WF_TRY(readback(ctx, &depth, pdep, sizeof(int32_t)));
             d.n_nodes = n - 1;
+            // collapse to four-wide nodes
+            const int ni = n - 1;
+            void *pnd, *pev, *pmap, *pkint, *pcnt;
+            WF_TRY(workspace(ctx, "bvh_node_depth", sizeof(int32_t) * ni, &pnd));
+            WF_TRY(workspace(ctx, "bvh_even", sizeof(uint32_t) * ni, &pev));
+            WF_TRY(workspace(ctx, "bvh_map4", sizeof(uint32_t) * ni, &pmap));
+            WF_TRY(workspace(ctx, "bvh_kint", sizeof(int32_t) * ni, &pkint));
+            WF_TRY(workspace(ctx, "bvh4_counts", sizeof(int32_t) * 4, &pcnt));
+            k_bvh_node_depth<<<grid_for(ni, 256), 256, 0, ctx->stream>>>(
+                ni, (int32_t*)pip, (int32_t*)pnd, (uint32_t*)pev);
+            WF_HIP(hipGetLastError());
+            WF_TRY(exclusive_scan_u32(ctx, (uint32_t*)pev, (uint32_t*)pmap, ni, (int32_t*)pcnt,
+                                      "bvh_scan"));
+            k_bvh4_collapse<<<grid_for(ni, 256), 256, 0, ctx->stream>>>(
+                ni, (int32_t*)pl, (int32_t*)pr, (uint32_t*)pev, (uint32_t*)pmap, (float4*)pslo,
+                (float4*)pshi, (float4*)pnlo, (float4*)pnhi, d.nodes4, (int32_t*)pkint);
+            WF_HIP(hipGetLastError());
+            WF_TRY(wf_fill_u32(ctx, (int32_t*)pcnt + 1, 0u, 1));
+            k_bvh4_stack_bound<<<grid_for(ni, 256), 256, 0, ctx->stream>>>(
+                ni, (int32_t*)pip, (uint32_t*)pev, (uint32_t*)pmap, (int32_t*)pkint,
+                (int32_t*)pcnt + 1);
+            WF_HIP(hipGetLastError());
+            int32_t c[2] = {0, 0};
+            WF_TRY(readback(ctx, c, pcnt, sizeof(c)));
+            d.n_nodes4 = c[0];
+            d.stack_bound = c[1];
         }
         d.max_depth = depth;
     } else {
         d.n_nodes = 0;
+        d.n_nodes4 = 0;
+        d.stack_bound = 0;
         d.max_depth = 0;
     }
     WF_HIP(hipEventRecord(e1, ctx->stream));
@@ -715,8 +816,10 @@ wf_status wf_accel_build(wf_scene* s, wf_build_stats* stats) {
         stats->mesh_count = d.n_meshes;
         stats->triangle_count = d.n_tris;
         stats->vertex_count = d.n_vertices;
-        stats->node_count = d.n_nodes;
+        stats->node_count = d.n_nodes4;
         stats->max_depth = d.max_depth;
+        stats->stack_bound = d.stack_bound;
+        stats->reserved = 0;
         stats->build_ms = ms;
         for (int k = 0; k < 3; ++k) {
             stats->bounds_min[k] = d.bounds_min[k];

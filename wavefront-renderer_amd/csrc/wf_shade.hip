@@ -600,10 +600,11 @@ __global__ void k_miss(const SceneDev s, wf_ray_soa q, int n, int physics, float
 
 // EmbreeAccelerator::occluded for the NEE rays of each listed entry, then the contributions in
 // the reference order: area/environment light first, directional light second (cpp:678, 720).
+template <int STACK>
 __global__ __launch_bounds__(kTraceBlock) void k_shadow(const SceneDev s, wf_shade_soa q, ShadeOut o,
                                                         const int32_t* __restrict__ list, int count,
                                                         float* __restrict__ accum) {
-    __shared__ int32_t stack[kStack * kTraceBlock];
+    __shared__ int32_t stack[STACK * kTraceBlock];
     const int k = blockIdx.x * kTraceBlock + threadIdx.x;
     if (k >= count) return;
     const int i = list[k];
@@ -614,7 +615,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(const SceneDev s, wf_sha
         const float tfar = o.lmax[i] - 0.001f;
         TraceHit h;
         const bool occ = tfar < 0.0f ||
-                         traverse<true>(s.nodes, s.tri, s.n_tris, P, ld3(o.ldx, o.ldy, o.ldz, i),
+                         traverse<true, STACK>(s.nodes4, s.tri, s.n_tris, P, ld3(o.ldx, o.ldy, o.ldz, i),
                                         0.001f, tfar, stack, h);
         if (!occ) {
             acc[0] = acc[0] + o.lr[i];
@@ -626,7 +627,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(const SceneDev s, wf_sha
         const V3 tl = V3(s.directional.direction[0], s.directional.direction[1],
                          s.directional.direction[2]) * -1.0f;
         TraceHit h;
-        const bool occ = traverse<true>(s.nodes, s.tri, s.n_tris, P, tl, 0.001f, 1e6f - 0.001f, stack, h);
+        const bool occ = traverse<true, STACK>(s.nodes4, s.tri, s.n_tris, P, tl, 0.001f, 1e6f - 0.001f, stack, h);
         if (!occ) {
             acc[0] = acc[0] + o.dr[i];
             acc[1] = acc[1] + o.dg[i];
@@ -704,8 +705,11 @@ wf_status shade_bounce(const wf_scene* s, const wf_shade_soa* q, int32_t n,
             o.shadow_flag, (int32_t*)plist, n_shadow, counts + 1);
         WF_HIP(hipGetLastError());
         Timed tm(ctx, "shadow", 41.0 * n_shadow);
-        k_shadow<<<grid_for(n_shadow, kTraceBlock), kTraceBlock, 0, ctx->stream>>>(
-            s->dev, *q, o, (int32_t*)plist, n_shadow, accum);
+        with_stack(s->dev.stack_bound, [&](auto S) {
+            k_shadow<decltype(S)::value>
+                <<<grid_for(n_shadow, kTraceBlock), kTraceBlock, 0, ctx->stream>>>(
+                    s->dev, *q, o, (int32_t*)plist, n_shadow, accum);
+        });
         WF_HIP(hipGetLastError());
     }
     // next rays: stable compaction in shading order

@@ -9,6 +9,8 @@
 //      are matched with 8 ballots, the rank is the popcount of matching lower lanes plus the
 //      wave's running count for that digit; waves are then offset by an exclusive prefix over
 //      waves, so equal digits keep input order (stability).
+#include <algorithm>
+
 #include "wf_device.h"
 #include "wf_internal.h"
 
@@ -192,6 +194,32 @@ __global__ void k_fill_u32(uint32_t* dst, uint32_t value, size_t n) {
 
 __global__ void k_set_i32(int32_t* dst, int32_t value) { *dst = value; }
 
+__global__ __launch_bounds__(256) void k_stream_copy(const float4* __restrict__ src,
+                                                     float4* __restrict__ dst, size_t n) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        dst[i] = src[i];
+}
+
+// AND and OR of all keys: bits where they differ are the only ones a sort needs to look at.
+__global__ void k_key_and_or(const uint64_t* __restrict__ keys, int n,
+                             unsigned long long* __restrict__ out) {
+    unsigned long long a = ~0ull, o = 0ull;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const unsigned long long k = keys[i];
+        a &= k;
+        o |= k;
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        a &= __shfl_xor(a, off, 64);
+        o |= __shfl_xor(o, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAnd(&out[0], a);
+        atomicOr(&out[1], o);
+    }
+}
+
 static wf_status scan_level(wf_context* ctx, const uint32_t* in, uint32_t* out, int n,
                             int32_t* total, int level, const char* tag) {
     const int tiles = (n + kScanTile - 1) / kScanTile;
@@ -254,8 +282,38 @@ wf_status radix_sort_pairs(wf_context* ctx, const uint64_t* keys_in, const int32
                               ctx->stream));
         return WF_OK;
     }
+    // Only the bit range where keys differ can change the order: find it and sort just that
+    // range (stable, so the result is the same permutation as sorting all key_bits).
+    int lo_bit = 0, hi_bit = key_bits - 1;
+    {
+        void* pr = nullptr;
+        WF_TRY(workspace(ctx, "sort_and_or", 16, &pr));
+        unsigned long long init[2] = {~0ull, 0ull};
+        WF_HIP(hipMemcpyAsync(pr, init, sizeof(init), hipMemcpyHostToDevice, ctx->stream));
+        Timed t(ctx, tag, 8.0 * n);
+        const int blocks = (int)std::min<int64_t>(grid_for(n, 256), 1024);
+        k_key_and_or<<<blocks, 256, 0, ctx->stream>>>(keys_in, n, (unsigned long long*)pr);
+        WF_HIP(hipGetLastError());
+        unsigned long long r[2];
+        WF_TRY(readback(ctx, r, pr, sizeof(r)));
+        unsigned long long diff = (r[0] ^ r[1]);
+        if (key_bits < 64) diff &= (1ull << key_bits) - 1ull;
+        if (diff == 0) {
+            if (vals_in)
+                WF_HIP(hipMemcpyAsync(vals_out, vals_in, sizeof(int32_t) * n,
+                                      hipMemcpyDeviceToDevice, ctx->stream));
+            else
+                WF_TRY(iota_i32(ctx, vals_out, n));
+            WF_HIP(hipMemcpyAsync(keys_out, keys_in, sizeof(uint64_t) * n,
+                                  hipMemcpyDeviceToDevice, ctx->stream));
+            return WF_OK;
+        }
+        lo_bit = __builtin_ctzll(diff);
+        hi_bit = 63 - __builtin_clzll(diff);
+    }
+    const int span = hi_bit - lo_bit + 1;
     const int tiles = (n + kSortTile - 1) / kSortTile;
-    const int passes = (key_bits + 7) / 8;
+    const int passes = (span + 7) / 8;
     void *pk = nullptr, *pv = nullptr, *ph = nullptr;
     WF_TRY(workspace(ctx, "sort_keys_alt", sizeof(uint64_t) * (size_t)n, &pk));
     WF_TRY(workspace(ctx, "sort_vals_alt", sizeof(int32_t) * (size_t)n, &pv));
@@ -265,8 +323,8 @@ wf_status radix_sort_pairs(wf_context* ctx, const uint64_t* keys_in, const int32
     const uint64_t* ksrc = keys_in;
     const int32_t* vsrc = vals_in;
     for (int p = 0; p < passes; ++p) {
-        const int shift = p * 8;
-        const int nbits = (key_bits - shift) < 8 ? (key_bits - shift) : 8;
+        const int shift = lo_bit + p * 8;
+        const int nbits = (hi_bit + 1 - shift) < 8 ? (hi_bit + 1 - shift) : 8;
         const uint32_t mask = (1u << nbits) - 1u;
         const bool to_out = ((passes - 1 - p) % 2) == 0;
         uint64_t* kdst = to_out ? keys_out : (uint64_t*)pk;
@@ -290,6 +348,20 @@ wf_status radix_sort_pairs(wf_context* ctx, const uint64_t* keys_in, const int32
 }
 
 }  // namespace wf
+
+extern "C" wf_status wf_stream_copy(wf_context* ctx, void* dst, const void* src, size_t bytes) {
+    WF_REQUIRE(ctx, "wf_stream_copy: NULL context");
+    WF_REQUIRE(bytes % 16 == 0, "wf_stream_copy: size %zu is not a multiple of 16", bytes);
+    WF_REQUIRE(bytes == 0 || (dst && src), "wf_stream_copy: NULL buffer");
+    if (bytes == 0) return WF_OK;
+    const size_t n = bytes / 16;
+    // 8 resident 256-lane blocks per CU over 256 CUs, or fewer for small copies
+    const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 2048);
+    wf::Timed tm(ctx, "stream_copy", 2.0 * (double)bytes);
+    wf::k_stream_copy<<<grid, 256, 0, ctx->stream>>>((const float4*)src, (float4*)dst, n);
+    WF_HIP(hipGetLastError());
+    return WF_OK;
+}
 
 extern "C" wf_status wf_fill_u32(wf_context* ctx, void* dst, uint32_t value, size_t count) {
     WF_REQUIRE(ctx, "wf_fill_u32: NULL context");

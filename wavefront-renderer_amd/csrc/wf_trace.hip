@@ -2,8 +2,9 @@
 // (src/accelerator/Embree.cpp:122-279) and the hit/miss split of WavefrontRenderer::intersectAll
 // (src/render/WavefrontRenderer.cpp:459-567).
 //
-// One lane per ray. The traversal stack lives in LDS, one 64-entry column per lane
-// (stack[depth][lane] so that a wave's pushes and pops hit 64 different banks). The triangle test
+// One lane per ray over the four-wide BVH. The traversal stack lives in LDS, one column per lane
+// (stack[entry][lane] so that a wave's pushes and pops hit 64 different banks), sized from the
+// build's exact stack bound. The triangle test
 // is the reference's Moller-Trumbore (include/geometry/Triangle.h:6-42) with the same float
 // operations; closest hit = lexicographic minimum of (t, global triangle index) over every
 // triangle the test accepts in [tnear, tfar], which makes the result independent of traversal
@@ -12,17 +13,18 @@
 
 namespace wf {
 
+template <int STACK>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(
-    const BvhNode* __restrict__ nodes, const float4* __restrict__ tri, int n_tris,
+    const Bvh4Node* __restrict__ nodes, const float4* __restrict__ tri, int n_tris,
     const float* __restrict__ ox, const float* __restrict__ oy, const float* __restrict__ oz,
     const float* __restrict__ dx, const float* __restrict__ dy, const float* __restrict__ dz,
     int n, float tnear, float tfar, int32_t* __restrict__ out_leaf, float* __restrict__ out_t,
     float* __restrict__ out_u, float* __restrict__ out_v, uint32_t* __restrict__ out_flag) {
-    __shared__ int32_t stack[kStack * kTraceBlock];
+    __shared__ int32_t stack[STACK * kTraceBlock];
     const int i = blockIdx.x * kTraceBlock + threadIdx.x;
     if (i >= n) return;
     TraceHit h;
-    traverse<false>(nodes, tri, n_tris, V3(ox[i], oy[i], oz[i]), V3(dx[i], dy[i], dz[i]), tnear,
+    traverse<false, STACK>(nodes, tri, n_tris, V3(ox[i], oy[i], oz[i]), V3(dx[i], dy[i], dz[i]), tnear,
                     tfar, stack, h);
     out_leaf[i] = h.leaf;
     out_t[i] = h.t;
@@ -33,19 +35,20 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(
 
 // occluded(): tnear 0.001, tfar = maxDistance - 0.001; a negative tfar reports occluded
 // (Embree.cpp:268-278).
+template <int STACK>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_occluded(
-    const BvhNode* __restrict__ nodes, const float4* __restrict__ tri, int n_tris,
+    const Bvh4Node* __restrict__ nodes, const float4* __restrict__ tri, int n_tris,
     const float* __restrict__ ox, const float* __restrict__ oy, const float* __restrict__ oz,
     const float* __restrict__ dx, const float* __restrict__ dy, const float* __restrict__ dz,
     const float* __restrict__ maxd, int n, uint8_t* __restrict__ out) {
-    __shared__ int32_t stack[kStack * kTraceBlock];
+    __shared__ int32_t stack[STACK * kTraceBlock];
     const int i = blockIdx.x * kTraceBlock + threadIdx.x;
     if (i >= n) return;
     const float tfar = maxd[i] - 0.001f;
     bool occ = tfar < 0.0f;
     if (!occ) {
         TraceHit h;
-        occ = traverse<true>(nodes, tri, n_tris, V3(ox[i], oy[i], oz[i]),
+        occ = traverse<true, STACK>(nodes, tri, n_tris, V3(ox[i], oy[i], oz[i]),
                              V3(dx[i], dy[i], dz[i]), 0.001f, tfar, stack, h);
     }
     out[i] = occ ? 1 : 0;
@@ -156,9 +159,10 @@ __global__ void k_hits_to_soa(const SceneDev s, int n, const int32_t* __restrict
 wf_status scene_ready(const wf_scene* s) {
     if (!s) return set_error(WF_ERR_INVALID_ARGUMENT, "NULL scene");
     if (!s->dev.built) return set_error(WF_ERR_NOT_BUILT, "accelerator not built (call wf_accel_build)");
-    if (s->dev.max_depth > kStack)
-        return set_error(WF_ERR_UNSUPPORTED, "BVH depth %d exceeds the traversal stack (%d)",
-                         s->dev.max_depth, kStack);
+    if (s->dev.stack_bound > kStackMax)
+        return set_error(WF_ERR_UNSUPPORTED,
+                         "BVH needs a traversal stack of %d entries (at most %d supported)",
+                         s->dev.stack_bound, kStackMax);
     return WF_OK;
 }
 
@@ -168,10 +172,13 @@ wf_status trace_closest(const wf_scene* s, const wf_ray_soa* r, int32_t n, float
     WF_TRY(scene_ready(s));
     if (n <= 0) return WF_OK;
     wf_context* ctx = s->ctx;
-    Timed tm(ctx, "traverse_closest", 40.0 * n);
-    k_trace_closest<<<grid_for(n, kTraceBlock), kTraceBlock, 0, ctx->stream>>>(
-        s->dev.nodes, s->dev.tri, s->dev.n_tris, r->ox, r->oy, r->oz, r->dx, r->dy, r->dz, n,
-        tnear, tfar, hit_leaf, hit_t, hit_bu, hit_bv, flags);
+    Timed tm(ctx, "traverse_closest", 44.0 * n);
+    with_stack(s->dev.stack_bound, [&](auto S) {
+        k_trace_closest<decltype(S)::value>
+            <<<grid_for(n, kTraceBlock), kTraceBlock, 0, ctx->stream>>>(
+                s->dev.nodes4, s->dev.tri, s->dev.n_tris, r->ox, r->oy, r->oz, r->dx, r->dy,
+                r->dz, n, tnear, tfar, hit_leaf, hit_t, hit_bu, hit_bv, flags);
+    });
     WF_HIP(hipGetLastError());
     return WF_OK;
 }
@@ -183,8 +190,11 @@ wf_status trace_occluded(const wf_scene* s, const float* ox, const float* oy, co
     if (n <= 0) return WF_OK;
     wf_context* ctx = s->ctx;
     Timed tm(ctx, "traverse_any", 29.0 * n);
-    k_trace_occluded<<<grid_for(n, kTraceBlock), kTraceBlock, 0, ctx->stream>>>(
-        s->dev.nodes, s->dev.tri, s->dev.n_tris, ox, oy, oz, dx, dy, dz, maxd, n, out);
+    with_stack(s->dev.stack_bound, [&](auto S) {
+        k_trace_occluded<decltype(S)::value>
+            <<<grid_for(n, kTraceBlock), kTraceBlock, 0, ctx->stream>>>(
+                s->dev.nodes4, s->dev.tri, s->dev.n_tris, ox, oy, oz, dx, dy, dz, maxd, n, out);
+    });
     WF_HIP(hipGetLastError());
     return WF_OK;
 }

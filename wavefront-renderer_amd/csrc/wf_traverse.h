@@ -1,8 +1,10 @@
 // Ray traversal shared by the closest-hit, occlusion and shadow kernels: the reference's
 // Moller-Trumbore test (include/geometry/Triangle.h:6-42) with the same float operations, a
-// conservative slab test, and a stack walk over the two-child BVH with the stack in LDS
-// (stack[depth * kTraceBlock + lane]).
+// conservative slab test, and a stack walk over the four-wide BVH with the stack in LDS
+// (stack[entry * kTraceBlock + lane], so a wave's pushes and pops hit 64 different banks).
 #pragma once
+
+#include <type_traits>
 
 #include "wf_device.h"
 #include "wf_internal.h"
@@ -10,7 +12,19 @@
 namespace wf {
 
 constexpr int kTraceBlock = 128;
-constexpr int kStack = 64;
+// Largest traversal stack (entries per lane) a kernel is instantiated for; the scene build
+// measures the exact bound and the launch picks the smallest instantiation >= it.
+constexpr int kStackMax = 64;
+
+// Calls f(std::integral_constant<int, S>) for the smallest stack size S in {16, 32, 48, 64} that
+// holds `bound` entries (LDS per 128-lane block: 8, 16, 24, 32 KB).
+template <class F>
+inline void with_stack(int bound, F&& f) {
+    if (bound <= 16) f(std::integral_constant<int, 16>{});
+    else if (bound <= 32) f(std::integral_constant<int, 32>{});
+    else if (bound <= 48) f(std::integral_constant<int, 48>{});
+    else f(std::integral_constant<int, 64>{});
+}
 
 __device__ __forceinline__ bool mt_test(const V3& o, const V3& d, const V3& v0, const V3& e1,
                                         const V3& e2, float tmin, float tmax, float& t, float& u,
@@ -39,11 +53,13 @@ __device__ __forceinline__ float safe_rcp(float x) {
     return 1.0f / a;
 }
 
-__device__ __forceinline__ bool slab(const float4& lo, const float4& hi, const V3& o,
-                                     const V3& inv, float tmax, float& tenter) {
-    float tx0 = (lo.x - o.x) * inv.x, tx1 = (hi.x - o.x) * inv.x;
-    float ty0 = (lo.y - o.y) * inv.y, ty1 = (hi.y - o.y) * inv.y;
-    float tz0 = (lo.z - o.z) * inv.z, tz1 = (hi.z - o.z) * inv.z;
+// Entry distance of the box [lo, hi] along the ray, clipped to [0, tmax]; true when the
+// interval is non-empty (with a relative slack of 2 ulp so that rounding never drops a box).
+__device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, float hy, float hz,
+                                     const V3& o, const V3& inv, float tmax, float& tenter) {
+    float tx0 = (lx - o.x) * inv.x, tx1 = (hx - o.x) * inv.x;
+    float ty0 = (ly - o.y) * inv.y, ty1 = (hy - o.y) * inv.y;
+    float tz0 = (lz - o.z) * inv.z, tz1 = (hz - o.z) * inv.z;
     float t0 = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
     float t1 = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
     tenter = t0;
@@ -57,8 +73,12 @@ struct TraceHit {
     float u, v;
 };
 
-template <bool ANY>
-__device__ __forceinline__ bool traverse(const BvhNode* __restrict__ nodes,
+// Walks the four-wide BVH. At each node the four child boxes are tested; leaf children are
+// tested at once, the hit inner children are visited nearest first (the others are pushed far to
+// near). ANY stops at the first accepted triangle. The stack never holds more than the build's
+// stack bound, which the caller's STACK covers.
+template <bool ANY, int STACK>
+__device__ __forceinline__ bool traverse(const Bvh4Node* __restrict__ nodes,
                                          const float4* __restrict__ tri, int n_tris,
                                          const V3& o, const V3& d, float tnear, float tfar,
                                          int32_t* __restrict__ stack, TraceHit& h) {
@@ -69,55 +89,84 @@ __device__ __forceinline__ bool traverse(const BvhNode* __restrict__ nodes,
     if (n_tris <= 0) return false;
     const V3 inv(safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z));
     const int lane = threadIdx.x;
+    const float kInf = __int_as_float(0x7F800000);
     int sp = 0;
     int node = 0;
-    auto test = [&](int leaf) -> bool {
-        const float4 a = tri[3 * (size_t)leaf];
-        const float4 b = tri[3 * (size_t)leaf + 1];
-        const float4 c = tri[3 * (size_t)leaf + 2];
-        float t, u, v;
-        if (!mt_test(o, d, xyz(a), xyz(b), xyz(c), tnear, tfar, t, u, v)) return false;
-        const int32_t g = __float_as_int(a.w);
-        if (ANY) {
-            h.t = t;
-            h.leaf = leaf;
-            h.g = g;
-            return true;
-        }
-        if (h.leaf < 0 || t < h.t || (t == h.t && g < h.g)) {
-            h.t = t;
-            h.leaf = leaf;
-            h.g = g;
-            h.u = u;
-            h.v = v;
-        }
-        return false;
-    };
     for (;;) {
-        const BvhNode nd = nodes[node];
-        const int c0 = __float_as_int(nd.a.w);
-        const int c1 = __float_as_int(nd.b.w);
-        float t0, t1;
-        const float tcur = ANY ? tfar : h.t;
-        bool h0 = slab(nd.a, nd.b, o, inv, tcur, t0);
-        bool h1 = slab(nd.c, nd.d, o, inv, tcur, t1);
-        if (h0 && c0 < 0) {
-            if (test(~c0)) return true;
-            h0 = false;
+        const Bvh4Node* nd = nodes + node;
+        const float4 lox = nd->lox, loy = nd->loy, loz = nd->loz;
+        const float4 hix = nd->hix, hiy = nd->hiy, hiz = nd->hiz;
+        const int4 ch = nd->child;
+        const float lx[4] = {lox.x, lox.y, lox.z, lox.w};
+        const float ly[4] = {loy.x, loy.y, loy.z, loy.w};
+        const float lz[4] = {loz.x, loz.y, loz.z, loz.w};
+        const float hx[4] = {hix.x, hix.y, hix.z, hix.w};
+        const float hy[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
+        const float hz[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
+        int c[4] = {ch.x, ch.y, ch.z, ch.w};
+        float dist[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            dist[k] = kInf;
+            if (c[k] == kEmptyChild) continue;
+            float t0;
+            if (!slab(lx[k], ly[k], lz[k], hx[k], hy[k], hz[k], o, inv, ANY ? tfar : h.t, t0))
+                continue;
+            if (c[k] >= 0) {
+                dist[k] = t0;
+                continue;
+            }
+            const int leaf = ~c[k];
+            const float4 a = tri[3 * (size_t)leaf];
+            const float4 b = tri[3 * (size_t)leaf + 1];
+            const float4 e = tri[3 * (size_t)leaf + 2];
+            float t, u, v;
+            if (!mt_test(o, d, xyz(a), xyz(b), xyz(e), tnear, tfar, t, u, v)) continue;
+            const int32_t g = __float_as_int(a.w);
+            if (ANY) {
+                h.t = t;
+                h.leaf = leaf;
+                h.g = g;
+                return true;
+            }
+            if (h.leaf < 0 || t < h.t || (t == h.t && g < h.g)) {
+                h.t = t;
+                h.leaf = leaf;
+                h.g = g;
+                h.u = u;
+                h.v = v;
+            }
         }
-        if (h1 && c1 < 0) {
-            if (test(~c1)) return true;
-            h1 = false;
+        // order the inner children nearest first (misses and leaves sort last as +inf)
+        auto cswap = [&](int x, int y) {
+            if (dist[y] < dist[x]) {
+                const float td = dist[x];
+                dist[x] = dist[y];
+                dist[y] = td;
+                const int tc = c[x];
+                c[x] = c[y];
+                c[y] = tc;
+            }
+        };
+        if (!ANY) {
+            cswap(0, 1);
+            cswap(2, 3);
+            cswap(0, 2);
+            cswap(1, 3);
+            cswap(1, 2);
+        } else {
+            cswap(0, 1);
+            cswap(0, 2);
+            cswap(0, 3);
         }
-        if (h0 && h1) {
-            const bool first0 = t0 <= t1;
-            stack[sp * kTraceBlock + lane] = first0 ? c1 : c0;
-            ++sp;
-            node = first0 ? c0 : c1;
-        } else if (h0) {
-            node = c0;
-        } else if (h1) {
-            node = c1;
+#pragma unroll
+        for (int k = 3; k >= 1; --k)
+            if (dist[k] < kInf) {
+                stack[sp * kTraceBlock + lane] = c[k];
+                ++sp;
+            }
+        if (dist[0] < kInf) {
+            node = c[0];
         } else {
             if (sp == 0) break;
             --sp;
