@@ -340,12 +340,15 @@ wf_status wf_render(wf_scene* scene, const wf_render_params* params, float* imag
 
 /* Enables/disables event timing around every library kernel launch and clears the totals. */
 wf_status wf_profile_enable(wf_context* ctx, int32_t enable);
-/* Totals for a kernel name ("ray_keys", "radix_pass", "traverse_closest", "split", ...):
+/* Totals for a kernel name ("ray_keys", "ray_sort", "traverse_closest", "split", ...):
  * summed device milliseconds, launch count and algorithmic HBM bytes accounted to it. */
 wf_status wf_profile_query(wf_context* ctx, const char* kernel, double* total_ms,
                            int64_t* launches, double* algorithmic_bytes);
 /* Newline-separated list of profiled kernel names into buf (truncated to cap). */
 wf_status wf_profile_names(wf_context* ctx, char* buf, int32_t cap);
+/* Restricts timing to one kernel name (NULL or "" = every kernel), so that a timed region can
+ * carry events around its dominant kernel only. Clears the totals. */
+wf_status wf_profile_filter(wf_context* ctx, const char* kernel);
 
 #ifdef __cplusplus
 }

@@ -83,6 +83,9 @@ class Context:
     def profile(self, enable: bool):
         check(self._lib.wf_profile_enable(self.handle, 1 if enable else 0))
 
+    def profile_filter(self, name: Optional[str]):
+        check(self._lib.wf_profile_filter(self.handle, name.encode() if name else None))
+
     def profile_names(self):
         buf = C.create_string_buffer(1 << 16)
         check(self._lib.wf_profile_names(self.handle, buf, len(buf)))

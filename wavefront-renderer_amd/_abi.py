@@ -166,6 +166,7 @@ _PROTOS = {
     "wf_profile_query": (_S, [P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                               C.POINTER(C.c_double)]),
     "wf_profile_names": (_S, [P, C.c_char_p, C.c_int32]),
+    "wf_profile_filter": (_S, [P, C.c_char_p]),
 }
 
 _lib = None

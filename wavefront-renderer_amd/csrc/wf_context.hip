@@ -77,6 +77,7 @@ static hipEvent_t take_event(wf_context* ctx) {
 Timed::Timed(wf_context* c, const char* n, double algorithmic_bytes)
     : ctx(c), name(n), bytes(algorithmic_bytes) {
     if (!ctx || !ctx->profile) return;
+    if (!ctx->profile_only.empty() && ctx->profile_only != n) return;
     a = take_event(ctx);
     b = take_event(ctx);
     if (a && b) (void)hipEventRecord(a, ctx->stream);
@@ -231,6 +232,14 @@ wf_status wf_profile_query(wf_context* ctx, const char* kernel, double* total_ms
     if (total_ms) *total_ms = e.ms;
     if (launches) *launches = e.launches;
     if (algorithmic_bytes) *algorithmic_bytes = e.bytes;
+    return WF_OK;
+}
+
+wf_status wf_profile_filter(wf_context* ctx, const char* kernel) {
+    WF_REQUIRE(ctx, "wf_profile_filter: NULL context");
+    WF_TRY(harvest_timings(ctx));
+    ctx->prof.clear();
+    ctx->profile_only = kernel ? kernel : "";
     return WF_OK;
 }
 

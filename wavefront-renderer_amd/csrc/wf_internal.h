@@ -67,6 +67,7 @@ struct wf_context {
     size_t pinned_bytes = 0;
     // timing
     bool profile = false;
+    std::string profile_only;          // non-empty: time only this kernel name
     std::map<std::string, wf::ProfEntry> prof;
     std::vector<wf::PendingTiming> pending;
     std::vector<hipEvent_t> event_pool;

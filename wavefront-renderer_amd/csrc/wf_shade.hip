@@ -604,7 +604,7 @@ template <int STACK>
 __global__ __launch_bounds__(kTraceBlock) void k_shadow(const SceneDev s, wf_shade_soa q, ShadeOut o,
                                                         const int32_t* __restrict__ list, int count,
                                                         float* __restrict__ accum) {
-    __shared__ int32_t stack[STACK * kTraceBlock];
+    __shared__ int32_t stack[lds_entries<STACK>() * kTraceBlock];
     const int k = blockIdx.x * kTraceBlock + threadIdx.x;
     if (k >= count) return;
     const int i = list[k];

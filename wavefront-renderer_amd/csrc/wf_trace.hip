@@ -2,9 +2,9 @@
 // (src/accelerator/Embree.cpp:122-279) and the hit/miss split of WavefrontRenderer::intersectAll
 // (src/render/WavefrontRenderer.cpp:459-567).
 //
-// One lane per ray over the four-wide BVH. The traversal stack lives in LDS, one column per lane
-// (stack[entry][lane] so that a wave's pushes and pops hit 64 different banks), sized from the
-// build's exact stack bound. The triangle test
+// One lane per ray over the four-wide BVH. The top of the traversal stack lives in LDS, one
+// column per lane (stack[entry][lane] so that a wave's pushes and pops hit 64 different banks),
+// deeper entries in a per-lane private array sized from the build's exact stack bound. The triangle test
 // is the reference's Moller-Trumbore (include/geometry/Triangle.h:6-42) with the same float
 // operations; closest hit = lexicographic minimum of (t, global triangle index) over every
 // triangle the test accepts in [tnear, tfar], which makes the result independent of traversal
@@ -20,7 +20,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(
     const float* __restrict__ dx, const float* __restrict__ dy, const float* __restrict__ dz,
     int n, float tnear, float tfar, int32_t* __restrict__ out_leaf, float* __restrict__ out_t,
     float* __restrict__ out_u, float* __restrict__ out_v, uint32_t* __restrict__ out_flag) {
-    __shared__ int32_t stack[STACK * kTraceBlock];
+    __shared__ int32_t stack[lds_entries<STACK>() * kTraceBlock];
     const int i = blockIdx.x * kTraceBlock + threadIdx.x;
     if (i >= n) return;
     TraceHit h;
@@ -41,7 +41,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_occluded(
     const float* __restrict__ ox, const float* __restrict__ oy, const float* __restrict__ oz,
     const float* __restrict__ dx, const float* __restrict__ dy, const float* __restrict__ dz,
     const float* __restrict__ maxd, int n, uint8_t* __restrict__ out) {
-    __shared__ int32_t stack[STACK * kTraceBlock];
+    __shared__ int32_t stack[lds_entries<STACK>() * kTraceBlock];
     const int i = blockIdx.x * kTraceBlock + threadIdx.x;
     if (i >= n) return;
     const float tfar = maxd[i] - 0.001f;

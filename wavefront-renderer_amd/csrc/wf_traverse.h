@@ -1,7 +1,8 @@
 // Ray traversal shared by the closest-hit, occlusion and shadow kernels: the reference's
 // Moller-Trumbore test (include/geometry/Triangle.h:6-42) with the same float operations, a
-// conservative slab test, and a stack walk over the four-wide BVH with the stack in LDS
-// (stack[entry * kTraceBlock + lane], so a wave's pushes and pops hit 64 different banks).
+// conservative slab test, and a stack walk over the four-wide BVH. The first kStackLds stack
+// entries live in LDS (stack[entry * kTraceBlock + lane], so a wave's pushes and pops hit 64
+// different banks); deeper entries, which few rays reach, spill to a per-lane private array.
 #pragma once
 
 #include <type_traits>
@@ -15,9 +16,13 @@ constexpr int kTraceBlock = 128;
 // Largest traversal stack (entries per lane) a kernel is instantiated for; the scene build
 // measures the exact bound and the launch picks the smallest instantiation >= it.
 constexpr int kStackMax = 64;
+// Stack entries per lane kept in LDS (8 KB per 128-lane block, so LDS never limits occupancy).
+constexpr int kStackLds = 16;
+template <int STACK>
+constexpr int lds_entries() { return STACK < kStackLds ? STACK : kStackLds; }
 
 // Calls f(std::integral_constant<int, S>) for the smallest stack size S in {16, 32, 48, 64} that
-// holds `bound` entries (LDS per 128-lane block: 8, 16, 24, 32 KB).
+// holds `bound` entries.
 template <class F>
 inline void with_stack(int bound, F&& f) {
     if (bound <= 16) f(std::integral_constant<int, 16>{});
@@ -76,7 +81,8 @@ struct TraceHit {
 // Walks the four-wide BVH. At each node the four child boxes are tested; leaf children are
 // tested at once, the hit inner children are visited nearest first (the others are pushed far to
 // near). ANY stops at the first accepted triangle. The stack never holds more than the build's
-// stack bound, which the caller's STACK covers.
+// stack bound, which the caller's STACK covers; `stack` is the block's LDS column array of
+// lds_entries<STACK>() rows.
 template <bool ANY, int STACK>
 __device__ __forceinline__ bool traverse(const Bvh4Node* __restrict__ nodes,
                                          const float4* __restrict__ tri, int n_tris,
@@ -90,6 +96,9 @@ __device__ __forceinline__ bool traverse(const Bvh4Node* __restrict__ nodes,
     const V3 inv(safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z));
     const int lane = threadIdx.x;
     const float kInf = __int_as_float(0x7F800000);
+    constexpr int L = lds_entries<STACK>();
+    constexpr int X = STACK - L;
+    volatile int32_t spill[X > 0 ? X : 1];   // private (scratch) memory, rarely touched
     int sp = 0;
     int node = 0;
     for (;;) {
@@ -162,7 +171,8 @@ __device__ __forceinline__ bool traverse(const Bvh4Node* __restrict__ nodes,
 #pragma unroll
         for (int k = 3; k >= 1; --k)
             if (dist[k] < kInf) {
-                stack[sp * kTraceBlock + lane] = c[k];
+                if (X == 0 || sp < L) stack[sp * kTraceBlock + lane] = c[k];
+                else spill[sp - L] = c[k];
                 ++sp;
             }
         if (dist[0] < kInf) {
@@ -170,7 +180,7 @@ __device__ __forceinline__ bool traverse(const Bvh4Node* __restrict__ nodes,
         } else {
             if (sp == 0) break;
             --sp;
-            node = stack[sp * kTraceBlock + lane];
+            node = (X == 0 || sp < L) ? stack[sp * kTraceBlock + lane] : spill[sp - L];
         }
     }
     return h.leaf >= 0;
