@@ -50,7 +50,8 @@ def workload(name: str, wf, scenes, quick: bool):
 # Kernels whose bound is HBM bandwidth (sort, compaction, gathers, shading); traversal and the
 # shadow any-hit walks are latency- and divergence-bound and are reported as rays/s.
 HBM_KERNELS = ("ray_keys", "ray_sort", "gather_rays", "split", "compact", "policy_keys",
-               "shading_sort", "gather_shading", "shade", "next_rays", "miss", "raygen")
+               "shading_sort", "gather_shading", "shade", "next_rays", "miss", "raygen",
+               "regen", "finish")
 
 
 def stream_copy_gbs(wf, ctx, nbytes: int, reps: int) -> float:
@@ -89,6 +90,9 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="sorting_stress")
     ap.add_argument("--policy", default="none")
+    ap.add_argument("--schedule", default="regenerate", choices=["regenerate", "passes"],
+                    help="regenerate: full wavefront, pixels start their next sample as soon as "
+                         "their path ends; passes: the reference's pass-by-pass loop (same image)")
     ap.add_argument("--quick", action="store_true", help="reduced sizes (CPU-side checks)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-policies", action="store_true")
@@ -124,13 +128,14 @@ def main() -> None:
         f"four-wide nodes {scene.stats.node_count}, stack bound {scene.stats.stack_bound})")
     physics = wf.PHYSICS_UNBIASED if args.workload == "sorting_stress" else wf.PHYSICS_REFERENCE
 
-    def params(policy, seed=1, w=W, h=H, s=spp):
+    def params(policy, seed=1, w=W, h=H, s=spp, schedule=args.schedule):
         return wf.render_params(width=w, height=h, spp=s, max_depth=depth,
                                 camera=data.camera, policy=policy, physics=physics,
-                                firefly=1e30, seed=seed)
+                                firefly=1e30, seed=seed, schedule=schedule)
 
-    def frame(policy, seed=1):
-        img, cnt, st = wf.render(scene, params(policy, seed), keep_on_device=True)
+    def frame(policy, seed=1, schedule=args.schedule):
+        img, cnt, st = wf.render(scene, params(policy, seed, schedule=schedule),
+                                 keep_on_device=True)
         return st
 
     for w in range(args.warmup):
@@ -193,6 +198,16 @@ def main() -> None:
     if roofline is not None:
         roofline["stream_copy_GB_s"] = copy
 
+    # the other schedule of the same frame, for comparison (identical image)
+    other = "passes" if args.schedule == "regenerate" else "regenerate"
+    t1 = time.perf_counter()
+    st = frame(args.policy, seed=5, schedule=other)
+    ctx.synchronize()
+    dt = time.perf_counter() - t1
+    other_sched = {"schedule": other, "ms": round(dt * 1e3, 1),
+                   "mrays_s": round((st.rays_traced + st.shadow_rays) / dt / 1e6, 2)}
+    log(f"[bench] schedule {other}: {other_sched}")
+
     policies = {}
     if not args.no_policies:
         for pol in ("none", "material", "texture", "costBenefit"):
@@ -244,7 +259,8 @@ def main() -> None:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded generator)",
-        "config": {"workload": desc, "policy": args.policy, "physics":
+        "config": {"workload": desc, "policy": args.policy, "schedule": args.schedule,
+                   "other_schedule": other_sched, "physics":
                    "unbiased" if physics else "reference", "triangles": data.triangle_count,
                    "bvh_build_ms": round(scene.stats.build_ms, 1), "policies": policies},
         "roofline": roofline,

@@ -311,6 +311,7 @@ typedef struct wf_render_params {
     float adaptive_threshold;  /* AdaptiveSampler relative threshold (reference default 0.05) */
     int32_t adaptive_min_samples;   /* reference: 32 */
     int32_t adaptive_interval;      /* reference: 16 */
+    int32_t schedule;          /* WF_SCHEDULE_*: same image, different order of work */
     wf_shade_params shade;
     wf_camera camera;
 } wf_render_params;
@@ -330,11 +331,95 @@ typedef struct wf_render_stats {  /* the reference's stats blocks (cpp:321-370) 
     double miss_ms;
 } wf_render_stats;
 
+/* wf_render_params::schedule. REGENERATE keeps a full wavefront: a pixel starts its next sample
+ * as soon as its previous path ends (one path per pixel in flight, so its accumulator receives
+ * the same additions in the same order as the pass loop). PASSES is the reference's loop: all
+ * pixels sample pass by pass, bounce by bounce (cpp:141-312). */
+#define WF_SCHEDULE_REGENERATE 0
+#define WF_SCHEDULE_PASSES 1
+
 /* Renders one image. image_rgb: caller-owned device float buffer width*height*3 receiving
  * accumulator / per-pixel sample count (writeAveragedImage, cpp:127-137). sample_counts
  * (optional device int32[width*height]) receives the per-pixel sample counts. */
 wf_status wf_render(wf_scene* scene, const wf_render_params* params, float* image_rgb,
                     int32_t* sample_counts, wf_render_stats* stats);
+
+/* ---- batched and served forms: the job-tagged wavefront engine ---------------------------
+ * No reference counterpart (the reference renders one image per process, main.cpp:149-215).
+ * An engine keeps one scene's wavefront resident: a ray queue of `capacity` slots shared by the
+ * paths of every admitted job, and framebuffers for up to `pixel_budget` pixels. Jobs are
+ * admitted while a job slot and a contiguous framebuffer range are free; each job's image is
+ * identical to wf_render of the same job with the same seed and job tag. */
+typedef struct wf_engine wf_engine;
+
+typedef struct wf_engine_config {
+    int32_t capacity;          /* ray slots (paths in flight) */
+    int32_t max_jobs;          /* job slots (<= 2048) */
+    int64_t pixel_budget;      /* framebuffer pixels across admitted jobs */
+    int32_t policy;            /* WF_POLICY_* shading-queue order */
+    int32_t ray_sort;          /* RayQueue Morton sort of every iteration's rays */
+    int32_t use_graph;         /* replay each iteration from a captured HIP graph */
+    int32_t reserved;
+    wf_shade_params shade;     /* physics, RR and firefly settings (seed/sample/job unused) */
+} wf_engine_config;
+
+typedef struct wf_job_desc {
+    int32_t width;
+    int32_t height;
+    int32_t spp;               /* sample budget per pixel */
+    int32_t max_depth;
+    uint32_t seed;
+    int32_t job_tag;           /* RNG job id: equal (seed, job_tag) give equal images */
+    int32_t adaptive;
+    float adaptive_threshold;
+    int32_t adaptive_min_samples;
+    int32_t adaptive_interval;
+    wf_camera camera;
+} wf_job_desc;
+
+typedef struct wf_job_result {
+    int32_t width;
+    int32_t height;
+    int32_t samples_run;       /* most samples any pixel ran */
+    int32_t active_pixels_end; /* adaptive: pixels that used the whole budget unconverged */
+} wf_job_result;
+
+typedef struct wf_engine_stats {
+    int64_t rays_traced;
+    int64_t shadow_rays;
+    int64_t shaded_hits;
+    int64_t misses;
+    int64_t paths;             /* samples started */
+    int64_t iterations;
+    int64_t queued_pixels;     /* pixels waiting to start a sample */
+    int32_t live_rays;
+    int32_t running_jobs;
+    int32_t free_pixels;
+    int32_t reserved;
+    double phase_ms[5];        /* sort, intersect, schedule, shade, miss (with phase timing) */
+} wf_engine_stats;
+
+/* Device bytes an engine of this capacity and pixel budget needs (admission planning). */
+wf_status wf_engine_bytes(int32_t capacity, int64_t pixel_budget, size_t* out);
+/* The engine runs on a stream of its own on ctx's device; it waits for work queued on the
+ * scene's context before each run. */
+wf_status wf_engine_create(wf_context* ctx, wf_scene* scene, const wf_engine_config* config,
+                           wf_engine** out);
+wf_status wf_engine_destroy(wf_engine* engine);
+wf_status wf_engine_device_bytes(const wf_engine* engine, size_t* out);
+/* Admits a job; WF_ERR_ADMISSION when no job slot or framebuffer range is free. */
+wf_status wf_engine_admit(wf_engine* engine, const wf_job_desc* job, int32_t* job_id);
+/* Runs iterations until at least one job finishes, max_iterations (<= 0: no limit) have run or
+ * nothing is left to do; the finished job ids (up to finished_cap) go to `finished`. */
+wf_status wf_engine_run(wf_engine* engine, int32_t max_iterations, int32_t* finished,
+                        int32_t finished_cap, int32_t* n_finished);
+/* Resolves a finished job into caller-owned device buffers (image_rgb: width*height*3 floats;
+ * sample_counts optional) and frees its slot and framebuffer range. */
+wf_status wf_engine_result(wf_engine* engine, int32_t job_id, float* image_rgb,
+                           int32_t* sample_counts, wf_job_result* result);
+wf_status wf_engine_query(wf_engine* engine, wf_engine_stats* out);
+/* Per-phase event timing of every iteration (disables graph replay while on). */
+wf_status wf_engine_phase_timing(wf_engine* engine, int32_t enable);
 
 /* ---- per-kernel device timing (HIP events on the context stream) ------------------------ */
 

@@ -37,13 +37,16 @@ def test_version_and_error_without_device():
 
 @pytest.mark.parametrize("name", ["Material", "TextureDesc", "MeshDesc", "SceneDesc", "RaySoA",
                                   "ShadeSoA", "HitSoA", "BuildStats", "ShadeParams",
-                                  "RenderParams", "RenderStats", "Camera"])
+                                  "RenderParams", "RenderStats", "Camera", "EngineConfig",
+                                  "JobDesc", "JobResult", "EngineStats"])
 def test_struct_layout_matches_header(tmp_path, name):
     cname = {"Material": "wf_material", "TextureDesc": "wf_texture_desc",
              "MeshDesc": "wf_mesh_desc", "SceneDesc": "wf_scene_desc", "RaySoA": "wf_ray_soa",
              "ShadeSoA": "wf_shade_soa", "HitSoA": "wf_hit_soa", "BuildStats": "wf_build_stats",
              "ShadeParams": "wf_shade_params", "RenderParams": "wf_render_params",
-             "RenderStats": "wf_render_stats", "Camera": "wf_camera"}[name]
+             "RenderStats": "wf_render_stats", "Camera": "wf_camera",
+             "EngineConfig": "wf_engine_config", "JobDesc": "wf_job_desc",
+             "JobResult": "wf_job_result", "EngineStats": "wf_engine_stats"}[name]
     cls = getattr(abi, name)
     fields = [f[0] for f in cls._fields_]
     nl = "\\n"

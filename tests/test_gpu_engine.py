@@ -1,0 +1,171 @@
+"""The job-tagged regenerating wavefront (wf_engine_*, and wf_render's default schedule).
+
+A pixel keeps at most one path in flight and starts its next sample only when the previous path
+ended, so its accumulator receives the additions of the reference's pass-by-pass loop
+(WavefrontRenderer::renderScene, src/render/WavefrontRenderer.cpp:141-312) in the same order.
+With the counter-based RNG every image must therefore be bit-identical to the pass loop's, for
+every policy, with adaptive sampling, and for every job of a batch rendered alongside others.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+f32 = np.float32
+
+
+@pytest.fixture(scope="module")
+def cornell(wf, ctx, scenes):
+    data = scenes.cornell_box()
+    return data, wf.Scene(ctx, data)
+
+
+@pytest.fixture(scope="module")
+def stress(wf, ctx, scenes):
+    data = scenes.sorting_stress(subdiv=5, tex_size=64)
+    return data, wf.Scene(ctx, data)
+
+
+def both_schedules(wf, scene, data, **kw):
+    out = []
+    for sched in ("passes", "regenerate"):
+        p = wf.render_params(camera=data.camera, schedule=sched, **kw)
+        out.append(wf.render(scene, p))
+    return out
+
+
+def assert_same(a, b):
+    (ia, ca, sa), (ib, cb, sb) = a, b
+    assert np.array_equal(ca, cb)
+    assert np.array_equal(ia, ib), f"max |diff| {np.abs(ia - ib).max()}"
+    assert sa.rays_traced == sb.rays_traced and sa.shadow_rays == sb.shadow_rays
+    assert sa.shaded_hits == sb.shaded_hits and sa.misses == sb.misses
+    assert sa.samples_run == sb.samples_run and sa.active_pixels_end == sb.active_pixels_end
+
+
+@pytest.mark.parametrize("policy,ray_sort", [("none", True), ("none", False), ("material", True),
+                                             ("texture", True), ("costBenefit", True)])
+def test_regenerate_equals_passes_cornell(wf, cornell, policy, ray_sort):
+    data, scene = cornell
+    a, b = both_schedules(wf, scene, data, width=48, height=40, spp=4, max_depth=8, seed=21,
+                          firefly=1e30, policy=policy, ray_sort=ray_sort)
+    assert_same(a, b)
+
+
+@pytest.mark.parametrize("policy", ["none", "costBenefit"])
+def test_regenerate_equals_passes_textured_unbiased(wf, stress, policy):
+    data, scene = stress
+    a, b = both_schedules(wf, scene, data, width=64, height=48, spp=3, max_depth=12, seed=7,
+                          physics=wf.PHYSICS_UNBIASED, firefly=1e30, policy=policy)
+    assert_same(a, b)
+
+
+def test_regenerate_equals_passes_adaptive(wf, cornell):
+    data, scene = cornell
+    a, b = both_schedules(wf, scene, data, width=40, height=32, spp=96, max_depth=6, seed=11,
+                          adaptive=True, adaptive_threshold=0.2, adaptive_min_samples=8,
+                          adaptive_interval=8, firefly=10.0)
+    assert_same(a, b)
+    assert a[2].samples_run <= 96 and a[1].min() >= 8
+
+
+def jobs_for(wf, data, rng, n, adaptive_every=3):
+    out = []
+    for k in range(n):
+        w, h = int(rng.integers(8, 56)), int(rng.integers(8, 48))
+        cam = dict(data.camera)
+        cam["eye"] = tuple(np.array(cam["eye"]) + rng.uniform(-0.05, 0.05, 3))
+        spp = int(rng.choice([1, 2, 4, 8]))
+        depth = int(rng.integers(2, 9))
+        adaptive = k % adaptive_every == 2
+        out.append(dict(width=w, height=h, spp=spp * (8 if adaptive else 1), max_depth=depth,
+                        camera=cam, seed=100 + k, job_tag=1000 + k, adaptive=adaptive,
+                        adaptive_threshold=0.3, adaptive_min_samples=4, adaptive_interval=4))
+    return out
+
+
+def alone(wf, scene, j, policy="none"):
+    p = wf.render_params(width=j["width"], height=j["height"], spp=j["spp"],
+                         max_depth=j["max_depth"], camera=j["camera"], seed=j["seed"],
+                         job=j["job_tag"], adaptive=j["adaptive"],
+                         adaptive_threshold=j["adaptive_threshold"],
+                         adaptive_min_samples=j["adaptive_min_samples"],
+                         adaptive_interval=j["adaptive_interval"], policy=policy,
+                         schedule="passes")
+    return wf.render(scene, p)
+
+
+def run_batch(wf, engine, jobs, on_idle=None):
+    ids = {engine.admit(wf.job_desc(**j)): k for k, j in enumerate(jobs)}
+    results = {}
+    while len(results) < len(jobs):
+        for jid in engine.run():
+            results[ids[jid]] = engine.result(jid)
+        if on_idle:
+            on_idle()
+    return results
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batched_jobs_equal_alone(wf, ctx, cornell, rng, use_graph):
+    """Jobs of different sizes, budgets, depths, cameras and seeds share one wavefront that is
+    smaller than their pixels (so pixels wait in the ready queue); every image equals the job
+    rendered alone with the pass loop. With graph replay, eager renders run between replays."""
+    data, scene = cornell
+    jobs = jobs_for(wf, data, rng, 7)
+    eng = wf.Engine(ctx, scene, capacity=2500, pixel_budget=20000, max_jobs=8, use_graph=use_graph)
+    side = []
+
+    def eager():
+        if use_graph:
+            side.append(alone(wf, scene, jobs[0])[0])
+
+    got = run_batch(wf, eng, jobs, eager)
+    total_rays = 0
+    for k, j in enumerate(jobs):
+        img, cnt, st = alone(wf, scene, j)
+        gimg, gcnt, res = got[k]
+        assert np.array_equal(gcnt, cnt), k
+        assert np.array_equal(gimg, img), (k, np.abs(gimg - img).max())
+        assert res.samples_run == st.samples_run
+        total_rays += st.rays_traced
+    for s in side:
+        assert np.array_equal(s, side[0])
+    es = eng.stats()
+    assert es.rays_traced == total_rays
+    assert es.running_jobs == 0 and es.free_pixels == 20000 and es.queued_pixels == 0
+    eng.close()
+
+
+def test_batched_policy_textured(wf, ctx, stress, rng):
+    data, scene = stress
+    jobs = jobs_for(wf, data, rng, 4, adaptive_every=100)
+    eng = wf.Engine(ctx, scene, capacity=4096, pixel_budget=12000, max_jobs=4,
+                    policy="costBenefit", physics=wf.PHYSICS_REFERENCE)
+    got = run_batch(wf, eng, jobs)
+    for k, j in enumerate(jobs):
+        img, cnt, _ = alone(wf, scene, j, policy="none")
+        assert np.array_equal(got[k][0], img), k
+    eng.close()
+
+
+def test_engine_admission(wf, ctx, cornell):
+    data, scene = cornell
+    eng = wf.Engine(ctx, scene, capacity=512, pixel_budget=1000, max_jobs=2)
+    big = wf.job_desc(40, 40, 1, 2, data.camera)
+    with pytest.raises(wf.WavefrontError):
+        eng.admit(big)                                    # larger than the whole budget
+    a = eng.admit(wf.job_desc(20, 20, 2, 3, data.camera, seed=1))
+    b = eng.admit(wf.job_desc(20, 20, 2, 3, data.camera, seed=2))
+    assert eng.try_admit(wf.job_desc(4, 4, 1, 1, data.camera)) is None   # no job slot
+    done = []
+    while len(done) < 2:
+        done += eng.run()
+    assert sorted(done) == sorted([a, b])
+    for j in done:
+        eng.result(j)
+    # slots and framebuffer ranges are reusable; an empty budget finishes at once
+    z = eng.admit(wf.job_desc(30, 30, 0, 4, data.camera))
+    assert eng.run() == [z]
+    img, cnt, res = eng.result(z)
+    assert not img.any() and not cnt.any() and res.samples_run == 0
+    eng.close()

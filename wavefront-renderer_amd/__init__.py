@@ -37,10 +37,13 @@ class Context:
         self.handle = h
         self.device = device
         self._scenes = weakref.WeakSet()
+        self._engines = weakref.WeakSet()
 
     def close(self):
         if self.handle:
-            for sc in list(self._scenes):   # scenes first: they were created on this context
+            for en in list(self._engines):  # engines first: they run over the scenes
+                en.close()
+            for sc in list(self._scenes):   # scenes next: they were created on this context
                 sc.close()
             self._lib.wf_context_destroy(self.handle)
             self.handle = None
@@ -318,17 +321,25 @@ def shade_misses(scene: Scene, q: RayQueue, n: int, accum: DeviceBuffer, physics
                                          C.c_void_p(accum.ptr)))
 
 
+SCHEDULES = {"regenerate": _abi.SCHEDULE_REGENERATE, "passes": _abi.SCHEDULE_PASSES}
+
+
+def camera_struct(camera) -> _abi.Camera:
+    return _abi.Camera(_abi.f3(*camera["eye"]), _abi.f3(*camera["target"]),
+                       _abi.f3(*camera.get("up", (0.0, 1.0, 0.0))), float(camera["vfov"]))
+
+
 def render_params(width, height, spp, max_depth, camera, policy="none", ray_sort=True,
                   adaptive=False, adaptive_threshold=0.05, adaptive_min_samples=32,
                   adaptive_interval=16, physics=PHYSICS_REFERENCE, rr_min_depth=3,
-                  cost_aware_rr=False, firefly=10.0, seed=0, job=0) -> _abi.RenderParams:
+                  cost_aware_rr=False, firefly=10.0, seed=0, job=0,
+                  schedule="regenerate") -> _abi.RenderParams:
     pol = POLICIES[policy] if isinstance(policy, str) else int(policy)
-    cam = _abi.Camera(_abi.f3(*camera["eye"]), _abi.f3(*camera["target"]),
-                      _abi.f3(*camera.get("up", (0.0, 1.0, 0.0))), float(camera["vfov"]))
+    sched = SCHEDULES[schedule] if isinstance(schedule, str) else int(schedule)
     sp = shade_params(physics, rr_min_depth, cost_aware_rr, firefly, seed, 0, job)
     return _abi.RenderParams(width, height, spp, max_depth, pol, 1 if ray_sort else 0,
                              1 if adaptive else 0, adaptive_threshold, adaptive_min_samples,
-                             adaptive_interval, sp, cam)
+                             adaptive_interval, sched, sp, camera_struct(camera))
 
 
 def render(scene: Scene, params: _abi.RenderParams, keep_on_device: bool = False):
@@ -344,3 +355,96 @@ def render(scene: Scene, params: _abi.RenderParams, keep_on_device: bool = False
     image = img.download().reshape(params.height, params.width, 3)
     counts = cnt.download().reshape(params.height, params.width)
     return image, counts, st
+
+
+# ---- batched / served forms -----------------------------------------------------------------
+def job_desc(width, height, spp, max_depth, camera, seed=0, job_tag=0, adaptive=False,
+             adaptive_threshold=0.05, adaptive_min_samples=32,
+             adaptive_interval=16) -> _abi.JobDesc:
+    return _abi.JobDesc(width, height, spp, max_depth, seed, job_tag, 1 if adaptive else 0,
+                        adaptive_threshold, adaptive_min_samples, adaptive_interval,
+                        camera_struct(camera))
+
+
+class Engine:
+    """Job-tagged regenerating wavefront over one resident scene (wf_engine_*)."""
+
+    def __init__(self, ctx: Context, scene: Scene, capacity: int, pixel_budget: int,
+                 max_jobs: int = 64, policy="none", ray_sort=True, use_graph=False,
+                 physics=PHYSICS_REFERENCE, rr_min_depth=3, cost_aware_rr=False,
+                 firefly=10.0):
+        self.ctx = ctx
+        self.scene = scene
+        self._lib = ctx._lib
+        pol = POLICIES[policy] if isinstance(policy, str) else int(policy)
+        cfg = _abi.EngineConfig(capacity, max_jobs, pixel_budget, pol, 1 if ray_sort else 0,
+                                1 if use_graph else 0, 0,
+                                shade_params(physics, rr_min_depth, cost_aware_rr, firefly,
+                                             0, 0, 0))
+        h = C.c_void_p()
+        check(self._lib.wf_engine_create(ctx.handle, scene.handle, C.byref(cfg), C.byref(h)))
+        self.handle = h.value
+        self.jobs = {}
+        ctx._engines.add(self)
+
+    def admit(self, job: _abi.JobDesc) -> int:
+        jid = C.c_int32()
+        check(self._lib.wf_engine_admit(self.handle, C.byref(job), C.byref(jid)))
+        self.jobs[jid.value] = (job.width, job.height)
+        return jid.value
+
+    def try_admit(self, job: _abi.JobDesc) -> Optional[int]:
+        jid = C.c_int32()
+        st = self._lib.wf_engine_admit(self.handle, C.byref(job), C.byref(jid))
+        if st == _abi.WF_ERR_ADMISSION:
+            return None
+        check(st)
+        self.jobs[jid.value] = (job.width, job.height)
+        return jid.value
+
+    def run(self, max_iterations: int = 0, cap: int = 64) -> list:
+        buf = (C.c_int32 * cap)()
+        n = C.c_int32()
+        check(self._lib.wf_engine_run(self.handle, max_iterations, buf, cap, C.byref(n)))
+        return list(buf[:n.value])
+
+    def result(self, job_id: int, keep_on_device: bool = False):
+        w, h = self.jobs.pop(job_id)
+        img = DeviceBuffer(self.ctx, w * h * 3, np.float32)
+        cnt = DeviceBuffer(self.ctx, w * h, np.int32)
+        res = _abi.JobResult()
+        check(self._lib.wf_engine_result(self.handle, job_id, C.c_void_p(img.ptr),
+                                         C.c_void_p(cnt.ptr), C.byref(res)))
+        if keep_on_device:
+            return img, cnt, res
+        return img.download().reshape(h, w, 3), cnt.download().reshape(h, w), res
+
+    def stats(self) -> _abi.EngineStats:
+        st = _abi.EngineStats()
+        check(self._lib.wf_engine_query(self.handle, C.byref(st)))
+        return st
+
+    def device_bytes(self) -> int:
+        out = C.c_size_t()
+        check(self._lib.wf_engine_device_bytes(self.handle, C.byref(out)))
+        return out.value
+
+    def phase_timing(self, enable: bool):
+        check(self._lib.wf_engine_phase_timing(self.handle, 1 if enable else 0))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.wf_engine_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def engine_bytes(capacity: int, pixel_budget: int) -> int:
+    out = C.c_size_t()
+    check(_abi.load_library().wf_engine_bytes(capacity, pixel_budget, C.byref(out)))
+    return out.value

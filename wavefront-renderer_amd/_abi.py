@@ -27,6 +27,7 @@ POLICY_NONE, POLICY_MATERIAL, POLICY_TEXTURE, POLICY_COST_BENEFIT = range(4)
 POLICIES = {"none": POLICY_NONE, "material": POLICY_MATERIAL, "texture": POLICY_TEXTURE,
             "costBenefit": POLICY_COST_BENEFIT}
 PHYSICS_REFERENCE, PHYSICS_UNBIASED = 0, 1
+SCHEDULE_REGENERATE, SCHEDULE_PASSES = 0, 1
 TEX_RGBA8_SRGB, TEX_RGB_F32 = 0, 1
 
 f3 = C.c_float * 3
@@ -114,7 +115,7 @@ class RenderParams(C.Structure):
                 ("max_depth", C.c_int32), ("policy", C.c_int32), ("ray_sort", C.c_int32),
                 ("adaptive", C.c_int32), ("adaptive_threshold", C.c_float),
                 ("adaptive_min_samples", C.c_int32), ("adaptive_interval", C.c_int32),
-                ("shade", ShadeParams), ("camera", Camera)]
+                ("schedule", C.c_int32), ("shade", ShadeParams), ("camera", Camera)]
 
 
 class RenderStats(C.Structure):
@@ -124,6 +125,34 @@ class RenderStats(C.Structure):
                 ("frame_ms", C.c_double), ("sort_ms", C.c_double),
                 ("intersect_ms", C.c_double), ("schedule_ms", C.c_double),
                 ("shade_ms", C.c_double), ("miss_ms", C.c_double)]
+
+
+class EngineConfig(C.Structure):
+    _fields_ = [("capacity", C.c_int32), ("max_jobs", C.c_int32), ("pixel_budget", C.c_int64),
+                ("policy", C.c_int32), ("ray_sort", C.c_int32), ("use_graph", C.c_int32),
+                ("reserved", C.c_int32), ("shade", ShadeParams)]
+
+
+class JobDesc(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32),
+                ("max_depth", C.c_int32), ("seed", C.c_uint32), ("job_tag", C.c_int32),
+                ("adaptive", C.c_int32), ("adaptive_threshold", C.c_float),
+                ("adaptive_min_samples", C.c_int32), ("adaptive_interval", C.c_int32),
+                ("camera", Camera)]
+
+
+class JobResult(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("samples_run", C.c_int32),
+                ("active_pixels_end", C.c_int32)]
+
+
+class EngineStats(C.Structure):
+    _fields_ = [("rays_traced", C.c_int64), ("shadow_rays", C.c_int64),
+                ("shaded_hits", C.c_int64), ("misses", C.c_int64), ("paths", C.c_int64),
+                ("iterations", C.c_int64), ("queued_pixels", C.c_int64),
+                ("live_rays", C.c_int32), ("running_jobs", C.c_int32),
+                ("free_pixels", C.c_int32), ("reserved", C.c_int32),
+                ("phase_ms", C.c_double * 5)]
 
 
 _S = C.c_int32
@@ -162,6 +191,16 @@ _PROTOS = {
                       C.POINTER(RaySoA), C.POINTER(C.c_int32)]),
     "wf_shade_misses": (_S, [P, C.POINTER(RaySoA), C.c_int32, C.c_int32, P]),
     "wf_render": (_S, [P, C.POINTER(RenderParams), P, P, C.POINTER(RenderStats)]),
+    "wf_engine_bytes": (_S, [C.c_int32, C.c_int64, C.POINTER(C.c_size_t)]),
+    "wf_engine_create": (_S, [P, P, C.POINTER(EngineConfig), C.POINTER(P)]),
+    "wf_engine_destroy": (_S, [P]),
+    "wf_engine_device_bytes": (_S, [P, C.POINTER(C.c_size_t)]),
+    "wf_engine_admit": (_S, [P, C.POINTER(JobDesc), C.POINTER(C.c_int32)]),
+    "wf_engine_run": (_S, [P, C.c_int32, C.POINTER(C.c_int32), C.c_int32,
+                           C.POINTER(C.c_int32)]),
+    "wf_engine_result": (_S, [P, C.c_int32, P, P, C.POINTER(JobResult)]),
+    "wf_engine_query": (_S, [P, C.POINTER(EngineStats)]),
+    "wf_engine_phase_timing": (_S, [P, C.c_int32]),
     "wf_profile_enable": (_S, [P, C.c_int32]),
     "wf_profile_query": (_S, [P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                               C.POINTER(C.c_double)]),

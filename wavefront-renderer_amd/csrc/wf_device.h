@@ -121,4 +121,20 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
     return lane == 0 ? 0ull : ((~0ull) >> (64u - lane));
 }
 
+// Camera::getRay for pixel (x, y) with jitter (jx, jy) (cpp:401-457, Camera.h:35-40); cam = eye,
+// top-left corner, horizontal, vertical.
+__device__ __forceinline__ V3 primary_dir(const float* cam, int x, int y, int width, int height,
+                                          float jx, float jy) {
+    const float u = ((float)x + jx) / (float)(width - 1);
+    const float v = 1.0f - ((float)y + jy) / (float)(height - 1);
+    const V3 O(cam[0], cam[1], cam[2]), TL(cam[3], cam[4], cam[5]), H(cam[6], cam[7], cam[8]);
+    const V3 Vv(cam[9], cam[10], cam[11]);
+    return (TL + H * u + Vv * v - O).normalized();
+}
+
+// Element count of a launch: the device counter when present, else the host value.
+__device__ __forceinline__ int load_count(int n, const int32_t* __restrict__ dev) {
+    return dev ? *dev : n;
+}
+
 }  // namespace wf

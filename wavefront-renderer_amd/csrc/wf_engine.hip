@@ -1,0 +1,868 @@
+// Regenerating, job-tagged wavefront over one device-resident scene: the batched and served
+// forms, and the default schedule of wf_render.
+//
+// The reference renders sample pass by sample pass, and inside a pass bounce by bounce
+// (WavefrontRenderer::renderScene, src/render/WavefrontRenderer.cpp:141-312), so every bounce
+// after the first runs on a shrinking queue. Here a fixed-capacity ray queue is refilled at every
+// iteration: when a path ends, its pixel goes back to a ready queue and starts its next sample at
+// the next iteration. Each pixel has at most one path in flight, so its samples still run in
+// order and its accumulator receives exactly the additions of the pass-by-pass loop, in the same
+// order; with the counter-based RNG the image is identical. Rays carry a job tag (job slot,
+// sample index), so paths of many jobs of different sizes share one wavefront.
+//
+// One iteration (all counts stay on the device, so the sequence can be captured in a graph):
+//   regenerate -> [ray keys + radix sort + gather] -> closest hit + split -> [shading policy sort
+//   + gather] -> shade (emission, NEE, scatter, RR, depth limit) -> shadow rays -> next-ray
+//   compaction -> misses -> per-pixel completion (adaptive sampling, sample budget, requeue).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "wf_device.h"
+#include "wf_internal.h"
+
+namespace wf {
+
+// Camera constructor (include/core/Camera.h:13-33) in host float arithmetic.
+CamBasis camera_basis(const wf_camera& c, int width, int height) {
+    struct H {
+        float x, y, z;
+        H operator+(const H& o) const { return {x + o.x, y + o.y, z + o.z}; }
+        H operator-(const H& o) const { return {x - o.x, y - o.y, z - o.z}; }
+        H operator*(float s) const { return {x * s, y * s, z * s}; }
+        H operator/(float s) const { return {x / s, y / s, z / s}; }
+        H cross(const H& o) const { return {y * o.z - z * o.y, z * o.x - x * o.z, x * o.y - y * o.x}; }
+        H normalized() const { return *this / std::sqrt(x * x + y * y + z * z); }
+    };
+    const H from{c.eye[0], c.eye[1], c.eye[2]}, at{c.target[0], c.target[1], c.target[2]};
+    const H up{c.up[0], c.up[1], c.up[2]};
+    const float aspect = static_cast<float>(width) / height;
+    const float theta = c.vfov_deg * 3.14159265f / 180.0f;
+    const float vh = 2.0f * std::tan(theta / 2.0f);
+    const float vw = aspect * vh;
+    const H fwd = (from - at).normalized();
+    const H right = up.cross(fwd).normalized();
+    const H upv = fwd.cross(right);
+    const H horizontal = right * vw, vertical = upv * vh;
+    const H tl = from - horizontal / 2.0f - vertical / 2.0f - fwd;
+    return CamBasis{{from.x, from.y, from.z, tl.x, tl.y, tl.z, horizontal.x, horizontal.y,
+                     horizontal.z, vertical.x, vertical.y, vertical.z}};
+}
+
+// Device counters of an engine (int32 slots).
+enum : int {
+    C_LIVE = 0,        // rays in the queue at the start of an iteration
+    C_HITS = 1,        // hits of the current iteration
+    C_MISSES = 2,      // misses of the current iteration
+    C_NEXT = 4,        // next rays (shade_bounce counts_dev[0..2])
+    C_SHADOW = 5,
+    C_SHADOW_ENTRIES = 6,
+    C_COUNT = 16,
+};
+// 64-bit device statistics.
+enum : int { S_RAYS = 0, S_SHADOW, S_HITS, S_MISSES, S_PATHS, S_HEAD, S_TAIL, S_COUNT = 8 };
+
+struct EngineView {
+    int32_t cap;               // ray slots
+    int32_t px;                // framebuffer pixels (ready-queue capacity)
+    int32_t* counters;
+    unsigned long long* stats; // S_HEAD / S_TAIL: ready-queue cursors (monotonic)
+    int32_t* ready;
+    float* accum;              // 3 per pixel
+    float* before;             // 3 per pixel: accum at the start of the pixel's current sample
+    float* mean;
+    float* m2;
+    int32_t* count;            // finished samples per pixel (= index of the next sample)
+    int32_t* pixel_job;
+    const JobDev* jobs;
+    int32_t* job_left;         // pixels of each job not finished yet
+    int32_t* job_max_count;    // most samples any pixel of the job ran
+    int32_t* job_unconverged;  // adaptive: pixels that ran out of budget without converging
+};
+
+// Admits pixels [base, base + n) of job j: clears their state and queues them.
+__global__ void k_admit(EngineView e, int job, int base, int n, int queue) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int p = base + k;
+    e.accum[3 * (size_t)p] = 0.0f;
+    e.accum[3 * (size_t)p + 1] = 0.0f;
+    e.accum[3 * (size_t)p + 2] = 0.0f;
+    e.before[3 * (size_t)p] = 0.0f;
+    e.before[3 * (size_t)p + 1] = 0.0f;
+    e.before[3 * (size_t)p + 2] = 0.0f;
+    e.mean[p] = 0.0f;
+    e.m2[p] = 0.0f;
+    e.count[p] = 0;
+    e.pixel_job[p] = job;
+    if (queue) e.ready[(e.stats[S_TAIL] + (unsigned long long)k) % (unsigned long long)e.px] = p;
+}
+
+__global__ void k_admit_commit(EngineView e, int job, int n, int queue) {
+    if (queue) e.stats[S_TAIL] += (unsigned long long)n;
+    e.job_left[job] = queue ? n : 0;
+    e.job_max_count[job] = 0;
+    e.job_unconverged[job] = 0;
+}
+
+__device__ __forceinline__ int regen_take(const EngineView& e) {
+    const int live = e.counters[C_LIVE];
+    const unsigned long long avail = e.stats[S_TAIL] - e.stats[S_HEAD];
+    const int room = e.cap - live;
+    return (unsigned long long)room < avail ? room : (int)avail;
+}
+
+// Primary rays for the next samples of queued pixels, appended after the live rays
+// (generatePrimaryRays, cpp:401-457, per pixel instead of per pass).
+__global__ void k_regen(EngineView e, wf_ray_soa r, int32_t* __restrict__ tags) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int take = regen_take(e);
+    if (i >= take) return;
+    const int p = e.ready[(e.stats[S_HEAD] + (unsigned long long)i) % (unsigned long long)e.px];
+    const int j = e.pixel_job[p];
+    const JobDev& J = e.jobs[j];
+    const int s = e.count[p];
+    if (J.adaptive) {
+        e.before[3 * (size_t)p] = e.accum[3 * (size_t)p];
+        e.before[3 * (size_t)p + 1] = e.accum[3 * (size_t)p + 1];
+        e.before[3 * (size_t)p + 2] = e.accum[3 * (size_t)p + 2];
+    }
+    const int local = p - J.pixel_base;
+    PathRng rng{J.seed, (uint32_t)J.tag, (uint32_t)local, (uint32_t)s, kRaygenDepth};
+    const V3 d = primary_dir(J.cam, local % J.width, local / J.width, J.width, J.height, rng(0),
+                             rng(1));
+    const int k = e.counters[C_LIVE] + i;
+    st3(r.ox, r.oy, r.oz, k, V3(J.cam[0], J.cam[1], J.cam[2]));
+    st3(r.dx, r.dy, r.dz, k, d);
+    r.tr[k] = 1.0f;
+    r.tg[k] = 1.0f;
+    r.tb[k] = 1.0f;
+    r.pixel[k] = p;
+    r.depth[k] = 0;
+    r.pdf[k] = -1.0f;
+    tags[k] = (j << kTagJobShift) | s;
+}
+
+__global__ void k_regen_commit(EngineView e) {
+    const int take = regen_take(e);
+    e.stats[S_HEAD] += (unsigned long long)take;
+    const int live = e.counters[C_LIVE] + take;
+    e.counters[C_LIVE] = live;
+    e.stats[S_RAYS] += (unsigned long long)live;
+    e.stats[S_PATHS] += (unsigned long long)take;
+}
+
+// A path of pixel p ended: count the sample, update the adaptive estimate
+// (AdaptiveSampler::addSample/updateConvergence, include/render/AdaptiveSampler.h:21-105, checked
+// when the pixel's sample count is a multiple of the interval, as the pass loop does), then
+// requeue the pixel or retire it.
+__device__ __forceinline__ void finish_path(const EngineView& e, int p, bool valid) {
+    bool requeue = false;
+    if (valid) {
+        const int j = e.pixel_job[p];
+        const JobDev& J = e.jobs[j];
+        const int cnt = e.count[p] + 1;
+        e.count[p] = cnt;
+        bool converged = false;
+        if (J.adaptive) {
+            const size_t b = 3 * (size_t)p;
+            const V3 c(e.accum[b] - e.before[b], e.accum[b + 1] - e.before[b + 1],
+                       e.accum[b + 2] - e.before[b + 2]);
+            const float x = luminance(c);
+            const float mb = e.mean[p];
+            const float dev = x - mb;
+            const float mn = mb + dev / (float)cnt;
+            e.mean[p] = mn;
+            const float dev2 = x - mn;
+            const float m2 = e.m2[p] + dev * dev2;
+            e.m2[p] = m2;
+            if (cnt % J.adaptive_interval == 0 && cnt >= J.adaptive_min_samples) {
+                const float var = m2 / (float)(cnt - 1);
+                const float se = __builtin_sqrtf(var / (float)cnt);
+                converged = se < J.adaptive_threshold * (mn + 0.01f);
+            }
+        }
+        const bool budget = cnt >= J.spp;
+        if (converged || budget) {
+            atomicMax(&e.job_max_count[j], cnt);
+            if (J.adaptive && !converged) atomicAdd(&e.job_unconverged[j], 1);
+            atomicSub(&e.job_left[j], 1);
+        } else {
+            requeue = true;
+        }
+    }
+    // wave-aggregated append to the ready queue
+    const uint64_t b = __ballot(requeue);
+    if (b == 0) return;
+    const int leader = __ffsll((unsigned long long)b) - 1;
+    unsigned long long base = 0;
+    if ((int)__lane_id() == leader) base = atomicAdd(&e.stats[S_TAIL], (unsigned long long)__popcll(b));
+    base = __shfl(base, leader, 64);
+    if (requeue) {
+        const unsigned long long pos = base + (unsigned long long)__popcll(b & lanemask_lt());
+        e.ready[pos % (unsigned long long)e.px] = p;
+    }
+}
+
+__global__ void k_finish_hits(EngineView e, const int32_t* __restrict__ pixel,
+                              const uint32_t* __restrict__ alive) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = e.counters[C_HITS];
+    if (blockIdx.x * blockDim.x >= n) return;
+    const bool valid = i < n && !alive[i];
+    finish_path(e, valid ? pixel[i] : 0, valid);
+}
+
+__global__ void k_finish_misses(EngineView e, const int32_t* __restrict__ pixel) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = e.counters[C_MISSES];
+    if (blockIdx.x * blockDim.x >= n) return;
+    const bool valid = i < n;
+    finish_path(e, valid ? pixel[i] : 0, valid);
+}
+
+__global__ void k_iteration_end(EngineView e) {
+    e.stats[S_SHADOW] += (unsigned long long)e.counters[C_SHADOW];
+    e.stats[S_HITS] += (unsigned long long)e.counters[C_HITS];
+    e.stats[S_MISSES] += (unsigned long long)e.counters[C_MISSES];
+    e.counters[C_LIVE] = e.counters[C_NEXT];
+}
+
+__global__ void k_resolve_job(const float* __restrict__ accum, const int32_t* __restrict__ count,
+                              int base, int n, float* __restrict__ image,
+                              int32_t* __restrict__ counts_out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int p = base + k;
+    int c = count[p];
+    c = c > 1 ? c : 1;
+    image[3 * (size_t)k] = accum[3 * (size_t)p] / (float)c;
+    image[3 * (size_t)k + 1] = accum[3 * (size_t)p + 1] / (float)c;
+    image[3 * (size_t)k + 2] = accum[3 * (size_t)p + 2] / (float)c;
+    if (counts_out) counts_out[k] = count[p];
+}
+
+}  // namespace wf
+
+using namespace wf;
+
+struct wf_engine {
+    wf_context* parent = nullptr;
+    wf_context* ctx = nullptr;          // own stream and workspaces
+    wf_scene* scene = nullptr;
+    wf_engine_config cfg{};
+    int32_t cap = 0, px = 0, max_jobs = 0;
+    std::vector<void*> allocations;
+    size_t bytes = 0;
+    // device state
+    wf_ray_soa r0{}, r1{}, miss{};
+    wf_shade_soa s0{}, s1{};
+    int32_t *t0 = nullptr, *t1 = nullptr, *st0 = nullptr, *st1 = nullptr, *idx = nullptr;
+    EngineView view{};
+    JobDev* jobs_dev = nullptr;
+    // host mirrors
+    std::vector<JobDev> jobs;
+    std::vector<int32_t> job_state;     // 0 free, 1 running, 2 finished (result not collected)
+    std::vector<int64_t> job_iter_start;
+    std::vector<std::pair<int32_t, int32_t>> free_ranges;   // [begin, end) of free pixels
+    int32_t* status_host = nullptr;     // pinned: counters + job_left per slot (two buffers)
+    hipEvent_t status_ev[2] = {nullptr, nullptr};
+    hipEvent_t phase_ev[2][8] = {};    // per status buffer: marks between the phases
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int64_t iterations = 0;
+    int32_t running = 0;
+    double phase_ms[5] = {0, 0, 0, 0, 0};
+    bool phase_timing = false;
+};
+
+namespace {
+
+wf_status engine_alloc(wf_engine* e, size_t bytes, void** out) {
+    void* p = nullptr;
+    hipError_t err = hipMalloc(&p, bytes < 16 ? 16 : bytes);
+    if (err != hipSuccess) {
+        (void)hipGetLastError();
+        return set_error(WF_ERR_OUT_OF_MEMORY, "engine: hipMalloc(%zu) failed: %s", bytes,
+                         hipGetErrorString(err));
+    }
+    e->allocations.push_back(p);
+    e->bytes += bytes;
+    *out = p;
+    return WF_OK;
+}
+
+wf_status carve_rays(wf_engine* e, size_t cap, wf_ray_soa* r) {
+    void* p = nullptr;
+    WF_TRY(engine_alloc(e, cap * 48, &p));
+    float* f = (float*)p;
+    r->ox = f; r->oy = f + cap; r->oz = f + 2 * cap;
+    r->dx = f + 3 * cap; r->dy = f + 4 * cap; r->dz = f + 5 * cap;
+    r->tr = f + 6 * cap; r->tg = f + 7 * cap; r->tb = f + 8 * cap;
+    r->pixel = (int32_t*)(f + 9 * cap);
+    r->depth = (int32_t*)(f + 10 * cap);
+    r->pdf = f + 11 * cap;
+    return WF_OK;
+}
+
+wf_status carve_shading(wf_engine* e, size_t cap, wf_shade_soa* q) {
+    void* p = nullptr;
+    WF_TRY(engine_alloc(e, cap * 104, &p));
+    float* f = (float*)p;
+    float** fl[19] = {&q->px, &q->py, &q->pz, &q->nx, &q->ny, &q->nz, &q->t, &q->u, &q->v,
+                      &q->ox, &q->oy, &q->oz, &q->dx, &q->dy, &q->dz, &q->tr, &q->tg, &q->tb,
+                      &q->pdf};
+    for (int k = 0; k < 19; ++k) *fl[k] = f + k * cap;
+    int32_t** in[6] = {&q->tri, &q->mat, &q->tex, &q->pixel, &q->depth, &q->geom};
+    for (int k = 0; k < 6; ++k) *in[k] = (int32_t*)(f + (19 + k) * cap);
+    q->front = (uint8_t*)(f + 25 * cap);
+    return WF_OK;
+}
+
+template <typename T>
+wf_status engine_array(wf_engine* e, size_t count, T** out) {
+    void* p = nullptr;
+    WF_TRY(engine_alloc(e, count * sizeof(T), &p));
+    *out = (T*)p;
+    return WF_OK;
+}
+
+// Bytes of device memory per ray slot and per framebuffer pixel (admission accounting).
+constexpr size_t kBytesPerSlot = 48 * 3 + 104 * 2 + 4 * 5 + 400;   // queues + tags + workspaces
+constexpr size_t kBytesPerPixel = 12 + 12 + 4 + 4 + 4 + 4 + 4;
+
+int g_phase_set = 0;   // event set of the iteration being enqueued
+
+void phase_mark(wf_engine* e, int k) {
+    if (e->phase_timing) (void)hipEventRecord(e->phase_ev[g_phase_set][k], e->ctx->stream);
+}
+
+// Adds the phase times of the iteration recorded in event set b (already complete).
+void accumulate_phases(wf_engine* e, int b) {
+    if (!e->phase_timing) return;
+    const int phase_of[7] = {-1, 0, 1, 2, 3, 4, -1};
+    for (int k = 0; k < 7; ++k) {
+        if (phase_of[k] < 0) continue;
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e->phase_ev[b][k], e->phase_ev[b][k + 1]);
+        e->phase_ms[phase_of[k]] += ms;
+    }
+}
+
+// One iteration on the engine's stream (no host synchronisation).
+wf_status enqueue_iteration(wf_engine* e) {
+    wf_context* ctx = e->ctx;
+    const wf_scene* s = e->scene;
+    const EngineView& v = e->view;
+    const int cap = e->cap;
+    int32_t* c = v.counters;
+    const Count live{cap, c + C_LIVE}, hits{cap, c + C_HITS}, misses{cap, c + C_MISSES};
+    phase_mark(e, 0);
+    {
+        Timed tm(ctx, "regen", 56.0 * cap);
+        k_regen<<<grid_for(cap, 256), 256, 0, ctx->stream>>>(v, e->r0, e->t0);
+        WF_HIP(hipGetLastError());
+        k_regen_commit<<<1, 1, 0, ctx->stream>>>(v);
+        WF_HIP(hipGetLastError());
+    }
+    phase_mark(e, 1);
+    const wf_ray_soa* rq = &e->r0;
+    const int32_t* rt = e->t0;
+    if (e->cfg.ray_sort) {
+        WF_TRY(ray_schedule(ctx, &e->r0, live, s->dev.bounds_min, s->dev.bounds_max, e->idx,
+                            nullptr));
+        WF_TRY(gather_rays(ctx, &e->r0, e->idx, live, &e->r1, e->t0, e->t1));
+        rq = &e->r1;
+        rt = e->t1;
+    }
+    phase_mark(e, 2);
+    WF_TRY(intersect_split(ctx, s, rq, live, &e->s0, &e->miss, c + C_HITS, rt, e->st0));
+    phase_mark(e, 3);
+    const wf_shade_soa* q = &e->s0;
+    const int32_t* qt = e->st0;
+    if (e->cfg.policy != WF_POLICY_NONE) {
+        WF_TRY(shading_schedule(ctx, s, &e->s0, hits, e->cfg.policy, e->idx));
+        WF_TRY(gather_shading(ctx, &e->s0, e->idx, hits, &e->s1, e->st0, e->st1));
+        q = &e->s1;
+        qt = e->st1;
+    }
+    phase_mark(e, 4);
+    const uint32_t* alive = nullptr;
+    ShadeJobs jobs;
+    jobs.tags = qt;
+    jobs.jobs = e->jobs_dev;
+    jobs.next_tags = e->t0;
+    WF_TRY(shade_bounce(ctx, s, q, hits, &e->cfg.shade, v.accum, &e->r0, c + C_NEXT, jobs,
+                        &alive));
+    phase_mark(e, 5);
+    WF_TRY(shade_misses(ctx, s, &e->miss, misses, e->cfg.shade.physics, v.accum));
+    phase_mark(e, 6);
+    {
+        Timed tm(ctx, "finish", 12.0 * cap);
+        k_finish_hits<<<grid_for(cap, 256), 256, 0, ctx->stream>>>(v, q->pixel, alive);
+        WF_HIP(hipGetLastError());
+        k_finish_misses<<<grid_for(cap, 256), 256, 0, ctx->stream>>>(v, e->miss.pixel);
+        WF_HIP(hipGetLastError());
+        k_iteration_end<<<1, 1, 0, ctx->stream>>>(v);
+        WF_HIP(hipGetLastError());
+    }
+    phase_mark(e, 7);
+    return WF_OK;
+}
+
+// Status of iteration results: counters block + per-job pixels left, into pinned buffer b.
+constexpr int kStatusHeader = 8;   // int32: live, ready (low 31 bits of tail - head)
+wf_status enqueue_status(wf_engine* e, int b) {
+    int32_t* dst = e->status_host + (size_t)b * (kStatusHeader + e->max_jobs);
+    WF_HIP(hipMemcpyAsync(dst, e->view.counters, sizeof(int32_t) * 2, hipMemcpyDeviceToHost,
+                          e->ctx->stream));
+    WF_HIP(hipMemcpyAsync(dst + 2, e->view.stats + S_HEAD, sizeof(unsigned long long) * 2,
+                          hipMemcpyDeviceToHost, e->ctx->stream));
+    WF_HIP(hipMemcpyAsync(dst + kStatusHeader, e->view.job_left, sizeof(int32_t) * e->max_jobs,
+                          hipMemcpyDeviceToHost, e->ctx->stream));
+    WF_HIP(hipEventRecord(e->status_ev[b], e->ctx->stream));
+    return WF_OK;
+}
+
+void copy_profile_flags(wf_engine* e) {
+    e->ctx->profile = e->parent->profile;
+    e->ctx->profile_only = e->parent->profile_only;
+}
+
+void merge_profile(wf_engine* e) {
+    if (!e->ctx->profile) return;
+    (void)harvest_timings(e->ctx);
+    for (auto& kv : e->ctx->prof) {
+        auto& d = e->parent->prof[kv.first];
+        d.ms += kv.second.ms;
+        d.launches += kv.second.launches;
+        d.bytes += kv.second.bytes;
+    }
+    e->ctx->prof.clear();
+}
+
+wf_status launch_iteration(wf_engine* e, int b) {
+    g_phase_set = b;
+    const bool graph_ok = e->cfg.use_graph && !e->ctx->profile && !e->phase_timing;
+    if (graph_ok && !e->graph_exec && e->iterations > 0) {
+        // the first iteration ran directly and sized every workspace: capture from here on
+        WF_HIP(hipStreamBeginCapture(e->ctx->stream, hipStreamCaptureModeThreadLocal));
+        wf_status st = enqueue_iteration(e);
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(e->ctx->stream, &g);
+        if (st != WF_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+        if (ce != hipSuccess) return set_error(WF_ERR_HIP, "engine graph capture: %s", hipGetErrorString(ce));
+        e->graph = g;
+        WF_HIP(hipGraphInstantiate(&e->graph_exec, g, nullptr, nullptr, 0));
+    }
+    if (graph_ok && e->graph_exec) {
+        WF_HIP(hipGraphLaunch(e->graph_exec, e->ctx->stream));
+    } else {
+        WF_TRY(enqueue_iteration(e));
+    }
+    ++e->iterations;
+    return WF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+wf_status wf_engine_create(wf_context* ctx, wf_scene* scene, const wf_engine_config* cfg,
+                           wf_engine** out) {
+    clear_error();
+    WF_REQUIRE(ctx && scene && cfg && out, "wf_engine_create: NULL argument");
+    WF_REQUIRE(cfg->policy >= WF_POLICY_NONE && cfg->policy <= WF_POLICY_COST_BENEFIT,
+               "wf_engine_create: unknown policy %d", cfg->policy);
+    WF_REQUIRE(cfg->shade.physics == WF_PHYSICS_REFERENCE || cfg->shade.physics == WF_PHYSICS_UNBIASED,
+               "wf_engine_create: unknown physics %d", cfg->shade.physics);
+    WF_REQUIRE(cfg->capacity > 0 && cfg->capacity < (1 << 29), "wf_engine_create: capacity %d out of range",
+               cfg->capacity);
+    WF_REQUIRE(cfg->pixel_budget > 0 && cfg->pixel_budget < (1ll << 30),
+               "wf_engine_create: pixel budget %lld out of range", (long long)cfg->pixel_budget);
+    WF_REQUIRE(cfg->max_jobs > 0 && cfg->max_jobs <= kMaxJobs, "wf_engine_create: max_jobs %d not in [1, %d]",
+               cfg->max_jobs, kMaxJobs);
+    WF_TRY(scene_ready(scene));
+    wf_engine* e = new wf_engine();
+    e->parent = ctx;
+    e->scene = scene;
+    e->cfg = *cfg;
+    e->cap = cfg->capacity;
+    e->px = (int32_t)cfg->pixel_budget;
+    e->max_jobs = cfg->max_jobs;
+    auto fail = [&](wf_status st) {
+        wf_engine_destroy(e);
+        return st;
+    };
+    wf_status st = wf_context_create(ctx->device, &e->ctx);
+    if (st != WF_OK) {
+        delete e;
+        return st;
+    }
+    const size_t cap = (size_t)e->cap, px = (size_t)e->px;
+#define ETRY(x)                         \
+    do {                                \
+        wf_status s_ = (x);             \
+        if (s_ != WF_OK) return fail(s_); \
+    } while (0)
+    ETRY(carve_rays(e, cap, &e->r0));
+    ETRY(carve_rays(e, cap, &e->r1));
+    ETRY(carve_rays(e, cap, &e->miss));
+    ETRY(carve_shading(e, cap, &e->s0));
+    ETRY(carve_shading(e, cap, &e->s1));
+    ETRY(engine_array(e, cap, &e->t0));
+    ETRY(engine_array(e, cap, &e->t1));
+    ETRY(engine_array(e, cap, &e->st0));
+    ETRY(engine_array(e, cap, &e->st1));
+    ETRY(engine_array(e, cap, &e->idx));
+    EngineView& v = e->view;
+    v.cap = e->cap;
+    v.px = e->px;
+    ETRY(engine_array(e, (size_t)C_COUNT, &v.counters));
+    ETRY(engine_array(e, (size_t)S_COUNT, &v.stats));
+    ETRY(engine_array(e, px, &v.ready));
+    ETRY(engine_array(e, 3 * px, &v.accum));
+    ETRY(engine_array(e, 3 * px, &v.before));
+    ETRY(engine_array(e, px, &v.mean));
+    ETRY(engine_array(e, px, &v.m2));
+    ETRY(engine_array(e, px, &v.count));
+    ETRY(engine_array(e, px, &v.pixel_job));
+    ETRY(engine_array(e, (size_t)e->max_jobs, &e->jobs_dev));
+    v.jobs = e->jobs_dev;
+    ETRY(engine_array(e, (size_t)e->max_jobs, &v.job_left));
+    ETRY(engine_array(e, (size_t)e->max_jobs, &v.job_max_count));
+    ETRY(engine_array(e, (size_t)e->max_jobs, &v.job_unconverged));
+    ETRY(wf_fill_u32(e->ctx, v.counters, 0u, C_COUNT));
+    ETRY(wf_fill_u32(e->ctx, v.stats, 0u, 2 * S_COUNT));
+    ETRY(wf_fill_u32(e->ctx, v.job_left, 0u, (size_t)e->max_jobs));
+    ETRY(wf_fill_u32(e->ctx, v.pixel_job, 0u, px));
+    ETRY(wf_fill_u32(e->ctx, v.count, 0u, px));
+#undef ETRY
+    e->jobs.assign((size_t)e->max_jobs, JobDev{});
+    e->job_state.assign((size_t)e->max_jobs, 0);
+    e->job_iter_start.assign((size_t)e->max_jobs, 0);
+    e->free_ranges.push_back({0, e->px});
+    hipError_t he = hipHostMalloc((void**)&e->status_host,
+                                  sizeof(int32_t) * 2 * (kStatusHeader + (size_t)e->max_jobs));
+    if (he != hipSuccess) return fail(set_error(WF_ERR_OUT_OF_MEMORY, "engine: pinned status buffer"));
+    for (auto& ev : e->status_ev)
+        if (hipEventCreate(&ev) != hipSuccess) return fail(set_error(WF_ERR_HIP, "engine: event"));
+    for (auto& set : e->phase_ev)
+        for (auto& ev : set)
+            if (hipEventCreate(&ev) != hipSuccess) return fail(set_error(WF_ERR_HIP, "engine: event"));
+    he = hipStreamSynchronize(e->ctx->stream);
+    if (he != hipSuccess) return fail(set_error(WF_ERR_HIP, "engine init: %s", hipGetErrorString(he)));
+    *out = e;
+    return WF_OK;
+}
+
+wf_status wf_engine_destroy(wf_engine* e) {
+    if (!e) return WF_OK;
+    if (e->ctx) (void)hipStreamSynchronize(e->ctx->stream);
+    if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    for (auto& ev : e->status_ev)
+        if (ev) (void)hipEventDestroy(ev);
+    for (auto& set : e->phase_ev)
+        for (auto& ev : set)
+            if (ev) (void)hipEventDestroy(ev);
+    if (e->status_host) (void)hipHostFree(e->status_host);
+    for (void* p : e->allocations) (void)hipFree(p);
+    if (e->ctx) wf_context_destroy(e->ctx);
+    delete e;
+    return WF_OK;
+}
+
+wf_status wf_engine_bytes(int32_t capacity, int64_t pixel_budget, size_t* out) {
+    WF_REQUIRE(out && capacity >= 0 && pixel_budget >= 0, "wf_engine_bytes: bad argument");
+    *out = kBytesPerSlot * (size_t)capacity + kBytesPerPixel * (size_t)pixel_budget;
+    return WF_OK;
+}
+
+wf_status wf_engine_device_bytes(const wf_engine* e, size_t* out) {
+    WF_REQUIRE(e && out, "wf_engine_device_bytes: NULL argument");
+    size_t ws = 0;
+    for (auto& kv : e->ctx->workspace) ws += kv.second.bytes;
+    *out = e->bytes + ws;
+    return WF_OK;
+}
+
+wf_status wf_engine_admit(wf_engine* e, const wf_job_desc* job, int32_t* job_id) {
+    clear_error();
+    WF_REQUIRE(e && job && job_id, "wf_engine_admit: NULL argument");
+    WF_REQUIRE(job->width >= 2 && job->height >= 2, "wf_engine_admit: image must be at least 2x2 (got %dx%d)",
+               job->width, job->height);
+    WF_REQUIRE(job->spp >= 0 && job->spp <= kTagSampleMask, "wf_engine_admit: spp %d out of range", job->spp);
+    WF_REQUIRE(job->max_depth >= 0, "wf_engine_admit: negative max_depth");
+    WF_REQUIRE(!job->adaptive || job->adaptive_interval > 0, "wf_engine_admit: adaptive_interval must be > 0");
+    const int64_t P64 = (int64_t)job->width * job->height;
+    WF_REQUIRE(P64 <= e->px, "wf_engine_admit: %lld pixels exceed the engine's pixel budget %d",
+               (long long)P64, e->px);
+    const int P = (int)P64;
+    int slot = -1;
+    for (int j = 0; j < e->max_jobs; ++j)
+        if (e->job_state[j] == 0) {
+            slot = j;
+            break;
+        }
+    if (slot < 0) return set_error(WF_ERR_ADMISSION, "wf_engine_admit: all %d job slots busy", e->max_jobs);
+    int range = -1;
+    for (size_t k = 0; k < e->free_ranges.size(); ++k)
+        if (e->free_ranges[k].second - e->free_ranges[k].first >= P) {
+            range = (int)k;
+            break;
+        }
+    if (range < 0)
+        return set_error(WF_ERR_ADMISSION, "wf_engine_admit: no free framebuffer range of %d pixels", P);
+    const int base = e->free_ranges[range].first;
+    e->free_ranges[range].first += P;
+    if (e->free_ranges[range].first == e->free_ranges[range].second)
+        e->free_ranges.erase(e->free_ranges.begin() + range);
+    JobDev& J = e->jobs[slot];
+    std::memset(&J, 0, sizeof(J));
+    J.pixel_base = base;
+    J.width = job->width;
+    J.height = job->height;
+    J.spp = job->spp;
+    J.max_depth = job->max_depth;
+    J.seed = job->seed;
+    J.tag = job->job_tag;
+    J.adaptive = job->adaptive ? 1 : 0;
+    J.adaptive_threshold = job->adaptive_threshold;
+    J.adaptive_min_samples = job->adaptive_min_samples;
+    J.adaptive_interval = job->adaptive ? job->adaptive_interval : 1;
+    const CamBasis cb = camera_basis(job->camera, job->width, job->height);
+    std::memcpy(J.cam, cb.v, sizeof(J.cam));
+    wf_context* ctx = e->ctx;
+    WF_HIP(hipMemcpyAsync(e->jobs_dev + slot, &J, sizeof(JobDev), hipMemcpyHostToDevice, ctx->stream));
+    const int queue = (job->spp > 0 && job->max_depth > 0) ? 1 : 0;
+    k_admit<<<grid_for(P, 256), 256, 0, ctx->stream>>>(e->view, slot, base, P, queue);
+    WF_HIP(hipGetLastError());
+    k_admit_commit<<<1, 1, 0, ctx->stream>>>(e->view, slot, P, queue);
+    WF_HIP(hipGetLastError());
+    // the host copy of J must outlive the asynchronous upload
+    WF_HIP(hipStreamSynchronize(ctx->stream));
+    e->job_state[slot] = queue ? 1 : 2;
+    e->job_iter_start[slot] = e->iterations;
+    if (queue) ++e->running;
+    *job_id = slot;
+    return WF_OK;
+}
+
+wf_status wf_engine_run(wf_engine* e, int32_t max_iterations, int32_t* finished,
+                        int32_t finished_cap, int32_t* n_finished) {
+    clear_error();
+    WF_REQUIRE(e && n_finished && (finished || finished_cap == 0), "wf_engine_run: NULL argument");
+    *n_finished = 0;
+    // work the caller queued on the scene's context (uploads, cost tables) comes first
+    {
+        hipEvent_t dep;
+        WF_HIP(hipEventCreateWithFlags(&dep, hipEventDisableTiming));
+        WF_HIP(hipEventRecord(dep, e->scene->ctx->stream));
+        WF_HIP(hipStreamWaitEvent(e->ctx->stream, dep, 0));
+        (void)hipEventDestroy(dep);
+    }
+    copy_profile_flags(e);
+    // already finished jobs (spp 0) are reported at once
+    for (int j = 0; j < e->max_jobs && *n_finished < finished_cap; ++j)
+        if (e->job_state[j] == 2 && e->job_iter_start[j] >= 0) {
+            finished[(*n_finished)++] = j;
+            e->job_iter_start[j] = -1;
+        }
+    if (*n_finished > 0 || e->running == 0) {
+        merge_profile(e);
+        return WF_OK;
+    }
+    // two iterations in flight: enqueue k+1 before reading the status of k
+    int64_t done_iters = 0;
+    int b = 0;
+    WF_TRY(launch_iteration(e, b));
+    WF_TRY(enqueue_status(e, b));
+    for (;;) {
+        const bool more = max_iterations <= 0 || done_iters + 1 < max_iterations;
+        if (more) {
+            WF_TRY(launch_iteration(e, b ^ 1));
+            WF_TRY(enqueue_status(e, b ^ 1));
+        }
+        WF_HIP(hipEventSynchronize(e->status_ev[b]));
+        accumulate_phases(e, b);
+        ++done_iters;
+        const int32_t* st = e->status_host + (size_t)b * (kStatusHeader + e->max_jobs);
+        for (int j = 0; j < e->max_jobs; ++j)
+            if (e->job_state[j] == 1 && st[kStatusHeader + j] == 0) {
+                e->job_state[j] = 2;
+                --e->running;
+                if (*n_finished < finished_cap) {
+                    finished[(*n_finished)++] = j;
+                    e->job_iter_start[j] = -1;
+                }
+            }
+        if (!more || *n_finished > 0 || e->running == 0) {
+            if (more) {
+                WF_HIP(hipEventSynchronize(e->status_ev[b ^ 1]));
+                accumulate_phases(e, b ^ 1);
+            }
+            break;
+        }
+        b ^= 1;
+    }
+    merge_profile(e);
+    return WF_OK;
+}
+
+wf_status wf_engine_result(wf_engine* e, int32_t job_id, float* image_rgb, int32_t* sample_counts,
+                           wf_job_result* result) {
+    clear_error();
+    WF_REQUIRE(e && image_rgb, "wf_engine_result: NULL argument");
+    WF_REQUIRE(job_id >= 0 && job_id < e->max_jobs && e->job_state[job_id] == 2,
+               "wf_engine_result: job %d is not finished", job_id);
+    const JobDev& J = e->jobs[job_id];
+    const int P = J.width * J.height;
+    wf_context* ctx = e->ctx;
+    k_resolve_job<<<grid_for(P, 256), 256, 0, ctx->stream>>>(e->view.accum, e->view.count,
+                                                             J.pixel_base, P, image_rgb,
+                                                             sample_counts);
+    WF_HIP(hipGetLastError());
+    int32_t vals[3] = {0, 0, 0};
+    WF_HIP(hipMemcpyAsync(&vals[0], e->view.job_max_count + job_id, sizeof(int32_t),
+                          hipMemcpyDeviceToHost, ctx->stream));
+    WF_HIP(hipMemcpyAsync(&vals[1], e->view.job_unconverged + job_id, sizeof(int32_t),
+                          hipMemcpyDeviceToHost, ctx->stream));
+    WF_HIP(hipStreamSynchronize(ctx->stream));
+    if (result) {
+        result->width = J.width;
+        result->height = J.height;
+        result->samples_run = vals[0];
+        result->active_pixels_end = J.adaptive ? vals[1] : 0;
+    }
+    // release the slot and its framebuffer range (merged with free neighbours)
+    e->job_state[job_id] = 0;
+    std::pair<int32_t, int32_t> r{J.pixel_base, J.pixel_base + P};
+    e->free_ranges.push_back(r);
+    std::sort(e->free_ranges.begin(), e->free_ranges.end());
+    std::vector<std::pair<int32_t, int32_t>> merged;
+    for (auto& x : e->free_ranges) {
+        if (!merged.empty() && merged.back().second == x.first) merged.back().second = x.second;
+        else merged.push_back(x);
+    }
+    e->free_ranges.swap(merged);
+    return WF_OK;
+}
+
+wf_status wf_engine_query(wf_engine* e, wf_engine_stats* out) {
+    clear_error();
+    WF_REQUIRE(e && out, "wf_engine_query: NULL argument");
+    unsigned long long s[S_COUNT];
+    int32_t live = 0;
+    WF_HIP(hipMemcpyAsync(s, e->view.stats, sizeof(s), hipMemcpyDeviceToHost, e->ctx->stream));
+    WF_HIP(hipMemcpyAsync(&live, e->view.counters + C_LIVE, sizeof(int32_t), hipMemcpyDeviceToHost,
+                          e->ctx->stream));
+    WF_HIP(hipStreamSynchronize(e->ctx->stream));
+    out->rays_traced = (int64_t)s[S_RAYS];
+    out->shadow_rays = (int64_t)s[S_SHADOW];
+    out->shaded_hits = (int64_t)s[S_HITS];
+    out->misses = (int64_t)s[S_MISSES];
+    out->paths = (int64_t)s[S_PATHS];
+    out->iterations = e->iterations;
+    out->live_rays = live;
+    out->queued_pixels = (int64_t)(s[S_TAIL] - s[S_HEAD]);
+    out->running_jobs = e->running;
+    int free_px = 0;
+    for (auto& r : e->free_ranges) free_px += r.second - r.first;
+    out->free_pixels = free_px;
+    for (int k = 0; k < 5; ++k) out->phase_ms[k] = e->phase_ms[k];
+    return WF_OK;
+}
+
+wf_status wf_engine_phase_timing(wf_engine* e, int32_t enable) {
+    WF_REQUIRE(e, "wf_engine_phase_timing: NULL engine");
+    e->phase_timing = enable != 0;
+    for (double& m : e->phase_ms) m = 0.0;
+    return WF_OK;
+}
+
+}  // extern "C"
+
+namespace wf {
+
+// wf_render with WF_SCHEDULE_REGENERATE: one job on an engine cached in the scene.
+wf_status render_regenerate(wf_scene* s, const wf_render_params* rp, float* image_rgb,
+                            int32_t* sample_counts, wf_render_stats* stats) {
+    const int P = rp->width * rp->height;
+    wf_engine* e = s->render_engine;
+    if (e && (e->cap < P || e->px < P)) {
+        wf_engine_destroy(e);
+        e = s->render_engine = nullptr;
+    }
+    wf_engine_config cfg{};
+    cfg.capacity = P;
+    cfg.max_jobs = 1;
+    cfg.pixel_budget = P;
+    cfg.policy = rp->policy;
+    cfg.ray_sort = rp->ray_sort;
+    cfg.use_graph = 0;
+    cfg.shade = rp->shade;
+    if (!e) {
+        WF_TRY(wf_engine_create(s->ctx, s, &cfg, &e));
+        s->render_engine = e;
+    } else {
+        cfg.capacity = e->cap;
+        cfg.pixel_budget = e->px;
+        e->cfg = cfg;
+    }
+    WF_TRY(wf_engine_phase_timing(e, 1));
+    wf_engine_stats st0{}, st1{};
+    WF_TRY(wf_engine_query(e, &st0));
+    hipEvent_t f0, f1;
+    WF_HIP(hipEventCreate(&f0));
+    WF_HIP(hipEventCreate(&f1));
+    WF_HIP(hipEventRecord(f0, e->ctx->stream));
+    wf_job_desc job{};
+    job.width = rp->width;
+    job.height = rp->height;
+    job.spp = rp->spp;
+    job.max_depth = rp->max_depth;
+    job.seed = rp->shade.seed;
+    job.job_tag = rp->shade.job;
+    job.adaptive = rp->adaptive;
+    job.adaptive_threshold = rp->adaptive_threshold;
+    job.adaptive_min_samples = rp->adaptive_min_samples;
+    job.adaptive_interval = rp->adaptive_interval;
+    job.camera = rp->camera;
+    int32_t id = -1, fin = -1, nf = 0;
+    WF_TRY(wf_engine_admit(e, &job, &id));
+    while (nf == 0) WF_TRY(wf_engine_run(e, 0, &fin, 1, &nf));
+    wf_job_result res{};
+    WF_TRY(wf_engine_result(e, id, image_rgb, sample_counts, &res));
+    WF_HIP(hipEventRecord(f1, e->ctx->stream));
+    WF_HIP(hipEventSynchronize(f1));
+    float fms = 0.0f;
+    (void)hipEventElapsedTime(&fms, f0, f1);
+    (void)hipEventDestroy(f0);
+    (void)hipEventDestroy(f1);
+    WF_TRY(wf_engine_query(e, &st1));
+    if (stats) {
+        wf_render_stats r{};
+        r.rays_traced = st1.rays_traced - st0.rays_traced;
+        r.shadow_rays = st1.shadow_rays - st0.shadow_rays;
+        r.shaded_hits = st1.shaded_hits - st0.shaded_hits;
+        r.misses = st1.misses - st0.misses;
+        r.samples_run = res.samples_run;
+        r.active_pixels_end = res.active_pixels_end;
+        r.frame_ms = fms;
+        r.sort_ms = st1.phase_ms[0];
+        r.intersect_ms = st1.phase_ms[1];
+        r.schedule_ms = st1.phase_ms[2];
+        r.shade_ms = st1.phase_ms[3];
+        r.miss_ms = st1.phase_ms[4];
+        *stats = r;
+    }
+    return WF_OK;
+}
+
+}  // namespace wf

@@ -200,6 +200,8 @@ struct wf_scene {
     // host copies needed for table rebuilds
     std::vector<wf_material> materials;
     std::vector<uint64_t> tex_mem;
+    // engine reused by wf_render's regenerating schedule
+    wf_engine* render_engine = nullptr;
 };
 
 namespace wf {
@@ -218,49 +220,107 @@ wf_status scene_upload(wf_scene* s, const T* host, size_t count, T** out) {
     return WF_OK;
 }
 
+// Element count of a stage. With `dev` null, `n` is exact. Otherwise kernels read the count from
+// device memory (*dev <= n, written by an earlier kernel on the same stream) and `n` only sizes
+// the launch grid: no host round trip, so the stage can run inside a captured graph.
+struct Count {
+    int32_t n = 0;
+    const int32_t* dev = nullptr;
+};
+inline Count host_count(int32_t n) { return Count{n, nullptr}; }
+
+// Per-ray job tags of an engine wavefront: tag = job slot << kTagJobShift | sample index.
+constexpr int kTagJobShift = 20;
+constexpr int32_t kTagSampleMask = (1 << kTagJobShift) - 1;
+constexpr int kMaxJobs = 1 << (31 - kTagJobShift);
+
+// Per-job constants of an engine wavefront (device table indexed by job slot).
+struct JobDev {
+    int32_t pixel_base;        // first framebuffer pixel of the job
+    int32_t width, height;
+    int32_t spp;               // sample budget per pixel
+    int32_t max_depth;
+    uint32_t seed;
+    int32_t tag;               // job id in the counter-based RNG (wf_shade_params::job)
+    int32_t adaptive;
+    float adaptive_threshold;
+    int32_t adaptive_min_samples;
+    int32_t adaptive_interval;
+    int32_t pad;
+    float cam[12];             // eye, top-left, horizontal, vertical (Camera.h:13-33)
+};
+
+// Camera basis (Camera.h:13-33): v = eye, top-left corner, horizontal, vertical (12 floats).
+struct CamBasis {
+    float v[12];
+};
+CamBasis camera_basis(const wf_camera& c, int width, int height);
+
 // ---- primitives (wf_sort.hip) ------------------------------------------------------------------
 // Stable LSD radix sort of (key, value) pairs on key bits [0, key_bits). When vals_in is NULL
 // the values start as 0..n-1. The sorted pairs end in keys_out/vals_out (which must not alias
-// the inputs); keys_in is not modified.
+// the inputs); keys_in is not modified. A host count sorts only the bit range where the keys
+// differ (one readback); a device count sorts all key_bits.
 wf_status radix_sort_pairs(wf_context* ctx, const uint64_t* keys_in, const int32_t* vals_in,
-                           uint64_t* keys_out, int32_t* vals_out, int32_t n, int32_t key_bits,
+                           uint64_t* keys_out, int32_t* vals_out, Count n, int32_t key_bits,
                            const char* tag);
 // Exclusive prefix sum of n u32 flags/counts into out (may alias in); total (optional) is
 // written to a device int32.
-wf_status exclusive_scan_u32(wf_context* ctx, const uint32_t* in, uint32_t* out, int32_t n,
+wf_status exclusive_scan_u32(wf_context* ctx, const uint32_t* in, uint32_t* out, Count n,
                              int32_t* total_dev, const char* tag);
 // dst[i] = i
 wf_status iota_i32(wf_context* ctx, int32_t* dst, int32_t n);
+wf_status iota_count(wf_context* ctx, int32_t* dst, Count n);
+// *dst = *a - *b (device int32s), one thread.
+wf_status sub_count(wf_context* ctx, int32_t* dst, const int32_t* a, const int32_t* b);
 
-// ---- stage entry points (used by the C-ABI and the render loop) -------------------------------
+// ---- stage entry points (used by the C-ABI, the render loop and the engine) -------------------
+// Every stage runs on ctx's stream with ctx's workspaces; the scene may belong to another
+// context on the same device.
 wf_status scene_ready(const wf_scene* s);
-wf_status trace_closest(const wf_scene* s, const wf_ray_soa* rays, int32_t n, float tnear,
-                        float tfar, int32_t* hit_leaf, float* hit_t, float* hit_bu, float* hit_bv,
-                        uint32_t* flags);
+wf_status trace_closest(wf_context* ctx, const wf_scene* s, const wf_ray_soa* rays, Count n,
+                        float tnear, float tfar, int32_t* hit_leaf, float* hit_t, float* hit_bu,
+                        float* hit_bv, uint32_t* flags);
 wf_status trace_occluded(const wf_scene* s, const float* ox, const float* oy, const float* oz,
                          const float* dx, const float* dy, const float* dz, const float* maxd,
                          int32_t n, uint8_t* out);
-// Closest hit + stable split; counts_dev[0] receives the hit count (device int32).
-wf_status intersect_split(const wf_scene* s, const wf_ray_soa* rays, int32_t n,
-                          wf_shade_soa* hits_out, wf_ray_soa* misses_out, int32_t* counts_dev);
-wf_status ray_schedule(wf_context* ctx, const wf_ray_soa* rays, int32_t n, const float bmin[3],
+// Closest hit + stable split; counts_dev[0] = hits, counts_dev[1] = misses (device int32).
+// tags_in/hit_tags (optional) carry job tags from the rays to the hit queue.
+wf_status intersect_split(wf_context* ctx, const wf_scene* s, const wf_ray_soa* rays, Count n,
+                          wf_shade_soa* hits_out, wf_ray_soa* misses_out, int32_t* counts_dev,
+                          const int32_t* tags_in = nullptr, int32_t* hit_tags = nullptr);
+wf_status ray_schedule(wf_context* ctx, const wf_ray_soa* rays, Count n, const float bmin[3],
                        const float bmax[3], int32_t* sorted, uint64_t* keys_out);
-wf_status shading_schedule(const wf_scene* s, const wf_shade_soa* q, int32_t n, int32_t policy,
-                           int32_t* sorted);
+wf_status shading_schedule(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q, Count n,
+                           int32_t policy, int32_t* sorted);
+
+// Job context of a tagged wavefront for the shading kernel: per-entry tags plus the job table.
+struct ShadeJobs {
+    const int32_t* tags = nullptr;     // per shading-queue entry
+    const JobDev* jobs = nullptr;
+    int32_t* next_tags = nullptr;      // tags of the compacted next rays
+};
 // One bounce of shadeAll: emission into accum, shadow rays resolved, next rays compacted into
-// `next`; counts_dev[0] = next rays, counts_dev[1] = shadow rays traced (device int32).
-wf_status shade_bounce(const wf_scene* s, const wf_shade_soa* q, int32_t n,
+// `next`; counts_dev[0] = next rays, counts_dev[1] = shadow rays traced, counts_dev[2] = shadow
+// entries (device int32). With jobs.tags, RNG sample/job/seed and the depth limit come from each
+// entry's job; *alive_out (optional) receives the per-entry "path continues" flags.
+wf_status shade_bounce(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q, Count n,
                        const wf_shade_params* p, float* accum, wf_ray_soa* next,
-                       int32_t* counts_dev);
-wf_status shade_misses(const wf_scene* s, const wf_ray_soa* q, int32_t n, int32_t physics,
-                       float* accum);
-wf_status ray_keys(wf_context* ctx, const wf_ray_soa* rays, int32_t n, const float bmin[3],
+                       int32_t* counts_dev, const ShadeJobs& jobs = ShadeJobs{},
+                       const uint32_t** alive_out = nullptr);
+wf_status shade_misses(wf_context* ctx, const wf_scene* s, const wf_ray_soa* q, Count n,
+                       int32_t physics, float* accum);
+wf_status render_regenerate(wf_scene* s, const wf_render_params* rp, float* image_rgb,
+                            int32_t* sample_counts, wf_render_stats* stats);
+wf_status ray_keys(wf_context* ctx, const wf_ray_soa* rays, Count n, const float bmin[3],
                    const float bmax[3], uint64_t* keys);
-wf_status shading_keys(const wf_scene* s, const wf_shade_soa* q, int32_t n, int32_t policy,
-                       uint64_t* keys, int32_t* key_bits);
-wf_status gather_rays(wf_context* ctx, const wf_ray_soa* src, const int32_t* idx, int32_t n,
-                      wf_ray_soa* dst);
-wf_status gather_shading(wf_context* ctx, const wf_shade_soa* src, const int32_t* idx, int32_t n,
-                         wf_shade_soa* dst);
+wf_status shading_keys(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q, Count n,
+                       int32_t policy, uint64_t* keys, int32_t* key_bits);
+wf_status gather_rays(wf_context* ctx, const wf_ray_soa* src, const int32_t* idx, Count n,
+                      wf_ray_soa* dst, const int32_t* tags_in = nullptr,
+                      int32_t* tags_out = nullptr);
+wf_status gather_shading(wf_context* ctx, const wf_shade_soa* src, const int32_t* idx, Count n,
+                         wf_shade_soa* dst, const int32_t* tags_in = nullptr,
+                         int32_t* tags_out = nullptr);
 
 }  // namespace wf

@@ -38,25 +38,15 @@ wf_status carve_shade_soa(wf_context* ctx, const char* name, size_t cap, wf_shad
     return WF_OK;
 }
 
-struct CamBasis {
-    float ox, oy, oz, tlx, tly, tlz, hx, hy, hz, vx, vy, vz;
-};
-
 // generatePrimaryRays (cpp:401-457) with the Camera basis (Camera.h:13-40) computed on the host.
 __global__ void k_raygen(CamBasis c, int width, int height, const int32_t* __restrict__ pixels,
                          int n, uint32_t seed, int job, int sample, wf_ray_soa r) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int p = pixels ? pixels[i] : i;
-    const int x = p % width, y = p / width;
     PathRng rng{seed, (uint32_t)job, (uint32_t)p, (uint32_t)sample, kRaygenDepth};
-    const float jx = rng(0);
-    const float jy = rng(1);
-    const float u = ((float)x + jx) / (float)(width - 1);
-    const float v = 1.0f - ((float)y + jy) / (float)(height - 1);
-    const V3 O(c.ox, c.oy, c.oz), TL(c.tlx, c.tly, c.tlz), H(c.hx, c.hy, c.hz), Vv(c.vx, c.vy, c.vz);
-    const V3 d = (TL + H * u + Vv * v - O).normalized();
-    st3(r.ox, r.oy, r.oz, i, O);
+    const V3 d = primary_dir(c.v, p % width, p / width, width, height, rng(0), rng(1));
+    st3(r.ox, r.oy, r.oz, i, V3(c.v[0], c.v[1], c.v[2]));
     st3(r.dx, r.dy, r.dz, i, d);
     r.tr[i] = 1.0f;
     r.tg[i] = 1.0f;
@@ -122,32 +112,6 @@ __global__ void k_resolve(const float* __restrict__ accum, const int32_t* __rest
 }
 
 
-// Camera constructor (include/core/Camera.h:13-33) in host float arithmetic.
-static CamBasis camera_basis(const wf_camera& c, int width, int height) {
-    struct H {
-        float x, y, z;
-        H operator+(const H& o) const { return {x + o.x, y + o.y, z + o.z}; }
-        H operator-(const H& o) const { return {x - o.x, y - o.y, z - o.z}; }
-        H operator*(float s) const { return {x * s, y * s, z * s}; }
-        H operator/(float s) const { return {x / s, y / s, z / s}; }
-        H cross(const H& o) const { return {y * o.z - z * o.y, z * o.x - x * o.z, x * o.y - y * o.x}; }
-        H normalized() const { return *this / std::sqrt(x * x + y * y + z * z); }
-    };
-    const H from{c.eye[0], c.eye[1], c.eye[2]}, at{c.target[0], c.target[1], c.target[2]};
-    const H up{c.up[0], c.up[1], c.up[2]};
-    const float aspect = static_cast<float>(width) / height;
-    const float theta = c.vfov_deg * 3.14159265f / 180.0f;
-    const float vh = 2.0f * std::tan(theta / 2.0f);
-    const float vw = aspect * vh;
-    const H fwd = (from - at).normalized();
-    const H right = up.cross(fwd).normalized();
-    const H upv = fwd.cross(right);
-    const H horizontal = right * vw, vertical = upv * vh;
-    const H tl = from - horizontal / 2.0f - vertical / 2.0f - fwd;
-    return CamBasis{from.x, from.y, from.z, tl.x, tl.y, tl.z, horizontal.x, horizontal.y,
-                    horizontal.z, vertical.x, vertical.y, vertical.z};
-}
-
 __global__ void k_copy_f32(const float* __restrict__ a, float* __restrict__ b, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) b[i] = a[i];
@@ -210,7 +174,11 @@ extern "C" wf_status wf_render(wf_scene* s, const wf_render_params* rp, float* i
     WF_REQUIRE(rp->policy >= WF_POLICY_NONE && rp->policy <= WF_POLICY_COST_BENEFIT,
                "wf_render: unknown policy %d", rp->policy);
     WF_REQUIRE(!rp->adaptive || rp->adaptive_interval > 0, "wf_render: adaptive_interval must be > 0");
+    WF_REQUIRE(rp->schedule == WF_SCHEDULE_REGENERATE || rp->schedule == WF_SCHEDULE_PASSES,
+               "wf_render: unknown schedule %d", rp->schedule);
     WF_TRY(scene_ready(s));
+    WF_REQUIRE((int64_t)rp->width * rp->height < (1ll << 30), "wf_render: image too large");
+    if (rp->schedule == WF_SCHEDULE_REGENERATE) return render_regenerate(s, rp, image_rgb, sample_counts, stats);
     wf_context* ctx = s->ctx;
     const int W = rp->width, H = rp->height;
     const int64_t P64 = (int64_t)W * H;
@@ -259,7 +227,7 @@ extern "C" wf_status wf_render(wf_scene* s, const wf_render_params* rp, float* i
         if (rp->adaptive) {
             k_copy_f32<<<grid_for(P64 * 3, 256), 256, 0, ctx->stream>>>(accum, (float*)pbefore, cap * 3);
             WF_HIP(hipGetLastError());
-            WF_TRY(exclusive_scan_u32(ctx, (uint32_t*)pactive, (uint32_t*)pprefix, P, counts + 8,
+            WF_TRY(exclusive_scan_u32(ctx, (uint32_t*)pactive, (uint32_t*)pprefix, host_count(P), counts + 8,
                                       "compact"));
             k_active_list<<<grid_for(P, 256), 256, 0, ctx->stream>>>(
                 (uint32_t*)pactive, (uint32_t*)pprefix, P, (int32_t*)plist);
@@ -281,14 +249,14 @@ extern "C" wf_status wf_render(wf_scene* s, const wf_render_params* rp, float* i
             st.rays_traced += n;
             if (rp->ray_sort) {
                 pt.begin(0);
-                WF_TRY(ray_schedule(ctx, cur, n, s->dev.bounds_min, s->dev.bounds_max,
+                WF_TRY(ray_schedule(ctx, cur, host_count(n), s->dev.bounds_min, s->dev.bounds_max,
                                     (int32_t*)pidx, nullptr));
-                WF_TRY(gather_rays(ctx, cur, (int32_t*)pidx, n, alt));
+                WF_TRY(gather_rays(ctx, cur, (int32_t*)pidx, host_count(n), alt));
                 std::swap(cur, alt);
                 pt.end();
             }
             pt.begin(1);
-            WF_TRY(intersect_split(s, cur, n, &shA, &miss, counts));
+            WF_TRY(intersect_split(ctx, s, cur, host_count(n), &shA, &miss, counts));
             pt.end();
             int32_t nh = 0;
             WF_TRY(readback(ctx, &nh, counts, sizeof(int32_t)));
@@ -298,16 +266,16 @@ extern "C" wf_status wf_render(wf_scene* s, const wf_render_params* rp, float* i
             wf_shade_soa* q = &shA;
             if (rp->policy != WF_POLICY_NONE && nh > 0) {
                 pt.begin(2);
-                WF_TRY(shading_schedule(s, &shA, nh, rp->policy, (int32_t*)pidx));
-                WF_TRY(gather_shading(ctx, &shA, (int32_t*)pidx, nh, &shB));
+                WF_TRY(shading_schedule(ctx, s, &shA, host_count(nh), rp->policy, (int32_t*)pidx));
+                WF_TRY(gather_shading(ctx, &shA, (int32_t*)pidx, host_count(nh), &shB));
                 q = &shB;
                 pt.end();
             }
             pt.begin(3);
-            WF_TRY(shade_bounce(s, q, nh, &sp, accum, alt, counts + 4));
+            WF_TRY(shade_bounce(ctx, s, q, host_count(nh), &sp, accum, alt, counts + 4));
             pt.end();
             pt.begin(4);
-            WF_TRY(shade_misses(s, &miss, nm, rp->shade.physics, accum));
+            WF_TRY(shade_misses(ctx, s, &miss, host_count(nm), rp->shade.physics, accum));
             pt.end();
             int32_t c[2] = {0, 0};
             WF_TRY(readback(ctx, c, counts + 4, sizeof(c)));

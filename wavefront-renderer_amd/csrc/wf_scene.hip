@@ -677,6 +677,7 @@ wf_status wf_scene_destroy(wf_scene* s) {
     // the owning context may already be gone: synchronise the device, not the context stream
     (void)hipSetDevice(s->device);
     (void)hipDeviceSynchronize();
+    if (s->render_engine) wf_engine_destroy(s->render_engine);
     for (void* p : s->allocations) (void)hipFree(p);
     delete s;
     return WF_OK;
@@ -724,7 +725,7 @@ wf_status wf_accel_build(wf_scene* s, wf_build_stats* stats) {
                                                                  (uint32_t*)pcb, (uint64_t*)pkeys);
         WF_HIP(hipGetLastError());
         WF_TRY(radix_sort_pairs(ctx, (uint64_t*)pkeys, nullptr, (uint64_t*)pkeys2,
-                                (int32_t*)porder, n, 48, "bvh_sort"));
+                                (int32_t*)porder, host_count(n), 48, "bvh_sort"));
         if (!d.tri) {
             void* pt = nullptr;
             WF_TRY(scene_alloc(s, sizeof(float4) * 3 * (size_t)n, &pt));
@@ -782,7 +783,7 @@ wf_status wf_accel_build(wf_scene* s, wf_build_stats* stats) {
             k_bvh_node_depth<<<grid_for(ni, 256), 256, 0, ctx->stream>>>(
                 ni, (int32_t*)pip, (int32_t*)pnd, (uint32_t*)pev);
             WF_HIP(hipGetLastError());
-            WF_TRY(exclusive_scan_u32(ctx, (uint32_t*)pev, (uint32_t*)pmap, ni, (int32_t*)pcnt,
+            WF_TRY(exclusive_scan_u32(ctx, (uint32_t*)pev, (uint32_t*)pmap, host_count(ni), (int32_t*)pcnt,
                                       "bvh_scan"));
             k_bvh4_collapse<<<grid_for(ni, 256), 256, 0, ctx->stream>>>(
                 ni, (int32_t*)pl, (int32_t*)pr, (uint32_t*)pev, (uint32_t*)pmap, (float4*)pslo,

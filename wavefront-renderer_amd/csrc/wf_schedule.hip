@@ -27,11 +27,11 @@ __device__ __forceinline__ float clamp01(float v) { return v < 0.0f ? 0.0f : (1.
 
 __global__ void k_ray_keys(const float* __restrict__ ox, const float* __restrict__ oy,
                            const float* __restrict__ oz, const float* __restrict__ dx,
-                           const float* __restrict__ dy, const float* __restrict__ dz, int n,
-                           float minx, float miny, float minz, float ex, float ey, float ez,
-                           uint64_t* __restrict__ keys) {
+                           const float* __restrict__ dy, const float* __restrict__ dz, int n_host,
+                           const int32_t* __restrict__ nd, float minx, float miny, float minz,
+                           float ex, float ey, float ez, uint64_t* __restrict__ keys) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= load_count(n_host, nd)) return;
     const uint64_t octant = (dx[i] < 0.0f ? 1u : 0u) | (dy[i] < 0.0f ? 2u : 0u) |
                             (dz[i] < 0.0f ? 4u : 0u);
     const uint32_t gx = (uint32_t)(clamp01((ox[i] - minx) / ex) * 1023.0f);
@@ -41,27 +41,30 @@ __global__ void k_ray_keys(const float* __restrict__ ox, const float* __restrict
     keys[i] = (octant << 30) | (uint64_t)morton;
 }
 
-wf_status ray_keys(wf_context* ctx, const wf_ray_soa* r, int32_t n, const float bmin[3],
+wf_status ray_keys(wf_context* ctx, const wf_ray_soa* r, Count n, const float bmin[3],
                    const float bmax[3], uint64_t* keys) {
-    if (n <= 0) return WF_OK;
+    if (n.n <= 0) return WF_OK;
     // extent = max(max - min, 1e-6) per axis (RayQueue.h:143-145), host float math
     auto ext = [](float hi, float lo) {
         float e = hi - lo;
         return e < 1e-6f ? 1e-6f : e;
     };
-    Timed tm(ctx, "ray_keys", 32.0 * n);
-    k_ray_keys<<<grid_for(n, 256), 256, 0, ctx->stream>>>(
-        r->ox, r->oy, r->oz, r->dx, r->dy, r->dz, n, bmin[0], bmin[1], bmin[2],
+    Timed tm(ctx, "ray_keys", 32.0 * n.n);
+    k_ray_keys<<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(
+        r->ox, r->oy, r->oz, r->dx, r->dy, r->dz, n.n, n.dev, bmin[0], bmin[1], bmin[2],
         ext(bmax[0], bmin[0]), ext(bmax[1], bmin[1]), ext(bmax[2], bmin[2]), keys);
     WF_HIP(hipGetLastError());
     return WF_OK;
 }
 
 // ---- materialize ---------------------------------------------------------------------------------
-__global__ void k_gather_rays(wf_ray_soa s, const int32_t* __restrict__ idx, int n, wf_ray_soa d) {
+__global__ void k_gather_rays(wf_ray_soa s, const int32_t* __restrict__ idx, int n_host,
+                              const int32_t* __restrict__ nd, wf_ray_soa d,
+                              const int32_t* __restrict__ tags_in, int32_t* __restrict__ tags_out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= load_count(n_host, nd)) return;
     const int j = idx[i];
+    if (tags_out) tags_out[i] = tags_in[j];
     d.ox[i] = s.ox[j];
     d.oy[i] = s.oy[j];
     d.oz[i] = s.oz[j];
@@ -76,11 +79,14 @@ __global__ void k_gather_rays(wf_ray_soa s, const int32_t* __restrict__ idx, int
     d.pdf[i] = s.pdf[j];
 }
 
-__global__ void k_gather_shade(wf_shade_soa s, const int32_t* __restrict__ idx, int n,
-                               wf_shade_soa d) {
+__global__ void k_gather_shade(wf_shade_soa s, const int32_t* __restrict__ idx, int n_host,
+                               const int32_t* __restrict__ nd, wf_shade_soa d,
+                               const int32_t* __restrict__ tags_in,
+                               int32_t* __restrict__ tags_out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= load_count(n_host, nd)) return;
     const int j = idx[i];
+    if (tags_out) tags_out[i] = tags_in[j];
     d.px[i] = s.px[j];
     d.py[i] = s.py[j];
     d.pz[i] = s.pz[j];
@@ -109,20 +115,22 @@ __global__ void k_gather_shade(wf_shade_soa s, const int32_t* __restrict__ idx, 
     if (d.front) d.front[i] = s.front ? s.front[j] : 1;
 }
 
-wf_status gather_rays(wf_context* ctx, const wf_ray_soa* src, const int32_t* idx, int32_t n,
-                      wf_ray_soa* dst) {
-    if (n <= 0) return WF_OK;
-    Timed tm(ctx, "gather_rays", 100.0 * n);
-    k_gather_rays<<<grid_for(n, 256), 256, 0, ctx->stream>>>(*src, idx, n, *dst);
+wf_status gather_rays(wf_context* ctx, const wf_ray_soa* src, const int32_t* idx, Count n,
+                      wf_ray_soa* dst, const int32_t* tags_in, int32_t* tags_out) {
+    if (n.n <= 0) return WF_OK;
+    Timed tm(ctx, "gather_rays", (tags_out ? 108.0 : 100.0) * n.n);
+    k_gather_rays<<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(*src, idx, n.n, n.dev, *dst,
+                                                              tags_in, tags_out);
     WF_HIP(hipGetLastError());
     return WF_OK;
 }
 
-wf_status gather_shading(wf_context* ctx, const wf_shade_soa* src, const int32_t* idx, int32_t n,
-                         wf_shade_soa* dst) {
-    if (n <= 0) return WF_OK;
-    Timed tm(ctx, "gather_shading", 204.0 * n);
-    k_gather_shade<<<grid_for(n, 256), 256, 0, ctx->stream>>>(*src, idx, n, *dst);
+wf_status gather_shading(wf_context* ctx, const wf_shade_soa* src, const int32_t* idx, Count n,
+                         wf_shade_soa* dst, const int32_t* tags_in, int32_t* tags_out) {
+    if (n.n <= 0) return WF_OK;
+    Timed tm(ctx, "gather_shading", (tags_out ? 212.0 : 204.0) * n.n);
+    k_gather_shade<<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(*src, idx, n.n, n.dev, *dst,
+                                                               tags_in, tags_out);
     WF_HIP(hipGetLastError());
     return WF_OK;
 }
@@ -135,9 +143,11 @@ struct TypeStats {
 };
 
 __global__ void k_type_stats(const int32_t* __restrict__ mat, const int32_t* __restrict__ tex,
-                             int n, const DevMaterial* __restrict__ mats,
+                             int n_host, const int32_t* __restrict__ nd,
+                             const DevMaterial* __restrict__ mats,
                              const uint64_t* __restrict__ tex_mem, int n_tex,
                              TypeStats* __restrict__ st) {
+    const int n = load_count(n_host, nd);
     __shared__ uint32_t cnt[8];
     __shared__ unsigned long long bytes[8];
     if (threadIdx.x < 8) {
@@ -187,14 +197,15 @@ __global__ void k_type_rank(int policy, TypeStats* st) {
 }
 
 __global__ void k_policy_keys(int policy, const int32_t* __restrict__ mat,
-                              const int32_t* __restrict__ tex, int n,
+                              const int32_t* __restrict__ tex, int n_host,
+                              const int32_t* __restrict__ nd,
                               const DevMaterial* __restrict__ mats,
                               const TypeStats* __restrict__ st,
                               const int32_t* __restrict__ cost_rank, int cost_bits,
                               const int32_t* __restrict__ size_rank, int size_bits, int n_tex,
                               uint64_t* __restrict__ keys) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= load_count(n_host, nd)) return;
     const int m = mat[i];
     const uint64_t tr = (uint64_t)st->rank[mats[m].type & 7];
     uint64_t sub;
@@ -229,8 +240,10 @@ __device__ __forceinline__ float o2f(uint32_t u) {
 __global__ __launch_bounds__(kCbBlock) void k_cb_partials(
     const int32_t* __restrict__ mat, const float* __restrict__ tr, const float* __restrict__ tg,
     const float* __restrict__ tb, const float* __restrict__ px, const float* __restrict__ py,
-    const float* __restrict__ pz, int n, int n_mat, double* __restrict__ partials,
-    uint32_t* __restrict__ present, uint32_t* __restrict__ bounds) {
+    const float* __restrict__ pz, int n_host, const int32_t* __restrict__ nd, int n_mat,
+    double* __restrict__ partials, uint32_t* __restrict__ present,
+    uint32_t* __restrict__ bounds) {
+    const int n = load_count(n_host, nd);
     __shared__ double wsum[kCbBlock / 64][kCbMaxMaterials];
     __shared__ double vals[kCbBlock];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -327,12 +340,12 @@ __device__ __forceinline__ uint32_t quantise_hit(float v, float lo, float hi) {
 }
 
 __global__ void k_cb_keys(const int32_t* __restrict__ mat, const float* __restrict__ px,
-                          const float* __restrict__ py, const float* __restrict__ pz, int n,
-                          const DevMaterial* __restrict__ mats, const int32_t* __restrict__ rank,
-                          int rank_bits, const uint32_t* __restrict__ bounds,
-                          uint64_t* __restrict__ keys) {
+                          const float* __restrict__ py, const float* __restrict__ pz, int n_host,
+                          const int32_t* __restrict__ nd, const DevMaterial* __restrict__ mats,
+                          const int32_t* __restrict__ rank, int rank_bits,
+                          const uint32_t* __restrict__ bounds, uint64_t* __restrict__ keys) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= load_count(n_host, nd)) return;
     const float lx = o2f(bounds[0]), ly = o2f(bounds[1]), lz = o2f(bounds[2]);
     const float hx = o2f(bounds[3]), hy = o2f(bounds[4]), hz = o2f(bounds[5]);
     const uint32_t morton = spread_bits10(quantise_hit(px[i], lx, hx)) |
@@ -349,9 +362,8 @@ static int bits_for(int max_value) {
     return b;
 }
 
-wf_status shading_keys(const wf_scene* s, const wf_shade_soa* q, int32_t n, int32_t policy,
-                       uint64_t* keys, int32_t* key_bits) {
-    wf_context* ctx = s->ctx;
+wf_status shading_keys(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q, Count n,
+                       int32_t policy, uint64_t* keys, int32_t* key_bits) {
     const SceneDev& d = s->dev;
     if (policy == WF_POLICY_MATERIAL || policy == WF_POLICY_TEXTURE) {
         void* p = nullptr;
@@ -359,16 +371,17 @@ wf_status shading_keys(const wf_scene* s, const wf_shade_soa* q, int32_t n, int3
         TypeStats* st = (TypeStats*)p;
         WF_TRY(wf_fill_u32(ctx, st, 0u, sizeof(TypeStats) / 4));
         {
-            Timed tm(ctx, "policy_keys", (policy == WF_POLICY_MATERIAL ? 4.0 : 8.0) * n);
-            int blocks = (int)std::min<int64_t>(grid_for(n, 256), 1024);
-            k_type_stats<<<blocks, 256, 0, ctx->stream>>>(q->mat, q->tex, n, d.materials,
-                                                          d.tex_mem_bytes, d.n_textures, st);
+            Timed tm(ctx, "policy_keys", (policy == WF_POLICY_MATERIAL ? 4.0 : 8.0) * n.n);
+            int blocks = (int)std::min<int64_t>(grid_for(n.n, 256), 1024);
+            k_type_stats<<<blocks, 256, 0, ctx->stream>>>(q->mat, q->tex, n.n, n.dev,
+                                                          d.materials, d.tex_mem_bytes,
+                                                          d.n_textures, st);
             WF_HIP(hipGetLastError());
             k_type_rank<<<1, 1, 0, ctx->stream>>>(policy, st);
             WF_HIP(hipGetLastError());
-            k_policy_keys<<<grid_for(n, 256), 256, 0, ctx->stream>>>(
-                policy, q->mat, q->tex, n, d.materials, st, d.mat_cost_rank, d.cost_rank_bits,
-                d.tex_size_rank, d.size_rank_bits, d.n_textures, keys);
+            k_policy_keys<<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(
+                policy, q->mat, q->tex, n.n, n.dev, d.materials, st, d.mat_cost_rank,
+                d.cost_rank_bits, d.tex_size_rank, d.size_rank_bits, d.n_textures, keys);
             WF_HIP(hipGetLastError());
         }
         *key_bits = 3 + (policy == WF_POLICY_MATERIAL ? d.cost_rank_bits : d.size_rank_bits);
@@ -380,7 +393,7 @@ wf_status shading_keys(const wf_scene* s, const wf_shade_soa* q, int32_t n, int3
             return set_error(WF_ERR_UNSUPPORTED,
                              "costBenefit policy supports at most %d materials (scene has %d)",
                              kCbMaxMaterials, nm);
-        const int tiles = (n + kCbTile - 1) / kCbTile;
+        const int tiles = (n.n + kCbTile - 1) / kCbTile;
         void *pp, *ppres, *pb, *pprio, *prank;
         WF_TRY(workspace(ctx, "cb_partials", sizeof(double) * (size_t)tiles * nm, &pp));
         WF_TRY(workspace(ctx, "cb_present", sizeof(uint32_t) * nm, &ppres));
@@ -390,10 +403,11 @@ wf_status shading_keys(const wf_scene* s, const wf_shade_soa* q, int32_t n, int3
         WF_TRY(wf_fill_u32(ctx, ppres, 0u, (size_t)nm));
         WF_TRY(wf_fill_u32(ctx, pb, 0xFFFFFFFFu, 3));
         WF_TRY(wf_fill_u32(ctx, (uint32_t*)pb + 3, 0u, 5));
-        Timed tm(ctx, "policy_keys", 32.0 * n);
+        Timed tm(ctx, "policy_keys", 32.0 * n.n);
         k_cb_partials<<<tiles, kCbBlock, 0, ctx->stream>>>(q->mat, q->tr, q->tg, q->tb, q->px,
-                                                           q->py, q->pz, n, nm, (double*)pp,
-                                                           (uint32_t*)ppres, (uint32_t*)pb);
+                                                           q->py, q->pz, n.n, n.dev, nm,
+                                                           (double*)pp, (uint32_t*)ppres,
+                                                           (uint32_t*)pb);
         WF_HIP(hipGetLastError());
         k_cb_priority<<<grid_for(nm, 256), 256, 0, ctx->stream>>>(
             (double*)pp, tiles, nm, d.materials, d.mat_rel_cost, d.tex_rel_cost, (double*)pprio);
@@ -402,9 +416,10 @@ wf_status shading_keys(const wf_scene* s, const wf_shade_soa* q, int32_t n, int3
                                                               nm, (int32_t*)prank);
         WF_HIP(hipGetLastError());
         const int rb = nm <= 1 ? 0 : bits_for(nm - 1);
-        k_cb_keys<<<grid_for(n, 256), 256, 0, ctx->stream>>>(q->mat, q->px, q->py, q->pz, n,
-                                                             d.materials, (int32_t*)prank, rb,
-                                                             (uint32_t*)pb, keys);
+        k_cb_keys<<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(q->mat, q->px, q->py, q->pz, n.n,
+                                                               n.dev, d.materials,
+                                                               (int32_t*)prank, rb,
+                                                               (uint32_t*)pb, keys);
         WF_HIP(hipGetLastError());
         *key_bits = 3 + rb + 30;
         return WF_OK;
@@ -412,29 +427,28 @@ wf_status shading_keys(const wf_scene* s, const wf_shade_soa* q, int32_t n, int3
     return set_error(WF_ERR_INVALID_ARGUMENT, "unknown scheduling policy %d", policy);
 }
 
-wf_status shading_schedule(const wf_scene* s, const wf_shade_soa* q, int32_t n, int32_t policy,
-                           int32_t* sorted) {
-    wf_context* ctx = s->ctx;
-    if (n <= 0) return WF_OK;
-    if (policy == WF_POLICY_NONE) return iota_i32(ctx, sorted, n);
+wf_status shading_schedule(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q, Count n,
+                           int32_t policy, int32_t* sorted) {
+    if (n.n <= 0) return WF_OK;
+    if (policy == WF_POLICY_NONE) return iota_count(ctx, sorted, n);
     void *pk, *pk2;
-    WF_TRY(workspace(ctx, "sched_keys", sizeof(uint64_t) * n, &pk));
-    WF_TRY(workspace(ctx, "sched_keys2", sizeof(uint64_t) * n, &pk2));
+    WF_TRY(workspace(ctx, "sched_keys", sizeof(uint64_t) * n.n, &pk));
+    WF_TRY(workspace(ctx, "sched_keys2", sizeof(uint64_t) * n.n, &pk2));
     int32_t bits = 0;
-    WF_TRY(shading_keys(s, q, n, policy, (uint64_t*)pk, &bits));
+    WF_TRY(shading_keys(ctx, s, q, n, policy, (uint64_t*)pk, &bits));
     return radix_sort_pairs(ctx, (uint64_t*)pk, nullptr, (uint64_t*)pk2, sorted, n, bits,
                             "shading_sort");
 }
 
-wf_status ray_schedule(wf_context* ctx, const wf_ray_soa* rays, int32_t n, const float bmin[3],
+wf_status ray_schedule(wf_context* ctx, const wf_ray_soa* rays, Count n, const float bmin[3],
                        const float bmax[3], int32_t* sorted, uint64_t* keys_out) {
-    if (n <= 0) return WF_OK;
+    if (n.n <= 0) return WF_OK;
     void *pk, *pk2;
-    WF_TRY(workspace(ctx, "ray_keys", sizeof(uint64_t) * n, &pk));
-    WF_TRY(workspace(ctx, "ray_keys2", sizeof(uint64_t) * n, &pk2));
+    WF_TRY(workspace(ctx, "ray_keys", sizeof(uint64_t) * n.n, &pk));
+    WF_TRY(workspace(ctx, "ray_keys2", sizeof(uint64_t) * n.n, &pk2));
     WF_TRY(ray_keys(ctx, rays, n, bmin, bmax, (uint64_t*)pk));
     if (keys_out)
-        WF_HIP(hipMemcpyAsync(keys_out, pk, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice,
+        WF_HIP(hipMemcpyAsync(keys_out, pk, sizeof(uint64_t) * n.n, hipMemcpyDeviceToDevice,
                               ctx->stream));
     return radix_sort_pairs(ctx, (uint64_t*)pk, nullptr, (uint64_t*)pk2, sorted, n, 33,
                             "ray_sort");
@@ -453,7 +467,8 @@ wf_status wf_ray_schedule(wf_context* ctx, const wf_ray_soa* rays, int32_t n,
     WF_REQUIRE(ctx && rays && bounds_min && bounds_max && sorted_indices,
                "wf_ray_schedule: NULL argument");
     WF_REQUIRE(n >= 0, "wf_ray_schedule: negative ray count");
-    WF_TRY(ray_schedule(ctx, rays, n, bounds_min, bounds_max, sorted_indices, keys_out));
+    WF_TRY(ray_schedule(ctx, rays, host_count(n), bounds_min, bounds_max, sorted_indices,
+                        keys_out));
     WF_HIP(hipStreamSynchronize(ctx->stream));
     return WF_OK;
 }
@@ -463,7 +478,7 @@ wf_status wf_materialize_rays(wf_context* ctx, const wf_ray_soa* src, const int3
     clear_error();
     WF_REQUIRE(ctx && src && idx && dst, "wf_materialize_rays: NULL argument");
     WF_REQUIRE(n >= 0, "wf_materialize_rays: negative count");
-    WF_TRY(gather_rays(ctx, src, idx, n, dst));
+    WF_TRY(gather_rays(ctx, src, idx, host_count(n), dst));
     WF_HIP(hipStreamSynchronize(ctx->stream));
     return WF_OK;
 }
@@ -475,7 +490,7 @@ wf_status wf_shading_schedule(const wf_scene* s, const wf_shade_soa* q, int32_t 
     WF_REQUIRE(n >= 0, "wf_shading_schedule: negative count");
     WF_REQUIRE(policy >= WF_POLICY_NONE && policy <= WF_POLICY_COST_BENEFIT,
                "wf_shading_schedule: unknown policy %d", policy);
-    WF_TRY(shading_schedule(s, q, n, policy, sorted_indices));
+    WF_TRY(shading_schedule(s->ctx, s, q, host_count(n), policy, sorted_indices));
     WF_HIP(hipStreamSynchronize(s->ctx->stream));
     return WF_OK;
 }
@@ -485,7 +500,7 @@ wf_status wf_materialize_shading(wf_context* ctx, const wf_shade_soa* src, const
     clear_error();
     WF_REQUIRE(ctx && src && idx && dst, "wf_materialize_shading: NULL argument");
     WF_REQUIRE(n >= 0, "wf_materialize_shading: negative count");
-    WF_TRY(gather_shading(ctx, src, idx, n, dst));
+    WF_TRY(gather_shading(ctx, src, idx, host_count(n), dst));
     WF_HIP(hipStreamSynchronize(ctx->stream));
     return WF_OK;
 }

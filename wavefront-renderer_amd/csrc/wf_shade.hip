@@ -322,11 +322,15 @@ __device__ __forceinline__ float env_pdf_sa(const SceneDev& s, int texel_i, floa
     return p / (2.0f * kPi * kPi * lc);
 }
 
-__global__ void k_shade(const SceneDev s, wf_shade_soa q, int n, ShadeArgs a,
+// With tags, each entry's job supplies the RNG stream (seed, job id, job-local pixel, sample) and
+// the depth limit; otherwise they come from ShadeArgs and the caller bounds the depth.
+__global__ void k_shade(const SceneDev s, wf_shade_soa q, int n_host,
+                        const int32_t* __restrict__ nd, ShadeArgs a,
                         const double* __restrict__ mat_rel_cost, float* __restrict__ accum,
-                        ShadeOut o) {
+                        ShadeOut o, const int32_t* __restrict__ tags,
+                        const JobDev* __restrict__ jobs) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= load_count(n_host, nd)) return;
     const int32_t pixel = q.pixel[i];
     const int32_t depth = q.depth[i];
     const float pdf_in = q.pdf[i];
@@ -340,7 +344,19 @@ __global__ void k_shade(const SceneDev s, wf_shade_soa q, int n, ShadeArgs a,
     const bool front = q.front ? q.front[i] != 0 : true;
     const DevMaterial m = s.materials[mat_id];
     const bool unbiased = a.physics == WF_PHYSICS_UNBIASED;
-    PathRng rng{a.seed, (uint32_t)a.job, (uint32_t)pixel, (uint32_t)a.sample, (uint32_t)depth};
+    uint32_t seed = a.seed, job = (uint32_t)a.job, sample = (uint32_t)a.sample;
+    int32_t rng_pixel = pixel;
+    int32_t depth_limit = 0x7FFFFFFF;
+    if (tags) {
+        const int32_t tg = tags[i];
+        const JobDev& J = jobs[tg >> kTagJobShift];
+        seed = J.seed;
+        job = (uint32_t)J.tag;
+        sample = (uint32_t)(tg & kTagSampleMask);
+        rng_pixel = pixel - J.pixel_base;
+        depth_limit = J.max_depth;
+    }
+    PathRng rng{seed, job, (uint32_t)rng_pixel, sample, (uint32_t)depth};
     float* acc = accum + (size_t)pixel * 3;
     uint32_t sflag = 0;
 
@@ -532,6 +548,7 @@ __global__ void k_shade(const SceneDev s, wf_shade_soa q, int n, ShadeArgs a,
         else
             nt = nt / surv;
     }
+    if (depth + 1 >= depth_limit) alive = false;
     o.alive[i] = alive ? 1u : 0u;
     if (alive) {
         st3(o.ndx, o.ndy, o.ndz, i, dir);
@@ -542,22 +559,26 @@ __global__ void k_shade(const SceneDev s, wf_shade_soa q, int n, ShadeArgs a,
     }
 }
 
-__global__ void k_flags_nonzero(const uint32_t* __restrict__ in, int n, uint32_t* __restrict__ out) {
+__global__ void k_flags_nonzero(const uint32_t* __restrict__ in, int n_host,
+                                const int32_t* __restrict__ nd, uint32_t* __restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = in[i] != 0u ? 1u : 0u;
+    if (i < load_count(n_host, nd)) out[i] = in[i] != 0u ? 1u : 0u;
 }
 
 __global__ void k_compact_index(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ prefix,
-                                int n, int32_t* __restrict__ list) {
+                                int n_host, const int32_t* __restrict__ nd,
+                                int32_t* __restrict__ list) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && flag[i]) list[prefix[i]] = i;
+    if (i < load_count(n_host, nd) && flag[i]) list[prefix[i]] = i;
 }
 
-__global__ void k_next_rays(wf_shade_soa q, ShadeOut o, const uint32_t* __restrict__ prefix, int n,
-                            wf_ray_soa next) {
+__global__ void k_next_rays(wf_shade_soa q, ShadeOut o, const uint32_t* __restrict__ prefix,
+                            int n_host, const int32_t* __restrict__ nd, wf_ray_soa next,
+                            const int32_t* __restrict__ tags_in, int32_t* __restrict__ tags_out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !o.alive[i]) return;
+    if (i >= load_count(n_host, nd) || !o.alive[i]) return;
     const int k = (int)prefix[i];
+    if (tags_out) tags_out[k] = tags_in[i];
     next.ox[k] = q.px[i];
     next.oy[k] = q.py[i];
     next.oz[k] = q.pz[i];
@@ -574,9 +595,10 @@ __global__ void k_next_rays(wf_shade_soa q, ShadeOut o, const uint32_t* __restri
 
 // Miss loop (cpp:256-265): accum += T * getSkyColor(dir); the unbiased track weights environment
 // hits of non-delta bounces by the power heuristic against environment NEE.
-__global__ void k_miss(const SceneDev s, wf_ray_soa q, int n, int physics, float* __restrict__ accum) {
+__global__ void k_miss(const SceneDev s, wf_ray_soa q, int n_host, const int32_t* __restrict__ nd,
+                       int physics, float* __restrict__ accum) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= load_count(n_host, nd)) return;
     const V3 d(q.dx[i], q.dy[i], q.dz[i]);
     const V3 T(q.tr[i], q.tg[i], q.tb[i]);
     V3 c;
@@ -602,11 +624,12 @@ __global__ void k_miss(const SceneDev s, wf_ray_soa q, int n, int physics, float
 // the reference order: area/environment light first, directional light second (cpp:678, 720).
 template <int STACK>
 __global__ __launch_bounds__(kTraceBlock) void k_shadow(const SceneDev s, wf_shade_soa q, ShadeOut o,
-                                                        const int32_t* __restrict__ list, int count,
+                                                        const int32_t* __restrict__ list, int count_host,
+                                                        const int32_t* __restrict__ count_dev,
                                                         float* __restrict__ accum) {
     __shared__ int32_t stack[lds_entries<STACK>() * kTraceBlock];
     const int k = blockIdx.x * kTraceBlock + threadIdx.x;
-    if (k >= count) return;
+    if (k >= load_count(count_host, count_dev)) return;
     const int i = list[k];
     const uint32_t f = o.shadow_flag[i];
     const V3 P = ld3(q.px, q.py, q.pz, i);
@@ -637,10 +660,11 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(const SceneDev s, wf_sha
 }
 
 __global__ void k_count_shadow_rays(const uint32_t* __restrict__ flag, const int32_t* __restrict__ list,
-                                    int count, int32_t* __restrict__ total) {
+                                    int count_host, const int32_t* __restrict__ count_dev,
+                                    int32_t* __restrict__ total) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     int c = 0;
-    if (k < count) {
+    if (k < load_count(count_host, count_dev)) {
         const uint32_t f = flag[list[k]];
         c = (int)(f & 1u) + (int)((f >> 1) & 1u);
     }
@@ -664,70 +688,78 @@ static wf_status shade_outputs(wf_context* ctx, int n, ShadeOut& o) {
     return WF_OK;
 }
 
-wf_status shade_bounce(const wf_scene* s, const wf_shade_soa* q, int32_t n,
+wf_status shade_bounce(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q, Count n,
                        const wf_shade_params* p, float* accum, wf_ray_soa* next,
-                       int32_t* counts_dev) {
+                       int32_t* counts_dev, const ShadeJobs& jobs, const uint32_t** alive_out) {
     WF_TRY(scene_ready(s));
-    wf_context* ctx = s->ctx;
-    if (n <= 0) {
-        WF_TRY(wf_fill_u32(ctx, counts_dev, 0u, 2));
+    if (n.n <= 0) {
+        WF_TRY(wf_fill_u32(ctx, counts_dev, 0u, 3));
         return WF_OK;
     }
     ShadeOut o;
-    WF_TRY(shade_outputs(ctx, n, o));
+    WF_TRY(shade_outputs(ctx, n.n, o));
     void *pprefix, *plist, *pshflag;
-    const size_t nn = (size_t)n;
+    const size_t nn = (size_t)n.n;
     WF_TRY(workspace(ctx, "shade_prefix", nn * 4, &pprefix));
     WF_TRY(workspace(ctx, "shade_list", nn * 4, &plist));
     WF_TRY(workspace(ctx, "shade_sflag", nn * 4, &pshflag));
     ShadeArgs a{p->physics, p->rr_min_depth, p->cost_aware_rr, p->firefly_threshold, p->seed,
                 p->sample_index, p->job};
     {
-        Timed tm(ctx, "shade", 120.0 * n);
-        k_shade<<<grid_for(n, 128), 128, 0, ctx->stream>>>(s->dev, *q, n, a, s->dev.mat_rel_cost,
-                                                           accum, o);
+        Timed tm(ctx, "shade", 120.0 * n.n);
+        k_shade<<<grid_for(n.n, 128), 128, 0, ctx->stream>>>(s->dev, *q, n.n, n.dev, a,
+                                                             s->dev.mat_rel_cost, accum, o,
+                                                             jobs.tags, jobs.jobs);
         WF_HIP(hipGetLastError());
     }
     // shadow rays: compact the entries that emitted one, trace, accumulate
-    k_flags_nonzero<<<grid_for(n, 256), 256, 0, ctx->stream>>>(o.shadow_flag, n, (uint32_t*)pshflag);
+    k_flags_nonzero<<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(o.shadow_flag, n.n, n.dev,
+                                                                 (uint32_t*)pshflag);
     WF_HIP(hipGetLastError());
     int32_t* counts = counts_dev;
-    WF_TRY(exclusive_scan_u32(ctx, (uint32_t*)pshflag, (uint32_t*)pprefix, n, counts + 2, "compact"));
-    k_compact_index<<<grid_for(n, 256), 256, 0, ctx->stream>>>((uint32_t*)pshflag,
-                                                              (uint32_t*)pprefix, n, (int32_t*)plist);
+    WF_TRY(exclusive_scan_u32(ctx, (uint32_t*)pshflag, (uint32_t*)pprefix, n, counts + 2,
+                              "compact"));
+    k_compact_index<<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(
+        (uint32_t*)pshflag, (uint32_t*)pprefix, n.n, n.dev, (int32_t*)plist);
     WF_HIP(hipGetLastError());
-    // the shadow kernel runs over n entries worst case; entries past the count exit at once
-    int32_t n_shadow = 0;
-    WF_TRY(readback(ctx, &n_shadow, counts + 2, sizeof(int32_t)));
+    // a host count reads the number of shadow entries back to size the launch; a device count
+    // launches over the capacity and lets entries past the count exit at once
+    Count ns{n.n, counts + 2};
+    if (!n.dev) {
+        int32_t n_shadow = 0;
+        WF_TRY(readback(ctx, &n_shadow, counts + 2, sizeof(int32_t)));
+        ns = host_count(n_shadow);
+    }
     WF_TRY(wf_fill_u32(ctx, counts + 1, 0u, 1));
-    if (n_shadow > 0) {
-        k_count_shadow_rays<<<grid_for(n_shadow, 256), 256, 0, ctx->stream>>>(
-            o.shadow_flag, (int32_t*)plist, n_shadow, counts + 1);
+    if (ns.n > 0) {
+        k_count_shadow_rays<<<grid_for(ns.n, 256), 256, 0, ctx->stream>>>(
+            o.shadow_flag, (int32_t*)plist, ns.n, ns.dev, counts + 1);
         WF_HIP(hipGetLastError());
-        Timed tm(ctx, "shadow", 41.0 * n_shadow);
+        Timed tm(ctx, "shadow", 41.0 * ns.n);
         with_stack(s->dev.stack_bound, [&](auto S) {
             k_shadow<decltype(S)::value>
-                <<<grid_for(n_shadow, kTraceBlock), kTraceBlock, 0, ctx->stream>>>(
-                    s->dev, *q, o, (int32_t*)plist, n_shadow, accum);
+                <<<grid_for(ns.n, kTraceBlock), kTraceBlock, 0, ctx->stream>>>(
+                    s->dev, *q, o, (int32_t*)plist, ns.n, ns.dev, accum);
         });
         WF_HIP(hipGetLastError());
     }
     // next rays: stable compaction in shading order
     WF_TRY(exclusive_scan_u32(ctx, o.alive, (uint32_t*)pprefix, n, counts, "compact"));
     {
-        Timed tm(ctx, "next_rays", 76.0 * n);
-        k_next_rays<<<grid_for(n, 256), 256, 0, ctx->stream>>>(*q, o, (uint32_t*)pprefix, n, *next);
+        Timed tm(ctx, "next_rays", 76.0 * n.n);
+        k_next_rays<<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(
+            *q, o, (uint32_t*)pprefix, n.n, n.dev, *next, jobs.tags, jobs.next_tags);
         WF_HIP(hipGetLastError());
     }
+    if (alive_out) *alive_out = o.alive;
     return WF_OK;
 }
 
-wf_status shade_misses(const wf_scene* s, const wf_ray_soa* q, int32_t n, int32_t physics,
-                       float* accum) {
-    if (n <= 0) return WF_OK;
-    wf_context* ctx = s->ctx;
-    Timed tm(ctx, "miss", 52.0 * n);
-    k_miss<<<grid_for(n, 256), 256, 0, ctx->stream>>>(s->dev, *q, n, physics, accum);
+wf_status shade_misses(wf_context* ctx, const wf_scene* s, const wf_ray_soa* q, Count n,
+                       int32_t physics, float* accum) {
+    if (n.n <= 0) return WF_OK;
+    Timed tm(ctx, "miss", 52.0 * n.n);
+    k_miss<<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(s->dev, *q, n.n, n.dev, physics, accum);
     WF_HIP(hipGetLastError());
     return WF_OK;
 }
@@ -748,7 +780,7 @@ wf_status wf_shade(const wf_scene* s, const wf_shade_soa* queue, int32_t n,
                "wf_shade: unknown physics %d", params->physics);
     void* pc = nullptr;
     WF_TRY(workspace(s->ctx, "shade_counts", sizeof(int32_t) * 4, &pc));
-    WF_TRY(shade_bounce(s, queue, n, params, accum_rgb, next, (int32_t*)pc));
+    WF_TRY(shade_bounce(s->ctx, s, queue, host_count(n), params, accum_rgb, next, (int32_t*)pc));
     int32_t c[2] = {0, 0};
     WF_TRY(readback(s->ctx, c, pc, sizeof(c)));
     *n_next = c[0];
@@ -760,7 +792,7 @@ wf_status wf_shade_misses(const wf_scene* s, const wf_ray_soa* misses, int32_t n
     clear_error();
     WF_REQUIRE(s && misses && accum_rgb, "wf_shade_misses: NULL argument");
     WF_REQUIRE(n >= 0, "wf_shade_misses: negative count");
-    WF_TRY(shade_misses(s, misses, n, physics, accum_rgb));
+    WF_TRY(shade_misses(s->ctx, s, misses, host_count(n), physics, accum_rgb));
     WF_HIP(hipStreamSynchronize(s->ctx->stream));
     return WF_OK;
 }

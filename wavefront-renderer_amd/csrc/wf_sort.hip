@@ -22,9 +22,12 @@ constexpr int kSortTile = kSortBlock * kSortItems;
 constexpr int kSortWaves = kSortBlock / 64;
 
 __global__ __launch_bounds__(kSortBlock) void k_radix_hist(const uint64_t* __restrict__ keys,
-                                                           int n, int shift, uint32_t mask,
+                                                           int n_host,
+                                                           const int32_t* __restrict__ nd,
+                                                           int shift, uint32_t mask,
                                                            uint32_t* __restrict__ hist,
                                                            int tiles) {
+    const int n = load_count(n_host, nd);
     __shared__ uint32_t cnt[256];
     cnt[threadIdx.x] = 0;
     __syncthreads();
@@ -40,8 +43,11 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_hist(const uint64_t* __res
 
 __global__ __launch_bounds__(kSortBlock) void k_radix_scatter(
     const uint64_t* __restrict__ keys_in, const int32_t* __restrict__ vals_in,
-    uint64_t* __restrict__ keys_out, int32_t* __restrict__ vals_out, int n, int shift,
-    uint32_t mask, int nbits, const uint32_t* __restrict__ offsets, int tiles) {
+    uint64_t* __restrict__ keys_out, int32_t* __restrict__ vals_out, int n_host,
+    const int32_t* __restrict__ nd, int shift, uint32_t mask, int nbits,
+    const uint32_t* __restrict__ offsets, int tiles) {
+    const int n = load_count(n_host, nd);
+    if (blockIdx.x * kSortTile >= n) return;
     __shared__ uint32_t wave_cnt[kSortWaves][256];
     __shared__ uint32_t digit_base[256];
     const int lane = threadIdx.x & 63;
@@ -128,8 +134,10 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t x, uint32_t* l
 }
 
 __global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const uint32_t* __restrict__ in,
-                                                            int n,
+                                                            int n_host,
+                                                            const int32_t* __restrict__ nd,
                                                             uint32_t* __restrict__ sums) {
+    const int n = load_count(n_host, nd);
     __shared__ uint32_t lds[kScanBlock / 64];
     const int base = blockIdx.x * kScanTile;
     uint32_t s = 0;
@@ -144,9 +152,13 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const uint32_t* __re
 }
 
 __global__ __launch_bounds__(kScanBlock) void k_scan_down(const uint32_t* in, uint32_t* out,
-                                                          int n,
+                                                          int n_host,
+                                                          const int32_t* __restrict__ nd,
                                                           const uint32_t* __restrict__ offsets,
                                                           int32_t* __restrict__ total) {
+    const int n = load_count(n_host, nd);
+    if (total && n <= 0 && blockIdx.x == 0 && threadIdx.x == 0) *total = 0;
+    if (blockIdx.x * kScanTile >= n) return;
     __shared__ uint32_t tile[kScanTile];
     __shared__ uint32_t lds[kScanBlock / 64];
     const int base = blockIdx.x * kScanTile;
@@ -181,6 +193,28 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_down(const uint32_t* in, ui
         }
     }
 }
+
+__global__ void k_iota_dev(int32_t* dst, int n_host, const int32_t* __restrict__ nd) {
+    const int n = load_count(n_host, nd);
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = i;
+}
+
+__global__ void k_copy_keys(const uint64_t* __restrict__ a, uint64_t* __restrict__ b, int n_host,
+                            const int32_t* __restrict__ nd) {
+    const int n = load_count(n_host, nd);
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+__global__ void k_copy_vals(const int32_t* __restrict__ a, int32_t* __restrict__ b, int n_host,
+                            const int32_t* __restrict__ nd) {
+    const int n = load_count(n_host, nd);
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+__global__ void k_sub_count(int32_t* dst, const int32_t* a, const int32_t* b) { *dst = *a - *b; }
 
 __global__ void k_iota(int32_t* dst, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -220,12 +254,12 @@ __global__ void k_key_and_or(const uint64_t* __restrict__ keys, int n,
     }
 }
 
-static wf_status scan_level(wf_context* ctx, const uint32_t* in, uint32_t* out, int n,
+static wf_status scan_level(wf_context* ctx, const uint32_t* in, uint32_t* out, Count n,
                             int32_t* total, int level, const char* tag) {
-    const int tiles = (n + kScanTile - 1) / kScanTile;
+    const int tiles = (n.n + kScanTile - 1) / kScanTile;
     if (tiles <= 1) {
-        Timed t(ctx, tag, 8.0 * n);
-        k_scan_down<<<1, kScanBlock, 0, ctx->stream>>>(in, out, n, nullptr, total);
+        Timed t(ctx, tag, 8.0 * n.n);
+        k_scan_down<<<1, kScanBlock, 0, ctx->stream>>>(in, out, n.n, n.dev, nullptr, total);
         WF_HIP(hipGetLastError());
         return WF_OK;
     }
@@ -235,22 +269,22 @@ static wf_status scan_level(wf_context* ctx, const uint32_t* in, uint32_t* out, 
     WF_TRY(workspace(ctx, name, sizeof(uint32_t) * (size_t)tiles, &p));
     uint32_t* sums = (uint32_t*)p;
     {
-        Timed t(ctx, tag, 4.0 * n);
-        k_scan_reduce<<<tiles, kScanBlock, 0, ctx->stream>>>(in, n, sums);
+        Timed t(ctx, tag, 4.0 * n.n);
+        k_scan_reduce<<<tiles, kScanBlock, 0, ctx->stream>>>(in, n.n, n.dev, sums);
         WF_HIP(hipGetLastError());
     }
-    WF_TRY(scan_level(ctx, sums, sums, tiles, nullptr, level + 1, tag));
+    WF_TRY(scan_level(ctx, sums, sums, host_count(tiles), nullptr, level + 1, tag));
     {
-        Timed t(ctx, tag, 8.0 * n);
-        k_scan_down<<<tiles, kScanBlock, 0, ctx->stream>>>(in, out, n, sums, total);
+        Timed t(ctx, tag, 8.0 * n.n);
+        k_scan_down<<<tiles, kScanBlock, 0, ctx->stream>>>(in, out, n.n, n.dev, sums, total);
         WF_HIP(hipGetLastError());
     }
     return WF_OK;
 }
 
-wf_status exclusive_scan_u32(wf_context* ctx, const uint32_t* in, uint32_t* out, int32_t n,
+wf_status exclusive_scan_u32(wf_context* ctx, const uint32_t* in, uint32_t* out, Count n,
                              int32_t* total_dev, const char* tag) {
-    if (n <= 0) {
+    if (n.n <= 0) {
         if (total_dev) {
             k_set_i32<<<1, 1, 0, ctx->stream>>>(total_dev, 0);
             WF_HIP(hipGetLastError());
@@ -267,56 +301,65 @@ wf_status iota_i32(wf_context* ctx, int32_t* dst, int32_t n) {
     return WF_OK;
 }
 
-wf_status radix_sort_pairs(wf_context* ctx, const uint64_t* keys_in, const int32_t* vals_in,
-                           uint64_t* keys_out, int32_t* vals_out, int32_t n, int32_t key_bits,
-                           const char* tag) {
-    if (n <= 0) return WF_OK;
-    if (key_bits <= 0) {
-        // nothing to order by: identity permutation of the values
-        if (vals_in)
-            WF_HIP(hipMemcpyAsync(vals_out, vals_in, sizeof(int32_t) * n, hipMemcpyDeviceToDevice,
-                                  ctx->stream));
-        else
-            WF_TRY(iota_i32(ctx, vals_out, n));
-        WF_HIP(hipMemcpyAsync(keys_out, keys_in, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice,
-                              ctx->stream));
-        return WF_OK;
+wf_status iota_count(wf_context* ctx, int32_t* dst, Count n) {
+    if (n.n <= 0) return WF_OK;
+    k_iota_dev<<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(dst, n.n, n.dev);
+    WF_HIP(hipGetLastError());
+    return WF_OK;
+}
+
+wf_status sub_count(wf_context* ctx, int32_t* dst, const int32_t* a, const int32_t* b) {
+    k_sub_count<<<1, 1, 0, ctx->stream>>>(dst, a, b);
+    WF_HIP(hipGetLastError());
+    return WF_OK;
+}
+
+// Identity "sort": values 0..n-1 (or vals_in), keys copied.
+static wf_status sort_identity(wf_context* ctx, const uint64_t* keys_in, const int32_t* vals_in,
+                               uint64_t* keys_out, int32_t* vals_out, Count n) {
+    if (vals_in)
+        k_copy_vals<<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(vals_in, vals_out, n.n, n.dev);
+    else
+        k_iota_dev<<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(vals_out, n.n, n.dev);
+    WF_HIP(hipGetLastError());
+    if (keys_in && keys_out) {
+        k_copy_keys<<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(keys_in, keys_out, n.n, n.dev);
+        WF_HIP(hipGetLastError());
     }
-    // Only the bit range where keys differ can change the order: find it and sort just that
-    // range (stable, so the result is the same permutation as sorting all key_bits).
+    return WF_OK;
+}
+
+wf_status radix_sort_pairs(wf_context* ctx, const uint64_t* keys_in, const int32_t* vals_in,
+                           uint64_t* keys_out, int32_t* vals_out, Count n, int32_t key_bits,
+                           const char* tag) {
+    if (n.n <= 0) return WF_OK;
+    if (key_bits <= 0) return sort_identity(ctx, keys_in, vals_in, keys_out, vals_out, n);
     int lo_bit = 0, hi_bit = key_bits - 1;
-    {
+    if (!n.dev) {
+        // Only the bit range where keys differ can change the order: find it and sort just that
+        // range (stable, so the result is the same permutation as sorting all key_bits).
         void* pr = nullptr;
         WF_TRY(workspace(ctx, "sort_and_or", 16, &pr));
         unsigned long long init[2] = {~0ull, 0ull};
         WF_HIP(hipMemcpyAsync(pr, init, sizeof(init), hipMemcpyHostToDevice, ctx->stream));
-        Timed t(ctx, tag, 8.0 * n);
-        const int blocks = (int)std::min<int64_t>(grid_for(n, 256), 1024);
-        k_key_and_or<<<blocks, 256, 0, ctx->stream>>>(keys_in, n, (unsigned long long*)pr);
+        Timed t(ctx, tag, 8.0 * n.n);
+        const int blocks = (int)std::min<int64_t>(grid_for(n.n, 256), 1024);
+        k_key_and_or<<<blocks, 256, 0, ctx->stream>>>(keys_in, n.n, (unsigned long long*)pr);
         WF_HIP(hipGetLastError());
         unsigned long long r[2];
         WF_TRY(readback(ctx, r, pr, sizeof(r)));
         unsigned long long diff = (r[0] ^ r[1]);
         if (key_bits < 64) diff &= (1ull << key_bits) - 1ull;
-        if (diff == 0) {
-            if (vals_in)
-                WF_HIP(hipMemcpyAsync(vals_out, vals_in, sizeof(int32_t) * n,
-                                      hipMemcpyDeviceToDevice, ctx->stream));
-            else
-                WF_TRY(iota_i32(ctx, vals_out, n));
-            WF_HIP(hipMemcpyAsync(keys_out, keys_in, sizeof(uint64_t) * n,
-                                  hipMemcpyDeviceToDevice, ctx->stream));
-            return WF_OK;
-        }
+        if (diff == 0) return sort_identity(ctx, keys_in, vals_in, keys_out, vals_out, n);
         lo_bit = __builtin_ctzll(diff);
         hi_bit = 63 - __builtin_clzll(diff);
     }
     const int span = hi_bit - lo_bit + 1;
-    const int tiles = (n + kSortTile - 1) / kSortTile;
+    const int tiles = (n.n + kSortTile - 1) / kSortTile;
     const int passes = (span + 7) / 8;
     void *pk = nullptr, *pv = nullptr, *ph = nullptr;
-    WF_TRY(workspace(ctx, "sort_keys_alt", sizeof(uint64_t) * (size_t)n, &pk));
-    WF_TRY(workspace(ctx, "sort_vals_alt", sizeof(int32_t) * (size_t)n, &pv));
+    WF_TRY(workspace(ctx, "sort_keys_alt", sizeof(uint64_t) * (size_t)n.n, &pk));
+    WF_TRY(workspace(ctx, "sort_vals_alt", sizeof(int32_t) * (size_t)n.n, &pv));
     WF_TRY(workspace(ctx, "sort_hist", sizeof(uint32_t) * 256 * (size_t)tiles, &ph));
     uint32_t* hist = (uint32_t*)ph;
     // ping-pong so that the last pass writes keys_out/vals_out
@@ -330,15 +373,16 @@ wf_status radix_sort_pairs(wf_context* ctx, const uint64_t* keys_in, const int32
         uint64_t* kdst = to_out ? keys_out : (uint64_t*)pk;
         int32_t* vdst = to_out ? vals_out : (int32_t*)pv;
         {
-            Timed t(ctx, tag, 8.0 * n);
-            k_radix_hist<<<tiles, kSortBlock, 0, ctx->stream>>>(ksrc, n, shift, mask, hist, tiles);
+            Timed t(ctx, tag, 8.0 * n.n);
+            k_radix_hist<<<tiles, kSortBlock, 0, ctx->stream>>>(ksrc, n.n, n.dev, shift, mask,
+                                                                 hist, tiles);
             WF_HIP(hipGetLastError());
         }
-        WF_TRY(exclusive_scan_u32(ctx, hist, hist, 256 * tiles, nullptr, tag));
+        WF_TRY(exclusive_scan_u32(ctx, hist, hist, host_count(256 * tiles), nullptr, tag));
         {
-            Timed t(ctx, tag, 24.0 * n);
+            Timed t(ctx, tag, 24.0 * n.n);
             k_radix_scatter<<<tiles, kSortBlock, 0, ctx->stream>>>(
-                ksrc, vsrc, kdst, vdst, n, shift, mask, nbits, hist, tiles);
+                ksrc, vsrc, kdst, vdst, n.n, n.dev, shift, mask, nbits, hist, tiles);
             WF_HIP(hipGetLastError());
         }
         ksrc = kdst;

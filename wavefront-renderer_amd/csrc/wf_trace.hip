@@ -18,11 +18,12 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(
     const Bvh4Node* __restrict__ nodes, const float4* __restrict__ tri, int n_tris,
     const float* __restrict__ ox, const float* __restrict__ oy, const float* __restrict__ oz,
     const float* __restrict__ dx, const float* __restrict__ dy, const float* __restrict__ dz,
-    int n, float tnear, float tfar, int32_t* __restrict__ out_leaf, float* __restrict__ out_t,
-    float* __restrict__ out_u, float* __restrict__ out_v, uint32_t* __restrict__ out_flag) {
+    int n_host, const int32_t* __restrict__ nd, float tnear, float tfar,
+    int32_t* __restrict__ out_leaf, float* __restrict__ out_t, float* __restrict__ out_u,
+    float* __restrict__ out_v, uint32_t* __restrict__ out_flag) {
     __shared__ int32_t stack[lds_entries<STACK>() * kTraceBlock];
     const int i = blockIdx.x * kTraceBlock + threadIdx.x;
-    if (i >= n) return;
+    if (i >= load_count(n_host, nd)) return;
     TraceHit h;
     traverse<false, STACK>(nodes, tri, n_tris, V3(ox[i], oy[i], oz[i]), V3(dx[i], dy[i], dz[i]), tnear,
                     tfar, stack, h);
@@ -97,16 +98,18 @@ __device__ __forceinline__ void map_hit(const SceneDev& s, const wf_ray_soa& in,
 }
 
 // Stable split by the exclusive prefix of hit flags (intersectAll's hit and miss queues).
-__global__ void k_split(const SceneDev s, wf_ray_soa in, int n, const int32_t* __restrict__ leaf,
-                        const float* __restrict__ ht, const float* __restrict__ hu,
-                        const float* __restrict__ hv, const uint32_t* __restrict__ prefix,
-                        wf_shade_soa out, wf_ray_soa miss) {
+__global__ void k_split(const SceneDev s, wf_ray_soa in, int n_host, const int32_t* __restrict__ nd,
+                        const int32_t* __restrict__ leaf, const float* __restrict__ ht,
+                        const float* __restrict__ hu, const float* __restrict__ hv,
+                        const uint32_t* __restrict__ prefix, wf_shade_soa out, wf_ray_soa miss,
+                        const int32_t* __restrict__ tags_in, int32_t* __restrict__ hit_tags) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= load_count(n_host, nd)) return;
     const int L = leaf[i];
     const uint32_t pos = prefix[i];
     if (L >= 0) {
         map_hit(s, in, i, L, ht[i], hu[i], hv[i], out, (int)pos);
+        if (hit_tags) hit_tags[pos] = tags_in[i];
     } else {
         const int m = i - (int)pos;
         miss.ox[m] = in.ox[i];
@@ -166,18 +169,17 @@ wf_status scene_ready(const wf_scene* s) {
     return WF_OK;
 }
 
-wf_status trace_closest(const wf_scene* s, const wf_ray_soa* r, int32_t n, float tnear,
-                        float tfar, int32_t* hit_leaf, float* hit_t, float* hit_bu, float* hit_bv,
-                        uint32_t* flags) {
+wf_status trace_closest(wf_context* ctx, const wf_scene* s, const wf_ray_soa* r, Count n,
+                        float tnear, float tfar, int32_t* hit_leaf, float* hit_t, float* hit_bu,
+                        float* hit_bv, uint32_t* flags) {
     WF_TRY(scene_ready(s));
-    if (n <= 0) return WF_OK;
-    wf_context* ctx = s->ctx;
-    Timed tm(ctx, "traverse_closest", 44.0 * n);
+    if (n.n <= 0) return WF_OK;
+    Timed tm(ctx, "traverse_closest", 44.0 * n.n);
     with_stack(s->dev.stack_bound, [&](auto S) {
         k_trace_closest<decltype(S)::value>
-            <<<grid_for(n, kTraceBlock), kTraceBlock, 0, ctx->stream>>>(
+            <<<grid_for(n.n, kTraceBlock), kTraceBlock, 0, ctx->stream>>>(
                 s->dev.nodes4, s->dev.tri, s->dev.n_tris, r->ox, r->oy, r->oz, r->dx, r->dy,
-                r->dz, n, tnear, tfar, hit_leaf, hit_t, hit_bu, hit_bv, flags);
+                r->dz, n.n, n.dev, tnear, tfar, hit_leaf, hit_t, hit_bu, hit_bv, flags);
     });
     WF_HIP(hipGetLastError());
     return WF_OK;
@@ -200,25 +202,34 @@ wf_status trace_occluded(const wf_scene* s, const float* ox, const float* oy, co
 }
 
 // Closest hit + stable split; counts_dev[0] = hits, counts_dev[1] = misses (device int32).
-wf_status intersect_split(const wf_scene* s, const wf_ray_soa* rays, int32_t n,
-                          wf_shade_soa* hits_out, wf_ray_soa* misses_out, int32_t* counts_dev) {
-    wf_context* ctx = s->ctx;
+wf_status intersect_split(wf_context* ctx, const wf_scene* s, const wf_ray_soa* rays, Count n,
+                          wf_shade_soa* hits_out, wf_ray_soa* misses_out, int32_t* counts_dev,
+                          const int32_t* tags_in, int32_t* hit_tags) {
     void *pl, *pt, *pu, *pv, *pf;
-    size_t nn = (size_t)(n > 0 ? n : 1);
+    size_t nn = (size_t)(n.n > 0 ? n.n : 1);
     WF_TRY(workspace(ctx, "hit_leaf", sizeof(int32_t) * nn, &pl));
     WF_TRY(workspace(ctx, "hit_t", sizeof(float) * nn, &pt));
     WF_TRY(workspace(ctx, "hit_u", sizeof(float) * nn, &pu));
     WF_TRY(workspace(ctx, "hit_v", sizeof(float) * nn, &pv));
     WF_TRY(workspace(ctx, "hit_flag", sizeof(uint32_t) * nn, &pf));
-    WF_TRY(trace_closest(s, rays, n, 0.001f, 1e9f, (int32_t*)pl, (float*)pt, (float*)pu,
+    WF_TRY(trace_closest(ctx, s, rays, n, 0.001f, 1e9f, (int32_t*)pl, (float*)pt, (float*)pu,
                          (float*)pv, (uint32_t*)pf));
     WF_TRY(exclusive_scan_u32(ctx, (uint32_t*)pf, (uint32_t*)pf, n, counts_dev, "split"));
-    if (n > 0) {
-        Timed tm(ctx, "split", 97.0 * n);
-        k_split<<<grid_for(n, 256), 256, 0, ctx->stream>>>(s->dev, *rays, n, (int32_t*)pl,
-                                                           (float*)pt, (float*)pu, (float*)pv,
-                                                           (uint32_t*)pf, *hits_out, *misses_out);
+    if (n.n > 0) {
+        Timed tm(ctx, "split", 97.0 * n.n);
+        k_split<<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(
+            s->dev, *rays, n.n, n.dev, (int32_t*)pl, (float*)pt, (float*)pu, (float*)pv,
+            (uint32_t*)pf, *hits_out, *misses_out, tags_in, hit_tags);
         WF_HIP(hipGetLastError());
+    }
+    if (n.dev) {
+        WF_TRY(sub_count(ctx, counts_dev + 1, n.dev, counts_dev));
+    } else {
+        // misses = n - hits, on the device so that callers read both counts from one place
+        void* pn = nullptr;
+        WF_TRY(workspace(ctx, "split_n", sizeof(int32_t), &pn));
+        WF_TRY(wf_fill_u32(ctx, pn, (uint32_t)n.n, 1));
+        WF_TRY(sub_count(ctx, counts_dev + 1, (const int32_t*)pn, counts_dev));
     }
     return WF_OK;
 }
@@ -242,8 +253,8 @@ wf_status wf_intersect(const wf_scene* s, const wf_ray_soa* rays, int32_t n, flo
     WF_TRY(workspace(ctx, "hit_t", sizeof(float) * n, &pt));
     WF_TRY(workspace(ctx, "hit_u", sizeof(float) * n, &pu));
     WF_TRY(workspace(ctx, "hit_v", sizeof(float) * n, &pv));
-    WF_TRY(trace_closest(s, rays, n, tnear, tfar, (int32_t*)pl, (float*)pt, (float*)pu,
-                         (float*)pv, nullptr));
+    WF_TRY(trace_closest(ctx, s, rays, host_count(n), tnear, tfar, (int32_t*)pl, (float*)pt,
+                         (float*)pu, (float*)pv, nullptr));
     k_hits_to_soa<<<grid_for(n, 256), 256, 0, ctx->stream>>>(s->dev, n, (int32_t*)pl, (float*)pt,
                                                              (float*)pu, (float*)pv, *hits);
     WF_HIP(hipGetLastError());
@@ -264,8 +275,8 @@ wf_status wf_intersect_records(const wf_scene* s, const wf_ray_soa* rays, int32_
     WF_TRY(workspace(ctx, "hit_t", sizeof(float) * n, &pt));
     WF_TRY(workspace(ctx, "hit_u", sizeof(float) * n, &pu));
     WF_TRY(workspace(ctx, "hit_v", sizeof(float) * n, &pv));
-    WF_TRY(trace_closest(s, rays, n, tnear, tfar, (int32_t*)pl, (float*)pt, (float*)pu,
-                         (float*)pv, nullptr));
+    WF_TRY(trace_closest(ctx, s, rays, host_count(n), tnear, tfar, (int32_t*)pl, (float*)pt,
+                         (float*)pu, (float*)pv, nullptr));
     k_records<<<grid_for(n, 256), 256, 0, ctx->stream>>>(s->dev, *rays, n, (int32_t*)pl,
                                                          (float*)pt, (float*)pu, (float*)pv,
                                                          *records, hit_flags);
@@ -296,7 +307,7 @@ wf_status wf_intersect_split(const wf_scene* s, const wf_ray_soa* rays, int32_t 
     WF_TRY(scene_ready(s));
     void* pc = nullptr;
     WF_TRY(workspace(s->ctx, "split_count", sizeof(int32_t) * 2, &pc));
-    WF_TRY(intersect_split(s, rays, n, hits_out, misses_out, (int32_t*)pc));
+    WF_TRY(intersect_split(s->ctx, s, rays, host_count(n), hits_out, misses_out, (int32_t*)pc));
     int32_t h = 0;
     WF_TRY(readback(s->ctx, &h, pc, sizeof(int32_t)));
     *n_hits = h;
