@@ -73,6 +73,102 @@ def stream_copy_gbs(wf, ctx, nbytes: int, reps: int) -> float:
         ctx.free(dst)
 
 
+def camera_rays(camera, W, H, rng):
+    """Jittered pinhole rays of the workload's camera (numpy; for traversal statistics)."""
+    eye = np.array(camera["eye"], np.float64)
+    at = np.array(camera["target"], np.float64)
+    up = np.array(camera.get("up", (0.0, 1.0, 0.0)), np.float64)
+    vh = 2.0 * np.tan(np.radians(camera["vfov"]) / 2.0)
+    vw = vh * W / H
+    fwd = (eye - at) / np.linalg.norm(eye - at)
+    right = np.cross(up, fwd)
+    right /= np.linalg.norm(right)
+    upv = np.cross(fwd, right)
+    y, x = np.mgrid[0:H, 0:W]
+    u = (x.ravel() + rng.random(W * H)) / (W - 1)
+    v = 1.0 - (y.ravel() + rng.random(W * H)) / (H - 1)
+    tl = eye - right * vw / 2 - upv * vh / 2 - fwd
+    d = tl + np.outer(u, right * vw) + np.outer(v, upv * vh) - eye
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    o = np.broadcast_to(eye, d.shape)
+    return o.astype(np.float32), d.astype(np.float32)
+
+
+def traversal_stats(wf, ctx, scene, data, W, H, rng) -> dict:
+    """Per ray class: BVH nodes visited and triangles tested per ray, the SIMD efficiency of a
+    64-lane wave that loops until its slowest lane ends, and device rays/s of the trace."""
+    def queue(o, d):
+        n = len(o)
+        q = wf.RayQueue(ctx, n)
+        z = np.zeros(n, np.float32)
+        q.upload({"ox": o[:, 0], "oy": o[:, 1], "oz": o[:, 2], "dx": d[:, 0], "dy": d[:, 1],
+                  "dz": d[:, 2], "tr": z, "tg": z, "tb": z, "pixel": np.arange(n, dtype=np.int32),
+                  "depth": np.zeros(n, np.int32), "pdf": z})
+        return q
+
+    def timed(kernel, fn):
+        ctx.synchronize()
+        ctx.profile_filter(kernel)
+        ctx.profile(True)
+        fn()
+        ms, launches, _ = ctx.profile_query(kernel)
+        ctx.profile(False)
+        ctx.profile_filter(None)
+        return ms
+
+    out = {}
+    o, d = camera_rays(data.camera, W, H, rng)
+    q = queue(o, d)
+    n = len(o)
+    ms = timed("traverse_closest", lambda: scene.intersect(q, n))
+    hits = scene.intersect(q, n)
+    nodes, tris = scene.trace_stats(q, n)
+    out["primary"] = {"rays": n, "mrays_s": round(n / ms / 1e3, 1),
+                      "nodes_per_ray": round(float(nodes.mean()), 2),
+                      "tris_per_ray": round(float(tris.mean()), 2),
+                      "simd_efficiency": round(wf.simd_efficiency(nodes), 3)}
+    h = hits["prim"] >= 0
+    P = o[h] + d[h] * hits["t"][h][:, None]
+    m = len(P)
+    if m == 0:
+        return out
+    # diffuse-like bounce: uniform directions leaving the hit point
+    w = rng.normal(size=(m, 3))
+    w /= np.linalg.norm(w, axis=1, keepdims=True)
+    w = np.where(np.sum(w * d[h], axis=1, keepdims=True) > 0, -w, w).astype(np.float32)
+    q2 = queue(P.astype(np.float32), w)
+    ms = timed("traverse_closest", lambda: scene.intersect(q2, m))
+    nodes, tris = scene.trace_stats(q2, m)
+    out["bounce"] = {"rays": m, "mrays_s": round(m / ms / 1e3, 1),
+                     "nodes_per_ray": round(float(nodes.mean()), 2),
+                     "tris_per_ray": round(float(tris.mean()), 2),
+                     "simd_efficiency": round(wf.simd_efficiency(nodes), 3)}
+    # shadow rays toward random points of the emitters (any hit)
+    lights = [mesh for mesh in data.meshes
+              if data.materials[mesh.material_id]["type"] in (wf.MAT_EMISSIVE, wf.MAT_SPOTLIGHT)]
+    if lights:
+        tri = np.concatenate([mesh.positions[mesh.indices] for mesh in lights])
+        k = rng.integers(0, len(tri), m)
+        r1, r2 = rng.random(m), rng.random(m)
+        flip = r1 + r2 > 1
+        r1[flip], r2[flip] = 1 - r1[flip], 1 - r2[flip]
+        lp = (tri[k, 0] * (1 - r1 - r2)[:, None] + tri[k, 1] * r1[:, None] +
+              tri[k, 2] * r2[:, None])
+        tl = lp - P
+        dist = np.linalg.norm(tl, axis=1)
+        sd = (tl / dist[:, None]).astype(np.float32)
+        q3 = queue(P.astype(np.float32), sd)
+        nodes, tris = scene.trace_stats(q3, m, 0.001, 1e9, any_hit=True)
+        md = (dist - 0.001).astype(np.float32)
+        ms = timed("traverse_any", lambda: scene.occluded(P.astype(np.float32), sd, md))
+        out["shadow"] = {"rays": m, "mrays_s": round(m / ms / 1e3, 1),
+                         "nodes_per_ray": round(float(nodes.mean()), 2),
+                         "tris_per_ray": round(float(tris.mean()), 2),
+                         "simd_efficiency": round(wf.simd_efficiency(nodes), 3),
+                         "note": "any hit up to tfar 1e9 for the counts"}
+    return out
+
+
 def host_cpu() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -198,6 +294,9 @@ def main() -> None:
     if roofline is not None:
         roofline["stream_copy_GB_s"] = copy
 
+    tstats = traversal_stats(wf, ctx, scene, data, W, H, np.random.default_rng(5))
+    log(f"[bench] traversal: {json.dumps(tstats)}")
+
     # the other schedule of the same frame, for comparison (identical image)
     other = "passes" if args.schedule == "regenerate" else "regenerate"
     t1 = time.perf_counter()
@@ -264,6 +363,7 @@ def main() -> None:
                    "unbiased" if physics else "reference", "triangles": data.triangle_count,
                    "bvh_build_ms": round(scene.stats.build_ms, 1), "policies": policies},
         "roofline": roofline,
+        "traversal": tstats,
         "kernels": kernels,
         "cpu_baseline": cpu,
     }

@@ -228,6 +228,12 @@ wf_status wf_intersect(const wf_scene* scene, const wf_ray_soa* rays, int32_t n,
 wf_status wf_intersect_records(const wf_scene* scene, const wf_ray_soa* rays, int32_t n,
                                float tnear, float tfar, wf_shade_soa* records,
                                uint8_t* hit_flags);
+/* Traversal statistics for rays [0,n) (closest hit, or any hit when any_hit != 0): BVH nodes
+ * visited and triangles tested per ray into device int32 arrays. Lanes of a 64-wide wave are
+ * consecutive rays, so lane divergence follows from the counts. No reference counterpart. */
+wf_status wf_trace_stats(const wf_scene* scene, const wf_ray_soa* rays, int32_t n, float tnear,
+                         float tfar, int32_t any_hit, int32_t* nodes_visited,
+                         int32_t* triangles_tested);
 /* EmbreeAccelerator::occluded (Embree.cpp:258-279) for n shadow rays: occluded[i] = 1 when any
  * triangle lies in [0.001, max_dist[i]-0.001] (and, as in the reference, whenever that upper
  * bound is negative). Inputs are device arrays. */

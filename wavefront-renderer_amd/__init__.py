@@ -230,6 +230,17 @@ class Scene:
         check(self.ctx._lib.wf_intersect(self.handle, C.byref(r), n, tnear, tfar, C.byref(h)))
         return hits.download(n)
 
+    def trace_stats(self, rays: RayQueue, n: int, tnear: float = 0.001, tfar: float = 1e9,
+                    any_hit: bool = False):
+        """(nodes visited, triangles tested) per ray; lanes of a wave are rays 64w..64w+63."""
+        nodes = DeviceBuffer(self.ctx, max(n, 1), np.int32)
+        tris = DeviceBuffer(self.ctx, max(n, 1), np.int32)
+        r = rays.soa()
+        check(self.ctx._lib.wf_trace_stats(self.handle, C.byref(r), n, tnear, tfar,
+                                           1 if any_hit else 0, C.c_void_p(nodes.ptr),
+                                           C.c_void_p(tris.ptr)))
+        return nodes.download(n), tris.download(n)
+
     # EmbreeAccelerator::intersect4 with the HitRecord mapping, one record per input ray
     def intersect_records(self, rays: RayQueue, n: int, tnear: float = 0.001, tfar: float = 1e9):
         rec = ShadingQueue(self.ctx, n)
@@ -448,3 +459,15 @@ def engine_bytes(capacity: int, pixel_budget: int) -> int:
     out = C.c_size_t()
     check(_abi.load_library().wf_engine_bytes(capacity, pixel_budget, C.byref(out)))
     return out.value
+
+
+def simd_efficiency(per_ray: np.ndarray, wave: int = 64) -> float:
+    """Useful lane-iterations / issued lane-iterations when a wave loops until its slowest lane
+    finishes (per_ray: loop trip count of each ray, waves = consecutive groups of 64)."""
+    n = len(per_ray)
+    if n == 0:
+        return 1.0
+    pad = (-n) % wave
+    x = np.concatenate([per_ray.astype(np.int64), np.zeros(pad, np.int64)]).reshape(-1, wave)
+    issued = x.max(axis=1).sum() * wave
+    return float(x.sum() / issued) if issued else 1.0

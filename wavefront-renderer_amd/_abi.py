@@ -179,6 +179,8 @@ _PROTOS = {
     "wf_intersect_records": (_S, [P, C.POINTER(RaySoA), C.c_int32, C.c_float, C.c_float,
                                   C.POINTER(ShadeSoA), P]),
     "wf_occluded": (_S, [P, P, P, P, P, P, P, P, C.c_int32, P]),
+    "wf_trace_stats": (_S, [P, C.POINTER(RaySoA), C.c_int32, C.c_float, C.c_float, C.c_int32,
+                            P, P]),
     "wf_intersect_split": (_S, [P, C.POINTER(RaySoA), C.c_int32, C.POINTER(ShadeSoA),
                                 C.POINTER(C.c_int32), C.POINTER(RaySoA), C.POINTER(C.c_int32)]),
     "wf_ray_schedule": (_S, [P, C.POINTER(RaySoA), C.c_int32, C.POINTER(C.c_float),

@@ -83,11 +83,28 @@ Timed::Timed(wf_context* c, const char* n, double algorithmic_bytes)
     if (a && b) (void)hipEventRecord(a, ctx->stream);
 }
 
+Timed::Timed(wf_context* c, const char* n, double bytes_per_unit, const Count& count)
+    : Timed(c, n, count.dev ? bytes_per_unit : bytes_per_unit * count.n) {
+    count_dev = count.dev;
+}
+
 Timed::~Timed() {
     if (!ctx || !ctx->profile || !a || !b) return;
     (void)hipEventRecord(b, ctx->stream);
-    ctx->pending.push_back(PendingTiming{name, a, b, bytes});
-    if (ctx->pending.size() >= 8192) harvest_timings(ctx);
+    int slot = -1;
+    if (count_dev) {
+        if (!ctx->count_slots &&
+            hipHostMalloc((void**)&ctx->count_slots, sizeof(int32_t) * kCountSlots) != hipSuccess)
+            ctx->count_slots = nullptr;
+        if (ctx->count_slots) {
+            slot = ctx->next_count_slot;
+            ctx->next_count_slot = (slot + 1) % kCountSlots;
+            (void)hipMemcpyAsync(ctx->count_slots + slot, count_dev, sizeof(int32_t),
+                                 hipMemcpyDeviceToHost, ctx->stream);
+        }
+    }
+    ctx->pending.push_back(PendingTiming{name, a, b, bytes, slot});
+    if (ctx->pending.size() >= kCountSlots / 2) harvest_timings(ctx);
 }
 
 wf_status harvest_timings(wf_context* ctx) {
@@ -99,7 +116,7 @@ wf_status harvest_timings(wf_context* ctx) {
         auto& e = ctx->prof[p.name];
         e.ms += ms;
         e.launches += 1;
-        e.bytes += p.bytes;
+        e.bytes += p.count_slot >= 0 ? p.bytes * (double)ctx->count_slots[p.count_slot] : p.bytes;
         ctx->event_pool.push_back(p.start);
         ctx->event_pool.push_back(p.stop);
     }
@@ -153,6 +170,7 @@ wf_status wf_context_destroy(wf_context* ctx) {
     for (auto& kv : ctx->workspace)
         if (kv.second.ptr) (void)hipFree(kv.second.ptr);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->count_slots) (void)hipHostFree(ctx->count_slots);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return WF_OK;

@@ -154,11 +154,16 @@ __global__ void k_regen_commit(EngineView e) {
     e.stats[S_PATHS] += (unsigned long long)take;
 }
 
+constexpr int kFinishBlock = 1024;
+
 // A path of pixel p ended: count the sample, update the adaptive estimate
 // (AdaptiveSampler::addSample/updateConvergence, include/render/AdaptiveSampler.h:21-105, checked
 // when the pixel's sample count is a multiple of the interval, as the pass loop does), then
-// requeue the pixel or retire it.
+// requeue the pixel or retire it. Requeues are appended with one queue-cursor atomic per block.
+// Called by every thread of the block (valid = this thread has an ended path).
 __device__ __forceinline__ void finish_path(const EngineView& e, int p, bool valid) {
+    __shared__ uint32_t block_count;
+    __shared__ unsigned long long block_base;
     bool requeue = false;
     if (valid) {
         const int j = e.pixel_job[p];
@@ -193,21 +198,27 @@ __device__ __forceinline__ void finish_path(const EngineView& e, int p, bool val
             requeue = true;
         }
     }
-    // wave-aggregated append to the ready queue
+    if (threadIdx.x == 0) block_count = 0;
+    __syncthreads();
     const uint64_t b = __ballot(requeue);
-    if (b == 0) return;
-    const int leader = __ffsll((unsigned long long)b) - 1;
-    unsigned long long base = 0;
-    if ((int)__lane_id() == leader) base = atomicAdd(&e.stats[S_TAIL], (unsigned long long)__popcll(b));
-    base = __shfl(base, leader, 64);
+    const int lane = (int)__lane_id();
+    uint32_t wave_off = 0;
+    if (lane == 0 && b) wave_off = atomicAdd(&block_count, (uint32_t)__popcll(b));
+    wave_off = __shfl(wave_off, 0, 64);
+    __syncthreads();
+    if (threadIdx.x == 0)
+        block_base = block_count ? atomicAdd(&e.stats[S_TAIL], (unsigned long long)block_count) : 0ull;
+    __syncthreads();
     if (requeue) {
-        const unsigned long long pos = base + (unsigned long long)__popcll(b & lanemask_lt());
+        const unsigned long long pos =
+            block_base + wave_off + (unsigned long long)__popcll(b & lanemask_lt());
         e.ready[pos % (unsigned long long)e.px] = p;
     }
 }
 
-__global__ void k_finish_hits(EngineView e, const int32_t* __restrict__ pixel,
-                              const uint32_t* __restrict__ alive) {
+__global__ __launch_bounds__(kFinishBlock) void k_finish_hits(EngineView e,
+                                                              const int32_t* __restrict__ pixel,
+                                                              const uint32_t* __restrict__ alive) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = e.counters[C_HITS];
     if (blockIdx.x * blockDim.x >= n) return;
@@ -215,7 +226,8 @@ __global__ void k_finish_hits(EngineView e, const int32_t* __restrict__ pixel,
     finish_path(e, valid ? pixel[i] : 0, valid);
 }
 
-__global__ void k_finish_misses(EngineView e, const int32_t* __restrict__ pixel) {
+__global__ __launch_bounds__(kFinishBlock) void k_finish_misses(EngineView e,
+                                                                const int32_t* __restrict__ pixel) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = e.counters[C_MISSES];
     if (blockIdx.x * blockDim.x >= n) return;
@@ -361,7 +373,7 @@ wf_status enqueue_iteration(wf_engine* e) {
     const Count live{cap, c + C_LIVE}, hits{cap, c + C_HITS}, misses{cap, c + C_MISSES};
     phase_mark(e, 0);
     {
-        Timed tm(ctx, "regen", 56.0 * cap);
+        Timed tm(ctx, "regen", 56.0 * cap);   // upper bound: the new-path count is on the device
         k_regen<<<grid_for(cap, 256), 256, 0, ctx->stream>>>(v, e->r0, e->t0);
         WF_HIP(hipGetLastError());
         k_regen_commit<<<1, 1, 0, ctx->stream>>>(v);
@@ -400,10 +412,12 @@ wf_status enqueue_iteration(wf_engine* e) {
     WF_TRY(shade_misses(ctx, s, &e->miss, misses, e->cfg.shade.physics, v.accum));
     phase_mark(e, 6);
     {
-        Timed tm(ctx, "finish", 12.0 * cap);
-        k_finish_hits<<<grid_for(cap, 256), 256, 0, ctx->stream>>>(v, q->pixel, alive);
+        Timed tm(ctx, "finish", 12.0, live);
+        k_finish_hits<<<grid_for(cap, kFinishBlock), kFinishBlock, 0, ctx->stream>>>(v, q->pixel,
+                                                                                    alive);
         WF_HIP(hipGetLastError());
-        k_finish_misses<<<grid_for(cap, 256), 256, 0, ctx->stream>>>(v, e->miss.pixel);
+        k_finish_misses<<<grid_for(cap, kFinishBlock), kFinishBlock, 0, ctx->stream>>>(
+            v, e->miss.pixel);
         WF_HIP(hipGetLastError());
         k_iteration_end<<<1, 1, 0, ctx->stream>>>(v);
         WF_HIP(hipGetLastError());

@@ -48,7 +48,8 @@ struct ProfEntry {
 struct PendingTiming {
     std::string name;
     hipEvent_t start, stop;
-    double bytes;
+    double bytes;              // algorithmic bytes, or bytes per unit when count_slot >= 0
+    int count_slot;            // pinned slot receiving the launch's device-side element count
 };
 
 }  // namespace wf
@@ -71,6 +72,8 @@ struct wf_context {
     std::map<std::string, wf::ProfEntry> prof;
     std::vector<wf::PendingTiming> pending;
     std::vector<hipEvent_t> event_pool;
+    int32_t* count_slots = nullptr;    // pinned, kCountSlots entries
+    int next_count_slot = 0;
     int cu_count = 256;
 };
 
@@ -83,13 +86,19 @@ wf_status pinned(wf_context* ctx, size_t bytes, void** out);
 // Copies `bytes` from device to host through pinned memory and synchronises the stream.
 wf_status readback(wf_context* ctx, void* host, const void* dev, size_t bytes);
 
-// Brackets a kernel launch with events when profiling is on.
+struct Count;
+constexpr int kCountSlots = 16384;
+
+// Brackets a kernel launch with events when profiling is on. With a device-side element count
+// the algorithmic bytes are bytes_per_unit times the count, copied back after the launch.
 struct Timed {
     wf_context* ctx;
     const char* name;
     double bytes;
+    const int32_t* count_dev = nullptr;
     hipEvent_t a = nullptr, b = nullptr;
     Timed(wf_context* c, const char* n, double algorithmic_bytes);
+    Timed(wf_context* c, const char* n, double bytes_per_unit, const Count& count);
     ~Timed();
 };
 wf_status harvest_timings(wf_context* ctx);

@@ -49,7 +49,7 @@ wf_status ray_keys(wf_context* ctx, const wf_ray_soa* r, Count n, const float bm
         float e = hi - lo;
         return e < 1e-6f ? 1e-6f : e;
     };
-    Timed tm(ctx, "ray_keys", 32.0 * n.n);
+    Timed tm(ctx, "ray_keys", 32.0, n);
     k_ray_keys<<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(
         r->ox, r->oy, r->oz, r->dx, r->dy, r->dz, n.n, n.dev, bmin[0], bmin[1], bmin[2],
         ext(bmax[0], bmin[0]), ext(bmax[1], bmin[1]), ext(bmax[2], bmin[2]), keys);
@@ -118,7 +118,7 @@ __global__ void k_gather_shade(wf_shade_soa s, const int32_t* __restrict__ idx, 
 wf_status gather_rays(wf_context* ctx, const wf_ray_soa* src, const int32_t* idx, Count n,
                       wf_ray_soa* dst, const int32_t* tags_in, int32_t* tags_out) {
     if (n.n <= 0) return WF_OK;
-    Timed tm(ctx, "gather_rays", (tags_out ? 108.0 : 100.0) * n.n);
+    Timed tm(ctx, "gather_rays", tags_out ? 104.0 : 96.0, n);
     k_gather_rays<<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(*src, idx, n.n, n.dev, *dst,
                                                               tags_in, tags_out);
     WF_HIP(hipGetLastError());
@@ -128,7 +128,7 @@ wf_status gather_rays(wf_context* ctx, const wf_ray_soa* src, const int32_t* idx
 wf_status gather_shading(wf_context* ctx, const wf_shade_soa* src, const int32_t* idx, Count n,
                          wf_shade_soa* dst, const int32_t* tags_in, int32_t* tags_out) {
     if (n.n <= 0) return WF_OK;
-    Timed tm(ctx, "gather_shading", (tags_out ? 212.0 : 204.0) * n.n);
+    Timed tm(ctx, "gather_shading", tags_out ? 200.0 : 192.0, n);
     k_gather_shade<<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(*src, idx, n.n, n.dev, *dst,
                                                                tags_in, tags_out);
     WF_HIP(hipGetLastError());
@@ -236,7 +236,8 @@ __device__ __forceinline__ float o2f(uint32_t u) {
 }
 
 // Per-tile, per-material double sums of luminance(throughput) in a fixed order (wave chunks of 64
-// in queue order, peers summed by their lowest lane in lane order), plus hit-point bounds.
+// in queue order, peers summed by their lowest lane in lane order), presence flags and hit-point
+// bounds. Dynamic LDS: (kCbBlock / 64) * n_mat doubles + n_mat presence words.
 __global__ __launch_bounds__(kCbBlock) void k_cb_partials(
     const int32_t* __restrict__ mat, const float* __restrict__ tr, const float* __restrict__ tg,
     const float* __restrict__ tb, const float* __restrict__ px, const float* __restrict__ py,
@@ -244,11 +245,13 @@ __global__ __launch_bounds__(kCbBlock) void k_cb_partials(
     double* __restrict__ partials, uint32_t* __restrict__ present,
     uint32_t* __restrict__ bounds) {
     const int n = load_count(n_host, nd);
-    __shared__ double wsum[kCbBlock / 64][kCbMaxMaterials];
+    extern __shared__ double cb_lds[];
+    double* wsum = cb_lds;                                        // [wave][n_mat]
+    uint32_t* pres = (uint32_t*)(cb_lds + (kCbBlock / 64) * n_mat);
     __shared__ double vals[kCbBlock];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int k = threadIdx.x; k < (kCbBlock / 64) * n_mat; k += kCbBlock)
-        wsum[k / n_mat][k % n_mat] = 0.0;
+    for (int k = threadIdx.x; k < (kCbBlock / 64) * n_mat; k += kCbBlock) wsum[k] = 0.0;
+    for (int k = threadIdx.x; k < n_mat; k += kCbBlock) pres[k] = 0u;
     __syncthreads();
     const uint64_t lt = lanemask_lt();
     float mn[3] = {3.0e38f, 3.0e38f, 3.0e38f}, mx[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
@@ -273,24 +276,27 @@ __global__ __launch_bounds__(kCbBlock) void k_cb_partials(
         }
         __builtin_amdgcn_wave_barrier();
         if (valid && (peers & lt) == 0) {
-            double s = wsum[wave][m];
+            double acc = wsum[wave * n_mat + m];
             uint64_t p = peers;
             while (p) {
                 const int l = __ffsll((unsigned long long)p) - 1;
-                s += vals[wave * 64 + l];
+                acc += vals[wave * 64 + l];
                 p &= p - 1;
             }
-            wsum[wave][m] = s;
-            atomicOr(&present[m], 1u);
+            wsum[wave * n_mat + m] = acc;
+            pres[m] = 1u;
         }
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
     for (int m = threadIdx.x; m < n_mat; m += kCbBlock) {
-        double s = 0.0;
-        for (int w = 0; w < kCbBlock / 64; ++w) s += wsum[w][m];
-        partials[(size_t)blockIdx.x * n_mat + m] = s;
+        double acc = 0.0;
+        for (int w = 0; w < kCbBlock / 64; ++w) acc += wsum[w * n_mat + m];
+        partials[(size_t)blockIdx.x * n_mat + m] = acc;
+        if (pres[m]) atomicOr(&present[m], 1u);
     }
+    // hit-point bounds: wave reduction, block reduction, one atomic per bound per block
+    __shared__ float wb[kCbBlock / 64][6];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         float a = mn[k], b = mx[k];
@@ -299,9 +305,17 @@ __global__ __launch_bounds__(kCbBlock) void k_cb_partials(
             b = fmaxf(b, __shfl_xor(b, o, 64));
         }
         if (lane == 0) {
-            atomicMin(&bounds[k], f2o(a));
-            atomicMax(&bounds[3 + k], f2o(b));
+            wb[wave][k] = a;
+            wb[wave][3 + k] = b;
         }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        float r = wb[0][k];
+        for (int w = 1; w < kCbBlock / 64; ++w) r = k < 3 ? fminf(r, wb[w][k]) : fmaxf(r, wb[w][k]);
+        if (k < 3) atomicMin(&bounds[k], f2o(r));
+        else atomicMax(&bounds[k], f2o(r));
     }
 }
 
@@ -371,7 +385,7 @@ wf_status shading_keys(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q
         TypeStats* st = (TypeStats*)p;
         WF_TRY(wf_fill_u32(ctx, st, 0u, sizeof(TypeStats) / 4));
         {
-            Timed tm(ctx, "policy_keys", (policy == WF_POLICY_MATERIAL ? 4.0 : 8.0) * n.n);
+            Timed tm(ctx, "policy_keys", policy == WF_POLICY_MATERIAL ? 8.0 : 12.0, n);
             int blocks = (int)std::min<int64_t>(grid_for(n.n, 256), 1024);
             k_type_stats<<<blocks, 256, 0, ctx->stream>>>(q->mat, q->tex, n.n, n.dev,
                                                           d.materials, d.tex_mem_bytes,
@@ -403,8 +417,9 @@ wf_status shading_keys(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q
         WF_TRY(wf_fill_u32(ctx, ppres, 0u, (size_t)nm));
         WF_TRY(wf_fill_u32(ctx, pb, 0xFFFFFFFFu, 3));
         WF_TRY(wf_fill_u32(ctx, (uint32_t*)pb + 3, 0u, 5));
-        Timed tm(ctx, "policy_keys", 32.0 * n.n);
-        k_cb_partials<<<tiles, kCbBlock, 0, ctx->stream>>>(q->mat, q->tr, q->tg, q->tb, q->px,
+        Timed tm(ctx, "policy_keys", 32.0, n);
+        const size_t lds = sizeof(double) * (kCbBlock / 64) * (size_t)nm + sizeof(uint32_t) * nm;
+        k_cb_partials<<<tiles, kCbBlock, lds, ctx->stream>>>(q->mat, q->tr, q->tg, q->tb, q->px,
                                                            q->py, q->pz, n.n, n.dev, nm,
                                                            (double*)pp, (uint32_t*)ppres,
                                                            (uint32_t*)pb);

@@ -706,7 +706,7 @@ wf_status shade_bounce(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q
     ShadeArgs a{p->physics, p->rr_min_depth, p->cost_aware_rr, p->firefly_threshold, p->seed,
                 p->sample_index, p->job};
     {
-        Timed tm(ctx, "shade", 120.0 * n.n);
+        Timed tm(ctx, "shade", 120.0, n);
         k_shade<<<grid_for(n.n, 128), 128, 0, ctx->stream>>>(s->dev, *q, n.n, n.dev, a,
                                                              s->dev.mat_rel_cost, accum, o,
                                                              jobs.tags, jobs.jobs);
@@ -735,7 +735,7 @@ wf_status shade_bounce(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q
         k_count_shadow_rays<<<grid_for(ns.n, 256), 256, 0, ctx->stream>>>(
             o.shadow_flag, (int32_t*)plist, ns.n, ns.dev, counts + 1);
         WF_HIP(hipGetLastError());
-        Timed tm(ctx, "shadow", 41.0 * ns.n);
+        Timed tm(ctx, "shadow", 41.0, ns);
         with_stack(s->dev.stack_bound, [&](auto S) {
             k_shadow<decltype(S)::value>
                 <<<grid_for(ns.n, kTraceBlock), kTraceBlock, 0, ctx->stream>>>(
@@ -746,7 +746,7 @@ wf_status shade_bounce(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q
     // next rays: stable compaction in shading order
     WF_TRY(exclusive_scan_u32(ctx, o.alive, (uint32_t*)pprefix, n, counts, "compact"));
     {
-        Timed tm(ctx, "next_rays", 76.0 * n.n);
+        Timed tm(ctx, "next_rays", 76.0, n);
         k_next_rays<<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(
             *q, o, (uint32_t*)pprefix, n.n, n.dev, *next, jobs.tags, jobs.next_tags);
         WF_HIP(hipGetLastError());
@@ -758,7 +758,7 @@ wf_status shade_bounce(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q
 wf_status shade_misses(wf_context* ctx, const wf_scene* s, const wf_ray_soa* q, Count n,
                        int32_t physics, float* accum) {
     if (n.n <= 0) return WF_OK;
-    Timed tm(ctx, "miss", 52.0 * n.n);
+    Timed tm(ctx, "miss", 52.0, n);
     k_miss<<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(s->dev, *q, n.n, n.dev, physics, accum);
     WF_HIP(hipGetLastError());
     return WF_OK;

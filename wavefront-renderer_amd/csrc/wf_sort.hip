@@ -258,7 +258,7 @@ static wf_status scan_level(wf_context* ctx, const uint32_t* in, uint32_t* out, 
                             int32_t* total, int level, const char* tag) {
     const int tiles = (n.n + kScanTile - 1) / kScanTile;
     if (tiles <= 1) {
-        Timed t(ctx, tag, 8.0 * n.n);
+        Timed t(ctx, tag, 8.0, n);
         k_scan_down<<<1, kScanBlock, 0, ctx->stream>>>(in, out, n.n, n.dev, nullptr, total);
         WF_HIP(hipGetLastError());
         return WF_OK;
@@ -269,13 +269,13 @@ static wf_status scan_level(wf_context* ctx, const uint32_t* in, uint32_t* out, 
     WF_TRY(workspace(ctx, name, sizeof(uint32_t) * (size_t)tiles, &p));
     uint32_t* sums = (uint32_t*)p;
     {
-        Timed t(ctx, tag, 4.0 * n.n);
+        Timed t(ctx, tag, 4.0, n);
         k_scan_reduce<<<tiles, kScanBlock, 0, ctx->stream>>>(in, n.n, n.dev, sums);
         WF_HIP(hipGetLastError());
     }
     WF_TRY(scan_level(ctx, sums, sums, host_count(tiles), nullptr, level + 1, tag));
     {
-        Timed t(ctx, tag, 8.0 * n.n);
+        Timed t(ctx, tag, 8.0, n);
         k_scan_down<<<tiles, kScanBlock, 0, ctx->stream>>>(in, out, n.n, n.dev, sums, total);
         WF_HIP(hipGetLastError());
     }
@@ -373,14 +373,14 @@ wf_status radix_sort_pairs(wf_context* ctx, const uint64_t* keys_in, const int32
         uint64_t* kdst = to_out ? keys_out : (uint64_t*)pk;
         int32_t* vdst = to_out ? vals_out : (int32_t*)pv;
         {
-            Timed t(ctx, tag, 8.0 * n.n);
+            Timed t(ctx, tag, 8.0, n);
             k_radix_hist<<<tiles, kSortBlock, 0, ctx->stream>>>(ksrc, n.n, n.dev, shift, mask,
                                                                  hist, tiles);
             WF_HIP(hipGetLastError());
         }
         WF_TRY(exclusive_scan_u32(ctx, hist, hist, host_count(256 * tiles), nullptr, tag));
         {
-            Timed t(ctx, tag, 24.0 * n.n);
+            Timed t(ctx, tag, 24.0, n);
             k_radix_scatter<<<tiles, kSortBlock, 0, ctx->stream>>>(
                 ksrc, vsrc, kdst, vdst, n.n, n.dev, shift, mask, nbits, hist, tiles);
             WF_HIP(hipGetLastError());

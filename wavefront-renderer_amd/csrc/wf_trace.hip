@@ -55,6 +55,23 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_occluded(
     out[i] = occ ? 1 : 0;
 }
 
+// Traversal statistics: BVH nodes visited and triangles tested per ray (lanes of a wave are rays
+// 64w..64w+63, so SIMD efficiency follows from the per-ray counts).
+template <int STACK, bool ANY>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_stats(
+    const Bvh4Node* __restrict__ nodes, const float4* __restrict__ tri, int n_tris,
+    wf_ray_soa r, int n, float tnear, float tfar, int32_t* __restrict__ out_nodes,
+    int32_t* __restrict__ out_tris) {
+    __shared__ int32_t stack[lds_entries<STACK>() * kTraceBlock];
+    const int i = blockIdx.x * kTraceBlock + threadIdx.x;
+    if (i >= n) return;
+    TraceHit h;
+    traverse<ANY, STACK, true>(nodes, tri, n_tris, V3(r.ox[i], r.oy[i], r.oz[i]),
+                               V3(r.dx[i], r.dy[i], r.dz[i]), tnear, tfar, stack, h);
+    out_nodes[i] = h.nodes;
+    out_tris[i] = h.tris;
+}
+
 // HitRecord mapping of EmbreeAccelerator::intersect4 (Embree.cpp:240-252) for ray i into slot j.
 __device__ __forceinline__ void map_hit(const SceneDev& s, const wf_ray_soa& in, int i, int L,
                                         float t, float bu, float bv, wf_shade_soa& out, int j) {
@@ -174,7 +191,7 @@ wf_status trace_closest(wf_context* ctx, const wf_scene* s, const wf_ray_soa* r,
                         float* hit_bv, uint32_t* flags) {
     WF_TRY(scene_ready(s));
     if (n.n <= 0) return WF_OK;
-    Timed tm(ctx, "traverse_closest", 44.0 * n.n);
+    Timed tm(ctx, "traverse_closest", 44.0, n);
     with_stack(s->dev.stack_bound, [&](auto S) {
         k_trace_closest<decltype(S)::value>
             <<<grid_for(n.n, kTraceBlock), kTraceBlock, 0, ctx->stream>>>(
@@ -216,7 +233,7 @@ wf_status intersect_split(wf_context* ctx, const wf_scene* s, const wf_ray_soa* 
                          (float*)pv, (uint32_t*)pf));
     WF_TRY(exclusive_scan_u32(ctx, (uint32_t*)pf, (uint32_t*)pf, n, counts_dev, "split"));
     if (n.n > 0) {
-        Timed tm(ctx, "split", 97.0 * n.n);
+        Timed tm(ctx, "split", 97.0, n);
         k_split<<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(
             s->dev, *rays, n.n, n.dev, (int32_t*)pl, (float*)pt, (float*)pu, (float*)pv,
             (uint32_t*)pf, *hits_out, *misses_out, tags_in, hit_tags);
@@ -294,6 +311,32 @@ wf_status wf_occluded(const wf_scene* s, const float* ox, const float* oy, const
     WF_REQUIRE(n >= 0, "wf_occluded: negative ray count");
     WF_TRY(trace_occluded(s, ox, oy, oz, dx, dy, dz, max_dist, n, occluded));
     WF_HIP(hipStreamSynchronize(s->ctx->stream));
+    return WF_OK;
+}
+
+wf_status wf_trace_stats(const wf_scene* s, const wf_ray_soa* rays, int32_t n, float tnear,
+                         float tfar, int32_t any_hit, int32_t* nodes_visited,
+                         int32_t* triangles_tested) {
+    clear_error();
+    WF_REQUIRE(s && rays && nodes_visited && triangles_tested, "wf_trace_stats: NULL argument");
+    WF_REQUIRE(n >= 0, "wf_trace_stats: negative ray count");
+    WF_TRY(scene_ready(s));
+    if (n == 0) return WF_OK;
+    wf_context* ctx = s->ctx;
+    with_stack(s->dev.stack_bound, [&](auto S) {
+        if (any_hit)
+            k_trace_stats<decltype(S)::value, true>
+                <<<grid_for(n, kTraceBlock), kTraceBlock, 0, ctx->stream>>>(
+                    s->dev.nodes4, s->dev.tri, s->dev.n_tris, *rays, n, tnear, tfar,
+                    nodes_visited, triangles_tested);
+        else
+            k_trace_stats<decltype(S)::value, false>
+                <<<grid_for(n, kTraceBlock), kTraceBlock, 0, ctx->stream>>>(
+                    s->dev.nodes4, s->dev.tri, s->dev.n_tris, *rays, n, tnear, tfar,
+                    nodes_visited, triangles_tested);
+    });
+    WF_HIP(hipGetLastError());
+    WF_HIP(hipStreamSynchronize(ctx->stream));
     return WF_OK;
 }
 

@@ -76,6 +76,7 @@ struct TraceHit {
     int32_t leaf;   // leaf (sorted) triangle index, -1 = none
     int32_t g;      // global triangle index
     float u, v;
+    int32_t nodes = 0, tris = 0;   // visit counts (STATS instantiations only)
 };
 
 // Walks the four-wide BVH. At each node the four child boxes are tested; leaf children are
@@ -83,7 +84,7 @@ struct TraceHit {
 // near). ANY stops at the first accepted triangle. The stack never holds more than the build's
 // stack bound, which the caller's STACK covers; `stack` is the block's LDS column array of
 // lds_entries<STACK>() rows.
-template <bool ANY, int STACK>
+template <bool ANY, int STACK, bool STATS = false>
 __device__ __forceinline__ bool traverse(const Bvh4Node* __restrict__ nodes,
                                          const float4* __restrict__ tri, int n_tris,
                                          const V3& o, const V3& d, float tnear, float tfar,
@@ -102,6 +103,7 @@ __device__ __forceinline__ bool traverse(const Bvh4Node* __restrict__ nodes,
     int sp = 0;
     int node = 0;
     for (;;) {
+        if (STATS) ++h.nodes;
         const Bvh4Node* nd = nodes + node;
         const float4 lox = nd->lox, loy = nd->loy, loz = nd->loz;
         const float4 hix = nd->hix, hiy = nd->hiy, hiz = nd->hiz;
@@ -126,6 +128,7 @@ __device__ __forceinline__ bool traverse(const Bvh4Node* __restrict__ nodes,
                 continue;
             }
             const int leaf = ~c[k];
+            if (STATS) ++h.tris;
             const float4 a = tri[3 * (size_t)leaf];
             const float4 b = tri[3 * (size_t)leaf + 1];
             const float4 e = tri[3 * (size_t)leaf + 2];
