@@ -79,11 +79,35 @@ struct TraceHit {
     int32_t nodes = 0, tris = 0;   // visit counts (STATS instantiations only)
 };
 
+// Moller-Trumbore without early exits: every quantity is computed and `ok` is the conjunction of
+// the negated early-return conditions of mt_test, so accepted triangles get the same t, u, v.
+// Computing all of them keeps the triangle loads unconditional (one memory round trip for all
+// candidate triangles of a node).
+__device__ __forceinline__ bool mt_all(const V3& o, const V3& d, const V3& v0, const V3& e1,
+                                       const V3& e2, float tmin, float tmax, float& t, float& u,
+                                       float& v) {
+    const float EPSILON = 0.0000001f;
+    const V3 pvec = d.cross(e2);
+    const float det = e1.dot(pvec);
+    const float inv = 1.0f / det;
+    const V3 tvec = o - v0;
+    u = inv * tvec.dot(pvec);
+    const V3 qvec = tvec.cross(e1);
+    v = inv * d.dot(qvec);
+    t = inv * e2.dot(qvec);
+    const bool det_ok = !(det > -EPSILON && det < EPSILON);
+    const bool u_ok = !(u < 0.0f || u > 1.0f);
+    const bool v_ok = !(v < 0.0f || u + v > 1.0f);
+    const bool t_ok = !(t < tmin || t > tmax);
+    return det_ok && u_ok && v_ok && t_ok;
+}
+
 // Walks the four-wide BVH. At each node the four child boxes are tested; leaf children are
 // tested at once, the hit inner children are visited nearest first (the others are pushed far to
 // near). ANY stops at the first accepted triangle. The stack never holds more than the build's
 // stack bound, which the caller's STACK covers; `stack` is the block's LDS column array of
-// lds_entries<STACK>() rows.
+// lds_entries<STACK>() rows. All child boxes of a node, and all candidate triangles of a node,
+// are loaded and tested together so that each costs a single memory round trip.
 template <bool ANY, int STACK, bool STATS = false>
 __device__ __forceinline__ bool traverse(const Bvh4Node* __restrict__ nodes,
                                          const float4* __restrict__ tri, int n_tris,
@@ -104,10 +128,10 @@ __device__ __forceinline__ bool traverse(const Bvh4Node* __restrict__ nodes,
     int node = 0;
     for (;;) {
         if (STATS) ++h.nodes;
-        const Bvh4Node* nd = nodes + node;
-        const float4 lox = nd->lox, loy = nd->loy, loz = nd->loz;
-        const float4 hix = nd->hix, hiy = nd->hiy, hiz = nd->hiz;
-        const int4 ch = nd->child;
+        const float4* np = reinterpret_cast<const float4*>(nodes + node);
+        const float4 lox = np[0], loy = np[1], loz = np[2];
+        const float4 hix = np[3], hiy = np[4], hiz = np[5];
+        const int4 ch = *reinterpret_cast<const int4*>(np + 6);
         const float lx[4] = {lox.x, lox.y, lox.z, lox.w};
         const float ly[4] = {loy.x, loy.y, loy.z, loy.w};
         const float lz[4] = {loz.x, loz.y, loz.z, loz.w};
@@ -115,38 +139,51 @@ __device__ __forceinline__ bool traverse(const Bvh4Node* __restrict__ nodes,
         const float hy[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
         const float hz[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
         int c[4] = {ch.x, ch.y, ch.z, ch.w};
+        const float tcur = ANY ? tfar : h.t;
         float dist[4];
+        bool leaf_hit[4];
+        bool any_leaf = false;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            dist[k] = kInf;
-            if (c[k] == kEmptyChild) continue;
             float t0;
-            if (!slab(lx[k], ly[k], lz[k], hx[k], hy[k], hz[k], o, inv, ANY ? tfar : h.t, t0))
-                continue;
-            if (c[k] >= 0) {
-                dist[k] = t0;
-                continue;
+            const bool box = slab(lx[k], ly[k], lz[k], hx[k], hy[k], hz[k], o, inv, tcur, t0) &&
+                             c[k] != kEmptyChild;
+            leaf_hit[k] = box && c[k] < 0;
+            any_leaf = any_leaf || leaf_hit[k];
+            dist[k] = (box && c[k] >= 0) ? t0 : kInf;
+        }
+        if (any_leaf) {
+            // every candidate triangle of this node in one round trip (non-candidates re-read
+            // the first candidate and are ignored)
+            const int first = leaf_hit[0] ? ~c[0] : (leaf_hit[1] ? ~c[1] : (leaf_hit[2] ? ~c[2] : ~c[3]));
+            float4 A[4], B[4], C[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const size_t li = 3 * (size_t)(leaf_hit[k] ? ~c[k] : first);
+                A[k] = tri[li];
+                B[k] = tri[li + 1];
+                C[k] = tri[li + 2];
             }
-            const int leaf = ~c[k];
-            if (STATS) ++h.tris;
-            const float4 a = tri[3 * (size_t)leaf];
-            const float4 b = tri[3 * (size_t)leaf + 1];
-            const float4 e = tri[3 * (size_t)leaf + 2];
-            float t, u, v;
-            if (!mt_test(o, d, xyz(a), xyz(b), xyz(e), tnear, tfar, t, u, v)) continue;
-            const int32_t g = __float_as_int(a.w);
-            if (ANY) {
-                h.t = t;
-                h.leaf = leaf;
-                h.g = g;
-                return true;
-            }
-            if (h.leaf < 0 || t < h.t || (t == h.t && g < h.g)) {
-                h.t = t;
-                h.leaf = leaf;
-                h.g = g;
-                h.u = u;
-                h.v = v;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                float t, u, v;
+                const bool ok = mt_all(o, d, xyz(A[k]), xyz(B[k]), xyz(C[k]), tnear, tfar, t, u, v);
+                if (STATS && leaf_hit[k]) ++h.tris;
+                if (!(leaf_hit[k] && ok)) continue;
+                const int32_t g = __float_as_int(A[k].w);
+                if (ANY) {
+                    h.t = t;
+                    h.leaf = ~c[k];
+                    h.g = g;
+                    return true;
+                }
+                if (h.leaf < 0 || t < h.t || (t == h.t && g < h.g)) {
+                    h.t = t;
+                    h.leaf = ~c[k];
+                    h.g = g;
+                    h.u = u;
+                    h.v = v;
+                }
             }
         }
         // order the inner children nearest first (misses and leaves sort last as +inf)
