@@ -626,35 +626,77 @@ template <int STACK>
 __global__ __launch_bounds__(kTraceBlock) void k_shadow(const SceneDev s, wf_shade_soa q, ShadeOut o,
                                                         const int32_t* __restrict__ list, int count_host,
                                                         const int32_t* __restrict__ count_dev,
-                                                        float* __restrict__ accum) {
+                                                        float* __restrict__ accum,
+                                                        int32_t* __restrict__ counter) {
     __shared__ int32_t stack[lds_entries<STACK>() * kTraceBlock];
-    const int k = blockIdx.x * kTraceBlock + threadIdx.x;
-    if (k >= load_count(count_host, count_dev)) return;
-    const int i = list[k];
-    const uint32_t f = o.shadow_flag[i];
-    const V3 P = ld3(q.px, q.py, q.pz, i);
-    float* acc = accum + (size_t)q.pixel[i] * 3;
-    if (f & 1u) {
-        const float tfar = o.lmax[i] - 0.001f;
-        TraceHit h;
-        const bool occ = tfar < 0.0f ||
-                         traverse<true, STACK>(s.nodes4, s.tri, s.n_tris, P, ld3(o.ldx, o.ldy, o.ldz, i),
-                                        0.001f, tfar, stack, h);
-        if (!occ) {
-            acc[0] = acc[0] + o.lr[i];
-            acc[1] = acc[1] + o.lg[i];
-            acc[2] = acc[2] + o.lb[i];
+    using W = Walker<true, STACK>;
+    volatile int32_t spill[W::X > 0 ? W::X : 1];
+    const int n = load_count(count_host, count_dev);
+    const V3 tl = V3(s.directional.direction[0], s.directional.direction[1],
+                     s.directional.direction[2]) * -1.0f;
+    W w;
+    // per lane: one shadow entry at a time, its area/environment ray (phase 0) then its
+    // directional ray (phase 1); phase 2 = both resolved
+    int item = -1, i = 0, phase = 2;
+    uint32_t f = 0;
+    bool occ_area = false, occ_dir = false, exhausted = false;
+    for (;;) {
+        const int r = refill(counter, n, item < 0, exhausted);
+        if (r >= 0) {
+            item = r;
+            i = list[r];
+            f = o.shadow_flag[i];
+            occ_area = false;
+            occ_dir = false;
+            phase = 2;
+            const V3 P = ld3(q.px, q.py, q.pz, i);
+            if (f & 1u) {
+                const float tfar = o.lmax[i] - 0.001f;
+                if (tfar < 0.0f) {
+                    occ_area = true;
+                } else {
+                    phase = 0;
+                    w.start(P, ld3(o.ldx, o.ldy, o.ldz, i), 0.001f, tfar);
+                }
+            }
+            if (phase == 2 && (f & 2u)) {
+                phase = 1;
+                w.start(P, tl, 0.001f, 1e6f - 0.001f);
+            }
         }
-    }
-    if (f & 2u) {
-        const V3 tl = V3(s.directional.direction[0], s.directional.direction[1],
-                         s.directional.direction[2]) * -1.0f;
-        TraceHit h;
-        const bool occ = traverse<true, STACK>(s.nodes4, s.tri, s.n_tris, P, tl, 0.001f, 1e6f - 0.001f, stack, h);
-        if (!occ) {
-            acc[0] = acc[0] + o.dr[i];
-            acc[1] = acc[1] + o.dg[i];
-            acc[2] = acc[2] + o.db[i];
+        if (__ballot(item >= 0) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        if (item < 0) continue;
+        if (phase != 2 && (s.n_tris <= 0 || w.step(s.nodes4, s.tri, stack, spill))) {
+            const bool occ = w.h.leaf >= 0;
+            if (phase == 0) {
+                occ_area = occ;
+                phase = 2;
+                if (f & 2u) {
+                    phase = 1;
+                    w.start(ld3(q.px, q.py, q.pz, i), tl, 0.001f, 1e6f - 0.001f);
+                }
+            } else {
+                occ_dir = occ;
+                phase = 2;
+            }
+        }
+        if (phase == 2) {
+            // contributions in the reference order: area/environment light, then directional
+            float* acc = accum + (size_t)q.pixel[i] * 3;
+            if ((f & 1u) && !occ_area) {
+                acc[0] = acc[0] + o.lr[i];
+                acc[1] = acc[1] + o.lg[i];
+                acc[2] = acc[2] + o.lb[i];
+            }
+            if ((f & 2u) && !occ_dir) {
+                acc[0] = acc[0] + o.dr[i];
+                acc[1] = acc[1] + o.dg[i];
+                acc[2] = acc[2] + o.db[i];
+            }
+            item = -1;
         }
     }
 }
@@ -735,11 +777,14 @@ wf_status shade_bounce(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q
         k_count_shadow_rays<<<grid_for(ns.n, 256), 256, 0, ctx->stream>>>(
             o.shadow_flag, (int32_t*)plist, ns.n, ns.dev, counts + 1);
         WF_HIP(hipGetLastError());
+        void* pc = nullptr;
+        WF_TRY(workspace(ctx, "shadow_counter", sizeof(int32_t), &pc));
+        WF_TRY(wf_fill_u32(ctx, pc, 0u, 1));
         Timed tm(ctx, "shadow", 41.0, ns);
         with_stack(s->dev.stack_bound, [&](auto S) {
             k_shadow<decltype(S)::value>
-                <<<grid_for(ns.n, kTraceBlock), kTraceBlock, 0, ctx->stream>>>(
-                    s->dev, *q, o, (int32_t*)plist, ns.n, ns.dev, accum);
+                <<<persistent_grid(ns.n, ctx->cu_count), kTraceBlock, 0, ctx->stream>>>(
+                    s->dev, *q, o, (int32_t*)plist, ns.n, ns.dev, accum, (int32_t*)pc);
         });
         WF_HIP(hipGetLastError());
     }

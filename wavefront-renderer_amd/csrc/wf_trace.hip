@@ -13,6 +13,8 @@
 
 namespace wf {
 
+// Persistent closest-hit kernel: each lane walks one ray at a time and takes a new ray as soon as
+// its walk ends (dynamic refill keeps lanes busy while the slowest rays of a wave finish).
 template <int STACK>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(
     const Bvh4Node* __restrict__ nodes, const float4* __restrict__ tri, int n_tris,
@@ -20,18 +22,33 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(
     const float* __restrict__ dx, const float* __restrict__ dy, const float* __restrict__ dz,
     int n_host, const int32_t* __restrict__ nd, float tnear, float tfar,
     int32_t* __restrict__ out_leaf, float* __restrict__ out_t, float* __restrict__ out_u,
-    float* __restrict__ out_v, uint32_t* __restrict__ out_flag) {
+    float* __restrict__ out_v, uint32_t* __restrict__ out_flag, int32_t* __restrict__ counter) {
     __shared__ int32_t stack[lds_entries<STACK>() * kTraceBlock];
-    const int i = blockIdx.x * kTraceBlock + threadIdx.x;
-    if (i >= load_count(n_host, nd)) return;
-    TraceHit h;
-    traverse<false, STACK>(nodes, tri, n_tris, V3(ox[i], oy[i], oz[i]), V3(dx[i], dy[i], dz[i]), tnear,
-                    tfar, stack, h);
-    out_leaf[i] = h.leaf;
-    out_t[i] = h.t;
-    out_u[i] = h.u;
-    out_v[i] = h.v;
-    if (out_flag) out_flag[i] = h.leaf >= 0 ? 1u : 0u;
+    using W = Walker<false, STACK>;
+    volatile int32_t spill[W::X > 0 ? W::X : 1];
+    const int n = load_count(n_host, nd);
+    W w;
+    int ray = -1;
+    bool exhausted = false;
+    for (;;) {
+        const int r = refill(counter, n, ray < 0, exhausted);
+        if (r >= 0) {
+            ray = r;
+            w.start(V3(ox[r], oy[r], oz[r]), V3(dx[r], dy[r], dz[r]), tnear, tfar);
+        }
+        if (__ballot(ray >= 0) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        if (ray >= 0 && (n_tris <= 0 || w.step(nodes, tri, stack, spill))) {
+            out_leaf[ray] = w.h.leaf;
+            out_t[ray] = w.h.t;
+            out_u[ray] = w.h.u;
+            out_v[ray] = w.h.v;
+            if (out_flag) out_flag[ray] = w.h.leaf >= 0 ? 1u : 0u;
+            ray = -1;
+        }
+    }
 }
 
 // occluded(): tnear 0.001, tfar = maxDistance - 0.001; a negative tfar reports occluded
@@ -191,12 +208,16 @@ wf_status trace_closest(wf_context* ctx, const wf_scene* s, const wf_ray_soa* r,
                         float* hit_bv, uint32_t* flags) {
     WF_TRY(scene_ready(s));
     if (n.n <= 0) return WF_OK;
+    void* pc = nullptr;
+    WF_TRY(workspace(ctx, "trace_counter", sizeof(int32_t), &pc));
+    WF_TRY(wf_fill_u32(ctx, pc, 0u, 1));
     Timed tm(ctx, "traverse_closest", 44.0, n);
     with_stack(s->dev.stack_bound, [&](auto S) {
         k_trace_closest<decltype(S)::value>
-            <<<grid_for(n.n, kTraceBlock), kTraceBlock, 0, ctx->stream>>>(
+            <<<persistent_grid(n.n, ctx->cu_count), kTraceBlock, 0, ctx->stream>>>(
                 s->dev.nodes4, s->dev.tri, s->dev.n_tris, r->ox, r->oy, r->oz, r->dx, r->dy,
-                r->dz, n.n, n.dev, tnear, tfar, hit_leaf, hit_t, hit_bu, hit_bv, flags);
+                r->dz, n.n, n.dev, tnear, tfar, hit_leaf, hit_t, hit_bu, hit_bv, flags,
+                (int32_t*)pc);
     });
     WF_HIP(hipGetLastError());
     return WF_OK;

@@ -102,31 +102,43 @@ __device__ __forceinline__ bool mt_all(const V3& o, const V3& d, const V3& v0, c
     return det_ok && u_ok && v_ok && t_ok;
 }
 
-// Walks the four-wide BVH. At each node the four child boxes are tested; leaf children are
-// tested at once, the hit inner children are visited nearest first (the others are pushed far to
-// near). ANY stops at the first accepted triangle. The stack never holds more than the build's
-// stack bound, which the caller's STACK covers; `stack` is the block's LDS column array of
-// lds_entries<STACK>() rows. All child boxes of a node, and all candidate triangles of a node,
-// are loaded and tested together so that each costs a single memory round trip.
+// Resumable walk over the four-wide BVH. At each node the four child boxes are tested; leaf
+// children are tested at once, the hit inner children are visited nearest first (the others are
+// pushed far to near). ANY stops at the first accepted triangle. The stack never holds more than
+// the build's stack bound, which STACK covers: the first lds_entries<STACK>() entries live in the
+// block's LDS column array (stack[entry * kTraceBlock + lane]), deeper ones in `spill`. All child
+// boxes of a node, and all candidate triangles of a node, are loaded and tested together so that
+// each costs a single memory round trip.
 template <bool ANY, int STACK, bool STATS = false>
-__device__ __forceinline__ bool traverse(const Bvh4Node* __restrict__ nodes,
-                                         const float4* __restrict__ tri, int n_tris,
-                                         const V3& o, const V3& d, float tnear, float tfar,
-                                         int32_t* __restrict__ stack, TraceHit& h) {
-    h.t = tfar;
-    h.leaf = -1;
-    h.g = 0x7FFFFFFF;
-    h.u = h.v = 0.0f;
-    if (n_tris <= 0) return false;
-    const V3 inv(safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z));
-    const int lane = threadIdx.x;
-    const float kInf = __int_as_float(0x7F800000);
-    constexpr int L = lds_entries<STACK>();
-    constexpr int X = STACK - L;
-    volatile int32_t spill[X > 0 ? X : 1];   // private (scratch) memory, rarely touched
-    int sp = 0;
-    int node = 0;
-    for (;;) {
+struct Walker {
+    static constexpr int L = lds_entries<STACK>();
+    static constexpr int X = STACK - L;
+    V3 o, d, inv;
+    float tnear, tfar;
+    TraceHit h;
+    int node, sp;
+
+    __device__ __forceinline__ void start(const V3& ro, const V3& rd, float t0, float t1) {
+        o = ro;
+        d = rd;
+        inv = V3(safe_rcp(rd.x), safe_rcp(rd.y), safe_rcp(rd.z));
+        tnear = t0;
+        tfar = t1;
+        h.t = t1;
+        h.leaf = -1;
+        h.g = 0x7FFFFFFF;
+        h.u = h.v = 0.0f;
+        h.nodes = h.tris = 0;
+        node = 0;
+        sp = 0;
+    }
+
+    // Processes one node; returns true when the walk is over (h holds the result).
+    __device__ __forceinline__ bool step(const Bvh4Node* __restrict__ nodes,
+                                         const float4* __restrict__ tri,
+                                         int32_t* __restrict__ stack, volatile int32_t* spill) {
+        const int lane = threadIdx.x;
+        const float kInf = __int_as_float(0x7F800000);
         if (STATS) ++h.nodes;
         const float4* np = reinterpret_cast<const float4*>(nodes + node);
         const float4 lox = np[0], loy = np[1], loz = np[2];
@@ -217,13 +229,56 @@ __device__ __forceinline__ bool traverse(const Bvh4Node* __restrict__ nodes,
             }
         if (dist[0] < kInf) {
             node = c[0];
-        } else {
-            if (sp == 0) break;
-            --sp;
-            node = (X == 0 || sp < L) ? stack[sp * kTraceBlock + lane] : spill[sp - L];
+            return false;
+        }
+        if (sp == 0) return true;
+        --sp;
+        node = (X == 0 || sp < L) ? stack[sp * kTraceBlock + lane] : spill[sp - L];
+        return false;
+    }
+};
+
+// Whole walk of one ray (callers without dynamic ray refill).
+template <bool ANY, int STACK, bool STATS = false>
+__device__ __forceinline__ bool traverse(const Bvh4Node* __restrict__ nodes,
+                                         const float4* __restrict__ tri, int n_tris,
+                                         const V3& o, const V3& d, float tnear, float tfar,
+                                         int32_t* __restrict__ stack, TraceHit& h) {
+    Walker<ANY, STACK, STATS> w;
+    w.start(o, d, tnear, tfar);
+    if (n_tris > 0) {
+        volatile int32_t spill[Walker<ANY, STACK, STATS>::X > 0 ? Walker<ANY, STACK, STATS>::X : 1];
+        while (!w.step(nodes, tri, stack, spill)) {
         }
     }
+    h = w.h;
     return h.leaf >= 0;
+}
+
+// Dynamic ray refill for persistent traversal kernels: lanes whose ray finished take the next
+// unclaimed ray indices from a device work counter, one atomic per wave, once at least
+// kRefillLanes lanes are idle (or every lane is). Returns the new ray index or -1.
+constexpr int kRefillLanes = 16;
+__device__ __forceinline__ int refill(int32_t* __restrict__ counter, int n, bool idle,
+                                      bool& exhausted) {
+    const uint64_t want = __ballot(idle && !exhausted);
+    const int k = __popcll(want);
+    if (k == 0 || (k < kRefillLanes && __ballot(!idle) != 0)) return -1;
+    const int leader = __ffsll((unsigned long long)want) - 1;
+    int base = 0;
+    if ((int)__lane_id() == leader) base = atomicAdd(counter, k);
+    base = __shfl(base, leader, 64);
+    if (base + k >= n) exhausted = true;
+    if (!idle) return -1;
+    const int r = base + __popcll(want & lanemask_lt());
+    return r < n ? r : -1;
+}
+
+// Persistent-grid size for traversal kernels: enough resident blocks to fill every CU.
+inline unsigned persistent_grid(int n, int cu_count) {
+    const unsigned need = grid_for(n, kTraceBlock);
+    const unsigned fill = (unsigned)cu_count * 16u;
+    return need < fill ? need : fill;
 }
 
 }  // namespace wf
