@@ -427,6 +427,17 @@ wf_status wf_engine_query(wf_engine* engine, wf_engine_stats* out);
 /* Per-phase event timing of every iteration (disables graph replay while on). */
 wf_status wf_engine_phase_timing(wf_engine* engine, int32_t enable);
 
+/* ---- launch tuning (the autotuner's knobs; results never depend on them) ------------------ */
+typedef struct wf_tuning {
+    int32_t refill_lanes;        /* persistent traversal: idle lanes of a wave before it claims new
+                                  * rays (1..64; 64 = only when the whole wave is idle) */
+    int32_t trace_blocks_per_cu; /* persistent traversal grid: 128-lane blocks per CU (1..64) */
+    int32_t reserved[6];
+} wf_tuning;
+/* Engines created on the context take its tuning at every run. */
+wf_status wf_context_set_tuning(wf_context* ctx, const wf_tuning* tuning);
+wf_status wf_context_get_tuning(wf_context* ctx, wf_tuning* tuning);
+
 /* ---- per-kernel device timing (HIP events on the context stream) ------------------------ */
 
 /* Enables/disables event timing around every library kernel launch and clears the totals. */

@@ -38,7 +38,7 @@ def test_version_and_error_without_device():
 @pytest.mark.parametrize("name", ["Material", "TextureDesc", "MeshDesc", "SceneDesc", "RaySoA",
                                   "ShadeSoA", "HitSoA", "BuildStats", "ShadeParams",
                                   "RenderParams", "RenderStats", "Camera", "EngineConfig",
-                                  "JobDesc", "JobResult", "EngineStats"])
+                                  "JobDesc", "JobResult", "EngineStats", "Tuning"])
 def test_struct_layout_matches_header(tmp_path, name):
     cname = {"Material": "wf_material", "TextureDesc": "wf_texture_desc",
              "MeshDesc": "wf_mesh_desc", "SceneDesc": "wf_scene_desc", "RaySoA": "wf_ray_soa",
@@ -46,7 +46,8 @@ def test_struct_layout_matches_header(tmp_path, name):
              "ShadeParams": "wf_shade_params", "RenderParams": "wf_render_params",
              "RenderStats": "wf_render_stats", "Camera": "wf_camera",
              "EngineConfig": "wf_engine_config", "JobDesc": "wf_job_desc",
-             "JobResult": "wf_job_result", "EngineStats": "wf_engine_stats"}[name]
+             "JobResult": "wf_job_result", "EngineStats": "wf_engine_stats",
+             "Tuning": "wf_tuning"}[name]
     cls = getattr(abi, name)
     fields = [f[0] for f in cls._fields_]
     nl = "\\n"

@@ -77,6 +77,17 @@ class Context:
     def fill_u32(self, ptr: int, value: int, count: int):
         check(self._lib.wf_fill_u32(self.handle, C.c_void_p(ptr), value, count))
 
+    def tuning(self) -> dict:
+        t = _abi.Tuning()
+        check(self._lib.wf_context_get_tuning(self.handle, C.byref(t)))
+        return {"refill_lanes": t.refill_lanes, "trace_blocks_per_cu": t.trace_blocks_per_cu}
+
+    def set_tuning(self, **kw):
+        cur = self.tuning()
+        cur.update(kw)
+        t = _abi.Tuning(cur["refill_lanes"], cur["trace_blocks_per_cu"])
+        check(self._lib.wf_context_set_tuning(self.handle, C.byref(t)))
+
     def stream_copy(self, dst: int, src: int, nbytes: int):
         check(self._lib.wf_stream_copy(self.handle, C.c_void_p(dst), C.c_void_p(src), nbytes))
 

@@ -155,6 +155,11 @@ class EngineStats(C.Structure):
                 ("phase_ms", C.c_double * 5)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("refill_lanes", C.c_int32), ("trace_blocks_per_cu", C.c_int32),
+                ("reserved", C.c_int32 * 6)]
+
+
 _S = C.c_int32
 _PROTOS = {
     "wf_last_error": (C.c_char_p, []),
@@ -203,6 +208,8 @@ _PROTOS = {
     "wf_engine_result": (_S, [P, C.c_int32, P, P, C.POINTER(JobResult)]),
     "wf_engine_query": (_S, [P, C.POINTER(EngineStats)]),
     "wf_engine_phase_timing": (_S, [P, C.c_int32]),
+    "wf_context_set_tuning": (_S, [P, C.POINTER(Tuning)]),
+    "wf_context_get_tuning": (_S, [P, C.POINTER(Tuning)]),
     "wf_profile_enable": (_S, [P, C.c_int32]),
     "wf_profile_query": (_S, [P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                               C.POINTER(C.c_double)]),

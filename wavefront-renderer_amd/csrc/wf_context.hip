@@ -253,6 +253,24 @@ wf_status wf_profile_query(wf_context* ctx, const char* kernel, double* total_ms
     return WF_OK;
 }
 
+wf_status wf_context_set_tuning(wf_context* ctx, const wf_tuning* t) {
+    clear_error();
+    WF_REQUIRE(ctx && t, "wf_context_set_tuning: NULL argument");
+    WF_REQUIRE(t->refill_lanes >= 1 && t->refill_lanes <= 64,
+               "wf_context_set_tuning: refill_lanes %d not in [1, 64]", t->refill_lanes);
+    WF_REQUIRE(t->trace_blocks_per_cu >= 1 && t->trace_blocks_per_cu <= 64,
+               "wf_context_set_tuning: trace_blocks_per_cu %d not in [1, 64]", t->trace_blocks_per_cu);
+    ctx->tuning = *t;
+    return WF_OK;
+}
+
+wf_status wf_context_get_tuning(wf_context* ctx, wf_tuning* t) {
+    clear_error();
+    WF_REQUIRE(ctx && t, "wf_context_get_tuning: NULL argument");
+    *t = ctx->tuning;
+    return WF_OK;
+}
+
 wf_status wf_profile_filter(wf_context* ctx, const char* kernel) {
     WF_REQUIRE(ctx, "wf_profile_filter: NULL context");
     WF_TRY(harvest_timings(ctx));

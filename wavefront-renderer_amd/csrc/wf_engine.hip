@@ -443,6 +443,14 @@ wf_status enqueue_status(wf_engine* e, int b) {
 void copy_profile_flags(wf_engine* e) {
     e->ctx->profile = e->parent->profile;
     e->ctx->profile_only = e->parent->profile_only;
+    if (std::memcmp(&e->ctx->tuning, &e->parent->tuning, sizeof(wf_tuning)) != 0) {
+        e->ctx->tuning = e->parent->tuning;
+        // a captured iteration bakes the launch shapes in: capture again
+        if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
+        if (e->graph) (void)hipGraphDestroy(e->graph);
+        e->graph_exec = nullptr;
+        e->graph = nullptr;
+    }
 }
 
 void merge_profile(wf_engine* e) {

@@ -627,7 +627,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(const SceneDev s, wf_sha
                                                         const int32_t* __restrict__ list, int count_host,
                                                         const int32_t* __restrict__ count_dev,
                                                         float* __restrict__ accum,
-                                                        int32_t* __restrict__ counter) {
+                                                        int32_t* __restrict__ counter,
+                                                        int refill_lanes) {
     __shared__ int32_t stack[lds_entries<STACK>() * kTraceBlock];
     using W = Walker<true, STACK>;
     volatile int32_t spill[W::X > 0 ? W::X : 1];
@@ -641,7 +642,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(const SceneDev s, wf_sha
     uint32_t f = 0;
     bool occ_area = false, occ_dir = false, exhausted = false;
     for (;;) {
-        const int r = refill(counter, n, item < 0, exhausted);
+        const int r = refill(counter, n, item < 0, exhausted, refill_lanes);
         if (r >= 0) {
             item = r;
             i = list[r];
@@ -783,8 +784,9 @@ wf_status shade_bounce(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q
         Timed tm(ctx, "shadow", 41.0, ns);
         with_stack(s->dev.stack_bound, [&](auto S) {
             k_shadow<decltype(S)::value>
-                <<<persistent_grid(ns.n, ctx->cu_count), kTraceBlock, 0, ctx->stream>>>(
-                    s->dev, *q, o, (int32_t*)plist, ns.n, ns.dev, accum, (int32_t*)pc);
+                <<<persistent_grid(ns.n, ctx), kTraceBlock, 0, ctx->stream>>>(
+                    s->dev, *q, o, (int32_t*)plist, ns.n, ns.dev, accum, (int32_t*)pc,
+                    ctx->tuning.refill_lanes);
         });
         WF_HIP(hipGetLastError());
     }

@@ -22,7 +22,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(
     const float* __restrict__ dx, const float* __restrict__ dy, const float* __restrict__ dz,
     int n_host, const int32_t* __restrict__ nd, float tnear, float tfar,
     int32_t* __restrict__ out_leaf, float* __restrict__ out_t, float* __restrict__ out_u,
-    float* __restrict__ out_v, uint32_t* __restrict__ out_flag, int32_t* __restrict__ counter) {
+    float* __restrict__ out_v, uint32_t* __restrict__ out_flag, int32_t* __restrict__ counter,
+    int refill_lanes) {
     __shared__ int32_t stack[lds_entries<STACK>() * kTraceBlock];
     using W = Walker<false, STACK>;
     volatile int32_t spill[W::X > 0 ? W::X : 1];
@@ -31,7 +32,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(
     int ray = -1;
     bool exhausted = false;
     for (;;) {
-        const int r = refill(counter, n, ray < 0, exhausted);
+        const int r = refill(counter, n, ray < 0, exhausted, refill_lanes);
         if (r >= 0) {
             ray = r;
             w.start(V3(ox[r], oy[r], oz[r]), V3(dx[r], dy[r], dz[r]), tnear, tfar);
@@ -214,10 +215,10 @@ wf_status trace_closest(wf_context* ctx, const wf_scene* s, const wf_ray_soa* r,
     Timed tm(ctx, "traverse_closest", 44.0, n);
     with_stack(s->dev.stack_bound, [&](auto S) {
         k_trace_closest<decltype(S)::value>
-            <<<persistent_grid(n.n, ctx->cu_count), kTraceBlock, 0, ctx->stream>>>(
+            <<<persistent_grid(n.n, ctx), kTraceBlock, 0, ctx->stream>>>(
                 s->dev.nodes4, s->dev.tri, s->dev.n_tris, r->ox, r->oy, r->oz, r->dx, r->dy,
                 r->dz, n.n, n.dev, tnear, tfar, hit_leaf, hit_t, hit_bu, hit_bv, flags,
-                (int32_t*)pc);
+                (int32_t*)pc, ctx->tuning.refill_lanes);
     });
     WF_HIP(hipGetLastError());
     return WF_OK;

@@ -257,13 +257,12 @@ __device__ __forceinline__ bool traverse(const Bvh4Node* __restrict__ nodes,
 
 // Dynamic ray refill for persistent traversal kernels: lanes whose ray finished take the next
 // unclaimed ray indices from a device work counter, one atomic per wave, once at least
-// kRefillLanes lanes are idle (or every lane is). Returns the new ray index or -1.
-constexpr int kRefillLanes = 16;
+// refill_lanes lanes are idle (or every lane is). Returns the new ray index or -1.
 __device__ __forceinline__ int refill(int32_t* __restrict__ counter, int n, bool idle,
-                                      bool& exhausted) {
+                                      bool& exhausted, int refill_lanes) {
     const uint64_t want = __ballot(idle && !exhausted);
     const int k = __popcll(want);
-    if (k == 0 || (k < kRefillLanes && __ballot(!idle) != 0)) return -1;
+    if (k == 0 || (k < refill_lanes && __ballot(!idle) != 0)) return -1;
     const int leader = __ffsll((unsigned long long)want) - 1;
     int base = 0;
     if ((int)__lane_id() == leader) base = atomicAdd(counter, k);
@@ -274,10 +273,10 @@ __device__ __forceinline__ int refill(int32_t* __restrict__ counter, int n, bool
     return r < n ? r : -1;
 }
 
-// Persistent-grid size for traversal kernels: enough resident blocks to fill every CU.
-inline unsigned persistent_grid(int n, int cu_count) {
+// Persistent-grid size for traversal kernels: blocks_per_cu resident blocks on every CU.
+inline unsigned persistent_grid(int n, const wf_context* ctx) {
     const unsigned need = grid_for(n, kTraceBlock);
-    const unsigned fill = (unsigned)cu_count * 16u;
+    const unsigned fill = (unsigned)ctx->cu_count * (unsigned)ctx->tuning.trace_blocks_per_cu;
     return need < fill ? need : fill;
 }
 
