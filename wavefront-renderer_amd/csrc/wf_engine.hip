@@ -64,6 +64,10 @@ enum : int {
 // 64-bit device statistics.
 enum : int { S_RAYS = 0, S_SHADOW, S_HITS, S_MISSES, S_PATHS, S_HEAD, S_TAIL, S_COUNT = 8 };
 
+// Host-visible status block, copied back once per iteration: live rays, ready-queue cursors,
+// then the pixels left of every job slot.
+constexpr int kStatusHeader = 8;   // int32: [0] live rays, [2..3] head, [4..5] tail (u64)
+
 struct EngineView {
     int32_t cap;               // ray slots
     int32_t px;                // framebuffer pixels (ready-queue capacity)
@@ -74,10 +78,10 @@ struct EngineView {
     float* before;             // 3 per pixel: accum at the start of the pixel's current sample
     float* mean;
     float* m2;
-    int32_t* count;            // finished samples per pixel (= index of the next sample)
-    int32_t* pixel_job;
+    int32_t* pixel_tag;        // job slot << kTagJobShift | finished samples (= next sample)
     const JobDev* jobs;
-    int32_t* job_left;         // pixels of each job not finished yet
+    int32_t* status;           // kStatusHeader + job slots
+    int32_t* job_left;         // = status + kStatusHeader: pixels of each job not finished yet
     int32_t* job_max_count;    // most samples any pixel of the job ran
     int32_t* job_unconverged;  // adaptive: pixels that ran out of budget without converging
 };
@@ -95,8 +99,7 @@ __global__ void k_admit(EngineView e, int job, int base, int n, int queue) {
     e.before[3 * (size_t)p + 2] = 0.0f;
     e.mean[p] = 0.0f;
     e.m2[p] = 0.0f;
-    e.count[p] = 0;
-    e.pixel_job[p] = job;
+    e.pixel_tag[p] = job << kTagJobShift;
     if (queue) e.ready[(e.stats[S_TAIL] + (unsigned long long)k) % (unsigned long long)e.px] = p;
 }
 
@@ -121,9 +124,10 @@ __global__ void k_regen(EngineView e, wf_ray_soa r, int32_t* __restrict__ tags) 
     const int take = regen_take(e);
     if (i >= take) return;
     const int p = e.ready[(e.stats[S_HEAD] + (unsigned long long)i) % (unsigned long long)e.px];
-    const int j = e.pixel_job[p];
+    const int32_t pt = e.pixel_tag[p];
+    const int j = pt >> kTagJobShift;
     const JobDev& J = e.jobs[j];
-    const int s = e.count[p];
+    const int s = pt & kTagSampleMask;
     if (J.adaptive) {
         e.before[3 * (size_t)p] = e.accum[3 * (size_t)p];
         e.before[3 * (size_t)p + 1] = e.accum[3 * (size_t)p + 1];
@@ -160,16 +164,17 @@ constexpr int kFinishBlock = 1024;
 // (AdaptiveSampler::addSample/updateConvergence, include/render/AdaptiveSampler.h:21-105, checked
 // when the pixel's sample count is a multiple of the interval, as the pass loop does), then
 // requeue the pixel or retire it. Requeues are appended with one queue-cursor atomic per block.
-// Called by every thread of the block (valid = this thread has an ended path).
-__device__ __forceinline__ void finish_path(const EngineView& e, int p, bool valid) {
+// Called by every thread of the block (valid = this thread has an ended path; tag = its job slot
+// and sample index).
+__device__ __forceinline__ void finish_path(const EngineView& e, int p, int32_t tag, bool valid) {
     __shared__ uint32_t block_count;
     __shared__ unsigned long long block_base;
     bool requeue = false;
     if (valid) {
-        const int j = e.pixel_job[p];
+        const int j = tag >> kTagJobShift;
         const JobDev& J = e.jobs[j];
-        const int cnt = e.count[p] + 1;
-        e.count[p] = cnt;
+        const int cnt = (tag & kTagSampleMask) + 1;
+        e.pixel_tag[p] = (j << kTagJobShift) | cnt;
         bool converged = false;
         if (J.adaptive) {
             const size_t b = 3 * (size_t)p;
@@ -218,42 +223,50 @@ __device__ __forceinline__ void finish_path(const EngineView& e, int p, bool val
 
 __global__ __launch_bounds__(kFinishBlock) void k_finish_hits(EngineView e,
                                                               const int32_t* __restrict__ pixel,
+                                                              const int32_t* __restrict__ tags,
                                                               const uint32_t* __restrict__ alive) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = e.counters[C_HITS];
     if (blockIdx.x * blockDim.x >= n) return;
     const bool valid = i < n && !alive[i];
-    finish_path(e, valid ? pixel[i] : 0, valid);
+    finish_path(e, valid ? pixel[i] : 0, valid ? tags[i] : 0, valid);
 }
 
 __global__ __launch_bounds__(kFinishBlock) void k_finish_misses(EngineView e,
-                                                                const int32_t* __restrict__ pixel) {
+                                                                const int32_t* __restrict__ pixel,
+                                                                const int32_t* __restrict__ tags) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = e.counters[C_MISSES];
     if (blockIdx.x * blockDim.x >= n) return;
     const bool valid = i < n;
-    finish_path(e, valid ? pixel[i] : 0, valid);
+    finish_path(e, valid ? pixel[i] : 0, valid ? tags[i] : 0, valid);
 }
 
 __global__ void k_iteration_end(EngineView e) {
     e.stats[S_SHADOW] += (unsigned long long)e.counters[C_SHADOW];
     e.stats[S_HITS] += (unsigned long long)e.counters[C_HITS];
     e.stats[S_MISSES] += (unsigned long long)e.counters[C_MISSES];
-    e.counters[C_LIVE] = e.counters[C_NEXT];
+    const int live = e.counters[C_NEXT];
+    e.counters[C_LIVE] = live;
+    e.status[0] = live;
+    unsigned long long* cur = (unsigned long long*)(e.status + 2);
+    cur[0] = e.stats[S_HEAD];
+    cur[1] = e.stats[S_TAIL];
 }
 
-__global__ void k_resolve_job(const float* __restrict__ accum, const int32_t* __restrict__ count,
+__global__ void k_resolve_job(const float* __restrict__ accum, const int32_t* __restrict__ pixel_tag,
                               int base, int n, float* __restrict__ image,
                               int32_t* __restrict__ counts_out) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const int p = base + k;
-    int c = count[p];
+    const int count = pixel_tag[p] & kTagSampleMask;
+    int c = count;
     c = c > 1 ? c : 1;
     image[3 * (size_t)k] = accum[3 * (size_t)p] / (float)c;
     image[3 * (size_t)k + 1] = accum[3 * (size_t)p + 1] / (float)c;
     image[3 * (size_t)k + 2] = accum[3 * (size_t)p + 2] / (float)c;
-    if (counts_out) counts_out[k] = count[p];
+    if (counts_out) counts_out[k] = count;
 }
 
 }  // namespace wf
@@ -271,7 +284,8 @@ struct wf_engine {
     // device state
     wf_ray_soa r0{}, r1{}, miss{};
     wf_shade_soa s0{}, s1{};
-    int32_t *t0 = nullptr, *t1 = nullptr, *st0 = nullptr, *st1 = nullptr, *idx = nullptr;
+    int32_t *t0 = nullptr, *t1 = nullptr, *st0 = nullptr, *st1 = nullptr, *mt = nullptr;
+    int32_t* idx = nullptr;
     EngineView view{};
     JobDev* jobs_dev = nullptr;
     // host mirrors
@@ -390,7 +404,8 @@ wf_status enqueue_iteration(wf_engine* e) {
         rt = e->t1;
     }
     phase_mark(e, 2);
-    WF_TRY(intersect_split(ctx, s, rq, live, &e->s0, &e->miss, c + C_HITS, rt, e->st0));
+    WF_TRY(intersect_split(ctx, s, rq, live, &e->s0, &e->miss, c + C_HITS, rt, e->st0, e->mt,
+                           true));
     phase_mark(e, 3);
     const wf_shade_soa* q = &e->s0;
     const int32_t* qt = e->st0;
@@ -406,6 +421,7 @@ wf_status enqueue_iteration(wf_engine* e) {
     jobs.tags = qt;
     jobs.jobs = e->jobs_dev;
     jobs.next_tags = e->t0;
+    jobs.unordered = true;
     WF_TRY(shade_bounce(ctx, s, q, hits, &e->cfg.shade, v.accum, &e->r0, c + C_NEXT, jobs,
                         &alive));
     phase_mark(e, 5);
@@ -414,10 +430,10 @@ wf_status enqueue_iteration(wf_engine* e) {
     {
         Timed tm(ctx, "finish", 12.0, live);
         k_finish_hits<<<grid_for(cap, kFinishBlock), kFinishBlock, 0, ctx->stream>>>(v, q->pixel,
-                                                                                    alive);
+                                                                                    qt, alive);
         WF_HIP(hipGetLastError());
         k_finish_misses<<<grid_for(cap, kFinishBlock), kFinishBlock, 0, ctx->stream>>>(
-            v, e->miss.pixel);
+            v, e->miss.pixel, e->mt);
         WF_HIP(hipGetLastError());
         k_iteration_end<<<1, 1, 0, ctx->stream>>>(v);
         WF_HIP(hipGetLastError());
@@ -426,15 +442,10 @@ wf_status enqueue_iteration(wf_engine* e) {
     return WF_OK;
 }
 
-// Status of iteration results: counters block + per-job pixels left, into pinned buffer b.
-constexpr int kStatusHeader = 8;   // int32: live, ready (low 31 bits of tail - head)
+// Status of an iteration (one copy of the status block) into pinned buffer b.
 wf_status enqueue_status(wf_engine* e, int b) {
     int32_t* dst = e->status_host + (size_t)b * (kStatusHeader + e->max_jobs);
-    WF_HIP(hipMemcpyAsync(dst, e->view.counters, sizeof(int32_t) * 2, hipMemcpyDeviceToHost,
-                          e->ctx->stream));
-    WF_HIP(hipMemcpyAsync(dst + 2, e->view.stats + S_HEAD, sizeof(unsigned long long) * 2,
-                          hipMemcpyDeviceToHost, e->ctx->stream));
-    WF_HIP(hipMemcpyAsync(dst + kStatusHeader, e->view.job_left, sizeof(int32_t) * e->max_jobs,
+    WF_HIP(hipMemcpyAsync(dst, e->view.status, sizeof(int32_t) * (kStatusHeader + e->max_jobs),
                           hipMemcpyDeviceToHost, e->ctx->stream));
     WF_HIP(hipEventRecord(e->status_ev[b], e->ctx->stream));
     return WF_OK;
@@ -541,6 +552,7 @@ wf_status wf_engine_create(wf_context* ctx, wf_scene* scene, const wf_engine_con
     ETRY(engine_array(e, cap, &e->t1));
     ETRY(engine_array(e, cap, &e->st0));
     ETRY(engine_array(e, cap, &e->st1));
+    ETRY(engine_array(e, cap, &e->mt));
     ETRY(engine_array(e, cap, &e->idx));
     EngineView& v = e->view;
     v.cap = e->cap;
@@ -552,18 +564,17 @@ wf_status wf_engine_create(wf_context* ctx, wf_scene* scene, const wf_engine_con
     ETRY(engine_array(e, 3 * px, &v.before));
     ETRY(engine_array(e, px, &v.mean));
     ETRY(engine_array(e, px, &v.m2));
-    ETRY(engine_array(e, px, &v.count));
-    ETRY(engine_array(e, px, &v.pixel_job));
+    ETRY(engine_array(e, px, &v.pixel_tag));
     ETRY(engine_array(e, (size_t)e->max_jobs, &e->jobs_dev));
     v.jobs = e->jobs_dev;
-    ETRY(engine_array(e, (size_t)e->max_jobs, &v.job_left));
+    ETRY(engine_array(e, (size_t)(kStatusHeader + e->max_jobs), &v.status));
+    v.job_left = v.status + kStatusHeader;
     ETRY(engine_array(e, (size_t)e->max_jobs, &v.job_max_count));
     ETRY(engine_array(e, (size_t)e->max_jobs, &v.job_unconverged));
     ETRY(wf_fill_u32(e->ctx, v.counters, 0u, C_COUNT));
     ETRY(wf_fill_u32(e->ctx, v.stats, 0u, 2 * S_COUNT));
-    ETRY(wf_fill_u32(e->ctx, v.job_left, 0u, (size_t)e->max_jobs));
-    ETRY(wf_fill_u32(e->ctx, v.pixel_job, 0u, px));
-    ETRY(wf_fill_u32(e->ctx, v.count, 0u, px));
+    ETRY(wf_fill_u32(e->ctx, v.status, 0u, (size_t)(kStatusHeader + e->max_jobs)));
+    ETRY(wf_fill_u32(e->ctx, v.pixel_tag, 0u, px));
 #undef ETRY
     e->jobs.assign((size_t)e->max_jobs, JobDev{});
     e->job_state.assign((size_t)e->max_jobs, 0);
@@ -746,7 +757,7 @@ wf_status wf_engine_result(wf_engine* e, int32_t job_id, float* image_rgb, int32
     const JobDev& J = e->jobs[job_id];
     const int P = J.width * J.height;
     wf_context* ctx = e->ctx;
-    k_resolve_job<<<grid_for(P, 256), 256, 0, ctx->stream>>>(e->view.accum, e->view.count,
+    k_resolve_job<<<grid_for(P, 256), 256, 0, ctx->stream>>>(e->view.accum, e->view.pixel_tag,
                                                              J.pixel_base, P, image_rgb,
                                                              sample_counts);
     WF_HIP(hipGetLastError());

@@ -73,7 +73,7 @@ struct wf_context {
     std::vector<wf::PendingTiming> pending;
     std::vector<hipEvent_t> event_pool;
     int32_t* count_slots = nullptr;    // pinned, kCountSlots entries
-    wf_tuning tuning{64, 16, {0, 0, 0, 0, 0, 0}};
+    wf_tuning tuning{48, 6, {0, 0, 0, 0, 0, 0}};
     int next_count_slot = 0;
     int cu_count = 256;
 };
@@ -296,9 +296,11 @@ wf_status trace_occluded(const wf_scene* s, const float* ox, const float* oy, co
                          int32_t n, uint8_t* out);
 // Closest hit + stable split; counts_dev[0] = hits, counts_dev[1] = misses (device int32).
 // tags_in/hit_tags (optional) carry job tags from the rays to the hit queue.
+// unordered: hits and misses are appended in any order (one kernel, no scan).
 wf_status intersect_split(wf_context* ctx, const wf_scene* s, const wf_ray_soa* rays, Count n,
                           wf_shade_soa* hits_out, wf_ray_soa* misses_out, int32_t* counts_dev,
-                          const int32_t* tags_in = nullptr, int32_t* hit_tags = nullptr);
+                          const int32_t* tags_in = nullptr, int32_t* hit_tags = nullptr,
+                          int32_t* miss_tags = nullptr, bool unordered = false);
 wf_status ray_schedule(wf_context* ctx, const wf_ray_soa* rays, Count n, const float bmin[3],
                        const float bmax[3], int32_t* sorted, uint64_t* keys_out);
 wf_status shading_schedule(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q, Count n,
@@ -309,6 +311,7 @@ struct ShadeJobs {
     const int32_t* tags = nullptr;     // per shading-queue entry
     const JobDev* jobs = nullptr;
     int32_t* next_tags = nullptr;      // tags of the compacted next rays
+    bool unordered = false;            // next rays / shadow entries may come in any order
 };
 // One bounce of shadeAll: emission into accum, shadow rays resolved, next rays compacted into
 // `next`; counts_dev[0] = next rays, counts_dev[1] = shadow rays traced, counts_dev[2] = shadow

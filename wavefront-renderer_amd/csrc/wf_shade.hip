@@ -34,7 +34,27 @@ struct ShadeOut {
     float* ntr; float* ntg; float* ntb;                // next throughput
     float* npdf;
     uint32_t* alive;
+    // unordered append mode (queue order free): shadow entries and next rays are appended with
+    // wave-aggregated atomics instead of a scan + compaction
+    int32_t append;
+    int32_t* shadow_list;
+    int32_t* shadow_entries;                           // device counter
+    int32_t* shadow_rays;                              // device counter
+    int32_t* next_count;                               // device counter
+    wf_ray_soa next;
+    int32_t* next_tags;
 };
+
+// Appends `take` lanes of the wave at a device cursor; returns this lane's slot (or -1).
+__device__ __forceinline__ int wave_append(int32_t* cursor, bool take) {
+    const uint64_t b = __ballot(take);
+    if (b == 0) return -1;
+    const int leader = __ffsll((unsigned long long)b) - 1;
+    int base = 0;
+    if ((int)__lane_id() == leader) base = atomicAdd(cursor, __popcll(b));
+    base = __shfl(base, leader, 64);
+    return take ? base + __popcll(b & lanemask_lt()) : -1;
+}
 
 __device__ __forceinline__ V3 texel(const SceneDev& s, int tid, float u, float v) {
     const DevTexture T = s.textures[tid];
@@ -550,7 +570,25 @@ __global__ void k_shade(const SceneDev s, wf_shade_soa q, int n_host,
     }
     if (depth + 1 >= depth_limit) alive = false;
     o.alive[i] = alive ? 1u : 0u;
-    if (alive) {
+    if (o.append) {
+        const int si = wave_append(o.shadow_entries, sflag != 0u);
+        if (si >= 0) o.shadow_list[si] = i;
+        const int nrays = __popcll(__ballot((sflag & 1u) != 0u)) + __popcll(__ballot((sflag & 2u) != 0u));
+        if ((int)__lane_id() == __ffsll((unsigned long long)__ballot(true)) - 1 && nrays)
+            atomicAdd(o.shadow_rays, nrays);
+        const int k = wave_append(o.next_count, alive);
+        if (k >= 0) {
+            st3(o.next.ox, o.next.oy, o.next.oz, k, P);
+            st3(o.next.dx, o.next.dy, o.next.dz, k, dir);
+            o.next.tr[k] = nt.x;
+            o.next.tg[k] = nt.y;
+            o.next.tb[k] = nt.z;
+            o.next.pixel[k] = pixel;
+            o.next.depth[k] = depth + 1;
+            o.next.pdf[k] = spdf;
+            if (o.next_tags && tags) o.next_tags[k] = tags[i];
+        }
+    } else if (alive) {
         st3(o.ndx, o.ndy, o.ndz, i, dir);
         o.ntr[i] = nt.x;
         o.ntg[i] = nt.y;
@@ -728,6 +766,11 @@ static wf_status shade_outputs(wf_context* ctx, int n, ShadeOut& o) {
     o.ntr = f + 14 * nn; o.ntg = f + 15 * nn; o.ntb = f + 16 * nn;
     o.npdf = f + 17 * nn;
     o.alive = (uint32_t*)(f + 18 * nn);
+    o.append = 0;
+    o.shadow_list = nullptr;
+    o.shadow_entries = o.shadow_rays = o.next_count = nullptr;
+    o.next = wf_ray_soa{};
+    o.next_tags = nullptr;
     return WF_OK;
 }
 
@@ -748,6 +791,38 @@ wf_status shade_bounce(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q
     WF_TRY(workspace(ctx, "shade_sflag", nn * 4, &pshflag));
     ShadeArgs a{p->physics, p->rr_min_depth, p->cost_aware_rr, p->firefly_threshold, p->seed,
                 p->sample_index, p->job};
+    if (jobs.unordered) {
+        // queue order is free: one shading kernel appends the shadow entries and next rays
+        o.append = 1;
+        o.shadow_list = (int32_t*)plist;
+        o.next_count = counts_dev;
+        o.shadow_rays = counts_dev + 1;
+        o.shadow_entries = counts_dev + 2;
+        o.next = *next;
+        o.next_tags = jobs.next_tags;
+        WF_TRY(wf_fill_u32(ctx, counts_dev, 0u, 3));
+        {
+            Timed tm(ctx, "shade", 120.0, n);
+            k_shade<<<grid_for(n.n, 128), 128, 0, ctx->stream>>>(s->dev, *q, n.n, n.dev, a,
+                                                                 s->dev.mat_rel_cost, accum, o,
+                                                                 jobs.tags, jobs.jobs);
+            WF_HIP(hipGetLastError());
+        }
+        const Count ns{n.n, counts_dev + 2};
+        void* pc = nullptr;
+        WF_TRY(workspace(ctx, "shadow_counter", sizeof(int32_t), &pc));
+        WF_TRY(wf_fill_u32(ctx, pc, 0u, 1));
+        Timed tm(ctx, "shadow", 41.0, ns);
+        with_stack(s->dev.stack_bound, [&](auto S) {
+            k_shadow<decltype(S)::value>
+                <<<persistent_grid(ns.n, ctx), kTraceBlock, 0, ctx->stream>>>(
+                    s->dev, *q, o, (int32_t*)plist, ns.n, ns.dev, accum, (int32_t*)pc,
+                    ctx->tuning.refill_lanes);
+        });
+        WF_HIP(hipGetLastError());
+        if (alive_out) *alive_out = o.alive;
+        return WF_OK;
+    }
     {
         Timed tm(ctx, "shade", 120.0, n);
         k_shade<<<grid_for(n.n, 128), 128, 0, ctx->stream>>>(s->dev, *q, n.n, n.dev, a,

@@ -162,6 +162,50 @@ __global__ void k_split(const SceneDev s, wf_ray_soa in, int n_host, const int32
     }
 }
 
+// Unordered split: hits and misses appended with wave-aggregated atomics on counts[0] / counts[1].
+__global__ void k_split_append(const SceneDev s, wf_ray_soa in, int n_host,
+                               const int32_t* __restrict__ nd, const int32_t* __restrict__ leaf,
+                               const float* __restrict__ ht, const float* __restrict__ hu,
+                               const float* __restrict__ hv, wf_shade_soa out, wf_ray_soa miss,
+                               int32_t* __restrict__ counts, const int32_t* __restrict__ tags_in,
+                               int32_t* __restrict__ hit_tags, int32_t* __restrict__ miss_tags) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= load_count(n_host, nd)) return;
+    const int L = leaf[i];
+    const bool hit = L >= 0;
+    const uint64_t active = __ballot(true);
+    const uint64_t hb = __ballot(hit);
+    const int leader = __ffsll((unsigned long long)active) - 1;
+    int base_h = 0, base_m = 0;
+    if ((int)__lane_id() == leader) {
+        base_h = atomicAdd(&counts[0], __popcll(hb));
+        base_m = atomicAdd(&counts[1], __popcll(active & ~hb));
+    }
+    base_h = __shfl(base_h, leader, 64);
+    base_m = __shfl(base_m, leader, 64);
+    const uint64_t lt = lanemask_lt();
+    if (hit) {
+        const int pos = base_h + __popcll(hb & lt);
+        map_hit(s, in, i, L, ht[i], hu[i], hv[i], out, pos);
+        if (hit_tags) hit_tags[pos] = tags_in[i];
+    } else {
+        const int m = base_m + __popcll(active & ~hb & lt);
+        miss.ox[m] = in.ox[i];
+        miss.oy[m] = in.oy[i];
+        miss.oz[m] = in.oz[i];
+        miss.dx[m] = in.dx[i];
+        miss.dy[m] = in.dy[i];
+        miss.dz[m] = in.dz[i];
+        miss.tr[m] = in.tr[i];
+        miss.tg[m] = in.tg[i];
+        miss.tb[m] = in.tb[i];
+        miss.pixel[m] = in.pixel[i];
+        miss.depth[m] = in.depth[i];
+        miss.pdf[m] = in.pdf[i];
+        if (miss_tags) miss_tags[m] = tags_in[i];
+    }
+}
+
 // One HitRecord per input ray (slot i), hit flag per ray.
 __global__ void k_records(const SceneDev s, wf_ray_soa in, int n, const int32_t* __restrict__ leaf,
                           const float* __restrict__ ht, const float* __restrict__ hu,
@@ -243,7 +287,8 @@ wf_status trace_occluded(const wf_scene* s, const float* ox, const float* oy, co
 // Closest hit + stable split; counts_dev[0] = hits, counts_dev[1] = misses (device int32).
 wf_status intersect_split(wf_context* ctx, const wf_scene* s, const wf_ray_soa* rays, Count n,
                           wf_shade_soa* hits_out, wf_ray_soa* misses_out, int32_t* counts_dev,
-                          const int32_t* tags_in, int32_t* hit_tags) {
+                          const int32_t* tags_in, int32_t* hit_tags, int32_t* miss_tags,
+                          bool unordered) {
     void *pl, *pt, *pu, *pv, *pf;
     size_t nn = (size_t)(n.n > 0 ? n.n : 1);
     WF_TRY(workspace(ctx, "hit_leaf", sizeof(int32_t) * nn, &pl));
@@ -251,6 +296,19 @@ wf_status intersect_split(wf_context* ctx, const wf_scene* s, const wf_ray_soa* 
     WF_TRY(workspace(ctx, "hit_u", sizeof(float) * nn, &pu));
     WF_TRY(workspace(ctx, "hit_v", sizeof(float) * nn, &pv));
     WF_TRY(workspace(ctx, "hit_flag", sizeof(uint32_t) * nn, &pf));
+    if (unordered) {
+        WF_TRY(wf_fill_u32(ctx, counts_dev, 0u, 2));
+        WF_TRY(trace_closest(ctx, s, rays, n, 0.001f, 1e9f, (int32_t*)pl, (float*)pt,
+                             (float*)pu, (float*)pv, nullptr));
+        if (n.n > 0) {
+            Timed tm(ctx, "split", 97.0, n);
+            k_split_append<<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(
+                s->dev, *rays, n.n, n.dev, (int32_t*)pl, (float*)pt, (float*)pu, (float*)pv,
+                *hits_out, *misses_out, counts_dev, tags_in, hit_tags, miss_tags);
+            WF_HIP(hipGetLastError());
+        }
+        return WF_OK;
+    }
     WF_TRY(trace_closest(ctx, s, rays, n, 0.001f, 1e9f, (int32_t*)pl, (float*)pt, (float*)pu,
                          (float*)pv, (uint32_t*)pf));
     WF_TRY(exclusive_scan_u32(ctx, (uint32_t*)pf, (uint32_t*)pf, n, counts_dev, "split"));
