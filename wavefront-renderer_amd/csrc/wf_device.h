@@ -132,6 +132,24 @@ __device__ __forceinline__ V3 primary_dir(const float* cam, int x, int y, int wi
     return (TL + H * u + Vv * v - O).normalized();
 }
 
+// Unordered append of the lanes with `take` set at a device cursor, one global atomic per block.
+// Every thread of the block must call it (same number of times); returns this lane's slot or -1.
+// `scratch` is two words of LDS owned by the caller.
+__device__ __forceinline__ int block_append(int32_t* cursor, bool take, uint32_t* scratch) {
+    if (threadIdx.x == 0) scratch[0] = 0u;
+    __syncthreads();
+    const uint64_t b = __ballot(take);
+    uint32_t off = 0;
+    if (__lane_id() == 0 && b) off = atomicAdd(&scratch[0], (uint32_t)__popcll(b));
+    off = __shfl(off, 0, 64);
+    __syncthreads();
+    if (threadIdx.x == 0) scratch[1] = scratch[0] ? (uint32_t)atomicAdd(cursor, (int32_t)scratch[0]) : 0u;
+    __syncthreads();
+    const uint32_t base = scratch[1];
+    __syncthreads();   // scratch is reused by the next call
+    return take ? (int)(base + off + (uint32_t)__popcll(b & lanemask_lt())) : -1;
+}
+
 // Element count of a launch: the device counter when present, else the host value.
 __device__ __forceinline__ int load_count(int n, const int32_t* __restrict__ dev) {
     return dev ? *dev : n;

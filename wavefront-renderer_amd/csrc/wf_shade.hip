@@ -45,16 +45,6 @@ struct ShadeOut {
     int32_t* next_tags;
 };
 
-// Appends `take` lanes of the wave at a device cursor; returns this lane's slot (or -1).
-__device__ __forceinline__ int wave_append(int32_t* cursor, bool take) {
-    const uint64_t b = __ballot(take);
-    if (b == 0) return -1;
-    const int leader = __ffsll((unsigned long long)b) - 1;
-    int base = 0;
-    if ((int)__lane_id() == leader) base = atomicAdd(cursor, __popcll(b));
-    base = __shfl(base, leader, 64);
-    return take ? base + __popcll(b & lanemask_lt()) : -1;
-}
 
 __device__ __forceinline__ V3 texel(const SceneDev& s, int tid, float u, float v) {
     const DevTexture T = s.textures[tid];
@@ -570,30 +560,52 @@ __global__ void k_shade(const SceneDev s, wf_shade_soa q, int n_host,
     }
     if (depth + 1 >= depth_limit) alive = false;
     o.alive[i] = alive ? 1u : 0u;
-    if (o.append) {
-        const int si = wave_append(o.shadow_entries, sflag != 0u);
-        if (si >= 0) o.shadow_list[si] = i;
-        const int nrays = __popcll(__ballot((sflag & 1u) != 0u)) + __popcll(__ballot((sflag & 2u) != 0u));
-        if ((int)__lane_id() == __ffsll((unsigned long long)__ballot(true)) - 1 && nrays)
-            atomicAdd(o.shadow_rays, nrays);
-        const int k = wave_append(o.next_count, alive);
-        if (k >= 0) {
-            st3(o.next.ox, o.next.oy, o.next.oz, k, P);
-            st3(o.next.dx, o.next.dy, o.next.dz, k, dir);
-            o.next.tr[k] = nt.x;
-            o.next.tg[k] = nt.y;
-            o.next.tb[k] = nt.z;
-            o.next.pixel[k] = pixel;
-            o.next.depth[k] = depth + 1;
-            o.next.pdf[k] = spdf;
-            if (o.next_tags && tags) o.next_tags[k] = tags[i];
-        }
-    } else if (alive) {
+    if (alive) {
         st3(o.ndx, o.ndy, o.ndz, i, dir);
         o.ntr[i] = nt.x;
         o.ntg[i] = nt.y;
         o.ntb[i] = nt.z;
         o.npdf[i] = spdf;
+    }
+}
+
+// Unordered compaction after k_shade: shadow entries and next rays appended at device cursors
+// (one atomic per block and cursor), shadow rays counted.
+constexpr int kAppendBlock = 1024;
+__global__ __launch_bounds__(kAppendBlock) void k_shade_append(wf_shade_soa q, ShadeOut o, int n_host,
+                                                               const int32_t* __restrict__ nd,
+                                                               const int32_t* __restrict__ tags) {
+    __shared__ uint32_t scratch[2];
+    const int n = load_count(n_host, nd);
+    if ((int)(blockIdx.x * blockDim.x) >= n) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    const uint32_t f = valid ? o.shadow_flag[i] : 0u;
+    const bool alive = valid && o.alive[i] != 0u;
+    const int si = block_append(o.shadow_entries, f != 0u, scratch);
+    if (si >= 0) o.shadow_list[si] = i;
+    __shared__ uint32_t block_rays;
+    if (threadIdx.x == 0) block_rays = 0u;
+    __syncthreads();
+    const int rays = __popcll(__ballot((f & 1u) != 0u)) + __popcll(__ballot((f & 2u) != 0u));
+    if (__lane_id() == 0 && rays) atomicAdd(&block_rays, (uint32_t)rays);
+    __syncthreads();
+    if (threadIdx.x == 0 && block_rays) atomicAdd(o.shadow_rays, (int32_t)block_rays);
+    const int k = block_append(o.next_count, alive, scratch);
+    if (k >= 0) {
+        o.next.ox[k] = q.px[i];
+        o.next.oy[k] = q.py[i];
+        o.next.oz[k] = q.pz[i];
+        o.next.dx[k] = o.ndx[i];
+        o.next.dy[k] = o.ndy[i];
+        o.next.dz[k] = o.ndz[i];
+        o.next.tr[k] = o.ntr[i];
+        o.next.tg[k] = o.ntg[i];
+        o.next.tb[k] = o.ntb[i];
+        o.next.pixel[k] = q.pixel[i];
+        o.next.depth[k] = q.depth[i] + 1;
+        o.next.pdf[k] = o.npdf[i];
+        if (o.next_tags) o.next_tags[k] = tags[i];
     }
 }
 
@@ -806,6 +818,12 @@ wf_status shade_bounce(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q
             k_shade<<<grid_for(n.n, 128), 128, 0, ctx->stream>>>(s->dev, *q, n.n, n.dev, a,
                                                                  s->dev.mat_rel_cost, accum, o,
                                                                  jobs.tags, jobs.jobs);
+            WF_HIP(hipGetLastError());
+        }
+        {
+            Timed tm(ctx, "next_rays", 56.0, n);
+            k_shade_append<<<grid_for(n.n, kAppendBlock), kAppendBlock, 0, ctx->stream>>>(
+                *q, o, n.n, n.dev, jobs.tags);
             WF_HIP(hipGetLastError());
         }
         const Count ns{n.n, counts_dev + 2};

@@ -162,34 +162,26 @@ __global__ void k_split(const SceneDev s, wf_ray_soa in, int n_host, const int32
     }
 }
 
-// Unordered split: hits and misses appended with wave-aggregated atomics on counts[0] / counts[1].
-__global__ void k_split_append(const SceneDev s, wf_ray_soa in, int n_host,
-                               const int32_t* __restrict__ nd, const int32_t* __restrict__ leaf,
-                               const float* __restrict__ ht, const float* __restrict__ hu,
-                               const float* __restrict__ hv, wf_shade_soa out, wf_ray_soa miss,
-                               int32_t* __restrict__ counts, const int32_t* __restrict__ tags_in,
-                               int32_t* __restrict__ hit_tags, int32_t* __restrict__ miss_tags) {
+// Unordered split: hits and misses appended at counts[0] / counts[1] (one atomic per block).
+constexpr int kAppendBlock = 1024;
+__global__ __launch_bounds__(kAppendBlock) void k_split_append(
+    const SceneDev s, wf_ray_soa in, int n_host, const int32_t* __restrict__ nd,
+    const int32_t* __restrict__ leaf, const float* __restrict__ ht, const float* __restrict__ hu,
+    const float* __restrict__ hv, wf_shade_soa out, wf_ray_soa miss, int32_t* __restrict__ counts,
+    const int32_t* __restrict__ tags_in, int32_t* __restrict__ hit_tags,
+    int32_t* __restrict__ miss_tags) {
+    __shared__ uint32_t scratch[2];
+    const int n = load_count(n_host, nd);
+    if ((int)(blockIdx.x * blockDim.x) >= n) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= load_count(n_host, nd)) return;
-    const int L = leaf[i];
-    const bool hit = L >= 0;
-    const uint64_t active = __ballot(true);
-    const uint64_t hb = __ballot(hit);
-    const int leader = __ffsll((unsigned long long)active) - 1;
-    int base_h = 0, base_m = 0;
-    if ((int)__lane_id() == leader) {
-        base_h = atomicAdd(&counts[0], __popcll(hb));
-        base_m = atomicAdd(&counts[1], __popcll(active & ~hb));
-    }
-    base_h = __shfl(base_h, leader, 64);
-    base_m = __shfl(base_m, leader, 64);
-    const uint64_t lt = lanemask_lt();
-    if (hit) {
-        const int pos = base_h + __popcll(hb & lt);
+    const bool valid = i < n;
+    const int L = valid ? leaf[i] : -1;
+    const int pos = block_append(&counts[0], valid && L >= 0, scratch);
+    const int m = block_append(&counts[1], valid && L < 0, scratch);
+    if (pos >= 0) {
         map_hit(s, in, i, L, ht[i], hu[i], hv[i], out, pos);
         if (hit_tags) hit_tags[pos] = tags_in[i];
-    } else {
-        const int m = base_m + __popcll(active & ~hb & lt);
+    } else if (m >= 0) {
         miss.ox[m] = in.ox[i];
         miss.oy[m] = in.oy[i];
         miss.oz[m] = in.oz[i];
@@ -302,7 +294,7 @@ wf_status intersect_split(wf_context* ctx, const wf_scene* s, const wf_ray_soa* 
                              (float*)pu, (float*)pv, nullptr));
         if (n.n > 0) {
             Timed tm(ctx, "split", 97.0, n);
-            k_split_append<<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(
+            k_split_append<<<grid_for(n.n, kAppendBlock), kAppendBlock, 0, ctx->stream>>>(
                 s->dev, *rays, n.n, n.dev, (int32_t*)pl, (float*)pt, (float*)pu, (float*)pv,
                 *hits_out, *misses_out, counts_dev, tags_in, hit_tags, miss_tags);
             WF_HIP(hipGetLastError());
