@@ -428,11 +428,15 @@ wf_status wf_engine_query(wf_engine* engine, wf_engine_stats* out);
 wf_status wf_engine_phase_timing(wf_engine* engine, int32_t enable);
 
 /* ---- launch tuning (the autotuner's knobs; results never depend on them) ------------------ */
+#define WF_BVH_PLOC 0   /* locally-ordered clustering of the Morton-sorted triangles */
+#define WF_BVH_LBVH 1   /* Karras radix tree over the Morton codes */
 typedef struct wf_tuning {
     int32_t refill_lanes;        /* persistent traversal: idle lanes of a wave before it claims new
                                   * rays (1..64; 64 = only when the whole wave is idle) */
     int32_t trace_blocks_per_cu; /* persistent traversal grid: 128-lane blocks per CU (1..64) */
-    int32_t reserved[6];
+    int32_t bvh_builder;         /* WF_BVH_* hierarchy of the next wf_accel_build */
+    int32_t ploc_radius;         /* PLOC neighbour search radius (1..32) */
+    int32_t reserved[4];
 } wf_tuning;
 /* Engines created on the context take its tuning at every run. */
 wf_status wf_context_set_tuning(wf_context* ctx, const wf_tuning* tuning);

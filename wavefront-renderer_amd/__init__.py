@@ -80,12 +80,14 @@ class Context:
     def tuning(self) -> dict:
         t = _abi.Tuning()
         check(self._lib.wf_context_get_tuning(self.handle, C.byref(t)))
-        return {"refill_lanes": t.refill_lanes, "trace_blocks_per_cu": t.trace_blocks_per_cu}
+        return {"refill_lanes": t.refill_lanes, "trace_blocks_per_cu": t.trace_blocks_per_cu,
+                "bvh_builder": t.bvh_builder, "ploc_radius": t.ploc_radius}
 
     def set_tuning(self, **kw):
         cur = self.tuning()
         cur.update(kw)
-        t = _abi.Tuning(cur["refill_lanes"], cur["trace_blocks_per_cu"])
+        t = _abi.Tuning(cur["refill_lanes"], cur["trace_blocks_per_cu"], cur["bvh_builder"],
+                        cur["ploc_radius"])
         check(self._lib.wf_context_set_tuning(self.handle, C.byref(t)))
 
     def stream_copy(self, dst: int, src: int, nbytes: int):

@@ -155,9 +155,13 @@ class EngineStats(C.Structure):
                 ("phase_ms", C.c_double * 5)]
 
 
+BVH_PLOC, BVH_LBVH = 0, 1
+
+
 class Tuning(C.Structure):
     _fields_ = [("refill_lanes", C.c_int32), ("trace_blocks_per_cu", C.c_int32),
-                ("reserved", C.c_int32 * 6)]
+                ("bvh_builder", C.c_int32), ("ploc_radius", C.c_int32),
+                ("reserved", C.c_int32 * 4)]
 
 
 _S = C.c_int32

@@ -1,0 +1,359 @@
+// Higher-quality accelerator build: PLOC clustering of the Morton-ordered leaves (Meister and
+// Bittner, "Parallel Locally-Ordered Clustering for Bounding Volume Hierarchy Construction",
+// IEEE TVCG 2018) and a greedy surface-area collapse of the binary tree into four-wide nodes.
+//
+// PLOC, per iteration over the current clusters (in Morton order):
+//   1. every cluster finds, within `radius` positions, the cluster whose union box has the
+//      smallest surface area (ties: nearer position, then pairs starting at an even position,
+//      then the lower position, so that equal boxes still pair up two by two);
+//   2. mutual nearest neighbours merge into a new internal node kept at the lower position;
+//   3. the surviving clusters are compacted in order.
+// Internal nodes are numbered downwards from n - 2, so the last merge (the root) is node 0: the
+// output arrays have the layout of the Karras build (left/right child: node index, or ~leaf).
+//
+// The collapse is top-down, level by level: a four-wide node starts from the two children of its
+// binary node and keeps opening the inner child with the largest box surface area until it has
+// four children or only leaves. Nodes of a level are numbered in order after the previous level,
+// and the exact traversal stack bound is carried down the levels.
+#include <algorithm>
+
+#include "wf_device.h"
+#include "wf_internal.h"
+
+namespace wf {
+
+constexpr int kPlocBlock = 256;
+constexpr int kPlocMaxRadius = 32;
+
+__device__ __forceinline__ float half_area(const float4& l, const float4& h) {
+    const float dx = h.x - l.x, dy = h.y - l.y, dz = h.z - l.z;
+    return dx * dy + dy * dz + dz * dx;
+}
+
+__device__ __forceinline__ float union_area(const float4& la, const float4& ha, const float4& lb,
+                                            const float4& hb) {
+    const float4 l = make_float4(fminf(la.x, lb.x), fminf(la.y, lb.y), fminf(la.z, lb.z), 0.0f);
+    const float4 h = make_float4(fmaxf(ha.x, hb.x), fmaxf(ha.y, hb.y), fmaxf(ha.z, hb.z), 0.0f);
+    return half_area(l, h);
+}
+
+__global__ void k_ploc_init(int n, const float4* __restrict__ slo, const float4* __restrict__ shi,
+                            int32_t* __restrict__ cref, float4* __restrict__ clo,
+                            float4* __restrict__ chi, int32_t* __restrict__ lparent) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    cref[i] = ~i;
+    clo[i] = slo[i];
+    chi[i] = shi[i];
+    lparent[i] = -1;
+}
+
+// Nearest neighbour of every cluster within the window (boxes of the block's range plus the
+// halo staged in LDS).
+__global__ __launch_bounds__(kPlocBlock) void k_ploc_nn(int m, int radius,
+                                                        const float4* __restrict__ clo,
+                                                        const float4* __restrict__ chi,
+                                                        int32_t* __restrict__ nn) {
+    __shared__ float4 tl[kPlocBlock + 2 * kPlocMaxRadius];
+    __shared__ float4 th[kPlocBlock + 2 * kPlocMaxRadius];
+    const int b0 = blockIdx.x * kPlocBlock - radius;
+    const int span = kPlocBlock + 2 * radius;
+    for (int k = threadIdx.x; k < span; k += kPlocBlock) {
+        const int j = b0 + k;
+        if (j >= 0 && j < m) {
+            tl[k] = clo[j];
+            th[k] = chi[j];
+        }
+    }
+    __syncthreads();
+    const int i = blockIdx.x * kPlocBlock + threadIdx.x;
+    if (i >= m) return;
+    const int li = i - b0;
+    const float4 la = tl[li], ha = th[li];
+    float best = __int_as_float(0x7F800000);
+    int bj = -1, bdist = 0x7FFFFFFF, bpar = 2, bmin = 0x7FFFFFFF;
+    const int lo = i - radius < 0 ? 0 : i - radius;
+    const int hi = i + radius > m - 1 ? m - 1 : i + radius;
+    for (int j = lo; j <= hi; ++j) {
+        if (j == i) continue;
+        const float d = union_area(la, ha, tl[j - b0], th[j - b0]);
+        const int dist = j > i ? j - i : i - j;
+        const int mn = j < i ? j : i;
+        const int par = mn & 1;
+        const bool better = d < best ||
+                            (d == best && (dist < bdist ||
+                                           (dist == bdist && (par < bpar ||
+                                                              (par == bpar && mn < bmin)))));
+        if (better) {
+            best = d;
+            bj = j;
+            bdist = dist;
+            bpar = par;
+            bmin = mn;
+        }
+    }
+    nn[i] = bj;
+}
+
+// Merge flags (lower position of a mutual pair) and survival flags.
+__global__ void k_ploc_flags(int m, const int32_t* __restrict__ nn, uint32_t* __restrict__ merge,
+                             uint32_t* __restrict__ keep) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int j = nn[i];
+    const bool mutual = j >= 0 && nn[j] == i;
+    merge[i] = (mutual && i < j) ? 1u : 0u;
+    keep[i] = (mutual && i > j) ? 0u : 1u;
+}
+
+// Creates the merged nodes and writes the surviving clusters compacted into the other buffer.
+__global__ void k_ploc_write(int m, int n, const int32_t* __restrict__ created,
+                             const int32_t* __restrict__ nn, const uint32_t* __restrict__ merge,
+                             const uint32_t* __restrict__ mpos, const uint32_t* __restrict__ keep,
+                             const uint32_t* __restrict__ kpos, const int32_t* __restrict__ cref,
+                             const float4* __restrict__ clo, const float4* __restrict__ chi,
+                             int32_t* __restrict__ oref, float4* __restrict__ olo,
+                             float4* __restrict__ ohi, int32_t* __restrict__ left,
+                             int32_t* __restrict__ right, int32_t* __restrict__ iparent,
+                             int32_t* __restrict__ lparent, float4* __restrict__ nlo,
+                             float4* __restrict__ nhi) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m || !keep[i]) return;
+    const int k = (int)kpos[i];
+    if (!merge[i]) {
+        oref[k] = cref[i];
+        olo[k] = clo[i];
+        ohi[k] = chi[i];
+        return;
+    }
+    const int j = nn[i];
+    const int node = n - 2 - (*created + (int)mpos[i]);
+    const int a = cref[i], b = cref[j];
+    const float4 la = clo[i], ha = chi[i], lb = clo[j], hb = chi[j];
+    const float4 L = make_float4(fminf(la.x, lb.x), fminf(la.y, lb.y), fminf(la.z, lb.z), 0.0f);
+    const float4 H = make_float4(fmaxf(ha.x, hb.x), fmaxf(ha.y, hb.y), fmaxf(ha.z, hb.z), 0.0f);
+    left[node] = a;
+    right[node] = b;
+    if (a < 0) lparent[~a] = node; else iparent[a] = node;
+    if (b < 0) lparent[~b] = node; else iparent[b] = node;
+    nlo[node] = L;
+    nhi[node] = H;
+    oref[k] = node;
+    olo[k] = L;
+    ohi[k] = H;
+}
+
+__global__ void k_ploc_commit(int32_t* created, const int32_t* merges) { *created += *merges; }
+
+// ---- greedy surface-area collapse --------------------------------------------------------------
+__device__ __forceinline__ float ref_area(int ref, const float4* slo, const float4* shi,
+                                          const float4* nlo, const float4* nhi) {
+    return ref < 0 ? half_area(slo[~ref], shi[~ref]) : half_area(nlo[ref], nhi[ref]);
+}
+
+// Chooses the (up to four) children of each queued binary node; kint = inner children.
+__global__ void k_collapse_pick(int count, const int32_t* __restrict__ qb,
+                                const int32_t* __restrict__ left, const int32_t* __restrict__ right,
+                                const float4* __restrict__ slo, const float4* __restrict__ shi,
+                                const float4* __restrict__ nlo, const float4* __restrict__ nhi,
+                                int4* __restrict__ picked, uint32_t* __restrict__ kint) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int b = qb[t];
+    int c[4] = {left[b], right[b], kEmptyChild, kEmptyChild};
+    int m = 2;
+    while (m < 4) {
+        int best = -1;
+        float ba = -1.0f;
+        for (int k = 0; k < m; ++k)
+            if (c[k] >= 0) {
+                const float a = ref_area(c[k], slo, shi, nlo, nhi);
+                if (a > ba) {
+                    ba = a;
+                    best = k;
+                }
+            }
+        if (best < 0) break;
+        const int open = c[best];
+        c[best] = left[open];
+        c[m++] = right[open];
+    }
+    uint32_t k = 0;
+    for (int q = 0; q < 4; ++q) k += (c[q] >= 0 && c[q] != kEmptyChild) ? 1u : 0u;
+    picked[t] = make_int4(c[0], c[1], c[2], c[3]);
+    kint[t] = k;
+}
+
+// Writes the four-wide nodes of a level and queues their inner children for the next level.
+__global__ void k_collapse_write(int count, int next_base, const int32_t* __restrict__ qw,
+                                 const int32_t* __restrict__ qd, const int4* __restrict__ picked,
+                                 const uint32_t* __restrict__ kint,
+                                 const uint32_t* __restrict__ koff,
+                                 const float4* __restrict__ slo, const float4* __restrict__ shi,
+                                 const float4* __restrict__ nlo, const float4* __restrict__ nhi,
+                                 Bvh4Node* __restrict__ out, int32_t* __restrict__ nqb,
+                                 int32_t* __restrict__ nqw, int32_t* __restrict__ nqd,
+                                 int32_t* __restrict__ stack_bound) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const int4 p = picked[t];
+    const int c[4] = {p.x, p.y, p.z, p.w};
+    const int k = (int)kint[t];
+    const int depth_in = qd[t];
+    const int child_depth = depth_in + (k > 0 ? k - 1 : 0);
+    atomicMax(stack_bound, child_depth);
+    float4 lo[4], hi[4];
+    int ch[4];
+    int r = 0;
+    for (int q = 0; q < 4; ++q) {
+        if (c[q] == kEmptyChild) {
+            lo[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            hi[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            ch[q] = kEmptyChild;
+        } else if (c[q] < 0) {
+            lo[q] = slo[~c[q]];
+            hi[q] = shi[~c[q]];
+            ch[q] = c[q];
+        } else {
+            lo[q] = nlo[c[q]];
+            hi[q] = nhi[c[q]];
+            const int slot = (int)koff[t] + r++;
+            ch[q] = next_base + slot;
+            nqb[slot] = c[q];
+            nqw[slot] = next_base + slot;
+            nqd[slot] = child_depth;
+        }
+    }
+    Bvh4Node nd;
+    nd.lox = make_float4(lo[0].x, lo[1].x, lo[2].x, lo[3].x);
+    nd.loy = make_float4(lo[0].y, lo[1].y, lo[2].y, lo[3].y);
+    nd.loz = make_float4(lo[0].z, lo[1].z, lo[2].z, lo[3].z);
+    nd.hix = make_float4(hi[0].x, hi[1].x, hi[2].x, hi[3].x);
+    nd.hiy = make_float4(hi[0].y, hi[1].y, hi[2].y, hi[3].y);
+    nd.hiz = make_float4(hi[0].z, hi[1].z, hi[2].z, hi[3].z);
+    nd.child = make_int4(ch[0], ch[1], ch[2], ch[3]);
+    nd.pad = make_int4(0, 0, 0, 0);
+    out[qw[t]] = nd;
+}
+
+__global__ void k_set_root(int32_t* qb, int32_t* qw, int32_t* qd, int32_t* iparent) {
+    qb[0] = 0;
+    qw[0] = 0;
+    qd[0] = 0;
+    iparent[0] = -1;
+}
+
+wf_status ploc_build(wf_context* ctx, int n, int radius, const float4* slo, const float4* shi,
+                     int32_t* left, int32_t* right, int32_t* iparent, int32_t* lparent,
+                     float4* nlo, float4* nhi) {
+    radius = std::max(1, std::min(radius, kPlocMaxRadius));
+    void *pr[2], *pl[2], *ph[2], *pnn, *pm, *pmp, *pk, *pkp, *pcnt;
+    WF_TRY(workspace(ctx, "ploc_ref0", sizeof(int32_t) * n, &pr[0]));
+    WF_TRY(workspace(ctx, "ploc_ref1", sizeof(int32_t) * n, &pr[1]));
+    WF_TRY(workspace(ctx, "ploc_lo0", sizeof(float4) * n, &pl[0]));
+    WF_TRY(workspace(ctx, "ploc_lo1", sizeof(float4) * n, &pl[1]));
+    WF_TRY(workspace(ctx, "ploc_hi0", sizeof(float4) * n, &ph[0]));
+    WF_TRY(workspace(ctx, "ploc_hi1", sizeof(float4) * n, &ph[1]));
+    WF_TRY(workspace(ctx, "ploc_nn", sizeof(int32_t) * n, &pnn));
+    WF_TRY(workspace(ctx, "ploc_merge", sizeof(uint32_t) * n, &pm));
+    WF_TRY(workspace(ctx, "ploc_mpos", sizeof(uint32_t) * n, &pmp));
+    WF_TRY(workspace(ctx, "ploc_keep", sizeof(uint32_t) * n, &pk));
+    WF_TRY(workspace(ctx, "ploc_kpos", sizeof(uint32_t) * n, &pkp));
+    WF_TRY(workspace(ctx, "ploc_counts", sizeof(int32_t) * 4, &pcnt));
+    int32_t* cnt = (int32_t*)pcnt;   // [0] created, [1] merges, [2] survivors
+    WF_TRY(wf_fill_u32(ctx, pcnt, 0u, 4));
+    WF_TRY(wf_fill_u32(ctx, iparent, 0xFFFFFFFFu, (size_t)n));
+    k_ploc_init<<<grid_for(n, 256), 256, 0, ctx->stream>>>(n, slo, shi, (int32_t*)pr[0],
+                                                           (float4*)pl[0], (float4*)ph[0], lparent);
+    WF_HIP(hipGetLastError());
+    int m = n, cur = 0, iters = 0;
+    // every iteration merges at least the globally closest pair; the pairing rule halves runs of
+    // equal boxes, so the count is small in practice: the cap only guards against pathologies
+    const int max_iters = 4096;
+    while (m > 1) {
+        if (++iters > max_iters)
+            return set_error(WF_ERR_UNSUPPORTED, "PLOC did not converge in %d iterations", max_iters);
+        const int nxt = cur ^ 1;
+        k_ploc_nn<<<grid_for(m, kPlocBlock), kPlocBlock, 0, ctx->stream>>>(
+            m, radius, (float4*)pl[cur], (float4*)ph[cur], (int32_t*)pnn);
+        WF_HIP(hipGetLastError());
+        k_ploc_flags<<<grid_for(m, 256), 256, 0, ctx->stream>>>(m, (int32_t*)pnn, (uint32_t*)pm,
+                                                                (uint32_t*)pk);
+        WF_HIP(hipGetLastError());
+        WF_TRY(exclusive_scan_u32(ctx, (uint32_t*)pm, (uint32_t*)pmp, host_count(m), cnt + 1,
+                                  "bvh_build"));
+        WF_TRY(exclusive_scan_u32(ctx, (uint32_t*)pk, (uint32_t*)pkp, host_count(m), cnt + 2,
+                                  "bvh_build"));
+        k_ploc_write<<<grid_for(m, 256), 256, 0, ctx->stream>>>(
+            m, n, cnt, (int32_t*)pnn, (uint32_t*)pm, (uint32_t*)pmp, (uint32_t*)pk,
+            (uint32_t*)pkp, (int32_t*)pr[cur], (float4*)pl[cur], (float4*)ph[cur],
+            (int32_t*)pr[nxt], (float4*)pl[nxt], (float4*)ph[nxt], left, right, iparent, lparent,
+            nlo, nhi);
+        WF_HIP(hipGetLastError());
+        k_ploc_commit<<<1, 1, 0, ctx->stream>>>(cnt, cnt + 1);
+        WF_HIP(hipGetLastError());
+        int32_t survivors = 0;
+        WF_TRY(readback(ctx, &survivors, cnt + 2, sizeof(int32_t)));
+        if (survivors >= m) return set_error(WF_ERR_HIP, "PLOC made no progress");
+        m = survivors;
+        cur = nxt;
+    }
+    int32_t created = 0;
+    WF_TRY(readback(ctx, &created, cnt, sizeof(int32_t)));
+    if (created != n - 1) return set_error(WF_ERR_HIP, "PLOC created %d nodes for %d leaves", created, n);
+    return WF_OK;
+}
+
+wf_status collapse_greedy(wf_context* ctx, int n, const int32_t* left, const int32_t* right,
+                          int32_t* iparent, const float4* slo, const float4* shi,
+                          const float4* nlo, const float4* nhi, Bvh4Node* out,
+                          int32_t* n_nodes4, int32_t* stack_bound) {
+    const int ni = n - 1;
+    void *pqb[2], *pqw[2], *pqd[2], *ppick, *pk, *pkoff, *pcnt;
+    for (int k = 0; k < 2; ++k) {
+        char name[32];
+        snprintf(name, sizeof(name), "col_qb%d", k);
+        WF_TRY(workspace(ctx, name, sizeof(int32_t) * ni, &pqb[k]));
+        snprintf(name, sizeof(name), "col_qw%d", k);
+        WF_TRY(workspace(ctx, name, sizeof(int32_t) * ni, &pqw[k]));
+        snprintf(name, sizeof(name), "col_qd%d", k);
+        WF_TRY(workspace(ctx, name, sizeof(int32_t) * ni, &pqd[k]));
+    }
+    WF_TRY(workspace(ctx, "col_pick", sizeof(int4) * ni, &ppick));
+    WF_TRY(workspace(ctx, "col_kint", sizeof(uint32_t) * ni, &pk));
+    WF_TRY(workspace(ctx, "col_koff", sizeof(uint32_t) * ni, &pkoff));
+    WF_TRY(workspace(ctx, "col_counts", sizeof(int32_t) * 4, &pcnt));
+    int32_t* cnt = (int32_t*)pcnt;   // [0] next level size, [1] stack bound
+    WF_TRY(wf_fill_u32(ctx, pcnt, 0u, 4));
+    k_set_root<<<1, 1, 0, ctx->stream>>>((int32_t*)pqb[0], (int32_t*)pqw[0], (int32_t*)pqd[0],
+                                         iparent);
+    WF_HIP(hipGetLastError());
+    int count = 1, total = 1, cur = 0;
+    while (count > 0) {
+        const int nxt = cur ^ 1;
+        k_collapse_pick<<<grid_for(count, 256), 256, 0, ctx->stream>>>(
+            count, (int32_t*)pqb[cur], left, right, slo, shi, nlo, nhi, (int4*)ppick,
+            (uint32_t*)pk);
+        WF_HIP(hipGetLastError());
+        WF_TRY(exclusive_scan_u32(ctx, (uint32_t*)pk, (uint32_t*)pkoff, host_count(count), cnt,
+                                  "bvh_build"));
+        k_collapse_write<<<grid_for(count, 256), 256, 0, ctx->stream>>>(
+            count, total, (int32_t*)pqw[cur], (int32_t*)pqd[cur], (int4*)ppick, (uint32_t*)pk,
+            (uint32_t*)pkoff, slo, shi, nlo, nhi, out, (int32_t*)pqb[nxt], (int32_t*)pqw[nxt],
+            (int32_t*)pqd[nxt], cnt + 1);
+        WF_HIP(hipGetLastError());
+        int32_t next = 0;
+        WF_TRY(readback(ctx, &next, cnt, sizeof(int32_t)));
+        total += next;
+        if (total > ni) return set_error(WF_ERR_HIP, "four-wide collapse overflow");
+        count = next;
+        cur = nxt;
+    }
+    int32_t bound = 0;
+    WF_TRY(readback(ctx, &bound, cnt + 1, sizeof(int32_t)));
+    *n_nodes4 = total;
+    *stack_bound = bound;
+    return WF_OK;
+}
+
+}  // namespace wf

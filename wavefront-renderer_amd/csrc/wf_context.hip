@@ -260,6 +260,10 @@ wf_status wf_context_set_tuning(wf_context* ctx, const wf_tuning* t) {
                "wf_context_set_tuning: refill_lanes %d not in [1, 64]", t->refill_lanes);
     WF_REQUIRE(t->trace_blocks_per_cu >= 1 && t->trace_blocks_per_cu <= 64,
                "wf_context_set_tuning: trace_blocks_per_cu %d not in [1, 64]", t->trace_blocks_per_cu);
+    WF_REQUIRE(t->bvh_builder == WF_BVH_PLOC || t->bvh_builder == WF_BVH_LBVH,
+               "wf_context_set_tuning: unknown bvh_builder %d", t->bvh_builder);
+    WF_REQUIRE(t->ploc_radius >= 1 && t->ploc_radius <= 32,
+               "wf_context_set_tuning: ploc_radius %d not in [1, 32]", t->ploc_radius);
     ctx->tuning = *t;
     return WF_OK;
 }

@@ -73,7 +73,7 @@ struct wf_context {
     std::vector<wf::PendingTiming> pending;
     std::vector<hipEvent_t> event_pool;
     int32_t* count_slots = nullptr;    // pinned, kCountSlots entries
-    wf_tuning tuning{48, 6, {0, 0, 0, 0, 0, 0}};
+    wf_tuning tuning{48, 6, WF_BVH_PLOC, 16, {0, 0, 0, 0}};
     int next_count_slot = 0;
     int cu_count = 256;
 };
@@ -325,6 +325,15 @@ wf_status shade_misses(wf_context* ctx, const wf_scene* s, const wf_ray_soa* q, 
                        int32_t physics, float* accum);
 wf_status render_regenerate(wf_scene* s, const wf_render_params* rp, float* image_rgb,
                             int32_t* sample_counts, wf_render_stats* stats);
+// Binary hierarchy by PLOC over n >= 2 Morton-ordered leaf boxes (Karras-layout outputs).
+wf_status ploc_build(wf_context* ctx, int n, int radius, const float4* slo, const float4* shi,
+                     int32_t* left, int32_t* right, int32_t* iparent, int32_t* lparent,
+                     float4* nlo, float4* nhi);
+// Greedy surface-area collapse of a binary hierarchy (root 0) into four-wide nodes.
+wf_status collapse_greedy(wf_context* ctx, int n, const int32_t* left, const int32_t* right,
+                          int32_t* iparent, const float4* slo, const float4* shi,
+                          const float4* nlo, const float4* nhi, Bvh4Node* out,
+                          int32_t* n_nodes4, int32_t* stack_bound);
 wf_status ray_keys(wf_context* ctx, const wf_ray_soa* rays, Count n, const float bmin[3],
                    const float bmax[3], uint64_t* keys);
 wf_status shading_keys(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q, Count n,

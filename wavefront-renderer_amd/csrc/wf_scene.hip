@@ -292,97 +292,16 @@ __global__ void k_bvh_depth(int n, const int32_t* __restrict__ iparent,
                             const int32_t* __restrict__ lparent, int32_t* max_depth) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
+    // a valid hierarchy has fewer than n ancestors per leaf: the bound turns a malformed tree
+    // into a reported error instead of an endless walk
     int d = 0;
     int p = lparent[j];
-    while (p >= 0) {
+    while (p >= 0 && d < n) {
         ++d;
-        p = iparent[p];
+        p = (p < n - 1) ? iparent[p] : -1;
     }
+    if (p >= 0 || lparent[j] >= n - 1) d = n;
     atomicMax(max_depth, d);
-}
-
-// Depth of every two-child internal node (root = 0).
-__global__ void k_bvh_node_depth(int n_internal, const int32_t* __restrict__ iparent,
-                                 int32_t* __restrict__ depth, uint32_t* __restrict__ even) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_internal) return;
-    int d = 0;
-    for (int p = iparent[i]; p >= 0; p = iparent[p]) ++d;
-    depth[i] = d;
-    even[i] = (d & 1) == 0 ? 1u : 0u;
-}
-
-// Collapse: every even-depth two-child node becomes a four-wide node whose children are its
-// grandchildren (or its children where those are leaves).
-__global__ void k_bvh4_collapse(int n_internal, const int32_t* __restrict__ left,
-                                const int32_t* __restrict__ right,
-                                const uint32_t* __restrict__ even,
-                                const uint32_t* __restrict__ map4,
-                                const float4* __restrict__ slo, const float4* __restrict__ shi,
-                                const float4* __restrict__ nlo, const float4* __restrict__ nhi,
-                                Bvh4Node* __restrict__ out, int32_t* __restrict__ kint) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_internal || !even[i]) return;
-    int32_t ch[4];
-    float4 lo[4], hi[4];
-    int m = 0, internal = 0;
-    const int c2[2] = {left[i], right[i]};
-    for (int a = 0; a < 2; ++a) {
-        const int c = c2[a];
-        if (c < 0) {
-            ch[m] = c;
-            lo[m] = slo[~c];
-            hi[m] = shi[~c];
-            ++m;
-            continue;
-        }
-        const int g2[2] = {left[c], right[c]};
-        for (int b = 0; b < 2; ++b) {
-            const int g = g2[b];
-            if (g < 0) {
-                ch[m] = g;
-                lo[m] = slo[~g];
-                hi[m] = shi[~g];
-            } else {
-                ch[m] = (int32_t)map4[g];
-                lo[m] = nlo[g];
-                hi[m] = nhi[g];
-                ++internal;
-            }
-            ++m;
-        }
-    }
-    for (int k = m; k < 4; ++k) {
-        ch[k] = kEmptyChild;
-        lo[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        hi[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    }
-    Bvh4Node nd;
-    nd.lox = make_float4(lo[0].x, lo[1].x, lo[2].x, lo[3].x);
-    nd.loy = make_float4(lo[0].y, lo[1].y, lo[2].y, lo[3].y);
-    nd.loz = make_float4(lo[0].z, lo[1].z, lo[2].z, lo[3].z);
-    nd.hix = make_float4(hi[0].x, hi[1].x, hi[2].x, hi[3].x);
-    nd.hiy = make_float4(hi[0].y, hi[1].y, hi[2].y, hi[3].y);
-    nd.hiz = make_float4(hi[0].z, hi[1].z, hi[2].z, hi[3].z);
-    nd.child = make_int4(ch[0], ch[1], ch[2], ch[3]);
-    nd.pad = make_int4(0, 0, 0, 0);
-    const uint32_t j = map4[i];
-    out[j] = nd;
-    kint[j] = internal;
-}
-
-// Traversal stack bound: at a four-wide node the stack holds at most the unvisited internal
-// siblings pushed at each four-wide ancestor, plus the ones pushed at the node itself.
-__global__ void k_bvh4_stack_bound(int n_internal, const int32_t* __restrict__ iparent,
-                                   const uint32_t* __restrict__ even,
-                                   const uint32_t* __restrict__ map4,
-                                   const int32_t* __restrict__ kint, int32_t* __restrict__ bound) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_internal || !even[i]) return;
-    int s = 0;
-    for (int p = i; p >= 0; p = iparent[p])
-        if (even[p]) s += kint[map4[p]] > 0 ? kint[map4[p]] - 1 : 0;
-    atomicMax(bound, s);
 }
 
 __global__ void k_bvh4_single(const float4* __restrict__ slo, const float4* __restrict__ shi,
@@ -760,44 +679,36 @@ wf_status wf_accel_build(wf_scene* s, wf_build_stats* stats) {
             WF_TRY(workspace(ctx, "bvh_depth", sizeof(int32_t), &pdep));
             WF_TRY(wf_fill_u32(ctx, pvis, 0u, (size_t)n));
             WF_TRY(wf_fill_u32(ctx, pdep, 0u, 1));
-            k_bvh_karras<<<grid_for(n - 1, 256), 256, 0, ctx->stream>>>(
-                (uint64_t*)pkeys2, n, (int32_t*)pl, (int32_t*)pr, (int32_t*)pip, (int32_t*)plp);
-            WF_HIP(hipGetLastError());
-            k_bvh_refit<<<grid_for(n, 256), 256, 0, ctx->stream>>>(
-                n, (int32_t*)pl, (int32_t*)pr, (int32_t*)pip, (int32_t*)plp, (float4*)pslo,
-                (float4*)pshi, (float4*)pnlo, (float4*)pnhi, (uint32_t*)pvis);
-            WF_HIP(hipGetLastError());
+            // binary hierarchy: PLOC clustering (default) or the Karras radix tree
+            bool built = false;
+            if (ctx->tuning.bvh_builder == WF_BVH_PLOC) {
+                wf_status pst = ploc_build(ctx, n, ctx->tuning.ploc_radius, (float4*)pslo,
+                                           (float4*)pshi, (int32_t*)pl, (int32_t*)pr,
+                                           (int32_t*)pip, (int32_t*)plp, (float4*)pnlo,
+                                           (float4*)pnhi);
+                if (pst == WF_OK) built = true;
+                else if (pst != WF_ERR_UNSUPPORTED) return pst;
+                else clear_error();
+            }
+            if (!built) {
+                k_bvh_karras<<<grid_for(n - 1, 256), 256, 0, ctx->stream>>>(
+                    (uint64_t*)pkeys2, n, (int32_t*)pl, (int32_t*)pr, (int32_t*)pip, (int32_t*)plp);
+                WF_HIP(hipGetLastError());
+                k_bvh_refit<<<grid_for(n, 256), 256, 0, ctx->stream>>>(
+                    n, (int32_t*)pl, (int32_t*)pr, (int32_t*)pip, (int32_t*)plp, (float4*)pslo,
+                    (float4*)pshi, (float4*)pnlo, (float4*)pnhi, (uint32_t*)pvis);
+                WF_HIP(hipGetLastError());
+            }
             k_bvh_depth<<<grid_for(n, 256), 256, 0, ctx->stream>>>(n, (int32_t*)pip,
                                                                    (int32_t*)plp, (int32_t*)pdep);
             WF_HIP(hipGetLastError());
             WF_TRY(readback(ctx, &depth, pdep, sizeof(int32_t)));
+            if (depth >= n) return set_error(WF_ERR_HIP, "BVH build produced a malformed hierarchy");
             d.n_nodes = n - 1;
-            // collapse to four-wide nodes
-            const int ni = n - 1;
-            void *pnd, *pev, *pmap, *pkint, *pcnt;
-            WF_TRY(workspace(ctx, "bvh_node_depth", sizeof(int32_t) * ni, &pnd));
-            WF_TRY(workspace(ctx, "bvh_even", sizeof(uint32_t) * ni, &pev));
-            WF_TRY(workspace(ctx, "bvh_map4", sizeof(uint32_t) * ni, &pmap));
-            WF_TRY(workspace(ctx, "bvh_kint", sizeof(int32_t) * ni, &pkint));
-            WF_TRY(workspace(ctx, "bvh4_counts", sizeof(int32_t) * 4, &pcnt));
-            k_bvh_node_depth<<<grid_for(ni, 256), 256, 0, ctx->stream>>>(
-                ni, (int32_t*)pip, (int32_t*)pnd, (uint32_t*)pev);
-            WF_HIP(hipGetLastError());
-            WF_TRY(exclusive_scan_u32(ctx, (uint32_t*)pev, (uint32_t*)pmap, host_count(ni), (int32_t*)pcnt,
-                                      "bvh_scan"));
-            k_bvh4_collapse<<<grid_for(ni, 256), 256, 0, ctx->stream>>>(
-                ni, (int32_t*)pl, (int32_t*)pr, (uint32_t*)pev, (uint32_t*)pmap, (float4*)pslo,
-                (float4*)pshi, (float4*)pnlo, (float4*)pnhi, d.nodes4, (int32_t*)pkint);
-            WF_HIP(hipGetLastError());
-            WF_TRY(wf_fill_u32(ctx, (int32_t*)pcnt + 1, 0u, 1));
-            k_bvh4_stack_bound<<<grid_for(ni, 256), 256, 0, ctx->stream>>>(
-                ni, (int32_t*)pip, (uint32_t*)pev, (uint32_t*)pmap, (int32_t*)pkint,
-                (int32_t*)pcnt + 1);
-            WF_HIP(hipGetLastError());
-            int32_t c[2] = {0, 0};
-            WF_TRY(readback(ctx, c, pcnt, sizeof(c)));
-            d.n_nodes4 = c[0];
-            d.stack_bound = c[1];
+            // greedy surface-area collapse to four-wide nodes
+            WF_TRY(collapse_greedy(ctx, n, (int32_t*)pl, (int32_t*)pr, (int32_t*)pip,
+                                   (float4*)pslo, (float4*)pshi, (float4*)pnlo, (float4*)pnhi,
+                                   d.nodes4, &d.n_nodes4, &d.stack_bound));
         }
         d.max_depth = depth;
     } else {
