@@ -106,11 +106,20 @@ def chain_scene(scenes):
     return scenes.SceneData([scenes.Mesh(pos, idx, 0)], [{"type": 0}]), c
 
 
-def test_deep_tree_closest_hit_matches_bruteforce(wf, ctx, scenes, orc, rng):
+@pytest.mark.parametrize("builder", ["lbvh", "ploc"])
+def test_deep_tree_closest_hit_matches_bruteforce(wf, ctx, scenes, orc, rng, builder):
     data, centres = chain_scene(scenes)
-    sc = wf.Scene(ctx, data)
-    print("chain scene: binary depth", sc.stats.max_depth, "stack bound", sc.stats.stack_bound)
-    assert sc.stats.max_depth > 32 and sc.stats.stack_bound <= 64
+    saved = ctx.tuning()
+    ctx.set_tuning(bvh_builder=wf._abi.BVH_LBVH if builder == "lbvh" else wf._abi.BVH_PLOC)
+    try:
+        sc = wf.Scene(ctx, data)
+    finally:
+        ctx.set_tuning(**saved)
+    print("chain scene:", builder, "binary depth", sc.stats.max_depth, "stack bound",
+          sc.stats.stack_bound)
+    if builder == "lbvh":
+        assert sc.stats.max_depth > 32     # the Karras tree follows the code bits one by one
+    assert sc.stats.stack_bound <= 64
     osc = orc.scene(data)
     n = 6000
     o = rng.uniform(-0.2, 1.2, (n, 3))
@@ -128,11 +137,17 @@ def test_deep_tree_closest_hit_matches_bruteforce(wf, ctx, scenes, orc, rng):
     assert np.array_equal(hits["t"][h], ref["t"][h])
 
 
-def test_large_scene_closest_hit_matches_oracle(wf, ctx, scenes, orc, rng):
+@pytest.mark.parametrize("builder", ["lbvh", "ploc"])
+def test_large_scene_closest_hit_matches_oracle(wf, ctx, scenes, orc, rng, builder):
     """~330 k triangles: the oracle's SAH BVH (itself pinned against brute force) against the
-    device LBVH, over camera rays and random rays."""
+    device BVH of either builder, over camera rays and random rays."""
     data = scenes.sorting_stress(subdiv=6, tex_size=8)
-    sc = wf.Scene(ctx, data)
+    saved = ctx.tuning()
+    ctx.set_tuning(bvh_builder=wf._abi.BVH_LBVH if builder == "lbvh" else wf._abi.BVH_PLOC)
+    try:
+        sc = wf.Scene(ctx, data)
+    finally:
+        ctx.set_tuning(**saved)
     osc = orc.scene(data)
     rays = camera_rays(orc, data, 128, 96)
     extra = random_rays(rng, 20000)
