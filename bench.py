@@ -309,6 +309,9 @@ def main() -> None:
 
     policies = {}
     if not args.no_policies:
+        # phase times need events between the phases: time no kernel, but keep timing on
+        ctx.profile_filter("phases")
+        ctx.profile(True)
         for pol in ("none", "material", "texture", "costBenefit"):
             t1 = time.perf_counter()
             st = frame(pol, seed=7)
@@ -320,6 +323,8 @@ def main() -> None:
                              "schedule_ms": round(st.schedule_ms, 1),
                              "shade_ms": round(st.shade_ms, 1)}
             log(f"[bench] policy {pol}: {policies[pol]}")
+        ctx.profile(False)
+        ctx.profile_filter(None)
 
     cpu = None
     if not args.no_cpu_baseline:

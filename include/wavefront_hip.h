@@ -344,6 +344,10 @@ typedef struct wf_render_stats {  /* the reference's stats blocks (cpp:321-370) 
 #define WF_SCHEDULE_REGENERATE 0
 #define WF_SCHEDULE_PASSES 1
 
+/* Phase times in wf_render_stats (sort/intersect/schedule/shade/miss) are measured while kernel
+ * timing is enabled (wf_profile_enable); otherwise they are 0 and the regenerating schedule
+ * replays each iteration from a captured HIP graph. */
+
 /* Renders one image. image_rgb: caller-owned device float buffer width*height*3 receiving
  * accumulator / per-pixel sample count (writeAveragedImage, cpp:127-137). sample_counts
  * (optional device int32[width*height]) receives the per-pixel sample counts. */

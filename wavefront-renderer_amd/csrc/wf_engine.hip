@@ -451,16 +451,20 @@ wf_status enqueue_status(wf_engine* e, int b) {
     return WF_OK;
 }
 
+void drop_graph(wf_engine* e) {
+    if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    e->graph_exec = nullptr;
+    e->graph = nullptr;
+}
+
 void copy_profile_flags(wf_engine* e) {
     e->ctx->profile = e->parent->profile;
     e->ctx->profile_only = e->parent->profile_only;
     if (std::memcmp(&e->ctx->tuning, &e->parent->tuning, sizeof(wf_tuning)) != 0) {
         e->ctx->tuning = e->parent->tuning;
         // a captured iteration bakes the launch shapes in: capture again
-        if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
-        if (e->graph) (void)hipGraphDestroy(e->graph);
-        e->graph_exec = nullptr;
-        e->graph = nullptr;
+        drop_graph(e);
     }
 }
 
@@ -838,7 +842,7 @@ wf_status render_regenerate(wf_scene* s, const wf_render_params* rp, float* imag
     cfg.pixel_budget = P;
     cfg.policy = rp->policy;
     cfg.ray_sort = rp->ray_sort;
-    cfg.use_graph = 0;
+    cfg.use_graph = 1;
     cfg.shade = rp->shade;
     if (!e) {
         WF_TRY(wf_engine_create(s->ctx, s, &cfg, &e));
@@ -846,9 +850,15 @@ wf_status render_regenerate(wf_scene* s, const wf_render_params* rp, float* imag
     } else {
         cfg.capacity = e->cap;
         cfg.pixel_budget = e->px;
-        e->cfg = cfg;
+        if (std::memcmp(&cfg, &e->cfg, sizeof(cfg)) != 0) {
+            // the captured iteration bakes the policy and shading settings in
+            drop_graph(e);
+            e->cfg = cfg;
+        }
     }
-    WF_TRY(wf_engine_phase_timing(e, 1));
+    // phase times only while the context times kernels (they need events between the phases,
+    // which a replayed graph does not have)
+    WF_TRY(wf_engine_phase_timing(e, s->ctx->profile ? 1 : 0));
     wf_engine_stats st0{}, st1{};
     WF_TRY(wf_engine_query(e, &st0));
     hipEvent_t f0, f1;
