@@ -2,13 +2,12 @@
 // prefix sum over u32 counts. Both are deterministic: ranks inside a tile are computed with
 // wave-level ballots in tile order, never with atomics whose order could vary.
 //
-// Radix sort pass (8-bit digit, 2048-key tiles, 256 threads = 4 waves of 64):
-//   1. k_radix_hist    per-tile digit histogram -> hist[digit * tiles + tile]
-//   2. exclusive scan  over hist (digit-major), giving each (digit, tile) its output base
-//   3. k_radix_scatter each wave ranks its 512 keys round by round: lanes holding the same digit
-//      are matched with 8 ballots, the rank is the popcount of matching lower lanes plus the
-//      wave's running count for that digit; waves are then offset by an exclusive prefix over
-//      waves, so equal digits keep input order (stability).
+// Radix sort (8-bit digits, 2048-key tiles, 256 threads = 4 waves of 64): one histogram sweep
+// gives every pass's digit totals; each pass is then a single kernel (k_sweep_pass) in which
+// each wave ranks its 512 keys round by round: lanes holding the same digit are matched with 8
+// ballots, the rank is the popcount of matching lower lanes plus the wave's running count for
+// that digit; waves are offset by an exclusive prefix over waves (so equal digits keep input
+// order), and tiles by the published counts of all earlier tiles (decoupled look-back).
 #include <algorithm>
 
 #include "wf_device.h"
@@ -21,43 +20,89 @@ constexpr int kSortItems = 8;
 constexpr int kSortTile = kSortBlock * kSortItems;
 constexpr int kSortWaves = kSortBlock / 64;
 
-__global__ __launch_bounds__(kSortBlock) void k_radix_hist(const uint64_t* __restrict__ keys,
+// ---- one-sweep passes ------------------------------------------------------------------------
+// Digit counts of every pass come from one histogram sweep over the keys; each pass is then one
+// kernel: a block takes the next tile id from a counter, ranks its 2048 keys as above, publishes
+// its per-digit counts, and resolves the counts of all earlier tiles by looking back through the
+// published states (a state is the tile's own count, or the inclusive prefix once known).
+// Every earlier tile id was taken by a block that is already running and publishes its count
+// before looking back itself, so every wait ends; the spin is still bounded, and an exhausted
+// bound is reported through `fault` instead of hanging.
+constexpr uint32_t kStateCount = 1u << 30;    // flag: tile count published
+constexpr uint32_t kStatePrefix = 2u << 30;   // flag: inclusive prefix published
+constexpr uint32_t kStateMask = (1u << 30) - 1u;
+constexpr int kMaxPasses = 8;
+constexpr int kSpinLimit = 1 << 22;
+
+__global__ __launch_bounds__(kSortBlock) void k_sweep_hist(const uint64_t* __restrict__ keys,
                                                            int n_host,
                                                            const int32_t* __restrict__ nd,
-                                                           int shift, uint32_t mask,
-                                                           uint32_t* __restrict__ hist,
-                                                           int tiles) {
+                                                           int lo_bit, int hi_bit, int passes,
+                                                           uint32_t* __restrict__ hist) {
     const int n = load_count(n_host, nd);
-    __shared__ uint32_t cnt[256];
-    cnt[threadIdx.x] = 0;
+    __shared__ uint32_t cnt[kMaxPasses][256];
+    for (int p = 0; p < passes; ++p) cnt[p][threadIdx.x] = 0;
     __syncthreads();
-    const int base = blockIdx.x * kSortTile;
-#pragma unroll
-    for (int k = 0; k < kSortItems; ++k) {
-        int i = base + k * kSortBlock + threadIdx.x;
-        if (i < n) atomicAdd(&cnt[(uint32_t)(keys[i] >> shift) & mask], 1u);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint64_t k = keys[i];
+        for (int p = 0; p < passes; ++p) {
+            // the same digit the pass ranks by: bits [shift, min(shift + 8, hi_bit + 1))
+            const int shift = lo_bit + 8 * p;
+            const int nbits = (hi_bit + 1 - shift) < 8 ? (hi_bit + 1 - shift) : 8;
+            atomicAdd(&cnt[p][(uint32_t)(k >> shift) & ((1u << nbits) - 1u)], 1u);
+        }
     }
     __syncthreads();
-    hist[(size_t)threadIdx.x * tiles + blockIdx.x] = cnt[threadIdx.x];
+    for (int p = 0; p < passes; ++p)
+        if (cnt[p][threadIdx.x]) atomicAdd(&hist[p * 256 + threadIdx.x], cnt[p][threadIdx.x]);
 }
 
-__global__ __launch_bounds__(kSortBlock) void k_radix_scatter(
+// hist[p][d] -> exclusive prefix over digits (the pass's output base of digit d).
+__global__ __launch_bounds__(256) void k_sweep_bases(uint32_t* hist) {
+    __shared__ uint32_t lds[4];
+    uint32_t* h = hist + blockIdx.x * 256;
+    const uint32_t x = h[threadIdx.x];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t v = x;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(v, o, 64);
+        if (lane >= o) v += y;
+    }
+    if (lane == 63) lds[wave] = v;
+    __syncthreads();
+    uint32_t off = 0;
+    for (int w = 0; w < wave; ++w) off += lds[w];
+    h[threadIdx.x] = off + v - x;
+}
+
+__device__ __forceinline__ uint32_t state_load(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void state_store(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ __launch_bounds__(kSortBlock) void k_sweep_pass(
     const uint64_t* __restrict__ keys_in, const int32_t* __restrict__ vals_in,
     uint64_t* __restrict__ keys_out, int32_t* __restrict__ vals_out, int n_host,
     const int32_t* __restrict__ nd, int shift, uint32_t mask, int nbits,
-    const uint32_t* __restrict__ offsets, int tiles) {
-    const int n = load_count(n_host, nd);
-    if (blockIdx.x * kSortTile >= n) return;
+    const uint32_t* __restrict__ bases, uint32_t* state, int32_t* tile_counter,
+    int32_t* __restrict__ fault) {
     __shared__ uint32_t wave_cnt[kSortWaves][256];
     __shared__ uint32_t digit_base[256];
+    __shared__ int tile_s;
+    const int n = load_count(n_host, nd);
+    if (threadIdx.x == 0) tile_s = atomicAdd(tile_counter, 1);
+    __syncthreads();
+    const int tile = tile_s;
+    if (tile * kSortTile >= n) return;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
 #pragma unroll
     for (int w = 0; w < kSortWaves; ++w) wave_cnt[w][threadIdx.x] = 0;
-    digit_base[threadIdx.x] = offsets[(size_t)threadIdx.x * tiles + blockIdx.x];
     __syncthreads();
 
-    const int chunk = blockIdx.x * kSortTile + wave * 64 * kSortItems;
+    const int chunk = tile * kSortTile + wave * 64 * kSortItems;
     uint64_t k[kSortItems];
     int32_t v[kSortItems];
     uint32_t d[kSortItems];
@@ -83,15 +128,43 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_scatter(
         if (valid && rank_in == 0) wave_cnt[wave][dig] = before + (uint32_t)__popcll(peers);
     }
     __syncthreads();
+    // per digit (thread = digit): wave offsets, tile count, publish, look back
+    uint32_t tile_count = 0;
     {
-        uint32_t run = 0;
 #pragma unroll
         for (int w = 0; w < kSortWaves; ++w) {
-            uint32_t c = wave_cnt[w][threadIdx.x];
-            wave_cnt[w][threadIdx.x] = run;
-            run += c;
+            const uint32_t c = wave_cnt[w][threadIdx.x];
+            wave_cnt[w][threadIdx.x] = tile_count;
+            tile_count += c;
         }
     }
+    uint32_t* my = state + (size_t)tile * 256 + threadIdx.x;
+    uint32_t excl = 0;
+    if (tile == 0) {
+        state_store(my, kStatePrefix | tile_count);
+    } else {
+        state_store(my, kStateCount | tile_count);
+        int j = tile - 1;
+        int spins = 0;
+        while (j >= 0) {
+            const uint32_t st = state_load(state + (size_t)j * 256 + threadIdx.x);
+            if (st & kStatePrefix) {
+                excl += st & kStateMask;
+                break;
+            }
+            if (st & kStateCount) {
+                excl += st & kStateMask;
+                --j;
+                continue;
+            }
+            if (++spins > kSpinLimit) {
+                atomicOr(fault, 1);
+                break;
+            }
+        }
+        state_store(my, kStatePrefix | (excl + tile_count));
+    }
+    digit_base[threadIdx.x] = bases[threadIdx.x] + excl;
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < kSortItems; ++it) {
@@ -357,11 +430,29 @@ wf_status radix_sort_pairs(wf_context* ctx, const uint64_t* keys_in, const int32
     const int span = hi_bit - lo_bit + 1;
     const int tiles = (n.n + kSortTile - 1) / kSortTile;
     const int passes = (span + 7) / 8;
-    void *pk = nullptr, *pv = nullptr, *ph = nullptr;
+    if (passes > kMaxPasses) return set_error(WF_ERR_INVALID_ARGUMENT, "sort key too wide");
+    void *pk = nullptr, *pv = nullptr, *ph = nullptr, *pst = nullptr, *pc = nullptr;
     WF_TRY(workspace(ctx, "sort_keys_alt", sizeof(uint64_t) * (size_t)n.n, &pk));
     WF_TRY(workspace(ctx, "sort_vals_alt", sizeof(int32_t) * (size_t)n.n, &pv));
-    WF_TRY(workspace(ctx, "sort_hist", sizeof(uint32_t) * 256 * (size_t)tiles, &ph));
+    WF_TRY(workspace(ctx, "sort_hist", sizeof(uint32_t) * 256 * kMaxPasses, &ph));
+    WF_TRY(workspace(ctx, "sort_state", sizeof(uint32_t) * 256 * (size_t)tiles * passes, &pst));
+    // [0, kMaxPasses) tile counters, [kMaxPasses] fault flag
+    WF_TRY(workspace(ctx, "sort_counters", sizeof(int32_t) * (kMaxPasses + 1), &pc));
     uint32_t* hist = (uint32_t*)ph;
+    uint32_t* state = (uint32_t*)pst;
+    int32_t* counters = (int32_t*)pc;
+    WF_TRY(wf_fill_u32(ctx, hist, 0u, 256 * (size_t)passes));
+    WF_TRY(wf_fill_u32(ctx, state, 0u, 256 * (size_t)tiles * passes));
+    WF_TRY(wf_fill_u32(ctx, counters, 0u, kMaxPasses));
+    {
+        Timed t(ctx, tag, 8.0, n);
+        const unsigned blocks = std::min<unsigned>(grid_for(n.n, kSortBlock), 1024u);
+        k_sweep_hist<<<blocks, kSortBlock, 0, ctx->stream>>>(keys_in, n.n, n.dev, lo_bit, hi_bit,
+                                                             passes, hist);
+        WF_HIP(hipGetLastError());
+        k_sweep_bases<<<passes, 256, 0, ctx->stream>>>(hist);
+        WF_HIP(hipGetLastError());
+    }
     // ping-pong so that the last pass writes keys_out/vals_out
     const uint64_t* ksrc = keys_in;
     const int32_t* vsrc = vals_in;
@@ -373,16 +464,10 @@ wf_status radix_sort_pairs(wf_context* ctx, const uint64_t* keys_in, const int32
         uint64_t* kdst = to_out ? keys_out : (uint64_t*)pk;
         int32_t* vdst = to_out ? vals_out : (int32_t*)pv;
         {
-            Timed t(ctx, tag, 8.0, n);
-            k_radix_hist<<<tiles, kSortBlock, 0, ctx->stream>>>(ksrc, n.n, n.dev, shift, mask,
-                                                                 hist, tiles);
-            WF_HIP(hipGetLastError());
-        }
-        WF_TRY(exclusive_scan_u32(ctx, hist, hist, host_count(256 * tiles), nullptr, tag));
-        {
             Timed t(ctx, tag, 24.0, n);
-            k_radix_scatter<<<tiles, kSortBlock, 0, ctx->stream>>>(
-                ksrc, vsrc, kdst, vdst, n.n, n.dev, shift, mask, nbits, hist, tiles);
+            k_sweep_pass<<<tiles, kSortBlock, 0, ctx->stream>>>(
+                ksrc, vsrc, kdst, vdst, n.n, n.dev, shift, mask, nbits, hist + 256 * p,
+                state + (size_t)256 * tiles * p, counters + p, counters + kMaxPasses);
             WF_HIP(hipGetLastError());
         }
         ksrc = kdst;
