@@ -298,6 +298,8 @@ struct wf_engine {
     hipEvent_t phase_ev[2][8] = {};    // per status buffer: marks between the phases
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    bool warm = false;                  // a direct iteration has sized every workspace since the
+                                        // last change of the iteration's shape
     int64_t iterations = 0;
     int32_t running = 0;
     double phase_ms[5] = {0, 0, 0, 0, 0};
@@ -456,6 +458,7 @@ void drop_graph(wf_engine* e) {
     if (e->graph) (void)hipGraphDestroy(e->graph);
     e->graph_exec = nullptr;
     e->graph = nullptr;
+    e->warm = false;
 }
 
 void copy_profile_flags(wf_engine* e) {
@@ -483,8 +486,8 @@ void merge_profile(wf_engine* e) {
 wf_status launch_iteration(wf_engine* e, int b) {
     g_phase_set = b;
     const bool graph_ok = e->cfg.use_graph && !e->ctx->profile && !e->phase_timing;
-    if (graph_ok && !e->graph_exec && e->iterations > 0) {
-        // the first iteration ran directly and sized every workspace: capture from here on
+    if (graph_ok && !e->graph_exec && e->warm) {
+        // a direct iteration of this shape sized every workspace: capture from here on
         WF_HIP(hipStreamBeginCapture(e->ctx->stream, hipStreamCaptureModeThreadLocal));
         wf_status st = enqueue_iteration(e);
         hipGraph_t g = nullptr;
@@ -501,6 +504,7 @@ wf_status launch_iteration(wf_engine* e, int b) {
         WF_HIP(hipGraphLaunch(e->graph_exec, e->ctx->stream));
     } else {
         WF_TRY(enqueue_iteration(e));
+        e->warm = true;
     }
     ++e->iterations;
     return WF_OK;
