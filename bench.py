@@ -179,6 +179,81 @@ def host_cpu() -> str:
     return "unknown"
 
 
+def run_batched(wf, ctx, scenes, args) -> dict:
+    """Config 3: 64 jobs from the Cornell-box and sorting-stress generators, sequential wf_render
+    against one job-tagged engine per scene (all jobs admitted at once)."""
+    from dataclasses import replace
+    forms = importlib.import_module("wavefront-renderer_amd.forms")
+    datas = [scenes.cornell_box(), scenes.sorting_stress()]
+    scs = [wf.Scene(ctx, d) for d in datas]
+    physics = [wf.PHYSICS_REFERENCE, wf.PHYSICS_UNBIASED]
+    jobs = forms.batched_jobs(args.jobs)
+    for k in range(2):   # size wf_render's cached engines before timing
+        mine = [j for j in jobs if j.scene == k]
+        if mine:
+            forms.sequential(ctx, scs, datas, physics, [replace(max(mine, key=lambda j: j.pixels), spp=1)])
+    secs, rays_seq = forms.sequential(ctx, scs, datas, physics, jobs)
+    t_seq = sum(secs)
+    log(f"[bench] batched: sequential {t_seq:.2f}s for {len(jobs)} jobs")
+    pix = [max(1, sum(j.pixels for j in jobs if j.scene == k)) for k in range(2)]
+    engines = [wf.Engine(ctx, scs[k], capacity=min(pix[k], 4 << 20), pixel_budget=pix[k],
+                         max_jobs=max(1, len(jobs)), physics=physics[k], firefly=1e30,
+                         use_graph=True) for k in range(2)]
+    res = forms.serve(engines, datas, jobs, realtime=False)
+    done = [j.done for j in jobs]
+    for e in engines:
+        e.close()
+    out = {"jobs": len(jobs), "scenes": ["cornell_box", "sorting_stress"],
+           "pixels": int(sum(j.pixels for j in jobs)),
+           "samples": int(sum(j.pixels * j.spp for j in jobs)),
+           "sequential_s": round(t_seq, 3), "batched_s": round(res["makespan_s"], 3),
+           "sequential_jobs_s": round(len(jobs) / t_seq, 2),
+           "batched_jobs_s": round(len(jobs) / res["makespan_s"], 2),
+           "speedup": round(t_seq / res["makespan_s"], 2),
+           "sequential_mrays_s": round(rays_seq / t_seq / 1e6, 1),
+           "batched_mrays_s": round(res["rays"] / res["makespan_s"] / 1e6, 1),
+           "batched_p50_s": round(forms.pct(done, 50), 3),
+           "batched_p99_s": round(forms.pct(done, 99), 3)}
+    log(f"[bench] batched: {json.dumps(out)}")
+    return out
+
+
+def run_served(wf, ctx, scenes, args) -> dict:
+    """Config 4: Poisson requests against three resident scenes at ~0.7 of the measured
+    capacity, adaptive sampling to a relMSE target; p50/p99 time-to-converged-image."""
+    forms = importlib.import_module("wavefront-renderer_amd.forms")
+    datas = [scenes.cornell_box(), scenes.sorting_stress(), scenes.sphere_cloud()]
+    scs = [wf.Scene(ctx, d) for d in datas]
+    physics = [wf.PHYSICS_REFERENCE, wf.PHYSICS_UNBIASED, wf.PHYSICS_UNBIASED]
+    engines = forms.make_engines(ctx, scs, physics, capacity=2 << 20, pixel_budget=8 << 20,
+                                 max_jobs=64)
+    # capacity: a burst of requests admitted at once
+    calib = forms.served_jobs(args.jobs // 3 or 1, rate=1.0, seed=0x5EED0104,
+                              max_spp=args.served_spp)
+    res = forms.serve(engines, datas, calib, realtime=False)
+    capacity = len(calib) / res["makespan_s"]
+    rate = 0.7 * capacity
+    jobs = forms.served_jobs(args.jobs, rate=rate, max_spp=args.served_spp)
+    res = forms.serve(engines, datas, jobs, realtime=True)
+    lat = [j.done - j.arrival for j in jobs]
+    for e in engines:
+        e.close()
+    secs, rays_seq = forms.sequential(ctx, scs, datas, physics, jobs)
+    seq_lat = forms.fcfs_latency([j.arrival for j in jobs], secs)
+    out = {"requests": len(jobs), "scenes": ["cornell_box", "sorting_stress", "sphere_cloud"],
+           "rate_jobs_s": round(rate, 2), "capacity_jobs_s": round(capacity, 2),
+           "rel_mse_target": 1e-3, "max_spp": args.served_spp,
+           "served_jobs_s": round(len(jobs) / res["makespan_s"], 2),
+           "served_mrays_s": round(res["rays"] / res["makespan_s"] / 1e6, 1),
+           "p50_s": round(forms.pct(lat, 50), 3), "p99_s": round(forms.pct(lat, 99), 3),
+           "sequential_capacity_jobs_s": round(len(jobs) / sum(secs), 2),
+           "sequential_p50_s": round(forms.pct(seq_lat, 50), 3),
+           "sequential_p99_s": round(forms.pct(seq_lat, 99), 3),
+           "sequential_mrays_s": round(rays_seq / sum(secs) / 1e6, 1)}
+    log(f"[bench] served: {json.dumps(out)}")
+    return out
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -198,6 +273,9 @@ def main() -> None:
     ap.add_argument("--cpu-spp", type=int, default=16,
                     help="samples per pixel of the CPU baseline sample (same image size)")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--jobs", type=int, default=64, help="jobs of the batched/served workloads")
+    ap.add_argument("--served-spp", type=int, default=256,
+                    help="per-pixel sample budget of served requests")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -213,6 +291,18 @@ def main() -> None:
         gbs = stream_copy_gbs(wf, ctx, copy_bytes, max(args.steps, 1))
         print(json.dumps({"stream_copy_bytes": copy_bytes, "launches": max(args.steps, 1),
                           "GB_s": round(gbs, 1)}))
+        return
+    if args.workload in ("batched", "served"):
+        t0 = time.perf_counter()
+        out = run_batched(wf, ctx, scenes, args) if args.workload == "batched" else \
+            run_served(wf, ctx, scenes, args)
+        key = "batched_jobs_s" if args.workload == "batched" else "served_jobs_s"
+        print(json.dumps({"metric": "jobs/s", "value": out[key], "unit": "jobs/s", "n_gpus": 1,
+                          "steps": 1, "warmup": 0,
+                          "ms_per_step": round((time.perf_counter() - t0) * 1e3, 1),
+                          "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                          "dtype": "f32", "data": "synthetic (seeded generator)",
+                          "config": {"workload": args.workload, **out}}))
         return
     t0 = time.time()
     data, W, H, spp, depth, desc = workload(args.workload, wf, scenes, args.quick)

@@ -423,6 +423,13 @@ wf_status wf_engine_admit(wf_engine* engine, const wf_job_desc* job, int32_t* jo
  * nothing is left to do; the finished job ids (up to finished_cap) go to `finished`. */
 wf_status wf_engine_run(wf_engine* engine, int32_t max_iterations, int32_t* finished,
                         int32_t finished_cap, int32_t* n_finished);
+/* Non-blocking form for driving several engines (one per resident scene, each on its own stream)
+ * from one host thread: launch enqueues `iterations` iterations and returns at once; wait blocks
+ * until they ran and reports the jobs finished by then. wf_engine_run may not be called between
+ * them. */
+wf_status wf_engine_launch(wf_engine* engine, int32_t iterations);
+wf_status wf_engine_wait(wf_engine* engine, int32_t* finished, int32_t finished_cap,
+                         int32_t* n_finished);
 /* Resolves a finished job into caller-owned device buffers (image_rgb: width*height*3 floats;
  * sample_counts optional) and frees its slot and framebuffer range. */
 wf_status wf_engine_result(wf_engine* engine, int32_t job_id, float* image_rgb,

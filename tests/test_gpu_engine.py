@@ -169,3 +169,28 @@ def test_engine_admission(wf, ctx, cornell):
     img, cnt, res = eng.result(z)
     assert not img.any() and not cnt.any() and res.samples_run == 0
     eng.close()
+
+
+def test_two_engines_overlapped_equal_alone(wf, ctx, cornell, stress, rng):
+    """Served form: one engine per resident scene, each on its own stream, driven together with
+    launch/wait (their iterations overlap on the device); every job equals its lone render."""
+    scenes_ = {"cornell": cornell, "stress": stress}
+    engines = {k: wf.Engine(ctx, sc, capacity=3000, pixel_budget=12000, max_jobs=4)
+               for k, (_, sc) in scenes_.items()}
+    jobs = {k: jobs_for(wf, scenes_[k][0], rng, 3) for k in scenes_}
+    ids = {k: {engines[k].admit(wf.job_desc(**j)): n for n, j in enumerate(jobs[k])} for k in jobs}
+    got = {k: {} for k in jobs}
+    while any(len(got[k]) < len(jobs[k]) for k in jobs):
+        for k, e in engines.items():
+            e.launch(2)
+        for k, e in engines.items():
+            for jid in e.wait():
+                got[k][ids[k][jid]] = e.result(jid)
+    for k in jobs:
+        _, scene = scenes_[k]
+        for n, j in enumerate(jobs[k]):
+            img, cnt, _ = alone(wf, scene, j)
+            assert np.array_equal(got[k][n][1], cnt), (k, n)
+            assert np.array_equal(got[k][n][0], img), (k, n)
+    for e in engines.values():
+        e.close()

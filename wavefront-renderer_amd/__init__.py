@@ -432,6 +432,15 @@ class Engine:
         check(self._lib.wf_engine_run(self.handle, max_iterations, buf, cap, C.byref(n)))
         return list(buf[:n.value])
 
+    def launch(self, iterations: int = 1):
+        check(self._lib.wf_engine_launch(self.handle, iterations))
+
+    def wait(self, cap: int = 256) -> list:
+        buf = (C.c_int32 * cap)()
+        n = C.c_int32()
+        check(self._lib.wf_engine_wait(self.handle, buf, cap, C.byref(n)))
+        return list(buf[:n.value])
+
     def result(self, job_id: int, keep_on_device: bool = False):
         w, h = self.jobs.pop(job_id)
         img = DeviceBuffer(self.ctx, w * h * 3, np.float32)

@@ -209,6 +209,8 @@ _PROTOS = {
     "wf_engine_admit": (_S, [P, C.POINTER(JobDesc), C.POINTER(C.c_int32)]),
     "wf_engine_run": (_S, [P, C.c_int32, C.POINTER(C.c_int32), C.c_int32,
                            C.POINTER(C.c_int32)]),
+    "wf_engine_launch": (_S, [P, C.c_int32]),
+    "wf_engine_wait": (_S, [P, C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_int32)]),
     "wf_engine_result": (_S, [P, C.c_int32, P, P, C.POINTER(JobResult)]),
     "wf_engine_query": (_S, [P, C.POINTER(EngineStats)]),
     "wf_engine_phase_timing": (_S, [P, C.c_int32]),

@@ -300,6 +300,10 @@ struct wf_engine {
     hipGraphExec_t graph_exec = nullptr;
     bool warm = false;                  // a direct iteration has sized every workspace since the
                                         // last change of the iteration's shape
+    bool launched = false;              // wf_engine_launch issued, wf_engine_wait pending
+    bool launch_empty = false;          // ... with nothing enqueued
+    int launch_buf = 0;
+    int next_buf = 0;
     int64_t iterations = 0;
     int32_t running = 0;
     double phase_ms[5] = {0, 0, 0, 0, 0};
@@ -695,26 +699,86 @@ wf_status wf_engine_admit(wf_engine* e, const wf_job_desc* job, int32_t* job_id)
     return WF_OK;
 }
 
+namespace {
+
+// Work the caller queued on the scene's context (uploads, cost tables) comes first.
+wf_status wait_for_scene(wf_engine* e) {
+    hipEvent_t dep;
+    WF_HIP(hipEventCreateWithFlags(&dep, hipEventDisableTiming));
+    WF_HIP(hipEventRecord(dep, e->scene->ctx->stream));
+    WF_HIP(hipStreamWaitEvent(e->ctx->stream, dep, 0));
+    (void)hipEventDestroy(dep);
+    return WF_OK;
+}
+
+// Jobs finished without running (empty budget) or not reported yet.
+void report_unreported(wf_engine* e, int32_t* finished, int32_t cap, int32_t* n) {
+    for (int j = 0; j < e->max_jobs && *n < cap; ++j)
+        if (e->job_state[j] == 2 && e->job_iter_start[j] >= 0) {
+            finished[(*n)++] = j;
+            e->job_iter_start[j] = -1;
+        }
+}
+
+// Marks the jobs whose pixels are all done in status buffer b as finished.
+void scan_status(wf_engine* e, int b, int32_t* finished, int32_t cap, int32_t* n) {
+    const int32_t* st = e->status_host + (size_t)b * (kStatusHeader + e->max_jobs);
+    for (int j = 0; j < e->max_jobs; ++j)
+        if (e->job_state[j] == 1 && st[kStatusHeader + j] == 0) {
+            e->job_state[j] = 2;
+            --e->running;
+            if (*n < cap) {
+                finished[(*n)++] = j;
+                e->job_iter_start[j] = -1;
+            }
+        }
+}
+
+}  // namespace
+
+wf_status wf_engine_launch(wf_engine* e, int32_t iterations) {
+    clear_error();
+    WF_REQUIRE(e, "wf_engine_launch: NULL engine");
+    WF_REQUIRE(!e->launched, "wf_engine_launch: the previous launch was not waited for");
+    WF_TRY(wait_for_scene(e));
+    copy_profile_flags(e);
+    e->launched = true;
+    e->launch_empty = e->running == 0;
+    if (e->launch_empty) return WF_OK;
+    const int b = e->next_buf;
+    for (int k = 0; k < (iterations > 1 ? iterations : 1); ++k) WF_TRY(launch_iteration(e, b));
+    WF_TRY(enqueue_status(e, b));
+    e->launch_buf = b;
+    e->next_buf = b ^ 1;
+    return WF_OK;
+}
+
+wf_status wf_engine_wait(wf_engine* e, int32_t* finished, int32_t finished_cap,
+                         int32_t* n_finished) {
+    clear_error();
+    WF_REQUIRE(e && n_finished && (finished || finished_cap == 0), "wf_engine_wait: NULL argument");
+    WF_REQUIRE(e->launched, "wf_engine_wait: nothing launched");
+    *n_finished = 0;
+    report_unreported(e, finished, finished_cap, n_finished);
+    if (!e->launch_empty) {
+        WF_HIP(hipEventSynchronize(e->status_ev[e->launch_buf]));
+        scan_status(e, e->launch_buf, finished, finished_cap, n_finished);
+    }
+    e->launched = false;
+    merge_profile(e);
+    return WF_OK;
+}
+
 wf_status wf_engine_run(wf_engine* e, int32_t max_iterations, int32_t* finished,
                         int32_t finished_cap, int32_t* n_finished) {
     clear_error();
     WF_REQUIRE(e && n_finished && (finished || finished_cap == 0), "wf_engine_run: NULL argument");
+    WF_REQUIRE(!e->launched, "wf_engine_run: a launch is pending (call wf_engine_wait)");
     *n_finished = 0;
-    // work the caller queued on the scene's context (uploads, cost tables) comes first
-    {
-        hipEvent_t dep;
-        WF_HIP(hipEventCreateWithFlags(&dep, hipEventDisableTiming));
-        WF_HIP(hipEventRecord(dep, e->scene->ctx->stream));
-        WF_HIP(hipStreamWaitEvent(e->ctx->stream, dep, 0));
-        (void)hipEventDestroy(dep);
-    }
+    WF_TRY(wait_for_scene(e));
     copy_profile_flags(e);
     // already finished jobs (spp 0) are reported at once
-    for (int j = 0; j < e->max_jobs && *n_finished < finished_cap; ++j)
-        if (e->job_state[j] == 2 && e->job_iter_start[j] >= 0) {
-            finished[(*n_finished)++] = j;
-            e->job_iter_start[j] = -1;
-        }
+    report_unreported(e, finished, finished_cap, n_finished);
     if (*n_finished > 0 || e->running == 0) {
         merge_profile(e);
         return WF_OK;
@@ -733,16 +797,7 @@ wf_status wf_engine_run(wf_engine* e, int32_t max_iterations, int32_t* finished,
         WF_HIP(hipEventSynchronize(e->status_ev[b]));
         accumulate_phases(e, b);
         ++done_iters;
-        const int32_t* st = e->status_host + (size_t)b * (kStatusHeader + e->max_jobs);
-        for (int j = 0; j < e->max_jobs; ++j)
-            if (e->job_state[j] == 1 && st[kStatusHeader + j] == 0) {
-                e->job_state[j] = 2;
-                --e->running;
-                if (*n_finished < finished_cap) {
-                    finished[(*n_finished)++] = j;
-                    e->job_iter_start[j] = -1;
-                }
-            }
+        scan_status(e, b, finished, finished_cap, n_finished);
         if (!more || *n_finished > 0 || e->running == 0) {
             if (more) {
                 WF_HIP(hipEventSynchronize(e->status_ev[b ^ 1]));

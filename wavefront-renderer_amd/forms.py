@@ -1,0 +1,163 @@
+"""Batched and served forms over device-resident scenes (one wavefront engine per scene).
+
+Batched (config 3): independent render jobs drawn by seed from the Cornell-box and sorting-stress
+generators (resolution log-uniform 128^2..1024^2, spp in {4, 16, 64, 256}, depth 8..12), all
+admitted at once and merged into each scene's job-tagged wavefront; compared with the same jobs
+rendered one after another with wf_render.
+
+Served (config 4): a Poisson stream of requests against the three resident scenes (Cornell box,
+sorting stress, sphere cloud), resolution lognormal (median 512^2, clamped to 64^2..1920x1080),
+adaptive sampling towards a per-pixel relative standard error whose square is the target relMSE;
+requests are admitted as job slots and framebuffer ranges free up and their paths share the
+scene's wavefront while other requests run. Reports jobs/s, Mrays/s and p50/p99
+time-to-converged-image, next to the same requests served by sequential wf_render calls (their
+measured service times through a first-come-first-served queue with the same arrivals).
+
+Every image is identical to the job rendered alone (tests/test_gpu_engine.py).
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import PHYSICS_REFERENCE, PHYSICS_UNBIASED, Engine, job_desc, render, render_params
+
+
+@dataclass
+class Job:
+    scene: int
+    width: int
+    height: int
+    spp: int
+    max_depth: int
+    seed: int
+    tag: int
+    adaptive: bool = False
+    threshold: float = 0.05
+    min_samples: int = 16
+    interval: int = 16
+    arrival: float = 0.0
+    done: float = field(default=-1.0)
+
+    @property
+    def pixels(self) -> int:
+        return self.width * self.height
+
+
+def batched_jobs(n: int = 64, seed: int = 0x5EED0003) -> list:
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i in range(n):
+        side = int(round(128.0 * 8.0 ** rng.random()))
+        jobs.append(Job(scene=int(rng.integers(0, 2)), width=side, height=side,
+                        spp=int(rng.choice([4, 16, 64, 256])), max_depth=int(rng.integers(8, 13)),
+                        seed=1000 + i, tag=i))
+    return jobs
+
+
+def served_jobs(n: int, rate: float, n_scenes: int = 3, seed: int = 0x5EED0004,
+                rel_mse: float = 1e-3, max_spp: int = 256) -> list:
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    thr = float(np.sqrt(rel_mse))
+    jobs = []
+    for i in range(n):
+        t += rng.exponential(1.0 / rate)
+        side = 512.0 * np.exp(0.5 * rng.normal())
+        jobs.append(Job(scene=int(rng.integers(0, n_scenes)),
+                        width=int(min(1920, max(64, round(side)))),
+                        height=int(min(1080, max(64, round(side)))), spp=max_spp, max_depth=12,
+                        seed=2000 + i, tag=i, adaptive=True, threshold=thr, min_samples=16,
+                        interval=16, arrival=t))
+    return jobs
+
+
+def _desc(job: Job, camera):
+    return job_desc(job.width, job.height, job.spp, job.max_depth, camera, seed=job.seed,
+                    job_tag=job.tag, adaptive=job.adaptive, adaptive_threshold=job.threshold,
+                    adaptive_min_samples=job.min_samples, adaptive_interval=job.interval)
+
+
+def _params(job: Job, camera, physics):
+    return render_params(job.width, job.height, job.spp, job.max_depth, camera, seed=job.seed,
+                         job=job.tag, adaptive=job.adaptive, adaptive_threshold=job.threshold,
+                         adaptive_min_samples=job.min_samples, adaptive_interval=job.interval,
+                         physics=physics, firefly=1e30)
+
+
+def sequential(ctx, scenes, datas, physics, jobs) -> tuple:
+    """wf_render of every job in turn: (per-job seconds, rays)."""
+    secs, rays = [], 0
+    for j in jobs:
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        img, cnt, st = render(scenes[j.scene], _params(j, datas[j.scene].camera, physics[j.scene]),
+                              keep_on_device=True)
+        ctx.synchronize()
+        secs.append(time.perf_counter() - t0)
+        rays += st.rays_traced + st.shadow_rays
+    return secs, rays
+
+
+def make_engines(ctx, scenes, physics, capacity, pixel_budget, max_jobs=64):
+    return [Engine(ctx, sc, capacity=capacity, pixel_budget=pixel_budget, max_jobs=max_jobs,
+                   physics=ph, firefly=1e30, use_graph=True) for sc, ph in zip(scenes, physics)]
+
+
+def serve(engines, datas, jobs, iterations: int = 2, realtime: bool = True) -> dict:
+    """Admits jobs at their arrival times (all at once when realtime is False), runs every engine
+    with work, collects finished jobs. Returns makespan, rays and per-job completion times."""
+    order = sorted(range(len(jobs)), key=lambda k: jobs[k].arrival)
+    pending = []           # arrived, not admitted yet (FIFO)
+    slot = [dict() for _ in engines]
+    done = 0
+    nxt = 0
+    r0 = sum(e.stats().rays_traced + e.stats().shadow_rays for e in engines)
+    t0 = time.perf_counter()
+    while done < len(jobs):
+        now = time.perf_counter() - t0
+        while nxt < len(order) and (not realtime or jobs[order[nxt]].arrival <= now):
+            pending.append(order[nxt])
+            nxt += 1
+        still = []
+        for k in pending:
+            j = jobs[k]
+            jid = engines[j.scene].try_admit(_desc(j, datas[j.scene].camera))
+            if jid is None:
+                still.append(k)
+            else:
+                slot[j.scene][jid] = k
+        pending = still
+        active = [e for i, e in enumerate(engines) if slot[i]]
+        if not active:
+            if nxt < len(order):
+                time.sleep(max(0.0, jobs[order[nxt]].arrival - (time.perf_counter() - t0)))
+            continue
+        for e in active:
+            e.launch(iterations)
+        for i, e in enumerate(engines):
+            if not slot[i]:
+                continue
+            for jid in e.wait():
+                k = slot[i].pop(jid)
+                e.result(jid, keep_on_device=True)
+                jobs[k].done = time.perf_counter() - t0
+                done += 1
+    makespan = time.perf_counter() - t0
+    rays = sum(e.stats().rays_traced + e.stats().shadow_rays for e in engines) - r0
+    return {"makespan_s": makespan, "rays": rays}
+
+
+def fcfs_latency(arrivals, service) -> np.ndarray:
+    """Completion minus arrival for a single first-come-first-served server."""
+    t, out = 0.0, []
+    for a, s in sorted(zip(arrivals, service)):
+        t = max(t, a) + s
+        out.append(t - a)
+    return np.array(out)
+
+
+def pct(x, q) -> float:
+    return float(np.percentile(np.asarray(x), q)) if len(x) else 0.0
