@@ -169,6 +169,27 @@ def traversal_stats(wf, ctx, scene, data, W, H, rng) -> dict:
     return out
 
 
+# Library kernel names (per-launch HIP-event timing) -> device kernel symbols in rocprofv3 output
+KERNEL_SYMBOLS = {"traverse_closest": "k_trace_closest", "shadow": "k_shadow",
+                  "split": "k_split", "shade": "k_shade", "gather_rays": "k_gather_rays",
+                  "miss": "k_miss", "regen": "k_regen", "ray_keys": "k_ray_keys"}
+
+
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary of this workload (profiles/traffic_<workload>_1080p_64spp.json), or None."""
+    path = ROOT / "profiles" / f"traffic_{workload}_1080p_64spp.json"
+    sym = KERNEL_SYMBOLS.get(kernel)
+    if not sym or not path.exists():
+        return None
+    t = json.loads(path.read_text())
+    for name, v in t["kernels"].items():
+        if name.split("<")[0] == sym:
+            return {"bytes_per_launch": v["bytes_per_launch"], "source": str(path.relative_to(ROOT)),
+                    "fetch_scale": t["fetch_scale"], "write_scale": t["write_scale"]}
+    return None
+
+
 def host_cpu() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -276,6 +297,10 @@ def main() -> None:
     ap.add_argument("--jobs", type=int, default=64, help="jobs of the batched/served workloads")
     ap.add_argument("--served-spp", type=int, default=256,
                     help="per-pixel sample budget of served requests")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the batched (config 3) and served (config 4) legs")
+    ap.add_argument("--profile-run", action="store_true",
+                    help="only the timed frames (no comparison legs): for profiler runs")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -370,6 +395,10 @@ def main() -> None:
         if dom.startswith("traverse") or dom == "shadow":
             roofline["note"] = ("BVH traversal is latency/divergence-bound; achieved counts only "
                                 "its algorithmic ray in/hit out bytes")
+        tr = measured_traffic(args.workload, dom)
+        if tr:
+            roofline["traffic"] = tr["bytes_per_launch"]
+            roofline["traffic_source"] = tr["source"]
     ctx.profile(False)
     ctx.profile_filter(None)
     hbm = [k for k in kernels if k in HBM_KERNELS]
@@ -379,26 +408,28 @@ def main() -> None:
         agg = bytes_sum / (ms_sum * 1e-3) / 1e9 if ms_sum > 0 else 0.0
         roofline["hbm_kernels"] = {"kernels": hbm, "ms_per_frame": round(ms_sum, 2),
                                    "GB_s": round(agg, 1), "frac": round(agg / HBM_PEAK_GBS, 4)}
-    copy = round(stream_copy_gbs(wf, ctx, copy_bytes, 5), 1)
-    log(f"[bench] streaming copy {args.copy_mib} MiB: {copy} GB/s")
-    if roofline is not None:
-        roofline["stream_copy_GB_s"] = copy
+    if not args.profile_run:
+        copy = round(stream_copy_gbs(wf, ctx, copy_bytes, 5), 1)
+        log(f"[bench] streaming copy {args.copy_mib} MiB: {copy} GB/s")
+        if roofline is not None:
+            roofline["stream_copy_GB_s"] = copy
 
-    tstats = traversal_stats(wf, ctx, scene, data, W, H, np.random.default_rng(5))
-    log(f"[bench] traversal: {json.dumps(tstats)}")
-
-    # the other schedule of the same frame, for comparison (identical image)
-    other = "passes" if args.schedule == "regenerate" else "regenerate"
-    t1 = time.perf_counter()
-    st = frame(args.policy, seed=5, schedule=other)
-    ctx.synchronize()
-    dt = time.perf_counter() - t1
-    other_sched = {"schedule": other, "ms": round(dt * 1e3, 1),
-                   "mrays_s": round((st.rays_traced + st.shadow_rays) / dt / 1e6, 2)}
-    log(f"[bench] schedule {other}: {other_sched}")
+    tstats, other_sched = None, None
+    if not args.profile_run:
+        tstats = traversal_stats(wf, ctx, scene, data, W, H, np.random.default_rng(5))
+        log(f"[bench] traversal: {json.dumps(tstats)}")
+        # the other schedule of the same frame, for comparison (identical image)
+        other = "passes" if args.schedule == "regenerate" else "regenerate"
+        t1 = time.perf_counter()
+        st = frame(args.policy, seed=5, schedule=other)
+        ctx.synchronize()
+        dt = time.perf_counter() - t1
+        other_sched = {"schedule": other, "ms": round(dt * 1e3, 1),
+                       "mrays_s": round((st.rays_traced + st.shadow_rays) / dt / 1e6, 2)}
+        log(f"[bench] schedule {other}: {other_sched}")
 
     policies = {}
-    if not args.no_policies:
+    if not args.no_policies and not args.profile_run:
         # phase times need events between the phases: time no kernel, but keep timing on
         ctx.profile_filter("phases")
         ctx.profile(True)
@@ -417,7 +448,7 @@ def main() -> None:
         ctx.profile_filter(None)
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and not args.profile_run:
         sys.path.insert(0, str(ROOT / "tests"))
         import oracle_lib  # CPU baseline leg only
 
@@ -440,6 +471,12 @@ def main() -> None:
                          f"{os.cpu_count()} logical CPUs visible"}
         log(f"[bench] cpu baseline: {cpu}")
 
+    forms_out = None
+    if not args.no_forms and not args.profile_run and not args.quick:
+        forms_out = {"batched": run_batched(wf, ctx, scenes, args),
+                     "served": run_served(wf, ctx, scenes, argparse.Namespace(
+                         jobs=max(3, args.jobs * 9 // 16), served_spp=args.served_spp))}
+
     out = {
         "metric": "Mrays/s @1080p depth 12",
         "value": round(mrays, 2),
@@ -459,6 +496,7 @@ def main() -> None:
                    "bvh_build_ms": round(scene.stats.build_ms, 1), "policies": policies},
         "roofline": roofline,
         "traversal": tstats,
+        "forms": forms_out,
         "kernels": kernels,
         "cpu_baseline": cpu,
     }

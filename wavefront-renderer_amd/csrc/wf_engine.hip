@@ -73,12 +73,12 @@ struct EngineView {
     int32_t px;                // framebuffer pixels (ready-queue capacity)
     int32_t* counters;
     unsigned long long* stats; // S_HEAD / S_TAIL: ready-queue cursors (monotonic)
-    int32_t* ready;
+    int2* ready;               // (pixel, job slot << kTagJobShift | next sample index)
     float* accum;              // 3 per pixel
     float* before;             // 3 per pixel: accum at the start of the pixel's current sample
     float* mean;
     float* m2;
-    int32_t* pixel_tag;        // job slot << kTagJobShift | finished samples (= next sample)
+    int32_t* pixel_count;      // samples of a retired pixel (written when it retires)
     const JobDev* jobs;
     int32_t* status;           // kStatusHeader + job slots
     int32_t* job_left;         // = status + kStatusHeader: pixels of each job not finished yet
@@ -99,8 +99,10 @@ __global__ void k_admit(EngineView e, int job, int base, int n, int queue) {
     e.before[3 * (size_t)p + 2] = 0.0f;
     e.mean[p] = 0.0f;
     e.m2[p] = 0.0f;
-    e.pixel_tag[p] = job << kTagJobShift;
-    if (queue) e.ready[(e.stats[S_TAIL] + (unsigned long long)k) % (unsigned long long)e.px] = p;
+    e.pixel_count[p] = 0;
+    if (queue)
+        e.ready[(e.stats[S_TAIL] + (unsigned long long)k) % (unsigned long long)e.px] =
+            make_int2(p, job << kTagJobShift);
 }
 
 __global__ void k_admit_commit(EngineView e, int job, int n, int queue) {
@@ -123,8 +125,9 @@ __global__ void k_regen(EngineView e, wf_ray_soa r, int32_t* __restrict__ tags) 
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int take = regen_take(e);
     if (i >= take) return;
-    const int p = e.ready[(e.stats[S_HEAD] + (unsigned long long)i) % (unsigned long long)e.px];
-    const int32_t pt = e.pixel_tag[p];
+    const int2 entry = e.ready[(e.stats[S_HEAD] + (unsigned long long)i) % (unsigned long long)e.px];
+    const int p = entry.x;
+    const int32_t pt = entry.y;
     const int j = pt >> kTagJobShift;
     const JobDev& J = e.jobs[j];
     const int s = pt & kTagSampleMask;
@@ -170,11 +173,11 @@ __device__ __forceinline__ void finish_path(const EngineView& e, int p, int32_t 
     __shared__ uint32_t block_count;
     __shared__ unsigned long long block_base;
     bool requeue = false;
+    int32_t next = 0;
     if (valid) {
         const int j = tag >> kTagJobShift;
         const JobDev& J = e.jobs[j];
         const int cnt = (tag & kTagSampleMask) + 1;
-        e.pixel_tag[p] = (j << kTagJobShift) | cnt;
         bool converged = false;
         if (J.adaptive) {
             const size_t b = 3 * (size_t)p;
@@ -196,11 +199,13 @@ __device__ __forceinline__ void finish_path(const EngineView& e, int p, int32_t 
         }
         const bool budget = cnt >= J.spp;
         if (converged || budget) {
+            e.pixel_count[p] = cnt;
             atomicMax(&e.job_max_count[j], cnt);
             if (J.adaptive && !converged) atomicAdd(&e.job_unconverged[j], 1);
             atomicSub(&e.job_left[j], 1);
         } else {
             requeue = true;
+            next = (j << kTagJobShift) | cnt;
         }
     }
     if (threadIdx.x == 0) block_count = 0;
@@ -217,7 +222,7 @@ __device__ __forceinline__ void finish_path(const EngineView& e, int p, int32_t 
     if (requeue) {
         const unsigned long long pos =
             block_base + wave_off + (unsigned long long)__popcll(b & lanemask_lt());
-        e.ready[pos % (unsigned long long)e.px] = p;
+        e.ready[pos % (unsigned long long)e.px] = make_int2(p, next);
     }
 }
 
@@ -254,13 +259,13 @@ __global__ void k_iteration_end(EngineView e) {
     cur[1] = e.stats[S_TAIL];
 }
 
-__global__ void k_resolve_job(const float* __restrict__ accum, const int32_t* __restrict__ pixel_tag,
+__global__ void k_resolve_job(const float* __restrict__ accum, const int32_t* __restrict__ pixel_count,
                               int base, int n, float* __restrict__ image,
                               int32_t* __restrict__ counts_out) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const int p = base + k;
-    const int count = pixel_tag[p] & kTagSampleMask;
+    const int count = pixel_count[p];
     int c = count;
     c = c > 1 ? c : 1;
     image[3 * (size_t)k] = accum[3 * (size_t)p] / (float)c;
@@ -363,7 +368,7 @@ wf_status engine_array(wf_engine* e, size_t count, T** out) {
 
 // Bytes of device memory per ray slot and per framebuffer pixel (admission accounting).
 constexpr size_t kBytesPerSlot = 48 * 3 + 104 * 2 + 4 * 5 + 400;   // queues + tags + workspaces
-constexpr size_t kBytesPerPixel = 12 + 12 + 4 + 4 + 4 + 4 + 4;
+constexpr size_t kBytesPerPixel = 12 + 12 + 4 + 4 + 4 + 8;   // accum, start copy, mean, M2, count, queue
 
 int g_phase_set = 0;   // event set of the iteration being enqueued
 
@@ -576,7 +581,7 @@ wf_status wf_engine_create(wf_context* ctx, wf_scene* scene, const wf_engine_con
     ETRY(engine_array(e, 3 * px, &v.before));
     ETRY(engine_array(e, px, &v.mean));
     ETRY(engine_array(e, px, &v.m2));
-    ETRY(engine_array(e, px, &v.pixel_tag));
+    ETRY(engine_array(e, px, &v.pixel_count));
     ETRY(engine_array(e, (size_t)e->max_jobs, &e->jobs_dev));
     v.jobs = e->jobs_dev;
     ETRY(engine_array(e, (size_t)(kStatusHeader + e->max_jobs), &v.status));
@@ -586,7 +591,7 @@ wf_status wf_engine_create(wf_context* ctx, wf_scene* scene, const wf_engine_con
     ETRY(wf_fill_u32(e->ctx, v.counters, 0u, C_COUNT));
     ETRY(wf_fill_u32(e->ctx, v.stats, 0u, 2 * S_COUNT));
     ETRY(wf_fill_u32(e->ctx, v.status, 0u, (size_t)(kStatusHeader + e->max_jobs)));
-    ETRY(wf_fill_u32(e->ctx, v.pixel_tag, 0u, px));
+    ETRY(wf_fill_u32(e->ctx, v.pixel_count, 0u, px));
 #undef ETRY
     e->jobs.assign((size_t)e->max_jobs, JobDev{});
     e->job_state.assign((size_t)e->max_jobs, 0);
@@ -820,7 +825,7 @@ wf_status wf_engine_result(wf_engine* e, int32_t job_id, float* image_rgb, int32
     const JobDev& J = e->jobs[job_id];
     const int P = J.width * J.height;
     wf_context* ctx = e->ctx;
-    k_resolve_job<<<grid_for(P, 256), 256, 0, ctx->stream>>>(e->view.accum, e->view.pixel_tag,
+    k_resolve_job<<<grid_for(P, 256), 256, 0, ctx->stream>>>(e->view.accum, e->view.pixel_count,
                                                              J.pixel_base, P, image_rgb,
                                                              sample_counts);
     WF_HIP(hipGetLastError());
