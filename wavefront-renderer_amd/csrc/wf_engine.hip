@@ -172,8 +172,8 @@ constexpr int kFinishBlock = 1024;
 __device__ __forceinline__ void finish_path(const EngineView& e, int p, int32_t tag, bool valid) {
     __shared__ uint32_t block_count;
     __shared__ unsigned long long block_base;
-    bool requeue = false;
-    int32_t next = 0;
+    bool requeue = false, retire = false, unconverged = false;
+    int32_t next = 0, job = 0, samples = 0;
     if (valid) {
         const int j = tag >> kTagJobShift;
         const JobDev& J = e.jobs[j];
@@ -200,12 +200,35 @@ __device__ __forceinline__ void finish_path(const EngineView& e, int p, int32_t 
         const bool budget = cnt >= J.spp;
         if (converged || budget) {
             e.pixel_count[p] = cnt;
-            atomicMax(&e.job_max_count[j], cnt);
-            if (J.adaptive && !converged) atomicAdd(&e.job_unconverged[j], 1);
-            atomicSub(&e.job_left[j], 1);
+            retire = true;
+            unconverged = J.adaptive && !converged;
+            job = j;
+            samples = cnt;
         } else {
             requeue = true;
             next = (j << kTagJobShift) | cnt;
+        }
+    }
+    // retired pixels: one set of job-counter atomics per wave for the lanes sharing the first
+    // retiring lane's job, single atomics for the others
+    const uint64_t rmask = __ballot(retire);
+    if (rmask) {
+        const int leader = __ffsll((unsigned long long)rmask) - 1;
+        const int jl = __shfl(job, leader, 64);
+        const bool same = retire && job == jl;
+        const uint64_t smask = __ballot(same);
+        int cmax = same ? samples : 0;
+        for (int o = 32; o >= 1; o >>= 1) cmax = max(cmax, __shfl_xor(cmax, o, 64));
+        const int un = __popcll(__ballot(same && unconverged));
+        if ((int)__lane_id() == leader) {
+            atomicSub(&e.job_left[jl], __popcll(smask));
+            atomicMax(&e.job_max_count[jl], cmax);
+            if (un) atomicAdd(&e.job_unconverged[jl], un);
+        }
+        if (retire && !same) {
+            atomicSub(&e.job_left[job], 1);
+            atomicMax(&e.job_max_count[job], samples);
+            if (unconverged) atomicAdd(&e.job_unconverged[job], 1);
         }
     }
     if (threadIdx.x == 0) block_count = 0;
