@@ -285,6 +285,9 @@ def main() -> None:
     ap.add_argument("--schedule", default="regenerate", choices=["regenerate", "passes"],
                     help="regenerate: full wavefront, pixels start their next sample as soon as "
                          "their path ends; passes: the reference's pass-by-pass loop (same image)")
+    ap.add_argument("--lanes", type=int, default=8, choices=[1, 2, 4, 8, 16],
+                    help="regenerate: samples of a pixel in flight at once (1 = the pass loop's "
+                         "accumulation order bit for bit)")
     ap.add_argument("--quick", action="store_true", help="reduced sizes (CPU-side checks)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-policies", action="store_true")
@@ -336,13 +339,15 @@ def main() -> None:
     scene = wf.Scene(ctx, data)
     log(f"[bench] uploaded + BVH built in {time.time() - t0:.1f}s "
         f"(device build {scene.stats.build_ms:.1f} ms, binary depth {scene.stats.max_depth}, "
-        f"four-wide nodes {scene.stats.node_count}, stack bound {scene.stats.stack_bound})")
+        f"{scene.stats.bvh_width}-wide nodes {scene.stats.node_count}, stack bound "
+        f"{scene.stats.stack_bound})")
     physics = wf.PHYSICS_UNBIASED if args.workload == "sorting_stress" else wf.PHYSICS_REFERENCE
 
     def params(policy, seed=1, w=W, h=H, s=spp, schedule=args.schedule):
         return wf.render_params(width=w, height=h, spp=s, max_depth=depth,
                                 camera=data.camera, policy=policy, physics=physics,
-                                firefly=1e30, seed=seed, schedule=schedule)
+                                firefly=1e30, seed=seed, schedule=schedule,
+                                sample_lanes=args.lanes)
 
     def frame(policy, seed=1, schedule=args.schedule):
         img, cnt, st = wf.render(scene, params(policy, seed, schedule=schedule),
@@ -418,7 +423,8 @@ def main() -> None:
     if not args.profile_run:
         tstats = traversal_stats(wf, ctx, scene, data, W, H, np.random.default_rng(5))
         log(f"[bench] traversal: {json.dumps(tstats)}")
-        # the other schedule of the same frame, for comparison (identical image)
+        # the other schedule of the same frame, for comparison (same paths; the same image up to
+        # the order of the float additions, bit for bit with one sample lane)
         other = "passes" if args.schedule == "regenerate" else "regenerate"
         t1 = time.perf_counter()
         st = frame(args.policy, seed=5, schedule=other)
@@ -491,6 +497,7 @@ def main() -> None:
         "dtype": "f32",
         "data": "synthetic (seeded generator)",
         "config": {"workload": desc, "policy": args.policy, "schedule": args.schedule,
+                   "sample_lanes": args.lanes, "bvh_width": scene.stats.bvh_width,
                    "other_schedule": other_sched, "physics":
                    "unbiased" if physics else "reference", "triangles": data.triangle_count,
                    "bvh_build_ms": round(scene.stats.build_ms, 1), "policies": policies},

@@ -199,10 +199,11 @@ typedef struct wf_build_stats {   /* EmbreeAccelerator m_meshCount/m_totalTriang
     int32_t mesh_count;
     int32_t triangle_count;
     int32_t vertex_count;
-    int32_t node_count;           /* four-wide BVH nodes */
-    int32_t max_depth;            /* depth of the binary LBVH the four-wide BVH is collapsed from */
+    int32_t node_count;           /* wide BVH nodes (of bvh_width children each) */
+    int32_t max_depth;            /* depth of the binary hierarchy the wide BVH is collapsed from */
     int32_t stack_bound;          /* most entries the traversal stack can hold (<= 64 supported) */
-    int32_t reserved;
+    int32_t bvh_width;            /* 4 or 8: the node width the build used (a tuning of 8 falls
+                                     back to 4 for one triangle or a stack bound above 64) */
     double build_ms;              /* device time of the BVH build kernels */
     float bounds_min[3];          /* Scene::computeBounds over all vertices (Scene.cpp:59-77) */
     float bounds_max[3];
@@ -318,6 +319,8 @@ typedef struct wf_render_params {
     int32_t adaptive_min_samples;   /* reference: 32 */
     int32_t adaptive_interval;      /* reference: 16 */
     int32_t schedule;          /* WF_SCHEDULE_*: same image, different order of work */
+    int32_t sample_lanes;      /* REGENERATE, non-adaptive: samples of a pixel in flight at once
+                                  (1, 2, 4, 8 or 16; 0 = 8); see wf_engine_config::sample_lanes */
     wf_shade_params shade;
     wf_camera camera;
 } wf_render_params;
@@ -338,9 +341,11 @@ typedef struct wf_render_stats {  /* the reference's stats blocks (cpp:321-370) 
 } wf_render_stats;
 
 /* wf_render_params::schedule. REGENERATE keeps a full wavefront: a pixel starts its next sample
- * as soon as its previous path ends (one path per pixel in flight, so its accumulator receives
- * the same additions in the same order as the pass loop). PASSES is the reference's loop: all
- * pixels sample pass by pass, bounce by bounce (cpp:141-312). */
+ * as soon as a previous path of it ends. With sample_lanes = 1 a pixel has one path in flight,
+ * so its accumulator receives the same additions in the same order as the pass loop and the
+ * image is bit-identical to PASSES; with L lanes it has L, each summing its own samples, and the
+ * image differs from the pass loop only by the order of the float additions. PASSES is the
+ * reference's loop: all pixels sample pass by pass, bounce by bounce (cpp:141-312). */
 #define WF_SCHEDULE_REGENERATE 0
 #define WF_SCHEDULE_PASSES 1
 
@@ -369,7 +374,12 @@ typedef struct wf_engine_config {
     int32_t policy;            /* WF_POLICY_* shading-queue order */
     int32_t ray_sort;          /* RayQueue Morton sort of every iteration's rays */
     int32_t use_graph;         /* replay each iteration from a captured HIP graph */
-    int32_t reserved;
+    int32_t sample_lanes;      /* samples of a pixel in flight at once (1, 2, 4, 8, 16; 0 = 8).
+                                  Lane l of a non-adaptive job runs samples l, l+L, l+2L, ... into
+                                  an accumulator of its own; the lanes are summed in lane order
+                                  when the job resolves, so an image depends on L but not on the
+                                  schedule. A job uses the largest power of two <= min(L, spp)
+                                  lanes; adaptive jobs use one (the estimate is sequential). */
     wf_shade_params shade;     /* physics, RR and firefly settings (seed/sample/job unused) */
 } wf_engine_config;
 
@@ -409,8 +419,10 @@ typedef struct wf_engine_stats {
     double phase_ms[5];        /* sort, intersect, schedule, shade, miss (with phase timing) */
 } wf_engine_stats;
 
-/* Device bytes an engine of this capacity and pixel budget needs (admission planning). */
-wf_status wf_engine_bytes(int32_t capacity, int64_t pixel_budget, size_t* out);
+/* Device bytes an engine of this capacity, pixel budget and sample lanes (0 = 8) needs
+ * (admission planning). */
+wf_status wf_engine_bytes(int32_t capacity, int64_t pixel_budget, int32_t sample_lanes,
+                          size_t* out);
 /* The engine runs on a stream of its own on ctx's device; it waits for work queued on the
  * scene's context before each run. */
 wf_status wf_engine_create(wf_context* ctx, wf_scene* scene, const wf_engine_config* config,
@@ -447,7 +459,10 @@ typedef struct wf_tuning {
     int32_t trace_blocks_per_cu; /* persistent traversal grid: 128-lane blocks per CU (1..64) */
     int32_t bvh_builder;         /* WF_BVH_* hierarchy of the next wf_accel_build */
     int32_t ploc_radius;         /* PLOC neighbour search radius (1..32) */
-    int32_t reserved[4];
+    int32_t bvh_width;           /* node width of the next wf_accel_build: 4 (default), float
+                                  * boxes; 8, boxes quantised outward to 8 bits per node grid
+                                  * (fewer, smaller nodes; more box tests per node) */
+    int32_t reserved[3];
 } wf_tuning;
 /* Engines created on the context take its tuning at every run. */
 wf_status wf_context_set_tuning(wf_context* ctx, const wf_tuning* tuning);

@@ -82,9 +82,32 @@ def test_build_stats(cornell):
     bmin, bmax, area, ntri = osc.info()
     assert list(st.bounds_min) == bmin.tolist() and list(st.bounds_max) == bmax.tolist()
     assert 0 < st.max_depth
-    # four-wide collapse: at most 3 pushes per wide node on a root-to-leaf path
-    assert 0 < st.stack_bound <= min(64, 3 * ((st.max_depth + 1) // 2))
+    assert st.bvh_width in (4, 8)
+    assert 0 < st.stack_bound <= 64
     assert st.node_count < st.triangle_count
+
+
+@pytest.mark.parametrize("width", [4, 8])
+def test_build_stats_width(wf, ctx, cornell, width):
+    data = cornell[0]
+    sc = build_with(wf, ctx, data, bvh_width=width)
+    st = sc.stats
+    assert st.bvh_width == width
+    # a wide node has at most width-1 pushed siblings and covers at least one binary level
+    assert 0 < st.stack_bound <= min(64, (width - 1) * st.max_depth)
+    # every wide node but the root is the child of another and holds >= 2 children
+    assert st.node_count <= st.triangle_count - 1
+    if width == 4:
+        assert 0 < st.stack_bound <= 3 * ((st.max_depth + 1) // 2)
+
+
+def build_with(wf, ctx, data, **tuning):
+    saved = ctx.tuning()
+    ctx.set_tuning(**tuning)
+    try:
+        return wf.Scene(ctx, data)
+    finally:
+        ctx.set_tuning(**saved)
 
 
 def chain_scene(scenes):
@@ -106,17 +129,14 @@ def chain_scene(scenes):
     return scenes.SceneData([scenes.Mesh(pos, idx, 0)], [{"type": 0}]), c
 
 
+@pytest.mark.parametrize("width", [4, 8])
 @pytest.mark.parametrize("builder", ["lbvh", "ploc"])
-def test_deep_tree_closest_hit_matches_bruteforce(wf, ctx, scenes, orc, rng, builder):
+def test_deep_tree_closest_hit_matches_bruteforce(wf, ctx, scenes, orc, rng, builder, width):
     data, centres = chain_scene(scenes)
-    saved = ctx.tuning()
-    ctx.set_tuning(bvh_builder=wf._abi.BVH_LBVH if builder == "lbvh" else wf._abi.BVH_PLOC)
-    try:
-        sc = wf.Scene(ctx, data)
-    finally:
-        ctx.set_tuning(**saved)
+    sc = build_with(wf, ctx, data, bvh_width=width,
+                    bvh_builder=wf._abi.BVH_LBVH if builder == "lbvh" else wf._abi.BVH_PLOC)
     print("chain scene:", builder, "binary depth", sc.stats.max_depth, "stack bound",
-          sc.stats.stack_bound)
+          sc.stats.stack_bound, "width", sc.stats.bvh_width)
     if builder == "lbvh":
         assert sc.stats.max_depth > 32     # the Karras tree follows the code bits one by one
     assert sc.stats.stack_bound <= 64
@@ -137,17 +157,14 @@ def test_deep_tree_closest_hit_matches_bruteforce(wf, ctx, scenes, orc, rng, bui
     assert np.array_equal(hits["t"][h], ref["t"][h])
 
 
-@pytest.mark.parametrize("builder", ["lbvh", "ploc"])
-def test_large_scene_closest_hit_matches_oracle(wf, ctx, scenes, orc, rng, builder):
+@pytest.mark.parametrize("builder,width", [("lbvh", 4), ("ploc", 4), ("ploc", 8)])
+def test_large_scene_closest_hit_matches_oracle(wf, ctx, scenes, orc, rng, builder, width):
     """~330 k triangles: the oracle's SAH BVH (itself pinned against brute force) against the
-    device BVH of either builder, over camera rays and random rays."""
+    device BVH of either builder and node width, over camera rays and random rays."""
     data = scenes.sorting_stress(subdiv=6, tex_size=8)
-    saved = ctx.tuning()
-    ctx.set_tuning(bvh_builder=wf._abi.BVH_LBVH if builder == "lbvh" else wf._abi.BVH_PLOC)
-    try:
-        sc = wf.Scene(ctx, data)
-    finally:
-        ctx.set_tuning(**saved)
+    sc = build_with(wf, ctx, data, bvh_width=width,
+                    bvh_builder=wf._abi.BVH_LBVH if builder == "lbvh" else wf._abi.BVH_PLOC)
+    assert sc.stats.bvh_width == width
     osc = orc.scene(data)
     rays = camera_rays(orc, data, 128, 96)
     extra = random_rays(rng, 20000)
@@ -162,8 +179,10 @@ def test_large_scene_closest_hit_matches_oracle(wf, ctx, scenes, orc, rng, build
     assert np.array_equal(hits["t"][h], ref["t"][h])
 
 
-def test_closest_hit_matches_bruteforce(wf, ctx, cornell, orc, rng):
-    data, scene, osc = cornell
+@pytest.mark.parametrize("width", [4, 8])
+def test_closest_hit_matches_bruteforce(wf, ctx, cornell, orc, rng, width):
+    data, _, osc = cornell
+    scene = build_with(wf, ctx, data, bvh_width=width)
     rays = camera_rays(orc, data, 96, 96)
     extra = random_rays(rng, 8000)
     for k in ("ox", "oy", "oz", "dx", "dy", "dz"):
@@ -211,8 +230,10 @@ def test_empty_scene_misses(wf, ctx, scenes, rng):
     assert np.all(h["geom"] == -1) and np.all(h["prim"] == -1)
 
 
-def test_occluded_matches_bruteforce(wf, ctx, cornell, rng):
-    data, scene, osc = cornell
+@pytest.mark.parametrize("width", [4, 8])
+def test_occluded_matches_bruteforce(wf, ctx, cornell, rng, width):
+    data, _, osc = cornell
+    scene = build_with(wf, ctx, data, bvh_width=width)
     n = 20000
     o = (rng.uniform(-0.45, 0.45, (n, 3)) + [0, 0.5, 0]).astype(f32)
     tgt = (rng.uniform(-0.45, 0.45, (n, 3)) + [0, 0.5, 0]).astype(f32)
@@ -398,6 +419,24 @@ def test_render_independent_of_policy_and_ray_sort(wf, cornell, policy):
     img1, _, st1 = wf.render(scene, p)
     assert np.array_equal(img0, img1)
     assert st0.rays_traced == st1.rays_traced
+
+
+def test_render_independent_of_bvh_width(wf, ctx, cornell):
+    """The closest hit is the (t, triangle) minimum over every accepted triangle and the
+    quantised eight-wide boxes contain the four-wide ones: both node widths render the same
+    image, ray for ray."""
+    data = cornell[0]
+    p = wf.render_params(camera=data.camera, width=64, height=48, spp=4, max_depth=8, seed=9,
+                         firefly=1e30)
+    out = []
+    for width in (4, 8):
+        sc = build_with(wf, ctx, data, bvh_width=width)
+        assert sc.stats.bvh_width == width
+        img, cnt, st = wf.render(sc, p)
+        out.append((img, st.rays_traced, st.shadow_rays))
+        sc.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[0][1:] == out[1][1:]
 
 
 def test_render_adaptive_matches_oracle(wf, cornell):

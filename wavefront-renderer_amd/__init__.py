@@ -81,13 +81,14 @@ class Context:
         t = _abi.Tuning()
         check(self._lib.wf_context_get_tuning(self.handle, C.byref(t)))
         return {"refill_lanes": t.refill_lanes, "trace_blocks_per_cu": t.trace_blocks_per_cu,
-                "bvh_builder": t.bvh_builder, "ploc_radius": t.ploc_radius}
+                "bvh_builder": t.bvh_builder, "ploc_radius": t.ploc_radius,
+                "bvh_width": t.bvh_width}
 
     def set_tuning(self, **kw):
         cur = self.tuning()
         cur.update(kw)
         t = _abi.Tuning(cur["refill_lanes"], cur["trace_blocks_per_cu"], cur["bvh_builder"],
-                        cur["ploc_radius"])
+                        cur["ploc_radius"], cur["bvh_width"])
         check(self._lib.wf_context_set_tuning(self.handle, C.byref(t)))
 
     def stream_copy(self, dst: int, src: int, nbytes: int):
@@ -357,13 +358,15 @@ def render_params(width, height, spp, max_depth, camera, policy="none", ray_sort
                   adaptive=False, adaptive_threshold=0.05, adaptive_min_samples=32,
                   adaptive_interval=16, physics=PHYSICS_REFERENCE, rr_min_depth=3,
                   cost_aware_rr=False, firefly=10.0, seed=0, job=0,
-                  schedule="regenerate") -> _abi.RenderParams:
+                  schedule="regenerate", sample_lanes=0) -> _abi.RenderParams:
+    """sample_lanes (regenerating schedule): samples of a pixel in flight at once, 1/2/4/8/16 or
+    0 for the default (8); 1 reproduces the pass loop's accumulation order bit for bit."""
     pol = POLICIES[policy] if isinstance(policy, str) else int(policy)
     sched = SCHEDULES[schedule] if isinstance(schedule, str) else int(schedule)
     sp = shade_params(physics, rr_min_depth, cost_aware_rr, firefly, seed, 0, job)
     return _abi.RenderParams(width, height, spp, max_depth, pol, 1 if ray_sort else 0,
                              1 if adaptive else 0, adaptive_threshold, adaptive_min_samples,
-                             adaptive_interval, sched, sp, camera_struct(camera))
+                             adaptive_interval, sched, sample_lanes, sp, camera_struct(camera))
 
 
 def render(scene: Scene, params: _abi.RenderParams, keep_on_device: bool = False):
@@ -396,13 +399,13 @@ class Engine:
     def __init__(self, ctx: Context, scene: Scene, capacity: int, pixel_budget: int,
                  max_jobs: int = 64, policy="none", ray_sort=True, use_graph=False,
                  physics=PHYSICS_REFERENCE, rr_min_depth=3, cost_aware_rr=False,
-                 firefly=10.0):
+                 firefly=10.0, sample_lanes=0):
         self.ctx = ctx
         self.scene = scene
         self._lib = ctx._lib
         pol = POLICIES[policy] if isinstance(policy, str) else int(policy)
         cfg = _abi.EngineConfig(capacity, max_jobs, pixel_budget, pol, 1 if ray_sort else 0,
-                                1 if use_graph else 0, 0,
+                                1 if use_graph else 0, sample_lanes,
                                 shade_params(physics, rr_min_depth, cost_aware_rr, firefly,
                                              0, 0, 0))
         h = C.c_void_p()
@@ -477,9 +480,10 @@ class Engine:
             pass
 
 
-def engine_bytes(capacity: int, pixel_budget: int) -> int:
+def engine_bytes(capacity: int, pixel_budget: int, sample_lanes: int = 0) -> int:
     out = C.c_size_t()
-    check(_abi.load_library().wf_engine_bytes(capacity, pixel_budget, C.byref(out)))
+    check(_abi.load_library().wf_engine_bytes(capacity, pixel_budget, sample_lanes,
+                                              C.byref(out)))
     return out.value
 
 

@@ -96,7 +96,7 @@ class BuildStats(C.Structure):
     _fields_ = [("mesh_count", C.c_int32), ("triangle_count", C.c_int32),
                 ("vertex_count", C.c_int32), ("node_count", C.c_int32),
                 ("max_depth", C.c_int32), ("stack_bound", C.c_int32),
-                ("reserved", C.c_int32), ("build_ms", C.c_double),
+                ("bvh_width", C.c_int32), ("build_ms", C.c_double),
                 ("bounds_min", f3), ("bounds_max", f3)]
 
 
@@ -115,7 +115,8 @@ class RenderParams(C.Structure):
                 ("max_depth", C.c_int32), ("policy", C.c_int32), ("ray_sort", C.c_int32),
                 ("adaptive", C.c_int32), ("adaptive_threshold", C.c_float),
                 ("adaptive_min_samples", C.c_int32), ("adaptive_interval", C.c_int32),
-                ("schedule", C.c_int32), ("shade", ShadeParams), ("camera", Camera)]
+                ("schedule", C.c_int32), ("sample_lanes", C.c_int32), ("shade", ShadeParams),
+                ("camera", Camera)]
 
 
 class RenderStats(C.Structure):
@@ -130,7 +131,7 @@ class RenderStats(C.Structure):
 class EngineConfig(C.Structure):
     _fields_ = [("capacity", C.c_int32), ("max_jobs", C.c_int32), ("pixel_budget", C.c_int64),
                 ("policy", C.c_int32), ("ray_sort", C.c_int32), ("use_graph", C.c_int32),
-                ("reserved", C.c_int32), ("shade", ShadeParams)]
+                ("sample_lanes", C.c_int32), ("shade", ShadeParams)]
 
 
 class JobDesc(C.Structure):
@@ -161,7 +162,7 @@ BVH_PLOC, BVH_LBVH = 0, 1
 class Tuning(C.Structure):
     _fields_ = [("refill_lanes", C.c_int32), ("trace_blocks_per_cu", C.c_int32),
                 ("bvh_builder", C.c_int32), ("ploc_radius", C.c_int32),
-                ("reserved", C.c_int32 * 4)]
+                ("bvh_width", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 _S = C.c_int32
@@ -202,7 +203,7 @@ _PROTOS = {
                       C.POINTER(RaySoA), C.POINTER(C.c_int32)]),
     "wf_shade_misses": (_S, [P, C.POINTER(RaySoA), C.c_int32, C.c_int32, P]),
     "wf_render": (_S, [P, C.POINTER(RenderParams), P, P, C.POINTER(RenderStats)]),
-    "wf_engine_bytes": (_S, [C.c_int32, C.c_int64, C.POINTER(C.c_size_t)]),
+    "wf_engine_bytes": (_S, [C.c_int32, C.c_int64, C.c_int32, C.POINTER(C.c_size_t)]),
     "wf_engine_create": (_S, [P, P, C.POINTER(EngineConfig), C.POINTER(P)]),
     "wf_engine_destroy": (_S, [P]),
     "wf_engine_device_bytes": (_S, [P, C.POINTER(C.c_size_t)]),

@@ -11,10 +11,11 @@
 // Internal nodes are numbered downwards from n - 2, so the last merge (the root) is node 0: the
 // output arrays have the layout of the Karras build (left/right child: node index, or ~leaf).
 //
-// The collapse is top-down, level by level: a four-wide node starts from the two children of its
-// binary node and keeps opening the inner child with the largest box surface area until it has
-// four children or only leaves. Nodes of a level are numbered in order after the previous level,
-// and the exact traversal stack bound is carried down the levels.
+// The collapse is top-down, level by level: a wide node (four or eight children) starts from the
+// two children of its binary node and keeps opening the inner child with the largest box surface
+// area until it is full or has only leaves. Nodes of a level are numbered in order after the
+// previous level, and the exact traversal stack bound is carried down the levels. Eight-wide
+// nodes store their child boxes quantised outward to 8 bits on a per-node power-of-two grid.
 #include <algorithm>
 
 #include "wf_device.h"
@@ -151,18 +152,22 @@ __device__ __forceinline__ float ref_area(int ref, const float4* slo, const floa
     return ref < 0 ? half_area(slo[~ref], shi[~ref]) : half_area(nlo[ref], nhi[ref]);
 }
 
-// Chooses the (up to four) children of each queued binary node; kint = inner children.
+// Chooses the (up to W) children of each queued binary node; kint = inner children.
+template <int W>
 __global__ void k_collapse_pick(int count, const int32_t* __restrict__ qb,
                                 const int32_t* __restrict__ left, const int32_t* __restrict__ right,
                                 const float4* __restrict__ slo, const float4* __restrict__ shi,
                                 const float4* __restrict__ nlo, const float4* __restrict__ nhi,
-                                int4* __restrict__ picked, uint32_t* __restrict__ kint) {
+                                int32_t* __restrict__ picked, uint32_t* __restrict__ kint) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count) return;
     const int b = qb[t];
-    int c[4] = {left[b], right[b], kEmptyChild, kEmptyChild};
+    int c[W];
+    c[0] = left[b];
+    c[1] = right[b];
+    for (int q = 2; q < W; ++q) c[q] = kEmptyChild;
     int m = 2;
-    while (m < 4) {
+    while (m < W) {
         int best = -1;
         float ba = -1.0f;
         for (int k = 0; k < m; ++k)
@@ -179,33 +184,75 @@ __global__ void k_collapse_pick(int count, const int32_t* __restrict__ qb,
         c[m++] = right[open];
     }
     uint32_t k = 0;
-    for (int q = 0; q < 4; ++q) k += (c[q] >= 0 && c[q] != kEmptyChild) ? 1u : 0u;
-    picked[t] = make_int4(c[0], c[1], c[2], c[3]);
+    for (int q = 0; q < W; ++q) {
+        k += (c[q] >= 0) ? 1u : 0u;
+        picked[(size_t)t * W + q] = c[q];
+    }
     kint[t] = k;
 }
 
-// Writes the four-wide nodes of a level and queues their inner children for the next level.
+__device__ __forceinline__ float qscale(uint32_t ebits) { return __uint_as_float(ebits << 23); }
+__device__ __forceinline__ float qdecode(float o, uint32_t q, float scale) {
+    return o + (float)q * scale;
+}
+
+// Smallest power-of-two scale (biased exponent, >= 1) whose 255 steps from lo reach hi.
+__device__ __forceinline__ uint32_t quant_exponent(float lo, float hi) {
+    const float s = (hi - lo) / 255.0f;
+    uint32_t e = (__float_as_uint(s) >> 23) & 0xFFu;
+    if (s > qscale(e)) ++e;
+    if (e < 1) e = 1;
+    while (e < 254 && qdecode(lo, 255u, qscale(e)) < hi) ++e;
+    return e;
+}
+
+// Outward 8-bit quantisation of [clo, chi] on the grid (o, scale).
+__device__ __forceinline__ void quantise(float o, float scale, float clo, float chi, uint32_t& qlo,
+                                         uint32_t& qhi) {
+    float fl = floorf((clo - o) / scale), fh = ceilf((chi - o) / scale);
+    int l = (int)fminf(fmaxf(fl, 0.0f), 255.0f), h = (int)fminf(fmaxf(fh, 0.0f), 255.0f);
+    while (l > 0 && qdecode(o, (uint32_t)l, scale) > clo) --l;
+    while (h < 255 && qdecode(o, (uint32_t)h, scale) < chi) ++h;
+    qlo = (uint32_t)l;
+    qhi = (uint32_t)h;
+}
+
+// Writes the wide nodes of a level and queues their inner children for the next level.
+template <int W>
 __global__ void k_collapse_write(int count, int next_base, const int32_t* __restrict__ qw,
-                                 const int32_t* __restrict__ qd, const int4* __restrict__ picked,
+                                 const int32_t* __restrict__ qd,
+                                 const int32_t* __restrict__ picked,
                                  const uint32_t* __restrict__ kint,
                                  const uint32_t* __restrict__ koff,
                                  const float4* __restrict__ slo, const float4* __restrict__ shi,
                                  const float4* __restrict__ nlo, const float4* __restrict__ nhi,
-                                 Bvh4Node* __restrict__ out, int32_t* __restrict__ nqb,
-                                 int32_t* __restrict__ nqw, int32_t* __restrict__ nqd,
-                                 int32_t* __restrict__ stack_bound) {
+                                 Bvh4Node* __restrict__ out4, Bvh8Node* __restrict__ out8,
+                                 int32_t* __restrict__ nqb, int32_t* __restrict__ nqw,
+                                 int32_t* __restrict__ nqd, int32_t* __restrict__ stack_bound) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count) return;
-    const int4 p = picked[t];
-    const int c[4] = {p.x, p.y, p.z, p.w};
+    int c[W];
+    if (W == 8) {
+        // leaf children first, then inner children, then empty slots (the eight-wide walker tests
+        // the leaves of slots 0..3 and 4..7 in one round trip each)
+        int m = 0;
+        for (int pass = 0; pass < 3; ++pass)
+            for (int q = 0; q < W; ++q) {
+                const int v = picked[(size_t)t * W + q];
+                const int cls = v == kEmptyChild ? 2 : (v < 0 ? 0 : 1);
+                if (cls == pass) c[m++] = v;
+            }
+    } else {
+        for (int q = 0; q < W; ++q) c[q] = picked[(size_t)t * W + q];
+    }
     const int k = (int)kint[t];
     const int depth_in = qd[t];
     const int child_depth = depth_in + (k > 0 ? k - 1 : 0);
     atomicMax(stack_bound, child_depth);
-    float4 lo[4], hi[4];
-    int ch[4];
+    float4 lo[W], hi[W];
+    int ch[W];
     int r = 0;
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < W; ++q) {
         if (c[q] == kEmptyChild) {
             lo[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             hi[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -224,16 +271,49 @@ __global__ void k_collapse_write(int count, int next_base, const int32_t* __rest
             nqd[slot] = child_depth;
         }
     }
-    Bvh4Node nd;
-    nd.lox = make_float4(lo[0].x, lo[1].x, lo[2].x, lo[3].x);
-    nd.loy = make_float4(lo[0].y, lo[1].y, lo[2].y, lo[3].y);
-    nd.loz = make_float4(lo[0].z, lo[1].z, lo[2].z, lo[3].z);
-    nd.hix = make_float4(hi[0].x, hi[1].x, hi[2].x, hi[3].x);
-    nd.hiy = make_float4(hi[0].y, hi[1].y, hi[2].y, hi[3].y);
-    nd.hiz = make_float4(hi[0].z, hi[1].z, hi[2].z, hi[3].z);
-    nd.child = make_int4(ch[0], ch[1], ch[2], ch[3]);
-    nd.pad = make_int4(0, 0, 0, 0);
-    out[qw[t]] = nd;
+    if (W == 4) {
+        Bvh4Node nd;
+        nd.lox = make_float4(lo[0].x, lo[1].x, lo[2].x, lo[3].x);
+        nd.loy = make_float4(lo[0].y, lo[1].y, lo[2].y, lo[3].y);
+        nd.loz = make_float4(lo[0].z, lo[1].z, lo[2].z, lo[3].z);
+        nd.hix = make_float4(hi[0].x, hi[1].x, hi[2].x, hi[3].x);
+        nd.hiy = make_float4(hi[0].y, hi[1].y, hi[2].y, hi[3].y);
+        nd.hiz = make_float4(hi[0].z, hi[1].z, hi[2].z, hi[3].z);
+        nd.child = make_int4(ch[0], ch[1], ch[2], ch[3]);
+        nd.pad = make_int4(0, 0, 0, 0);
+        out4[qw[t]] = nd;
+        return;
+    }
+    // node grid: union of the valid children
+    float L[3] = {3.0e38f, 3.0e38f, 3.0e38f}, H[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+    for (int q = 0; q < W; ++q) {
+        if (ch[q] == kEmptyChild) continue;
+        L[0] = fminf(L[0], lo[q].x); L[1] = fminf(L[1], lo[q].y); L[2] = fminf(L[2], lo[q].z);
+        H[0] = fmaxf(H[0], hi[q].x); H[1] = fmaxf(H[1], hi[q].y); H[2] = fmaxf(H[2], hi[q].z);
+    }
+    uint32_t e[3];
+    for (int a = 0; a < 3; ++a) e[a] = quant_exponent(L[a], H[a]);
+    uint32_t bytes[6][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}};   // [bound][half]
+    for (int q = 0; q < W; ++q) {
+        if (ch[q] == kEmptyChild) continue;
+        const float cl[3] = {lo[q].x, lo[q].y, lo[q].z}, chh[3] = {hi[q].x, hi[q].y, hi[q].z};
+        for (int a = 0; a < 3; ++a) {
+            uint32_t ql, qh;
+            quantise(L[a], qscale(e[a]), cl[a], chh[a], ql, qh);
+            bytes[a][q >> 2] |= ql << (8 * (q & 3));
+            bytes[3 + a][q >> 2] |= qh << (8 * (q & 3));
+        }
+    }
+    Bvh8Node nd;
+    nd.origin = make_float4(L[0], L[1], L[2], __uint_as_float(e[0] | (e[1] << 8) | (e[2] << 16)));
+    nd.child_lo = make_int4(ch[0], ch[1], ch[2], ch[3]);
+    nd.child_hi = make_int4(ch[4 % W], ch[5 % W], ch[6 % W], ch[7 % W]);
+    nd.qa = make_uint4(bytes[0][0], bytes[0][1], bytes[1][0], bytes[1][1]);
+    nd.qb = make_uint4(bytes[2][0], bytes[2][1], bytes[3][0], bytes[3][1]);
+    nd.qc = make_uint4(bytes[4][0], bytes[4][1], bytes[5][0], bytes[5][1]);
+    nd.pad[0] = make_int4(0, 0, 0, 0);
+    nd.pad[1] = make_int4(0, 0, 0, 0);
+    out8[qw[t]] = nd;
 }
 
 __global__ void k_set_root(int32_t* qb, int32_t* qw, int32_t* qd, int32_t* iparent) {
@@ -304,10 +384,11 @@ wf_status ploc_build(wf_context* ctx, int n, int radius, const float4* slo, cons
     return WF_OK;
 }
 
-wf_status collapse_greedy(wf_context* ctx, int n, const int32_t* left, const int32_t* right,
-                          int32_t* iparent, const float4* slo, const float4* shi,
-                          const float4* nlo, const float4* nhi, Bvh4Node* out,
-                          int32_t* n_nodes4, int32_t* stack_bound) {
+template <int W>
+static wf_status collapse_levels(wf_context* ctx, int n, const int32_t* left, const int32_t* right,
+                                 int32_t* iparent, const float4* slo, const float4* shi,
+                                 const float4* nlo, const float4* nhi, Bvh4Node* out4,
+                                 Bvh8Node* out8, int32_t* n_nodes, int32_t* stack_bound) {
     const int ni = n - 1;
     void *pqb[2], *pqw[2], *pqd[2], *ppick, *pk, *pkoff, *pcnt;
     for (int k = 0; k < 2; ++k) {
@@ -319,7 +400,7 @@ wf_status collapse_greedy(wf_context* ctx, int n, const int32_t* left, const int
         snprintf(name, sizeof(name), "col_qd%d", k);
         WF_TRY(workspace(ctx, name, sizeof(int32_t) * ni, &pqd[k]));
     }
-    WF_TRY(workspace(ctx, "col_pick", sizeof(int4) * ni, &ppick));
+    WF_TRY(workspace(ctx, "col_pick", sizeof(int32_t) * W * (size_t)ni, &ppick));
     WF_TRY(workspace(ctx, "col_kint", sizeof(uint32_t) * ni, &pk));
     WF_TRY(workspace(ctx, "col_koff", sizeof(uint32_t) * ni, &pkoff));
     WF_TRY(workspace(ctx, "col_counts", sizeof(int32_t) * 4, &pcnt));
@@ -331,29 +412,40 @@ wf_status collapse_greedy(wf_context* ctx, int n, const int32_t* left, const int
     int count = 1, total = 1, cur = 0;
     while (count > 0) {
         const int nxt = cur ^ 1;
-        k_collapse_pick<<<grid_for(count, 256), 256, 0, ctx->stream>>>(
-            count, (int32_t*)pqb[cur], left, right, slo, shi, nlo, nhi, (int4*)ppick,
+        k_collapse_pick<W><<<grid_for(count, 256), 256, 0, ctx->stream>>>(
+            count, (int32_t*)pqb[cur], left, right, slo, shi, nlo, nhi, (int32_t*)ppick,
             (uint32_t*)pk);
         WF_HIP(hipGetLastError());
         WF_TRY(exclusive_scan_u32(ctx, (uint32_t*)pk, (uint32_t*)pkoff, host_count(count), cnt,
                                   "bvh_build"));
-        k_collapse_write<<<grid_for(count, 256), 256, 0, ctx->stream>>>(
-            count, total, (int32_t*)pqw[cur], (int32_t*)pqd[cur], (int4*)ppick, (uint32_t*)pk,
-            (uint32_t*)pkoff, slo, shi, nlo, nhi, out, (int32_t*)pqb[nxt], (int32_t*)pqw[nxt],
-            (int32_t*)pqd[nxt], cnt + 1);
+        k_collapse_write<W><<<grid_for(count, 256), 256, 0, ctx->stream>>>(
+            count, total, (int32_t*)pqw[cur], (int32_t*)pqd[cur], (int32_t*)ppick, (uint32_t*)pk,
+            (uint32_t*)pkoff, slo, shi, nlo, nhi, out4, out8, (int32_t*)pqb[nxt],
+            (int32_t*)pqw[nxt], (int32_t*)pqd[nxt], cnt + 1);
         WF_HIP(hipGetLastError());
         int32_t next = 0;
         WF_TRY(readback(ctx, &next, cnt, sizeof(int32_t)));
         total += next;
-        if (total > ni) return set_error(WF_ERR_HIP, "four-wide collapse overflow");
+        if (total > ni) return set_error(WF_ERR_HIP, "wide collapse overflow");
         count = next;
         cur = nxt;
     }
     int32_t bound = 0;
     WF_TRY(readback(ctx, &bound, cnt + 1, sizeof(int32_t)));
-    *n_nodes4 = total;
+    *n_nodes = total;
     *stack_bound = bound;
     return WF_OK;
+}
+
+wf_status collapse_greedy(wf_context* ctx, int n, const int32_t* left, const int32_t* right,
+                          int32_t* iparent, const float4* slo, const float4* shi,
+                          const float4* nlo, const float4* nhi, Bvh4Node* out4, Bvh8Node* out8,
+                          int32_t* n_nodes, int32_t* stack_bound) {
+    if (out8)
+        return collapse_levels<8>(ctx, n, left, right, iparent, slo, shi, nlo, nhi, nullptr, out8,
+                                  n_nodes, stack_bound);
+    return collapse_levels<4>(ctx, n, left, right, iparent, slo, shi, nlo, nhi, out4, nullptr,
+                              n_nodes, stack_bound);
 }
 
 }  // namespace wf

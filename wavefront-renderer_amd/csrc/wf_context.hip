@@ -264,6 +264,8 @@ wf_status wf_context_set_tuning(wf_context* ctx, const wf_tuning* t) {
                "wf_context_set_tuning: unknown bvh_builder %d", t->bvh_builder);
     WF_REQUIRE(t->ploc_radius >= 1 && t->ploc_radius <= 32,
                "wf_context_set_tuning: ploc_radius %d not in [1, 32]", t->ploc_radius);
+    WF_REQUIRE(t->bvh_width == 4 || t->bvh_width == 8,
+               "wf_context_set_tuning: bvh_width %d is not 4 or 8", t->bvh_width);
     ctx->tuning = *t;
     return WF_OK;
 }

@@ -5,10 +5,14 @@
 // (WavefrontRenderer::renderScene, src/render/WavefrontRenderer.cpp:141-312), so every bounce
 // after the first runs on a shrinking queue. Here a fixed-capacity ray queue is refilled at every
 // iteration: when a path ends, its pixel goes back to a ready queue and starts its next sample at
-// the next iteration. Each pixel has at most one path in flight, so its samples still run in
-// order and its accumulator receives exactly the additions of the pass-by-pass loop, in the same
-// order; with the counter-based RNG the image is identical. Rays carry a job tag (job slot,
-// sample index), so paths of many jobs of different sizes share one wavefront.
+// the next iteration. A pixel has L sample lanes (L = 1 for adaptive jobs): lane l runs samples
+// l, l+L, l+2L, ... one after another into an accumulator of its own, and the lanes are summed
+// in lane order when the job resolves. With L = 1 the accumulator receives exactly the additions
+// of the pass-by-pass loop, in the same order, and with the counter-based RNG the image is
+// identical; with L > 1 only the order of the float additions differs. More lanes keep the
+// wavefront full while pixels with long paths finish their samples. Rays carry the accumulator
+// index in their pixel field and a job tag (job slot, sample index), so paths of many jobs of
+// different sizes share one wavefront.
 //
 // One iteration (all counts stay on the device, so the sequence can be captured in a graph):
 //   regenerate -> [ray keys + radix sort + gather] -> closest hit + split -> [shading policy sort
@@ -70,11 +74,13 @@ constexpr int kStatusHeader = 8;   // int32: [0] live rays, [2..3] head, [4..5] 
 
 struct EngineView {
     int32_t cap;               // ray slots
-    int32_t px;                // framebuffer pixels (ready-queue capacity)
+    int32_t px;                // framebuffer pixels (accumulator lane stride)
+    unsigned long long ring;   // ready-queue entries: px * most lanes
     int32_t* counters;
     unsigned long long* stats; // S_HEAD / S_TAIL: ready-queue cursors (monotonic)
-    int2* ready;               // (pixel, job slot << kTagJobShift | next sample index)
-    float* accum;              // 3 per pixel
+    int2* ready;               // (accumulator index, job slot << kTagJobShift | next sample)
+    float* accum;              // 3 per pixel and lane (lane-major)
+    int32_t* lanes_done;       // per pixel: lanes that ran their last sample
     float* before;             // 3 per pixel: accum at the start of the pixel's current sample
     float* mean;
     float* m2;
@@ -86,27 +92,32 @@ struct EngineView {
     int32_t* job_unconverged;  // adaptive: pixels that ran out of budget without converging
 };
 
-// Admits pixels [base, base + n) of job j: clears their state and queues them.
-__global__ void k_admit(EngineView e, int job, int base, int n, int queue) {
+// Admits pixels [base, base + n) of job j: clears their state and queues their first sample of
+// every lane (lane-major, so that lane 0 of every pixel starts first).
+__global__ void k_admit(EngineView e, int job, int base, int n, int lanes, int queue) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const int p = base + k;
-    e.accum[3 * (size_t)p] = 0.0f;
-    e.accum[3 * (size_t)p + 1] = 0.0f;
-    e.accum[3 * (size_t)p + 2] = 0.0f;
+    for (int l = 0; l < lanes; ++l) {
+        const size_t a = (size_t)p + (size_t)l * (size_t)e.px;
+        e.accum[3 * a] = 0.0f;
+        e.accum[3 * a + 1] = 0.0f;
+        e.accum[3 * a + 2] = 0.0f;
+        if (queue)
+            e.ready[(e.stats[S_TAIL] + (unsigned long long)l * n + (unsigned long long)k) % e.ring] =
+                make_int2((int)a, (job << kTagJobShift) | l);
+    }
     e.before[3 * (size_t)p] = 0.0f;
     e.before[3 * (size_t)p + 1] = 0.0f;
     e.before[3 * (size_t)p + 2] = 0.0f;
     e.mean[p] = 0.0f;
     e.m2[p] = 0.0f;
     e.pixel_count[p] = 0;
-    if (queue)
-        e.ready[(e.stats[S_TAIL] + (unsigned long long)k) % (unsigned long long)e.px] =
-            make_int2(p, job << kTagJobShift);
+    e.lanes_done[p] = 0;
 }
 
-__global__ void k_admit_commit(EngineView e, int job, int n, int queue) {
-    if (queue) e.stats[S_TAIL] += (unsigned long long)n;
+__global__ void k_admit_commit(EngineView e, int job, int n, int lanes, int queue) {
+    if (queue) e.stats[S_TAIL] += (unsigned long long)n * (unsigned long long)lanes;
     e.job_left[job] = queue ? n : 0;
     e.job_max_count[job] = 0;
     e.job_unconverged[job] = 0;
@@ -125,18 +136,18 @@ __global__ void k_regen(EngineView e, wf_ray_soa r, int32_t* __restrict__ tags) 
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int take = regen_take(e);
     if (i >= take) return;
-    const int2 entry = e.ready[(e.stats[S_HEAD] + (unsigned long long)i) % (unsigned long long)e.px];
-    const int p = entry.x;
+    const int2 entry = e.ready[(e.stats[S_HEAD] + (unsigned long long)i) % e.ring];
+    const int a = entry.x;
     const int32_t pt = entry.y;
     const int j = pt >> kTagJobShift;
     const JobDev& J = e.jobs[j];
     const int s = pt & kTagSampleMask;
-    if (J.adaptive) {
-        e.before[3 * (size_t)p] = e.accum[3 * (size_t)p];
-        e.before[3 * (size_t)p + 1] = e.accum[3 * (size_t)p + 1];
-        e.before[3 * (size_t)p + 2] = e.accum[3 * (size_t)p + 2];
+    if (J.adaptive) {   // one lane: a is the pixel
+        e.before[3 * (size_t)a] = e.accum[3 * (size_t)a];
+        e.before[3 * (size_t)a + 1] = e.accum[3 * (size_t)a + 1];
+        e.before[3 * (size_t)a + 2] = e.accum[3 * (size_t)a + 2];
     }
-    const int local = p - J.pixel_base;
+    const int local = lane_pixel(J, a, s) - J.pixel_base;
     PathRng rng{J.seed, (uint32_t)J.tag, (uint32_t)local, (uint32_t)s, kRaygenDepth};
     const V3 d = primary_dir(J.cam, local % J.width, local / J.width, J.width, J.height, rng(0),
                              rng(1));
@@ -146,7 +157,7 @@ __global__ void k_regen(EngineView e, wf_ray_soa r, int32_t* __restrict__ tags) 
     r.tr[k] = 1.0f;
     r.tg[k] = 1.0f;
     r.tb[k] = 1.0f;
-    r.pixel[k] = p;
+    r.pixel[k] = a;
     r.depth[k] = 0;
     r.pdf[k] = -1.0f;
     tags[k] = (j << kTagJobShift) | s;
@@ -163,13 +174,14 @@ __global__ void k_regen_commit(EngineView e) {
 
 constexpr int kFinishBlock = 1024;
 
-// A path of pixel p ended: count the sample, update the adaptive estimate
+// A path of accumulator a ended: count the sample, update the adaptive estimate
 // (AdaptiveSampler::addSample/updateConvergence, include/render/AdaptiveSampler.h:21-105, checked
 // when the pixel's sample count is a multiple of the interval, as the pass loop does), then
-// requeue the pixel or retire it. Requeues are appended with one queue-cursor atomic per block.
-// Called by every thread of the block (valid = this thread has an ended path; tag = its job slot
-// and sample index).
-__device__ __forceinline__ void finish_path(const EngineView& e, int p, int32_t tag, bool valid) {
+// requeue the lane's next sample, or end the lane; the pixel retires when its last lane ends (or
+// when it converged). Requeues are appended with one queue-cursor atomic per block. Called by
+// every thread of the block (valid = this thread has an ended path; tag = its job slot and
+// sample index).
+__device__ __forceinline__ void finish_path(const EngineView& e, int a, int32_t tag, bool valid) {
     __shared__ uint32_t block_count;
     __shared__ unsigned long long block_base;
     bool requeue = false, retire = false, unconverged = false;
@@ -177,9 +189,11 @@ __device__ __forceinline__ void finish_path(const EngineView& e, int p, int32_t 
     if (valid) {
         const int j = tag >> kTagJobShift;
         const JobDev& J = e.jobs[j];
-        const int cnt = (tag & kTagSampleMask) + 1;
+        const int s = tag & kTagSampleMask;
+        const int p = lane_pixel(J, a, s);
         bool converged = false;
-        if (J.adaptive) {
+        if (J.adaptive) {   // one lane: a == p, and s + 1 samples have run
+            const int cnt = s + 1;
             const size_t b = 3 * (size_t)p;
             const V3 c(e.accum[b] - e.before[b], e.accum[b + 1] - e.before[b + 1],
                        e.accum[b + 2] - e.before[b + 2]);
@@ -197,16 +211,22 @@ __device__ __forceinline__ void finish_path(const EngineView& e, int p, int32_t 
                 converged = se < J.adaptive_threshold * (mn + 0.01f);
             }
         }
-        const bool budget = cnt >= J.spp;
-        if (converged || budget) {
-            e.pixel_count[p] = cnt;
+        const int ns = s + J.lanes;
+        if (converged) {
+            e.pixel_count[p] = s + 1;
             retire = true;
-            unconverged = J.adaptive && !converged;
             job = j;
-            samples = cnt;
-        } else {
+            samples = s + 1;
+        } else if (ns < J.spp) {
             requeue = true;
-            next = (j << kTagJobShift) | cnt;
+            next = (j << kTagJobShift) | ns;
+        } else if (J.lanes == 1 || atomicAdd(&e.lanes_done[p], 1) == J.lanes - 1) {
+            // the pixel's last lane ran its last sample: all spp samples are in
+            e.pixel_count[p] = J.spp;
+            retire = true;
+            unconverged = J.adaptive != 0;
+            job = j;
+            samples = J.spp;
         }
     }
     // retired pixels: one set of job-counter atomics per wave for the lanes sharing the first
@@ -245,7 +265,7 @@ __device__ __forceinline__ void finish_path(const EngineView& e, int p, int32_t 
     if (requeue) {
         const unsigned long long pos =
             block_base + wave_off + (unsigned long long)__popcll(b & lanemask_lt());
-        e.ready[pos % (unsigned long long)e.px] = make_int2(p, next);
+        e.ready[pos % e.ring] = make_int2(a, next);
     }
 }
 
@@ -282,8 +302,9 @@ __global__ void k_iteration_end(EngineView e) {
     cur[1] = e.stats[S_TAIL];
 }
 
+// Image of a job: the lane accumulators summed in lane order, divided by the sample count.
 __global__ void k_resolve_job(const float* __restrict__ accum, const int32_t* __restrict__ pixel_count,
-                              int base, int n, float* __restrict__ image,
+                              int base, int n, int lanes, int stride, float* __restrict__ image,
                               int32_t* __restrict__ counts_out) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
@@ -291,9 +312,16 @@ __global__ void k_resolve_job(const float* __restrict__ accum, const int32_t* __
     const int count = pixel_count[p];
     int c = count;
     c = c > 1 ? c : 1;
-    image[3 * (size_t)k] = accum[3 * (size_t)p] / (float)c;
-    image[3 * (size_t)k + 1] = accum[3 * (size_t)p + 1] / (float)c;
-    image[3 * (size_t)k + 2] = accum[3 * (size_t)p + 2] / (float)c;
+    float r = accum[3 * (size_t)p], g = accum[3 * (size_t)p + 1], b = accum[3 * (size_t)p + 2];
+    for (int l = 1; l < lanes; ++l) {
+        const size_t a = 3 * ((size_t)p + (size_t)l * (size_t)stride);
+        r = r + accum[a];
+        g = g + accum[a + 1];
+        b = b + accum[a + 2];
+    }
+    image[3 * (size_t)k] = r / (float)c;
+    image[3 * (size_t)k + 1] = g / (float)c;
+    image[3 * (size_t)k + 2] = b / (float)c;
     if (counts_out) counts_out[k] = count;
 }
 
@@ -306,7 +334,7 @@ struct wf_engine {
     wf_context* ctx = nullptr;          // own stream and workspaces
     wf_scene* scene = nullptr;
     wf_engine_config cfg{};
-    int32_t cap = 0, px = 0, max_jobs = 0;
+    int32_t cap = 0, px = 0, max_jobs = 0, lanes = 1;
     std::vector<void*> allocations;
     size_t bytes = 0;
     // device state
@@ -389,9 +417,19 @@ wf_status engine_array(wf_engine* e, size_t count, T** out) {
     return WF_OK;
 }
 
-// Bytes of device memory per ray slot and per framebuffer pixel (admission accounting).
+// Bytes of device memory per ray slot, per framebuffer pixel and per further sample lane of a
+// pixel (admission accounting).
 constexpr size_t kBytesPerSlot = 48 * 3 + 104 * 2 + 4 * 5 + 400;   // queues + tags + workspaces
-constexpr size_t kBytesPerPixel = 12 + 12 + 4 + 4 + 4 + 8;   // accum, start copy, mean, M2, count, queue
+constexpr size_t kBytesPerPixel = 12 + 12 + 4 + 4 + 4 + 4 + 8;   // accum, start copy, mean, M2,
+                                                                  // count, lanes done, queue
+constexpr size_t kBytesPerLane = 12 + 8;                          // accum, queue
+constexpr int kDefaultLanes = 8;
+
+int lanes_of(int32_t sample_lanes) { return sample_lanes == 0 ? kDefaultLanes : sample_lanes; }
+bool lanes_ok(int32_t sample_lanes) {
+    return sample_lanes == 0 || sample_lanes == 1 || sample_lanes == 2 || sample_lanes == 4 ||
+           sample_lanes == 8 || sample_lanes == 16;
+}
 
 int g_phase_set = 0;   // event set of the iteration being enqueued
 
@@ -560,6 +598,10 @@ wf_status wf_engine_create(wf_context* ctx, wf_scene* scene, const wf_engine_con
                "wf_engine_create: pixel budget %lld out of range", (long long)cfg->pixel_budget);
     WF_REQUIRE(cfg->max_jobs > 0 && cfg->max_jobs <= kMaxJobs, "wf_engine_create: max_jobs %d not in [1, %d]",
                cfg->max_jobs, kMaxJobs);
+    WF_REQUIRE(lanes_ok(cfg->sample_lanes), "wf_engine_create: sample_lanes %d not in {0, 1, 2, 4, 8, 16}",
+               cfg->sample_lanes);
+    WF_REQUIRE(cfg->pixel_budget * lanes_of(cfg->sample_lanes) < (1ll << 31),
+               "wf_engine_create: pixel budget x sample lanes must stay below 2^31");
     WF_TRY(scene_ready(scene));
     wf_engine* e = new wf_engine();
     e->parent = ctx;
@@ -568,6 +610,7 @@ wf_status wf_engine_create(wf_context* ctx, wf_scene* scene, const wf_engine_con
     e->cap = cfg->capacity;
     e->px = (int32_t)cfg->pixel_budget;
     e->max_jobs = cfg->max_jobs;
+    e->lanes = lanes_of(cfg->sample_lanes);
     auto fail = [&](wf_status st) {
         wf_engine_destroy(e);
         return st;
@@ -597,10 +640,12 @@ wf_status wf_engine_create(wf_context* ctx, wf_scene* scene, const wf_engine_con
     EngineView& v = e->view;
     v.cap = e->cap;
     v.px = e->px;
+    v.ring = (unsigned long long)px * (unsigned long long)e->lanes;
     ETRY(engine_array(e, (size_t)C_COUNT, &v.counters));
     ETRY(engine_array(e, (size_t)S_COUNT, &v.stats));
-    ETRY(engine_array(e, px, &v.ready));
-    ETRY(engine_array(e, 3 * px, &v.accum));
+    ETRY(engine_array(e, px * (size_t)e->lanes, &v.ready));
+    ETRY(engine_array(e, 3 * px * (size_t)e->lanes, &v.accum));
+    ETRY(engine_array(e, px, &v.lanes_done));
     ETRY(engine_array(e, 3 * px, &v.before));
     ETRY(engine_array(e, px, &v.mean));
     ETRY(engine_array(e, px, &v.m2));
@@ -651,9 +696,14 @@ wf_status wf_engine_destroy(wf_engine* e) {
     return WF_OK;
 }
 
-wf_status wf_engine_bytes(int32_t capacity, int64_t pixel_budget, size_t* out) {
+wf_status wf_engine_bytes(int32_t capacity, int64_t pixel_budget, int32_t sample_lanes,
+                          size_t* out) {
     WF_REQUIRE(out && capacity >= 0 && pixel_budget >= 0, "wf_engine_bytes: bad argument");
-    *out = kBytesPerSlot * (size_t)capacity + kBytesPerPixel * (size_t)pixel_budget;
+    WF_REQUIRE(lanes_ok(sample_lanes), "wf_engine_bytes: sample_lanes %d not in {0, 1, 2, 4, 8, 16}",
+               sample_lanes);
+    *out = kBytesPerSlot * (size_t)capacity +
+           (kBytesPerPixel + kBytesPerLane * (size_t)(lanes_of(sample_lanes) - 1)) *
+               (size_t)pixel_budget;
     return WF_OK;
 }
 
@@ -709,14 +759,20 @@ wf_status wf_engine_admit(wf_engine* e, const wf_job_desc* job, int32_t* job_id)
     J.adaptive_threshold = job->adaptive_threshold;
     J.adaptive_min_samples = job->adaptive_min_samples;
     J.adaptive_interval = job->adaptive ? job->adaptive_interval : 1;
+    int lanes = 1;
+    const int most = std::min(e->lanes, lanes_of(e->cfg.sample_lanes));
+    if (!job->adaptive)
+        while (lanes * 2 <= most && lanes * 2 <= job->spp) lanes *= 2;
+    J.lanes = lanes;
+    J.lane_stride = e->px;
     const CamBasis cb = camera_basis(job->camera, job->width, job->height);
     std::memcpy(J.cam, cb.v, sizeof(J.cam));
     wf_context* ctx = e->ctx;
     WF_HIP(hipMemcpyAsync(e->jobs_dev + slot, &J, sizeof(JobDev), hipMemcpyHostToDevice, ctx->stream));
     const int queue = (job->spp > 0 && job->max_depth > 0) ? 1 : 0;
-    k_admit<<<grid_for(P, 256), 256, 0, ctx->stream>>>(e->view, slot, base, P, queue);
+    k_admit<<<grid_for(P, 256), 256, 0, ctx->stream>>>(e->view, slot, base, P, lanes, queue);
     WF_HIP(hipGetLastError());
-    k_admit_commit<<<1, 1, 0, ctx->stream>>>(e->view, slot, P, queue);
+    k_admit_commit<<<1, 1, 0, ctx->stream>>>(e->view, slot, P, lanes, queue);
     WF_HIP(hipGetLastError());
     // the host copy of J must outlive the asynchronous upload
     WF_HIP(hipStreamSynchronize(ctx->stream));
@@ -849,8 +905,8 @@ wf_status wf_engine_result(wf_engine* e, int32_t job_id, float* image_rgb, int32
     const int P = J.width * J.height;
     wf_context* ctx = e->ctx;
     k_resolve_job<<<grid_for(P, 256), 256, 0, ctx->stream>>>(e->view.accum, e->view.pixel_count,
-                                                             J.pixel_base, P, image_rgb,
-                                                             sample_counts);
+                                                             J.pixel_base, P, J.lanes, e->px,
+                                                             image_rgb, sample_counts);
     WF_HIP(hipGetLastError());
     int32_t vals[3] = {0, 0, 0};
     WF_HIP(hipMemcpyAsync(&vals[0], e->view.job_max_count + job_id, sizeof(int32_t),
@@ -919,7 +975,10 @@ wf_status render_regenerate(wf_scene* s, const wf_render_params* rp, float* imag
                             int32_t* sample_counts, wf_render_stats* stats) {
     const int P = rp->width * rp->height;
     wf_engine* e = s->render_engine;
-    if (e && (e->cap < P || e->px < P)) {
+    WF_REQUIRE(lanes_ok(rp->sample_lanes), "wf_render: sample_lanes %d not in {0, 1, 2, 4, 8, 16}",
+               rp->sample_lanes);
+    const int lanes = rp->adaptive ? 1 : lanes_of(rp->sample_lanes);
+    if (e && (e->cap < P || e->px < P || e->lanes < lanes)) {
         wf_engine_destroy(e);
         e = s->render_engine = nullptr;
     }
@@ -930,6 +989,7 @@ wf_status render_regenerate(wf_scene* s, const wf_render_params* rp, float* imag
     cfg.policy = rp->policy;
     cfg.ray_sort = rp->ray_sort;
     cfg.use_graph = 1;
+    cfg.sample_lanes = lanes;
     cfg.shade = rp->shade;
     if (!e) {
         WF_TRY(wf_engine_create(s->ctx, s, &cfg, &e));

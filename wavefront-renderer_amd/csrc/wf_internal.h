@@ -73,7 +73,7 @@ struct wf_context {
     std::vector<wf::PendingTiming> pending;
     std::vector<hipEvent_t> event_pool;
     int32_t* count_slots = nullptr;    // pinned, kCountSlots entries
-    wf_tuning tuning{48, 6, WF_BVH_PLOC, 16, {0, 0, 0, 0}};
+    wf_tuning tuning{48, 6, WF_BVH_PLOC, 16, 4, {0, 0, 0}};
     int next_count_slot = 0;
     int cu_count = 256;
 };
@@ -138,6 +138,20 @@ struct Bvh4Node {
 };
 constexpr int32_t kEmptyChild = (int32_t)0x80000000;
 
+// Eight-wide node (128 B) with child boxes quantised to 8 bits per bound on a per-node grid:
+// bound = origin + q * 2^(e - 127) per axis (origin in `origin.xyz`, the three biased exponents
+// in the bytes of `origin.w`). Quantisation rounds outward, checked with the decode's own float
+// operations, so every decoded box contains its child.
+struct Bvh8Node {
+    float4 origin;
+    int4 child_lo;     // children 0..3 (>= 0 node, < 0 ~leaf, kEmptyChild empty)
+    int4 child_hi;     // children 4..7
+    uint4 qa;          // lo.x bytes of children 0..7, then lo.y
+    uint4 qb;          // lo.z, then hi.x
+    uint4 qc;          // hi.y, then hi.z
+    int4 pad[2];
+};
+
 struct SceneDev {
     int32_t n_tris = 0, n_meshes = 0, n_materials = 0, n_textures = 0, n_vertices = 0;
     // triangles in BVH leaf order: v0 (w = global index bits), e1, e2
@@ -190,7 +204,9 @@ struct SceneDev {
     float env_tint[3] = {1.0f, 1.0f, 1.0f};
     float env_choice_prob = 0.0f;
     // BVH (two-child LBVH collapsed to four-wide nodes)
-    Bvh4Node* nodes4 = nullptr;
+    Bvh4Node* nodes4 = nullptr;        // four-wide nodes (width 4)
+    Bvh8Node* nodes8 = nullptr;        // eight-wide quantised nodes (width 8)
+    int32_t width = 4;
     int32_t n_nodes = 0;               // two-child internal nodes of the build
     int32_t n_nodes4 = 0;
     int32_t max_depth = 0;             // two-child depth (deepest leaf's ancestor count)
@@ -256,9 +272,17 @@ struct JobDev {
     float adaptive_threshold;
     int32_t adaptive_min_samples;
     int32_t adaptive_interval;
-    int32_t pad;
+    int32_t lanes;             // samples in flight per pixel (power of two); sample s runs in
+                               // lane s & (lanes - 1), whose accumulator is pixel + lane * stride
     float cam[12];             // eye, top-left, horizontal, vertical (Camera.h:13-33)
+    int32_t lane_stride;       // accumulator index distance between the lanes of a pixel
+    int32_t pad;
 };
+
+// Framebuffer pixel of an accumulator index a of job J running sample s.
+__host__ __device__ __forceinline__ int32_t lane_pixel(const JobDev& J, int32_t a, int32_t s) {
+    return a - (s & (J.lanes - 1)) * J.lane_stride;
+}
 
 // Camera basis (Camera.h:13-33): v = eye, top-left corner, horizontal, vertical (12 floats).
 struct CamBasis {
@@ -329,11 +353,12 @@ wf_status render_regenerate(wf_scene* s, const wf_render_params* rp, float* imag
 wf_status ploc_build(wf_context* ctx, int n, int radius, const float4* slo, const float4* shi,
                      int32_t* left, int32_t* right, int32_t* iparent, int32_t* lparent,
                      float4* nlo, float4* nhi);
-// Greedy surface-area collapse of a binary hierarchy (root 0) into four-wide nodes.
+// Greedy surface-area collapse of a binary hierarchy (root 0) into four-wide nodes (out4) or
+// eight-wide quantised nodes (out8), whichever is given.
 wf_status collapse_greedy(wf_context* ctx, int n, const int32_t* left, const int32_t* right,
                           int32_t* iparent, const float4* slo, const float4* shi,
-                          const float4* nlo, const float4* nhi, Bvh4Node* out,
-                          int32_t* n_nodes4, int32_t* stack_bound);
+                          const float4* nlo, const float4* nhi, Bvh4Node* out4, Bvh8Node* out8,
+                          int32_t* n_nodes, int32_t* stack_bound);
 wf_status ray_keys(wf_context* ctx, const wf_ray_soa* rays, Count n, const float bmin[3],
                    const float bmax[3], uint64_t* keys);
 wf_status shading_keys(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q, Count n,

@@ -19,6 +19,7 @@
 
 #include "wf_device.h"
 #include "wf_internal.h"
+#include "wf_traverse.h"
 
 namespace wf {
 
@@ -655,10 +656,17 @@ wf_status wf_accel_build(wf_scene* s, wf_build_stats* stats) {
             (float4*)pslo, (float4*)pshi);
         WF_HIP(hipGetLastError());
         int32_t depth = 0;
-        if (!d.nodes4) {
+        // a single triangle always gets the four-wide root
+        d.width = (n > 1 && ctx->tuning.bvh_width == 8) ? 8 : 4;
+        if (d.width == 4 && !d.nodes4) {
             void* pn4 = nullptr;
             WF_TRY(scene_alloc(s, sizeof(Bvh4Node) * (size_t)std::max(1, n - 1), &pn4));
             d.nodes4 = (Bvh4Node*)pn4;
+        }
+        if (d.width == 8 && !d.nodes8) {
+            void* pn8 = nullptr;
+            WF_TRY(scene_alloc(s, sizeof(Bvh8Node) * (size_t)std::max(1, n - 1), &pn8));
+            d.nodes8 = (Bvh8Node*)pn8;
         }
         if (n == 1) {
             k_bvh4_single<<<1, 1, 0, ctx->stream>>>((float4*)pslo, (float4*)pshi, d.nodes4);
@@ -705,10 +713,26 @@ wf_status wf_accel_build(wf_scene* s, wf_build_stats* stats) {
             WF_TRY(readback(ctx, &depth, pdep, sizeof(int32_t)));
             if (depth >= n) return set_error(WF_ERR_HIP, "BVH build produced a malformed hierarchy");
             d.n_nodes = n - 1;
-            // greedy surface-area collapse to four-wide nodes
+            // greedy surface-area collapse to four- or eight-wide nodes
             WF_TRY(collapse_greedy(ctx, n, (int32_t*)pl, (int32_t*)pr, (int32_t*)pip,
                                    (float4*)pslo, (float4*)pshi, (float4*)pnlo, (float4*)pnhi,
-                                   d.nodes4, &d.n_nodes4, &d.stack_bound));
+                                   d.width == 4 ? d.nodes4 : nullptr,
+                                   d.width == 8 ? d.nodes8 : nullptr, &d.n_nodes4,
+                                   &d.stack_bound));
+            if (d.width == 8 && d.stack_bound > kStackMax) {
+                // an eight-wide node pushes up to seven entries: fall back to four-wide nodes
+                // when the eight-wide stack bound exceeds the largest traversal instantiation
+                d.width = 4;
+                if (!d.nodes4) {
+                    void* pn4 = nullptr;
+                    WF_TRY(scene_alloc(s, sizeof(Bvh4Node) * (size_t)(n - 1), &pn4));
+                    d.nodes4 = (Bvh4Node*)pn4;
+                }
+                WF_TRY(collapse_greedy(ctx, n, (int32_t*)pl, (int32_t*)pr, (int32_t*)pip,
+                                       (float4*)pslo, (float4*)pshi, (float4*)pnlo,
+                                       (float4*)pnhi, d.nodes4, nullptr, &d.n_nodes4,
+                                       &d.stack_bound));
+            }
         }
         d.max_depth = depth;
     } else {
@@ -731,7 +755,7 @@ wf_status wf_accel_build(wf_scene* s, wf_build_stats* stats) {
         stats->node_count = d.n_nodes4;
         stats->max_depth = d.max_depth;
         stats->stack_bound = d.stack_bound;
-        stats->reserved = 0;
+        stats->bvh_width = d.width;
         stats->build_ms = ms;
         for (int k = 0; k < 3; ++k) {
             stats->bounds_min[k] = d.bounds_min[k];

@@ -363,7 +363,7 @@ __global__ void k_shade(const SceneDev s, wf_shade_soa q, int n_host,
         seed = J.seed;
         job = (uint32_t)J.tag;
         sample = (uint32_t)(tg & kTagSampleMask);
-        rng_pixel = pixel - J.pixel_base;
+        rng_pixel = lane_pixel(J, pixel, (int32_t)sample) - J.pixel_base;
         depth_limit = J.max_depth;
     }
     PathRng rng{seed, job, (uint32_t)rng_pixel, sample, (uint32_t)depth};
@@ -672,8 +672,9 @@ __global__ void k_miss(const SceneDev s, wf_ray_soa q, int n_host, const int32_t
 
 // EmbreeAccelerator::occluded for the NEE rays of each listed entry, then the contributions in
 // the reference order: area/environment light first, directional light second (cpp:678, 720).
-template <int STACK>
-__global__ __launch_bounds__(kTraceBlock) void k_shadow(const SceneDev s, wf_shade_soa q, ShadeOut o,
+template <class NODE, int STACK>
+__global__ __launch_bounds__(kTraceBlock) void k_shadow(const SceneDev s, const NODE* __restrict__ nodes,
+                                                        wf_shade_soa q, ShadeOut o,
                                                         const int32_t* __restrict__ list, int count_host,
                                                         const int32_t* __restrict__ count_dev,
                                                         float* __restrict__ accum,
@@ -720,7 +721,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(const SceneDev s, wf_sha
             continue;
         }
         if (item < 0) continue;
-        if (phase != 2 && (s.n_tris <= 0 || w.step(s.nodes4, s.tri, stack, spill))) {
+        if (phase != 2 && (s.n_tris <= 0 || w.step(nodes, s.tri, stack, spill))) {
             const bool occ = w.h.leaf >= 0;
             if (phase == 0) {
                 occ_area = occ;
@@ -831,10 +832,10 @@ wf_status shade_bounce(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q
         WF_TRY(workspace(ctx, "shadow_counter", sizeof(int32_t), &pc));
         WF_TRY(wf_fill_u32(ctx, pc, 0u, 1));
         Timed tm(ctx, "shadow", 41.0, ns);
-        with_stack(s->dev.stack_bound, [&](auto S) {
-            k_shadow<decltype(S)::value>
+        with_bvh(s->dev, [&](auto nodes, auto S) {
+            k_shadow<std::remove_cv_t<std::remove_pointer_t<decltype(nodes)>>, decltype(S)::value>
                 <<<persistent_grid(ns.n, ctx), kTraceBlock, 0, ctx->stream>>>(
-                    s->dev, *q, o, (int32_t*)plist, ns.n, ns.dev, accum, (int32_t*)pc,
+                    s->dev, nodes, *q, o, (int32_t*)plist, ns.n, ns.dev, accum, (int32_t*)pc,
                     ctx->tuning.refill_lanes);
         });
         WF_HIP(hipGetLastError());
@@ -875,10 +876,10 @@ wf_status shade_bounce(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q
         WF_TRY(workspace(ctx, "shadow_counter", sizeof(int32_t), &pc));
         WF_TRY(wf_fill_u32(ctx, pc, 0u, 1));
         Timed tm(ctx, "shadow", 41.0, ns);
-        with_stack(s->dev.stack_bound, [&](auto S) {
-            k_shadow<decltype(S)::value>
+        with_bvh(s->dev, [&](auto nodes, auto S) {
+            k_shadow<std::remove_cv_t<std::remove_pointer_t<decltype(nodes)>>, decltype(S)::value>
                 <<<persistent_grid(ns.n, ctx), kTraceBlock, 0, ctx->stream>>>(
-                    s->dev, *q, o, (int32_t*)plist, ns.n, ns.dev, accum, (int32_t*)pc,
+                    s->dev, nodes, *q, o, (int32_t*)plist, ns.n, ns.dev, accum, (int32_t*)pc,
                     ctx->tuning.refill_lanes);
         });
         WF_HIP(hipGetLastError());

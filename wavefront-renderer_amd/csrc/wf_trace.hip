@@ -2,7 +2,7 @@
 // (src/accelerator/Embree.cpp:122-279) and the hit/miss split of WavefrontRenderer::intersectAll
 // (src/render/WavefrontRenderer.cpp:459-567).
 //
-// One lane per ray over the four-wide BVH. The top of the traversal stack lives in LDS, one
+// One lane per ray over the four- or eight-wide BVH. The top of the traversal stack lives in LDS, one
 // column per lane (stack[entry][lane] so that a wave's pushes and pops hit 64 different banks),
 // deeper entries in a per-lane private array sized from the build's exact stack bound. The triangle test
 // is the reference's Moller-Trumbore (include/geometry/Triangle.h:6-42) with the same float
@@ -15,9 +15,9 @@ namespace wf {
 
 // Persistent closest-hit kernel: each lane walks one ray at a time and takes a new ray as soon as
 // its walk ends (dynamic refill keeps lanes busy while the slowest rays of a wave finish).
-template <int STACK>
+template <class NODE, int STACK>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(
-    const Bvh4Node* __restrict__ nodes, const float4* __restrict__ tri, int n_tris,
+    const NODE* __restrict__ nodes, const float4* __restrict__ tri, int n_tris,
     const float* __restrict__ ox, const float* __restrict__ oy, const float* __restrict__ oz,
     const float* __restrict__ dx, const float* __restrict__ dy, const float* __restrict__ dz,
     int n_host, const int32_t* __restrict__ nd, float tnear, float tfar,
@@ -54,9 +54,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(
 
 // occluded(): tnear 0.001, tfar = maxDistance - 0.001; a negative tfar reports occluded
 // (Embree.cpp:268-278).
-template <int STACK>
+template <class NODE, int STACK>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_occluded(
-    const Bvh4Node* __restrict__ nodes, const float4* __restrict__ tri, int n_tris,
+    const NODE* __restrict__ nodes, const float4* __restrict__ tri, int n_tris,
     const float* __restrict__ ox, const float* __restrict__ oy, const float* __restrict__ oz,
     const float* __restrict__ dx, const float* __restrict__ dy, const float* __restrict__ dz,
     const float* __restrict__ maxd, int n, uint8_t* __restrict__ out) {
@@ -75,9 +75,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_occluded(
 
 // Traversal statistics: BVH nodes visited and triangles tested per ray (lanes of a wave are rays
 // 64w..64w+63, so SIMD efficiency follows from the per-ray counts).
-template <int STACK, bool ANY>
+template <class NODE, int STACK, bool ANY>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_stats(
-    const Bvh4Node* __restrict__ nodes, const float4* __restrict__ tri, int n_tris,
+    const NODE* __restrict__ nodes, const float4* __restrict__ tri, int n_tris,
     wf_ray_soa r, int n, float tnear, float tfar, int32_t* __restrict__ out_nodes,
     int32_t* __restrict__ out_tris) {
     __shared__ int32_t stack[lds_entries<STACK>() * kTraceBlock];
@@ -249,10 +249,11 @@ wf_status trace_closest(wf_context* ctx, const wf_scene* s, const wf_ray_soa* r,
     WF_TRY(workspace(ctx, "trace_counter", sizeof(int32_t), &pc));
     WF_TRY(wf_fill_u32(ctx, pc, 0u, 1));
     Timed tm(ctx, "traverse_closest", 44.0, n);
-    with_stack(s->dev.stack_bound, [&](auto S) {
-        k_trace_closest<decltype(S)::value>
+    with_bvh(s->dev, [&](auto nodes, auto S) {
+        k_trace_closest<std::remove_cv_t<std::remove_pointer_t<decltype(nodes)>>,
+                        decltype(S)::value>
             <<<persistent_grid(n.n, ctx), kTraceBlock, 0, ctx->stream>>>(
-                s->dev.nodes4, s->dev.tri, s->dev.n_tris, r->ox, r->oy, r->oz, r->dx, r->dy,
+                nodes, s->dev.tri, s->dev.n_tris, r->ox, r->oy, r->oz, r->dx, r->dy,
                 r->dz, n.n, n.dev, tnear, tfar, hit_leaf, hit_t, hit_bu, hit_bv, flags,
                 (int32_t*)pc, ctx->tuning.refill_lanes);
     });
@@ -267,10 +268,11 @@ wf_status trace_occluded(const wf_scene* s, const float* ox, const float* oy, co
     if (n <= 0) return WF_OK;
     wf_context* ctx = s->ctx;
     Timed tm(ctx, "traverse_any", 29.0 * n);
-    with_stack(s->dev.stack_bound, [&](auto S) {
-        k_trace_occluded<decltype(S)::value>
+    with_bvh(s->dev, [&](auto nodes, auto S) {
+        k_trace_occluded<std::remove_cv_t<std::remove_pointer_t<decltype(nodes)>>,
+                         decltype(S)::value>
             <<<grid_for(n, kTraceBlock), kTraceBlock, 0, ctx->stream>>>(
-                s->dev.nodes4, s->dev.tri, s->dev.n_tris, ox, oy, oz, dx, dy, dz, maxd, n, out);
+                nodes, s->dev.tri, s->dev.n_tris, ox, oy, oz, dx, dy, dz, maxd, n, out);
     });
     WF_HIP(hipGetLastError());
     return WF_OK;
@@ -395,17 +397,18 @@ wf_status wf_trace_stats(const wf_scene* s, const wf_ray_soa* rays, int32_t n, f
     WF_TRY(scene_ready(s));
     if (n == 0) return WF_OK;
     wf_context* ctx = s->ctx;
-    with_stack(s->dev.stack_bound, [&](auto S) {
+    with_bvh(s->dev, [&](auto nodes, auto S) {
+        using N = std::remove_cv_t<std::remove_pointer_t<decltype(nodes)>>;
         if (any_hit)
-            k_trace_stats<decltype(S)::value, true>
+            k_trace_stats<N, decltype(S)::value, true>
                 <<<grid_for(n, kTraceBlock), kTraceBlock, 0, ctx->stream>>>(
-                    s->dev.nodes4, s->dev.tri, s->dev.n_tris, *rays, n, tnear, tfar,
-                    nodes_visited, triangles_tested);
+                    nodes, s->dev.tri, s->dev.n_tris, *rays, n, tnear, tfar, nodes_visited,
+                    triangles_tested);
         else
-            k_trace_stats<decltype(S)::value, false>
+            k_trace_stats<N, decltype(S)::value, false>
                 <<<grid_for(n, kTraceBlock), kTraceBlock, 0, ctx->stream>>>(
-                    s->dev.nodes4, s->dev.tri, s->dev.n_tris, *rays, n, tnear, tfar,
-                    nodes_visited, triangles_tested);
+                    nodes, s->dev.tri, s->dev.n_tris, *rays, n, tnear, tfar, nodes_visited,
+                    triangles_tested);
     });
     WF_HIP(hipGetLastError());
     WF_HIP(hipStreamSynchronize(ctx->stream));

@@ -1,6 +1,6 @@
 // Ray traversal shared by the closest-hit, occlusion and shadow kernels: the reference's
 // Moller-Trumbore test (include/geometry/Triangle.h:6-42) with the same float operations, a
-// conservative slab test, and a stack walk over the four-wide BVH. The first kStackLds stack
+// conservative slab test, and a stack walk over the four- or eight-wide BVH. The first kStackLds stack
 // entries live in LDS (stack[entry * kTraceBlock + lane], so a wave's pushes and pops hit 64
 // different banks); deeper entries, which few rays reach, spill to a per-lane private array.
 #pragma once
@@ -29,6 +29,16 @@ inline void with_stack(int bound, F&& f) {
     else if (bound <= 32) f(std::integral_constant<int, 32>{});
     else if (bound <= 48) f(std::integral_constant<int, 48>{});
     else f(std::integral_constant<int, 64>{});
+}
+
+// Calls f(nodes, std::integral_constant<int, S>) with the scene's node array (four- or
+// eight-wide) and the stack instantiation for its stack bound.
+template <class F>
+inline void with_bvh(const SceneDev& d, F&& f) {
+    with_stack(d.stack_bound, [&](auto S) {
+        if (d.width == 8) f((const Bvh8Node*)d.nodes8, S);
+        else f((const Bvh4Node*)d.nodes4, S);
+    });
 }
 
 __device__ __forceinline__ bool mt_test(const V3& o, const V3& d, const V3& v0, const V3& e1,
@@ -236,11 +246,134 @@ struct Walker {
         node = (X == 0 || sp < L) ? stack[sp * kTraceBlock + lane] : spill[sp - L];
         return false;
     }
+
+    // Tests up to four leaf children of an eight-wide node (child words cw, hit flags lh) in one
+    // round trip. Returns true when an ANY walk accepted a triangle.
+    __device__ __forceinline__ bool leaves4(const float4* __restrict__ tri, int4 cw, bool l0,
+                                            bool l1, bool l2, bool l3) {
+        if (!(l0 || l1 || l2 || l3)) return false;
+        const int cc[4] = {cw.x, cw.y, cw.z, cw.w};
+        const bool lh[4] = {l0, l1, l2, l3};
+        const int first = l0 ? ~cw.x : (l1 ? ~cw.y : (l2 ? ~cw.z : ~cw.w));
+        float4 A[4], B[4], C[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const size_t li = 3 * (size_t)(lh[k] ? ~cc[k] : first);
+            A[k] = tri[li];
+            B[k] = tri[li + 1];
+            C[k] = tri[li + 2];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            float t, u, v;
+            const bool ok = mt_all(o, d, xyz(A[k]), xyz(B[k]), xyz(C[k]), tnear, tfar, t, u, v);
+            if (STATS && lh[k]) ++h.tris;
+            if (!(lh[k] && ok)) continue;
+            const int32_t g = __float_as_int(A[k].w);
+            if (ANY) {
+                h.t = t;
+                h.leaf = ~cc[k];
+                h.g = g;
+                return true;
+            }
+            if (h.leaf < 0 || t < h.t || (t == h.t && g < h.g)) {
+                h.t = t;
+                h.leaf = ~cc[k];
+                h.g = g;
+                h.u = u;
+                h.v = v;
+            }
+        }
+        return false;
+    }
+
+    // Eight-wide quantised node: child boxes are decoded exactly as the build encoded them
+    // (origin + byte * power-of-two scale, so the product is exact and the decoded box contains
+    // the child), then walked like the four-wide node. The build stores leaf children first, so
+    // nodes with at most four leaves test their triangles in one round trip.
+    __device__ __forceinline__ bool step(const Bvh8Node* __restrict__ nodes,
+                                         const float4* __restrict__ tri,
+                                         int32_t* __restrict__ stack, volatile int32_t* spill) {
+        const int lane = threadIdx.x;
+        const float kInf = __int_as_float(0x7F800000);
+        if (STATS) ++h.nodes;
+        const uint4* np = reinterpret_cast<const uint4*>(nodes + node);
+        const uint4 org = np[0], cl = np[1], chh = np[2];
+        const uint4 qa = np[3], qb = np[4], qc = np[5];
+        const float ox = __uint_as_float(org.x), oy = __uint_as_float(org.y);
+        const float oz = __uint_as_float(org.z);
+        const float sx = __uint_as_float((org.w & 0xFFu) << 23);
+        const float sy = __uint_as_float(((org.w >> 8) & 0xFFu) << 23);
+        const float sz = __uint_as_float(((org.w >> 16) & 0xFFu) << 23);
+        int c[8] = {(int)cl.x, (int)cl.y, (int)cl.z, (int)cl.w,
+                    (int)chh.x, (int)chh.y, (int)chh.z, (int)chh.w};
+        const float tcur = ANY ? tfar : h.t;
+        float dist[8];
+        bool leaf_hit[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const bool hi4 = k >= 4;
+            const uint32_t b = 8u * (uint32_t)(k & 3);
+            const float lx = ox + (float)(((hi4 ? qa.y : qa.x) >> b) & 0xFFu) * sx;
+            const float ly = oy + (float)(((hi4 ? qa.w : qa.z) >> b) & 0xFFu) * sy;
+            const float lz = oz + (float)(((hi4 ? qb.y : qb.x) >> b) & 0xFFu) * sz;
+            const float hx = ox + (float)(((hi4 ? qb.w : qb.z) >> b) & 0xFFu) * sx;
+            const float hy = oy + (float)(((hi4 ? qc.y : qc.x) >> b) & 0xFFu) * sy;
+            const float hz = oz + (float)(((hi4 ? qc.w : qc.z) >> b) & 0xFFu) * sz;
+            float t0;
+            const bool box = slab(lx, ly, lz, hx, hy, hz, o, inv, tcur, t0) && c[k] != kEmptyChild;
+            leaf_hit[k] = box && c[k] < 0;
+            dist[k] = (box && c[k] >= 0) ? t0 : kInf;
+        }
+        if (leaves4(tri, make_int4(c[0], c[1], c[2], c[3]), leaf_hit[0], leaf_hit[1], leaf_hit[2],
+                    leaf_hit[3]))
+            return true;
+        if (leaves4(tri, make_int4(c[4], c[5], c[6], c[7]), leaf_hit[4], leaf_hit[5], leaf_hit[6],
+                    leaf_hit[7]))
+            return true;
+        auto cswap = [&](int x, int y) {
+            if (dist[y] < dist[x]) {
+                const float td = dist[x];
+                dist[x] = dist[y];
+                dist[y] = td;
+                const int tc = c[x];
+                c[x] = c[y];
+                c[y] = tc;
+            }
+        };
+        if (!ANY) {
+            // 19-comparator sorting network for eight keys
+            cswap(0, 2); cswap(1, 3); cswap(4, 6); cswap(5, 7);
+            cswap(0, 4); cswap(1, 5); cswap(2, 6); cswap(3, 7);
+            cswap(0, 1); cswap(2, 3); cswap(4, 5); cswap(6, 7);
+            cswap(2, 4); cswap(3, 5);
+            cswap(1, 4); cswap(3, 6);
+            cswap(1, 2); cswap(3, 4); cswap(5, 6);
+        } else {
+#pragma unroll
+            for (int k = 1; k < 8; ++k) cswap(0, k);
+        }
+#pragma unroll
+        for (int k = 7; k >= 1; --k)
+            if (dist[k] < kInf) {
+                if (X == 0 || sp < L) stack[sp * kTraceBlock + lane] = c[k];
+                else spill[sp - L] = c[k];
+                ++sp;
+            }
+        if (dist[0] < kInf) {
+            node = c[0];
+            return false;
+        }
+        if (sp == 0) return true;
+        --sp;
+        node = (X == 0 || sp < L) ? stack[sp * kTraceBlock + lane] : spill[sp - L];
+        return false;
+    }
 };
 
 // Whole walk of one ray (callers without dynamic ray refill).
-template <bool ANY, int STACK, bool STATS = false>
-__device__ __forceinline__ bool traverse(const Bvh4Node* __restrict__ nodes,
+template <bool ANY, int STACK, bool STATS = false, class NODE>
+__device__ __forceinline__ bool traverse(const NODE* __restrict__ nodes,
                                          const float4* __restrict__ tri, int n_tris,
                                          const V3& o, const V3& d, float tnear, float tfar,
                                          int32_t* __restrict__ stack, TraceHit& h) {
