@@ -200,6 +200,39 @@ def host_cpu() -> str:
     return "unknown"
 
 
+def autotune_batched(wf, ctx, scs, datas, physics, jobs, args) -> list:
+    """Engine capacity and sample lanes per scene, and the context's traversal block shape,
+    refill threshold and radix digit width, picked by the autotuner on a calibration subset of
+    each scene's jobs (budgets capped at 64 spp). The context knobs are tuned on the scene with
+    the most samples and kept for both engines."""
+    autotune = importlib.import_module("wavefront-renderer_amd.autotune")
+    forms = importlib.import_module("wavefront-renderer_amd.forms")
+    from dataclasses import replace
+    work = [sum(j.pixels * j.spp for j in jobs if j.scene == k) for k in range(2)]
+    order = sorted(range(2), key=lambda k: -work[k])
+    out = [None, None]
+    t0 = time.perf_counter()
+    for n, k in enumerate(order):
+        mine = [j for j in jobs if j.scene == k]
+        if not mine:
+            out[k] = {"capacity": 1 << 20, "sample_lanes": 0}
+            continue
+        calib = [replace(j, spp=min(j.spp, 64)) for j in mine[::max(1, len(mine) // 6)]][:6]
+        budget = sum(j.pixels for j in calib)
+        space = dict(autotune.ENGINE_SPACE)
+        if n > 0:
+            for knob in autotune.CONTEXT_KNOBS:
+                space.pop(knob)
+        res = autotune.tune_engine(
+            ctx, scs[k], lambda: [forms._desc(j, datas[k].camera) for j in calib], budget,
+            len(calib), {"physics": physics[k], "firefly": 1e30}, space=space, rounds=1)
+        out[k] = {"capacity": res["best"]["capacity"],
+                  "sample_lanes": res["best"]["sample_lanes"]}
+        log(f"[bench] autotune scene {k}: {res['best']} ({res['trials']} trials)")
+    log(f"[bench] autotune: {time.perf_counter() - t0:.1f}s")
+    return out
+
+
 def run_batched(wf, ctx, scenes, args) -> dict:
     """Config 3: 64 jobs from the Cornell-box and sorting-stress generators, sequential wf_render
     against one job-tagged engine per scene (all jobs admitted at once)."""
@@ -217,9 +250,12 @@ def run_batched(wf, ctx, scenes, args) -> dict:
     t_seq = sum(secs)
     log(f"[bench] batched: sequential {t_seq:.2f}s for {len(jobs)} jobs")
     pix = [max(1, sum(j.pixels for j in jobs if j.scene == k)) for k in range(2)]
-    engines = [wf.Engine(ctx, scs[k], capacity=min(pix[k], 4 << 20), pixel_budget=pix[k],
+    tuned = autotune_batched(wf, ctx, scs, datas, physics, jobs, args) if args.autotune_batch \
+        else [{"capacity": min(pix[k], 4 << 20), "sample_lanes": 0} for k in range(2)]
+    engines = [wf.Engine(ctx, scs[k], capacity=tuned[k]["capacity"], pixel_budget=pix[k],
                          max_jobs=max(1, len(jobs)), physics=physics[k], firefly=1e30,
-                         use_graph=True) for k in range(2)]
+                         use_graph=True, sample_lanes=tuned[k]["sample_lanes"])
+               for k in range(2)]
     res = forms.serve(engines, datas, jobs, realtime=False)
     done = [j.done for j in jobs]
     for e in engines:
@@ -234,7 +270,8 @@ def run_batched(wf, ctx, scenes, args) -> dict:
            "sequential_mrays_s": round(rays_seq / t_seq / 1e6, 1),
            "batched_mrays_s": round(res["rays"] / res["makespan_s"] / 1e6, 1),
            "batched_p50_s": round(forms.pct(done, 50), 3),
-           "batched_p99_s": round(forms.pct(done, 99), 3)}
+           "batched_p99_s": round(forms.pct(done, 99), 3),
+           "engines": tuned, "tuning": ctx.tuning()}
     log(f"[bench] batched: {json.dumps(out)}")
     return out
 
@@ -285,9 +322,11 @@ def main() -> None:
     ap.add_argument("--schedule", default="regenerate", choices=["regenerate", "passes"],
                     help="regenerate: full wavefront, pixels start their next sample as soon as "
                          "their path ends; passes: the reference's pass-by-pass loop (same image)")
-    ap.add_argument("--lanes", type=int, default=8, choices=[1, 2, 4, 8, 16],
+    ap.add_argument("--lanes", type=int, default=32, choices=[1, 2, 4, 8, 16, 32, 64],
                     help="regenerate: samples of a pixel in flight at once (1 = the pass loop's "
                          "accumulation order bit for bit)")
+    ap.add_argument("--wavefront", type=int, default=0,
+                    help="regenerate: ray slots of the wavefront (0 = automatic)")
     ap.add_argument("--quick", action="store_true", help="reduced sizes (CPU-side checks)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-policies", action="store_true")
@@ -302,6 +341,11 @@ def main() -> None:
                     help="per-pixel sample budget of served requests")
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the batched (config 3) and served (config 4) legs")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick the plain frame's wavefront size, sample lanes, traversal block "
+                         "shape and radix digit width with the autotuner before timing")
+    ap.add_argument("--no-autotune-batch", dest="autotune_batch", action="store_false",
+                    help="batched leg: fixed engine settings instead of the autotuner's")
     ap.add_argument("--profile-run", action="store_true",
                     help="only the timed frames (no comparison legs): for profiler runs")
     args = ap.parse_args()
@@ -347,7 +391,22 @@ def main() -> None:
         return wf.render_params(width=w, height=h, spp=s, max_depth=depth,
                                 camera=data.camera, policy=policy, physics=physics,
                                 firefly=1e30, seed=seed, schedule=schedule,
-                                sample_lanes=args.lanes)
+                                sample_lanes=args.lanes, wavefront=args.wavefront)
+
+    tuned = None
+    if args.autotune:
+        autotune = importlib.import_module("wavefront-renderer_amd.autotune")
+        t0 = time.perf_counter()
+
+        def params_for(lanes, wave):
+            return wf.render_params(width=W, height=H, spp=spp, max_depth=depth,
+                                    camera=data.camera, policy=args.policy, physics=physics,
+                                    firefly=1e30, seed=7, sample_lanes=lanes, wavefront=wave)
+        tuned = autotune.tune_render(ctx, scene, params_for, W * H, rounds=1, repeats=1)
+        args.lanes = tuned["best"]["sample_lanes"]
+        args.wavefront = tuned["best"]["wavefront_per_pixel"] * W * H
+        log(f"[bench] autotune ({time.perf_counter() - t0:.1f}s, {tuned['trials']} trials): "
+            f"{tuned['best']}")
 
     def frame(policy, seed=1, schedule=args.schedule):
         img, cnt, st = wf.render(scene, params(policy, seed, schedule=schedule),
@@ -497,7 +556,9 @@ def main() -> None:
         "dtype": "f32",
         "data": "synthetic (seeded generator)",
         "config": {"workload": desc, "policy": args.policy, "schedule": args.schedule,
-                   "sample_lanes": args.lanes, "bvh_width": scene.stats.bvh_width,
+                   "sample_lanes": args.lanes, "wavefront": args.wavefront or "auto",
+                   "tuning": ctx.tuning(), "autotuned": tuned is not None,
+                   "bvh_width": scene.stats.bvh_width,
                    "other_schedule": other_sched, "physics":
                    "unbiased" if physics else "reference", "triangles": data.triangle_count,
                    "bvh_build_ms": round(scene.stats.build_ms, 1), "policies": policies},

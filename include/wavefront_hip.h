@@ -320,7 +320,9 @@ typedef struct wf_render_params {
     int32_t adaptive_interval;      /* reference: 16 */
     int32_t schedule;          /* WF_SCHEDULE_*: same image, different order of work */
     int32_t sample_lanes;      /* REGENERATE, non-adaptive: samples of a pixel in flight at once
-                                  (1, 2, 4, 8 or 16; 0 = 8); see wf_engine_config::sample_lanes */
+                                  (a power of two <= 64; 0 = 32); see
+                                  wf_engine_config::sample_lanes */
+    int32_t wavefront;         /* REGENERATE: ray slots (paths in flight); 0 = automatic */
     wf_shade_params shade;
     wf_camera camera;
 } wf_render_params;
@@ -374,7 +376,8 @@ typedef struct wf_engine_config {
     int32_t policy;            /* WF_POLICY_* shading-queue order */
     int32_t ray_sort;          /* RayQueue Morton sort of every iteration's rays */
     int32_t use_graph;         /* replay each iteration from a captured HIP graph */
-    int32_t sample_lanes;      /* samples of a pixel in flight at once (1, 2, 4, 8, 16; 0 = 8).
+    int32_t sample_lanes;      /* samples of a pixel in flight at once (a power of two <= 64;
+                                  0 = 32).
                                   Lane l of a non-adaptive job runs samples l, l+L, l+2L, ... into
                                   an accumulator of its own; the lanes are summed in lane order
                                   when the job resolves, so an image depends on L but not on the
@@ -419,7 +422,7 @@ typedef struct wf_engine_stats {
     double phase_ms[5];        /* sort, intersect, schedule, shade, miss (with phase timing) */
 } wf_engine_stats;
 
-/* Device bytes an engine of this capacity, pixel budget and sample lanes (0 = 8) needs
+/* Device bytes an engine of this capacity, pixel budget and sample lanes (0 = 32) needs
  * (admission planning). */
 wf_status wf_engine_bytes(int32_t capacity, int64_t pixel_budget, int32_t sample_lanes,
                           size_t* out);
@@ -462,7 +465,8 @@ typedef struct wf_tuning {
     int32_t bvh_width;           /* node width of the next wf_accel_build: 4 (default), float
                                   * boxes; 8, boxes quantised outward to 8 bits per node grid
                                   * (fewer, smaller nodes; more box tests per node) */
-    int32_t reserved[3];
+    int32_t radix_bits;          /* digit width of the radix sort passes: 6, 8 (default) or 11 */
+    int32_t reserved[2];
 } wf_tuning;
 /* Engines created on the context take its tuning at every run. */
 wf_status wf_context_set_tuning(wf_context* ctx, const wf_tuning* tuning);

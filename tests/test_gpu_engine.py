@@ -67,6 +67,21 @@ def test_regenerate_equals_passes_textured_unbiased(wf, stress, policy):
     assert_same(a, b)
 
 
+@pytest.mark.parametrize("radix_bits", [6, 11])
+def test_regenerate_equals_passes_radix_widths(wf, ctx, cornell, radix_bits):
+    """Device-count sorts (ray keys and shading keys inside the engine) with other digit widths:
+    the same permutations, so the same image bit for bit."""
+    data, scene = cornell
+    saved = ctx.tuning()
+    ctx.set_tuning(radix_bits=radix_bits)
+    try:
+        a, b = both_schedules(wf, scene, data, width=48, height=40, spp=4, max_depth=8, seed=3,
+                              firefly=1e30, policy="costBenefit")
+    finally:
+        ctx.set_tuning(**saved)
+    assert_same(a, b)
+
+
 @pytest.mark.parametrize("lanes", [2, 4, 8, 16])
 def test_regenerate_lanes_match_passes(wf, cornell, stress, lanes):
     """Several samples of a pixel in flight: same paths, counts and statistics as the pass loop;
@@ -199,7 +214,7 @@ def test_two_engines_overlapped_equal_alone(wf, ctx, cornell, stress, rng):
     launch/wait (their iterations overlap on the device); every job equals its lone render."""
     scenes_ = {"cornell": cornell, "stress": stress}
     engines = {k: wf.Engine(ctx, sc, capacity=3000, pixel_budget=12000, max_jobs=4)
-               for k, (_, sc) in scenes_.items()}   # default lanes (8)
+               for k, (_, sc) in scenes_.items()}   # default lanes (32)
     jobs = {k: jobs_for(wf, scenes_[k][0], rng, 3) for k in scenes_}
     ids = {k: {engines[k].admit(wf.job_desc(**j)): n for n, j in enumerate(jobs[k])} for k in jobs}
     got = {k: {} for k in jobs}
@@ -212,8 +227,45 @@ def test_two_engines_overlapped_equal_alone(wf, ctx, cornell, stress, rng):
     for k in jobs:
         _, scene = scenes_[k]
         for n, j in enumerate(jobs[k]):
-            img, cnt, _ = alone(wf, scene, j, lanes=8)
+            img, cnt, _ = alone(wf, scene, j, lanes=32)
             assert np.array_equal(got[k][n][1], cnt), (k, n)
             assert np.array_equal(got[k][n][0], img), (k, n)
     for e in engines.values():
         e.close()
+
+
+def test_autotune_render_and_engine(wf, ctx, cornell):
+    """The autotuner measures real renders and engine runs, returns settings from its space and
+    leaves the context tuned; tuned renders trace exactly the rays of the defaults."""
+    autotune = __import__("importlib").import_module("wavefront-renderer_amd.autotune")
+    data, scene = cornell
+    W, H = 64, 48
+    saved = ctx.tuning()
+
+    def params_for(lanes, wave):
+        return wf.render_params(camera=data.camera, width=W, height=H, spp=8, max_depth=6,
+                                seed=2, firefly=1e30, sample_lanes=lanes, wavefront=wave)
+
+    space = {"wavefront_per_pixel": [1, 2], "sample_lanes": [4, 8],
+             "trace_blocks_per_cu": [6, 10], "radix_bits": [8, 11]}
+    try:
+        res = autotune.tune_render(ctx, scene, params_for, W * H, space=space, rounds=1,
+                                   repeats=1)
+        best = res["best"]
+        for k, vals in space.items():
+            assert best[k] in vals
+        assert ctx.tuning()["radix_bits"] == best["radix_bits"]
+        _, _, st_t = wf.render(scene, params_for(best["sample_lanes"],
+                                                 best["wavefront_per_pixel"] * W * H))
+        ctx.set_tuning(**saved)
+        _, _, st_d = wf.render(scene, params_for(0, 0))
+        assert (st_t.rays_traced, st_t.shadow_rays) == (st_d.rays_traced, st_d.shadow_rays)
+
+        def jobs():
+            return [wf.job_desc(32, 24, 4, 5, data.camera, seed=s) for s in range(3)]
+        eres = autotune.tune_engine(ctx, scene, jobs, pixel_budget=4000, max_jobs=4,
+                                    engine_kw={}, rounds=1,
+                                    space={"capacity": [1024, 4096], "sample_lanes": [1, 4]})
+        assert eres["best"]["capacity"] in (1024, 4096) and eres["seconds"] > 0
+    finally:
+        ctx.set_tuning(**saved)

@@ -46,8 +46,17 @@ def camera_rays(orc, data, w, h, sample=0, seed=7):
     return orc.raygen(data.camera, w, h, np.arange(w * h), seed, 0, sample)
 
 
+@pytest.fixture(params=[8, 6, 11], ids=lambda b: f"radix{b}")
+def radix_bits(request, ctx):
+    """Runs a test with each digit width of the radix sort passes."""
+    saved = ctx.tuning()
+    ctx.set_tuning(radix_bits=request.param)
+    yield request.param
+    ctx.set_tuning(**saved)
+
+
 # ---- RayQueue::schedule / materialize ---------------------------------------------------------
-def test_ray_schedule_bit_exact(wf, ctx, orc, rng):
+def test_ray_schedule_bit_exact(wf, ctx, orc, rng, radix_bits):
     for n in [1, 7, 2048, 2049, 100_003]:
         rays = random_rays(rng, n, box=12.0, center=(0, 0, 0))
         rays["ox"] = np.round(rays["ox"], 0).astype(f32)          # ties in the key
@@ -289,7 +298,7 @@ def policy_scene(wf, ctx, scenes):
 
 @pytest.mark.parametrize("policy", ["none", "material", "texture", "costBenefit"])
 @pytest.mark.parametrize("n", [1, 5, 3000, 70_001])
-def test_shading_schedule_bit_exact(wf, ctx, orc, policy_scene, policy, n, rng):
+def test_shading_schedule_bit_exact(wf, ctx, orc, policy_scene, policy, n, rng, radix_bits):
     data, scene, types, mtex, tmem = policy_scene
     q = stress_queue(rng, n, len(types))
     q["tex"] = np.asarray(mtex, np.int32)[q["mat"]]

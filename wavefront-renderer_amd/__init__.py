@@ -82,13 +82,13 @@ class Context:
         check(self._lib.wf_context_get_tuning(self.handle, C.byref(t)))
         return {"refill_lanes": t.refill_lanes, "trace_blocks_per_cu": t.trace_blocks_per_cu,
                 "bvh_builder": t.bvh_builder, "ploc_radius": t.ploc_radius,
-                "bvh_width": t.bvh_width}
+                "bvh_width": t.bvh_width, "radix_bits": t.radix_bits}
 
     def set_tuning(self, **kw):
         cur = self.tuning()
         cur.update(kw)
         t = _abi.Tuning(cur["refill_lanes"], cur["trace_blocks_per_cu"], cur["bvh_builder"],
-                        cur["ploc_radius"], cur["bvh_width"])
+                        cur["ploc_radius"], cur["bvh_width"], cur["radix_bits"])
         check(self._lib.wf_context_set_tuning(self.handle, C.byref(t)))
 
     def stream_copy(self, dst: int, src: int, nbytes: int):
@@ -358,15 +358,17 @@ def render_params(width, height, spp, max_depth, camera, policy="none", ray_sort
                   adaptive=False, adaptive_threshold=0.05, adaptive_min_samples=32,
                   adaptive_interval=16, physics=PHYSICS_REFERENCE, rr_min_depth=3,
                   cost_aware_rr=False, firefly=10.0, seed=0, job=0,
-                  schedule="regenerate", sample_lanes=0) -> _abi.RenderParams:
-    """sample_lanes (regenerating schedule): samples of a pixel in flight at once, 1/2/4/8/16 or
-    0 for the default (8); 1 reproduces the pass loop's accumulation order bit for bit."""
+                  schedule="regenerate", sample_lanes=0, wavefront=0) -> _abi.RenderParams:
+    """Regenerating schedule: sample_lanes = samples of a pixel in flight at once (a power of
+    two <= 64, 0 for the default 32; 1 reproduces the pass loop's accumulation order bit for
+    bit); wavefront = ray slots (0 = automatic)."""
     pol = POLICIES[policy] if isinstance(policy, str) else int(policy)
     sched = SCHEDULES[schedule] if isinstance(schedule, str) else int(schedule)
     sp = shade_params(physics, rr_min_depth, cost_aware_rr, firefly, seed, 0, job)
     return _abi.RenderParams(width, height, spp, max_depth, pol, 1 if ray_sort else 0,
                              1 if adaptive else 0, adaptive_threshold, adaptive_min_samples,
-                             adaptive_interval, sched, sample_lanes, sp, camera_struct(camera))
+                             adaptive_interval, sched, sample_lanes, wavefront, sp,
+                             camera_struct(camera))
 
 
 def render(scene: Scene, params: _abi.RenderParams, keep_on_device: bool = False):

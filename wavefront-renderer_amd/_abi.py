@@ -115,8 +115,8 @@ class RenderParams(C.Structure):
                 ("max_depth", C.c_int32), ("policy", C.c_int32), ("ray_sort", C.c_int32),
                 ("adaptive", C.c_int32), ("adaptive_threshold", C.c_float),
                 ("adaptive_min_samples", C.c_int32), ("adaptive_interval", C.c_int32),
-                ("schedule", C.c_int32), ("sample_lanes", C.c_int32), ("shade", ShadeParams),
-                ("camera", Camera)]
+                ("schedule", C.c_int32), ("sample_lanes", C.c_int32), ("wavefront", C.c_int32),
+                ("shade", ShadeParams), ("camera", Camera)]
 
 
 class RenderStats(C.Structure):
@@ -162,7 +162,8 @@ BVH_PLOC, BVH_LBVH = 0, 1
 class Tuning(C.Structure):
     _fields_ = [("refill_lanes", C.c_int32), ("trace_blocks_per_cu", C.c_int32),
                 ("bvh_builder", C.c_int32), ("ploc_radius", C.c_int32),
-                ("bvh_width", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("bvh_width", C.c_int32), ("radix_bits", C.c_int32),
+                ("reserved", C.c_int32 * 2)]
 
 
 _S = C.c_int32

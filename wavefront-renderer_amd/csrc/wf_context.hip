@@ -266,6 +266,8 @@ wf_status wf_context_set_tuning(wf_context* ctx, const wf_tuning* t) {
                "wf_context_set_tuning: ploc_radius %d not in [1, 32]", t->ploc_radius);
     WF_REQUIRE(t->bvh_width == 4 || t->bvh_width == 8,
                "wf_context_set_tuning: bvh_width %d is not 4 or 8", t->bvh_width);
+    WF_REQUIRE(t->radix_bits == 6 || t->radix_bits == 8 || t->radix_bits == 11,
+               "wf_context_set_tuning: radix_bits %d is not 6, 8 or 11", t->radix_bits);
     ctx->tuning = *t;
     return WF_OK;
 }

@@ -423,12 +423,14 @@ constexpr size_t kBytesPerSlot = 48 * 3 + 104 * 2 + 4 * 5 + 400;   // queues + t
 constexpr size_t kBytesPerPixel = 12 + 12 + 4 + 4 + 4 + 4 + 8;   // accum, start copy, mean, M2,
                                                                   // count, lanes done, queue
 constexpr size_t kBytesPerLane = 12 + 8;                          // accum, queue
-constexpr int kDefaultLanes = 8;
+constexpr int kDefaultLanes = 32;
+constexpr int kAutoPathsPerPixel = 4;
+constexpr int64_t kAutoWavefront = 1 << 23;
+constexpr int kMaxLanes = 64;
 
 int lanes_of(int32_t sample_lanes) { return sample_lanes == 0 ? kDefaultLanes : sample_lanes; }
 bool lanes_ok(int32_t sample_lanes) {
-    return sample_lanes == 0 || sample_lanes == 1 || sample_lanes == 2 || sample_lanes == 4 ||
-           sample_lanes == 8 || sample_lanes == 16;
+    return sample_lanes >= 0 && sample_lanes <= kMaxLanes && (sample_lanes & (sample_lanes - 1)) == 0;
 }
 
 int g_phase_set = 0;   // event set of the iteration being enqueued
@@ -598,7 +600,7 @@ wf_status wf_engine_create(wf_context* ctx, wf_scene* scene, const wf_engine_con
                "wf_engine_create: pixel budget %lld out of range", (long long)cfg->pixel_budget);
     WF_REQUIRE(cfg->max_jobs > 0 && cfg->max_jobs <= kMaxJobs, "wf_engine_create: max_jobs %d not in [1, %d]",
                cfg->max_jobs, kMaxJobs);
-    WF_REQUIRE(lanes_ok(cfg->sample_lanes), "wf_engine_create: sample_lanes %d not in {0, 1, 2, 4, 8, 16}",
+    WF_REQUIRE(lanes_ok(cfg->sample_lanes), "wf_engine_create: sample_lanes %d is not 0 or a power of two <= 64",
                cfg->sample_lanes);
     WF_REQUIRE(cfg->pixel_budget * lanes_of(cfg->sample_lanes) < (1ll << 31),
                "wf_engine_create: pixel budget x sample lanes must stay below 2^31");
@@ -699,7 +701,7 @@ wf_status wf_engine_destroy(wf_engine* e) {
 wf_status wf_engine_bytes(int32_t capacity, int64_t pixel_budget, int32_t sample_lanes,
                           size_t* out) {
     WF_REQUIRE(out && capacity >= 0 && pixel_budget >= 0, "wf_engine_bytes: bad argument");
-    WF_REQUIRE(lanes_ok(sample_lanes), "wf_engine_bytes: sample_lanes %d not in {0, 1, 2, 4, 8, 16}",
+    WF_REQUIRE(lanes_ok(sample_lanes), "wf_engine_bytes: sample_lanes %d is not 0 or a power of two <= 64",
                sample_lanes);
     *out = kBytesPerSlot * (size_t)capacity +
            (kBytesPerPixel + kBytesPerLane * (size_t)(lanes_of(sample_lanes) - 1)) *
@@ -975,15 +977,24 @@ wf_status render_regenerate(wf_scene* s, const wf_render_params* rp, float* imag
                             int32_t* sample_counts, wf_render_stats* stats) {
     const int P = rp->width * rp->height;
     wf_engine* e = s->render_engine;
-    WF_REQUIRE(lanes_ok(rp->sample_lanes), "wf_render: sample_lanes %d not in {0, 1, 2, 4, 8, 16}",
+    WF_REQUIRE(lanes_ok(rp->sample_lanes), "wf_render: sample_lanes %d is not 0 or a power of two <= 64",
                rp->sample_lanes);
+    WF_REQUIRE(rp->wavefront >= 0 && rp->wavefront < (1 << 29), "wf_render: wavefront %d out of range",
+               rp->wavefront);
     const int lanes = rp->adaptive ? 1 : lanes_of(rp->sample_lanes);
-    if (e && (e->cap < P || e->px < P || e->lanes < lanes)) {
+    // automatic wavefront: up to kAutoPathsPerPixel paths per pixel, at most kAutoWavefront rays
+    // (larger launches amortise the per-iteration fixed costs and the traversal tail)
+    const int slots =
+        rp->wavefront > 0
+            ? rp->wavefront
+            : (int)std::min<int64_t>((int64_t)P * std::min(lanes, kAutoPathsPerPixel),
+                                     std::max<int64_t>(P, kAutoWavefront));
+    if (e && (e->cap != slots || e->px < P || e->lanes < lanes)) {
         wf_engine_destroy(e);
         e = s->render_engine = nullptr;
     }
     wf_engine_config cfg{};
-    cfg.capacity = P;
+    cfg.capacity = slots;
     cfg.max_jobs = 1;
     cfg.pixel_budget = P;
     cfg.policy = rp->policy;

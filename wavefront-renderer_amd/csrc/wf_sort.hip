@@ -2,12 +2,14 @@
 // prefix sum over u32 counts. Both are deterministic: ranks inside a tile are computed with
 // wave-level ballots in tile order, never with atomics whose order could vary.
 //
-// Radix sort (8-bit digits, 2048-key tiles, 256 threads = 4 waves of 64): one histogram sweep
-// gives every pass's digit totals; each pass is then a single kernel (k_sweep_pass) in which
-// each wave ranks its 512 keys round by round: lanes holding the same digit are matched with 8
-// ballots, the rank is the popcount of matching lower lanes plus the wave's running count for
-// that digit; waves are offset by an exclusive prefix over waves (so equal digits keep input
-// order), and tiles by the published counts of all earlier tiles (decoupled look-back).
+// Radix sort (6-, 8- or 11-bit digits by tuning, 2048-key tiles, 256 threads = 4 waves of 64):
+// one histogram sweep gives every pass's digit totals; each pass is then a single kernel
+// (k_sweep_pass) in which each wave ranks its 512 keys round by round: lanes holding the same
+// digit are matched with one ballot per digit bit, the rank is the popcount of matching lower
+// lanes plus the wave's running count for that digit; waves are offset by an exclusive prefix
+// over waves (so equal digits keep input order), and tiles by the published counts of all earlier tiles (decoupled look-back). The tile
+// is staged in LDS in output order and written out with consecutive threads on consecutive
+// output slots.
 #include <algorithm>
 
 #include "wf_device.h"
@@ -31,48 +33,78 @@ constexpr int kSortWaves = kSortBlock / 64;
 constexpr uint32_t kStateCount = 1u << 30;    // flag: tile count published
 constexpr uint32_t kStatePrefix = 2u << 30;   // flag: inclusive prefix published
 constexpr uint32_t kStateMask = (1u << 30) - 1u;
-constexpr int kMaxPasses = 8;
+constexpr int kMaxPasses = 11;
 constexpr int kSpinLimit = 1 << 22;
 
+// Digit width of the one-sweep passes: BITS in {6, 8, 11} (R = 2^BITS digits). A thread owns
+// DPT consecutive digits in the per-digit steps.
+template <int BITS>
+struct Digits {
+    static constexpr int R = 1 << BITS;
+    static constexpr int DPT = R > kSortBlock ? R / kSortBlock : 1;
+    static constexpr int MaxPasses = (64 + BITS - 1) / BITS;
+};
+
+template <int BITS>
 __global__ __launch_bounds__(kSortBlock) void k_sweep_hist(const uint64_t* __restrict__ keys,
                                                            int n_host,
                                                            const int32_t* __restrict__ nd,
                                                            int lo_bit, int hi_bit, int passes,
                                                            uint32_t* __restrict__ hist) {
+    constexpr int R = Digits<BITS>::R;
     const int n = load_count(n_host, nd);
-    __shared__ uint32_t cnt[kMaxPasses][256];
-    for (int p = 0; p < passes; ++p) cnt[p][threadIdx.x] = 0;
+    __shared__ uint32_t cnt[Digits<BITS>::MaxPasses * R];
+    for (int i = threadIdx.x; i < passes * R; i += kSortBlock) cnt[i] = 0;
     __syncthreads();
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint64_t k = keys[i];
         for (int p = 0; p < passes; ++p) {
-            // the same digit the pass ranks by: bits [shift, min(shift + 8, hi_bit + 1))
-            const int shift = lo_bit + 8 * p;
-            const int nbits = (hi_bit + 1 - shift) < 8 ? (hi_bit + 1 - shift) : 8;
-            atomicAdd(&cnt[p][(uint32_t)(k >> shift) & ((1u << nbits) - 1u)], 1u);
+            // the same digit the pass ranks by: bits [shift, min(shift + BITS, hi_bit + 1))
+            const int shift = lo_bit + BITS * p;
+            const int nbits = (hi_bit + 1 - shift) < BITS ? (hi_bit + 1 - shift) : BITS;
+            atomicAdd(&cnt[p * R + ((uint32_t)(k >> shift) & ((1u << nbits) - 1u))], 1u);
         }
     }
     __syncthreads();
-    for (int p = 0; p < passes; ++p)
-        if (cnt[p][threadIdx.x]) atomicAdd(&hist[p * 256 + threadIdx.x], cnt[p][threadIdx.x]);
+    for (int i = threadIdx.x; i < passes * R; i += kSortBlock)
+        if (cnt[i]) atomicAdd(&hist[i], cnt[i]);
 }
 
-// hist[p][d] -> exclusive prefix over digits (the pass's output base of digit d).
-__global__ __launch_bounds__(256) void k_sweep_bases(uint32_t* hist) {
-    __shared__ uint32_t lds[4];
-    uint32_t* h = hist + blockIdx.x * 256;
-    const uint32_t x = h[threadIdx.x];
+// Exclusive prefix over the 256 threads of a block of one value per thread (wave shuffles + LDS).
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t x, uint32_t* part) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t v = x;
     for (int o = 1; o < 64; o <<= 1) {
         const uint32_t y = __shfl_up(v, o, 64);
         if (lane >= o) v += y;
     }
-    if (lane == 63) lds[wave] = v;
+    if (lane == 63) part[wave] = v;
     __syncthreads();
     uint32_t off = 0;
-    for (int w = 0; w < wave; ++w) off += lds[w];
-    h[threadIdx.x] = off + v - x;
+    for (int w = 0; w < wave; ++w) off += part[w];
+    return off + v - x;
+}
+
+// hist[p][d] -> exclusive prefix over digits (the pass's output base of digit d).
+template <int BITS>
+__global__ __launch_bounds__(kSortBlock) void k_sweep_bases(uint32_t* hist) {
+    constexpr int R = Digits<BITS>::R, DPT = Digits<BITS>::DPT;
+    __shared__ uint32_t part[kSortWaves];
+    uint32_t* h = hist + blockIdx.x * R;
+    const int base = threadIdx.x * DPT;
+    uint32_t x[DPT], sum = 0;
+#pragma unroll
+    for (int k = 0; k < DPT; ++k) {
+        x[k] = base + k < R ? h[base + k] : 0u;
+        sum += x[k];
+    }
+    uint32_t run = block_exclusive_scan(sum, part);
+#pragma unroll
+    for (int k = 0; k < DPT; ++k)
+        if (base + k < R) {
+            h[base + k] = run;
+            run += x[k];
+        }
 }
 
 __device__ __forceinline__ uint32_t state_load(const uint32_t* p) {
@@ -82,14 +114,22 @@ __device__ __forceinline__ void state_store(uint32_t* p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+template <int BITS>
 __global__ __launch_bounds__(kSortBlock) void k_sweep_pass(
     const uint64_t* __restrict__ keys_in, const int32_t* __restrict__ vals_in,
     uint64_t* __restrict__ keys_out, int32_t* __restrict__ vals_out, int n_host,
     const int32_t* __restrict__ nd, int shift, uint32_t mask, int nbits,
     const uint32_t* __restrict__ bases, uint32_t* state, int32_t* tile_counter,
     int32_t* __restrict__ fault) {
-    __shared__ uint32_t wave_cnt[kSortWaves][256];
-    __shared__ uint32_t digit_base[256];
+    constexpr int R = Digits<BITS>::R, DPT = Digits<BITS>::DPT;
+    // the per-wave digit counts and, once every slot is known, the staged tile share one buffer
+    constexpr int kCntBytes = kSortWaves * R * 4, kStageBytes = kSortTile * 12;
+    __shared__ __align__(16) unsigned char lds[kCntBytes > kStageBytes ? kCntBytes : kStageBytes];
+    uint32_t(*wave_cnt)[R] = reinterpret_cast<uint32_t(*)[R]>(lds);
+    uint64_t* stage_k = reinterpret_cast<uint64_t*>(lds);
+    int32_t* stage_v = reinterpret_cast<int32_t*>(lds + kSortTile * 8);
+    __shared__ uint32_t digit_base[R];   // global position of tile slot 0 of each digit's run
+    __shared__ uint32_t scan_part[kSortWaves];
     __shared__ int tile_s;
     const int n = load_count(n_host, nd);
     if (threadIdx.x == 0) tile_s = atomicAdd(tile_counter, 1);
@@ -98,8 +138,7 @@ __global__ __launch_bounds__(kSortBlock) void k_sweep_pass(
     if (tile * kSortTile >= n) return;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int w = 0; w < kSortWaves; ++w) wave_cnt[w][threadIdx.x] = 0;
+    for (int i = threadIdx.x; i < kSortWaves * R; i += kSortBlock) (&wave_cnt[0][0])[i] = 0;
     __syncthreads();
 
     const int chunk = tile * kSortTile + wave * 64 * kSortItems;
@@ -128,51 +167,87 @@ __global__ __launch_bounds__(kSortBlock) void k_sweep_pass(
         if (valid && rank_in == 0) wave_cnt[wave][dig] = before + (uint32_t)__popcll(peers);
     }
     __syncthreads();
-    // per digit (thread = digit): wave offsets, tile count, publish, look back
-    uint32_t tile_count = 0;
-    {
+    // per digit (thread t owns digits t*DPT ..): wave offsets, tile counts, the tile's digit-run
+    // starts, publish, look back
+    const int dbase = threadIdx.x * DPT;
+    uint32_t tc[DPT], tsum = 0;
 #pragma unroll
-        for (int w = 0; w < kSortWaves; ++w) {
-            const uint32_t c = wave_cnt[w][threadIdx.x];
-            wave_cnt[w][threadIdx.x] = tile_count;
-            tile_count += c;
-        }
-    }
-    uint32_t* my = state + (size_t)tile * 256 + threadIdx.x;
-    uint32_t excl = 0;
-    if (tile == 0) {
-        state_store(my, kStatePrefix | tile_count);
-    } else {
-        state_store(my, kStateCount | tile_count);
-        int j = tile - 1;
-        int spins = 0;
-        while (j >= 0) {
-            const uint32_t st = state_load(state + (size_t)j * 256 + threadIdx.x);
-            if (st & kStatePrefix) {
-                excl += st & kStateMask;
-                break;
-            }
-            if (st & kStateCount) {
-                excl += st & kStateMask;
-                --j;
-                continue;
-            }
-            if (++spins > kSpinLimit) {
-                atomicOr(fault, 1);
-                break;
+    for (int q = 0; q < DPT; ++q) {
+        const int dg = dbase + q;
+        uint32_t c = 0;
+        if (dg < R) {
+#pragma unroll
+            for (int w = 0; w < kSortWaves; ++w) {
+                const uint32_t x = wave_cnt[w][dg];
+                wave_cnt[w][dg] = c;
+                c += x;
             }
         }
-        state_store(my, kStatePrefix | (excl + tile_count));
+        tc[q] = c;
+        tsum += c;
     }
-    digit_base[threadIdx.x] = bases[threadIdx.x] + excl;
+    uint32_t run = block_exclusive_scan(tsum, scan_part);
+    uint32_t* my = state + (size_t)tile * R;
+#pragma unroll
+    for (int q = 0; q < DPT; ++q)
+        if (dbase + q < R) state_store(my + dbase + q, (tile == 0 ? kStatePrefix : kStateCount) | tc[q]);
+#pragma unroll
+    for (int q = 0; q < DPT; ++q) {
+        const int dg = dbase + q;
+        if (dg >= R) break;
+        uint32_t excl = 0;
+        if (tile > 0) {
+            int j = tile - 1;
+            int spins = 0;
+            while (j >= 0) {
+                const uint32_t st = state_load(state + (size_t)j * R + dg);
+                if (st & kStatePrefix) {
+                    excl += st & kStateMask;
+                    break;
+                }
+                if (st & kStateCount) {
+                    excl += st & kStateMask;
+                    --j;
+                    continue;
+                }
+                if (++spins > kSpinLimit) {
+                    atomicOr(fault, 1);
+                    break;
+                }
+            }
+            state_store(my + dg, kStatePrefix | (excl + tc[q]));
+        }
+        digit_base[dg] = bases[dg] + excl - run;
+        // wave offsets become tile slots: slot of an element = run start + wave offset + rank
+#pragma unroll
+        for (int w = 0; w < kSortWaves; ++w) wave_cnt[w][dg] += run;
+        run += tc[q];
+    }
+    __syncthreads();
+    // stage the tile in LDS in output order, then write it out with consecutive threads on
+    // consecutive slots (each digit's run is contiguous in the output)
+#pragma unroll
+    for (int it = 0; it < kSortItems; ++it) r[it] += wave_cnt[wave][d[it]];
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < kSortItems; ++it) {
         const int i = chunk + it * 64 + lane;
         if (i < n) {
-            const uint32_t pos = digit_base[d[it]] + wave_cnt[wave][d[it]] + r[it];
-            keys_out[pos] = k[it];
-            vals_out[pos] = v[it];
+            stage_k[r[it]] = k[it];
+            stage_v[r[it]] = v[it];
+        }
+    }
+    __syncthreads();
+    const int valid = min(kSortTile, n - tile * kSortTile);
+#pragma unroll
+    for (int it = 0; it < kSortItems; ++it) {
+        const int j = it * kSortBlock + (int)threadIdx.x;
+        if (j < valid) {
+            const uint64_t key = stage_k[j];
+            const uint32_t dig = (uint32_t)(key >> shift) & mask;
+            const uint32_t pos = digit_base[dig] + (uint32_t)j;
+            keys_out[pos] = key;
+            vals_out[pos] = stage_v[j];
         }
     }
 }
@@ -402,6 +477,63 @@ static wf_status sort_identity(wf_context* ctx, const uint64_t* keys_in, const i
     return WF_OK;
 }
 
+// Passes of BITS-wide digits over key bits [lo_bit, hi_bit]; the last pass writes keys_out and
+// vals_out.
+template <int BITS>
+static wf_status sweep_sort(wf_context* ctx, const uint64_t* keys_in, const int32_t* vals_in,
+                            uint64_t* keys_out, int32_t* vals_out, Count n, int lo_bit, int hi_bit,
+                            const char* tag) {
+    constexpr int R = Digits<BITS>::R;
+    const int span = hi_bit - lo_bit + 1;
+    const int tiles = (n.n + kSortTile - 1) / kSortTile;
+    const int passes = (span + BITS - 1) / BITS;
+    if (passes > Digits<BITS>::MaxPasses || passes > kMaxPasses)
+        return set_error(WF_ERR_INVALID_ARGUMENT, "sort key too wide");
+    void *pk = nullptr, *pv = nullptr, *ph = nullptr, *pst = nullptr, *pc = nullptr;
+    WF_TRY(workspace(ctx, "sort_keys_alt", sizeof(uint64_t) * (size_t)n.n, &pk));
+    WF_TRY(workspace(ctx, "sort_vals_alt", sizeof(int32_t) * (size_t)n.n, &pv));
+    WF_TRY(workspace(ctx, "sort_hist", sizeof(uint32_t) * (size_t)R * passes, &ph));
+    WF_TRY(workspace(ctx, "sort_state", sizeof(uint32_t) * (size_t)R * tiles * passes, &pst));
+    // [0, kMaxPasses) tile counters, [kMaxPasses] fault flag
+    WF_TRY(workspace(ctx, "sort_counters", sizeof(int32_t) * (kMaxPasses + 1), &pc));
+    uint32_t* hist = (uint32_t*)ph;
+    uint32_t* state = (uint32_t*)pst;
+    int32_t* counters = (int32_t*)pc;
+    WF_TRY(wf_fill_u32(ctx, hist, 0u, (size_t)R * passes));
+    WF_TRY(wf_fill_u32(ctx, state, 0u, (size_t)R * tiles * passes));
+    WF_TRY(wf_fill_u32(ctx, counters, 0u, kMaxPasses));
+    {
+        Timed t(ctx, tag, 8.0, n);
+        const unsigned blocks = std::min<unsigned>(grid_for(n.n, kSortBlock), 1024u);
+        k_sweep_hist<BITS><<<blocks, kSortBlock, 0, ctx->stream>>>(keys_in, n.n, n.dev, lo_bit,
+                                                                   hi_bit, passes, hist);
+        WF_HIP(hipGetLastError());
+        k_sweep_bases<BITS><<<passes, kSortBlock, 0, ctx->stream>>>(hist);
+        WF_HIP(hipGetLastError());
+    }
+    // ping-pong so that the last pass writes keys_out/vals_out
+    const uint64_t* ksrc = keys_in;
+    const int32_t* vsrc = vals_in;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = lo_bit + p * BITS;
+        const int nbits = (hi_bit + 1 - shift) < BITS ? (hi_bit + 1 - shift) : BITS;
+        const uint32_t mask = (1u << nbits) - 1u;
+        const bool to_out = ((passes - 1 - p) % 2) == 0;
+        uint64_t* kdst = to_out ? keys_out : (uint64_t*)pk;
+        int32_t* vdst = to_out ? vals_out : (int32_t*)pv;
+        {
+            Timed t(ctx, tag, 24.0, n);
+            k_sweep_pass<BITS><<<tiles, kSortBlock, 0, ctx->stream>>>(
+                ksrc, vsrc, kdst, vdst, n.n, n.dev, shift, mask, nbits, hist + (size_t)R * p,
+                state + (size_t)R * tiles * p, counters + p, counters + kMaxPasses);
+            WF_HIP(hipGetLastError());
+        }
+        ksrc = kdst;
+        vsrc = vdst;
+    }
+    return WF_OK;
+}
+
 wf_status radix_sort_pairs(wf_context* ctx, const uint64_t* keys_in, const int32_t* vals_in,
                            uint64_t* keys_out, int32_t* vals_out, Count n, int32_t key_bits,
                            const char* tag) {
@@ -427,53 +559,11 @@ wf_status radix_sort_pairs(wf_context* ctx, const uint64_t* keys_in, const int32
         lo_bit = __builtin_ctzll(diff);
         hi_bit = 63 - __builtin_clzll(diff);
     }
-    const int span = hi_bit - lo_bit + 1;
-    const int tiles = (n.n + kSortTile - 1) / kSortTile;
-    const int passes = (span + 7) / 8;
-    if (passes > kMaxPasses) return set_error(WF_ERR_INVALID_ARGUMENT, "sort key too wide");
-    void *pk = nullptr, *pv = nullptr, *ph = nullptr, *pst = nullptr, *pc = nullptr;
-    WF_TRY(workspace(ctx, "sort_keys_alt", sizeof(uint64_t) * (size_t)n.n, &pk));
-    WF_TRY(workspace(ctx, "sort_vals_alt", sizeof(int32_t) * (size_t)n.n, &pv));
-    WF_TRY(workspace(ctx, "sort_hist", sizeof(uint32_t) * 256 * kMaxPasses, &ph));
-    WF_TRY(workspace(ctx, "sort_state", sizeof(uint32_t) * 256 * (size_t)tiles * passes, &pst));
-    // [0, kMaxPasses) tile counters, [kMaxPasses] fault flag
-    WF_TRY(workspace(ctx, "sort_counters", sizeof(int32_t) * (kMaxPasses + 1), &pc));
-    uint32_t* hist = (uint32_t*)ph;
-    uint32_t* state = (uint32_t*)pst;
-    int32_t* counters = (int32_t*)pc;
-    WF_TRY(wf_fill_u32(ctx, hist, 0u, 256 * (size_t)passes));
-    WF_TRY(wf_fill_u32(ctx, state, 0u, 256 * (size_t)tiles * passes));
-    WF_TRY(wf_fill_u32(ctx, counters, 0u, kMaxPasses));
-    {
-        Timed t(ctx, tag, 8.0, n);
-        const unsigned blocks = std::min<unsigned>(grid_for(n.n, kSortBlock), 1024u);
-        k_sweep_hist<<<blocks, kSortBlock, 0, ctx->stream>>>(keys_in, n.n, n.dev, lo_bit, hi_bit,
-                                                             passes, hist);
-        WF_HIP(hipGetLastError());
-        k_sweep_bases<<<passes, 256, 0, ctx->stream>>>(hist);
-        WF_HIP(hipGetLastError());
+    switch (ctx->tuning.radix_bits) {
+        case 6: return sweep_sort<6>(ctx, keys_in, vals_in, keys_out, vals_out, n, lo_bit, hi_bit, tag);
+        case 11: return sweep_sort<11>(ctx, keys_in, vals_in, keys_out, vals_out, n, lo_bit, hi_bit, tag);
+        default: return sweep_sort<8>(ctx, keys_in, vals_in, keys_out, vals_out, n, lo_bit, hi_bit, tag);
     }
-    // ping-pong so that the last pass writes keys_out/vals_out
-    const uint64_t* ksrc = keys_in;
-    const int32_t* vsrc = vals_in;
-    for (int p = 0; p < passes; ++p) {
-        const int shift = lo_bit + p * 8;
-        const int nbits = (hi_bit + 1 - shift) < 8 ? (hi_bit + 1 - shift) : 8;
-        const uint32_t mask = (1u << nbits) - 1u;
-        const bool to_out = ((passes - 1 - p) % 2) == 0;
-        uint64_t* kdst = to_out ? keys_out : (uint64_t*)pk;
-        int32_t* vdst = to_out ? vals_out : (int32_t*)pv;
-        {
-            Timed t(ctx, tag, 24.0, n);
-            k_sweep_pass<<<tiles, kSortBlock, 0, ctx->stream>>>(
-                ksrc, vsrc, kdst, vdst, n.n, n.dev, shift, mask, nbits, hist + 256 * p,
-                state + (size_t)256 * tiles * p, counters + p, counters + kMaxPasses);
-            WF_HIP(hipGetLastError());
-        }
-        ksrc = kdst;
-        vsrc = vdst;
-    }
-    return WF_OK;
 }
 
 }  // namespace wf
