@@ -150,6 +150,60 @@ __device__ __forceinline__ int block_append(int32_t* cursor, bool take, uint32_t
     return take ? (int)(base + off + (uint32_t)__popcll(b & lanemask_lt())) : -1;
 }
 
+// Appends the items of ITEMS rounds (round j of every thread of the block, then round j + 1) to
+// a list at *cursor with one cursor atomic per block (device-scope atomics on one address
+// serialise, so fewer and larger claims keep the cursor off the critical path). Slots run in
+// round order, then wave order, then lane order; slot[j] = (T)-1 where take[j] is false. Every
+// thread of the BLOCK-thread block must call it; lds holds at least ITEMS * BLOCK / 64 words
+// owned by the caller. T: int32_t or unsigned long long cursors.
+template <int ITEMS, int BLOCK, typename T>
+__device__ __forceinline__ void block_append_rounds(T* cursor, const bool (&take)[ITEMS],
+                                                    T (&slot)[ITEMS], uint32_t* lds) {
+    constexpr int W = BLOCK / 64;
+    constexpr int E = ITEMS * W;                 // (round, wave) counts
+    constexpr int PER = (E + 63) / 64;           // counts per lane of the scanning wave
+    __shared__ T s_base;
+    const int lane = (int)__lane_id(), wave = (int)(threadIdx.x >> 6);
+    uint64_t b[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+        b[j] = __ballot(take[j]);
+        if (lane == 0) lds[j * W + wave] = (uint32_t)__popcll(b[j]);
+    }
+    __syncthreads();
+    if (wave == 0) {
+        // exclusive scan of the E counts in (round, wave) order; lane l owns PER consecutive
+        uint32_t c[PER], sum = 0;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = lane * PER + q;
+            c[q] = e < E ? lds[e] : 0u;
+            sum += c[q];
+        }
+        uint32_t inc = sum;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(inc, o, 64);
+            if (lane >= o) inc += y;
+        }
+        const uint32_t total = __shfl(inc, 63, 64);
+        uint32_t run = inc - sum;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = lane * PER + q;
+            if (e < E) lds[e] = run;
+            run += c[q];
+        }
+        if (lane == 0) s_base = total ? atomicAdd(cursor, (T)total) : (T)0;
+    }
+    __syncthreads();
+    const T base = s_base;
+    const uint64_t lt = lanemask_lt();
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j)
+        slot[j] = take[j] ? base + (T)(lds[j * W + wave] + (uint32_t)__popcll(b[j] & lt)) : (T)-1;
+    __syncthreads();   // lds and s_base are reused by the next call
+}
+
 // Element count of a launch: the device counter when present, else the host value.
 __device__ __forceinline__ int load_count(int n, const int32_t* __restrict__ dev) {
     return dev ? *dev : n;

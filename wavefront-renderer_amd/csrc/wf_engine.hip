@@ -173,19 +173,18 @@ __global__ void k_regen_commit(EngineView e) {
 }
 
 constexpr int kFinishBlock = 1024;
+constexpr int kFinishItems = 4;
 
 // A path of accumulator a ended: count the sample, update the adaptive estimate
 // (AdaptiveSampler::addSample/updateConvergence, include/render/AdaptiveSampler.h:21-105, checked
 // when the pixel's sample count is a multiple of the interval, as the pass loop does), then
-// requeue the lane's next sample, or end the lane; the pixel retires when its last lane ends (or
-// when it converged). Requeues are appended with one queue-cursor atomic per block. Called by
-// every thread of the block (valid = this thread has an ended path; tag = its job slot and
-// sample index).
-__device__ __forceinline__ void finish_path(const EngineView& e, int a, int32_t tag, bool valid) {
-    __shared__ uint32_t block_count;
-    __shared__ unsigned long long block_base;
+// requeue the lane's next sample (returned in *entry), or end the lane; the pixel retires when its
+// last lane ends (or when it converged). Called by every thread of the wave (valid = this thread
+// has an ended path; tag = its job slot and sample index).
+__device__ __forceinline__ bool finish_eval(const EngineView& e, int a, int32_t tag, bool valid,
+                                            int2* entry) {
     bool requeue = false, retire = false, unconverged = false;
-    int32_t next = 0, job = 0, samples = 0;
+    int32_t job = 0, samples = 0;
     if (valid) {
         const int j = tag >> kTagJobShift;
         const JobDev& J = e.jobs[j];
@@ -219,7 +218,7 @@ __device__ __forceinline__ void finish_path(const EngineView& e, int a, int32_t 
             samples = s + 1;
         } else if (ns < J.spp) {
             requeue = true;
-            next = (j << kTagJobShift) | ns;
+            *entry = make_int2(a, (j << kTagJobShift) | ns);
         } else if (J.lanes == 1 || atomicAdd(&e.lanes_done[p], 1) == J.lanes - 1) {
             // the pixel's last lane ran its last sample: all spp samples are in
             e.pixel_count[p] = J.spp;
@@ -251,43 +250,56 @@ __device__ __forceinline__ void finish_path(const EngineView& e, int a, int32_t 
             if (unconverged) atomicAdd(&e.job_unconverged[job], 1);
         }
     }
-    if (threadIdx.x == 0) block_count = 0;
-    __syncthreads();
-    const uint64_t b = __ballot(requeue);
-    const int lane = (int)__lane_id();
-    uint32_t wave_off = 0;
-    if (lane == 0 && b) wave_off = atomicAdd(&block_count, (uint32_t)__popcll(b));
-    wave_off = __shfl(wave_off, 0, 64);
-    __syncthreads();
-    if (threadIdx.x == 0)
-        block_base = block_count ? atomicAdd(&e.stats[S_TAIL], (unsigned long long)block_count) : 0ull;
-    __syncthreads();
-    if (requeue) {
-        const unsigned long long pos =
-            block_base + wave_off + (unsigned long long)__popcll(b & lanemask_lt());
-        e.ready[pos % e.ring] = make_int2(a, next);
-    }
+    return requeue;
+}
+
+// Requeues of kFinishItems rounds, appended to the ready ring with one cursor atomic per block.
+__device__ __forceinline__ void finish_requeue(const EngineView& e, const bool (&rq)[kFinishItems],
+                                               const int2 (&entry)[kFinishItems], uint32_t* lds) {
+    unsigned long long pos[kFinishItems];
+    block_append_rounds<kFinishItems, kFinishBlock>(&e.stats[S_TAIL], rq, pos, lds);
+#pragma unroll
+    for (int j = 0; j < kFinishItems; ++j)
+        if (rq[j]) e.ready[pos[j] % e.ring] = entry[j];
 }
 
 __global__ __launch_bounds__(kFinishBlock) void k_finish_hits(EngineView e,
                                                               const int32_t* __restrict__ pixel,
                                                               const int32_t* __restrict__ tags,
                                                               const uint32_t* __restrict__ alive) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint32_t lds[kFinishItems * kFinishBlock / 64];
     const int n = e.counters[C_HITS];
-    if (blockIdx.x * blockDim.x >= n) return;
-    const bool valid = i < n && !alive[i];
-    finish_path(e, valid ? pixel[i] : 0, valid ? tags[i] : 0, valid);
+    const int base = blockIdx.x * kFinishBlock * kFinishItems;
+    if (base >= n) return;
+    bool rq[kFinishItems];
+    int2 entry[kFinishItems];
+#pragma unroll
+    for (int j = 0; j < kFinishItems; ++j) {
+        const int i = base + j * kFinishBlock + (int)threadIdx.x;
+        const bool valid = i < n && !alive[i];
+        entry[j] = make_int2(0, 0);
+        rq[j] = finish_eval(e, valid ? pixel[i] : 0, valid ? tags[i] : 0, valid, &entry[j]);
+    }
+    finish_requeue(e, rq, entry, lds);
 }
 
 __global__ __launch_bounds__(kFinishBlock) void k_finish_misses(EngineView e,
                                                                 const int32_t* __restrict__ pixel,
                                                                 const int32_t* __restrict__ tags) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint32_t lds[kFinishItems * kFinishBlock / 64];
     const int n = e.counters[C_MISSES];
-    if (blockIdx.x * blockDim.x >= n) return;
-    const bool valid = i < n;
-    finish_path(e, valid ? pixel[i] : 0, valid ? tags[i] : 0, valid);
+    const int base = blockIdx.x * kFinishBlock * kFinishItems;
+    if (base >= n) return;
+    bool rq[kFinishItems];
+    int2 entry[kFinishItems];
+#pragma unroll
+    for (int j = 0; j < kFinishItems; ++j) {
+        const int i = base + j * kFinishBlock + (int)threadIdx.x;
+        const bool valid = i < n;
+        entry[j] = make_int2(0, 0);
+        rq[j] = finish_eval(e, valid ? pixel[i] : 0, valid ? tags[i] : 0, valid, &entry[j]);
+    }
+    finish_requeue(e, rq, entry, lds);
 }
 
 __global__ void k_iteration_end(EngineView e) {
@@ -503,11 +515,10 @@ wf_status enqueue_iteration(wf_engine* e) {
     phase_mark(e, 6);
     {
         Timed tm(ctx, "finish", 12.0, live);
-        k_finish_hits<<<grid_for(cap, kFinishBlock), kFinishBlock, 0, ctx->stream>>>(v, q->pixel,
-                                                                                    qt, alive);
+        const unsigned grid = grid_for(cap, kFinishBlock * kFinishItems);
+        k_finish_hits<<<grid, kFinishBlock, 0, ctx->stream>>>(v, q->pixel, qt, alive);
         WF_HIP(hipGetLastError());
-        k_finish_misses<<<grid_for(cap, kFinishBlock), kFinishBlock, 0, ctx->stream>>>(
-            v, e->miss.pixel, e->mt);
+        k_finish_misses<<<grid, kFinishBlock, 0, ctx->stream>>>(v, e->miss.pixel, e->mt);
         WF_HIP(hipGetLastError());
         k_iteration_end<<<1, 1, 0, ctx->stream>>>(v);
         WF_HIP(hipGetLastError());

@@ -320,14 +320,26 @@ __global__ __launch_bounds__(kCbBlock) void k_cb_partials(
 }
 
 // priority = benefit / max(relCost(mat) + relCost(tex of mat), 1e-6)   (ShadingQueue.h:458-471)
-__global__ void k_cb_priority(const double* __restrict__ partials, int tiles, int n_mat,
-                              const DevMaterial* __restrict__ mats,
-                              const double* __restrict__ mat_cost,
-                              const double* __restrict__ tex_cost, double* __restrict__ prio) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= n_mat) return;
-    double s = 0.0;
-    for (int t = 0; t < tiles; ++t) s += partials[(size_t)t * n_mat + m];
+// One block per material: thread t sums tiles t, t + 256, ... in order, then a fixed pairwise
+// tree over the threads (the same association on every run).
+__global__ __launch_bounds__(256) void k_cb_priority(const double* __restrict__ partials,
+                                                     int tiles, int n_mat,
+                                                     const DevMaterial* __restrict__ mats,
+                                                     const double* __restrict__ mat_cost,
+                                                     const double* __restrict__ tex_cost,
+                                                     double* __restrict__ prio) {
+    __shared__ double red[256];
+    const int m = blockIdx.x;
+    double acc = 0.0;
+    for (int t = threadIdx.x; t < tiles; t += 256) acc += partials[(size_t)t * n_mat + m];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const double s = red[0];
     const double sc = mat_cost[m];
     const int tid = mats[m].texture_id;
     const double tc = tid >= 0 ? tex_cost[tid] : 0.0;
@@ -403,6 +415,7 @@ wf_status shading_keys(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q
     }
     if (policy == WF_POLICY_COST_BENEFIT) {
         const int nm = d.n_materials;
+        if (nm <= 0) return set_error(WF_ERR_INVALID_ARGUMENT, "costBenefit: scene has no materials");
         if (nm > kCbMaxMaterials)
             return set_error(WF_ERR_UNSUPPORTED,
                              "costBenefit policy supports at most %d materials (scene has %d)",
@@ -424,7 +437,7 @@ wf_status shading_keys(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q
                                                            (double*)pp, (uint32_t*)ppres,
                                                            (uint32_t*)pb);
         WF_HIP(hipGetLastError());
-        k_cb_priority<<<grid_for(nm, 256), 256, 0, ctx->stream>>>(
+        k_cb_priority<<<nm, 256, 0, ctx->stream>>>(
             (double*)pp, tiles, nm, d.materials, d.mat_rel_cost, d.tex_rel_cost, (double*)pprio);
         WF_HIP(hipGetLastError());
         k_cb_rank<<<grid_for(nm, 256), 256, 0, ctx->stream>>>((double*)pprio, (uint32_t*)ppres,

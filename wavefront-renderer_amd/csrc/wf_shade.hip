@@ -570,42 +570,55 @@ __global__ void k_shade(const SceneDev s, wf_shade_soa q, int n_host,
 }
 
 // Unordered compaction after k_shade: shadow entries and next rays appended at device cursors
-// (one atomic per block and cursor), shadow rays counted.
+// (kAppendItems entries per thread in coalesced rounds, one atomic per block and cursor), shadow
+// rays counted (one atomic per block).
 constexpr int kAppendBlock = 1024;
+constexpr int kAppendItems = 4;
 __global__ __launch_bounds__(kAppendBlock) void k_shade_append(wf_shade_soa q, ShadeOut o, int n_host,
                                                                const int32_t* __restrict__ nd,
                                                                const int32_t* __restrict__ tags) {
-    __shared__ uint32_t scratch[2];
-    const int n = load_count(n_host, nd);
-    if ((int)(blockIdx.x * blockDim.x) >= n) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = i < n;
-    const uint32_t f = valid ? o.shadow_flag[i] : 0u;
-    const bool alive = valid && o.alive[i] != 0u;
-    const int si = block_append(o.shadow_entries, f != 0u, scratch);
-    if (si >= 0) o.shadow_list[si] = i;
+    __shared__ uint32_t lds[kAppendItems * kAppendBlock / 64];
     __shared__ uint32_t block_rays;
+    const int n = load_count(n_host, nd);
+    const int base = blockIdx.x * kAppendBlock * kAppendItems;
+    if (base >= n) return;
+    bool shadow[kAppendItems], alive[kAppendItems];
+    int si[kAppendItems], k[kAppendItems];
+    int rays = 0;
+#pragma unroll
+    for (int j = 0; j < kAppendItems; ++j) {
+        const int i = base + j * kAppendBlock + (int)threadIdx.x;
+        const uint32_t f = i < n ? o.shadow_flag[i] : 0u;
+        shadow[j] = f != 0u;
+        alive[j] = i < n && o.alive[i] != 0u;
+        rays += (int)(f & 1u) + (int)((f >> 1) & 1u);
+    }
     if (threadIdx.x == 0) block_rays = 0u;
-    __syncthreads();
-    const int rays = __popcll(__ballot((f & 1u) != 0u)) + __popcll(__ballot((f & 2u) != 0u));
+    block_append_rounds<kAppendItems, kAppendBlock>(o.shadow_entries, shadow, si, lds);
+    for (int off = 32; off >= 1; off >>= 1) rays += __shfl_xor(rays, off, 64);
     if (__lane_id() == 0 && rays) atomicAdd(&block_rays, (uint32_t)rays);
-    __syncthreads();
+    block_append_rounds<kAppendItems, kAppendBlock>(o.next_count, alive, k, lds);
     if (threadIdx.x == 0 && block_rays) atomicAdd(o.shadow_rays, (int32_t)block_rays);
-    const int k = block_append(o.next_count, alive, scratch);
-    if (k >= 0) {
-        o.next.ox[k] = q.px[i];
-        o.next.oy[k] = q.py[i];
-        o.next.oz[k] = q.pz[i];
-        o.next.dx[k] = o.ndx[i];
-        o.next.dy[k] = o.ndy[i];
-        o.next.dz[k] = o.ndz[i];
-        o.next.tr[k] = o.ntr[i];
-        o.next.tg[k] = o.ntg[i];
-        o.next.tb[k] = o.ntb[i];
-        o.next.pixel[k] = q.pixel[i];
-        o.next.depth[k] = q.depth[i] + 1;
-        o.next.pdf[k] = o.npdf[i];
-        if (o.next_tags) o.next_tags[k] = tags[i];
+#pragma unroll
+    for (int j = 0; j < kAppendItems; ++j) {
+        const int i = base + j * kAppendBlock + (int)threadIdx.x;
+        if (si[j] >= 0) o.shadow_list[si[j]] = i;
+        if (k[j] >= 0) {
+            const int d = k[j];
+            o.next.ox[d] = q.px[i];
+            o.next.oy[d] = q.py[i];
+            o.next.oz[d] = q.pz[i];
+            o.next.dx[d] = o.ndx[i];
+            o.next.dy[d] = o.ndy[i];
+            o.next.dz[d] = o.ndz[i];
+            o.next.tr[d] = o.ntr[i];
+            o.next.tg[d] = o.ntg[i];
+            o.next.tb[d] = o.ntb[i];
+            o.next.pixel[d] = q.pixel[i];
+            o.next.depth[d] = q.depth[i] + 1;
+            o.next.pdf[d] = o.npdf[i];
+            if (o.next_tags) o.next_tags[d] = tags[i];
+        }
     }
 }
 
@@ -691,9 +704,11 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(const SceneDev s, const 
     // directional ray (phase 1); phase 2 = both resolved
     int item = -1, i = 0, phase = 2;
     uint32_t f = 0;
-    bool occ_area = false, occ_dir = false, exhausted = false;
+    bool occ_area = false, occ_dir = false;
+    WorkQueue wq;
+    wq.init(counter, n);
     for (;;) {
-        const int r = refill(counter, n, item < 0, exhausted, refill_lanes);
+        const int r = refill(wq, item < 0, refill_lanes);
         if (r >= 0) {
             item = r;
             i = list[r];
@@ -717,7 +732,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(const SceneDev s, const 
             }
         }
         if (__ballot(item >= 0) == 0) {
-            if (exhausted) break;
+            if (wq.exhausted()) break;
             continue;
         }
         if (item < 0) continue;
@@ -823,14 +838,15 @@ wf_status shade_bounce(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q
         }
         {
             Timed tm(ctx, "next_rays", 56.0, n);
-            k_shade_append<<<grid_for(n.n, kAppendBlock), kAppendBlock, 0, ctx->stream>>>(
+            k_shade_append<<<grid_for(n.n, kAppendBlock * kAppendItems), kAppendBlock, 0,
+                             ctx->stream>>>(
                 *q, o, n.n, n.dev, jobs.tags);
             WF_HIP(hipGetLastError());
         }
         const Count ns{n.n, counts_dev + 2};
         void* pc = nullptr;
-        WF_TRY(workspace(ctx, "shadow_counter", sizeof(int32_t), &pc));
-        WF_TRY(wf_fill_u32(ctx, pc, 0u, 1));
+        WF_TRY(workspace(ctx, "shadow_counter", sizeof(int32_t) * kWorkInts, &pc));
+        WF_TRY(wf_fill_u32(ctx, pc, 0u, kWorkInts));
         Timed tm(ctx, "shadow", 41.0, ns);
         with_bvh(s->dev, [&](auto nodes, auto S) {
             k_shadow<std::remove_cv_t<std::remove_pointer_t<decltype(nodes)>>, decltype(S)::value>
@@ -873,8 +889,8 @@ wf_status shade_bounce(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q
             o.shadow_flag, (int32_t*)plist, ns.n, ns.dev, counts + 1);
         WF_HIP(hipGetLastError());
         void* pc = nullptr;
-        WF_TRY(workspace(ctx, "shadow_counter", sizeof(int32_t), &pc));
-        WF_TRY(wf_fill_u32(ctx, pc, 0u, 1));
+        WF_TRY(workspace(ctx, "shadow_counter", sizeof(int32_t) * kWorkInts, &pc));
+        WF_TRY(wf_fill_u32(ctx, pc, 0u, kWorkInts));
         Timed tm(ctx, "shadow", 41.0, ns);
         with_bvh(s->dev, [&](auto nodes, auto S) {
             k_shadow<std::remove_cv_t<std::remove_pointer_t<decltype(nodes)>>, decltype(S)::value>

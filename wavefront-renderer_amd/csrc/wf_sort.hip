@@ -35,6 +35,7 @@ constexpr uint32_t kStatePrefix = 2u << 30;   // flag: inclusive prefix publishe
 constexpr uint32_t kStateMask = (1u << 30) - 1u;
 constexpr int kMaxPasses = 11;
 constexpr int kSpinLimit = 1 << 22;
+constexpr int kLookBack = 8;   // predecessor states read per look-back round trip
 
 // Digit width of the one-sweep passes: BITS in {6, 8, 11} (R = 2^BITS digits). A thread owns
 // DPT consecutive digits in the per-digit steps.
@@ -197,20 +198,31 @@ __global__ __launch_bounds__(kSortBlock) void k_sweep_pass(
         if (dg >= R) break;
         uint32_t excl = 0;
         if (tile > 0) {
+            // look back kLookBack predecessors per round trip: sum published counts from the
+            // nearest tile down to the first inclusive prefix; an unpublished state ends the
+            // window and is loaded again (tile 0 always holds a prefix, so the walk ends)
             int j = tile - 1;
             int spins = 0;
-            while (j >= 0) {
-                const uint32_t st = state_load(state + (size_t)j * R + dg);
-                if (st & kStatePrefix) {
-                    excl += st & kStateMask;
-                    break;
+            bool found = false;
+            while (!found) {
+                uint32_t st[kLookBack];
+#pragma unroll
+                for (int w = 0; w < kLookBack; ++w)
+                    st[w] = j - w >= 0 ? state_load(state + (size_t)(j - w) * R + dg) : 0u;
+                int used = 0;
+#pragma unroll
+                for (int w = 0; w < kLookBack; ++w) {
+                    if (found || used < w || j - w < 0) continue;
+                    if (st[w] & kStatePrefix) {
+                        excl += st[w] & kStateMask;
+                        found = true;
+                    } else if (st[w] & kStateCount) {
+                        excl += st[w] & kStateMask;
+                        ++used;
+                    }
                 }
-                if (st & kStateCount) {
-                    excl += st & kStateMask;
-                    --j;
-                    continue;
-                }
-                if (++spins > kSpinLimit) {
+                j -= used;
+                if (!found && used == 0 && ++spins > kSpinLimit) {
                     atomicOr(fault, 1);
                     break;
                 }
@@ -504,7 +516,7 @@ static wf_status sweep_sort(wf_context* ctx, const uint64_t* keys_in, const int3
     WF_TRY(wf_fill_u32(ctx, counters, 0u, kMaxPasses));
     {
         Timed t(ctx, tag, 8.0, n);
-        const unsigned blocks = std::min<unsigned>(grid_for(n.n, kSortBlock), 1024u);
+        const unsigned blocks = std::min<unsigned>(grid_for(n.n, kSortBlock), 256u);
         k_sweep_hist<BITS><<<blocks, kSortBlock, 0, ctx->stream>>>(keys_in, n.n, n.dev, lo_bit,
                                                                    hi_bit, passes, hist);
         WF_HIP(hipGetLastError());

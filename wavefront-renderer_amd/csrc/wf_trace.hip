@@ -30,15 +30,16 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(
     const int n = load_count(n_host, nd);
     W w;
     int ray = -1;
-    bool exhausted = false;
+    WorkQueue wq;
+    wq.init(counter, n);
     for (;;) {
-        const int r = refill(counter, n, ray < 0, exhausted, refill_lanes);
+        const int r = refill(wq, ray < 0, refill_lanes);
         if (r >= 0) {
             ray = r;
             w.start(V3(ox[r], oy[r], oz[r]), V3(dx[r], dy[r], dz[r]), tnear, tfar);
         }
         if (__ballot(ray >= 0) == 0) {
-            if (exhausted) break;
+            if (wq.exhausted()) break;
             continue;
         }
         if (ray >= 0 && (n_tris <= 0 || w.step(nodes, tri, stack, spill))) {
@@ -162,39 +163,53 @@ __global__ void k_split(const SceneDev s, wf_ray_soa in, int n_host, const int32
     }
 }
 
-// Unordered split: hits and misses appended at counts[0] / counts[1] (one atomic per block).
+// Unordered split: hits and misses appended at counts[0] / counts[1], kAppendItems rays per
+// thread (coalesced rounds), one cursor atomic per block and list.
 constexpr int kAppendBlock = 1024;
+constexpr int kAppendItems = 4;
 __global__ __launch_bounds__(kAppendBlock) void k_split_append(
     const SceneDev s, wf_ray_soa in, int n_host, const int32_t* __restrict__ nd,
     const int32_t* __restrict__ leaf, const float* __restrict__ ht, const float* __restrict__ hu,
     const float* __restrict__ hv, wf_shade_soa out, wf_ray_soa miss, int32_t* __restrict__ counts,
     const int32_t* __restrict__ tags_in, int32_t* __restrict__ hit_tags,
     int32_t* __restrict__ miss_tags) {
-    __shared__ uint32_t scratch[2];
+    __shared__ uint32_t lds[kAppendItems * kAppendBlock / 64];
     const int n = load_count(n_host, nd);
-    if ((int)(blockIdx.x * blockDim.x) >= n) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = i < n;
-    const int L = valid ? leaf[i] : -1;
-    const int pos = block_append(&counts[0], valid && L >= 0, scratch);
-    const int m = block_append(&counts[1], valid && L < 0, scratch);
-    if (pos >= 0) {
-        map_hit(s, in, i, L, ht[i], hu[i], hv[i], out, pos);
-        if (hit_tags) hit_tags[pos] = tags_in[i];
-    } else if (m >= 0) {
-        miss.ox[m] = in.ox[i];
-        miss.oy[m] = in.oy[i];
-        miss.oz[m] = in.oz[i];
-        miss.dx[m] = in.dx[i];
-        miss.dy[m] = in.dy[i];
-        miss.dz[m] = in.dz[i];
-        miss.tr[m] = in.tr[i];
-        miss.tg[m] = in.tg[i];
-        miss.tb[m] = in.tb[i];
-        miss.pixel[m] = in.pixel[i];
-        miss.depth[m] = in.depth[i];
-        miss.pdf[m] = in.pdf[i];
-        if (miss_tags) miss_tags[m] = tags_in[i];
+    const int base = blockIdx.x * kAppendBlock * kAppendItems;
+    if (base >= n) return;
+    bool hit[kAppendItems], mis[kAppendItems];
+    int L[kAppendItems], pos[kAppendItems], m[kAppendItems];
+#pragma unroll
+    for (int j = 0; j < kAppendItems; ++j) {
+        const int i = base + j * kAppendBlock + (int)threadIdx.x;
+        L[j] = i < n ? leaf[i] : -1;
+        hit[j] = i < n && L[j] >= 0;
+        mis[j] = i < n && L[j] < 0;
+    }
+    block_append_rounds<kAppendItems, kAppendBlock>(&counts[0], hit, pos, lds);
+    block_append_rounds<kAppendItems, kAppendBlock>(&counts[1], mis, m, lds);
+#pragma unroll
+    for (int j = 0; j < kAppendItems; ++j) {
+        const int i = base + j * kAppendBlock + (int)threadIdx.x;
+        if (pos[j] >= 0) {
+            map_hit(s, in, i, L[j], ht[i], hu[i], hv[i], out, pos[j]);
+            if (hit_tags) hit_tags[pos[j]] = tags_in[i];
+        } else if (m[j] >= 0) {
+            const int k = m[j];
+            miss.ox[k] = in.ox[i];
+            miss.oy[k] = in.oy[i];
+            miss.oz[k] = in.oz[i];
+            miss.dx[k] = in.dx[i];
+            miss.dy[k] = in.dy[i];
+            miss.dz[k] = in.dz[i];
+            miss.tr[k] = in.tr[i];
+            miss.tg[k] = in.tg[i];
+            miss.tb[k] = in.tb[i];
+            miss.pixel[k] = in.pixel[i];
+            miss.depth[k] = in.depth[i];
+            miss.pdf[k] = in.pdf[i];
+            if (miss_tags) miss_tags[k] = tags_in[i];
+        }
     }
 }
 
@@ -246,8 +261,8 @@ wf_status trace_closest(wf_context* ctx, const wf_scene* s, const wf_ray_soa* r,
     WF_TRY(scene_ready(s));
     if (n.n <= 0) return WF_OK;
     void* pc = nullptr;
-    WF_TRY(workspace(ctx, "trace_counter", sizeof(int32_t), &pc));
-    WF_TRY(wf_fill_u32(ctx, pc, 0u, 1));
+    WF_TRY(workspace(ctx, "trace_counter", sizeof(int32_t) * kWorkInts, &pc));
+    WF_TRY(wf_fill_u32(ctx, pc, 0u, kWorkInts));
     Timed tm(ctx, "traverse_closest", 44.0, n);
     with_bvh(s->dev, [&](auto nodes, auto S) {
         k_trace_closest<std::remove_cv_t<std::remove_pointer_t<decltype(nodes)>>,
@@ -296,7 +311,8 @@ wf_status intersect_split(wf_context* ctx, const wf_scene* s, const wf_ray_soa* 
                              (float*)pu, (float*)pv, nullptr));
         if (n.n > 0) {
             Timed tm(ctx, "split", 97.0, n);
-            k_split_append<<<grid_for(n.n, kAppendBlock), kAppendBlock, 0, ctx->stream>>>(
+            k_split_append<<<grid_for(n.n, kAppendBlock * kAppendItems), kAppendBlock, 0,
+                             ctx->stream>>>(
                 s->dev, *rays, n.n, n.dev, (int32_t*)pl, (float*)pt, (float*)pu, (float*)pv,
                 *hits_out, *misses_out, counts_dev, tags_in, hit_tags, miss_tags);
             WF_HIP(hipGetLastError());

@@ -388,22 +388,59 @@ __device__ __forceinline__ bool traverse(const NODE* __restrict__ nodes,
     return h.leaf >= 0;
 }
 
-// Dynamic ray refill for persistent traversal kernels: lanes whose ray finished take the next
-// unclaimed ray indices from a device work counter, one atomic per wave, once at least
-// refill_lanes lanes are idle (or every lane is). Returns the new ray index or -1.
-__device__ __forceinline__ int refill(int32_t* __restrict__ counter, int n, bool idle,
-                                      bool& exhausted, int refill_lanes) {
-    const uint64_t want = __ballot(idle && !exhausted);
+// Work distribution for the persistent traversal kernels. Device-scope atomics on one address
+// serialise (about 11 ns each on MI355X, so a single claim counter caps a kernel at ~88M claims
+// per second), so the item range [0, n) is split into kWorkSegments contiguous segments, each
+// with its own claim counter on its own 128-byte line. A wave starts in segment (global wave
+// index mod kWorkSegments), claims chunks there until the segment is exhausted, then moves to the
+// next segment; once it has found kWorkSegments segments in a row exhausted (counters only grow,
+// so then every segment is) it is done. Chunks are contiguous, so sorted rays stay coherent.
+constexpr int kWorkSegments = 64;
+constexpr int kWorkStride = 32;   // int32 per counter slot
+constexpr int kWorkInts = kWorkSegments * kWorkStride;
+
+struct WorkQueue {
+    int32_t* ctr;   // kWorkInts, zero before the launch
+    int n, seg_size, seg, dry;
+
+    __device__ __forceinline__ void init(int32_t* counters, int count) {
+        ctr = counters;
+        n = count;
+        seg_size = (count + kWorkSegments - 1) / kWorkSegments;
+        seg = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) % kWorkSegments;
+        dry = 0;
+    }
+    __device__ __forceinline__ bool exhausted() const { return dry >= kWorkSegments; }
+};
+
+// Lanes whose item finished take the next unclaimed items, one atomic per wave, once at least
+// refill_lanes lanes are idle (or every lane is). Returns the new item index or -1.
+__device__ __forceinline__ int refill(WorkQueue& q, bool idle, int refill_lanes) {
+    const uint64_t want = __ballot(idle && !q.exhausted());
     const int k = __popcll(want);
     if (k == 0 || (k < refill_lanes && __ballot(!idle) != 0)) return -1;
     const int leader = __ffsll((unsigned long long)want) - 1;
-    int base = 0;
-    if ((int)__lane_id() == leader) base = atomicAdd(counter, k);
-    base = __shfl(base, leader, 64);
-    if (base + k >= n) exhausted = true;
-    if (!idle) return -1;
-    const int r = base + __popcll(want & lanemask_lt());
-    return r < n ? r : -1;
+    int first = -1, end = 0;
+    while (!q.exhausted()) {
+        const int s0 = q.seg * q.seg_size;
+        const int s1 = min(q.n, s0 + q.seg_size);
+        if (s0 < s1) {
+            int base = 0;
+            if ((int)__lane_id() == leader) base = atomicAdd(&q.ctr[q.seg * kWorkStride], k);
+            base = __shfl(base, leader, 64);
+            if (s0 + base < s1) {
+                first = s0 + base;
+                end = s1;
+                q.dry = 0;
+                break;
+            }
+        }
+        ++q.dry;
+        q.seg = q.seg + 1 == kWorkSegments ? 0 : q.seg + 1;
+    }
+    if (!idle || first < 0) return -1;
+    const int r = first + __popcll(want & lanemask_lt());
+    return r < end ? r : -1;
 }
 
 // Persistent-grid size for traversal kernels: blocks_per_cu resident blocks on every CU.
