@@ -322,7 +322,7 @@ def main() -> None:
     ap.add_argument("--schedule", default="regenerate", choices=["regenerate", "passes"],
                     help="regenerate: full wavefront, pixels start their next sample as soon as "
                          "their path ends; passes: the reference's pass-by-pass loop (same image)")
-    ap.add_argument("--lanes", type=int, default=32, choices=[1, 2, 4, 8, 16, 32, 64],
+    ap.add_argument("--lanes", type=int, default=64, choices=[1, 2, 4, 8, 16, 32, 64],
                     help="regenerate: samples of a pixel in flight at once (1 = the pass loop's "
                          "accumulation order bit for bit)")
     ap.add_argument("--wavefront", type=int, default=0,

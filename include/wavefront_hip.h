@@ -320,7 +320,7 @@ typedef struct wf_render_params {
     int32_t adaptive_interval;      /* reference: 16 */
     int32_t schedule;          /* WF_SCHEDULE_*: same image, different order of work */
     int32_t sample_lanes;      /* REGENERATE, non-adaptive: samples of a pixel in flight at once
-                                  (a power of two <= 64; 0 = 32); see
+                                  (a power of two <= 64; 0 = 64); see
                                   wf_engine_config::sample_lanes */
     int32_t wavefront;         /* REGENERATE: ray slots (paths in flight); 0 = automatic */
     wf_shade_params shade;
@@ -377,7 +377,7 @@ typedef struct wf_engine_config {
     int32_t ray_sort;          /* RayQueue Morton sort of every iteration's rays */
     int32_t use_graph;         /* replay each iteration from a captured HIP graph */
     int32_t sample_lanes;      /* samples of a pixel in flight at once (a power of two <= 64;
-                                  0 = 32).
+                                  0 = 64).
                                   Lane l of a non-adaptive job runs samples l, l+L, l+2L, ... into
                                   an accumulator of its own; the lanes are summed in lane order
                                   when the job resolves, so an image depends on L but not on the
@@ -422,7 +422,7 @@ typedef struct wf_engine_stats {
     double phase_ms[5];        /* sort, intersect, schedule, shade, miss (with phase timing) */
 } wf_engine_stats;
 
-/* Device bytes an engine of this capacity, pixel budget and sample lanes (0 = 32) needs
+/* Device bytes an engine of this capacity, pixel budget and sample lanes (0 = 64) needs
  * (admission planning). */
 wf_status wf_engine_bytes(int32_t capacity, int64_t pixel_budget, int32_t sample_lanes,
                           size_t* out);

@@ -1,7 +1,7 @@
 """Device images against a high-spp FP64-accumulated oracle image, statistically: relMSE next to
 the value noise alone explains, and a per-pixel 3-sigma test (tests/stat_image.py; the test's
 power is pinned on the CPU in test_oracle_statistics.py). The device estimate is the mean of 32
-independent renders (seeds 1..32) of the default regenerating schedule (32 sample lanes);
+independent renders (seeds 1..32) of the default regenerating schedule (64 sample lanes);
 the oracle image uses another seed."""
 import numpy as np
 import pytest

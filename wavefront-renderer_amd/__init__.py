@@ -360,7 +360,7 @@ def render_params(width, height, spp, max_depth, camera, policy="none", ray_sort
                   cost_aware_rr=False, firefly=10.0, seed=0, job=0,
                   schedule="regenerate", sample_lanes=0, wavefront=0) -> _abi.RenderParams:
     """Regenerating schedule: sample_lanes = samples of a pixel in flight at once (a power of
-    two <= 64, 0 for the default 32; 1 reproduces the pass loop's accumulation order bit for
+    two <= 64, 0 for the default 64; 1 reproduces the pass loop's accumulation order bit for
     bit); wavefront = ray slots (0 = automatic)."""
     pol = POLICIES[policy] if isinstance(policy, str) else int(policy)
     sched = SCHEDULES[schedule] if isinstance(schedule, str) else int(schedule)

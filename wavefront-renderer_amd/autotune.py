@@ -27,13 +27,13 @@ from typing import Callable, Dict, Iterable, List, Optional
 RENDER_SPACE: Dict[str, List] = {
     "wavefront_per_pixel": [1, 2, 4, 8],
     "sample_lanes": [8, 16, 32, 64],
-    "trace_blocks_per_cu": [4, 6, 8, 10, 12],
+    "trace_blocks_per_cu": [6, 8, 10, 12, 16],
     "radix_bits": [6, 8, 11],
     "refill_lanes": [32, 48, 64],
 }
 ENGINE_SPACE: Dict[str, List] = {
-    "capacity": [1 << 20, 2 << 20, 4 << 20, 8 << 20],
-    "sample_lanes": [4, 8, 16, 32],
+    "capacity": [1 << 20, 2 << 20, 4 << 20, 8 << 20, 16 << 20],
+    "sample_lanes": [8, 16, 32, 64],
     "trace_blocks_per_cu": [4, 6, 8, 10, 12],
     "radix_bits": [6, 8, 11],
     "refill_lanes": [32, 48, 64],
@@ -120,7 +120,7 @@ def tune_render(ctx, scene, params_for: Callable[[int, int], object], pixels: in
         ctx.set_tuning(**_ctx_part(hit["best"]))
         return hit
     base = ctx.tuning()
-    start = {"wavefront_per_pixel": 4, "sample_lanes": 32,
+    start = {"wavefront_per_pixel": 8, "sample_lanes": 64,
              **{k: base[k] for k in CONTEXT_KNOBS}}
 
     def measure(s):
@@ -161,7 +161,7 @@ def tune_engine(ctx, scene, make_jobs: Callable[[], Iterable], pixel_budget: int
         ctx.set_tuning(**_ctx_part(hit["best"]))
         return hit
     base = ctx.tuning()
-    start = {"capacity": 4 << 20, "sample_lanes": 16, **{k: base[k] for k in CONTEXT_KNOBS}}
+    start = {"capacity": 8 << 20, "sample_lanes": 64, **{k: base[k] for k in CONTEXT_KNOBS}}
 
     def run_all(eng):
         ids = [eng.admit(j) for j in make_jobs()]

@@ -204,6 +204,36 @@ __device__ __forceinline__ void block_append_rounds(T* cursor, const bool (&take
     __syncthreads();   // lds and s_base are reused by the next call
 }
 
+// Appends c items per thread (any count) to a list at *cursor with one cursor atomic per block;
+// returns the first slot of this thread's run (runs in thread order). Every thread of the
+// BLOCK-thread block must call it; lds holds at least BLOCK / 64 words owned by the caller.
+template <int BLOCK, typename T>
+__device__ __forceinline__ T block_append_counts(T* cursor, uint32_t c, uint32_t* lds) {
+    constexpr int W = BLOCK / 64;
+    __shared__ T s_base;
+    const int lane = (int)__lane_id(), wave = (int)(threadIdx.x >> 6);
+    uint32_t inc = c;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += y;
+    }
+    if (lane == 63) lds[wave] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int w = 0; w < W; ++w) {
+            const uint32_t x = lds[w];
+            lds[w] = run;
+            run += x;
+        }
+        s_base = run ? atomicAdd(cursor, (T)run) : (T)0;
+    }
+    __syncthreads();
+    const T slot = s_base + (T)(lds[wave] + inc - c);
+    __syncthreads();   // lds and s_base are reused by the next call
+    return slot;
+}
+
 // Element count of a launch: the device counter when present, else the host value.
 __device__ __forceinline__ int load_count(int n, const int32_t* __restrict__ dev) {
     return dev ? *dev : n;

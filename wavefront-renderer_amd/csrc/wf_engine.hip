@@ -68,6 +68,10 @@ enum : int {
 // 64-bit device statistics.
 enum : int { S_RAYS = 0, S_SHADOW, S_HITS, S_MISSES, S_PATHS, S_HEAD, S_TAIL, S_COUNT = 8 };
 
+// Adaptive jobs run their samples in rounds of at most this many lanes (a lane count that divides
+// the convergence-check interval), so the accumulator copies they need stay bounded.
+constexpr int kMaxAdaptiveLanes = 16;
+
 // Host-visible status block, copied back once per iteration: live rays, ready-queue cursors,
 // then the pixels left of every job slot.
 constexpr int kStatusHeader = 8;   // int32: [0] live rays, [2..3] head, [4..5] tail (u64)
@@ -81,7 +85,9 @@ struct EngineView {
     int2* ready;               // (accumulator index, job slot << kTagJobShift | next sample)
     float* accum;              // 3 per pixel and lane (lane-major)
     int32_t* lanes_done;       // per pixel: lanes that ran their last sample
-    float* before;             // 3 per pixel: accum at the start of the pixel's current sample
+    float* before;             // adaptive jobs: 3 per pixel and lane (lane-major, at most
+                               // kMaxAdaptiveLanes lanes), the lane accumulator at the start of
+                               // its current sample
     float* mean;
     float* m2;
     int32_t* pixel_count;      // samples of a retired pixel (written when it retires)
@@ -107,9 +113,12 @@ __global__ void k_admit(EngineView e, int job, int base, int n, int lanes, int q
             e.ready[(e.stats[S_TAIL] + (unsigned long long)l * n + (unsigned long long)k) % e.ring] =
                 make_int2((int)a, (job << kTagJobShift) | l);
     }
-    e.before[3 * (size_t)p] = 0.0f;
-    e.before[3 * (size_t)p + 1] = 0.0f;
-    e.before[3 * (size_t)p + 2] = 0.0f;
+    for (int l = 0; l < lanes && l < kMaxAdaptiveLanes; ++l) {
+        const size_t a = (size_t)p + (size_t)l * (size_t)e.px;
+        e.before[3 * a] = 0.0f;
+        e.before[3 * a + 1] = 0.0f;
+        e.before[3 * a + 2] = 0.0f;
+    }
     e.mean[p] = 0.0f;
     e.m2[p] = 0.0f;
     e.pixel_count[p] = 0;
@@ -142,7 +151,7 @@ __global__ void k_regen(EngineView e, wf_ray_soa r, int32_t* __restrict__ tags) 
     const int j = pt >> kTagJobShift;
     const JobDev& J = e.jobs[j];
     const int s = pt & kTagSampleMask;
-    if (J.adaptive) {   // one lane: a is the pixel
+    if (J.adaptive) {   // the lane's accumulator before this sample
         e.before[3 * (size_t)a] = e.accum[3 * (size_t)a];
         e.before[3 * (size_t)a + 1] = e.accum[3 * (size_t)a + 1];
         e.before[3 * (size_t)a + 2] = e.accum[3 * (size_t)a + 2];
@@ -175,57 +184,89 @@ __global__ void k_regen_commit(EngineView e) {
 constexpr int kFinishBlock = 1024;
 constexpr int kFinishItems = 4;
 
-// A path of accumulator a ended: count the sample, update the adaptive estimate
-// (AdaptiveSampler::addSample/updateConvergence, include/render/AdaptiveSampler.h:21-105, checked
-// when the pixel's sample count is a multiple of the interval, as the pass loop does), then
-// requeue the lane's next sample (returned in *entry), or end the lane; the pixel retires when its
-// last lane ends (or when it converged). Called by every thread of the wave (valid = this thread
-// has an ended path; tag = its job slot and sample index).
-__device__ __forceinline__ bool finish_eval(const EngineView& e, int a, int32_t tag, bool valid,
-                                            int2* entry) {
-    bool requeue = false, retire = false, unconverged = false;
+// A path of accumulator a ended: count the sample, then requeue the lane's next sample or end the
+// lane; the pixel retires when its last lane ends or it converged. Returns the number of ready
+// entries to requeue: entry q is (first.x + q * stride, first.y + q).
+//
+// Non-adaptive lanes are independent: lane l runs samples l, l + L, ... to the budget.
+// Adaptive jobs (AdaptiveSampler::addSample/updateConvergence,
+// include/render/AdaptiveSampler.h:21-105) run their samples in rounds of L: samples rL ..
+// rL + L - 1 on lanes 0 .. L - 1. The lane that ends a round last updates the pixel's running
+// mean and M2 with the round's samples in sample order and tests convergence where the sample
+// count is a multiple of the interval, as the pass loop does (L divides the interval, so such
+// counts only fall on round ends); then the next round is queued, or the pixel retires.
+// Called by every thread of the wave (valid = this thread has an ended path; tag = its job slot
+// and sample index).
+__device__ __forceinline__ int finish_eval(const EngineView& e, int a, int32_t tag, bool valid,
+                                           int2* first, int32_t* stride) {
+    int requeue = 0;
+    bool retire = false, unconverged = false;
     int32_t job = 0, samples = 0;
     if (valid) {
         const int j = tag >> kTagJobShift;
         const JobDev& J = e.jobs[j];
         const int s = tag & kTagSampleMask;
+        const int L = J.lanes;
         const int p = lane_pixel(J, a, s);
-        bool converged = false;
-        if (J.adaptive) {   // one lane: a == p, and s + 1 samples have run
-            const int cnt = s + 1;
-            const size_t b = 3 * (size_t)p;
-            const V3 c(e.accum[b] - e.before[b], e.accum[b + 1] - e.before[b + 1],
-                       e.accum[b + 2] - e.before[b + 2]);
-            const float x = luminance(c);
-            const float mb = e.mean[p];
-            const float dev = x - mb;
-            const float mn = mb + dev / (float)cnt;
-            e.mean[p] = mn;
-            const float dev2 = x - mn;
-            const float m2 = e.m2[p] + dev * dev2;
-            e.m2[p] = m2;
-            if (cnt % J.adaptive_interval == 0 && cnt >= J.adaptive_min_samples) {
-                const float var = m2 / (float)(cnt - 1);
-                const float se = __builtin_sqrtf(var / (float)cnt);
-                converged = se < J.adaptive_threshold * (mn + 0.01f);
+        if (J.adaptive) {
+            const int r0 = s - (s & (L - 1));              // first sample of the round
+            const int round_n = min(L, J.spp - r0);
+            const bool last = L == 1 || atomicAdd(&e.lanes_done[p], 1) == round_n - 1;
+            if (last) {
+                if (L > 1) e.lanes_done[p] = 0;
+                float mn = e.mean[p], m2 = e.m2[p];
+                bool converged = false;
+                int cnt = r0;
+                for (int q = 0; q < round_n; ++q) {
+                    const size_t b = 3 * ((size_t)p + (size_t)q * (size_t)J.lane_stride);
+                    if (converged) {   // samples past the converged count are taken back
+                        e.accum[b] = e.before[b];
+                        e.accum[b + 1] = e.before[b + 1];
+                        e.accum[b + 2] = e.before[b + 2];
+                        continue;
+                    }
+                    cnt = r0 + q + 1;
+                    const V3 c(e.accum[b] - e.before[b], e.accum[b + 1] - e.before[b + 1],
+                               e.accum[b + 2] - e.before[b + 2]);
+                    const float x = luminance(c);
+                    const float dev = x - mn;
+                    mn = mn + dev / (float)cnt;
+                    const float dev2 = x - mn;
+                    m2 = m2 + dev * dev2;
+                    if (cnt % J.adaptive_interval == 0 && cnt >= J.adaptive_min_samples) {
+                        const float var = m2 / (float)(cnt - 1);
+                        const float se = __builtin_sqrtf(var / (float)cnt);
+                        converged = se < J.adaptive_threshold * (mn + 0.01f);
+                    }
+                }
+                e.mean[p] = mn;
+                e.m2[p] = m2;
+                const int next = r0 + L;
+                if (converged || next >= J.spp) {
+                    e.pixel_count[p] = cnt;
+                    retire = true;
+                    unconverged = !converged;
+                    job = j;
+                    samples = cnt;
+                } else {
+                    requeue = min(L, J.spp - next);
+                    *first = make_int2(p, (j << kTagJobShift) | next);
+                    *stride = J.lane_stride;
+                }
             }
-        }
-        const int ns = s + J.lanes;
-        if (converged) {
-            e.pixel_count[p] = s + 1;
-            retire = true;
-            job = j;
-            samples = s + 1;
-        } else if (ns < J.spp) {
-            requeue = true;
-            *entry = make_int2(a, (j << kTagJobShift) | ns);
-        } else if (J.lanes == 1 || atomicAdd(&e.lanes_done[p], 1) == J.lanes - 1) {
-            // the pixel's last lane ran its last sample: all spp samples are in
-            e.pixel_count[p] = J.spp;
-            retire = true;
-            unconverged = J.adaptive != 0;
-            job = j;
-            samples = J.spp;
+        } else {
+            const int ns = s + L;
+            if (ns < J.spp) {
+                requeue = 1;
+                *first = make_int2(a, (j << kTagJobShift) | ns);
+                *stride = 0;
+            } else if (L == 1 || atomicAdd(&e.lanes_done[p], 1) == L - 1) {
+                // the pixel's last lane ran its last sample: all spp samples are in
+                e.pixel_count[p] = J.spp;
+                retire = true;
+                job = j;
+                samples = J.spp;
+            }
         }
     }
     // retired pixels: one set of job-counter atomics per wave for the lanes sharing the first
@@ -253,53 +294,61 @@ __device__ __forceinline__ bool finish_eval(const EngineView& e, int a, int32_t 
     return requeue;
 }
 
-// Requeues of kFinishItems rounds, appended to the ready ring with one cursor atomic per block.
-__device__ __forceinline__ void finish_requeue(const EngineView& e, const bool (&rq)[kFinishItems],
-                                               const int2 (&entry)[kFinishItems], uint32_t* lds) {
-    unsigned long long pos[kFinishItems];
-    block_append_rounds<kFinishItems, kFinishBlock>(&e.stats[S_TAIL], rq, pos, lds);
+// Requeues of the kFinishItems paths of every thread, appended to the ready ring with one cursor
+// atomic per block.
+__device__ __forceinline__ void finish_requeue(const EngineView& e, const int (&cnt)[kFinishItems],
+                                               const int2 (&first)[kFinishItems],
+                                               const int32_t (&stride)[kFinishItems],
+                                               uint32_t* lds) {
+    uint32_t total = 0;
+#pragma unroll
+    for (int j = 0; j < kFinishItems; ++j) total += (uint32_t)cnt[j];
+    unsigned long long pos = block_append_counts<kFinishBlock>(&e.stats[S_TAIL], total, lds);
 #pragma unroll
     for (int j = 0; j < kFinishItems; ++j)
-        if (rq[j]) e.ready[pos[j] % e.ring] = entry[j];
+        for (int q = 0; q < cnt[j]; ++q)
+            e.ready[pos++ % e.ring] = make_int2(first[j].x + q * stride[j], first[j].y + q);
 }
 
 __global__ __launch_bounds__(kFinishBlock) void k_finish_hits(EngineView e,
                                                               const int32_t* __restrict__ pixel,
                                                               const int32_t* __restrict__ tags,
                                                               const uint32_t* __restrict__ alive) {
-    __shared__ uint32_t lds[kFinishItems * kFinishBlock / 64];
+    __shared__ uint32_t lds[kFinishBlock / 64];
     const int n = e.counters[C_HITS];
     const int base = blockIdx.x * kFinishBlock * kFinishItems;
     if (base >= n) return;
-    bool rq[kFinishItems];
-    int2 entry[kFinishItems];
+    int cnt[kFinishItems];
+    int2 first[kFinishItems];
+    int32_t stride[kFinishItems];
 #pragma unroll
     for (int j = 0; j < kFinishItems; ++j) {
         const int i = base + j * kFinishBlock + (int)threadIdx.x;
         const bool valid = i < n && !alive[i];
-        entry[j] = make_int2(0, 0);
-        rq[j] = finish_eval(e, valid ? pixel[i] : 0, valid ? tags[i] : 0, valid, &entry[j]);
+        cnt[j] = finish_eval(e, valid ? pixel[i] : 0, valid ? tags[i] : 0, valid, &first[j],
+                             &stride[j]);
     }
-    finish_requeue(e, rq, entry, lds);
+    finish_requeue(e, cnt, first, stride, lds);
 }
 
 __global__ __launch_bounds__(kFinishBlock) void k_finish_misses(EngineView e,
                                                                 const int32_t* __restrict__ pixel,
                                                                 const int32_t* __restrict__ tags) {
-    __shared__ uint32_t lds[kFinishItems * kFinishBlock / 64];
+    __shared__ uint32_t lds[kFinishBlock / 64];
     const int n = e.counters[C_MISSES];
     const int base = blockIdx.x * kFinishBlock * kFinishItems;
     if (base >= n) return;
-    bool rq[kFinishItems];
-    int2 entry[kFinishItems];
+    int cnt[kFinishItems];
+    int2 first[kFinishItems];
+    int32_t stride[kFinishItems];
 #pragma unroll
     for (int j = 0; j < kFinishItems; ++j) {
         const int i = base + j * kFinishBlock + (int)threadIdx.x;
         const bool valid = i < n;
-        entry[j] = make_int2(0, 0);
-        rq[j] = finish_eval(e, valid ? pixel[i] : 0, valid ? tags[i] : 0, valid, &entry[j]);
+        cnt[j] = finish_eval(e, valid ? pixel[i] : 0, valid ? tags[i] : 0, valid, &first[j],
+                             &stride[j]);
     }
-    finish_requeue(e, rq, entry, lds);
+    finish_requeue(e, cnt, first, stride, lds);
 }
 
 __global__ void k_iteration_end(EngineView e) {
@@ -435,9 +484,10 @@ constexpr size_t kBytesPerSlot = 48 * 3 + 104 * 2 + 4 * 5 + 400;   // queues + t
 constexpr size_t kBytesPerPixel = 12 + 12 + 4 + 4 + 4 + 4 + 8;   // accum, start copy, mean, M2,
                                                                   // count, lanes done, queue
 constexpr size_t kBytesPerLane = 12 + 8;                          // accum, queue
-constexpr int kDefaultLanes = 32;
-constexpr int kAutoPathsPerPixel = 4;
-constexpr int64_t kAutoWavefront = 1 << 23;
+constexpr size_t kBytesPerAdaptiveLane = 12;                      // start copy
+constexpr int kDefaultLanes = 64;
+constexpr int kAutoPathsPerPixel = 8;
+constexpr int64_t kAutoWavefront = 1 << 24;
 constexpr int kMaxLanes = 64;
 
 int lanes_of(int32_t sample_lanes) { return sample_lanes == 0 ? kDefaultLanes : sample_lanes; }
@@ -659,7 +709,7 @@ wf_status wf_engine_create(wf_context* ctx, wf_scene* scene, const wf_engine_con
     ETRY(engine_array(e, px * (size_t)e->lanes, &v.ready));
     ETRY(engine_array(e, 3 * px * (size_t)e->lanes, &v.accum));
     ETRY(engine_array(e, px, &v.lanes_done));
-    ETRY(engine_array(e, 3 * px, &v.before));
+    ETRY(engine_array(e, 3 * px * (size_t)std::min(e->lanes, kMaxAdaptiveLanes), &v.before));
     ETRY(engine_array(e, px, &v.mean));
     ETRY(engine_array(e, px, &v.m2));
     ETRY(engine_array(e, px, &v.pixel_count));
@@ -714,8 +764,10 @@ wf_status wf_engine_bytes(int32_t capacity, int64_t pixel_budget, int32_t sample
     WF_REQUIRE(out && capacity >= 0 && pixel_budget >= 0, "wf_engine_bytes: bad argument");
     WF_REQUIRE(lanes_ok(sample_lanes), "wf_engine_bytes: sample_lanes %d is not 0 or a power of two <= 64",
                sample_lanes);
+    const int lanes = lanes_of(sample_lanes);
     *out = kBytesPerSlot * (size_t)capacity +
-           (kBytesPerPixel + kBytesPerLane * (size_t)(lanes_of(sample_lanes) - 1)) *
+           (kBytesPerPixel + kBytesPerLane * (size_t)(lanes - 1) +
+            kBytesPerAdaptiveLane * (size_t)(std::min(lanes, kMaxAdaptiveLanes) - 1)) *
                (size_t)pixel_budget;
     return WF_OK;
 }
@@ -774,8 +826,14 @@ wf_status wf_engine_admit(wf_engine* e, const wf_job_desc* job, int32_t* job_id)
     J.adaptive_interval = job->adaptive ? job->adaptive_interval : 1;
     int lanes = 1;
     const int most = std::min(e->lanes, lanes_of(e->cfg.sample_lanes));
-    if (!job->adaptive)
+    if (!job->adaptive) {
         while (lanes * 2 <= most && lanes * 2 <= job->spp) lanes *= 2;
+    } else {
+        // rounds of lanes that divide the convergence-check interval
+        while (lanes * 2 <= std::min(most, kMaxAdaptiveLanes) && lanes * 2 <= job->spp &&
+               job->adaptive_interval % (lanes * 2) == 0)
+            lanes *= 2;
+    }
     J.lanes = lanes;
     J.lane_stride = e->px;
     const CamBasis cb = camera_basis(job->camera, job->width, job->height);
@@ -992,7 +1050,7 @@ wf_status render_regenerate(wf_scene* s, const wf_render_params* rp, float* imag
                rp->sample_lanes);
     WF_REQUIRE(rp->wavefront >= 0 && rp->wavefront < (1 << 29), "wf_render: wavefront %d out of range",
                rp->wavefront);
-    const int lanes = rp->adaptive ? 1 : lanes_of(rp->sample_lanes);
+    const int lanes = lanes_of(rp->sample_lanes);
     // automatic wavefront: up to kAutoPathsPerPixel paths per pixel, at most kAutoWavefront rays
     // (larger launches amortise the per-iteration fixed costs and the traversal tail)
     const int slots =
