@@ -44,7 +44,25 @@ def workload(name: str, wf, scenes, quick: bool):
     if name == "cornell_box":
         data = scenes.cornell_box()
         return data, 256, 256, 16, 8, "cornell_box: 5132 tris, 256x256, 16 spp, depth 8"
+    if name == "sphere_cloud":
+        data = scenes.sphere_cloud(copies=256 if quick else 4096)
+        W, H = (480, 270) if quick else (1920, 1080)
+        spp = 64 if quick else 1024
+        desc = (f"sphere_cloud: {data.triangle_count} tris (noise-displaced spheres), 256 "
+                f"materials, 1024 emissive triangles, 2048x1024 HDRI, {W}x{H}, adaptive 16..{spp} "
+                f"spp to relative std-err 0.01, depth 12, costBenefit")
+        return data, W, H, spp, 12, desc
     raise SystemExit(f"unknown workload {name}")
+
+
+# Per-workload render settings beyond size and depth: physics, policy, adaptive sampling.
+def workload_settings(name: str, wf) -> dict:
+    if name == "sphere_cloud":
+        return {"physics": wf.PHYSICS_UNBIASED, "policy": "costBenefit", "adaptive": True,
+                "adaptive_threshold": 0.01, "adaptive_min_samples": 16, "adaptive_interval": 16}
+    if name == "sorting_stress":
+        return {"physics": wf.PHYSICS_UNBIASED, "policy": "none"}
+    return {"physics": wf.PHYSICS_REFERENCE, "policy": "none"}
 
 
 # Kernels whose bound is HBM bandwidth (sort, compaction, gathers, shading); traversal and the
@@ -318,7 +336,8 @@ def main() -> None:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="sorting_stress")
-    ap.add_argument("--policy", default="none")
+    ap.add_argument("--policy", default=None,
+                    help="shading-queue policy (default: the workload's own)")
     ap.add_argument("--schedule", default="regenerate", choices=["regenerate", "passes"],
                     help="regenerate: full wavefront, pixels start their next sample as soon as "
                          "their path ends; passes: the reference's pass-by-pass loop (same image)")
@@ -385,13 +404,18 @@ def main() -> None:
         f"(device build {scene.stats.build_ms:.1f} ms, binary depth {scene.stats.max_depth}, "
         f"{scene.stats.bvh_width}-wide nodes {scene.stats.node_count}, stack bound "
         f"{scene.stats.stack_bound})")
-    physics = wf.PHYSICS_UNBIASED if args.workload == "sorting_stress" else wf.PHYSICS_REFERENCE
+    ws = workload_settings(args.workload, wf)
+    physics = ws["physics"]
+    if args.policy is None:
+        args.policy = ws["policy"]
+    adaptive = {k: ws[k] for k in ("adaptive", "adaptive_threshold", "adaptive_min_samples",
+                                   "adaptive_interval") if k in ws}
 
     def params(policy, seed=1, w=W, h=H, s=spp, schedule=args.schedule):
         return wf.render_params(width=w, height=h, spp=s, max_depth=depth,
                                 camera=data.camera, policy=policy, physics=physics,
                                 firefly=1e30, seed=seed, schedule=schedule,
-                                sample_lanes=args.lanes, wavefront=args.wavefront)
+                                sample_lanes=args.lanes, wavefront=args.wavefront, **adaptive)
 
     tuned = None
     if args.autotune:
@@ -401,7 +425,8 @@ def main() -> None:
         def params_for(lanes, wave):
             return wf.render_params(width=W, height=H, spp=spp, max_depth=depth,
                                     camera=data.camera, policy=args.policy, physics=physics,
-                                    firefly=1e30, seed=7, sample_lanes=lanes, wavefront=wave)
+                                    firefly=1e30, seed=7, sample_lanes=lanes, wavefront=wave,
+                                    **adaptive)
         tuned = autotune.tune_render(ctx, scene, params_for, W * H, rounds=1, repeats=1)
         args.lanes = tuned["best"]["sample_lanes"]
         args.wavefront = tuned["best"]["wavefront_per_pixel"] * W * H

@@ -11,21 +11,28 @@ import stat_image
 pytestmark = pytest.mark.gpu
 
 CASES = {
-    # scene, physics, firefly threshold, oracle samples
-    "cornell_reference": ("cornell", 0, 10.0, 4096),
-    "stress_unbiased": ("stress", 1, 1e30, 2048),
+    # scene, physics, firefly threshold, oracle samples, policy
+    "cornell_reference": ("cornell", 0, 10.0, 4096, "none"),
+    "stress_unbiased": ("stress", 1, 1e30, 2048, "none"),
+    # an HDRI environment (alias-table NEE, MIS against BSDF hits of the sky) beside the area
+    # lights, cost-benefit shading order
+    "stress_env_costbenefit": ("stress_env", 1, 1e30, 2048, "costBenefit"),
 }
 
 
 @pytest.mark.parametrize("case", list(CASES))
 def test_image_statistics_vs_fp64_oracle(wf, ctx, scenes, orc, case):
-    name, physics, firefly, ref_spp = CASES[case]
-    data = scenes.cornell_box() if name == "cornell" else scenes.sorting_stress(subdiv=5,
-                                                                                tex_size=64)
+    name, physics, firefly, ref_spp, policy = CASES[case]
+    if name == "cornell":
+        data = scenes.cornell_box()
+    else:
+        data = scenes.sorting_stress(subdiv=5, tex_size=64)
+        if name == "stress_env":
+            data.environment = scenes.hdri_sky(64, 32, peak_over_mean=20.0)
     scene = wf.Scene(ctx, data)
     osc = orc.scene(data)
     kw = dict(width=24, height=18, max_depth=12, camera=data.camera, firefly=firefly,
-              physics=physics)
+              physics=physics, policy=policy)
     _, _, ref, _ = osc.render(wf.render_params(spp=ref_spp, seed=777, **kw), fp64=True,
                               threads=16)
     spp = 64

@@ -546,7 +546,7 @@ wf_status enqueue_iteration(wf_engine* e) {
     const wf_shade_soa* q = &e->s0;
     const int32_t* qt = e->st0;
     if (e->cfg.policy != WF_POLICY_NONE) {
-        WF_TRY(shading_schedule(ctx, s, &e->s0, hits, e->cfg.policy, e->idx));
+        WF_TRY(shading_schedule(ctx, s, &e->s0, hits, e->cfg.policy, e->idx, false));
         WF_TRY(gather_shading(ctx, &e->s0, e->idx, hits, &e->s1, e->st0, e->st1));
         q = &e->s1;
         qt = e->st1;

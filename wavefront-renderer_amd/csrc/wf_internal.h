@@ -327,8 +327,12 @@ wf_status intersect_split(wf_context* ctx, const wf_scene* s, const wf_ray_soa* 
                           int32_t* miss_tags = nullptr, bool unordered = false);
 wf_status ray_schedule(wf_context* ctx, const wf_ray_soa* rays, Count n, const float bmin[3],
                        const float bmax[3], int32_t* sorted, uint64_t* keys_out);
+// ShadingQueue::schedule. spatial = false (engine use): the cost-benefit key keeps only the
+// (material type, priority rank) group and the stable sort keeps the input order inside each
+// group (the rays were Morton-sorted before tracing, so hits arrive in spatial order); the
+// C-ABI keeps the reference's full key.
 wf_status shading_schedule(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q, Count n,
-                           int32_t policy, int32_t* sorted);
+                           int32_t policy, int32_t* sorted, bool spatial = true);
 
 // Job context of a tagged wavefront for the shading kernel: per-entry tags plus the job table.
 struct ShadeJobs {
@@ -362,7 +366,7 @@ wf_status collapse_greedy(wf_context* ctx, int n, const int32_t* left, const int
 wf_status ray_keys(wf_context* ctx, const wf_ray_soa* rays, Count n, const float bmin[3],
                    const float bmax[3], uint64_t* keys);
 wf_status shading_keys(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q, Count n,
-                       int32_t policy, uint64_t* keys, int32_t* key_bits);
+                       int32_t policy, bool spatial, uint64_t* keys, int32_t* key_bits);
 wf_status gather_rays(wf_context* ctx, const wf_ray_soa* src, const int32_t* idx, Count n,
                       wf_ray_soa* dst, const int32_t* tags_in = nullptr,
                       int32_t* tags_out = nullptr);

@@ -245,6 +245,7 @@ __global__ __launch_bounds__(kCbBlock) void k_cb_partials(
     double* __restrict__ partials, uint32_t* __restrict__ present,
     uint32_t* __restrict__ bounds) {
     const int n = load_count(n_host, nd);
+    if ((int)blockIdx.x * kCbTile >= n) return;   // the priority sums stop at the last used tile
     extern __shared__ double cb_lds[];
     double* wsum = cb_lds;                                        // [wave][n_mat]
     uint32_t* pres = (uint32_t*)(cb_lds + (kCbBlock / 64) * n_mat);
@@ -292,7 +293,7 @@ __global__ __launch_bounds__(kCbBlock) void k_cb_partials(
     for (int m = threadIdx.x; m < n_mat; m += kCbBlock) {
         double acc = 0.0;
         for (int w = 0; w < kCbBlock / 64; ++w) acc += wsum[w * n_mat + m];
-        partials[(size_t)blockIdx.x * n_mat + m] = acc;
+        partials[(size_t)m * gridDim.x + blockIdx.x] = acc;   // material-major
         if (pres[m]) atomicOr(&present[m], 1u);
     }
     // hit-point bounds: wave reduction, block reduction, one atomic per bound per block
@@ -323,15 +324,17 @@ __global__ __launch_bounds__(kCbBlock) void k_cb_partials(
 // One block per material: thread t sums tiles t, t + 256, ... in order, then a fixed pairwise
 // tree over the threads (the same association on every run).
 __global__ __launch_bounds__(256) void k_cb_priority(const double* __restrict__ partials,
-                                                     int tiles, int n_mat,
+                                                     int tiles, int n_host,
+                                                     const int32_t* __restrict__ nd, int n_mat,
                                                      const DevMaterial* __restrict__ mats,
                                                      const double* __restrict__ mat_cost,
                                                      const double* __restrict__ tex_cost,
                                                      double* __restrict__ prio) {
     __shared__ double red[256];
     const int m = blockIdx.x;
+    const int used = (load_count(n_host, nd) + kCbTile - 1) / kCbTile;   // tiles with entries
     double acc = 0.0;
-    for (int t = threadIdx.x; t < tiles; t += 256) acc += partials[(size_t)t * n_mat + m];
+    for (int t = threadIdx.x; t < used; t += 256) acc += partials[(size_t)m * tiles + t];
     red[threadIdx.x] = acc;
     __syncthreads();
     for (int w = 128; w >= 1; w >>= 1) {
@@ -369,9 +372,15 @@ __global__ void k_cb_keys(const int32_t* __restrict__ mat, const float* __restri
                           const float* __restrict__ py, const float* __restrict__ pz, int n_host,
                           const int32_t* __restrict__ nd, const DevMaterial* __restrict__ mats,
                           const int32_t* __restrict__ rank, int rank_bits,
-                          const uint32_t* __restrict__ bounds, uint64_t* __restrict__ keys) {
+                          const uint32_t* __restrict__ bounds, int spatial,
+                          uint64_t* __restrict__ keys) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= load_count(n_host, nd)) return;
+    if (!spatial) {   // group key only: a stable sort keeps the input order inside each group
+        const int m = mat[i];
+        keys[i] = ((uint64_t)(mats[m].type & 7) << rank_bits) | (uint64_t)rank[m];
+        return;
+    }
     const float lx = o2f(bounds[0]), ly = o2f(bounds[1]), lz = o2f(bounds[2]);
     const float hx = o2f(bounds[3]), hy = o2f(bounds[4]), hz = o2f(bounds[5]);
     const uint32_t morton = spread_bits10(quantise_hit(px[i], lx, hx)) |
@@ -389,7 +398,7 @@ static int bits_for(int max_value) {
 }
 
 wf_status shading_keys(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q, Count n,
-                       int32_t policy, uint64_t* keys, int32_t* key_bits) {
+                       int32_t policy, bool spatial, uint64_t* keys, int32_t* key_bits) {
     const SceneDev& d = s->dev;
     if (policy == WF_POLICY_MATERIAL || policy == WF_POLICY_TEXTURE) {
         void* p = nullptr;
@@ -437,8 +446,9 @@ wf_status shading_keys(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q
                                                            (double*)pp, (uint32_t*)ppres,
                                                            (uint32_t*)pb);
         WF_HIP(hipGetLastError());
-        k_cb_priority<<<nm, 256, 0, ctx->stream>>>(
-            (double*)pp, tiles, nm, d.materials, d.mat_rel_cost, d.tex_rel_cost, (double*)pprio);
+        k_cb_priority<<<nm, 256, 0, ctx->stream>>>((double*)pp, tiles, n.n, n.dev, nm,
+                                                    d.materials, d.mat_rel_cost, d.tex_rel_cost,
+                                                    (double*)pprio);
         WF_HIP(hipGetLastError());
         k_cb_rank<<<grid_for(nm, 256), 256, 0, ctx->stream>>>((double*)pprio, (uint32_t*)ppres,
                                                               nm, (int32_t*)prank);
@@ -447,23 +457,24 @@ wf_status shading_keys(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q
         k_cb_keys<<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(q->mat, q->px, q->py, q->pz, n.n,
                                                                n.dev, d.materials,
                                                                (int32_t*)prank, rb,
-                                                               (uint32_t*)pb, keys);
+                                                               (uint32_t*)pb, spatial ? 1 : 0,
+                                                               keys);
         WF_HIP(hipGetLastError());
-        *key_bits = 3 + rb + 30;
+        *key_bits = 3 + rb + (spatial ? 30 : 0);
         return WF_OK;
     }
     return set_error(WF_ERR_INVALID_ARGUMENT, "unknown scheduling policy %d", policy);
 }
 
 wf_status shading_schedule(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q, Count n,
-                           int32_t policy, int32_t* sorted) {
+                           int32_t policy, int32_t* sorted, bool spatial) {
     if (n.n <= 0) return WF_OK;
     if (policy == WF_POLICY_NONE) return iota_count(ctx, sorted, n);
     void *pk, *pk2;
     WF_TRY(workspace(ctx, "sched_keys", sizeof(uint64_t) * n.n, &pk));
     WF_TRY(workspace(ctx, "sched_keys2", sizeof(uint64_t) * n.n, &pk2));
     int32_t bits = 0;
-    WF_TRY(shading_keys(ctx, s, q, n, policy, (uint64_t*)pk, &bits));
+    WF_TRY(shading_keys(ctx, s, q, n, policy, spatial, (uint64_t*)pk, &bits));
     return radix_sort_pairs(ctx, (uint64_t*)pk, nullptr, (uint64_t*)pk2, sorted, n, bits,
                             "shading_sort");
 }

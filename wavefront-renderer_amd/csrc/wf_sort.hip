@@ -133,10 +133,12 @@ __global__ __launch_bounds__(kSortBlock) void k_sweep_pass(
     __shared__ uint32_t scan_part[kSortWaves];
     __shared__ int tile_s;
     const int n = load_count(n_host, nd);
+    // launches over a device count cover the capacity: exactly the blocks that have a tile claim
+    // one (an idle block would still pay a serialised counter atomic)
+    if ((int)blockIdx.x * kSortTile >= n) return;
     if (threadIdx.x == 0) tile_s = atomicAdd(tile_counter, 1);
     __syncthreads();
     const int tile = tile_s;
-    if (tile * kSortTile >= n) return;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < kSortWaves * R; i += kSortBlock) (&wave_cnt[0][0])[i] = 0;
