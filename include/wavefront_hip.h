@@ -366,7 +366,7 @@ wf_status wf_render(wf_scene* scene, const wf_render_params* params, float* imag
  * An engine keeps one scene's wavefront resident: a ray queue of `capacity` slots shared by the
  * paths of every admitted job, and framebuffers for up to `pixel_budget` pixels. Jobs are
  * admitted while a job slot and a contiguous framebuffer range are free; each job's image is
- * identical to wf_render of the same job with the same seed and job tag. */
+ * identical to wf_render of the same job with the same seed, job tag and sample lanes. */
 typedef struct wf_engine wf_engine;
 
 typedef struct wf_engine_config {

@@ -13,7 +13,8 @@ scene's wavefront while other requests run. Reports jobs/s, Mrays/s and p50/p99
 time-to-converged-image, next to the same requests served by sequential wf_render calls (their
 measured service times through a first-come-first-served queue with the same arrivals).
 
-Every image is identical to the job rendered alone (tests/test_gpu_engine.py).
+Every image is identical to the job rendered alone with the same sample lanes
+(tests/test_gpu_engine.py).
 """
 from __future__ import annotations
 
