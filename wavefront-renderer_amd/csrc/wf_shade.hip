@@ -694,9 +694,11 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(const SceneDev s, const 
                                                         int32_t* __restrict__ counter,
                                                         int refill_lanes) {
     __shared__ int32_t stack[lds_entries<STACK>() * kTraceBlock];
+    __shared__ int4 xchg[kTraceBlock];   // per wave: 64 candidate-test slots
     using W = Walker<true, STACK>;
     volatile int32_t spill[W::X > 0 ? W::X : 1];
     const int n = load_count(count_host, count_dev);
+    int4* wx = xchg + (threadIdx.x & ~63u);
     const V3 tl = V3(s.directional.direction[0], s.directional.direction[1],
                      s.directional.direction[2]) * -1.0f;
     W w;
@@ -735,8 +737,11 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(const SceneDev s, const 
             if (wq.exhausted()) break;
             continue;
         }
+        // every lane of the wave takes part in the step (idle lanes test others' triangles)
+        const bool walked = w.step_wave(item >= 0 && phase != 2 && s.n_tris > 0, nodes, s.tri,
+                                        stack, spill, wx);
         if (item < 0) continue;
-        if (phase != 2 && (s.n_tris <= 0 || w.step(nodes, s.tri, stack, spill))) {
+        if (phase != 2 && (s.n_tris <= 0 || walked)) {
             const bool occ = w.h.leaf >= 0;
             if (phase == 0) {
                 occ_area = occ;

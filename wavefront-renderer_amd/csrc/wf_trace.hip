@@ -25,9 +25,11 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(
     float* __restrict__ out_v, uint32_t* __restrict__ out_flag, int32_t* __restrict__ counter,
     int refill_lanes) {
     __shared__ int32_t stack[lds_entries<STACK>() * kTraceBlock];
+    __shared__ int4 xchg[kTraceBlock];   // per wave: 64 candidate-test slots
     using W = Walker<false, STACK>;
     volatile int32_t spill[W::X > 0 ? W::X : 1];
     const int n = load_count(n_host, nd);
+    int4* wx = xchg + (threadIdx.x & ~63u);
     W w;
     int ray = -1;
     WorkQueue wq;
@@ -42,7 +44,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(
             if (wq.exhausted()) break;
             continue;
         }
-        if (ray >= 0 && (n_tris <= 0 || w.step(nodes, tri, stack, spill))) {
+        // every lane of the wave takes part in the step (idle lanes test others' triangles)
+        const bool done = w.step_wave(ray >= 0 && n_tris > 0, nodes, tri, stack, spill, wx);
+        if (ray >= 0 && (n_tris <= 0 || done)) {
             out_leaf[ray] = w.h.leaf;
             out_t[ray] = w.h.t;
             out_u[ray] = w.h.u;

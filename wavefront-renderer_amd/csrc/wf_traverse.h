@@ -247,6 +247,165 @@ struct Walker {
         return false;
     }
 
+    // Wave-cooperative step over a four-wide node, called by all 64 lanes of the wave together
+    // (active = this lane has a walk in progress). The box tests are per lane as in step(); the
+    // candidate triangles of the whole wave are then compacted into consecutive slots and tested
+    // one (ray, triangle) pair per lane, 64 pairs per round, instead of four masked tests per
+    // lane: a lane with no candidate helps test the others'. xchg: this wave's 64 LDS slots.
+    // Results are the same as step(): every candidate is tested with the same operations and
+    // the closest hit is the (t, triangle) minimum.
+    __device__ __forceinline__ bool step_wave(bool active, const Bvh4Node* __restrict__ nodes,
+                                              const float4* __restrict__ tri,
+                                              int32_t* __restrict__ stack, volatile int32_t* spill,
+                                              int4* __restrict__ xchg) {
+        const int lane = threadIdx.x;
+        const int wl = (int)__lane_id();
+        const float kInf = __int_as_float(0x7F800000);
+        int c[4] = {kEmptyChild, kEmptyChild, kEmptyChild, kEmptyChild};
+        float dist[4] = {kInf, kInf, kInf, kInf};
+        uint32_t lm = 0;   // leaf children whose box the ray enters
+        if (active) {
+            if (STATS) ++h.nodes;
+            const float4* np = reinterpret_cast<const float4*>(nodes + node);
+            const float4 lox = np[0], loy = np[1], loz = np[2];
+            const float4 hix = np[3], hiy = np[4], hiz = np[5];
+            const int4 ch = *reinterpret_cast<const int4*>(np + 6);
+            const float lx[4] = {lox.x, lox.y, lox.z, lox.w};
+            const float ly[4] = {loy.x, loy.y, loy.z, loy.w};
+            const float lz[4] = {loz.x, loz.y, loz.z, loz.w};
+            const float hx[4] = {hix.x, hix.y, hix.z, hix.w};
+            const float hy[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
+            const float hz[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
+            c[0] = ch.x; c[1] = ch.y; c[2] = ch.z; c[3] = ch.w;
+            const float tcur = ANY ? tfar : h.t;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                float t0;
+                const bool box = slab(lx[k], ly[k], lz[k], hx[k], hy[k], hz[k], o, inv, tcur, t0) &&
+                                 c[k] != kEmptyChild;
+                if (box && c[k] < 0) lm |= 1u << k;
+                dist[k] = (box && c[k] >= 0) ? t0 : kInf;
+            }
+        }
+        // slots of this lane's candidates: [off, off + nc) of T in the wave
+        const int nc = __popc(lm);
+        const uint64_t lt = lanemask_lt();
+        const uint64_t b0 = __ballot(nc & 1), b1 = __ballot(nc & 2), b2 = __ballot(nc & 4);
+        const int off = __popcll(b0 & lt) + 2 * __popcll(b1 & lt) + 4 * __popcll(b2 & lt);
+        const int T = __popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2);
+        if (T > 0) {
+            int cand[4];
+            {
+                uint32_t m = lm;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int k = m ? __ffs(m) - 1 : 0;
+                    cand[q] = ~(k == 0 ? c[0] : (k == 1 ? c[1] : (k == 2 ? c[2] : c[3])));
+                    m &= m - 1u;
+                }
+            }
+            for (int base = 0; base < T; base += 64) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int sl = off + q - base;
+                    if (q < nc && sl >= 0 && sl < 64) xchg[sl] = make_int4(wl, cand[q], 0, 0);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                const bool work = base + wl < T;
+                const int4 job = work ? xchg[wl] : make_int4(wl, 0, 0, 0);
+                // the owner's ray
+                const V3 ro(__shfl(o.x, job.x, 64), __shfl(o.y, job.x, 64), __shfl(o.z, job.x, 64));
+                const V3 rd(__shfl(d.x, job.x, 64), __shfl(d.y, job.x, 64), __shfl(d.z, job.x, 64));
+                const float r0 = __shfl(tnear, job.x, 64), r1 = __shfl(tfar, job.x, 64);
+                int4 res = make_int4(-1, 0, 0, 0);
+                if (work) {
+                    const size_t li = 3 * (size_t)job.y;
+                    const float4 A = tri[li], B = tri[li + 1], C = tri[li + 2];
+                    float t, u, v;
+                    if (mt_all(ro, rd, xyz(A), xyz(B), xyz(C), r0, r1, t, u, v))
+                        res = make_int4(__float_as_int(A.w), __float_as_int(t), __float_as_int(u),
+                                        __float_as_int(v));
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                if (work) xchg[wl] = res;
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int sl = off + q - base;
+                    if (!(q < nc && sl >= 0 && sl < 64)) continue;
+                    if (STATS) ++h.tris;
+                    const int4 r = xchg[sl];
+                    if (r.x < 0) continue;
+                    const float t = __int_as_float(r.y);
+                    if (ANY) {
+                        if (h.leaf < 0) {
+                            h.t = t;
+                            h.leaf = cand[q];
+                            h.g = r.x;
+                        }
+                    } else if (h.leaf < 0 || t < h.t || (t == h.t && r.x < h.g)) {
+                        h.t = t;
+                        h.leaf = cand[q];
+                        h.g = r.x;
+                        h.u = __int_as_float(r.z);
+                        h.v = __int_as_float(r.w);
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+        if (!active) return false;
+        if (ANY && h.leaf >= 0) return true;
+        auto cswap = [&](int x, int y) {
+            if (dist[y] < dist[x]) {
+                const float td = dist[x];
+                dist[x] = dist[y];
+                dist[y] = td;
+                const int tc = c[x];
+                c[x] = c[y];
+                c[y] = tc;
+            }
+        };
+        if (!ANY) {
+            cswap(0, 1);
+            cswap(2, 3);
+            cswap(0, 2);
+            cswap(1, 3);
+            cswap(1, 2);
+        } else {
+            cswap(0, 1);
+            cswap(0, 2);
+            cswap(0, 3);
+        }
+#pragma unroll
+        for (int k = 3; k >= 1; --k)
+            if (dist[k] < kInf) {
+                if (X == 0 || sp < L) stack[sp * kTraceBlock + lane] = c[k];
+                else spill[sp - L] = c[k];
+                ++sp;
+            }
+        if (dist[0] < kInf) {
+            node = c[0];
+            return false;
+        }
+        if (sp == 0) return true;
+        --sp;
+        node = (X == 0 || sp < L) ? stack[sp * kTraceBlock + lane] : spill[sp - L];
+        return false;
+    }
+
+    // Eight-wide nodes keep the per-lane step (their leaves are tested in groups of four).
+    __device__ __forceinline__ bool step_wave(bool active, const Bvh8Node* __restrict__ nodes,
+                                              const float4* __restrict__ tri,
+                                              int32_t* __restrict__ stack, volatile int32_t* spill,
+                                              int4* __restrict__) {
+        return active ? step(nodes, tri, stack, spill) : false;
+    }
+
     // Tests up to four leaf children of an eight-wide node (child words cw, hit flags lh) in one
     // round trip. Returns true when an ANY walk accepted a triangle.
     __device__ __forceinline__ bool leaves4(const float4* __restrict__ tri, int4 cw, bool l0,
