@@ -26,6 +26,7 @@
 
 #include "wf_device.h"
 #include "wf_internal.h"
+#include "wf_rays.h"
 
 namespace wf {
 
@@ -63,6 +64,7 @@ enum : int {
     C_NEXT = 4,        // next rays (shade_bounce counts_dev[0..2])
     C_SHADOW = 5,
     C_SHADOW_ENTRIES = 6,
+    C_TAKEN = 7,       // new paths started by the current iteration's regeneration
     C_COUNT = 16,
 };
 // 64-bit device statistics.
@@ -141,7 +143,7 @@ __device__ __forceinline__ int regen_take(const EngineView& e) {
 
 // Primary rays for the next samples of queued pixels, appended after the live rays
 // (generatePrimaryRays, cpp:401-457, per pixel instead of per pass).
-__global__ void k_regen(EngineView e, wf_ray_soa r, int32_t* __restrict__ tags) {
+__global__ void k_regen(EngineView e, RayPacked r, int32_t* __restrict__ tags) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int take = regen_take(e);
     if (i >= take) return;
@@ -161,19 +163,20 @@ __global__ void k_regen(EngineView e, wf_ray_soa r, int32_t* __restrict__ tags) 
     const V3 d = primary_dir(J.cam, local % J.width, local / J.width, J.width, J.height, rng(0),
                              rng(1));
     const int k = e.counters[C_LIVE] + i;
-    st3(r.ox, r.oy, r.oz, k, V3(J.cam[0], J.cam[1], J.cam[2]));
-    st3(r.dx, r.dy, r.dz, k, d);
-    r.tr[k] = 1.0f;
-    r.tg[k] = 1.0f;
-    r.tb[k] = 1.0f;
-    r.pixel[k] = a;
-    r.depth[k] = 0;
-    r.pdf[k] = -1.0f;
+    RayVal ray;
+    ray.o = V3(J.cam[0], J.cam[1], J.cam[2]);
+    ray.d = d;
+    ray.t = V3(1.0f, 1.0f, 1.0f);
+    ray.pixel = a;
+    ray.depth = 0;
+    ray.pdf = -1.0f;
+    st_ray(r, k, ray);
     tags[k] = (j << kTagJobShift) | s;
 }
 
 __global__ void k_regen_commit(EngineView e) {
     const int take = regen_take(e);
+    e.counters[C_TAKEN] = take;
     e.stats[S_HEAD] += (unsigned long long)take;
     const int live = e.counters[C_LIVE] + take;
     e.counters[C_LIVE] = live;
@@ -399,7 +402,8 @@ struct wf_engine {
     std::vector<void*> allocations;
     size_t bytes = 0;
     // device state
-    wf_ray_soa r0{}, r1{}, miss{};
+    RayPacked r0{}, r1{};               // live and sorted rays (packed)
+    wf_ray_soa miss{};
     wf_shade_soa s0{}, s1{};
     int32_t *t0 = nullptr, *t1 = nullptr, *st0 = nullptr, *st1 = nullptr, *mt = nullptr;
     int32_t* idx = nullptr;
@@ -453,6 +457,16 @@ wf_status carve_rays(wf_engine* e, size_t cap, wf_ray_soa* r) {
     r->pixel = (int32_t*)(f + 9 * cap);
     r->depth = (int32_t*)(f + 10 * cap);
     r->pdf = f + 11 * cap;
+    return WF_OK;
+}
+
+wf_status carve_packed(wf_engine* e, size_t cap, RayPacked* r) {
+    void* p = nullptr;
+    WF_TRY(engine_alloc(e, cap * 48, &p));
+    float4* f = (float4*)p;
+    r->a = f;
+    r->b = f + cap;
+    r->c = f + 2 * cap;
     return WF_OK;
 }
 
@@ -523,14 +537,14 @@ wf_status enqueue_iteration(wf_engine* e) {
     const Count live{cap, c + C_LIVE}, hits{cap, c + C_HITS}, misses{cap, c + C_MISSES};
     phase_mark(e, 0);
     {
-        Timed tm(ctx, "regen", 56.0 * cap);   // upper bound: the new-path count is on the device
+        Timed tm(ctx, "regen", 56.0, Count{cap, c + C_TAKEN});   // bytes per new path
         k_regen<<<grid_for(cap, 256), 256, 0, ctx->stream>>>(v, e->r0, e->t0);
         WF_HIP(hipGetLastError());
         k_regen_commit<<<1, 1, 0, ctx->stream>>>(v);
         WF_HIP(hipGetLastError());
     }
     phase_mark(e, 1);
-    const wf_ray_soa* rq = &e->r0;
+    const RayPacked* rq = &e->r0;
     const int32_t* rt = e->t0;
     if (e->cfg.ray_sort) {
         WF_TRY(ray_schedule(ctx, &e->r0, live, s->dev.bounds_min, s->dev.bounds_max, e->idx,
@@ -540,8 +554,7 @@ wf_status enqueue_iteration(wf_engine* e) {
         rt = e->t1;
     }
     phase_mark(e, 2);
-    WF_TRY(intersect_split(ctx, s, rq, live, &e->s0, &e->miss, c + C_HITS, rt, e->st0, e->mt,
-                           true));
+    WF_TRY(intersect_split(ctx, s, rq, live, &e->s0, &e->miss, c + C_HITS, rt, e->st0, e->mt));
     phase_mark(e, 3);
     const wf_shade_soa* q = &e->s0;
     const int32_t* qt = e->st0;
@@ -557,8 +570,9 @@ wf_status enqueue_iteration(wf_engine* e) {
     jobs.tags = qt;
     jobs.jobs = e->jobs_dev;
     jobs.next_tags = e->t0;
+    jobs.next_packed = e->r0;
     jobs.unordered = true;
-    WF_TRY(shade_bounce(ctx, s, q, hits, &e->cfg.shade, v.accum, &e->r0, c + C_NEXT, jobs,
+    WF_TRY(shade_bounce(ctx, s, q, hits, &e->cfg.shade, v.accum, nullptr, c + C_NEXT, jobs,
                         &alive));
     phase_mark(e, 5);
     WF_TRY(shade_misses(ctx, s, &e->miss, misses, e->cfg.shade.physics, v.accum));
@@ -689,8 +703,8 @@ wf_status wf_engine_create(wf_context* ctx, wf_scene* scene, const wf_engine_con
         wf_status s_ = (x);             \
         if (s_ != WF_OK) return fail(s_); \
     } while (0)
-    ETRY(carve_rays(e, cap, &e->r0));
-    ETRY(carve_rays(e, cap, &e->r1));
+    ETRY(carve_packed(e, cap, &e->r0));
+    ETRY(carve_packed(e, cap, &e->r1));
     ETRY(carve_rays(e, cap, &e->miss));
     ETRY(carve_shading(e, cap, &e->s0));
     ETRY(carve_shading(e, cap, &e->s1));

@@ -335,10 +335,30 @@ wf_status shading_schedule(wf_context* ctx, const wf_scene* s, const wf_shade_so
                            int32_t policy, int32_t* sorted, bool spatial = true);
 
 // Job context of a tagged wavefront for the shading kernel: per-entry tags plus the job table.
+// Engine-internal ray queue: three 16-byte columns per ray, a = (origin, pixel bits),
+// b = (direction, pdf), c = (throughput, depth bits).
+struct RayPacked {
+    float4* a = nullptr;
+    float4* b = nullptr;
+    float4* c = nullptr;
+};
+wf_status trace_closest(wf_context* ctx, const wf_scene* s, const RayPacked* rays, Count n,
+                        float tnear, float tfar, int32_t* hit_leaf, float* hit_t, float* hit_bu,
+                        float* hit_bv, uint32_t* flags);
+// Unordered closest hit + split of packed rays (hits and misses appended at device cursors).
+wf_status intersect_split(wf_context* ctx, const wf_scene* s, const RayPacked* rays, Count n,
+                          wf_shade_soa* hits_out, wf_ray_soa* misses_out, int32_t* counts_dev,
+                          const int32_t* tags_in, int32_t* hit_tags, int32_t* miss_tags);
+wf_status ray_schedule(wf_context* ctx, const RayPacked* rays, Count n, const float bmin[3],
+                       const float bmax[3], int32_t* sorted, uint64_t* keys_out);
+wf_status gather_rays(wf_context* ctx, const RayPacked* src, const int32_t* idx, Count n,
+                      RayPacked* dst, const int32_t* tags_in, int32_t* tags_out);
+
 struct ShadeJobs {
     const int32_t* tags = nullptr;     // per shading-queue entry
     const JobDev* jobs = nullptr;
     int32_t* next_tags = nullptr;      // tags of the compacted next rays
+    RayPacked next_packed;             // unordered mode: next rays in the packed layout
     bool unordered = false;            // next rays / shadow entries may come in any order
 };
 // One bounce of shadeAll: emission into accum, shadow rays resolved, next rays compacted into

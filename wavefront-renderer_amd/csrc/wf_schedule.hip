@@ -9,6 +9,7 @@
 // ranks are dense descending ranks of the host tables; ties keep queue order.
 #include "wf_device.h"
 #include "wf_internal.h"
+#include "wf_rays.h"
 
 namespace wf {
 
@@ -25,24 +26,26 @@ __device__ __forceinline__ uint32_t spread_bits10(uint32_t v) {
 // std::clamp(v, 0, 1) for non-NaN v
 __device__ __forceinline__ float clamp01(float v) { return v < 0.0f ? 0.0f : (1.0f < v ? 1.0f : v); }
 
-__global__ void k_ray_keys(const float* __restrict__ ox, const float* __restrict__ oy,
-                           const float* __restrict__ oz, const float* __restrict__ dx,
-                           const float* __restrict__ dy, const float* __restrict__ dz, int n_host,
-                           const int32_t* __restrict__ nd, float minx, float miny, float minz,
-                           float ex, float ey, float ez, uint64_t* __restrict__ keys) {
+template <class R>
+__global__ void k_ray_keys(R rays, int n_host, const int32_t* __restrict__ nd, float minx,
+                           float miny, float minz, float ex, float ey, float ez,
+                           uint64_t* __restrict__ keys) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= load_count(n_host, nd)) return;
-    const uint64_t octant = (dx[i] < 0.0f ? 1u : 0u) | (dy[i] < 0.0f ? 2u : 0u) |
-                            (dz[i] < 0.0f ? 4u : 0u);
-    const uint32_t gx = (uint32_t)(clamp01((ox[i] - minx) / ex) * 1023.0f);
-    const uint32_t gy = (uint32_t)(clamp01((oy[i] - miny) / ey) * 1023.0f);
-    const uint32_t gz = (uint32_t)(clamp01((oz[i] - minz) / ez) * 1023.0f);
+    V3 o, d;
+    ld_od(rays, i, o, d);
+    const uint64_t octant = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) |
+                            (d.z < 0.0f ? 4u : 0u);
+    const uint32_t gx = (uint32_t)(clamp01((o.x - minx) / ex) * 1023.0f);
+    const uint32_t gy = (uint32_t)(clamp01((o.y - miny) / ey) * 1023.0f);
+    const uint32_t gz = (uint32_t)(clamp01((o.z - minz) / ez) * 1023.0f);
     const uint32_t morton = spread_bits10(gx) | (spread_bits10(gy) << 1) | (spread_bits10(gz) << 2);
     keys[i] = (octant << 30) | (uint64_t)morton;
 }
 
-wf_status ray_keys(wf_context* ctx, const wf_ray_soa* r, Count n, const float bmin[3],
-                   const float bmax[3], uint64_t* keys) {
+template <class R>
+static wf_status ray_keys_any(wf_context* ctx, const R& r, Count n, const float bmin[3],
+                              const float bmax[3], uint64_t* keys) {
     if (n.n <= 0) return WF_OK;
     // extent = max(max - min, 1e-6) per axis (RayQueue.h:143-145), host float math
     auto ext = [](float hi, float lo) {
@@ -50,11 +53,16 @@ wf_status ray_keys(wf_context* ctx, const wf_ray_soa* r, Count n, const float bm
         return e < 1e-6f ? 1e-6f : e;
     };
     Timed tm(ctx, "ray_keys", 32.0, n);
-    k_ray_keys<<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(
-        r->ox, r->oy, r->oz, r->dx, r->dy, r->dz, n.n, n.dev, bmin[0], bmin[1], bmin[2],
-        ext(bmax[0], bmin[0]), ext(bmax[1], bmin[1]), ext(bmax[2], bmin[2]), keys);
+    k_ray_keys<R><<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(
+        r, n.n, n.dev, bmin[0], bmin[1], bmin[2], ext(bmax[0], bmin[0]), ext(bmax[1], bmin[1]),
+        ext(bmax[2], bmin[2]), keys);
     WF_HIP(hipGetLastError());
     return WF_OK;
+}
+
+wf_status ray_keys(wf_context* ctx, const wf_ray_soa* r, Count n, const float bmin[3],
+                   const float bmax[3], uint64_t* keys) {
+    return ray_keys_any(ctx, *r, n, bmin, bmax, keys);
 }
 
 // ---- materialize ---------------------------------------------------------------------------------
@@ -121,6 +129,29 @@ wf_status gather_rays(wf_context* ctx, const wf_ray_soa* src, const int32_t* idx
     Timed tm(ctx, "gather_rays", tags_out ? 104.0 : 96.0, n);
     k_gather_rays<<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(*src, idx, n.n, n.dev, *dst,
                                                               tags_in, tags_out);
+    WF_HIP(hipGetLastError());
+    return WF_OK;
+}
+
+__global__ void k_gather_packed(RayPacked s, const int32_t* __restrict__ idx, int n_host,
+                                const int32_t* __restrict__ nd, RayPacked d,
+                                const int32_t* __restrict__ tags_in,
+                                int32_t* __restrict__ tags_out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= load_count(n_host, nd)) return;
+    const int j = idx[i];
+    if (tags_out) tags_out[i] = tags_in[j];
+    d.a[i] = s.a[j];
+    d.b[i] = s.b[j];
+    d.c[i] = s.c[j];
+}
+
+wf_status gather_rays(wf_context* ctx, const RayPacked* src, const int32_t* idx, Count n,
+                      RayPacked* dst, const int32_t* tags_in, int32_t* tags_out) {
+    if (n.n <= 0) return WF_OK;
+    Timed tm(ctx, "gather_rays", tags_out ? 104.0 : 96.0, n);
+    k_gather_packed<<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(*src, idx, n.n, n.dev, *dst,
+                                                                tags_in, tags_out);
     WF_HIP(hipGetLastError());
     return WF_OK;
 }
@@ -479,18 +510,29 @@ wf_status shading_schedule(wf_context* ctx, const wf_scene* s, const wf_shade_so
                             "shading_sort");
 }
 
-wf_status ray_schedule(wf_context* ctx, const wf_ray_soa* rays, Count n, const float bmin[3],
-                       const float bmax[3], int32_t* sorted, uint64_t* keys_out) {
+template <class R>
+static wf_status ray_schedule_any(wf_context* ctx, const R& rays, Count n, const float bmin[3],
+                                  const float bmax[3], int32_t* sorted, uint64_t* keys_out) {
     if (n.n <= 0) return WF_OK;
     void *pk, *pk2;
     WF_TRY(workspace(ctx, "ray_keys", sizeof(uint64_t) * n.n, &pk));
     WF_TRY(workspace(ctx, "ray_keys2", sizeof(uint64_t) * n.n, &pk2));
-    WF_TRY(ray_keys(ctx, rays, n, bmin, bmax, (uint64_t*)pk));
+    WF_TRY(ray_keys_any(ctx, rays, n, bmin, bmax, (uint64_t*)pk));
     if (keys_out)
         WF_HIP(hipMemcpyAsync(keys_out, pk, sizeof(uint64_t) * n.n, hipMemcpyDeviceToDevice,
                               ctx->stream));
     return radix_sort_pairs(ctx, (uint64_t*)pk, nullptr, (uint64_t*)pk2, sorted, n, 33,
                             "ray_sort");
+}
+
+wf_status ray_schedule(wf_context* ctx, const wf_ray_soa* rays, Count n, const float bmin[3],
+                       const float bmax[3], int32_t* sorted, uint64_t* keys_out) {
+    return ray_schedule_any(ctx, *rays, n, bmin, bmax, sorted, keys_out);
+}
+
+wf_status ray_schedule(wf_context* ctx, const RayPacked* rays, Count n, const float bmin[3],
+                       const float bmax[3], int32_t* sorted, uint64_t* keys_out) {
+    return ray_schedule_any(ctx, *rays, n, bmin, bmax, sorted, keys_out);
 }
 
 }  // namespace wf

@@ -8,6 +8,7 @@
 // atomics, and the per-pixel order is the reference's (emission, area NEE, directional NEE,
 // miss). Shadow rays are resolved by a second kernel over the compacted list of entries that
 // emitted one; surviving next rays are compacted in shading order.
+#include "wf_rays.h"
 #include "wf_traverse.h"
 
 namespace wf {
@@ -42,6 +43,7 @@ struct ShadeOut {
     int32_t* shadow_rays;                              // device counter
     int32_t* next_count;                               // device counter
     wf_ray_soa next;
+    RayPacked next_packed;                             // used instead of `next` when set
     int32_t* next_tags;
 };
 
@@ -604,20 +606,16 @@ __global__ __launch_bounds__(kAppendBlock) void k_shade_append(wf_shade_soa q, S
         const int i = base + j * kAppendBlock + (int)threadIdx.x;
         if (si[j] >= 0) o.shadow_list[si[j]] = i;
         if (k[j] >= 0) {
-            const int d = k[j];
-            o.next.ox[d] = q.px[i];
-            o.next.oy[d] = q.py[i];
-            o.next.oz[d] = q.pz[i];
-            o.next.dx[d] = o.ndx[i];
-            o.next.dy[d] = o.ndy[i];
-            o.next.dz[d] = o.ndz[i];
-            o.next.tr[d] = o.ntr[i];
-            o.next.tg[d] = o.ntg[i];
-            o.next.tb[d] = o.ntb[i];
-            o.next.pixel[d] = q.pixel[i];
-            o.next.depth[d] = q.depth[i] + 1;
-            o.next.pdf[d] = o.npdf[i];
-            if (o.next_tags) o.next_tags[d] = tags[i];
+            RayVal r;
+            r.o = ld3(q.px, q.py, q.pz, i);
+            r.d = ld3(o.ndx, o.ndy, o.ndz, i);
+            r.t = ld3(o.ntr, o.ntg, o.ntb, i);
+            r.pixel = q.pixel[i];
+            r.depth = q.depth[i] + 1;
+            r.pdf = o.npdf[i];
+            if (o.next_packed.a) st_ray(o.next_packed, k[j], r);
+            else st_ray(o.next, k[j], r);
+            if (o.next_tags) o.next_tags[k[j]] = tags[i];
         }
     }
 }
@@ -803,6 +801,7 @@ static wf_status shade_outputs(wf_context* ctx, int n, ShadeOut& o) {
     o.shadow_list = nullptr;
     o.shadow_entries = o.shadow_rays = o.next_count = nullptr;
     o.next = wf_ray_soa{};
+    o.next_packed = RayPacked{};
     o.next_tags = nullptr;
     return WF_OK;
 }
@@ -831,7 +830,8 @@ wf_status shade_bounce(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q
         o.next_count = counts_dev;
         o.shadow_rays = counts_dev + 1;
         o.shadow_entries = counts_dev + 2;
-        o.next = *next;
+        if (next) o.next = *next;
+        o.next_packed = jobs.next_packed;
         o.next_tags = jobs.next_tags;
         WF_TRY(wf_fill_u32(ctx, counts_dev, 0u, 3));
         {

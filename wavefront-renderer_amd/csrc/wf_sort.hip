@@ -224,9 +224,12 @@ __global__ __launch_bounds__(kSortBlock) void k_sweep_pass(
                     }
                 }
                 j -= used;
-                if (!found && used == 0 && ++spins > kSpinLimit) {
-                    atomicOr(fault, 1);
-                    break;
+                if (!found && used == 0) {
+                    if (++spins > kSpinLimit) {
+                        atomicOr(fault, 1);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);   // leave the issue slots to working waves
                 }
             }
             state_store(my + dg, kStatePrefix | (excl + tc[q]));

@@ -9,17 +9,16 @@
 // operations; closest hit = lexicographic minimum of (t, global triangle index) over every
 // triangle the test accepts in [tnear, tfar], which makes the result independent of traversal
 // order. Box tests are conservative (boxes are padded at build time).
+#include "wf_rays.h"
 #include "wf_traverse.h"
 
 namespace wf {
 
 // Persistent closest-hit kernel: each lane walks one ray at a time and takes a new ray as soon as
 // its walk ends (dynamic refill keeps lanes busy while the slowest rays of a wave finish).
-template <class NODE, int STACK>
+template <class NODE, int STACK, class R>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(
-    const NODE* __restrict__ nodes, const float4* __restrict__ tri, int n_tris,
-    const float* __restrict__ ox, const float* __restrict__ oy, const float* __restrict__ oz,
-    const float* __restrict__ dx, const float* __restrict__ dy, const float* __restrict__ dz,
+    const NODE* __restrict__ nodes, const float4* __restrict__ tri, int n_tris, R rays,
     int n_host, const int32_t* __restrict__ nd, float tnear, float tfar,
     int32_t* __restrict__ out_leaf, float* __restrict__ out_t, float* __restrict__ out_u,
     float* __restrict__ out_v, uint32_t* __restrict__ out_flag, int32_t* __restrict__ counter,
@@ -38,7 +37,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(
         const int r = refill(wq, ray < 0, refill_lanes);
         if (r >= 0) {
             ray = r;
-            w.start(V3(ox[r], oy[r], oz[r]), V3(dx[r], dy[r], dz[r]), tnear, tfar);
+            V3 o, d;
+            ld_od(rays, r, o, d);
+            w.start(o, d, tnear, tfar);
         }
         if (__ballot(ray >= 0) == 0) {
             if (wq.exhausted()) break;
@@ -96,10 +97,12 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_stats(
 }
 
 // HitRecord mapping of EmbreeAccelerator::intersect4 (Embree.cpp:240-252) for ray i into slot j.
-__device__ __forceinline__ void map_hit(const SceneDev& s, const wf_ray_soa& in, int i, int L,
+template <class R>
+__device__ __forceinline__ void map_hit(const SceneDev& s, const R& in, int i, int L,
                                         float t, float bu, float bv, wf_shade_soa& out, int j) {
-    const V3 o(in.ox[i], in.oy[i], in.oz[i]);
-    const V3 d(in.dx[i], in.dy[i], in.dz[i]);
+    const RayVal ray = ld_ray(in, i);
+    const V3 o = ray.o;
+    const V3 d = ray.d;
     const float4 A = s.tri[3 * (size_t)L];
     const float4 B = s.tri[3 * (size_t)L + 1];
     const float4 C = s.tri[3 * (size_t)L + 2];
@@ -127,12 +130,12 @@ __device__ __forceinline__ void map_hit(const SceneDev& s, const wf_ray_soa& in,
     out.tex[j] = s.materials[mat].texture_id;
     st3(out.ox, out.oy, out.oz, j, o);
     st3(out.dx, out.dy, out.dz, j, d);
-    out.tr[j] = in.tr[i];
-    out.tg[j] = in.tg[i];
-    out.tb[j] = in.tb[i];
-    out.pixel[j] = in.pixel[i];
-    out.depth[j] = in.depth[i];
-    out.pdf[j] = in.pdf[i];
+    out.tr[j] = ray.t.x;
+    out.tg[j] = ray.t.y;
+    out.tb[j] = ray.t.z;
+    out.pixel[j] = ray.pixel;
+    out.depth[j] = ray.depth;
+    out.pdf[j] = ray.pdf;
     if (out.geom) out.geom[j] = geom;
     if (out.front) out.front[j] = ff ? 1 : 0;
 }
@@ -171,8 +174,9 @@ __global__ void k_split(const SceneDev s, wf_ray_soa in, int n_host, const int32
 // thread (coalesced rounds), one cursor atomic per block and list.
 constexpr int kAppendBlock = 1024;
 constexpr int kAppendItems = 4;
+template <class R>
 __global__ __launch_bounds__(kAppendBlock) void k_split_append(
-    const SceneDev s, wf_ray_soa in, int n_host, const int32_t* __restrict__ nd,
+    const SceneDev s, R in, int n_host, const int32_t* __restrict__ nd,
     const int32_t* __restrict__ leaf, const float* __restrict__ ht, const float* __restrict__ hu,
     const float* __restrict__ hv, wf_shade_soa out, wf_ray_soa miss, int32_t* __restrict__ counts,
     const int32_t* __restrict__ tags_in, int32_t* __restrict__ hit_tags,
@@ -199,20 +203,8 @@ __global__ __launch_bounds__(kAppendBlock) void k_split_append(
             map_hit(s, in, i, L[j], ht[i], hu[i], hv[i], out, pos[j]);
             if (hit_tags) hit_tags[pos[j]] = tags_in[i];
         } else if (m[j] >= 0) {
-            const int k = m[j];
-            miss.ox[k] = in.ox[i];
-            miss.oy[k] = in.oy[i];
-            miss.oz[k] = in.oz[i];
-            miss.dx[k] = in.dx[i];
-            miss.dy[k] = in.dy[i];
-            miss.dz[k] = in.dz[i];
-            miss.tr[k] = in.tr[i];
-            miss.tg[k] = in.tg[i];
-            miss.tb[k] = in.tb[i];
-            miss.pixel[k] = in.pixel[i];
-            miss.depth[k] = in.depth[i];
-            miss.pdf[k] = in.pdf[i];
-            if (miss_tags) miss_tags[k] = tags_in[i];
+            st_ray(miss, m[j], ld_ray(in, i));
+            if (miss_tags) miss_tags[m[j]] = tags_in[i];
         }
     }
 }
@@ -259,9 +251,10 @@ wf_status scene_ready(const wf_scene* s) {
     return WF_OK;
 }
 
-wf_status trace_closest(wf_context* ctx, const wf_scene* s, const wf_ray_soa* r, Count n,
-                        float tnear, float tfar, int32_t* hit_leaf, float* hit_t, float* hit_bu,
-                        float* hit_bv, uint32_t* flags) {
+template <class R>
+static wf_status trace_closest_any(wf_context* ctx, const wf_scene* s, const R& r, Count n,
+                                   float tnear, float tfar, int32_t* hit_leaf, float* hit_t,
+                                   float* hit_bu, float* hit_bv, uint32_t* flags) {
     WF_TRY(scene_ready(s));
     if (n.n <= 0) return WF_OK;
     void* pc = nullptr;
@@ -270,14 +263,25 @@ wf_status trace_closest(wf_context* ctx, const wf_scene* s, const wf_ray_soa* r,
     Timed tm(ctx, "traverse_closest", 44.0, n);
     with_bvh(s->dev, [&](auto nodes, auto S) {
         k_trace_closest<std::remove_cv_t<std::remove_pointer_t<decltype(nodes)>>,
-                        decltype(S)::value>
+                        decltype(S)::value, R>
             <<<persistent_grid(n.n, ctx), kTraceBlock, 0, ctx->stream>>>(
-                nodes, s->dev.tri, s->dev.n_tris, r->ox, r->oy, r->oz, r->dx, r->dy,
-                r->dz, n.n, n.dev, tnear, tfar, hit_leaf, hit_t, hit_bu, hit_bv, flags,
-                (int32_t*)pc, ctx->tuning.refill_lanes);
+                nodes, s->dev.tri, s->dev.n_tris, r, n.n, n.dev, tnear, tfar, hit_leaf, hit_t,
+                hit_bu, hit_bv, flags, (int32_t*)pc, ctx->tuning.refill_lanes);
     });
     WF_HIP(hipGetLastError());
     return WF_OK;
+}
+
+wf_status trace_closest(wf_context* ctx, const wf_scene* s, const wf_ray_soa* r, Count n,
+                        float tnear, float tfar, int32_t* hit_leaf, float* hit_t, float* hit_bu,
+                        float* hit_bv, uint32_t* flags) {
+    return trace_closest_any(ctx, s, *r, n, tnear, tfar, hit_leaf, hit_t, hit_bu, hit_bv, flags);
+}
+
+wf_status trace_closest(wf_context* ctx, const wf_scene* s, const RayPacked* r, Count n,
+                        float tnear, float tfar, int32_t* hit_leaf, float* hit_t, float* hit_bu,
+                        float* hit_bv, uint32_t* flags) {
+    return trace_closest_any(ctx, s, *r, n, tnear, tfar, hit_leaf, hit_t, hit_bu, hit_bv, flags);
 }
 
 wf_status trace_occluded(const wf_scene* s, const float* ox, const float* oy, const float* oz,
@@ -297,6 +301,39 @@ wf_status trace_occluded(const wf_scene* s, const float* ox, const float* oy, co
     return WF_OK;
 }
 
+// Closest hit, then hits and misses appended at device cursors (queue order free).
+template <class R>
+static wf_status split_unordered(wf_context* ctx, const wf_scene* s, const R& rays, Count n,
+                                 wf_shade_soa* hits_out, wf_ray_soa* misses_out,
+                                 int32_t* counts_dev, const int32_t* tags_in, int32_t* hit_tags,
+                                 int32_t* miss_tags) {
+    void *pl, *pt, *pu, *pv;
+    size_t nn = (size_t)(n.n > 0 ? n.n : 1);
+    WF_TRY(workspace(ctx, "hit_leaf", sizeof(int32_t) * nn, &pl));
+    WF_TRY(workspace(ctx, "hit_t", sizeof(float) * nn, &pt));
+    WF_TRY(workspace(ctx, "hit_u", sizeof(float) * nn, &pu));
+    WF_TRY(workspace(ctx, "hit_v", sizeof(float) * nn, &pv));
+    WF_TRY(wf_fill_u32(ctx, counts_dev, 0u, 2));
+    WF_TRY(trace_closest_any(ctx, s, rays, n, 0.001f, 1e9f, (int32_t*)pl, (float*)pt,
+                             (float*)pu, (float*)pv, nullptr));
+    if (n.n > 0) {
+        Timed tm(ctx, "split", 97.0, n);
+        k_split_append<R><<<grid_for(n.n, kAppendBlock * kAppendItems), kAppendBlock, 0,
+                            ctx->stream>>>(
+            s->dev, rays, n.n, n.dev, (int32_t*)pl, (float*)pt, (float*)pu, (float*)pv,
+            *hits_out, *misses_out, counts_dev, tags_in, hit_tags, miss_tags);
+        WF_HIP(hipGetLastError());
+    }
+    return WF_OK;
+}
+
+wf_status intersect_split(wf_context* ctx, const wf_scene* s, const RayPacked* rays, Count n,
+                          wf_shade_soa* hits_out, wf_ray_soa* misses_out, int32_t* counts_dev,
+                          const int32_t* tags_in, int32_t* hit_tags, int32_t* miss_tags) {
+    return split_unordered(ctx, s, *rays, n, hits_out, misses_out, counts_dev, tags_in, hit_tags,
+                           miss_tags);
+}
+
 // Closest hit + stable split; counts_dev[0] = hits, counts_dev[1] = misses (device int32).
 wf_status intersect_split(wf_context* ctx, const wf_scene* s, const wf_ray_soa* rays, Count n,
                           wf_shade_soa* hits_out, wf_ray_soa* misses_out, int32_t* counts_dev,
@@ -309,20 +346,9 @@ wf_status intersect_split(wf_context* ctx, const wf_scene* s, const wf_ray_soa* 
     WF_TRY(workspace(ctx, "hit_u", sizeof(float) * nn, &pu));
     WF_TRY(workspace(ctx, "hit_v", sizeof(float) * nn, &pv));
     WF_TRY(workspace(ctx, "hit_flag", sizeof(uint32_t) * nn, &pf));
-    if (unordered) {
-        WF_TRY(wf_fill_u32(ctx, counts_dev, 0u, 2));
-        WF_TRY(trace_closest(ctx, s, rays, n, 0.001f, 1e9f, (int32_t*)pl, (float*)pt,
-                             (float*)pu, (float*)pv, nullptr));
-        if (n.n > 0) {
-            Timed tm(ctx, "split", 97.0, n);
-            k_split_append<<<grid_for(n.n, kAppendBlock * kAppendItems), kAppendBlock, 0,
-                             ctx->stream>>>(
-                s->dev, *rays, n.n, n.dev, (int32_t*)pl, (float*)pt, (float*)pu, (float*)pv,
-                *hits_out, *misses_out, counts_dev, tags_in, hit_tags, miss_tags);
-            WF_HIP(hipGetLastError());
-        }
-        return WF_OK;
-    }
+    if (unordered)
+        return split_unordered(ctx, s, *rays, n, hits_out, misses_out, counts_dev, tags_in,
+                               hit_tags, miss_tags);
     WF_TRY(trace_closest(ctx, s, rays, n, 0.001f, 1e9f, (int32_t*)pl, (float*)pt, (float*)pu,
                          (float*)pv, (uint32_t*)pf));
     WF_TRY(exclusive_scan_u32(ctx, (uint32_t*)pf, (uint32_t*)pf, n, counts_dev, "split"));
