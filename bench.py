@@ -301,7 +301,7 @@ def run_served(wf, ctx, scenes, args) -> dict:
     datas = [scenes.cornell_box(), scenes.sorting_stress(), scenes.sphere_cloud()]
     scs = [wf.Scene(ctx, d) for d in datas]
     physics = [wf.PHYSICS_REFERENCE, wf.PHYSICS_UNBIASED, wf.PHYSICS_UNBIASED]
-    engines = forms.make_engines(ctx, scs, physics, capacity=2 << 20, pixel_budget=8 << 20,
+    engines = forms.make_engines(ctx, scs, physics, capacity=8 << 20, pixel_budget=8 << 20,
                                  max_jobs=64)
     # capacity: a burst of requests admitted at once
     calib = forms.served_jobs(args.jobs // 3 or 1, rate=1.0, seed=0x5EED0104,

@@ -32,7 +32,7 @@ RENDER_SPACE: Dict[str, List] = {
     "refill_lanes": [32, 48, 64],
 }
 ENGINE_SPACE: Dict[str, List] = {
-    "capacity": [1 << 20, 2 << 20, 4 << 20, 8 << 20, 16 << 20],
+    "capacity": [2 << 20, 4 << 20, 8 << 20, 16 << 20, 32 << 20],
     "sample_lanes": [8, 16, 32, 64],
     "trace_blocks_per_cu": [4, 6, 8, 10, 12],
     "radix_bits": [6, 8, 11],
