@@ -521,7 +521,7 @@ static wf_status sweep_sort(wf_context* ctx, const uint64_t* keys_in, const int3
     WF_TRY(wf_fill_u32(ctx, counters, 0u, kMaxPasses));
     {
         Timed t(ctx, tag, 8.0, n);
-        const unsigned blocks = std::min<unsigned>(grid_for(n.n, kSortBlock), 256u);
+        const unsigned blocks = std::min<unsigned>(grid_for(n.n, kSortBlock), 1024u);
         k_sweep_hist<BITS><<<blocks, kSortBlock, 0, ctx->stream>>>(keys_in, n.n, n.dev, lo_bit,
                                                                    hi_bit, passes, hist);
         WF_HIP(hipGetLastError());
