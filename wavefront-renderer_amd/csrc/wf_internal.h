@@ -298,6 +298,14 @@ CamBasis camera_basis(const wf_camera& c, int width, int height);
 wf_status radix_sort_pairs(wf_context* ctx, const uint64_t* keys_in, const int32_t* vals_in,
                            uint64_t* keys_out, int32_t* vals_out, Count n, int32_t key_bits,
                            const char* tag);
+// Stable LSD radix sort of packed words: word i holds its key in bits [idx_bits, idx_bits +
+// key_bits) and its own position i in bits [0, idx_bits). Sorting on the key bits alone is the
+// same permutation as sorting (key, i) pairs, with 8 bytes moved per element instead of 12;
+// order receives the low idx_bits bits of the sorted words. scratch (n words) is overwritten;
+// words is not. A host count sorts only the key bits where the words differ.
+wf_status radix_sort_packed(wf_context* ctx, const uint64_t* words, uint64_t* scratch,
+                            int32_t* order, Count n, int32_t idx_bits, int32_t key_bits,
+                            const char* tag);
 // Exclusive prefix sum of n u32 flags/counts into out (may alias in); total (optional) is
 // written to a device int32.
 wf_status exclusive_scan_u32(wf_context* ctx, const uint32_t* in, uint32_t* out, Count n,

@@ -26,10 +26,12 @@ __device__ __forceinline__ uint32_t spread_bits10(uint32_t v) {
 // std::clamp(v, 0, 1) for non-NaN v
 __device__ __forceinline__ float clamp01(float v) { return v < 0.0f ? 0.0f : (1.0f < v ? 1.0f : v); }
 
+// keys[i] = the 33-bit sort key, or with idx_bits > 0 the packed word key << idx_bits | i (and
+// the bare key in raw[i] when raw is given)
 template <class R>
 __global__ void k_ray_keys(R rays, int n_host, const int32_t* __restrict__ nd, float minx,
                            float miny, float minz, float ex, float ey, float ez,
-                           uint64_t* __restrict__ keys) {
+                           uint64_t* __restrict__ keys, int idx_bits, uint64_t* __restrict__ raw) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= load_count(n_host, nd)) return;
     V3 o, d;
@@ -40,12 +42,15 @@ __global__ void k_ray_keys(R rays, int n_host, const int32_t* __restrict__ nd, f
     const uint32_t gy = (uint32_t)(clamp01((o.y - miny) / ey) * 1023.0f);
     const uint32_t gz = (uint32_t)(clamp01((o.z - minz) / ez) * 1023.0f);
     const uint32_t morton = spread_bits10(gx) | (spread_bits10(gy) << 1) | (spread_bits10(gz) << 2);
-    keys[i] = (octant << 30) | (uint64_t)morton;
+    const uint64_t key = (octant << 30) | (uint64_t)morton;
+    keys[i] = idx_bits ? (key << idx_bits) | (uint64_t)i : key;
+    if (raw) raw[i] = key;
 }
 
 template <class R>
 static wf_status ray_keys_any(wf_context* ctx, const R& r, Count n, const float bmin[3],
-                              const float bmax[3], uint64_t* keys) {
+                              const float bmax[3], uint64_t* keys, int idx_bits = 0,
+                              uint64_t* raw = nullptr) {
     if (n.n <= 0) return WF_OK;
     // extent = max(max - min, 1e-6) per axis (RayQueue.h:143-145), host float math
     auto ext = [](float hi, float lo) {
@@ -55,7 +60,7 @@ static wf_status ray_keys_any(wf_context* ctx, const R& r, Count n, const float 
     Timed tm(ctx, "ray_keys", 32.0, n);
     k_ray_keys<R><<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(
         r, n.n, n.dev, bmin[0], bmin[1], bmin[2], ext(bmax[0], bmin[0]), ext(bmax[1], bmin[1]),
-        ext(bmax[2], bmin[2]), keys);
+        ext(bmax[2], bmin[2]), keys, idx_bits, raw);
     WF_HIP(hipGetLastError());
     return WF_OK;
 }
@@ -514,15 +519,15 @@ template <class R>
 static wf_status ray_schedule_any(wf_context* ctx, const R& rays, Count n, const float bmin[3],
                                   const float bmax[3], int32_t* sorted, uint64_t* keys_out) {
     if (n.n <= 0) return WF_OK;
+    // packed words: the 33-bit key above the ray's index (stable order on equal keys)
+    int idx_bits = 1;
+    while ((1ll << idx_bits) < (int64_t)n.n) ++idx_bits;
     void *pk, *pk2;
     WF_TRY(workspace(ctx, "ray_keys", sizeof(uint64_t) * n.n, &pk));
     WF_TRY(workspace(ctx, "ray_keys2", sizeof(uint64_t) * n.n, &pk2));
-    WF_TRY(ray_keys_any(ctx, rays, n, bmin, bmax, (uint64_t*)pk));
-    if (keys_out)
-        WF_HIP(hipMemcpyAsync(keys_out, pk, sizeof(uint64_t) * n.n, hipMemcpyDeviceToDevice,
-                              ctx->stream));
-    return radix_sort_pairs(ctx, (uint64_t*)pk, nullptr, (uint64_t*)pk2, sorted, n, 33,
-                            "ray_sort");
+    WF_TRY(ray_keys_any(ctx, rays, n, bmin, bmax, (uint64_t*)pk, idx_bits, keys_out));
+    return radix_sort_packed(ctx, (uint64_t*)pk, (uint64_t*)pk2, sorted, n, idx_bits, 33,
+                             "ray_sort");
 }
 
 wf_status ray_schedule(wf_context* ctx, const wf_ray_soa* rays, Count n, const float bmin[3],

@@ -115,16 +115,21 @@ __device__ __forceinline__ void state_store(uint32_t* p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <int BITS>
+// What a pass moves: (key, value) pairs, or packed words (key above, element index in the low
+// idx_bits bits) written as words, or, in the last pass of a packed sort, as their indices.
+enum SweepPayload { kPairs = 0, kPacked = 1, kPackedIndex = 2 };
+
+template <int BITS, int PAYLOAD>
 __global__ __launch_bounds__(kSortBlock) void k_sweep_pass(
     const uint64_t* __restrict__ keys_in, const int32_t* __restrict__ vals_in,
     uint64_t* __restrict__ keys_out, int32_t* __restrict__ vals_out, int n_host,
     const int32_t* __restrict__ nd, int shift, uint32_t mask, int nbits,
     const uint32_t* __restrict__ bases, uint32_t* state, int32_t* tile_counter,
-    int32_t* __restrict__ fault) {
+    int32_t* __restrict__ fault, uint32_t idx_mask) {
     constexpr int R = Digits<BITS>::R, DPT = Digits<BITS>::DPT;
+    constexpr bool kVals = PAYLOAD == kPairs;
     // the per-wave digit counts and, once every slot is known, the staged tile share one buffer
-    constexpr int kCntBytes = kSortWaves * R * 4, kStageBytes = kSortTile * 12;
+    constexpr int kCntBytes = kSortWaves * R * 4, kStageBytes = kSortTile * (kVals ? 12 : 8);
     __shared__ __align__(16) unsigned char lds[kCntBytes > kStageBytes ? kCntBytes : kStageBytes];
     uint32_t(*wave_cnt)[R] = reinterpret_cast<uint32_t(*)[R]>(lds);
     uint64_t* stage_k = reinterpret_cast<uint64_t*>(lds);
@@ -155,7 +160,7 @@ __global__ __launch_bounds__(kSortBlock) void k_sweep_pass(
         const int i = chunk + it * 64 + lane;
         const bool valid = i < n;
         k[it] = valid ? keys_in[i] : 0ull;
-        v[it] = valid ? (vals_in ? vals_in[i] : i) : 0;
+        if (kVals) v[it] = valid ? (vals_in ? vals_in[i] : i) : 0;
         const uint32_t dig = valid ? ((uint32_t)(k[it] >> shift) & mask) : 0u;
         uint64_t peers = __ballot(valid);
         for (int b = 0; b < nbits; ++b) {
@@ -251,7 +256,7 @@ __global__ __launch_bounds__(kSortBlock) void k_sweep_pass(
         const int i = chunk + it * 64 + lane;
         if (i < n) {
             stage_k[r[it]] = k[it];
-            stage_v[r[it]] = v[it];
+            if (kVals) stage_v[r[it]] = v[it];
         }
     }
     __syncthreads();
@@ -263,8 +268,12 @@ __global__ __launch_bounds__(kSortBlock) void k_sweep_pass(
             const uint64_t key = stage_k[j];
             const uint32_t dig = (uint32_t)(key >> shift) & mask;
             const uint32_t pos = digit_base[dig] + (uint32_t)j;
-            keys_out[pos] = key;
-            vals_out[pos] = stage_v[j];
+            if (PAYLOAD == kPackedIndex) {
+                vals_out[pos] = (int32_t)((uint32_t)key & idx_mask);
+            } else {
+                keys_out[pos] = key;
+                if (kVals) vals_out[pos] = stage_v[j];
+            }
         }
     }
 }
@@ -494,12 +503,14 @@ static wf_status sort_identity(wf_context* ctx, const uint64_t* keys_in, const i
     return WF_OK;
 }
 
-// Passes of BITS-wide digits over key bits [lo_bit, hi_bit]; the last pass writes keys_out and
-// vals_out.
+// Passes of BITS-wide digits over key bits [lo_bit, hi_bit]. Pairs (idx_bits 0): the last pass
+// writes keys_out and vals_out. Packed words (idx_bits > 0, vals_in NULL): the passes move the
+// words alone, keys_out is the second ping-pong buffer, and the last pass writes the low idx_bits
+// bits of each word to vals_out.
 template <int BITS>
 static wf_status sweep_sort(wf_context* ctx, const uint64_t* keys_in, const int32_t* vals_in,
                             uint64_t* keys_out, int32_t* vals_out, Count n, int lo_bit, int hi_bit,
-                            const char* tag) {
+                            const char* tag, int idx_bits = 0) {
     constexpr int R = Digits<BITS>::R;
     const int span = hi_bit - lo_bit + 1;
     const int tiles = (n.n + kSortTile - 1) / kSortTile;
@@ -507,8 +518,9 @@ static wf_status sweep_sort(wf_context* ctx, const uint64_t* keys_in, const int3
     if (passes > Digits<BITS>::MaxPasses || passes > kMaxPasses)
         return set_error(WF_ERR_INVALID_ARGUMENT, "sort key too wide");
     void *pk = nullptr, *pv = nullptr, *ph = nullptr, *pst = nullptr, *pc = nullptr;
+    const bool packed = idx_bits > 0;
     WF_TRY(workspace(ctx, "sort_keys_alt", sizeof(uint64_t) * (size_t)n.n, &pk));
-    WF_TRY(workspace(ctx, "sort_vals_alt", sizeof(int32_t) * (size_t)n.n, &pv));
+    if (!packed) WF_TRY(workspace(ctx, "sort_vals_alt", sizeof(int32_t) * (size_t)n.n, &pv));
     WF_TRY(workspace(ctx, "sort_hist", sizeof(uint32_t) * (size_t)R * passes, &ph));
     WF_TRY(workspace(ctx, "sort_state", sizeof(uint32_t) * (size_t)R * tiles * passes, &pst));
     // [0, kMaxPasses) tile counters, [kMaxPasses] fault flag
@@ -528,27 +540,78 @@ static wf_status sweep_sort(wf_context* ctx, const uint64_t* keys_in, const int3
         k_sweep_bases<BITS><<<passes, kSortBlock, 0, ctx->stream>>>(hist);
         WF_HIP(hipGetLastError());
     }
-    // ping-pong so that the last pass writes keys_out/vals_out
+    // pairs ping-pong so that the last pass writes keys_out/vals_out; packed words alternate
+    // between the two key buffers and the last pass writes vals_out
     const uint64_t* ksrc = keys_in;
     const int32_t* vsrc = vals_in;
+    const uint32_t idx_mask = packed ? (uint32_t)((1ull << idx_bits) - 1ull) : 0u;
     for (int p = 0; p < passes; ++p) {
         const int shift = lo_bit + p * BITS;
         const int nbits = (hi_bit + 1 - shift) < BITS ? (hi_bit + 1 - shift) : BITS;
         const uint32_t mask = (1u << nbits) - 1u;
-        const bool to_out = ((passes - 1 - p) % 2) == 0;
+        const bool last = p == passes - 1;
+        const bool to_out = packed ? (p % 2) == 1 : ((passes - 1 - p) % 2) == 0;
         uint64_t* kdst = to_out ? keys_out : (uint64_t*)pk;
-        int32_t* vdst = to_out ? vals_out : (int32_t*)pv;
+        int32_t* vdst = packed ? vals_out : (to_out ? vals_out : (int32_t*)pv);
+        const uint32_t* hp = hist + (size_t)R * p;
+        uint32_t* sp = state + (size_t)R * tiles * p;
         {
-            Timed t(ctx, tag, 24.0, n);
-            k_sweep_pass<BITS><<<tiles, kSortBlock, 0, ctx->stream>>>(
-                ksrc, vsrc, kdst, vdst, n.n, n.dev, shift, mask, nbits, hist + (size_t)R * p,
-                state + (size_t)R * tiles * p, counters + p, counters + kMaxPasses);
+            Timed t(ctx, tag, packed ? (last ? 12.0 : 16.0) : 24.0, n);
+            if (!packed)
+                k_sweep_pass<BITS, kPairs><<<tiles, kSortBlock, 0, ctx->stream>>>(
+                    ksrc, vsrc, kdst, vdst, n.n, n.dev, shift, mask, nbits, hp, sp, counters + p,
+                    counters + kMaxPasses, 0u);
+            else if (!last)
+                k_sweep_pass<BITS, kPacked><<<tiles, kSortBlock, 0, ctx->stream>>>(
+                    ksrc, nullptr, kdst, nullptr, n.n, n.dev, shift, mask, nbits, hp, sp,
+                    counters + p, counters + kMaxPasses, idx_mask);
+            else
+                k_sweep_pass<BITS, kPackedIndex><<<tiles, kSortBlock, 0, ctx->stream>>>(
+                    ksrc, nullptr, nullptr, vals_out, n.n, n.dev, shift, mask, nbits, hp, sp,
+                    counters + p, counters + kMaxPasses, idx_mask);
             WF_HIP(hipGetLastError());
         }
         ksrc = kdst;
         vsrc = vdst;
     }
     return WF_OK;
+}
+
+wf_status radix_sort_packed(wf_context* ctx, const uint64_t* words, uint64_t* scratch,
+                            int32_t* order, Count n, int32_t idx_bits, int32_t key_bits,
+                            const char* tag) {
+    if (n.n <= 0) return WF_OK;
+    if (idx_bits < 1 || idx_bits > 31 || key_bits < 0 || idx_bits + key_bits > 64)
+        return set_error(WF_ERR_INVALID_ARGUMENT, "radix_sort_packed: %d key bits above %d "
+                         "index bits do not fit a 64-bit word", key_bits, idx_bits);
+    if ((int64_t)n.n > (1ll << idx_bits))
+        return set_error(WF_ERR_INVALID_ARGUMENT, "radix_sort_packed: %d elements need more than "
+                         "%d index bits", n.n, idx_bits);
+    if (key_bits == 0) return iota_count(ctx, order, n);
+    int lo_bit = idx_bits, hi_bit = idx_bits + key_bits - 1;
+    if (!n.dev) {
+        // as for pairs: sort only the key bits where the words differ
+        void* pr = nullptr;
+        WF_TRY(workspace(ctx, "sort_and_or", 16, &pr));
+        unsigned long long init[2] = {~0ull, 0ull};
+        WF_HIP(hipMemcpyAsync(pr, init, sizeof(init), hipMemcpyHostToDevice, ctx->stream));
+        Timed t(ctx, tag, 8.0 * n.n);
+        const int blocks = (int)std::min<int64_t>(grid_for(n.n, 256), 1024);
+        k_key_and_or<<<blocks, 256, 0, ctx->stream>>>(words, n.n, (unsigned long long*)pr);
+        WF_HIP(hipGetLastError());
+        unsigned long long r[2];
+        WF_TRY(readback(ctx, r, pr, sizeof(r)));
+        unsigned long long diff = (r[0] ^ r[1]) >> idx_bits;
+        if (key_bits < 64) diff &= (1ull << key_bits) - 1ull;
+        if (diff == 0) return iota_count(ctx, order, n);
+        lo_bit = idx_bits + __builtin_ctzll(diff);
+        hi_bit = idx_bits + 63 - __builtin_clzll(diff);
+    }
+    switch (ctx->tuning.radix_bits) {
+        case 6: return sweep_sort<6>(ctx, words, nullptr, scratch, order, n, lo_bit, hi_bit, tag, idx_bits);
+        case 11: return sweep_sort<11>(ctx, words, nullptr, scratch, order, n, lo_bit, hi_bit, tag, idx_bits);
+        default: return sweep_sort<8>(ctx, words, nullptr, scratch, order, n, lo_bit, hi_bit, tag, idx_bits);
+    }
 }
 
 wf_status radix_sort_pairs(wf_context* ctx, const uint64_t* keys_in, const int32_t* vals_in,
