@@ -54,6 +54,21 @@ wf_status pinned(wf_context* ctx, size_t bytes, void** out) {
     return WF_OK;
 }
 
+wf_status fault_error() {
+    return set_error(WF_ERR_HIP, "a device-wide wait of the radix sort exceeded its bound (a "
+                     "tile's predecessors never published); the results of the calls since the "
+                     "last synchronisation are invalid");
+}
+
+wf_status check_fault(wf_context* ctx) {
+    int32_t f = 0;
+    WF_TRY(readback(ctx, &f, ctx->fault, sizeof(f)));
+    if (f == 0) return WF_OK;
+    WF_TRY(wf_fill_u32(ctx, ctx->fault, 0u, 1));
+    WF_HIP(hipStreamSynchronize(ctx->stream));
+    return fault_error();
+}
+
 wf_status readback(wf_context* ctx, void* host, const void* dev, size_t bytes) {
     void* p = nullptr;
     WF_TRY(pinned(ctx, bytes, &p));
@@ -154,6 +169,16 @@ wf_status wf_context_create(int32_t device, wf_context** out) {
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->cu_count = prop.multiProcessorCount;
+    hipError_t fe = hipMalloc(&ctx->fault, sizeof(int32_t));
+    if (fe == hipSuccess) fe = hipMemsetAsync(ctx->fault, 0, sizeof(int32_t), ctx->stream);
+    if (fe == hipSuccess) fe = hipStreamSynchronize(ctx->stream);
+    if (fe != hipSuccess) {
+        if (ctx->fault) (void)hipFree(ctx->fault);
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return set_error(WF_ERR_HIP, "wf_context_create: fault word: %s", hipGetErrorString(fe));
+    }
+    ctx->fault_owned = true;
     *out = ctx;
     return WF_OK;
 }
@@ -171,6 +196,7 @@ wf_status wf_context_destroy(wf_context* ctx) {
         if (kv.second.ptr) (void)hipFree(kv.second.ptr);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->count_slots) (void)hipHostFree(ctx->count_slots);
+    if (ctx->fault_owned && ctx->fault) (void)hipFree(ctx->fault);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return WF_OK;
@@ -228,8 +254,7 @@ wf_status wf_memcpy_dtod(wf_context* ctx, void* dst, const void* src, size_t byt
 
 wf_status wf_synchronize(wf_context* ctx) {
     WF_REQUIRE(ctx, "wf_synchronize: NULL context");
-    WF_HIP(hipStreamSynchronize(ctx->stream));
-    return WF_OK;
+    return check_fault(ctx);
 }
 
 wf_status wf_profile_enable(wf_context* ctx, int32_t enable) {

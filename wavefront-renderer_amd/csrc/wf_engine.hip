@@ -76,7 +76,9 @@ constexpr int kMaxAdaptiveLanes = 16;
 
 // Host-visible status block, copied back once per iteration: live rays, ready-queue cursors,
 // then the pixels left of every job slot.
-constexpr int kStatusHeader = 8;   // int32: [0] live rays, [2..3] head, [4..5] tail (u64)
+constexpr int kStatusHeader = 8;   // int32: [0] live rays, [2..3] head, [4..5] tail (u64),
+                                   // [7] fault word of the engine's context
+constexpr int kStatusFault = 7;
 
 struct EngineView {
     int32_t cap;               // ray slots
@@ -731,6 +733,10 @@ wf_status wf_engine_create(wf_context* ctx, wf_scene* scene, const wf_engine_con
     v.jobs = e->jobs_dev;
     ETRY(engine_array(e, (size_t)(kStatusHeader + e->max_jobs), &v.status));
     v.job_left = v.status + kStatusHeader;
+    // the sort's fault word travels back with every iteration's status
+    if (e->ctx->fault_owned) (void)hipFree(e->ctx->fault);
+    e->ctx->fault = v.status + kStatusFault;
+    e->ctx->fault_owned = false;
     ETRY(engine_array(e, (size_t)e->max_jobs, &v.job_max_count));
     ETRY(engine_array(e, (size_t)e->max_jobs, &v.job_unconverged));
     ETRY(wf_fill_u32(e->ctx, v.counters, 0u, C_COUNT));
@@ -890,8 +896,13 @@ void report_unreported(wf_engine* e, int32_t* finished, int32_t cap, int32_t* n)
 }
 
 // Marks the jobs whose pixels are all done in status buffer b as finished.
-void scan_status(wf_engine* e, int b, int32_t* finished, int32_t cap, int32_t* n) {
+// Reports finished jobs; false when the iteration's fault word was set (the word is cleared).
+bool scan_status(wf_engine* e, int b, int32_t* finished, int32_t cap, int32_t* n) {
     const int32_t* st = e->status_host + (size_t)b * (kStatusHeader + e->max_jobs);
+    if (st[kStatusFault] != 0) {
+        (void)wf_fill_u32(e->ctx, e->view.status + kStatusFault, 0u, 1);
+        return false;
+    }
     for (int j = 0; j < e->max_jobs; ++j)
         if (e->job_state[j] == 1 && st[kStatusHeader + j] == 0) {
             e->job_state[j] = 2;
@@ -901,6 +912,7 @@ void scan_status(wf_engine* e, int b, int32_t* finished, int32_t cap, int32_t* n
                 e->job_iter_start[j] = -1;
             }
         }
+    return true;
 }
 
 }  // namespace
@@ -931,7 +943,10 @@ wf_status wf_engine_wait(wf_engine* e, int32_t* finished, int32_t finished_cap,
     report_unreported(e, finished, finished_cap, n_finished);
     if (!e->launch_empty) {
         WF_HIP(hipEventSynchronize(e->status_ev[e->launch_buf]));
-        scan_status(e, e->launch_buf, finished, finished_cap, n_finished);
+        if (!scan_status(e, e->launch_buf, finished, finished_cap, n_finished)) {
+            e->launched = false;
+            return fault_error();
+        }
     }
     e->launched = false;
     merge_profile(e);
@@ -966,7 +981,10 @@ wf_status wf_engine_run(wf_engine* e, int32_t max_iterations, int32_t* finished,
         WF_HIP(hipEventSynchronize(e->status_ev[b]));
         accumulate_phases(e, b);
         ++done_iters;
-        scan_status(e, b, finished, finished_cap, n_finished);
+        if (!scan_status(e, b, finished, finished_cap, n_finished)) {
+            if (more) (void)hipEventSynchronize(e->status_ev[b ^ 1]);
+            return fault_error();
+        }
         if (!more || *n_finished > 0 || e->running == 0) {
             if (more) {
                 WF_HIP(hipEventSynchronize(e->status_ev[b ^ 1]));

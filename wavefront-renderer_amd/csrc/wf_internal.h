@@ -76,6 +76,11 @@ struct wf_context {
     wf_tuning tuning{48, 12, WF_BVH_PLOC, 16, 4, 8, {0, 0}};
     int next_count_slot = 0;
     int cu_count = 256;
+    // Device word that kernels set when a bounded device-wide wait (the radix sort's look-back)
+    // runs out; checked at the next synchronisation (check_fault). Owned by the context, or a
+    // slot of an engine's status block (copied back with every iteration's status).
+    int32_t* fault = nullptr;
+    bool fault_owned = false;
 };
 
 namespace wf {
@@ -86,6 +91,11 @@ wf_status workspace(wf_context* ctx, const char* name, size_t bytes, void** out)
 wf_status pinned(wf_context* ctx, size_t bytes, void** out);
 // Copies `bytes` from device to host through pinned memory and synchronises the stream.
 wf_status readback(wf_context* ctx, void* host, const void* dev, size_t bytes);
+// Synchronises the stream and reads the context's fault word: WF_ERR_HIP (and the word cleared)
+// when a kernel reported an exhausted wait since the last check.
+wf_status check_fault(wf_context* ctx);
+// The error of a set fault word.
+wf_status fault_error();
 
 struct Count;
 constexpr int kCountSlots = 16384;

@@ -319,6 +319,7 @@ extern "C" wf_status wf_render(wf_scene* s, const wf_render_params* rp, float* i
     st.schedule_ms = pt.total(2);
     st.shade_ms = pt.total(3);
     st.miss_ms = pt.total(4);
+    WF_TRY(check_fault(ctx));
     if (stats) *stats = st;
     return WF_OK;
 }

@@ -588,6 +588,7 @@ wf_status wf_scene_create(wf_context* ctx, const wf_scene_desc* desc, wf_scene**
     if ((st = upload_cost_tables(s, nullptr, nullptr)) != WF_OK) return fail(st);
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return fail(set_error(WF_ERR_HIP, "scene upload: %s", hipGetErrorString(e)));
+    if ((st = check_fault(ctx)) != WF_OK) return fail(st);
     *out = s;
     return WF_OK;
 }
@@ -747,6 +748,7 @@ wf_status wf_accel_build(wf_scene* s, wf_build_stats* stats) {
     (void)hipEventElapsedTime(&ms, e0, e1);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    WF_TRY(check_fault(ctx));
     d.built = true;
     if (stats) {
         stats->mesh_count = d.n_meshes;

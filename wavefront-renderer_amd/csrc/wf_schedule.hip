@@ -555,8 +555,7 @@ wf_status wf_ray_schedule(wf_context* ctx, const wf_ray_soa* rays, int32_t n,
     WF_REQUIRE(n >= 0, "wf_ray_schedule: negative ray count");
     WF_TRY(ray_schedule(ctx, rays, host_count(n), bounds_min, bounds_max, sorted_indices,
                         keys_out));
-    WF_HIP(hipStreamSynchronize(ctx->stream));
-    return WF_OK;
+    return check_fault(ctx);
 }
 
 wf_status wf_materialize_rays(wf_context* ctx, const wf_ray_soa* src, const int32_t* idx,
@@ -577,8 +576,7 @@ wf_status wf_shading_schedule(const wf_scene* s, const wf_shade_soa* q, int32_t 
     WF_REQUIRE(policy >= WF_POLICY_NONE && policy <= WF_POLICY_COST_BENEFIT,
                "wf_shading_schedule: unknown policy %d", policy);
     WF_TRY(shading_schedule(s->ctx, s, q, host_count(n), policy, sorted_indices));
-    WF_HIP(hipStreamSynchronize(s->ctx->stream));
-    return WF_OK;
+    return check_fault(s->ctx);
 }
 
 wf_status wf_materialize_shading(wf_context* ctx, const wf_shade_soa* src, const int32_t* idx,

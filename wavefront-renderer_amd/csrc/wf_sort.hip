@@ -1,15 +1,16 @@
-// Device-wide primitives: stable LSD radix sort of (u64 key, i32 value) pairs and an exclusive
-// prefix sum over u32 counts. Both are deterministic: ranks inside a tile are computed with
-// wave-level ballots in tile order, never with atomics whose order could vary.
+// Device-wide primitives: stable LSD radix sort of (u64 key, i32 value) pairs or of packed u64
+// words (key above the element's index), and an exclusive prefix sum over u32 counts. Both are
+// deterministic: ranks inside a tile are computed with wave-level ballots in tile order, never
+// with atomics whose order could vary.
 //
-// Radix sort (6-, 8- or 11-bit digits by tuning, 2048-key tiles, 256 threads = 4 waves of 64):
-// one histogram sweep gives every pass's digit totals; each pass is then a single kernel
-// (k_sweep_pass) in which each wave ranks its 512 keys round by round: lanes holding the same
-// digit are matched with one ballot per digit bit, the rank is the popcount of matching lower
-// lanes plus the wave's running count for that digit; waves are offset by an exclusive prefix
-// over waves (so equal digits keep input order), and tiles by the published counts of all earlier tiles (decoupled look-back). The tile
-// is staged in LDS in output order and written out with consecutive threads on consecutive
-// output slots.
+// Radix sort (6-, 8- or 11-bit digits by tuning; 256 threads = 4 waves of 64; tiles of 2048 pairs
+// or 4096 packed words): one histogram sweep gives every pass's digit totals; each pass is then a
+// single kernel (k_sweep_pass) in which each wave ranks its keys round by round: lanes holding
+// the same digit are matched with one ballot per digit bit, the rank is the popcount of matching
+// lower lanes plus the wave's running count for that digit; waves are offset by an exclusive
+// prefix over waves (so equal digits keep input order), and tiles by the published counts of all
+// earlier tiles (decoupled look-back). The tile is staged in LDS in output order and written out
+// with consecutive threads on consecutive output slots.
 #include <algorithm>
 
 #include "wf_device.h"
@@ -18,13 +19,17 @@
 namespace wf {
 
 constexpr int kSortBlock = 256;
-constexpr int kSortItems = 8;
-constexpr int kSortTile = kSortBlock * kSortItems;
+// keys per thread of a pass: (key, value) pairs stage 12 bytes per key in LDS, packed words 8,
+// so packed tiles are twice as long (half the tiles, half the tile claims and look-back chains)
+template <int PAYLOAD>
+constexpr int sort_items() { return PAYLOAD == 0 ? 8 : 16; }
+template <int PAYLOAD>
+constexpr int sort_tile() { return kSortBlock * sort_items<PAYLOAD>(); }
 constexpr int kSortWaves = kSortBlock / 64;
 
 // ---- one-sweep passes ------------------------------------------------------------------------
 // Digit counts of every pass come from one histogram sweep over the keys; each pass is then one
-// kernel: a block takes the next tile id from a counter, ranks its 2048 keys as above, publishes
+// kernel: a block takes the next tile id from a counter, ranks its tile as above, publishes
 // its per-digit counts, and resolves the counts of all earlier tiles by looking back through the
 // published states (a state is the tile's own count, or the inclusive prefix once known).
 // Every earlier tile id was taken by a block that is already running and publishes its count
@@ -57,13 +62,31 @@ __global__ __launch_bounds__(kSortBlock) void k_sweep_hist(const uint64_t* __res
     __shared__ uint32_t cnt[Digits<BITS>::MaxPasses * R];
     for (int i = threadIdx.x; i < passes * R; i += kSortBlock) cnt[i] = 0;
     __syncthreads();
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint64_t k = keys[i];
-        for (int p = 0; p < passes; ++p) {
-            // the same digit the pass ranks by: bits [shift, min(shift + BITS, hi_bit + 1))
-            const int shift = lo_bit + BITS * p;
-            const int nbits = (hi_bit + 1 - shift) < BITS ? (hi_bit + 1 - shift) : BITS;
-            atomicAdd(&cnt[p * R + ((uint32_t)(k >> shift) & ((1u << nbits) - 1u))], 1u);
+    const int lane = threadIdx.x & 63;
+    const int span = hi_bit - lo_bit + 1;
+    const uint64_t span_mask = span >= 64 ? ~0ull : (1ull << span) - 1ull;
+    // Runs of equal keys are common (every primary ray of a camera shares one origin), and lanes
+    // adding to one LDS counter serialise: the lanes whose sorted bits equal those of the wave's
+    // first lane are counted once, by that lane. Trip counts are wave-uniform (ballots below).
+    for (int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n;
+         base += gridDim.x * blockDim.x) {
+        const int i = base + lane;
+        const bool valid = i < n;
+        const uint64_t k = valid ? keys[i] : 0ull;
+        const uint64_t bits = (k >> lo_bit) & span_mask;
+        const uint32_t b0lo = __shfl((uint32_t)bits, 0, 64);
+        const uint32_t b0hi = __shfl((uint32_t)(bits >> 32), 0, 64);
+        const bool same = valid && bits == (((uint64_t)b0hi << 32) | b0lo);
+        const uint64_t peers = __ballot(same);
+        const bool lead = lane == 0;   // lane 0 is valid whenever base < n
+        if (valid && (lead || !same)) {
+            const uint32_t c = lead ? (uint32_t)__popcll(peers) : 1u;
+            for (int p = 0; p < passes; ++p) {
+                // the same digit the pass ranks by: bits [shift, min(shift + BITS, hi_bit + 1))
+                const int shift = lo_bit + BITS * p;
+                const int nbits = (hi_bit + 1 - shift) < BITS ? (hi_bit + 1 - shift) : BITS;
+                atomicAdd(&cnt[p * R + ((uint32_t)(k >> shift) & ((1u << nbits) - 1u))], c);
+            }
         }
     }
     __syncthreads();
@@ -128,6 +151,7 @@ __global__ __launch_bounds__(kSortBlock) void k_sweep_pass(
     int32_t* __restrict__ fault, uint32_t idx_mask) {
     constexpr int R = Digits<BITS>::R, DPT = Digits<BITS>::DPT;
     constexpr bool kVals = PAYLOAD == kPairs;
+    constexpr int kSortItems = sort_items<PAYLOAD>(), kSortTile = sort_tile<PAYLOAD>();
     // the per-wave digit counts and, once every slot is known, the staged tile share one buffer
     constexpr int kCntBytes = kSortWaves * R * 4, kStageBytes = kSortTile * (kVals ? 12 : 8);
     __shared__ __align__(16) unsigned char lds[kCntBytes > kStageBytes ? kCntBytes : kStageBytes];
@@ -513,18 +537,19 @@ static wf_status sweep_sort(wf_context* ctx, const uint64_t* keys_in, const int3
                             const char* tag, int idx_bits = 0) {
     constexpr int R = Digits<BITS>::R;
     const int span = hi_bit - lo_bit + 1;
-    const int tiles = (n.n + kSortTile - 1) / kSortTile;
+    const bool packed = idx_bits > 0;
+    const int tile_keys = packed ? sort_tile<kPacked>() : sort_tile<kPairs>();
+    const int tiles = (n.n + tile_keys - 1) / tile_keys;
     const int passes = (span + BITS - 1) / BITS;
     if (passes > Digits<BITS>::MaxPasses || passes > kMaxPasses)
         return set_error(WF_ERR_INVALID_ARGUMENT, "sort key too wide");
     void *pk = nullptr, *pv = nullptr, *ph = nullptr, *pst = nullptr, *pc = nullptr;
-    const bool packed = idx_bits > 0;
     WF_TRY(workspace(ctx, "sort_keys_alt", sizeof(uint64_t) * (size_t)n.n, &pk));
     if (!packed) WF_TRY(workspace(ctx, "sort_vals_alt", sizeof(int32_t) * (size_t)n.n, &pv));
     WF_TRY(workspace(ctx, "sort_hist", sizeof(uint32_t) * (size_t)R * passes, &ph));
     WF_TRY(workspace(ctx, "sort_state", sizeof(uint32_t) * (size_t)R * tiles * passes, &pst));
-    // [0, kMaxPasses) tile counters, [kMaxPasses] fault flag
-    WF_TRY(workspace(ctx, "sort_counters", sizeof(int32_t) * (kMaxPasses + 1), &pc));
+    // one tile counter per pass; an exhausted look-back is reported through ctx->fault
+    WF_TRY(workspace(ctx, "sort_counters", sizeof(int32_t) * kMaxPasses, &pc));
     uint32_t* hist = (uint32_t*)ph;
     uint32_t* state = (uint32_t*)pst;
     int32_t* counters = (int32_t*)pc;
@@ -560,15 +585,15 @@ static wf_status sweep_sort(wf_context* ctx, const uint64_t* keys_in, const int3
             if (!packed)
                 k_sweep_pass<BITS, kPairs><<<tiles, kSortBlock, 0, ctx->stream>>>(
                     ksrc, vsrc, kdst, vdst, n.n, n.dev, shift, mask, nbits, hp, sp, counters + p,
-                    counters + kMaxPasses, 0u);
+                    ctx->fault, 0u);
             else if (!last)
                 k_sweep_pass<BITS, kPacked><<<tiles, kSortBlock, 0, ctx->stream>>>(
                     ksrc, nullptr, kdst, nullptr, n.n, n.dev, shift, mask, nbits, hp, sp,
-                    counters + p, counters + kMaxPasses, idx_mask);
+                    counters + p, ctx->fault, idx_mask);
             else
                 k_sweep_pass<BITS, kPackedIndex><<<tiles, kSortBlock, 0, ctx->stream>>>(
                     ksrc, nullptr, nullptr, vals_out, n.n, n.dev, shift, mask, nbits, hp, sp,
-                    counters + p, counters + kMaxPasses, idx_mask);
+                    counters + p, ctx->fault, idx_mask);
             WF_HIP(hipGetLastError());
         }
         ksrc = kdst;
