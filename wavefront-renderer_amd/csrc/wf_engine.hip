@@ -145,7 +145,8 @@ __device__ __forceinline__ int regen_take(const EngineView& e) {
 
 // Primary rays for the next samples of queued pixels, appended after the live rays
 // (generatePrimaryRays, cpp:401-457, per pixel instead of per pass).
-__global__ void k_regen(EngineView e, RayPacked r, int32_t* __restrict__ tags) {
+__global__ void k_regen(EngineView e, RayPacked r, int32_t* __restrict__ tags,
+                        uint64_t* __restrict__ keys, RayKeyFrame f) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int take = regen_take(e);
     if (i >= take) return;
@@ -174,6 +175,7 @@ __global__ void k_regen(EngineView e, RayPacked r, int32_t* __restrict__ tags) {
     ray.pdf = -1.0f;
     st_ray(r, k, ray);
     tags[k] = (j << kTagJobShift) | s;
+    if (keys) keys[k] = ray_sort_word(ray.o, ray.d, f, k);
 }
 
 __global__ void k_regen_commit(EngineView e) {
@@ -405,6 +407,8 @@ struct wf_engine {
     size_t bytes = 0;
     // device state
     RayPacked r0{}, r1{};               // live and sorted rays (packed)
+    uint64_t* keys = nullptr;           // sort word of each live ray (written by its producer)
+    uint64_t* keys2 = nullptr;          // radix sort scratch
     wf_ray_soa miss{};
     wf_shade_soa s0{}, s1{};
     int32_t *t0 = nullptr, *t1 = nullptr, *st0 = nullptr, *st1 = nullptr, *mt = nullptr;
@@ -537,10 +541,14 @@ wf_status enqueue_iteration(wf_engine* e) {
     const int cap = e->cap;
     int32_t* c = v.counters;
     const Count live{cap, c + C_LIVE}, hits{cap, c + C_HITS}, misses{cap, c + C_MISSES};
+    // the producers of live rays (regeneration, the next-ray append) write each ray's sort word
+    const bool sort = e->cfg.ray_sort != 0;
+    const RayKeyFrame kf = ray_key_frame(s->dev.bounds_min, s->dev.bounds_max, index_bits(cap));
     phase_mark(e, 0);
     {
-        Timed tm(ctx, "regen", 56.0, Count{cap, c + C_TAKEN});   // bytes per new path
-        k_regen<<<grid_for(cap, 256), 256, 0, ctx->stream>>>(v, e->r0, e->t0);
+        Timed tm(ctx, "regen", sort ? 64.0 : 56.0, Count{cap, c + C_TAKEN});   // bytes per new path
+        k_regen<<<grid_for(cap, 256), 256, 0, ctx->stream>>>(v, e->r0, e->t0,
+                                                             sort ? e->keys : nullptr, kf);
         WF_HIP(hipGetLastError());
         k_regen_commit<<<1, 1, 0, ctx->stream>>>(v);
         WF_HIP(hipGetLastError());
@@ -548,9 +556,9 @@ wf_status enqueue_iteration(wf_engine* e) {
     phase_mark(e, 1);
     const RayPacked* rq = &e->r0;
     const int32_t* rt = e->t0;
-    if (e->cfg.ray_sort) {
-        WF_TRY(ray_schedule(ctx, &e->r0, live, s->dev.bounds_min, s->dev.bounds_max, e->idx,
-                            nullptr));
+    if (sort) {
+        WF_TRY(radix_sort_packed(ctx, e->keys, e->keys2, e->idx, live, kf.idx_bits, 33,
+                                 "ray_sort"));
         WF_TRY(gather_rays(ctx, &e->r0, e->idx, live, &e->r1, e->t0, e->t1));
         rq = &e->r1;
         rt = e->t1;
@@ -573,6 +581,8 @@ wf_status enqueue_iteration(wf_engine* e) {
     jobs.jobs = e->jobs_dev;
     jobs.next_tags = e->t0;
     jobs.next_packed = e->r0;
+    jobs.next_keys = sort ? e->keys : nullptr;
+    jobs.key_frame = kf;
     jobs.unordered = true;
     WF_TRY(shade_bounce(ctx, s, q, hits, &e->cfg.shade, v.accum, nullptr, c + C_NEXT, jobs,
                         &alive));
@@ -716,6 +726,10 @@ wf_status wf_engine_create(wf_context* ctx, wf_scene* scene, const wf_engine_con
     ETRY(engine_array(e, cap, &e->st1));
     ETRY(engine_array(e, cap, &e->mt));
     ETRY(engine_array(e, cap, &e->idx));
+    if (cfg->ray_sort) {
+        ETRY(engine_array(e, cap, &e->keys));
+        ETRY(engine_array(e, cap, &e->keys2));
+    }
     EngineView& v = e->view;
     v.cap = e->cap;
     v.px = e->px;

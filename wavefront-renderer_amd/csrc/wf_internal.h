@@ -355,6 +355,18 @@ wf_status shading_schedule(wf_context* ctx, const wf_scene* s, const wf_shade_so
 // Job context of a tagged wavefront for the shading kernel: per-entry tags plus the job table.
 // Engine-internal ray queue: three 16-byte columns per ray, a = (origin, pixel bits),
 // b = (direction, pdf), c = (throughput, depth bits).
+// Frame of RayQueue::sortKey (include/scheduling/RayQueue.h:135-162): the scene bounds minimum,
+// the extents max(max - min, 1e-6) per axis, and the number of index bits below the 33-bit key
+// in a packed sort word (0: the bare key).
+struct RayKeyFrame {
+    float lo[3];
+    float ext[3];
+    int32_t idx_bits;
+};
+RayKeyFrame ray_key_frame(const float bmin[3], const float bmax[3], int32_t idx_bits);
+// Smallest b >= 1 with 2^b >= n: the index bits of packed sort words over n elements.
+int32_t index_bits(int64_t n);
+
 struct RayPacked {
     float4* a = nullptr;
     float4* b = nullptr;
@@ -377,6 +389,8 @@ struct ShadeJobs {
     const JobDev* jobs = nullptr;
     int32_t* next_tags = nullptr;      // tags of the compacted next rays
     RayPacked next_packed;             // unordered mode: next rays in the packed layout
+    uint64_t* next_keys = nullptr;     // unordered mode: sort words of the next rays
+    RayKeyFrame key_frame{};           // ... in this frame
     bool unordered = false;            // next rays / shadow entries may come in any order
 };
 // One bounce of shadeAll: emission into accum, shadow rays resolved, next rays compacted into

@@ -61,4 +61,37 @@ __device__ __forceinline__ void st_ray(const RayPacked& r, int i, const RayVal& 
     r.c[i] = make_float4(v.t.x, v.t.y, v.t.z, __int_as_float(v.depth));
 }
 
+// spreadBits10 (RayQueue.h:45-55)
+__device__ __forceinline__ uint32_t spread_bits10(uint32_t v) {
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x30000FFu;
+    v = (v | (v << 8)) & 0x300F00Fu;
+    v = (v | (v << 4)) & 0x30C30C3u;
+    v = (v | (v << 2)) & 0x9249249u;
+    return v;
+}
+
+// std::clamp(v, 0, 1) for non-NaN v
+__device__ __forceinline__ float clamp01(float v) { return v < 0.0f ? 0.0f : (1.0f < v ? 1.0f : v); }
+
+// RayQueue::sortKey (RayQueue.h:135-162): octant << 30 | Morton code of the origin quantised to
+// 10 bits per axis over the scene bounds (33 bits).
+__device__ __forceinline__ uint64_t ray_sort_key(const V3& o, const V3& d, const RayKeyFrame& f) {
+    const uint64_t octant = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) |
+                            (d.z < 0.0f ? 4u : 0u);
+    const uint32_t gx = (uint32_t)(clamp01((o.x - f.lo[0]) / f.ext[0]) * 1023.0f);
+    const uint32_t gy = (uint32_t)(clamp01((o.y - f.lo[1]) / f.ext[1]) * 1023.0f);
+    const uint32_t gz = (uint32_t)(clamp01((o.z - f.lo[2]) / f.ext[2]) * 1023.0f);
+    const uint32_t morton = spread_bits10(gx) | (spread_bits10(gy) << 1) | (spread_bits10(gz) << 2);
+    return (octant << 30) | (uint64_t)morton;
+}
+
+// The sort word of the ray in queue slot `slot`: the key above the slot index, or the bare key
+// when the frame has no index bits.
+__device__ __forceinline__ uint64_t ray_sort_word(const V3& o, const V3& d, const RayKeyFrame& f,
+                                                  int slot) {
+    const uint64_t key = ray_sort_key(o, d, f);
+    return f.idx_bits ? (key << f.idx_bits) | (uint64_t)(uint32_t)slot : key;
+}
+
 }  // namespace wf

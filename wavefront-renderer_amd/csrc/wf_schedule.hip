@@ -13,38 +13,35 @@
 
 namespace wf {
 
-// spreadBits10 (RayQueue.h:45-55)
-__device__ __forceinline__ uint32_t spread_bits10(uint32_t v) {
-    v &= 0x3FFu;
-    v = (v | (v << 16)) & 0x30000FFu;
-    v = (v | (v << 8)) & 0x300F00Fu;
-    v = (v | (v << 4)) & 0x30C30C3u;
-    v = (v | (v << 2)) & 0x9249249u;
-    return v;
+RayKeyFrame ray_key_frame(const float bmin[3], const float bmax[3], int32_t idx_bits) {
+    RayKeyFrame f;
+    for (int k = 0; k < 3; ++k) {
+        // extent = max(max - min, 1e-6) per axis (RayQueue.h:143-145), host float math
+        const float e = bmax[k] - bmin[k];
+        f.lo[k] = bmin[k];
+        f.ext[k] = e < 1e-6f ? 1e-6f : e;
+    }
+    f.idx_bits = idx_bits;
+    return f;
 }
 
-// std::clamp(v, 0, 1) for non-NaN v
-__device__ __forceinline__ float clamp01(float v) { return v < 0.0f ? 0.0f : (1.0f < v ? 1.0f : v); }
+int32_t index_bits(int64_t n) {
+    int32_t b = 1;
+    while ((1ll << b) < n) ++b;
+    return b;
+}
 
-// keys[i] = the 33-bit sort key, or with idx_bits > 0 the packed word key << idx_bits | i (and
-// the bare key in raw[i] when raw is given)
+// keys[i] = the sort word of ray i (the bare key when f.idx_bits is 0); raw[i] = the bare key
+// when raw is given
 template <class R>
-__global__ void k_ray_keys(R rays, int n_host, const int32_t* __restrict__ nd, float minx,
-                           float miny, float minz, float ex, float ey, float ez,
-                           uint64_t* __restrict__ keys, int idx_bits, uint64_t* __restrict__ raw) {
+__global__ void k_ray_keys(R rays, int n_host, const int32_t* __restrict__ nd, RayKeyFrame f,
+                           uint64_t* __restrict__ keys, uint64_t* __restrict__ raw) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= load_count(n_host, nd)) return;
     V3 o, d;
     ld_od(rays, i, o, d);
-    const uint64_t octant = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) |
-                            (d.z < 0.0f ? 4u : 0u);
-    const uint32_t gx = (uint32_t)(clamp01((o.x - minx) / ex) * 1023.0f);
-    const uint32_t gy = (uint32_t)(clamp01((o.y - miny) / ey) * 1023.0f);
-    const uint32_t gz = (uint32_t)(clamp01((o.z - minz) / ez) * 1023.0f);
-    const uint32_t morton = spread_bits10(gx) | (spread_bits10(gy) << 1) | (spread_bits10(gz) << 2);
-    const uint64_t key = (octant << 30) | (uint64_t)morton;
-    keys[i] = idx_bits ? (key << idx_bits) | (uint64_t)i : key;
-    if (raw) raw[i] = key;
+    keys[i] = ray_sort_word(o, d, f, i);
+    if (raw) raw[i] = ray_sort_key(o, d, f);
 }
 
 template <class R>
@@ -52,15 +49,9 @@ static wf_status ray_keys_any(wf_context* ctx, const R& r, Count n, const float 
                               const float bmax[3], uint64_t* keys, int idx_bits = 0,
                               uint64_t* raw = nullptr) {
     if (n.n <= 0) return WF_OK;
-    // extent = max(max - min, 1e-6) per axis (RayQueue.h:143-145), host float math
-    auto ext = [](float hi, float lo) {
-        float e = hi - lo;
-        return e < 1e-6f ? 1e-6f : e;
-    };
     Timed tm(ctx, "ray_keys", 32.0, n);
     k_ray_keys<R><<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(
-        r, n.n, n.dev, bmin[0], bmin[1], bmin[2], ext(bmax[0], bmin[0]), ext(bmax[1], bmin[1]),
-        ext(bmax[2], bmin[2]), keys, idx_bits, raw);
+        r, n.n, n.dev, ray_key_frame(bmin, bmax, idx_bits), keys, raw);
     WF_HIP(hipGetLastError());
     return WF_OK;
 }
@@ -520,8 +511,7 @@ static wf_status ray_schedule_any(wf_context* ctx, const R& rays, Count n, const
                                   const float bmax[3], int32_t* sorted, uint64_t* keys_out) {
     if (n.n <= 0) return WF_OK;
     // packed words: the 33-bit key above the ray's index (stable order on equal keys)
-    int idx_bits = 1;
-    while ((1ll << idx_bits) < (int64_t)n.n) ++idx_bits;
+    const int idx_bits = index_bits(n.n);
     void *pk, *pk2;
     WF_TRY(workspace(ctx, "ray_keys", sizeof(uint64_t) * n.n, &pk));
     WF_TRY(workspace(ctx, "ray_keys2", sizeof(uint64_t) * n.n, &pk2));

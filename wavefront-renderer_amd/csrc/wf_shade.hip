@@ -45,6 +45,8 @@ struct ShadeOut {
     wf_ray_soa next;
     RayPacked next_packed;                             // used instead of `next` when set
     int32_t* next_tags;
+    uint64_t* next_keys;                               // sort word of each next ray, when set
+    RayKeyFrame key_frame;
 };
 
 
@@ -616,6 +618,7 @@ __global__ __launch_bounds__(kAppendBlock) void k_shade_append(wf_shade_soa q, S
             if (o.next_packed.a) st_ray(o.next_packed, k[j], r);
             else st_ray(o.next, k[j], r);
             if (o.next_tags) o.next_tags[k[j]] = tags[i];
+            if (o.next_keys) o.next_keys[k[j]] = ray_sort_word(r.o, r.d, o.key_frame, k[j]);
         }
     }
 }
@@ -803,6 +806,8 @@ static wf_status shade_outputs(wf_context* ctx, int n, ShadeOut& o) {
     o.next = wf_ray_soa{};
     o.next_packed = RayPacked{};
     o.next_tags = nullptr;
+    o.next_keys = nullptr;
+    o.key_frame = RayKeyFrame{};
     return WF_OK;
 }
 
@@ -833,6 +838,8 @@ wf_status shade_bounce(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q
         if (next) o.next = *next;
         o.next_packed = jobs.next_packed;
         o.next_tags = jobs.next_tags;
+        o.next_keys = jobs.next_keys;
+        o.key_frame = jobs.key_frame;
         WF_TRY(wf_fill_u32(ctx, counts_dev, 0u, 3));
         {
             Timed tm(ctx, "shade", 120.0, n);
@@ -842,7 +849,7 @@ wf_status shade_bounce(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q
             WF_HIP(hipGetLastError());
         }
         {
-            Timed tm(ctx, "next_rays", 56.0, n);
+            Timed tm(ctx, "next_rays", o.next_keys ? 64.0 : 56.0, n);
             k_shade_append<<<grid_for(n.n, kAppendBlock * kAppendItems), kAppendBlock, 0,
                              ctx->stream>>>(
                 *q, o, n.n, n.dev, jobs.tags);
