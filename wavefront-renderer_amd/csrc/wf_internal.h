@@ -164,8 +164,11 @@ struct Bvh8Node {
 
 struct SceneDev {
     int32_t n_tris = 0, n_meshes = 0, n_materials = 0, n_textures = 0, n_vertices = 0;
-    // triangles in BVH leaf order: v0 (w = global index bits), e1, e2
+    // triangles in BVH leaf order: v0 (w = global index bits), e1 (w = mesh index bits), e2
     float4* tri = nullptr;
+    // their vertex UVs in leaf order, two records per leaf: (u0, v0, u1, v1), (u2, v2, 0, 0), so
+    // that a hit reads its shading data next to its triangle
+    float4* tri_uv_leaf = nullptr;
     // per global triangle
     int32_t* tri_geom = nullptr;
     float* tri_uv = nullptr;           // 6 per triangle (zeros without UVs)

@@ -183,16 +183,24 @@ __global__ void k_bvh_morton(const float4* __restrict__ lo, const float4* __rest
 __global__ void k_bvh_gather_tris(const int32_t* __restrict__ order, int n,
                                   const float4* __restrict__ v0, const float4* __restrict__ e1,
                                   const float4* __restrict__ e2, const float4* __restrict__ lo,
-                                  const float4* __restrict__ hi, float4* __restrict__ tri,
-                                  float4* __restrict__ slo, float4* __restrict__ shi) {
+                                  const float4* __restrict__ hi,
+                                  const int32_t* __restrict__ tri_geom,
+                                  const float* __restrict__ tri_uv, float4* __restrict__ tri,
+                                  float4* __restrict__ uv_leaf, float4* __restrict__ slo,
+                                  float4* __restrict__ shi) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     int g = order[j];
     float4 a = v0[g];
     a.w = __int_as_float(g);
+    float4 b = e1[g];
+    b.w = __int_as_float(tri_geom[g]);
     tri[3 * (size_t)j] = a;
-    tri[3 * (size_t)j + 1] = e1[g];
+    tri[3 * (size_t)j + 1] = b;
     tri[3 * (size_t)j + 2] = e2[g];
+    const float* uv = tri_uv + 6 * (size_t)g;
+    uv_leaf[2 * (size_t)j] = make_float4(uv[0], uv[1], uv[2], uv[3]);
+    uv_leaf[2 * (size_t)j + 1] = make_float4(uv[4], uv[5], 0.0f, 0.0f);
     slo[j] = lo[g];
     shi[j] = hi[g];
 }
@@ -651,10 +659,12 @@ wf_status wf_accel_build(wf_scene* s, wf_build_stats* stats) {
             void* pt = nullptr;
             WF_TRY(scene_alloc(s, sizeof(float4) * 3 * (size_t)n, &pt));
             d.tri = (float4*)pt;
+            WF_TRY(scene_alloc(s, sizeof(float4) * 2 * (size_t)n, &pt));
+            d.tri_uv_leaf = (float4*)pt;
         }
         k_bvh_gather_tris<<<grid_for(n, 256), 256, 0, ctx->stream>>>(
-            (int32_t*)porder, n, d.tri_v0, d.tri_e1, d.tri_e2, (float4*)plo, (float4*)phi, d.tri,
-            (float4*)pslo, (float4*)pshi);
+            (int32_t*)porder, n, d.tri_v0, d.tri_e1, d.tri_e2, (float4*)plo, (float4*)phi,
+            d.tri_geom, d.tri_uv, d.tri, d.tri_uv_leaf, (float4*)pslo, (float4*)pshi);
         WF_HIP(hipGetLastError());
         int32_t depth = 0;
         // a single triangle always gets the four-wide root

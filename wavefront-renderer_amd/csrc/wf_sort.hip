@@ -65,22 +65,31 @@ __global__ __launch_bounds__(kSortBlock) void k_sweep_hist(const uint64_t* __res
     const int lane = threadIdx.x & 63;
     const int span = hi_bit - lo_bit + 1;
     const uint64_t span_mask = span >= 64 ? ~0ull : (1ull << span) - 1ull;
-    // Runs of equal keys are common (every primary ray of a camera shares one origin), and lanes
-    // adding to one LDS counter serialise: the lanes whose sorted bits equal those of the wave's
-    // first lane are counted once, by that lane. Trip counts are wave-uniform (ballots below).
+    // Runs of equal keys are common (every primary ray of a camera shares one origin, up to its
+    // direction octant), and lanes adding to one LDS counter serialise: up to kHistLeaders times,
+    // the first lane not yet counted gathers the lanes whose sorted bits equal its own and counts
+    // them at once; the lanes left over count themselves. Trip counts are wave-uniform (ballots).
+    constexpr int kHistLeaders = 1;
     for (int base = blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n;
          base += gridDim.x * blockDim.x) {
         const int i = base + lane;
         const bool valid = i < n;
         const uint64_t k = valid ? keys[i] : 0ull;
         const uint64_t bits = (k >> lo_bit) & span_mask;
-        const uint32_t b0lo = __shfl((uint32_t)bits, 0, 64);
-        const uint32_t b0hi = __shfl((uint32_t)(bits >> 32), 0, 64);
-        const bool same = valid && bits == (((uint64_t)b0hi << 32) | b0lo);
-        const uint64_t peers = __ballot(same);
-        const bool lead = lane == 0;   // lane 0 is valid whenever base < n
-        if (valid && (lead || !same)) {
-            const uint32_t c = lead ? (uint32_t)__popcll(peers) : 1u;
+        uint64_t left = __ballot(valid);   // lanes not counted yet
+        uint32_t c = 1u;                   // what this lane adds when it counts
+        bool counted = false;
+        for (int r = 0; r < kHistLeaders && left; ++r) {
+            const int leader = __builtin_ctzll(left);
+            const uint32_t llo = __shfl((uint32_t)bits, leader, 64);
+            const uint32_t lhi = __shfl((uint32_t)(bits >> 32), leader, 64);
+            const bool same = !counted && valid && bits == (((uint64_t)lhi << 32) | llo);
+            const uint64_t peers = __ballot(same);
+            if (lane == leader) c = (uint32_t)__popcll(peers);
+            else if (same) counted = true;
+            left &= ~peers;
+        }
+        if (valid && !counted) {
             for (int p = 0; p < passes; ++p) {
                 // the same digit the pass ranks by: bits [shift, min(shift + BITS, hi_bit + 1))
                 const int shift = lo_bit + BITS * p;
@@ -543,19 +552,21 @@ static wf_status sweep_sort(wf_context* ctx, const uint64_t* keys_in, const int3
     const int passes = (span + BITS - 1) / BITS;
     if (passes > Digits<BITS>::MaxPasses || passes > kMaxPasses)
         return set_error(WF_ERR_INVALID_ARGUMENT, "sort key too wide");
-    void *pk = nullptr, *pv = nullptr, *ph = nullptr, *pst = nullptr, *pc = nullptr;
+    void *pk = nullptr, *pv = nullptr, *pm = nullptr;
     WF_TRY(workspace(ctx, "sort_keys_alt", sizeof(uint64_t) * (size_t)n.n, &pk));
     if (!packed) WF_TRY(workspace(ctx, "sort_vals_alt", sizeof(int32_t) * (size_t)n.n, &pv));
-    WF_TRY(workspace(ctx, "sort_hist", sizeof(uint32_t) * (size_t)R * passes, &ph));
-    WF_TRY(workspace(ctx, "sort_state", sizeof(uint32_t) * (size_t)R * tiles * passes, &pst));
-    // one tile counter per pass; an exhausted look-back is reported through ctx->fault
-    WF_TRY(workspace(ctx, "sort_counters", sizeof(int32_t) * kMaxPasses, &pc));
-    uint32_t* hist = (uint32_t*)ph;
-    uint32_t* state = (uint32_t*)pst;
-    int32_t* counters = (int32_t*)pc;
-    WF_TRY(wf_fill_u32(ctx, hist, 0u, (size_t)R * passes));
-    WF_TRY(wf_fill_u32(ctx, state, 0u, (size_t)R * tiles * passes));
-    WF_TRY(wf_fill_u32(ctx, counters, 0u, kMaxPasses));
+    // one zero-filled block: a tile counter per pass (an exhausted look-back is reported through
+    // ctx->fault), the digit totals of every pass, the look-back states of every pass
+    // (each part starts on a 256-byte boundary: a tile's state row must not share cache lines
+    // with another part)
+    auto up = [](size_t words) { return (words + 63) & ~(size_t)63; };
+    const size_t off_hist = up(kMaxPasses), off_state = off_hist + up((size_t)R * passes);
+    const size_t meta = off_state + (size_t)R * tiles * passes;
+    WF_TRY(workspace(ctx, "sort_meta", sizeof(uint32_t) * meta, &pm));
+    int32_t* counters = (int32_t*)pm;
+    uint32_t* hist = (uint32_t*)pm + off_hist;
+    uint32_t* state = (uint32_t*)pm + off_state;
+    WF_TRY(wf_fill_u32(ctx, pm, 0u, meta));
     {
         Timed t(ctx, tag, 8.0, n);
         const unsigned blocks = std::min<unsigned>(grid_for(n.n, kSortBlock), 1024u);

@@ -108,14 +108,15 @@ __device__ __forceinline__ void map_hit(const SceneDev& s, const R& in, int i, i
     const float4 C = s.tri[3 * (size_t)L + 2];
     const int32_t g = __float_as_int(A.w);
     const V3 P = o + d * t;
-    const int32_t geom = s.tri_geom[g];
+    const int32_t geom = __float_as_int(B.w);
     const int32_t mat = s.mesh_material[geom];
     float u = bu, v = bv;
     if (s.mesh_has_uv[geom]) {
-        const float* uv = s.tri_uv + 6 * (size_t)g;
+        // Mesh::interpolateUV (src/geometry/Mesh.cpp:46-63), UVs stored beside the leaf
+        const float4 ua = s.tri_uv_leaf[2 * (size_t)L], ub = s.tri_uv_leaf[2 * (size_t)L + 1];
         const float w0 = 1.0f - bu - bv;
-        u = w0 * uv[0] + bu * uv[2] + bv * uv[4];
-        v = w0 * uv[1] + bu * uv[3] + bv * uv[5];
+        u = w0 * ua.x + bu * ua.z + bv * ub.x;
+        v = w0 * ua.y + bu * ua.w + bv * ub.y;
     }
     const V3 outward = xyz(B).cross(xyz(C)).normalized();
     const bool ff = d.dot(outward) < 0.0f;
@@ -229,7 +230,7 @@ __global__ void k_hits_to_soa(const SceneDev s, int n, const int32_t* __restrict
     const int L = leaf[i];
     if (L >= 0) {
         const int32_t g = __float_as_int(s.tri[3 * (size_t)L].w);
-        const int32_t geom = s.tri_geom[g];
+        const int32_t geom = __float_as_int(s.tri[3 * (size_t)L + 1].w);
         out.geom[i] = geom;
         out.prim[i] = g - s.mesh_tri_offset[geom];
     } else {
