@@ -15,9 +15,10 @@
 // different sizes share one wavefront.
 //
 // One iteration (all counts stay on the device, so the sequence can be captured in a graph):
-//   regenerate -> [ray keys + radix sort + gather] -> closest hit + split -> [shading policy sort
-//   + gather] -> shade (emission, NEE, scatter, RR, depth limit) -> shadow rays -> next-ray
-//   compaction -> misses -> per-pixel completion (adaptive sampling, sample budget, requeue).
+//   regenerate -> [radix sort of the rays' sort words + gather] -> closest hit + split ->
+//   [shading policy sort + gather] -> shade (emission, NEE, scatter, RR, depth limit) -> shadow
+//   rays -> next-ray compaction -> misses -> per-pixel completion (adaptive sampling, sample
+//   budget, requeue).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
