@@ -274,6 +274,14 @@ wf_status wf_shading_schedule(const wf_scene* scene, const wf_shade_soa* queue, 
 /* ShadingQueue::materialize (ShadingQueue.h:233-275), out of place. */
 wf_status wf_materialize_shading(wf_context* ctx, const wf_shade_soa* src, const int32_t* idx,
                                  int32_t n, wf_shade_soa* dst);
+/* The coherence metrics of one shading queue in its shading order (renderScene's batch
+ * statistics, cpp:67-92 computeAverageRunLength and cpp:201-246 cacheLineHomogeneity):
+ * out[0] material run length and out[1] texture run length over the entries with a texture
+ * (texture_ids >= 0), out[2] / out[3] the fraction of 16-entry lines of material / texture IDs
+ * holding one ID (0 when cache_lines is 0). Device arrays of n IDs; out is host memory. */
+wf_status wf_coherence_stats(wf_context* ctx, const int32_t* material_ids,
+                             const int32_t* texture_ids, int32_t n, int32_t cache_lines,
+                             double out[4]);
 
 /* ---- shading --------------------------------------------------------------------------------- */
 
@@ -323,9 +331,21 @@ typedef struct wf_render_params {
                                   (a power of two <= 64; 0 = 64); see
                                   wf_engine_config::sample_lanes */
     int32_t wavefront;         /* REGENERATE: ray slots (paths in flight); 0 = automatic */
+    int32_t coherence_stats;   /* WF_COHERENCE_*: the reference's coherence metrics of every
+                                  shading batch (cpp:187-246), returned in wf_render_stats */
     wf_shade_params shade;
     wf_camera camera;
 } wf_render_params;
+
+/* wf_render_params::coherence_stats. RUN_LENGTH: average material and texture run lengths of
+ * every shading batch in shading order, over hits with a texture (computeAverageRunLength,
+ * cpp:67-92), weighted by batch size (cpp:187-199). CACHE_LINES adds the fraction of 16-int
+ * (64-byte) lines of the batch's material and texture IDs holding one ID (enableMemoryCoherenceStats,
+ * cpp:201-246). A batch is a (pass, depth) shading queue of the pass loop, or one iteration's
+ * shading queue of the regenerating schedule. */
+#define WF_COHERENCE_OFF 0
+#define WF_COHERENCE_RUN_LENGTH 1
+#define WF_COHERENCE_CACHE_LINES 2
 
 typedef struct wf_render_stats {  /* the reference's stats blocks (cpp:321-370) plus ray counts */
     int64_t rays_traced;       /* sum over bounces of rays entering the bounce */
@@ -340,6 +360,12 @@ typedef struct wf_render_stats {  /* the reference's stats blocks (cpp:321-370) 
     double schedule_ms;        /* shading-queue policy sort + materialize */
     double shade_ms;           /* shade + shadow resolve */
     double miss_ms;
+    /* with coherence_stats (else 0): "Average material run length", "Average texture run
+     * length", "Material ID cache line homogeneity", "Texture  ID cache line homogeneity" */
+    double material_run_length;
+    double texture_run_length;
+    double material_homogeneity;
+    double texture_homogeneity;
 } wf_render_stats;
 
 /* wf_render_params::schedule. REGENERATE keeps a full wavefront: a pixel starts its next sample

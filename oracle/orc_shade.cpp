@@ -935,6 +935,39 @@ void orc_shade_misses(void* sp, const wf_ray_soa* q, int32_t n, int32_t physics,
     }
 }
 
+// computeAverageRunLength (src/render/WavefrontRenderer.cpp:67-92): entries without a texture
+// are skipped; a run starts where the ID differs from the previous kept entry's.
+double orc_run_length(int32_t n, const int32_t* ids, const int32_t* texture_ids) {
+    if (n == 0) return 0.0;
+    int runs = 0, kept = 0, prev = -2;
+    for (int i = 0; i < n; ++i) {
+        if (texture_ids[i] < 0) continue;
+        if (ids[i] != prev) ++runs;
+        prev = ids[i];
+        ++kept;
+    }
+    return kept == 0 ? 0.0 : static_cast<double>(kept) / runs;
+}
+
+// cacheLineHomogeneity (src/render/WavefrontRenderer.cpp:210-234): lines of 16 IDs holding one
+// ID over all lines.
+double orc_line_homogeneity(int32_t n, const int32_t* ids) {
+    if (n == 0) return 0.0;
+    int mono = 0, lines = 0;
+    for (int start = 0; start < n; start += 16) {
+        const int end = std::min(start + 16, n);
+        bool same = true;
+        for (int j = start + 1; j < end; ++j)
+            if (ids[j] != ids[start]) {
+                same = false;
+                break;
+            }
+        mono += same ? 1 : 0;
+        ++lines;
+    }
+    return static_cast<double>(mono) / lines;
+}
+
 // AdaptiveSampler::addSample (AdaptiveSampler.h:40-53).
 void orc_adaptive_add(float* mean, float* m2, int32_t* count, int32_t i, float x) {
     count[i] += 1;
@@ -986,6 +1019,9 @@ int32_t orc_render(void* sp, const wf_render_params* rp, const double* mat_cost,
     auto now = [] { return std::chrono::steady_clock::now(); };
     auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
     auto t_frame = now();
+    // coherence batch sums (cpp:154-160, 187-246)
+    double coh[4] = {0.0, 0.0, 0.0, 0.0};
+    long long coh_weight = 0;
     for (int sample = 0; sample < rp->spp; ++sample) {
         if (rp->adaptive) before = accum;
         std::vector<int32_t> pix;
@@ -1063,6 +1099,15 @@ int32_t orc_render(void* sp, const wf_render_params* rp, const double* mat_cost,
                 for (int i = 0; i < nh; ++i) si[c][i] = isrc[c][order[i]];
             }
             for (int i = 0; i < nh; ++i) sfront[i] = hfront[order[i]];
+            if (rp->coherence_stats && nh > 0) {
+                coh[0] += orc_run_length(nh, si[1].data(), si[2].data()) * nh;
+                coh[1] += orc_run_length(nh, si[2].data(), si[2].data()) * nh;
+                if (rp->coherence_stats >= WF_COHERENCE_CACHE_LINES) {
+                    coh[2] += orc_line_homogeneity(nh, si[1].data()) * nh;
+                    coh[3] += orc_line_homogeneity(nh, si[2].data()) * nh;
+                }
+                coh_weight += nh;
+            }
             wf_shade_soa ss{sb[0].data(), sb[1].data(), sb[2].data(), sb[3].data(), sb[4].data(),
                             sb[5].data(), sb[6].data(), sb[7].data(), sb[8].data(), si[0].data(),
                             si[1].data(), si[2].data(), sb[9].data(), sb[10].data(),
@@ -1130,6 +1175,12 @@ int32_t orc_render(void* sp, const wf_render_params* rp, const double* mat_cost,
     st.samples_run = samples_run;
     st.active_pixels_end = rp->adaptive ? act : 0;
     st.frame_ms = ms(t_frame, now());
+    if (coh_weight > 0) {
+        st.material_run_length = coh[0] / coh_weight;
+        st.texture_run_length = coh[1] / coh_weight;
+        st.material_homogeneity = coh[2] / coh_weight;
+        st.texture_homogeneity = coh[3] / coh_weight;
+    }
     if (stats) *stats = st;
     return 0;
 }

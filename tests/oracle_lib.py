@@ -68,6 +68,10 @@ class Oracle:
         L.orc_hash_grid_cell.restype = C.c_uint32
         L.orc_hash_grid_cell.argtypes = [C.c_int32] * 3
         L.orc_gather_u32.argtypes = [C.c_int32, P, P, P]
+        L.orc_run_length.restype = C.c_double
+        L.orc_run_length.argtypes = [C.c_int32, P, P]
+        L.orc_line_homogeneity.restype = C.c_double
+        L.orc_line_homogeneity.argtypes = [C.c_int32, P]
 
     # -- RNG
     def philox(self, ctr, key):
@@ -98,6 +102,16 @@ class Oracle:
         out = np.zeros(n, np.int32)
         self.lib.orc_ray_schedule(n, *[fptr(c) for c in cols], fptr(bmin), fptr(bmax), fptr(out))
         return out
+
+    def coherence(self, mat, tex):
+        """computeAverageRunLength of material and texture IDs, cacheLineHomogeneity of both."""
+        m = np.ascontiguousarray(mat, np.int32)
+        t = np.ascontiguousarray(tex, np.int32)
+        n = len(m)
+        return (self.lib.orc_run_length(n, fptr(m), fptr(t)),
+                self.lib.orc_run_length(n, fptr(t), fptr(t)),
+                self.lib.orc_line_homogeneity(n, fptr(m)),
+                self.lib.orc_line_homogeneity(n, fptr(t)))
 
     def hit_morton(self, p, lo, hi):
         return self.lib.orc_hit_morton(*[float(x) for x in (*p, *lo, *hi)])

@@ -464,3 +464,57 @@ def test_render_rejects_bad_arguments(wf, ctx, cornell):
     p = wf.render_params(camera=data.camera, width=1, height=4, spp=1, max_depth=2)
     with pytest.raises(wf.WavefrontError):
         wf.render(scene, p)
+
+
+# ---- coherence metrics (src/render/WavefrontRenderer.cpp:67-92, 187-246) -----------------------
+@pytest.mark.parametrize("n", [1, 15, 16, 17, 1000, 70_001])
+@pytest.mark.parametrize("layout", ["random", "runs"])
+def test_coherence_stats_matches_oracle(wf, ctx, orc, rng, n, layout):
+    mat = rng.integers(0, 6, n).astype(np.int32)
+    tex = np.where(rng.random(n) < 0.3, -1, rng.integers(0, 4, n)).astype(np.int32)
+    if layout == "runs":   # long runs, as after a material-aware sort
+        mat = np.sort(mat)
+        tex = np.repeat(rng.integers(-1, 4, (n + 99) // 100), 100)[:n].astype(np.int32)
+    got = wf.coherence_stats(ctx, mat, tex)
+    assert got == orc.coherence(mat, tex), (got, orc.coherence(mat, tex))
+    assert wf.coherence_stats(ctx, mat, tex, cache_lines=False)[2:] == (0.0, 0.0)
+
+
+def test_coherence_stats_empty(wf, ctx):
+    assert wf.coherence_stats(ctx, np.zeros(0, np.int32), np.zeros(0, np.int32)) == (0.0,) * 4
+
+
+@pytest.fixture(scope="module")
+def textured(wf, ctx, scenes, orc):
+    data = scenes.sorting_stress(subdiv=3, n_materials=8, n_textures=4, tex_size=16)
+    return data, wf.Scene(ctx, data), orc.scene(data)
+
+
+@pytest.mark.parametrize("policy", ["none", "material", "texture"])
+def test_render_coherence_matches_oracle(wf, textured, policy):
+    # depth 1: the shading queues are the primary hits, bit-identical on both sides
+    data, scene, osc = textured
+    kw = dict(width=48, height=40, spp=3, max_depth=1, firefly=1e30, seed=9, policy=policy,
+              schedule="passes", physics=wf.PHYSICS_UNBIASED)
+    img, _, st, _, _, ost = render_both(wf, osc, scene, data, coherence_stats=2, **kw)
+    got = (st.material_run_length, st.texture_run_length, st.material_homogeneity,
+           st.texture_homogeneity)
+    want = (ost.material_run_length, ost.texture_run_length, ost.material_homogeneity,
+            ost.texture_homogeneity)
+    assert st.shaded_hits == ost.shaded_hits > 0
+    assert got == want and all(v > 0 for v in got), (got, want)
+    # statistics change no result, and are zero when not asked for
+    img0, _, st0 = wf.render(scene, wf.render_params(camera=data.camera, **kw))
+    assert np.array_equal(img, img0)
+    assert st0.material_run_length == 0.0 and st0.texture_homogeneity == 0.0
+
+
+def test_render_coherence_regenerate(wf, textured):
+    data, scene, _ = textured
+    kw = dict(width=48, height=40, spp=4, max_depth=4, firefly=1e30, seed=3, policy="material",
+              physics=wf.PHYSICS_UNBIASED)
+    img1, _, st1 = wf.render(scene, wf.render_params(camera=data.camera, coherence_stats=1, **kw))
+    img0, _, st0 = wf.render(scene, wf.render_params(camera=data.camera, **kw))
+    assert np.array_equal(img0, img1)
+    assert st1.material_run_length >= 1.0 and st1.texture_run_length >= 1.0
+    assert st1.material_homogeneity == 0.0 and st0.material_run_length == 0.0

@@ -233,3 +233,31 @@ def test_luminance_and_srgb_constants():
     v = 191 / 255.0
     lin = ((v + 0.055) / 1.055) ** 2.4
     assert abs(lin - 0.52) < 0.01
+
+
+# ---- coherence metrics (src/render/WavefrontRenderer.cpp:67-92, 201-246) ---------------------
+@pytest.mark.parametrize("mat, tex, expect", [
+    # hand-counted: kept entries / runs of equal IDs among them, for material then texture IDs
+    ([1, 1, 2, 2, 2, 3], [0, 0, 0, 0, 0, 0], (6 / 3, 6 / 1)),
+    ([5, 7, 5, 5], [0, -1, 0, 1], (3 / 1, 3 / 2)),
+    # a skipped entry does not end a run: the kept materials are 1, 1
+    ([1, 2, 1], [0, -1, 0], (2 / 1, 2 / 1)),
+    ([4, 4], [-1, -1], (0.0, 0.0)),   # nothing kept
+    ([], [], (0.0, 0.0)),
+    ([0, 1, 0, 1, 0], [3, 3, 3, 3, 3], (5 / 5, 5 / 1)),
+])
+def test_run_length_kats(orc, mat, tex, expect):
+    got = orc.coherence(mat, tex)
+    assert got[0] == expect[0] and got[1] == expect[1]
+
+
+def test_line_homogeneity_kats(orc):
+    ids = np.full(32, 7, np.int32)
+    assert orc.coherence(ids, ids)[2:] == (1.0, 1.0)
+    one_off = ids.copy()
+    one_off[17] = 8   # second line mixed
+    assert orc.coherence(one_off, ids)[2] == 0.5
+    tail = np.concatenate([ids, [9]]).astype(np.int32)   # a one-entry last line counts as one ID
+    assert orc.coherence(tail, tail)[2] == 1.0
+    assert orc.coherence(np.arange(20, dtype=np.int32), ids[:20])[2:] == (0.0, 1.0)
+    assert orc.coherence([], [])[2:] == (0.0, 0.0)

@@ -324,6 +324,27 @@ def materialize_shading(ctx: Context, q: ShadingQueue, idx: np.ndarray, n: int) 
     return out
 
 
+COHERENCE_OFF, COHERENCE_RUN_LENGTH, COHERENCE_CACHE_LINES = 0, 1, 2
+
+
+def coherence_stats(ctx: Context, material_ids: np.ndarray, texture_ids: np.ndarray,
+                    cache_lines: bool = True) -> tuple:
+    """The coherence metrics of one shading queue in its order (renderScene's batch statistics,
+    src/render/WavefrontRenderer.cpp:67-92, 201-246): (material run length, texture run length,
+    material cache-line homogeneity, texture cache-line homogeneity)."""
+    n = int(len(material_ids))
+    out = (C.c_double * 4)()
+    if n == 0:
+        check(ctx._lib.wf_coherence_stats(ctx.handle, None, None, 0, 1 if cache_lines else 0,
+                                          out))
+        return tuple(out)
+    m = DeviceBuffer.from_array(ctx, np.ascontiguousarray(material_ids, np.int32))
+    t = DeviceBuffer.from_array(ctx, np.ascontiguousarray(texture_ids, np.int32))
+    check(ctx._lib.wf_coherence_stats(ctx.handle, C.c_void_p(m.ptr), C.c_void_p(t.ptr), n,
+                                      1 if cache_lines else 0, out))
+    return tuple(out)
+
+
 def shade_params(physics=PHYSICS_REFERENCE, rr_min_depth=3, cost_aware_rr=False,
                  firefly=10.0, seed=0, sample=0, job=0) -> _abi.ShadeParams:
     return _abi.ShadeParams(physics, rr_min_depth, 1 if cost_aware_rr else 0, firefly, seed,
@@ -358,17 +379,19 @@ def render_params(width, height, spp, max_depth, camera, policy="none", ray_sort
                   adaptive=False, adaptive_threshold=0.05, adaptive_min_samples=32,
                   adaptive_interval=16, physics=PHYSICS_REFERENCE, rr_min_depth=3,
                   cost_aware_rr=False, firefly=10.0, seed=0, job=0,
-                  schedule="regenerate", sample_lanes=0, wavefront=0) -> _abi.RenderParams:
+                  schedule="regenerate", sample_lanes=0, wavefront=0,
+                  coherence_stats=0) -> _abi.RenderParams:
     """Regenerating schedule: sample_lanes = samples of a pixel in flight at once (a power of
     two <= 64, 0 for the default 64; 1 reproduces the pass loop's accumulation order bit for
-    bit); wavefront = ray slots (0 = automatic)."""
+    bit); wavefront = ray slots (0 = automatic). coherence_stats: 0 off, 1 material/texture run
+    lengths, 2 also cache-line homogeneity (COHERENCE_*), returned in the stats."""
     pol = POLICIES[policy] if isinstance(policy, str) else int(policy)
     sched = SCHEDULES[schedule] if isinstance(schedule, str) else int(schedule)
     sp = shade_params(physics, rr_min_depth, cost_aware_rr, firefly, seed, 0, job)
     return _abi.RenderParams(width, height, spp, max_depth, pol, 1 if ray_sort else 0,
                              1 if adaptive else 0, adaptive_threshold, adaptive_min_samples,
-                             adaptive_interval, sched, sample_lanes, wavefront, sp,
-                             camera_struct(camera))
+                             adaptive_interval, sched, sample_lanes, wavefront,
+                             int(coherence_stats), sp, camera_struct(camera))
 
 
 def render(scene: Scene, params: _abi.RenderParams, keep_on_device: bool = False):

@@ -116,7 +116,7 @@ class RenderParams(C.Structure):
                 ("adaptive", C.c_int32), ("adaptive_threshold", C.c_float),
                 ("adaptive_min_samples", C.c_int32), ("adaptive_interval", C.c_int32),
                 ("schedule", C.c_int32), ("sample_lanes", C.c_int32), ("wavefront", C.c_int32),
-                ("shade", ShadeParams), ("camera", Camera)]
+                ("coherence_stats", C.c_int32), ("shade", ShadeParams), ("camera", Camera)]
 
 
 class RenderStats(C.Structure):
@@ -125,7 +125,9 @@ class RenderStats(C.Structure):
                 ("samples_run", C.c_int32), ("active_pixels_end", C.c_int32),
                 ("frame_ms", C.c_double), ("sort_ms", C.c_double),
                 ("intersect_ms", C.c_double), ("schedule_ms", C.c_double),
-                ("shade_ms", C.c_double), ("miss_ms", C.c_double)]
+                ("shade_ms", C.c_double), ("miss_ms", C.c_double),
+                ("material_run_length", C.c_double), ("texture_run_length", C.c_double),
+                ("material_homogeneity", C.c_double), ("texture_homogeneity", C.c_double)]
 
 
 class EngineConfig(C.Structure):
@@ -200,6 +202,7 @@ _PROTOS = {
     "wf_scene_set_costs": (_S, [P, P, P]),
     "wf_shading_schedule": (_S, [P, C.POINTER(ShadeSoA), C.c_int32, C.c_int32, P]),
     "wf_materialize_shading": (_S, [P, C.POINTER(ShadeSoA), P, C.c_int32, C.POINTER(ShadeSoA)]),
+    "wf_coherence_stats": (_S, [P, P, P, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
     "wf_shade": (_S, [P, C.POINTER(ShadeSoA), C.c_int32, C.POINTER(ShadeParams), P,
                       C.POINTER(RaySoA), C.POINTER(C.c_int32)]),
     "wf_shade_misses": (_S, [P, C.POINTER(RaySoA), C.c_int32, C.c_int32, P]),

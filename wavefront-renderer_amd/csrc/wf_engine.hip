@@ -410,6 +410,9 @@ struct wf_engine {
     RayPacked r0{}, r1{};               // live and sorted rays (packed)
     uint64_t* keys = nullptr;           // sort word of each live ray (written by its producer)
     uint64_t* keys2 = nullptr;          // radix sort scratch
+    // coherence metrics of every iteration's shading queue (set by wf_render; 0 = off)
+    int coherence_mode = 0;
+    double* coherence = nullptr;        // 9 doubles, see coherence_accumulate
     wf_ray_soa miss{};
     wf_shade_soa s0{}, s1{};
     int32_t *t0 = nullptr, *t1 = nullptr, *st0 = nullptr, *st1 = nullptr, *mt = nullptr;
@@ -575,6 +578,9 @@ wf_status enqueue_iteration(wf_engine* e) {
         q = &e->s1;
         qt = e->st1;
     }
+    if (e->coherence_mode)
+        WF_TRY(coherence_accumulate(ctx, q->mat, q->tex, hits,
+                                    e->coherence_mode >= WF_COHERENCE_CACHE_LINES, e->coherence));
     phase_mark(e, 4);
     const uint32_t* alive = nullptr;
     ShadeJobs jobs;
@@ -1130,6 +1136,13 @@ wf_status render_regenerate(wf_scene* s, const wf_render_params* rp, float* imag
             e->cfg = cfg;
         }
     }
+    if (rp->coherence_stats && !e->coherence)
+        WF_TRY(engine_alloc(e, sizeof(double) * 9, (void**)&e->coherence));
+    if (e->coherence_mode != rp->coherence_stats) {
+        drop_graph(e);   // the captured iteration bakes the statistics launches in or out
+        e->coherence_mode = rp->coherence_stats;
+    }
+    if (e->coherence) WF_TRY(wf_fill_u32(e->ctx, e->coherence, 0u, 18));
     // phase times only while the context times kernels (they need events between the phases,
     // which a replayed graph does not have)
     WF_TRY(wf_engine_phase_timing(e, s->ctx->profile ? 1 : 0));
@@ -1177,6 +1190,7 @@ wf_status render_regenerate(wf_scene* s, const wf_render_params* rp, float* imag
         r.schedule_ms = st1.phase_ms[2];
         r.shade_ms = st1.phase_ms[3];
         r.miss_ms = st1.phase_ms[4];
+        if (rp->coherence_stats) WF_TRY(coherence_read(e->ctx, e->coherence, &r));
         *stats = r;
     }
     return WF_OK;

@@ -319,6 +319,13 @@ wf_status radix_sort_pairs(wf_context* ctx, const uint64_t* keys_in, const int32
 wf_status radix_sort_packed(wf_context* ctx, const uint64_t* words, uint64_t* scratch,
                             int32_t* order, Count n, int32_t idx_bits, int32_t key_bits,
                             const char* tag);
+// Coherence metrics of one shading batch (wf_coherence.hip): adds the batch's material/texture
+// run lengths (and, with lines, cache-line homogeneities) times its size into acc[0..3], its
+// size into acc[4], and stores the batch's own values in acc[5..8] (9 device doubles).
+wf_status coherence_accumulate(wf_context* ctx, const int32_t* mat, const int32_t* tex, Count n,
+                               bool lines, double* acc);
+// Reads acc back and stores the size-weighted averages in st (left 0 without batches).
+wf_status coherence_read(wf_context* ctx, const double* acc, wf_render_stats* st);
 // Exclusive prefix sum of n u32 flags/counts into out (may alias in); total (optional) is
 // written to a device int32.
 wf_status exclusive_scan_u32(wf_context* ctx, const uint32_t* in, uint32_t* out, Count n,

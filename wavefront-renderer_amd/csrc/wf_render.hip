@@ -176,6 +176,9 @@ extern "C" wf_status wf_render(wf_scene* s, const wf_render_params* rp, float* i
     WF_REQUIRE(!rp->adaptive || rp->adaptive_interval > 0, "wf_render: adaptive_interval must be > 0");
     WF_REQUIRE(rp->schedule == WF_SCHEDULE_REGENERATE || rp->schedule == WF_SCHEDULE_PASSES,
                "wf_render: unknown schedule %d", rp->schedule);
+    WF_REQUIRE(rp->coherence_stats >= WF_COHERENCE_OFF &&
+                   rp->coherence_stats <= WF_COHERENCE_CACHE_LINES,
+               "wf_render: unknown coherence_stats %d", rp->coherence_stats);
     WF_TRY(scene_ready(s));
     WF_REQUIRE((int64_t)rp->width * rp->height < (1ll << 30), "wf_render: image too large");
     if (rp->schedule == WF_SCHEDULE_REGENERATE) return render_regenerate(s, rp, image_rgb, sample_counts, stats);
@@ -204,6 +207,10 @@ extern "C" wf_status wf_render(wf_scene* s, const wf_render_params* rp, float* i
     WF_TRY(workspace(ctx, "r_list", cap * 4, &plist));
     WF_TRY(workspace(ctx, "r_prefix", cap * 4, &pprefix));
     WF_TRY(workspace(ctx, "r_counts", 64, &pcounts));
+    void* pcoh = nullptr;
+    WF_TRY(workspace(ctx, "r_coherence", sizeof(double) * 9, &pcoh));
+    double* coh = (double*)pcoh;
+    WF_TRY(wf_fill_u32(ctx, pcoh, 0u, 18));
     float* accum = (float*)pacc;
     int32_t* counts = (int32_t*)pcounts;
     WF_TRY(wf_fill_u32(ctx, pacc, 0u, cap * 3));
@@ -271,6 +278,10 @@ extern "C" wf_status wf_render(wf_scene* s, const wf_render_params* rp, float* i
                 q = &shB;
                 pt.end();
             }
+            if (rp->coherence_stats && nh > 0)
+                WF_TRY(coherence_accumulate(ctx, q->mat, q->tex, host_count(nh),
+                                            rp->coherence_stats >= WF_COHERENCE_CACHE_LINES,
+                                            coh));
             pt.begin(3);
             WF_TRY(shade_bounce(ctx, s, q, host_count(nh), &sp, accum, alt, counts + 4));
             pt.end();
@@ -319,6 +330,7 @@ extern "C" wf_status wf_render(wf_scene* s, const wf_render_params* rp, float* i
     st.schedule_ms = pt.total(2);
     st.shade_ms = pt.total(3);
     st.miss_ms = pt.total(4);
+    if (rp->coherence_stats) WF_TRY(coherence_read(ctx, coh, &st));
     WF_TRY(check_fault(ctx));
     if (stats) *stats = st;
     return WF_OK;
