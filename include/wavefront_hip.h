@@ -266,6 +266,30 @@ wf_status wf_materialize_rays(wf_context* ctx, const wf_ray_soa* src, const int3
 wf_status wf_scene_set_costs(wf_scene* scene, const double* material_rel_cost,
                              const double* texture_rel_cost);
 
+/* CostTracker (include/render/CostTracker.h:8-63): per item (material or texture) an
+ * exponential moving average (weight 0.05) of its shading cost in nanoseconds, the first sample
+ * taken as is; relativeCost = average / mean of the tracked items' averages once the item has 50
+ * samples, else 1. Host objects; no device work. */
+typedef struct wf_cost_tracker wf_cost_tracker;
+wf_status wf_cost_tracker_create(int32_t items, wf_cost_tracker** out);
+wf_status wf_cost_tracker_destroy(wf_cost_tracker* tracker);
+/* CostTracker::record */
+wf_status wf_cost_tracker_record(wf_cost_tracker* tracker, int32_t id, double cost_ns);
+/* averageCostNanoseconds, sampleCount and relativeCost of every item (NULL arrays skipped). */
+wf_status wf_cost_tracker_query(const wf_cost_tracker* tracker, double* average_ns,
+                                int32_t* samples, double* relative);
+/* Live cost tracking (shadeAll's cost sampling, cpp:612-627, 746-758, 800-844): while trackers
+ * are attached, every bounce of wf_render's pass loop times a quarter of its shaded hits on the
+ * device (a per-hit coin, the reference's shouldTimeThisRay): the surface-colour fetch of a
+ * textured material per texture and the BSDF scatter per material, both as the wave's wall-clock
+ * time. Each bounce's per-item mean is recorded into the trackers, and before every bounce's
+ * shading-queue schedule the scene's relative-cost tables (the policies' and cost-aware RR's
+ * input) are refreshed from them. The reference starts fresh trackers per render; attach new
+ * ones to do the same. NULL, NULL detaches: the tables then stay as last set. Either tracker
+ * must have as many items as the scene has materials / textures. */
+wf_status wf_scene_track_costs(wf_scene* scene, wf_cost_tracker* materials,
+                               wf_cost_tracker* textures);
+
 /* ShadingQueue::schedule (ShadingQueue.h:90-141, 277-329, 437-550) for the four policies.
  * sorted_indices[i] = queue index shaded at position i. Ties the reference leaves to an unstable
  * sort are broken by ascending queue index. */

@@ -237,6 +237,14 @@ class Scene:
             self.handle, None if mc is None else mc.ctypes.data_as(C.c_void_p),
             None if tc is None else tc.ctypes.data_as(C.c_void_p)))
 
+    def track_costs(self, materials: "CostTracker | None", textures: "CostTracker | None"):
+        """wf_scene_track_costs: live shading-cost tracking of the pass loop into these trackers
+        (None, None detaches)."""
+        self._trackers = (materials, textures)   # the library keeps pointers to them
+        check(self.ctx._lib.wf_scene_track_costs(
+            self.handle, None if materials is None else materials.handle,
+            None if textures is None else textures.handle))
+
     # EmbreeAccelerator::intersect4 over a batch
     def intersect(self, rays: RayQueue, n: int, tnear: float = 0.001, tfar: float = 1e9):
         hits = HitBuffer(self.ctx, n)
@@ -322,6 +330,47 @@ def materialize_shading(ctx: Context, q: ShadingQueue, idx: np.ndarray, n: int) 
     check(ctx._lib.wf_materialize_shading(ctx.handle, C.byref(s), C.c_void_p(d_idx.ptr), n,
                                           C.byref(d)))
     return out
+
+
+class CostTracker:
+    """CostTracker (include/render/CostTracker.h:8-63) held by the library: per item an EMA of
+    the shading cost in ns; relative() is relativeCost. Attach a material and a texture tracker
+    to a scene with Scene.track_costs to have wf_render's pass loop time its shading and refresh
+    the scene's cost tables every bounce."""
+
+    def __init__(self, items: int, library=None):
+        self._lib = library if library is not None else lib()
+        self.items = int(items)
+        h = C.c_void_p()
+        check(self._lib.wf_cost_tracker_create(self.items, C.byref(h)))
+        self.handle = h
+
+    def record(self, item: int, cost_ns: float):
+        check(self._lib.wf_cost_tracker_record(self.handle, int(item), float(cost_ns)))
+
+    def query(self):
+        """(average_ns, samples, relative) arrays."""
+        avg = np.zeros(max(self.items, 1), np.float64)
+        cnt = np.zeros(max(self.items, 1), np.int32)
+        rel = np.zeros(max(self.items, 1), np.float64)
+        check(self._lib.wf_cost_tracker_query(self.handle, avg.ctypes.data_as(C.c_void_p),
+                                              cnt.ctypes.data_as(C.c_void_p),
+                                              rel.ctypes.data_as(C.c_void_p)))
+        return avg[:self.items], cnt[:self.items], rel[:self.items]
+
+    def relative(self) -> np.ndarray:
+        return self.query()[2]
+
+    def close(self):
+        if self.handle:
+            self._lib.wf_cost_tracker_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 COHERENCE_OFF, COHERENCE_RUN_LENGTH, COHERENCE_CACHE_LINES = 0, 1, 2

@@ -203,6 +203,11 @@ _PROTOS = {
     "wf_shading_schedule": (_S, [P, C.POINTER(ShadeSoA), C.c_int32, C.c_int32, P]),
     "wf_materialize_shading": (_S, [P, C.POINTER(ShadeSoA), P, C.c_int32, C.POINTER(ShadeSoA)]),
     "wf_coherence_stats": (_S, [P, P, P, C.c_int32, C.c_int32, C.POINTER(C.c_double)]),
+    "wf_cost_tracker_create": (_S, [C.c_int32, C.POINTER(P)]),
+    "wf_cost_tracker_destroy": (_S, [P]),
+    "wf_cost_tracker_record": (_S, [P, C.c_int32, C.c_double]),
+    "wf_cost_tracker_query": (_S, [P, P, P, P]),
+    "wf_scene_track_costs": (_S, [P, P, P]),
     "wf_shade": (_S, [P, C.POINTER(ShadeSoA), C.c_int32, C.POINTER(ShadeParams), P,
                       C.POINTER(RaySoA), C.POINTER(C.c_int32)]),
     "wf_shade_misses": (_S, [P, C.POINTER(RaySoA), C.c_int32, C.c_int32, P]),

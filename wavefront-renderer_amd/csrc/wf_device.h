@@ -98,6 +98,7 @@ struct PathRng {
 };
 
 enum : uint32_t {
+    D_COST_COIN = 0,   // shouldTimeThisRay (cpp:612): no effect on the path
     D_LIGHT_PICK = 1,
     D_LIGHT_TRI = 2,
     D_LIGHT_R1 = 3,

@@ -230,6 +230,25 @@ struct SceneDev {
 
 }  // namespace wf
 
+// CostTracker (include/render/CostTracker.h:8-63), restated on the host.
+struct wf_cost_tracker {
+    std::vector<double> average;       // averageCostNanoseconds
+    std::vector<uint8_t> initialized;
+    std::vector<int32_t> samples;      // sampleCount
+    double total = 0.0;                // cachedTotalCost
+    int32_t tracked = 0;               // trackedItemCount
+};
+
+namespace wf {
+// Device sums of timed shading (wall-clock ticks) and their counts, per material and texture.
+struct CostProbe {
+    unsigned long long* mat_sum = nullptr;
+    uint32_t* mat_cnt = nullptr;
+    unsigned long long* tex_sum = nullptr;
+    uint32_t* tex_cnt = nullptr;
+};
+}  // namespace wf
+
 struct wf_scene {
     wf_context* ctx = nullptr;
     int device = 0;            // kept so that destroy does not depend on the context
@@ -241,6 +260,11 @@ struct wf_scene {
     std::vector<uint64_t> tex_mem;
     // engine reused by wf_render's regenerating schedule
     wf_engine* render_engine = nullptr;
+    // live cost tracking (wf_scene_track_costs)
+    wf_cost_tracker* track_mat = nullptr;
+    wf_cost_tracker* track_tex = nullptr;
+    wf::CostProbe probe;
+    double ns_per_tick = 10.0;
 };
 
 namespace wf {
@@ -410,7 +434,15 @@ struct ShadeJobs {
 wf_status shade_bounce(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q, Count n,
                        const wf_shade_params* p, float* accum, wf_ray_soa* next,
                        int32_t* counts_dev, const ShadeJobs& jobs = ShadeJobs{},
-                       const uint32_t** alive_out = nullptr);
+                       const uint32_t** alive_out = nullptr, const CostProbe* probe = nullptr);
+// Relative-cost tables of the scene (dense descending material ranks, relative costs); NULL =
+// neutral.
+wf_status scene_upload_costs(wf_scene* s, const double* mat_cost, const double* tex_cost);
+// Live cost tracking around one bounce of the pass loop (no-ops without attached trackers):
+// refresh the tables from the trackers, clear the probe, record the probe into the trackers.
+wf_status costs_refresh(wf_scene* s);
+wf_status costs_clear(wf_scene* s);
+wf_status costs_record(wf_scene* s);
 wf_status shade_misses(wf_context* ctx, const wf_scene* s, const wf_ray_soa* q, Count n,
                        int32_t physics, float* accum);
 wf_status render_regenerate(wf_scene* s, const wf_render_params* rp, float* image_rgb,

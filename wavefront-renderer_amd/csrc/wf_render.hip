@@ -271,6 +271,9 @@ extern "C" wf_status wf_render(wf_scene* s, const wf_render_params* rp, float* i
             st.shaded_hits += nh;
             st.misses += nm;
             wf_shade_soa* q = &shA;
+            // live cost tracking: this bounce's schedule uses the trackers as they stand
+            WF_TRY(costs_refresh(s));
+            WF_TRY(costs_clear(s));
             if (rp->policy != WF_POLICY_NONE && nh > 0) {
                 pt.begin(2);
                 WF_TRY(shading_schedule(ctx, s, &shA, host_count(nh), rp->policy, (int32_t*)pidx));
@@ -283,7 +286,9 @@ extern "C" wf_status wf_render(wf_scene* s, const wf_render_params* rp, float* i
                                             rp->coherence_stats >= WF_COHERENCE_CACHE_LINES,
                                             coh));
             pt.begin(3);
-            WF_TRY(shade_bounce(ctx, s, q, host_count(nh), &sp, accum, alt, counts + 4));
+            WF_TRY(shade_bounce(ctx, s, q, host_count(nh), &sp, accum, alt, counts + 4, ShadeJobs{},
+                                nullptr, s->track_mat ? &s->probe : nullptr));
+            WF_TRY(costs_record(s));
             pt.end();
             pt.begin(4);
             WF_TRY(shade_misses(ctx, s, &miss, host_count(nm), rp->shade.physics, accum));

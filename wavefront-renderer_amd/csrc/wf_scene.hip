@@ -87,7 +87,7 @@ int bits_for(int max_value) {
 }  // namespace
 
 // Dense rank of per-material relative costs, descending (equal costs share a rank).
-static wf_status upload_cost_tables(wf_scene* s, const double* mat_cost, const double* tex_cost) {
+wf_status scene_upload_costs(wf_scene* s, const double* mat_cost, const double* tex_cost) {
     auto& d = s->dev;
     const int nm = d.n_materials, nt = d.n_textures;
     std::vector<double> mc(nm, 1.0), tc(nt, 1.0);
@@ -593,7 +593,7 @@ wf_status wf_scene_create(wf_context* ctx, const wf_scene_desc* desc, wf_scene**
     d.mat_rel_cost = (double*)p;
     if ((st = scene_alloc(s, sizeof(double) * std::max(1, d.n_textures), &p)) != WF_OK) return fail(st);
     d.tex_rel_cost = (double*)p;
-    if ((st = upload_cost_tables(s, nullptr, nullptr)) != WF_OK) return fail(st);
+    if ((st = scene_upload_costs(s, nullptr, nullptr)) != WF_OK) return fail(st);
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return fail(set_error(WF_ERR_HIP, "scene upload: %s", hipGetErrorString(e)));
     if ((st = check_fault(ctx)) != WF_OK) return fail(st);
@@ -621,7 +621,7 @@ wf_status wf_scene_device_bytes(const wf_scene* s, size_t* out) {
 wf_status wf_scene_set_costs(wf_scene* s, const double* mat_cost, const double* tex_cost) {
     clear_error();
     WF_REQUIRE(s, "wf_scene_set_costs: NULL scene");
-    return upload_cost_tables(s, mat_cost, tex_cost);
+    return scene_upload_costs(s, mat_cost, tex_cost);
 }
 
 wf_status wf_accel_build(wf_scene* s, wf_build_stats* stats) {

@@ -47,6 +47,7 @@ struct ShadeOut {
     int32_t* next_tags;
     uint64_t* next_keys;                               // sort word of each next ray, when set
     RayKeyFrame key_frame;
+    CostProbe probe;                                   // live cost tracking, when set
 };
 
 
@@ -544,10 +545,23 @@ __global__ void k_shade(const SceneDev s, wf_shade_soa q, int n_host,
     }
     o.shadow_flag[i] = sflag;
 
-    // scatter + Russian roulette (cpp:739-796)
+    // scatter + Russian roulette (cpp:739-796); with live cost tracking a quarter of the hits
+    // (shouldTimeThisRay, cpp:612) time the scatter: the wave's wall-clock ticks, added to the
+    // material's and, when it has one, the texture's sums
     V3 att, dir;
     float spdf = -1.0f;
+    const bool timed = o.probe.mat_sum != nullptr && rng(D_COST_COIN) < 0.25f;
+    const unsigned long long t_start = timed ? wall_clock64() : 0ull;
     bool alive = scatter(s, m, D, N, P, u, v, front, rng, att, dir, spdf);
+    if (timed) {
+        const unsigned long long dt = wall_clock64() - t_start;
+        atomicAdd(&o.probe.mat_sum[mat_id], dt);
+        atomicAdd(&o.probe.mat_cnt[mat_id], 1u);
+        if (m.texture_id >= 0) {
+            atomicAdd(&o.probe.tex_sum[m.texture_id], dt);
+            atomicAdd(&o.probe.tex_cnt[m.texture_id], 1u);
+        }
+    }
     V3 nt = T * att;
     if (alive && depth + 1 >= a.rr_min_depth) {
         const float base = smin(0.95f, smax(smax(nt.x, nt.y), nt.z));
@@ -808,12 +822,14 @@ static wf_status shade_outputs(wf_context* ctx, int n, ShadeOut& o) {
     o.next_tags = nullptr;
     o.next_keys = nullptr;
     o.key_frame = RayKeyFrame{};
+    o.probe = CostProbe{};
     return WF_OK;
 }
 
 wf_status shade_bounce(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q, Count n,
                        const wf_shade_params* p, float* accum, wf_ray_soa* next,
-                       int32_t* counts_dev, const ShadeJobs& jobs, const uint32_t** alive_out) {
+                       int32_t* counts_dev, const ShadeJobs& jobs, const uint32_t** alive_out,
+                       const CostProbe* probe) {
     WF_TRY(scene_ready(s));
     if (n.n <= 0) {
         WF_TRY(wf_fill_u32(ctx, counts_dev, 0u, 3));
@@ -821,6 +837,7 @@ wf_status shade_bounce(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q
     }
     ShadeOut o;
     WF_TRY(shade_outputs(ctx, n.n, o));
+    if (probe) o.probe = *probe;
     void *pprefix, *plist, *pshflag;
     const size_t nn = (size_t)n.n;
     WF_TRY(workspace(ctx, "shade_prefix", nn * 4, &pprefix));
