@@ -357,6 +357,9 @@ typedef struct wf_render_params {
     int32_t wavefront;         /* REGENERATE: ray slots (paths in flight); 0 = automatic */
     int32_t coherence_stats;   /* WF_COHERENCE_*: the reference's coherence metrics of every
                                   shading batch (cpp:187-246), returned in wf_render_stats */
+    int32_t first_sample;      /* RNG index of the first sample: the render runs samples
+                                  first_sample .. first_sample + spp - 1, so consecutive calls
+                                  continue one sequence (progressive previews); 0 normally */
     wf_shade_params shade;
     wf_camera camera;
 } wf_render_params;
@@ -447,6 +450,7 @@ typedef struct wf_job_desc {
     float adaptive_threshold;
     int32_t adaptive_min_samples;
     int32_t adaptive_interval;
+    int32_t first_sample;      /* RNG index of the job's first sample (see wf_render_params) */
     wf_camera camera;
 } wf_job_desc;
 

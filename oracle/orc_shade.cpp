@@ -1030,7 +1030,7 @@ int32_t orc_render(void* sp, const wf_render_params* rp, const double* mat_cost,
         RayQ q;
         q.resize(pix.size());
         orc_raygen(&rp->camera, W, H, (int)pix.size(), pix.data(), rp->shade.seed, rp->shade.job,
-                   sample, q.ox.data(), q.oy.data(), q.oz.data(), q.dx.data(), q.dy.data(),
+                   rp->first_sample + sample, q.ox.data(), q.oy.data(), q.oz.data(), q.dx.data(), q.dy.data(),
                    q.dz.data());
         for (size_t i = 0; i < pix.size(); ++i) {
             q.tr[i] = q.tg[i] = q.tb[i] = 1.0f;
@@ -1125,7 +1125,7 @@ int32_t orc_render(void* sp, const wf_render_params* rp, const double* mat_cost,
             int32_t nn = 0;
             int64_t shadows = 0;
             wf_shade_params sp2 = rp->shade;
-            sp2.sample_index = sample;
+            sp2.sample_index = rp->first_sample + sample;
             orc_shade(sp, &ss, nh, &sp2, mat_cost, accum.data(), image64 ? acc64.data() : nullptr,
                       &nx, &nn, &shadows, threads);
             nq.resize(nn);

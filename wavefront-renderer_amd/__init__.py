@@ -429,7 +429,7 @@ def render_params(width, height, spp, max_depth, camera, policy="none", ray_sort
                   adaptive_interval=16, physics=PHYSICS_REFERENCE, rr_min_depth=3,
                   cost_aware_rr=False, firefly=10.0, seed=0, job=0,
                   schedule="regenerate", sample_lanes=0, wavefront=0,
-                  coherence_stats=0) -> _abi.RenderParams:
+                  coherence_stats=0, first_sample=0) -> _abi.RenderParams:
     """Regenerating schedule: sample_lanes = samples of a pixel in flight at once (a power of
     two <= 64, 0 for the default 64; 1 reproduces the pass loop's accumulation order bit for
     bit); wavefront = ray slots (0 = automatic). coherence_stats: 0 off, 1 material/texture run
@@ -440,7 +440,7 @@ def render_params(width, height, spp, max_depth, camera, policy="none", ray_sort
     return _abi.RenderParams(width, height, spp, max_depth, pol, 1 if ray_sort else 0,
                              1 if adaptive else 0, adaptive_threshold, adaptive_min_samples,
                              adaptive_interval, sched, sample_lanes, wavefront,
-                             int(coherence_stats), sp, camera_struct(camera))
+                             int(coherence_stats), int(first_sample), sp, camera_struct(camera))
 
 
 def render(scene: Scene, params: _abi.RenderParams, keep_on_device: bool = False):
@@ -461,10 +461,10 @@ def render(scene: Scene, params: _abi.RenderParams, keep_on_device: bool = False
 # ---- batched / served forms -----------------------------------------------------------------
 def job_desc(width, height, spp, max_depth, camera, seed=0, job_tag=0, adaptive=False,
              adaptive_threshold=0.05, adaptive_min_samples=32,
-             adaptive_interval=16) -> _abi.JobDesc:
+             adaptive_interval=16, first_sample=0) -> _abi.JobDesc:
     return _abi.JobDesc(width, height, spp, max_depth, seed, job_tag, 1 if adaptive else 0,
                         adaptive_threshold, adaptive_min_samples, adaptive_interval,
-                        camera_struct(camera))
+                        int(first_sample), camera_struct(camera))
 
 
 class Engine:

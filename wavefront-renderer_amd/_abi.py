@@ -116,7 +116,8 @@ class RenderParams(C.Structure):
                 ("adaptive", C.c_int32), ("adaptive_threshold", C.c_float),
                 ("adaptive_min_samples", C.c_int32), ("adaptive_interval", C.c_int32),
                 ("schedule", C.c_int32), ("sample_lanes", C.c_int32), ("wavefront", C.c_int32),
-                ("coherence_stats", C.c_int32), ("shade", ShadeParams), ("camera", Camera)]
+                ("coherence_stats", C.c_int32), ("first_sample", C.c_int32),
+                ("shade", ShadeParams), ("camera", Camera)]
 
 
 class RenderStats(C.Structure):
@@ -141,7 +142,7 @@ class JobDesc(C.Structure):
                 ("max_depth", C.c_int32), ("seed", C.c_uint32), ("job_tag", C.c_int32),
                 ("adaptive", C.c_int32), ("adaptive_threshold", C.c_float),
                 ("adaptive_min_samples", C.c_int32), ("adaptive_interval", C.c_int32),
-                ("camera", Camera)]
+                ("first_sample", C.c_int32), ("camera", Camera)]
 
 
 class JobResult(C.Structure):

@@ -163,7 +163,8 @@ __global__ void k_regen(EngineView e, RayPacked r, int32_t* __restrict__ tags,
         e.before[3 * (size_t)a + 2] = e.accum[3 * (size_t)a + 2];
     }
     const int local = lane_pixel(J, a, s) - J.pixel_base;
-    PathRng rng{J.seed, (uint32_t)J.tag, (uint32_t)local, (uint32_t)s, kRaygenDepth};
+    PathRng rng{J.seed, (uint32_t)J.tag, (uint32_t)local, (uint32_t)(s + J.first_sample),
+                kRaygenDepth};
     const V3 d = primary_dir(J.cam, local % J.width, local / J.width, J.width, J.height, rng(0),
                              rng(1));
     const int k = e.counters[C_LIVE] + i;
@@ -861,6 +862,7 @@ wf_status wf_engine_admit(wf_engine* e, const wf_job_desc* job, int32_t* job_id)
     J.max_depth = job->max_depth;
     J.seed = job->seed;
     J.tag = job->job_tag;
+    J.first_sample = job->first_sample;
     J.adaptive = job->adaptive ? 1 : 0;
     J.adaptive_threshold = job->adaptive_threshold;
     J.adaptive_min_samples = job->adaptive_min_samples;
@@ -1163,6 +1165,7 @@ wf_status render_regenerate(wf_scene* s, const wf_render_params* rp, float* imag
     job.adaptive_threshold = rp->adaptive_threshold;
     job.adaptive_min_samples = rp->adaptive_min_samples;
     job.adaptive_interval = rp->adaptive_interval;
+    job.first_sample = rp->first_sample;
     job.camera = rp->camera;
     int32_t id = -1, fin = -1, nf = 0;
     WF_TRY(wf_engine_admit(e, &job, &id));

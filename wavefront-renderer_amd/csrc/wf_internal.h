@@ -313,7 +313,7 @@ struct JobDev {
                                // lane s & (lanes - 1), whose accumulator is pixel + lane * stride
     float cam[12];             // eye, top-left, horizontal, vertical (Camera.h:13-33)
     int32_t lane_stride;       // accumulator index distance between the lanes of a pixel
-    int32_t pad;
+    int32_t first_sample;      // RNG index of sample 0
 };
 
 // Framebuffer pixel of an accumulator index a of job J running sample s.

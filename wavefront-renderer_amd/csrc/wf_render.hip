@@ -171,6 +171,7 @@ extern "C" wf_status wf_render(wf_scene* s, const wf_render_params* rp, float* i
     WF_REQUIRE(rp->width >= 2 && rp->height >= 2, "wf_render: image must be at least 2x2 (got %dx%d)",
                rp->width, rp->height);
     WF_REQUIRE(rp->spp >= 0 && rp->max_depth >= 0, "wf_render: negative spp/depth");
+    WF_REQUIRE(rp->first_sample >= 0, "wf_render: negative first_sample");
     WF_REQUIRE(rp->policy >= WF_POLICY_NONE && rp->policy <= WF_POLICY_COST_BENEFIT,
                "wf_render: unknown policy %d", rp->policy);
     WF_REQUIRE(!rp->adaptive || rp->adaptive_interval > 0, "wf_render: adaptive_interval must be > 0");
@@ -247,11 +248,12 @@ extern "C" wf_status wf_render(wf_scene* s, const wf_render_params* rp, float* i
         if (n > 0) {
             Timed tm(ctx, "raygen", 48.0 * n);
             k_raygen<<<grid_for(n, 256), 256, 0, ctx->stream>>>(cam, W, H, pixels, n, rp->shade.seed,
-                                                                rp->shade.job, sample, *cur);
+                                                                rp->shade.job,
+                                                                rp->first_sample + sample, *cur);
             WF_HIP(hipGetLastError());
         }
         wf_shade_params sp = rp->shade;
-        sp.sample_index = sample;
+        sp.sample_index = rp->first_sample + sample;
         for (int depth = 0; depth < rp->max_depth && n > 0; ++depth) {
             st.rays_traced += n;
             if (rp->ray_sort) {

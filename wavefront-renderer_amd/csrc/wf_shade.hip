@@ -367,8 +367,9 @@ __global__ void k_shade(const SceneDev s, wf_shade_soa q, int n_host,
         const JobDev& J = jobs[tg >> kTagJobShift];
         seed = J.seed;
         job = (uint32_t)J.tag;
-        sample = (uint32_t)(tg & kTagSampleMask);
-        rng_pixel = lane_pixel(J, pixel, (int32_t)sample) - J.pixel_base;
+        const int32_t s_job = tg & kTagSampleMask;
+        sample = (uint32_t)(s_job + J.first_sample);
+        rng_pixel = lane_pixel(J, pixel, s_job) - J.pixel_base;
         depth_limit = J.max_depth;
     }
     PathRng rng{seed, job, (uint32_t)rng_pixel, sample, (uint32_t)depth};
