@@ -520,7 +520,9 @@ typedef struct wf_tuning {
                                   * boxes; 8, boxes quantised outward to 8 bits per node grid
                                   * (fewer, smaller nodes; more box tests per node) */
     int32_t radix_bits;          /* digit width of the radix sort passes: 6, 8 (default) or 11 */
-    int32_t reserved[2];
+    int32_t shadow_blocks_per_cu;   /* the shadow-ray kernel's grid (0 = trace_blocks_per_cu): it
+                                     * traces few rays per lane, so its shape is tuned apart */
+    int32_t shadow_refill_lanes;    /* the shadow-ray kernel's refill threshold (0 = refill_lanes) */
 } wf_tuning;
 /* Engines created on the context take its tuning at every run. */
 wf_status wf_context_set_tuning(wf_context* ctx, const wf_tuning* tuning);

@@ -82,13 +82,16 @@ class Context:
         check(self._lib.wf_context_get_tuning(self.handle, C.byref(t)))
         return {"refill_lanes": t.refill_lanes, "trace_blocks_per_cu": t.trace_blocks_per_cu,
                 "bvh_builder": t.bvh_builder, "ploc_radius": t.ploc_radius,
-                "bvh_width": t.bvh_width, "radix_bits": t.radix_bits}
+                "bvh_width": t.bvh_width, "radix_bits": t.radix_bits,
+                "shadow_blocks_per_cu": t.shadow_blocks_per_cu,
+                "shadow_refill_lanes": t.shadow_refill_lanes}
 
     def set_tuning(self, **kw):
         cur = self.tuning()
         cur.update(kw)
         t = _abi.Tuning(cur["refill_lanes"], cur["trace_blocks_per_cu"], cur["bvh_builder"],
-                        cur["ploc_radius"], cur["bvh_width"], cur["radix_bits"])
+                        cur["ploc_radius"], cur["bvh_width"], cur["radix_bits"],
+                        cur["shadow_blocks_per_cu"], cur["shadow_refill_lanes"])
         check(self._lib.wf_context_set_tuning(self.handle, C.byref(t)))
 
     def stream_copy(self, dst: int, src: int, nbytes: int):

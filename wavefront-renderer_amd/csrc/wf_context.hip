@@ -293,6 +293,12 @@ wf_status wf_context_set_tuning(wf_context* ctx, const wf_tuning* t) {
                "wf_context_set_tuning: bvh_width %d is not 4 or 8", t->bvh_width);
     WF_REQUIRE(t->radix_bits == 6 || t->radix_bits == 8 || t->radix_bits == 11,
                "wf_context_set_tuning: radix_bits %d is not 6, 8 or 11", t->radix_bits);
+    WF_REQUIRE(t->shadow_blocks_per_cu >= 0 && t->shadow_blocks_per_cu <= 64,
+               "wf_context_set_tuning: shadow_blocks_per_cu %d not in [0, 64]",
+               t->shadow_blocks_per_cu);
+    WF_REQUIRE(t->shadow_refill_lanes >= 0 && t->shadow_refill_lanes <= 64,
+               "wf_context_set_tuning: shadow_refill_lanes %d not in [0, 64]",
+               t->shadow_refill_lanes);
     ctx->tuning = *t;
     return WF_OK;
 }

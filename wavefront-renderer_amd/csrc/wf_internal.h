@@ -73,7 +73,7 @@ struct wf_context {
     std::vector<wf::PendingTiming> pending;
     std::vector<hipEvent_t> event_pool;
     int32_t* count_slots = nullptr;    // pinned, kCountSlots entries
-    wf_tuning tuning{48, 12, WF_BVH_PLOC, 16, 4, 8, {0, 0}};
+    wf_tuning tuning{48, 12, WF_BVH_PLOC, 16, 4, 8, 0, 0};
     int next_count_slot = 0;
     int cu_count = 256;
     // Device word that kernels set when a bounded device-wide wait (the radix sort's look-back)

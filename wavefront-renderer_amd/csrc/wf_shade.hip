@@ -827,6 +827,12 @@ static wf_status shade_outputs(wf_context* ctx, int n, ShadeOut& o) {
     return WF_OK;
 }
 
+// refill threshold of the shadow-ray kernel
+static int shadow_refill(const wf_context* ctx) {
+    return ctx->tuning.shadow_refill_lanes > 0 ? ctx->tuning.shadow_refill_lanes
+                                               : ctx->tuning.refill_lanes;
+}
+
 wf_status shade_bounce(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q, Count n,
                        const wf_shade_params* p, float* accum, wf_ray_soa* next,
                        int32_t* counts_dev, const ShadeJobs& jobs, const uint32_t** alive_out,
@@ -880,9 +886,10 @@ wf_status shade_bounce(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q
         Timed tm(ctx, "shadow", 41.0, ns);
         with_bvh(s->dev, [&](auto nodes, auto S) {
             k_shadow<std::remove_cv_t<std::remove_pointer_t<decltype(nodes)>>, decltype(S)::value>
-                <<<persistent_grid(ns.n, ctx), kTraceBlock, 0, ctx->stream>>>(
+                <<<persistent_grid(ns.n, ctx, ctx->tuning.shadow_blocks_per_cu), kTraceBlock, 0,
+                   ctx->stream>>>(
                     s->dev, nodes, *q, o, (int32_t*)plist, ns.n, ns.dev, accum, (int32_t*)pc,
-                    ctx->tuning.refill_lanes);
+                    shadow_refill(ctx));
         });
         WF_HIP(hipGetLastError());
         if (alive_out) *alive_out = o.alive;
@@ -924,9 +931,10 @@ wf_status shade_bounce(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q
         Timed tm(ctx, "shadow", 41.0, ns);
         with_bvh(s->dev, [&](auto nodes, auto S) {
             k_shadow<std::remove_cv_t<std::remove_pointer_t<decltype(nodes)>>, decltype(S)::value>
-                <<<persistent_grid(ns.n, ctx), kTraceBlock, 0, ctx->stream>>>(
+                <<<persistent_grid(ns.n, ctx, ctx->tuning.shadow_blocks_per_cu), kTraceBlock, 0,
+                   ctx->stream>>>(
                     s->dev, nodes, *q, o, (int32_t*)plist, ns.n, ns.dev, accum, (int32_t*)pc,
-                    ctx->tuning.refill_lanes);
+                    shadow_refill(ctx));
         });
         WF_HIP(hipGetLastError());
     }

@@ -602,10 +602,12 @@ __device__ __forceinline__ int refill(WorkQueue& q, bool idle, int refill_lanes)
     return r < end ? r : -1;
 }
 
-// Persistent-grid size for traversal kernels: blocks_per_cu resident blocks on every CU.
-inline unsigned persistent_grid(int n, const wf_context* ctx) {
+// Persistent-grid size for traversal kernels: blocks_per_cu resident blocks on every CU
+// (0 = the context's trace_blocks_per_cu).
+inline unsigned persistent_grid(int n, const wf_context* ctx, int blocks_per_cu = 0) {
     const unsigned need = grid_for(n, kTraceBlock);
-    const unsigned fill = (unsigned)ctx->cu_count * (unsigned)ctx->tuning.trace_blocks_per_cu;
+    const int bpc = blocks_per_cu > 0 ? blocks_per_cu : ctx->tuning.trace_blocks_per_cu;
+    const unsigned fill = (unsigned)ctx->cu_count * (unsigned)bpc;
     return need < fill ? need : fill;
 }
 
