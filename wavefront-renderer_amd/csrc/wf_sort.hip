@@ -51,16 +51,19 @@ struct Digits {
     static constexpr int MaxPasses = (64 + BITS - 1) / BITS;
 };
 
+// Pass p ranks bits [lo_bit + BITS p, ...): BITS wide, the last pass up to BITS + 1 (the rest
+// of the key). Its digit totals go to hist[p * stride ..].
 template <int BITS>
 __global__ __launch_bounds__(kSortBlock) void k_sweep_hist(const uint64_t* __restrict__ keys,
                                                            int n_host,
                                                            const int32_t* __restrict__ nd,
                                                            int lo_bit, int hi_bit, int passes,
+                                                           int stride,
                                                            uint32_t* __restrict__ hist) {
-    constexpr int R = Digits<BITS>::R;
     const int n = load_count(n_host, nd);
-    __shared__ uint32_t cnt[Digits<BITS>::MaxPasses * R];
-    for (int i = threadIdx.x; i < passes * R; i += kSortBlock) cnt[i] = 0;
+    // (room for a 9-bit last pass of 8-bit digits)
+    __shared__ uint32_t cnt[Digits<BITS>::MaxPasses * (BITS == 8 ? 2 : 1) * Digits<BITS>::R];
+    for (int i = threadIdx.x; i < passes * stride; i += kSortBlock) cnt[i] = 0;
     __syncthreads();
     const int lane = threadIdx.x & 63;
     const int span = hi_bit - lo_bit + 1;
@@ -91,15 +94,15 @@ __global__ __launch_bounds__(kSortBlock) void k_sweep_hist(const uint64_t* __res
         }
         if (valid && !counted) {
             for (int p = 0; p < passes; ++p) {
-                // the same digit the pass ranks by: bits [shift, min(shift + BITS, hi_bit + 1))
+                // the same digit the pass ranks by
                 const int shift = lo_bit + BITS * p;
-                const int nbits = (hi_bit + 1 - shift) < BITS ? (hi_bit + 1 - shift) : BITS;
-                atomicAdd(&cnt[p * R + ((uint32_t)(k >> shift) & ((1u << nbits) - 1u))], c);
+                const int nbits = p == passes - 1 ? hi_bit + 1 - shift : BITS;
+                atomicAdd(&cnt[p * stride + ((uint32_t)(k >> shift) & ((1u << nbits) - 1u))], c);
             }
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < passes * R; i += kSortBlock)
+    for (int i = threadIdx.x; i < passes * stride; i += kSortBlock)
         if (cnt[i]) atomicAdd(&hist[i], cnt[i]);
 }
 
@@ -118,12 +121,14 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t x, uint32_t* p
     return off + v - x;
 }
 
-// hist[p][d] -> exclusive prefix over digits (the pass's output base of digit d).
+// hist[p][d] -> exclusive prefix over digits (the pass's output base of digit d); block b
+// handles pass first + b, whose digits start at hist + (first + b) * stride.
 template <int BITS>
-__global__ __launch_bounds__(kSortBlock) void k_sweep_bases(uint32_t* hist) {
+__global__ __launch_bounds__(kSortBlock) void k_sweep_bases(uint32_t* hist, int first,
+                                                            int stride) {
     constexpr int R = Digits<BITS>::R, DPT = Digits<BITS>::DPT;
     __shared__ uint32_t part[kSortWaves];
-    uint32_t* h = hist + blockIdx.x * R;
+    uint32_t* h = hist + (size_t)(first + blockIdx.x) * stride;
     const int base = threadIdx.x * DPT;
     uint32_t x[DPT], sum = 0;
 #pragma unroll
@@ -536,6 +541,27 @@ static wf_status sort_identity(wf_context* ctx, const uint64_t* keys_in, const i
     return WF_OK;
 }
 
+// One pass of BITS-wide digits with the payload of the sort (pairs, packed words, or the index
+// pass that ends a packed sort).
+template <int BITS>
+static void launch_pass(wf_context* ctx, bool packed, int tiles, const uint64_t* ksrc,
+                        const int32_t* vsrc, uint64_t* kdst, int32_t* vdst, int32_t* vals_out,
+                        Count n, int shift, uint32_t mask, int nbits, const uint32_t* hp,
+                        uint32_t* sp, int32_t* counter, uint32_t idx_mask, bool last) {
+    if (!packed)
+        k_sweep_pass<BITS, kPairs><<<tiles, kSortBlock, 0, ctx->stream>>>(
+            ksrc, vsrc, kdst, vdst, n.n, n.dev, shift, mask, nbits, hp, sp, counter, ctx->fault,
+            0u);
+    else if (!last)
+        k_sweep_pass<BITS, kPacked><<<tiles, kSortBlock, 0, ctx->stream>>>(
+            ksrc, nullptr, kdst, nullptr, n.n, n.dev, shift, mask, nbits, hp, sp, counter,
+            ctx->fault, idx_mask);
+    else
+        k_sweep_pass<BITS, kPackedIndex><<<tiles, kSortBlock, 0, ctx->stream>>>(
+            ksrc, nullptr, nullptr, vals_out, n.n, n.dev, shift, mask, nbits, hp, sp, counter,
+            ctx->fault, idx_mask);
+}
+
 // Passes of BITS-wide digits over key bits [lo_bit, hi_bit]. Pairs (idx_bits 0): the last pass
 // writes keys_out and vals_out. Packed words (idx_bits > 0, vals_in NULL): the passes move the
 // words alone, keys_out is the second ping-pong buffer, and the last pass writes the low idx_bits
@@ -549,7 +575,12 @@ static wf_status sweep_sort(wf_context* ctx, const uint64_t* keys_in, const int3
     const bool packed = idx_bits > 0;
     const int tile_keys = packed ? sort_tile<kPacked>() : sort_tile<kPairs>();
     const int tiles = (n.n + tile_keys - 1) / tile_keys;
-    const int passes = (span + BITS - 1) / BITS;
+    int passes = (span + BITS - 1) / BITS;
+    // a one-bit last pass (the 33-bit ray key over 8-bit digits) is folded into the pass before
+    // it, which then ranks a 9-bit digit: one pass fewer
+    const bool wide_last = BITS == 8 && passes >= 2 && span - BITS * (passes - 1) == 1;
+    if (wide_last) --passes;
+    const int stride = wide_last ? 2 * R : R;   // digit slots per pass in hist and state
     if (passes > Digits<BITS>::MaxPasses || passes > kMaxPasses)
         return set_error(WF_ERR_INVALID_ARGUMENT, "sort key too wide");
     void *pk = nullptr, *pv = nullptr, *pm = nullptr;
@@ -560,8 +591,8 @@ static wf_status sweep_sort(wf_context* ctx, const uint64_t* keys_in, const int3
     // (each part starts on a 256-byte boundary: a tile's state row must not share cache lines
     // with another part)
     auto up = [](size_t words) { return (words + 63) & ~(size_t)63; };
-    const size_t off_hist = up(kMaxPasses), off_state = off_hist + up((size_t)R * passes);
-    const size_t meta = off_state + (size_t)R * tiles * passes;
+    const size_t off_hist = up(kMaxPasses), off_state = off_hist + up((size_t)stride * passes);
+    const size_t meta = off_state + (size_t)stride * tiles * passes;
     WF_TRY(workspace(ctx, "sort_meta", sizeof(uint32_t) * meta, &pm));
     int32_t* counters = (int32_t*)pm;
     uint32_t* hist = (uint32_t*)pm + off_hist;
@@ -571,10 +602,18 @@ static wf_status sweep_sort(wf_context* ctx, const uint64_t* keys_in, const int3
         Timed t(ctx, tag, 8.0, n);
         const unsigned blocks = std::min<unsigned>(grid_for(n.n, kSortBlock), 1024u);
         k_sweep_hist<BITS><<<blocks, kSortBlock, 0, ctx->stream>>>(keys_in, n.n, n.dev, lo_bit,
-                                                                   hi_bit, passes, hist);
+                                                                   hi_bit, passes, stride, hist);
         WF_HIP(hipGetLastError());
-        k_sweep_bases<BITS><<<passes, kSortBlock, 0, ctx->stream>>>(hist);
-        WF_HIP(hipGetLastError());
+        const int narrow = wide_last ? passes - 1 : passes;
+        if (narrow > 0) {
+            k_sweep_bases<BITS><<<narrow, kSortBlock, 0, ctx->stream>>>(hist, 0, stride);
+            WF_HIP(hipGetLastError());
+        }
+        if (wide_last) {
+            k_sweep_bases<(BITS == 8 ? 9 : BITS)><<<1, kSortBlock, 0, ctx->stream>>>(
+                hist, passes - 1, stride);
+            WF_HIP(hipGetLastError());
+        }
     }
     // pairs ping-pong so that the last pass writes keys_out/vals_out; packed words alternate
     // between the two key buffers and the last pass writes vals_out
@@ -583,28 +622,23 @@ static wf_status sweep_sort(wf_context* ctx, const uint64_t* keys_in, const int3
     const uint32_t idx_mask = packed ? (uint32_t)((1ull << idx_bits) - 1ull) : 0u;
     for (int p = 0; p < passes; ++p) {
         const int shift = lo_bit + p * BITS;
-        const int nbits = (hi_bit + 1 - shift) < BITS ? (hi_bit + 1 - shift) : BITS;
-        const uint32_t mask = (1u << nbits) - 1u;
         const bool last = p == passes - 1;
+        const int nbits = last ? hi_bit + 1 - shift : BITS;
+        const uint32_t mask = (1u << nbits) - 1u;
         const bool to_out = packed ? (p % 2) == 1 : ((passes - 1 - p) % 2) == 0;
         uint64_t* kdst = to_out ? keys_out : (uint64_t*)pk;
         int32_t* vdst = packed ? vals_out : (to_out ? vals_out : (int32_t*)pv);
-        const uint32_t* hp = hist + (size_t)R * p;
-        uint32_t* sp = state + (size_t)R * tiles * p;
+        const uint32_t* hp = hist + (size_t)stride * p;
+        uint32_t* sp = state + (size_t)stride * tiles * p;
         {
             Timed t(ctx, tag, packed ? (last ? 12.0 : 16.0) : 24.0, n);
-            if (!packed)
-                k_sweep_pass<BITS, kPairs><<<tiles, kSortBlock, 0, ctx->stream>>>(
-                    ksrc, vsrc, kdst, vdst, n.n, n.dev, shift, mask, nbits, hp, sp, counters + p,
-                    ctx->fault, 0u);
-            else if (!last)
-                k_sweep_pass<BITS, kPacked><<<tiles, kSortBlock, 0, ctx->stream>>>(
-                    ksrc, nullptr, kdst, nullptr, n.n, n.dev, shift, mask, nbits, hp, sp,
-                    counters + p, ctx->fault, idx_mask);
+            if (last && wide_last)
+                launch_pass<(BITS == 8 ? 9 : BITS)>(ctx, packed, tiles, ksrc, vsrc, kdst, vdst,
+                                                    vals_out, n, shift, mask, nbits, hp, sp,
+                                                    counters + p, idx_mask, true);
             else
-                k_sweep_pass<BITS, kPackedIndex><<<tiles, kSortBlock, 0, ctx->stream>>>(
-                    ksrc, nullptr, nullptr, vals_out, n.n, n.dev, shift, mask, nbits, hp, sp,
-                    counters + p, ctx->fault, idx_mask);
+                launch_pass<BITS>(ctx, packed, tiles, ksrc, vsrc, kdst, vdst, vals_out, n, shift,
+                                  mask, nbits, hp, sp, counters + p, idx_mask, last);
             WF_HIP(hipGetLastError());
         }
         ksrc = kdst;
