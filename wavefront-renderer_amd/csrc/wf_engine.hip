@@ -728,6 +728,14 @@ wf_status wf_engine_create(wf_context* ctx, wf_scene* scene, const wf_engine_con
     ETRY(carve_rays(e, cap, &e->miss));
     ETRY(carve_shading(e, cap, &e->s0));
     ETRY(carve_shading(e, cap, &e->s1));
+    // columns no kernel of the engine reads: the split skips them (the ray origin of a hit, its
+    // mesh-local triangle index and mesh, the origin of a miss)
+    for (wf_shade_soa* q : {&e->s0, &e->s1}) {
+        q->ox = q->oy = q->oz = nullptr;
+        q->tri = nullptr;
+        q->geom = nullptr;
+    }
+    e->miss.ox = e->miss.oy = e->miss.oz = nullptr;
     ETRY(engine_array(e, cap, &e->t0));
     ETRY(engine_array(e, cap, &e->t1));
     ETRY(engine_array(e, cap, &e->st0));

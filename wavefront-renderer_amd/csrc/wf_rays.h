@@ -47,8 +47,11 @@ __device__ __forceinline__ RayVal ld_ray(const RayPacked& r, int i) {
     return v;
 }
 
+// A column queue without origin columns (ox NULL) stores the rest of the ray.
 __device__ __forceinline__ void st_ray(const wf_ray_soa& r, int i, const RayVal& v) {
-    r.ox[i] = v.o.x; r.oy[i] = v.o.y; r.oz[i] = v.o.z;
+    if (r.ox) {
+        r.ox[i] = v.o.x; r.oy[i] = v.o.y; r.oz[i] = v.o.z;
+    }
     r.dx[i] = v.d.x; r.dy[i] = v.d.y; r.dz[i] = v.d.z;
     r.tr[i] = v.t.x; r.tg[i] = v.t.y; r.tb[i] = v.t.z;
     r.pixel[i] = v.pixel;

@@ -100,12 +100,14 @@ __global__ void k_gather_shade(wf_shade_soa s, const int32_t* __restrict__ idx, 
     d.t[i] = s.t[j];
     d.u[i] = s.u[j];
     d.v[i] = s.v[j];
-    d.tri[i] = s.tri[j];
+    if (d.tri) d.tri[i] = s.tri[j];
     d.mat[i] = s.mat[j];
     d.tex[i] = s.tex[j];
-    d.ox[i] = s.ox[j];
-    d.oy[i] = s.oy[j];
-    d.oz[i] = s.oz[j];
+    if (d.ox) {
+        d.ox[i] = s.ox[j];
+        d.oy[i] = s.oy[j];
+        d.oz[i] = s.oz[j];
+    }
     d.dx[i] = s.dx[j];
     d.dy[i] = s.dy[j];
     d.dz[i] = s.dz[j];

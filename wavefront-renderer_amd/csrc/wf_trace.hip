@@ -126,10 +126,10 @@ __device__ __forceinline__ void map_hit(const SceneDev& s, const R& in, int i, i
     out.t[j] = t;
     out.u[j] = u;
     out.v[j] = v;
-    out.tri[j] = g - s.mesh_tri_offset[geom];
+    if (out.tri) out.tri[j] = g - s.mesh_tri_offset[geom];
     out.mat[j] = mat;
     out.tex[j] = s.materials[mat].texture_id;
-    st3(out.ox, out.oy, out.oz, j, o);
+    if (out.ox) st3(out.ox, out.oy, out.oz, j, o);
     st3(out.dx, out.dy, out.dz, j, d);
     out.tr[j] = ray.t.x;
     out.tg[j] = ray.t.y;
