@@ -76,7 +76,8 @@ enum : int { S_RAYS = 0, S_SHADOW, S_HITS, S_MISSES, S_PATHS, S_HEAD, S_TAIL, S_
 constexpr int kMaxAdaptiveLanes = 16;
 
 // Host-visible status block, copied back once per iteration: live rays, ready-queue cursors,
-// then the pixels left of every job slot.
+// then the work left of every job slot (pixels of an adaptive job, pixel lanes of a job with a
+// fixed sample count).
 constexpr int kStatusHeader = 8;   // int32: [0] live rays, [2..3] head, [4..5] tail (u64),
                                    // [7] fault word of the engine's context
 constexpr int kStatusFault = 7;
@@ -89,7 +90,7 @@ struct EngineView {
     unsigned long long* stats; // S_HEAD / S_TAIL: ready-queue cursors (monotonic)
     int2* ready;               // (accumulator index, job slot << kTagJobShift | next sample)
     float* accum;              // 3 per pixel and lane (lane-major)
-    int32_t* lanes_done;       // per pixel: lanes that ran their last sample
+    int32_t* lanes_done;       // per pixel of an adaptive job: lanes that ended the round
     float* before;             // adaptive jobs: 3 per pixel and lane (lane-major, at most
                                // kMaxAdaptiveLanes lanes), the lane accumulator at the start of
                                // its current sample
@@ -98,7 +99,8 @@ struct EngineView {
     int32_t* pixel_count;      // samples of a retired pixel (written when it retires)
     const JobDev* jobs;
     int32_t* status;           // kStatusHeader + job slots
-    int32_t* job_left;         // = status + kStatusHeader: pixels of each job not finished yet
+    int32_t* job_left;         // = status + kStatusHeader: units of each job not finished yet
+                               // (adaptive: pixels; fixed sample count: pixel lanes)
     int32_t* job_max_count;    // most samples any pixel of the job ran
     int32_t* job_unconverged;  // adaptive: pixels that ran out of budget without converging
 };
@@ -130,9 +132,9 @@ __global__ void k_admit(EngineView e, int job, int base, int n, int lanes, int q
     e.lanes_done[p] = 0;
 }
 
-__global__ void k_admit_commit(EngineView e, int job, int n, int lanes, int queue) {
+__global__ void k_admit_commit(EngineView e, int job, int n, int lanes, int queue, int left) {
     if (queue) e.stats[S_TAIL] += (unsigned long long)n * (unsigned long long)lanes;
-    e.job_left[job] = queue ? n : 0;
+    e.job_left[job] = queue ? left : 0;
     e.job_max_count[job] = 0;
     e.job_unconverged[job] = 0;
 }
@@ -193,6 +195,13 @@ __global__ void k_regen_commit(EngineView e) {
 constexpr int kFinishBlock = 1024;
 constexpr int kFinishItems = 4;
 
+// What an ended path retired: one unit of job_left (a pixel of an adaptive job, a pixel lane of
+// a job with a fixed sample count) with its pixel's sample count.
+struct Retired {
+    int32_t job, samples;
+    bool on, unconverged;
+};
+
 // A path of accumulator a ended: count the sample, then requeue the lane's next sample or end the
 // lane; the pixel retires when its last lane ends or it converged. Returns the number of ready
 // entries to requeue: entry q is (first.x + q * stride, first.y + q).
@@ -204,10 +213,10 @@ constexpr int kFinishItems = 4;
 // mean and M2 with the round's samples in sample order and tests convergence where the sample
 // count is a multiple of the interval, as the pass loop does (L divides the interval, so such
 // counts only fall on round ends); then the next round is queued, or the pixel retires.
-// Called by every thread of the wave (valid = this thread has an ended path; tag = its job slot
-// and sample index).
+// valid = this thread has an ended path; tag = its job slot and sample index. The job counters
+// of what it retired are left to finish_retire.
 __device__ __forceinline__ int finish_eval(const EngineView& e, int a, int32_t tag, bool valid,
-                                           int2* first, int32_t* stride) {
+                                           int2* first, int32_t* stride, Retired* out) {
     int requeue = 0;
     bool retire = false, unconverged = false;
     int32_t job = 0, samples = 0;
@@ -269,8 +278,9 @@ __device__ __forceinline__ int finish_eval(const EngineView& e, int a, int32_t t
                 requeue = 1;
                 *first = make_int2(a, (j << kTagJobShift) | ns);
                 *stride = 0;
-            } else if (L == 1 || atomicAdd(&e.lanes_done[p], 1) == L - 1) {
-                // the pixel's last lane ran its last sample: all spp samples are in
+            } else {
+                // the lane ran its last sample; the job ends when all its pixel lanes have (every
+                // lane of the pixel writes the same count, so no per-pixel counter is needed)
                 e.pixel_count[p] = J.spp;
                 retire = true;
                 job = j;
@@ -278,29 +288,60 @@ __device__ __forceinline__ int finish_eval(const EngineView& e, int a, int32_t t
             }
         }
     }
-    // retired pixels: one set of job-counter atomics per wave for the lanes sharing the first
-    // retiring lane's job, single atomics for the others
-    const uint64_t rmask = __ballot(retire);
-    if (rmask) {
-        const int leader = __ffsll((unsigned long long)rmask) - 1;
-        const int jl = __shfl(job, leader, 64);
-        const bool same = retire && job == jl;
-        const uint64_t smask = __ballot(same);
-        int cmax = same ? samples : 0;
-        for (int o = 32; o >= 1; o >>= 1) cmax = max(cmax, __shfl_xor(cmax, o, 64));
-        const int un = __popcll(__ballot(same && unconverged));
-        if ((int)__lane_id() == leader) {
-            atomicSub(&e.job_left[jl], __popcll(smask));
-            atomicMax(&e.job_max_count[jl], cmax);
-            if (un) atomicAdd(&e.job_unconverged[jl], un);
-        }
-        if (retire && !same) {
-            atomicSub(&e.job_left[job], 1);
-            atomicMax(&e.job_max_count[job], samples);
-            if (unconverged) atomicAdd(&e.job_unconverged[job], 1);
+    *out = Retired{job, samples, retire, unconverged};
+    return requeue;
+}
+
+// Job counters of the block's retirements: those of the job that one of them claims first are
+// summed in LDS and applied with one set of atomics per block, the others (a job boundary inside
+// the block) with single atomics. Every thread of the block calls it.
+__device__ __forceinline__ void finish_retire(const EngineView& e,
+                                              const Retired (&r)[kFinishItems]) {
+    __shared__ int32_t s_job, s_left, s_max, s_unconverged;
+    if (threadIdx.x == 0) {
+        s_job = -1;
+        s_left = s_max = s_unconverged = 0;
+    }
+    __syncthreads();
+    int32_t mine = -1;
+#pragma unroll
+    for (int j = 0; j < kFinishItems; ++j)
+        if (mine < 0 && r[j].on) mine = r[j].job;
+    const uint64_t claim = __ballot(mine >= 0);
+    if (claim && (int)__lane_id() == __ffsll((unsigned long long)claim) - 1)
+        atomicCAS(&s_job, -1, mine);
+    __syncthreads();
+    const int32_t jb = s_job;
+    int left = 0, mx = 0, un = 0;
+#pragma unroll
+    for (int j = 0; j < kFinishItems; ++j) {
+        if (!r[j].on) continue;
+        if (r[j].job == jb) {
+            left += 1;
+            mx = max(mx, r[j].samples);
+            un += r[j].unconverged ? 1 : 0;
+        } else {
+            atomicSub(&e.job_left[r[j].job], 1);
+            atomicMax(&e.job_max_count[r[j].job], r[j].samples);
+            if (r[j].unconverged) atomicAdd(&e.job_unconverged[r[j].job], 1);
         }
     }
-    return requeue;
+    for (int o = 32; o >= 1; o >>= 1) {
+        left += __shfl_xor(left, o, 64);
+        mx = max(mx, __shfl_xor(mx, o, 64));
+        un += __shfl_xor(un, o, 64);
+    }
+    if (__lane_id() == 0 && left) {
+        atomicAdd(&s_left, left);
+        atomicMax(&s_max, mx);
+        if (un) atomicAdd(&s_unconverged, un);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && s_left) {
+        atomicSub(&e.job_left[jb], s_left);
+        atomicMax(&e.job_max_count[jb], s_max);
+        if (s_unconverged) atomicAdd(&e.job_unconverged[jb], s_unconverged);
+    }
 }
 
 // Requeues of the kFinishItems paths of every thread, appended to the ready ring with one cursor
@@ -330,13 +371,15 @@ __global__ __launch_bounds__(kFinishBlock) void k_finish_hits(EngineView e,
     int cnt[kFinishItems];
     int2 first[kFinishItems];
     int32_t stride[kFinishItems];
+    Retired ret[kFinishItems];
 #pragma unroll
     for (int j = 0; j < kFinishItems; ++j) {
         const int i = base + j * kFinishBlock + (int)threadIdx.x;
         const bool valid = i < n && !alive[i];
         cnt[j] = finish_eval(e, valid ? pixel[i] : 0, valid ? tags[i] : 0, valid, &first[j],
-                             &stride[j]);
+                             &stride[j], &ret[j]);
     }
+    finish_retire(e, ret);
     finish_requeue(e, cnt, first, stride, lds);
 }
 
@@ -350,13 +393,15 @@ __global__ __launch_bounds__(kFinishBlock) void k_finish_misses(EngineView e,
     int cnt[kFinishItems];
     int2 first[kFinishItems];
     int32_t stride[kFinishItems];
+    Retired ret[kFinishItems];
 #pragma unroll
     for (int j = 0; j < kFinishItems; ++j) {
         const int i = base + j * kFinishBlock + (int)threadIdx.x;
         const bool valid = i < n;
         cnt[j] = finish_eval(e, valid ? pixel[i] : 0, valid ? tags[i] : 0, valid, &first[j],
-                             &stride[j]);
+                             &stride[j], &ret[j]);
     }
+    finish_retire(e, ret);
     finish_requeue(e, cnt, first, stride, lds);
 }
 
@@ -842,6 +887,18 @@ wf_status wf_engine_admit(wf_engine* e, const wf_job_desc* job, int32_t* job_id)
     WF_REQUIRE(P64 <= e->px, "wf_engine_admit: %lld pixels exceed the engine's pixel budget %d",
                (long long)P64, e->px);
     const int P = (int)P64;
+    int lanes = 1;
+    const int most = std::min(e->lanes, lanes_of(e->cfg.sample_lanes));
+    if (!job->adaptive) {
+        while (lanes * 2 <= most && lanes * 2 <= job->spp) lanes *= 2;
+    } else {
+        // rounds of lanes that divide the convergence-check interval
+        while (lanes * 2 <= std::min(most, kMaxAdaptiveLanes) && lanes * 2 <= job->spp &&
+               job->adaptive_interval % (lanes * 2) == 0)
+            lanes *= 2;
+    }
+    const long long left = job->adaptive ? (long long)P : (long long)P * lanes;
+    WF_REQUIRE(left <= INT32_MAX, "wf_engine_admit: %lld pixel lanes exceed the job counter", left);
     int slot = -1;
     for (int j = 0; j < e->max_jobs; ++j)
         if (e->job_state[j] == 0) {
@@ -875,16 +932,6 @@ wf_status wf_engine_admit(wf_engine* e, const wf_job_desc* job, int32_t* job_id)
     J.adaptive_threshold = job->adaptive_threshold;
     J.adaptive_min_samples = job->adaptive_min_samples;
     J.adaptive_interval = job->adaptive ? job->adaptive_interval : 1;
-    int lanes = 1;
-    const int most = std::min(e->lanes, lanes_of(e->cfg.sample_lanes));
-    if (!job->adaptive) {
-        while (lanes * 2 <= most && lanes * 2 <= job->spp) lanes *= 2;
-    } else {
-        // rounds of lanes that divide the convergence-check interval
-        while (lanes * 2 <= std::min(most, kMaxAdaptiveLanes) && lanes * 2 <= job->spp &&
-               job->adaptive_interval % (lanes * 2) == 0)
-            lanes *= 2;
-    }
     J.lanes = lanes;
     J.lane_stride = e->px;
     const CamBasis cb = camera_basis(job->camera, job->width, job->height);
@@ -894,7 +941,7 @@ wf_status wf_engine_admit(wf_engine* e, const wf_job_desc* job, int32_t* job_id)
     const int queue = (job->spp > 0 && job->max_depth > 0) ? 1 : 0;
     k_admit<<<grid_for(P, 256), 256, 0, ctx->stream>>>(e->view, slot, base, P, lanes, queue);
     WF_HIP(hipGetLastError());
-    k_admit_commit<<<1, 1, 0, ctx->stream>>>(e->view, slot, P, lanes, queue);
+    k_admit_commit<<<1, 1, 0, ctx->stream>>>(e->view, slot, P, lanes, queue, (int)left);
     WF_HIP(hipGetLastError());
     // the host copy of J must outlive the asynchronous upload
     WF_HIP(hipStreamSynchronize(ctx->stream));
