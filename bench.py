@@ -189,6 +189,7 @@ def traversal_stats(wf, ctx, scene, data, W, H, rng) -> dict:
 
 # Library kernel names (per-launch HIP-event timing) -> device kernel symbols in rocprofv3 output
 KERNEL_SYMBOLS = {"traverse_closest": "k_trace_closest", "shadow": "k_shadow",
+                  "traverse_closest_shadow": "k_closest_shadow",
                   "split": "k_split", "shade": "k_shade", "gather_rays": "k_gather_rays",
                   "miss": "k_miss", "regen": "k_regen", "ray_keys": "k_ray_keys"}
 

@@ -103,22 +103,33 @@ Timed::Timed(wf_context* c, const char* n, double bytes_per_unit, const Count& c
     count_dev = count.dev;
 }
 
+Timed::Timed(wf_context* c, const char* n, double bytes_per_unit, const Count& count,
+             double bytes_per_unit2, const Count& count2)
+    : Timed(c, n, bytes_per_unit, count) {
+    bytes2 = count2.dev ? bytes_per_unit2 : bytes_per_unit2 * count2.n;
+    count_dev2 = count2.dev;
+}
+
+// A pinned slot receiving the device count at dev once the stream gets there (-1: none).
+static int copy_count(wf_context* ctx, const int32_t* dev) {
+    if (!dev) return -1;
+    if (!ctx->count_slots &&
+        hipHostMalloc((void**)&ctx->count_slots, sizeof(int32_t) * kCountSlots) != hipSuccess)
+        ctx->count_slots = nullptr;
+    if (!ctx->count_slots) return -1;
+    const int slot = ctx->next_count_slot;
+    ctx->next_count_slot = (slot + 1) % kCountSlots;
+    (void)hipMemcpyAsync(ctx->count_slots + slot, dev, sizeof(int32_t), hipMemcpyDeviceToHost,
+                         ctx->stream);
+    return slot;
+}
+
 Timed::~Timed() {
     if (!ctx || !ctx->profile || !a || !b) return;
     (void)hipEventRecord(b, ctx->stream);
-    int slot = -1;
-    if (count_dev) {
-        if (!ctx->count_slots &&
-            hipHostMalloc((void**)&ctx->count_slots, sizeof(int32_t) * kCountSlots) != hipSuccess)
-            ctx->count_slots = nullptr;
-        if (ctx->count_slots) {
-            slot = ctx->next_count_slot;
-            ctx->next_count_slot = (slot + 1) % kCountSlots;
-            (void)hipMemcpyAsync(ctx->count_slots + slot, count_dev, sizeof(int32_t),
-                                 hipMemcpyDeviceToHost, ctx->stream);
-        }
-    }
-    ctx->pending.push_back(PendingTiming{name, a, b, bytes, slot});
+    const int slot = copy_count(ctx, count_dev);
+    const int slot2 = copy_count(ctx, count_dev2);
+    ctx->pending.push_back(PendingTiming{name, a, b, bytes, slot, bytes2, slot2});
     if (ctx->pending.size() >= kCountSlots / 2) harvest_timings(ctx);
 }
 
@@ -132,6 +143,8 @@ wf_status harvest_timings(wf_context* ctx) {
         e.ms += ms;
         e.launches += 1;
         e.bytes += p.count_slot >= 0 ? p.bytes * (double)ctx->count_slots[p.count_slot] : p.bytes;
+        e.bytes += p.count_slot2 >= 0 ? p.bytes2 * (double)ctx->count_slots[p.count_slot2]
+                                      : p.bytes2;
         ctx->event_pool.push_back(p.start);
         ctx->event_pool.push_back(p.stop);
     }

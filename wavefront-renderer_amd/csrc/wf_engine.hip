@@ -462,6 +462,10 @@ struct wf_engine {
     wf_ray_soa miss{};
     wf_shade_soa s0{}, s1{};
     int32_t *t0 = nullptr, *t1 = nullptr, *st0 = nullptr, *st1 = nullptr, *mt = nullptr;
+    // the last shaded bounce: its shadow walks run in the next iteration's closest-hit launch,
+    // after which its paths that ended at a hit are finished (pending_tags: their job tags)
+    DeferredShadow pending;
+    const int32_t* pending_tags = nullptr;
     int32_t* idx = nullptr;
     EngineView view{};
     JobDev* jobs_dev = nullptr;
@@ -614,7 +618,27 @@ wf_status enqueue_iteration(wf_engine* e) {
         rt = e->t1;
     }
     phase_mark(e, 2);
-    WF_TRY(intersect_split(ctx, s, rq, live, &e->s0, &e->miss, c + C_HITS, rt, e->st0, e->mt));
+    const unsigned finish_grid = grid_for(cap, kFinishBlock * kFinishItems);
+    {
+        int32_t* hl;
+        float *ht, *hu, *hv;
+        WF_TRY(hit_buffers(ctx, live, &hl, &ht, &hu, &hv));
+        const DeferredShadow& d = e->pending;
+        if (d.armed) {
+            WF_TRY(trace_closest_deferred(ctx, s, rq, live, hl, ht, hu, hv, d));
+            // the previous bounce's paths that ended at a hit, now that their shadow rays are in
+            // (its queue, tags and hit count are still those of that bounce)
+            Timed tm(ctx, "finish", 12.0, hits);
+            k_finish_hits<<<finish_grid, kFinishBlock, 0, ctx->stream>>>(v, d.q.pixel,
+                                                                         e->pending_tags,
+                                                                         d.o.alive);
+            WF_HIP(hipGetLastError());
+        } else {
+            WF_TRY(trace_closest(ctx, s, rq, live, 0.001f, 1e9f, hl, ht, hu, hv, nullptr));
+        }
+        WF_TRY(split_traced(ctx, s, rq, live, hl, ht, hu, hv, &e->s0, &e->miss, c + C_HITS, rt,
+                            e->st0, e->mt));
+    }
     phase_mark(e, 3);
     const wf_shade_soa* q = &e->s0;
     const int32_t* qt = e->st0;
@@ -628,7 +652,6 @@ wf_status enqueue_iteration(wf_engine* e) {
         WF_TRY(coherence_accumulate(ctx, q->mat, q->tex, hits,
                                     e->coherence_mode >= WF_COHERENCE_CACHE_LINES, e->coherence));
     phase_mark(e, 4);
-    const uint32_t* alive = nullptr;
     ShadeJobs jobs;
     jobs.tags = qt;
     jobs.jobs = e->jobs_dev;
@@ -637,17 +660,15 @@ wf_status enqueue_iteration(wf_engine* e) {
     jobs.next_keys = sort ? e->keys : nullptr;
     jobs.key_frame = kf;
     jobs.unordered = true;
-    WF_TRY(shade_bounce(ctx, s, q, hits, &e->cfg.shade, v.accum, nullptr, c + C_NEXT, jobs,
-                        &alive));
+    jobs.defer = &e->pending;
+    WF_TRY(shade_bounce(ctx, s, q, hits, &e->cfg.shade, v.accum, nullptr, c + C_NEXT, jobs));
+    e->pending_tags = qt;
     phase_mark(e, 5);
     WF_TRY(shade_misses(ctx, s, &e->miss, misses, e->cfg.shade.physics, v.accum));
     phase_mark(e, 6);
     {
-        Timed tm(ctx, "finish", 12.0, live);
-        const unsigned grid = grid_for(cap, kFinishBlock * kFinishItems);
-        k_finish_hits<<<grid, kFinishBlock, 0, ctx->stream>>>(v, q->pixel, qt, alive);
-        WF_HIP(hipGetLastError());
-        k_finish_misses<<<grid, kFinishBlock, 0, ctx->stream>>>(v, e->miss.pixel, e->mt);
+        Timed tm(ctx, "finish", 8.0, misses);
+        k_finish_misses<<<finish_grid, kFinishBlock, 0, ctx->stream>>>(v, e->miss.pixel, e->mt);
         WF_HIP(hipGetLastError());
         k_iteration_end<<<1, 1, 0, ctx->stream>>>(v);
         WF_HIP(hipGetLastError());

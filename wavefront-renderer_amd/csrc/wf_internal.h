@@ -50,6 +50,8 @@ struct PendingTiming {
     hipEvent_t start, stop;
     double bytes;              // algorithmic bytes, or bytes per unit when count_slot >= 0
     int count_slot;            // pinned slot receiving the launch's device-side element count
+    double bytes2 = 0.0;       // a second kind of unit: fixed bytes, or per unit of count_slot2
+    int count_slot2 = -1;
 };
 
 }  // namespace wf
@@ -107,9 +109,14 @@ struct Timed {
     const char* name;
     double bytes;
     const int32_t* count_dev = nullptr;
+    double bytes2 = 0.0;
+    const int32_t* count_dev2 = nullptr;
     hipEvent_t a = nullptr, b = nullptr;
     Timed(wf_context* c, const char* n, double algorithmic_bytes);
     Timed(wf_context* c, const char* n, double bytes_per_unit, const Count& count);
+    // a launch over two kinds of units (per-unit bytes and count of each)
+    Timed(wf_context* c, const char* n, double bytes_per_unit, const Count& count,
+          double bytes_per_unit2, const Count& count2);
     ~Timed();
 };
 wf_status harvest_timings(wf_context* ctx);
@@ -409,6 +416,20 @@ struct RayPacked {
 wf_status trace_closest(wf_context* ctx, const wf_scene* s, const RayPacked* rays, Count n,
                         float tnear, float tfar, int32_t* hit_leaf, float* hit_t, float* hit_bu,
                         float* hit_bv, uint32_t* flags);
+struct DeferredShadow;
+// Closest hits of packed rays into hit buffers (leaf, t, u, v per ray), with the shadow walks of
+// a deferred bounce traced by the same persistent launch.
+wf_status trace_closest_deferred(wf_context* ctx, const wf_scene* s, const RayPacked* rays,
+                                 Count n, int32_t* hit_leaf, float* hit_t, float* hit_bu,
+                                 float* hit_bv, const DeferredShadow& d);
+// The context's closest-hit buffers for n rays.
+wf_status hit_buffers(wf_context* ctx, Count n, int32_t** leaf, float** t, float** u, float** v);
+// The hit/miss split of traced packed rays (hits and misses appended at device cursors).
+wf_status split_traced(wf_context* ctx, const wf_scene* s, const RayPacked* rays, Count n,
+                       const int32_t* hit_leaf, const float* hit_t, const float* hit_bu,
+                       const float* hit_bv, wf_shade_soa* hits_out, wf_ray_soa* misses_out,
+                       int32_t* counts_dev, const int32_t* tags_in, int32_t* hit_tags,
+                       int32_t* miss_tags);
 // Unordered closest hit + split of packed rays (hits and misses appended at device cursors).
 wf_status intersect_split(wf_context* ctx, const wf_scene* s, const RayPacked* rays, Count n,
                           wf_shade_soa* hits_out, wf_ray_soa* misses_out, int32_t* counts_dev,
@@ -418,6 +439,42 @@ wf_status ray_schedule(wf_context* ctx, const RayPacked* rays, Count n, const fl
 wf_status gather_rays(wf_context* ctx, const RayPacked* src, const int32_t* idx, Count n,
                       RayPacked* dst, const int32_t* tags_in, int32_t* tags_out);
 
+// Per-entry outputs of k_shade consumed by the shadow and compaction kernels.
+struct ShadeOut {
+    float* ldx; float* ldy; float* ldz; float* lmax;   // area/environment shadow ray
+    float* lr; float* lg; float* lb;                   // its contribution
+    float* dr; float* dg; float* db;                   // directional-light contribution
+    uint32_t* shadow_flag;                             // bit0 area/env ray, bit1 directional
+    float* ndx; float* ndy; float* ndz;                // scattered direction
+    float* ntr; float* ntg; float* ntb;                // next throughput
+    float* npdf;
+    uint32_t* alive;
+    // unordered append mode (queue order free): shadow entries and next rays are appended with
+    // wave-aggregated atomics instead of a scan + compaction
+    int32_t append;
+    int32_t* shadow_list;
+    int32_t* shadow_entries;                           // device counter
+    int32_t* shadow_rays;                              // device counter
+    int32_t* next_count;                               // device counter
+    wf_ray_soa next;
+    RayPacked next_packed;                             // used instead of `next` when set
+    int32_t* next_tags;
+    uint64_t* next_keys;                               // sort word of each next ray, when set
+    RayKeyFrame key_frame;
+    CostProbe probe;                                   // live cost tracking, when set
+};
+
+// Shadow entries of a shaded bounce whose occlusion walks were handed to the next closest-hit
+// launch instead of a launch of their own: what that launch needs of the bounce.
+struct DeferredShadow {
+    bool armed = false;
+    wf_shade_soa q{};             // the bounce's shading queue (hit points, accumulators)
+    ShadeOut o{};                 // its shading outputs (shadow rays and contributions)
+    const int32_t* list = nullptr;   // shadow entries (shading-queue indices)
+    Count n{};                    // their count (device)
+    float* accum = nullptr;
+};
+
 struct ShadeJobs {
     const int32_t* tags = nullptr;     // per shading-queue entry
     const JobDev* jobs = nullptr;
@@ -426,6 +483,8 @@ struct ShadeJobs {
     uint64_t* next_keys = nullptr;     // unordered mode: sort words of the next rays
     RayKeyFrame key_frame{};           // ... in this frame
     bool unordered = false;            // next rays / shadow entries may come in any order
+    DeferredShadow* defer = nullptr;   // unordered mode: record the shadow walks here instead
+                                       // of running them (trace_closest_deferred runs them)
 };
 // One bounce of shadeAll: emission into accum, shadow rays resolved, next rays compacted into
 // `next`; counts_dev[0] = next rays, counts_dev[1] = shadow rays traced, counts_dev[2] = shadow

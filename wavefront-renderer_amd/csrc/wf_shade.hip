@@ -25,30 +25,6 @@ struct ShadeArgs {
     int32_t job;
 };
 
-// Per-entry outputs of k_shade consumed by the shadow and compaction kernels.
-struct ShadeOut {
-    float* ldx; float* ldy; float* ldz; float* lmax;   // area/environment shadow ray
-    float* lr; float* lg; float* lb;                   // its contribution
-    float* dr; float* dg; float* db;                   // directional-light contribution
-    uint32_t* shadow_flag;                             // bit0 area/env ray, bit1 directional
-    float* ndx; float* ndy; float* ndz;                // scattered direction
-    float* ntr; float* ntg; float* ntb;                // next throughput
-    float* npdf;
-    uint32_t* alive;
-    // unordered append mode (queue order free): shadow entries and next rays are appended with
-    // wave-aggregated atomics instead of a scan + compaction
-    int32_t append;
-    int32_t* shadow_list;
-    int32_t* shadow_entries;                           // device counter
-    int32_t* shadow_rays;                              // device counter
-    int32_t* next_count;                               // device counter
-    wf_ray_soa next;
-    RayPacked next_packed;                             // used instead of `next` when set
-    int32_t* next_tags;
-    uint64_t* next_keys;                               // sort word of each next ray, when set
-    RayKeyFrame key_frame;
-    CostProbe probe;                                   // live cost tracking, when set
-};
 
 
 __device__ __forceinline__ V3 texel(const SceneDev& s, int tid, float u, float v) {
@@ -701,19 +677,17 @@ __global__ void k_miss(const SceneDev s, wf_ray_soa q, int n_host, const int32_t
 
 // EmbreeAccelerator::occluded for the NEE rays of each listed entry, then the contributions in
 // the reference order: area/environment light first, directional light second (cpp:678, 720).
+// The body of the shadow kernels, run by a persistent block over entries [0, n) of list; stack,
+// xchg and spill as for closest_walks.
 template <class NODE, int STACK>
-__global__ __launch_bounds__(kTraceBlock) void k_shadow(const SceneDev s, const NODE* __restrict__ nodes,
-                                                        wf_shade_soa q, ShadeOut o,
-                                                        const int32_t* __restrict__ list, int count_host,
-                                                        const int32_t* __restrict__ count_dev,
-                                                        float* __restrict__ accum,
-                                                        int32_t* __restrict__ counter,
-                                                        int refill_lanes) {
-    __shared__ int32_t stack[lds_entries<STACK>() * kTraceBlock];
-    __shared__ int4 xchg[kTraceBlock];   // per wave: 64 candidate-test slots
+__device__ __forceinline__ void shadow_walks(const SceneDev& s, const NODE* __restrict__ nodes,
+                                             const wf_shade_soa& q, const ShadeOut& o,
+                                             const int32_t* __restrict__ list, int n,
+                                             float* __restrict__ accum,
+                                             int32_t* __restrict__ counter, int refill_lanes,
+                                             int32_t* __restrict__ stack, int4* __restrict__ xchg,
+                                             volatile int32_t* spill) {
     using W = Walker<true, STACK>;
-    volatile int32_t spill[W::X > 0 ? W::X : 1];
-    const int n = load_count(count_host, count_dev);
     int4* wx = xchg + (threadIdx.x & ~63u);
     const V3 tl = V3(s.directional.direction[0], s.directional.direction[1],
                      s.directional.direction[2]) * -1.0f;
@@ -789,6 +763,43 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(const SceneDev s, const 
     }
 }
 
+template <class NODE, int STACK>
+__global__ __launch_bounds__(kTraceBlock) void k_shadow(const SceneDev s, const NODE* __restrict__ nodes,
+                                                        wf_shade_soa q, ShadeOut o,
+                                                        const int32_t* __restrict__ list, int count_host,
+                                                        const int32_t* __restrict__ count_dev,
+                                                        float* __restrict__ accum,
+                                                        int32_t* __restrict__ counter,
+                                                        int refill_lanes) {
+    __shared__ int32_t stack[lds_entries<STACK>() * kTraceBlock];
+    __shared__ int4 xchg[kTraceBlock];   // per wave: 64 candidate-test slots
+    volatile int32_t spill[Walker<true, STACK>::X > 0 ? Walker<true, STACK>::X : 1];
+    shadow_walks<NODE, STACK>(s, nodes, q, o, list, load_count(count_host, count_dev), accum,
+                              counter, refill_lanes, stack, xchg, spill);
+}
+
+// Closest hits of a bounce's rays, then the deferred shadow walks of the bounce before it, in
+// one persistent launch: a wave that finds the ray queue drained moves on to shadow entries
+// while the slowest rays of other waves finish. The two queues are independent (the shadow
+// entries only add to the accumulators of paths shaded the bounce before).
+template <class NODE, int STACK>
+__global__ __launch_bounds__(kTraceBlock) void k_closest_shadow(
+    const SceneDev s, const NODE* __restrict__ nodes, RayPacked rays, int n_host,
+    const int32_t* __restrict__ nd, int32_t* __restrict__ out_leaf, float* __restrict__ out_t,
+    float* __restrict__ out_u, float* __restrict__ out_v, int32_t* __restrict__ ray_counter,
+    int refill_lanes, wf_shade_soa q, ShadeOut o, const int32_t* __restrict__ list,
+    int shadow_host, const int32_t* __restrict__ shadow_dev, float* __restrict__ accum,
+    int32_t* __restrict__ shadow_counter, int shadow_refill_lanes) {
+    __shared__ int32_t stack[lds_entries<STACK>() * kTraceBlock];
+    __shared__ int4 xchg[kTraceBlock];   // per wave: 64 candidate-test slots
+    volatile int32_t spill[Walker<false, STACK>::X > 0 ? Walker<false, STACK>::X : 1];
+    closest_walks<NODE, STACK>(nodes, s.tri, s.n_tris, rays, load_count(n_host, nd), 0.001f,
+                               1e9f, out_leaf, out_t, out_u, out_v, nullptr, ray_counter,
+                               refill_lanes, stack, xchg, spill);
+    shadow_walks<NODE, STACK>(s, nodes, q, o, list, load_count(shadow_host, shadow_dev), accum,
+                              shadow_counter, shadow_refill_lanes, stack, xchg, spill);
+}
+
 __global__ void k_count_shadow_rays(const uint32_t* __restrict__ flag, const int32_t* __restrict__ list,
                                     int count_host, const int32_t* __restrict__ count_dev,
                                     int32_t* __restrict__ total) {
@@ -831,6 +842,30 @@ static wf_status shade_outputs(wf_context* ctx, int n, ShadeOut& o) {
 static int shadow_refill(const wf_context* ctx) {
     return ctx->tuning.shadow_refill_lanes > 0 ? ctx->tuning.shadow_refill_lanes
                                                : ctx->tuning.refill_lanes;
+}
+
+wf_status trace_closest_deferred(wf_context* ctx, const wf_scene* s, const RayPacked* rays,
+                                 Count n, int32_t* hit_leaf, float* hit_t, float* hit_bu,
+                                 float* hit_bv, const DeferredShadow& d) {
+    WF_TRY(scene_ready(s));
+    if (n.n <= 0) return WF_OK;
+    void *prc, *psc;
+    WF_TRY(workspace(ctx, "trace_counter", sizeof(int32_t) * kWorkInts, &prc));
+    WF_TRY(workspace(ctx, "shadow_counter", sizeof(int32_t) * kWorkInts, &psc));
+    WF_TRY(wf_fill_u32(ctx, prc, 0u, kWorkInts));
+    WF_TRY(wf_fill_u32(ctx, psc, 0u, kWorkInts));
+    // rays as for the closest-hit kernel (44 B), shadow entries as for the shadow kernel (41 B)
+    Timed tm(ctx, "traverse_closest_shadow", 44.0, n, 41.0, d.n);
+    with_bvh(s->dev, [&](auto nodes, auto S) {
+        k_closest_shadow<std::remove_cv_t<std::remove_pointer_t<decltype(nodes)>>,
+                         decltype(S)::value>
+            <<<persistent_grid(n.n, ctx), kTraceBlock, 0, ctx->stream>>>(
+                s->dev, nodes, *rays, n.n, n.dev, hit_leaf, hit_t, hit_bu, hit_bv,
+                (int32_t*)prc, ctx->tuning.refill_lanes, d.q, d.o, d.list, d.n.n, d.n.dev,
+                d.accum, (int32_t*)psc, shadow_refill(ctx));
+    });
+    WF_HIP(hipGetLastError());
+    return WF_OK;
 }
 
 wf_status shade_bounce(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q, Count n,
@@ -880,6 +915,19 @@ wf_status shade_bounce(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q
             WF_HIP(hipGetLastError());
         }
         const Count ns{n.n, counts_dev + 2};
+        if (jobs.defer) {
+            // sized here so that a captured iteration finds the launch's counters allocated
+            void* pc = nullptr;
+            WF_TRY(workspace(ctx, "shadow_counter", sizeof(int32_t) * kWorkInts, &pc));
+            jobs.defer->armed = true;
+            jobs.defer->q = *q;
+            jobs.defer->o = o;
+            jobs.defer->list = (int32_t*)plist;
+            jobs.defer->n = ns;
+            jobs.defer->accum = accum;
+            if (alive_out) *alive_out = o.alive;
+            return WF_OK;
+        }
         void* pc = nullptr;
         WF_TRY(workspace(ctx, "shadow_counter", sizeof(int32_t) * kWorkInts, &pc));
         WF_TRY(wf_fill_u32(ctx, pc, 0u, kWorkInts));

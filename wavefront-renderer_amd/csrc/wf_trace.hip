@@ -25,37 +25,10 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(
     int refill_lanes) {
     __shared__ int32_t stack[lds_entries<STACK>() * kTraceBlock];
     __shared__ int4 xchg[kTraceBlock];   // per wave: 64 candidate-test slots
-    using W = Walker<false, STACK>;
-    volatile int32_t spill[W::X > 0 ? W::X : 1];
-    const int n = load_count(n_host, nd);
-    int4* wx = xchg + (threadIdx.x & ~63u);
-    W w;
-    int ray = -1;
-    WorkQueue wq;
-    wq.init(counter, n);
-    for (;;) {
-        const int r = refill(wq, ray < 0, refill_lanes);
-        if (r >= 0) {
-            ray = r;
-            V3 o, d;
-            ld_od(rays, r, o, d);
-            w.start(o, d, tnear, tfar);
-        }
-        if (__ballot(ray >= 0) == 0) {
-            if (wq.exhausted()) break;
-            continue;
-        }
-        // every lane of the wave takes part in the step (idle lanes test others' triangles)
-        const bool done = w.step_wave(ray >= 0 && n_tris > 0, nodes, tri, stack, spill, wx);
-        if (ray >= 0 && (n_tris <= 0 || done)) {
-            out_leaf[ray] = w.h.leaf;
-            out_t[ray] = w.h.t;
-            out_u[ray] = w.h.u;
-            out_v[ray] = w.h.v;
-            if (out_flag) out_flag[ray] = w.h.leaf >= 0 ? 1u : 0u;
-            ray = -1;
-        }
-    }
+    volatile int32_t spill[Walker<false, STACK>::X > 0 ? Walker<false, STACK>::X : 1];
+    closest_walks<NODE, STACK>(nodes, tri, n_tris, rays, load_count(n_host, nd), tnear, tfar,
+                               out_leaf, out_t, out_u, out_v, out_flag, counter, refill_lanes,
+                               stack, xchg, spill);
 }
 
 // occluded(): tnear 0.001, tfar = maxDistance - 0.001; a negative tfar reports occluded
@@ -302,30 +275,60 @@ wf_status trace_occluded(const wf_scene* s, const float* ox, const float* oy, co
     return WF_OK;
 }
 
+wf_status hit_buffers(wf_context* ctx, Count n, int32_t** leaf, float** t, float** u, float** v) {
+    const size_t nn = (size_t)(n.n > 0 ? n.n : 1);
+    void *pl, *pt, *pu, *pv;
+    WF_TRY(workspace(ctx, "hit_leaf", sizeof(int32_t) * nn, &pl));
+    WF_TRY(workspace(ctx, "hit_t", sizeof(float) * nn, &pt));
+    WF_TRY(workspace(ctx, "hit_u", sizeof(float) * nn, &pu));
+    WF_TRY(workspace(ctx, "hit_v", sizeof(float) * nn, &pv));
+    *leaf = (int32_t*)pl;
+    *t = (float*)pt;
+    *u = (float*)pu;
+    *v = (float*)pv;
+    return WF_OK;
+}
+
+// Hits and misses of traced rays appended at device cursors (queue order free).
+template <class R>
+static wf_status split_append(wf_context* ctx, const wf_scene* s, const R& rays, Count n,
+                              const int32_t* hl, const float* ht, const float* hu,
+                              const float* hv, wf_shade_soa* hits_out, wf_ray_soa* misses_out,
+                              int32_t* counts_dev, const int32_t* tags_in, int32_t* hit_tags,
+                              int32_t* miss_tags) {
+    WF_TRY(wf_fill_u32(ctx, counts_dev, 0u, 2));
+    if (n.n > 0) {
+        Timed tm(ctx, "split", 97.0, n);
+        k_split_append<R><<<grid_for(n.n, kAppendBlock * kAppendItems), kAppendBlock, 0,
+                            ctx->stream>>>(
+            s->dev, rays, n.n, n.dev, hl, ht, hu, hv, *hits_out, *misses_out, counts_dev,
+            tags_in, hit_tags, miss_tags);
+        WF_HIP(hipGetLastError());
+    }
+    return WF_OK;
+}
+
+wf_status split_traced(wf_context* ctx, const wf_scene* s, const RayPacked* rays, Count n,
+                       const int32_t* hit_leaf, const float* hit_t, const float* hit_bu,
+                       const float* hit_bv, wf_shade_soa* hits_out, wf_ray_soa* misses_out,
+                       int32_t* counts_dev, const int32_t* tags_in, int32_t* hit_tags,
+                       int32_t* miss_tags) {
+    return split_append(ctx, s, *rays, n, hit_leaf, hit_t, hit_bu, hit_bv, hits_out, misses_out,
+                        counts_dev, tags_in, hit_tags, miss_tags);
+}
+
 // Closest hit, then hits and misses appended at device cursors (queue order free).
 template <class R>
 static wf_status split_unordered(wf_context* ctx, const wf_scene* s, const R& rays, Count n,
                                  wf_shade_soa* hits_out, wf_ray_soa* misses_out,
                                  int32_t* counts_dev, const int32_t* tags_in, int32_t* hit_tags,
                                  int32_t* miss_tags) {
-    void *pl, *pt, *pu, *pv;
-    size_t nn = (size_t)(n.n > 0 ? n.n : 1);
-    WF_TRY(workspace(ctx, "hit_leaf", sizeof(int32_t) * nn, &pl));
-    WF_TRY(workspace(ctx, "hit_t", sizeof(float) * nn, &pt));
-    WF_TRY(workspace(ctx, "hit_u", sizeof(float) * nn, &pu));
-    WF_TRY(workspace(ctx, "hit_v", sizeof(float) * nn, &pv));
-    WF_TRY(wf_fill_u32(ctx, counts_dev, 0u, 2));
-    WF_TRY(trace_closest_any(ctx, s, rays, n, 0.001f, 1e9f, (int32_t*)pl, (float*)pt,
-                             (float*)pu, (float*)pv, nullptr));
-    if (n.n > 0) {
-        Timed tm(ctx, "split", 97.0, n);
-        k_split_append<R><<<grid_for(n.n, kAppendBlock * kAppendItems), kAppendBlock, 0,
-                            ctx->stream>>>(
-            s->dev, rays, n.n, n.dev, (int32_t*)pl, (float*)pt, (float*)pu, (float*)pv,
-            *hits_out, *misses_out, counts_dev, tags_in, hit_tags, miss_tags);
-        WF_HIP(hipGetLastError());
-    }
-    return WF_OK;
+    int32_t* hl;
+    float *ht, *hu, *hv;
+    WF_TRY(hit_buffers(ctx, n, &hl, &ht, &hu, &hv));
+    WF_TRY(trace_closest_any(ctx, s, rays, n, 0.001f, 1e9f, hl, ht, hu, hv, nullptr));
+    return split_append(ctx, s, rays, n, hl, ht, hu, hv, hits_out, misses_out, counts_dev,
+                        tags_in, hit_tags, miss_tags);
 }
 
 wf_status intersect_split(wf_context* ctx, const wf_scene* s, const RayPacked* rays, Count n,

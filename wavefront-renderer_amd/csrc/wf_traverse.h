@@ -9,6 +9,7 @@
 
 #include "wf_device.h"
 #include "wf_internal.h"
+#include "wf_rays.h"
 
 namespace wf {
 
@@ -600,6 +601,52 @@ __device__ __forceinline__ int refill(WorkQueue& q, bool idle, int refill_lanes)
     if (!idle || first < 0) return -1;
     const int r = first + __popcll(want & lanemask_lt());
     return r < end ? r : -1;
+}
+
+// Closest hits of rays [0, n) by the calling persistent block (the body of the closest-hit
+// kernels): each lane walks one ray at a time and takes the next from the work queue when done.
+// stack / xchg: the block's LDS stack columns and candidate-exchange slots; spill: this lane's
+// private stack overflow.
+template <class NODE, int STACK, class R>
+__device__ __forceinline__ void closest_walks(const NODE* __restrict__ nodes,
+                                              const float4* __restrict__ tri, int n_tris,
+                                              const R& rays, int n, float tnear, float tfar,
+                                              int32_t* __restrict__ out_leaf,
+                                              float* __restrict__ out_t, float* __restrict__ out_u,
+                                              float* __restrict__ out_v,
+                                              uint32_t* __restrict__ out_flag,
+                                              int32_t* __restrict__ counter, int refill_lanes,
+                                              int32_t* __restrict__ stack, int4* __restrict__ xchg,
+                                              volatile int32_t* spill) {
+    using W = Walker<false, STACK>;
+    int4* wx = xchg + (threadIdx.x & ~63u);
+    W w;
+    int ray = -1;
+    WorkQueue wq;
+    wq.init(counter, n);
+    for (;;) {
+        const int r = refill(wq, ray < 0, refill_lanes);
+        if (r >= 0) {
+            ray = r;
+            V3 o, d;
+            ld_od(rays, r, o, d);
+            w.start(o, d, tnear, tfar);
+        }
+        if (__ballot(ray >= 0) == 0) {
+            if (wq.exhausted()) break;
+            continue;
+        }
+        // every lane of the wave takes part in the step (idle lanes test others' triangles)
+        const bool done = w.step_wave(ray >= 0 && n_tris > 0, nodes, tri, stack, spill, wx);
+        if (ray >= 0 && (n_tris <= 0 || done)) {
+            out_leaf[ray] = w.h.leaf;
+            out_t[ray] = w.h.t;
+            out_u[ray] = w.h.u;
+            out_v[ray] = w.h.v;
+            if (out_flag) out_flag[ray] = w.h.leaf >= 0 ? 1u : 0u;
+            ray = -1;
+        }
+    }
 }
 
 // Persistent-grid size for traversal kernels: blocks_per_cu resident blocks on every CU
