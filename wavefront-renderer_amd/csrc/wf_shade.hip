@@ -25,8 +25,6 @@ struct ShadeArgs {
     int32_t job;
 };
 
-
-
 __device__ __forceinline__ V3 texel(const SceneDev& s, int tid, float u, float v) {
     const DevTexture T = s.textures[tid];
     u = u - floorf(u);
