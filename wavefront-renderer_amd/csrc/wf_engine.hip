@@ -292,9 +292,23 @@ __device__ __forceinline__ int finish_eval(const EngineView& e, int a, int32_t t
     return requeue;
 }
 
+// Which ended paths a completion pass handles. A lane of a job with a fixed sample count that
+// goes on with its next sample reads no accumulator, and its next path adds to the accumulator
+// only after the ended path's deferred shadow rays have (those run in the next closest-hit
+// launch, before any shading of that iteration), so it is requeued at once (kFinishNext); ends of
+// lanes and adaptive rounds read or publish the accumulators and wait for the shadow rays
+// (kFinishRest). Misses have no pending shadow rays and are finished in one pass.
+enum FinishMode : int { kFinishNext = 1, kFinishRest = 2 };
+
+__device__ __forceinline__ bool finish_selected(const EngineView& e, int32_t tag, int mode) {
+    const JobDev& J = e.jobs[tag >> kTagJobShift];
+    const bool next = !J.adaptive && (tag & kTagSampleMask) + J.lanes < J.spp;
+    return (mode == kFinishNext) == next;
+}
+
 // Job counters of the block's retirements: those of the job that one of them claims first are
-// summed in LDS and applied with one set of atomics per block, the others (a job boundary inside
-// the block) with single atomics. Every thread of the block calls it.
+// summed in LDS and applied with one set of atomics per block; those of other jobs (blocks that
+// mix jobs) with one set per wave and job. Every thread of the block calls it.
 __device__ __forceinline__ void finish_retire(const EngineView& e,
                                               const Retired (&r)[kFinishItems]) {
     __shared__ int32_t s_job, s_left, s_max, s_unconverged;
@@ -312,25 +326,52 @@ __device__ __forceinline__ void finish_retire(const EngineView& e,
         atomicCAS(&s_job, -1, mine);
     __syncthreads();
     const int32_t jb = s_job;
+    auto wave_sum = [](int& c, int& x, int& u) {
+        for (int o = 32; o >= 1; o >>= 1) {
+            c += __shfl_xor(c, o, 64);
+            x = max(x, __shfl_xor(x, o, 64));
+            u += __shfl_xor(u, o, 64);
+        }
+    };
     int left = 0, mx = 0, un = 0;
+    bool other[kFinishItems];
 #pragma unroll
     for (int j = 0; j < kFinishItems; ++j) {
-        if (!r[j].on) continue;
-        if (r[j].job == jb) {
+        other[j] = r[j].on && r[j].job != jb;
+        if (r[j].on && r[j].job == jb) {
             left += 1;
             mx = max(mx, r[j].samples);
             un += r[j].unconverged ? 1 : 0;
-        } else {
-            atomicSub(&e.job_left[r[j].job], 1);
-            atomicMax(&e.job_max_count[r[j].job], r[j].samples);
-            if (r[j].unconverged) atomicAdd(&e.job_unconverged[r[j].job], 1);
         }
     }
-    for (int o = 32; o >= 1; o >>= 1) {
-        left += __shfl_xor(left, o, 64);
-        mx = max(mx, __shfl_xor(mx, o, 64));
-        un += __shfl_xor(un, o, 64);
+    // the other jobs, one at a time: the job of the wave's first pending retirement (wave-uniform
+    // loop, one iteration per distinct job)
+    for (;;) {
+        int32_t mine = -1;
+#pragma unroll
+        for (int j = 0; j < kFinishItems; ++j)
+            if (mine < 0 && other[j]) mine = r[j].job;
+        const uint64_t m = __ballot(mine >= 0);
+        if (m == 0) break;
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        const int32_t jl = __shfl(mine, leader, 64);
+        int c = 0, x = 0, u = 0;
+#pragma unroll
+        for (int j = 0; j < kFinishItems; ++j)
+            if (other[j] && r[j].job == jl) {
+                c += 1;
+                x = max(x, r[j].samples);
+                u += r[j].unconverged ? 1 : 0;
+                other[j] = false;
+            }
+        wave_sum(c, x, u);
+        if ((int)__lane_id() == leader) {
+            atomicSub(&e.job_left[jl], c);
+            atomicMax(&e.job_max_count[jl], x);
+            if (u) atomicAdd(&e.job_unconverged[jl], u);
+        }
     }
+    wave_sum(left, mx, un);
     if (__lane_id() == 0 && left) {
         atomicAdd(&s_left, left);
         atomicMax(&s_max, mx);
@@ -363,7 +404,8 @@ __device__ __forceinline__ void finish_requeue(const EngineView& e, const int (&
 __global__ __launch_bounds__(kFinishBlock) void k_finish_hits(EngineView e,
                                                               const int32_t* __restrict__ pixel,
                                                               const int32_t* __restrict__ tags,
-                                                              const uint32_t* __restrict__ alive) {
+                                                              const uint32_t* __restrict__ alive,
+                                                              int mode) {
     __shared__ uint32_t lds[kFinishBlock / 64];
     const int n = e.counters[C_HITS];
     const int base = blockIdx.x * kFinishBlock * kFinishItems;
@@ -375,8 +417,9 @@ __global__ __launch_bounds__(kFinishBlock) void k_finish_hits(EngineView e,
 #pragma unroll
     for (int j = 0; j < kFinishItems; ++j) {
         const int i = base + j * kFinishBlock + (int)threadIdx.x;
-        const bool valid = i < n && !alive[i];
-        cnt[j] = finish_eval(e, valid ? pixel[i] : 0, valid ? tags[i] : 0, valid, &first[j],
+        const int32_t tag = i < n && !alive[i] ? tags[i] : -1;
+        const bool valid = tag >= 0 && finish_selected(e, tag, mode);
+        cnt[j] = finish_eval(e, valid ? pixel[i] : 0, valid ? tag : 0, valid, &first[j],
                              &stride[j], &ret[j]);
     }
     finish_retire(e, ret);
@@ -473,6 +516,12 @@ struct wf_engine {
     std::vector<JobDev> jobs;
     std::vector<int32_t> job_state;     // 0 free, 1 running, 2 finished (result not collected)
     std::vector<int64_t> job_iter_start;
+    // running jobs whose lanes take more than one sample (fixed count, fewer lanes than samples):
+    // only they need the end-of-iteration requeue pass; the captured iteration records whether
+    // it has that pass
+    std::vector<uint8_t> job_reuse;
+    int reuse_jobs = 0;
+    bool graph_reuse = false;
     std::vector<std::pair<int32_t, int32_t>> free_ranges;   // [begin, end) of free pixels
     int32_t* status_host = nullptr;     // pinned: counters + job_left per slot (two buffers)
     hipEvent_t status_ev[2] = {nullptr, nullptr};
@@ -626,12 +675,11 @@ wf_status enqueue_iteration(wf_engine* e) {
         const DeferredShadow& d = e->pending;
         if (d.armed) {
             WF_TRY(trace_closest_deferred(ctx, s, rq, live, hl, ht, hu, hv, d));
-            // the previous bounce's paths that ended at a hit, now that their shadow rays are in
-            // (its queue, tags and hit count are still those of that bounce)
+            // the rest of the previous bounce's paths that ended at a hit, now that their shadow
+            // rays are in (its queue, tags and hit count are still those of that bounce)
             Timed tm(ctx, "finish", 12.0, hits);
-            k_finish_hits<<<finish_grid, kFinishBlock, 0, ctx->stream>>>(v, d.q.pixel,
-                                                                         e->pending_tags,
-                                                                         d.o.alive);
+            k_finish_hits<<<finish_grid, kFinishBlock, 0, ctx->stream>>>(
+                v, d.q.pixel, e->pending_tags, d.o.alive, kFinishRest);
             WF_HIP(hipGetLastError());
         } else {
             WF_TRY(trace_closest(ctx, s, rq, live, 0.001f, 1e9f, hl, ht, hu, hv, nullptr));
@@ -652,6 +700,7 @@ wf_status enqueue_iteration(wf_engine* e) {
         WF_TRY(coherence_accumulate(ctx, q->mat, q->tex, hits,
                                     e->coherence_mode >= WF_COHERENCE_CACHE_LINES, e->coherence));
     phase_mark(e, 4);
+    const uint32_t* alive = nullptr;
     ShadeJobs jobs;
     jobs.tags = qt;
     jobs.jobs = e->jobs_dev;
@@ -661,8 +710,15 @@ wf_status enqueue_iteration(wf_engine* e) {
     jobs.key_frame = kf;
     jobs.unordered = true;
     jobs.defer = &e->pending;
-    WF_TRY(shade_bounce(ctx, s, q, hits, &e->cfg.shade, v.accum, nullptr, c + C_NEXT, jobs));
+    WF_TRY(shade_bounce(ctx, s, q, hits, &e->cfg.shade, v.accum, nullptr, c + C_NEXT, jobs,
+                        &alive));
     e->pending_tags = qt;
+    if (e->reuse_jobs > 0) {
+        Timed tm(ctx, "finish", 12.0, hits);
+        k_finish_hits<<<finish_grid, kFinishBlock, 0, ctx->stream>>>(v, q->pixel, qt, alive,
+                                                                     kFinishNext);
+        WF_HIP(hipGetLastError());
+    }
     phase_mark(e, 5);
     WF_TRY(shade_misses(ctx, s, &e->miss, misses, e->cfg.shade.physics, v.accum));
     phase_mark(e, 6);
@@ -719,7 +775,14 @@ void merge_profile(wf_engine* e) {
 wf_status launch_iteration(wf_engine* e, int b) {
     g_phase_set = b;
     const bool graph_ok = e->cfg.use_graph && !e->ctx->profile && !e->phase_timing;
+    if (e->graph_exec && e->graph_reuse != (e->reuse_jobs > 0)) {
+        // the captured iteration has (or lacks) the requeue pass: capture again, once the
+        // launches of the current graph have completed
+        WF_HIP(hipStreamSynchronize(e->ctx->stream));
+        drop_graph(e);
+    }
     if (graph_ok && !e->graph_exec && e->warm) {
+        e->graph_reuse = e->reuse_jobs > 0;
         // a direct iteration of this shape sized every workspace: capture from here on
         WF_HIP(hipStreamBeginCapture(e->ctx->stream, hipStreamCaptureModeThreadLocal));
         wf_status st = enqueue_iteration(e);
@@ -843,6 +906,7 @@ wf_status wf_engine_create(wf_context* ctx, wf_scene* scene, const wf_engine_con
     e->jobs.assign((size_t)e->max_jobs, JobDev{});
     e->job_state.assign((size_t)e->max_jobs, 0);
     e->job_iter_start.assign((size_t)e->max_jobs, 0);
+    e->job_reuse.assign((size_t)e->max_jobs, 0);
     e->free_ranges.push_back({0, e->px});
     hipError_t he = hipHostMalloc((void**)&e->status_host,
                                   sizeof(int32_t) * 2 * (kStatusHeader + (size_t)e->max_jobs));
@@ -967,6 +1031,10 @@ wf_status wf_engine_admit(wf_engine* e, const wf_job_desc* job, int32_t* job_id)
     // the host copy of J must outlive the asynchronous upload
     WF_HIP(hipStreamSynchronize(ctx->stream));
     e->job_state[slot] = queue ? 1 : 2;
+    if (queue && !job->adaptive && lanes < job->spp) {
+        e->job_reuse[slot] = 1;
+        ++e->reuse_jobs;
+    }
     e->job_iter_start[slot] = e->iterations;
     if (queue) ++e->running;
     *job_id = slot;
@@ -1005,6 +1073,10 @@ bool scan_status(wf_engine* e, int b, int32_t* finished, int32_t cap, int32_t* n
     for (int j = 0; j < e->max_jobs; ++j)
         if (e->job_state[j] == 1 && st[kStatusHeader + j] == 0) {
             e->job_state[j] = 2;
+            if (e->job_reuse[j]) {
+                e->job_reuse[j] = 0;
+                --e->reuse_jobs;
+            }
             --e->running;
             if (*n < cap) {
                 finished[(*n)++] = j;
