@@ -383,9 +383,11 @@ typedef struct wf_render_stats {  /* the reference's stats blocks (cpp:321-370) 
     int32_t active_pixels_end; /* "Pixels still active at end" */
     double frame_ms;           /* device time raygen .. final resolve */
     double sort_ms;            /* ray sort + materialize */
-    double intersect_ms;       /* traversal + hit/miss split */
+    double intersect_ms;       /* traversal + hit/miss split (regenerating schedule: with the
+                                * previous bounce's shadow rays, traced in the same launch) */
     double schedule_ms;        /* shading-queue policy sort + materialize */
-    double shade_ms;           /* shade + shadow resolve */
+    double shade_ms;           /* shade + shadow resolve (regenerating schedule: shade only,
+                                * its shadow rays count under intersect_ms) */
     double miss_ms;
     /* with coherence_stats (else 0): "Average material run length", "Average texture run
      * length", "Material ID cache line homogeneity", "Texture  ID cache line homogeneity" */
@@ -473,7 +475,8 @@ typedef struct wf_engine_stats {
     int32_t running_jobs;
     int32_t free_pixels;
     int32_t reserved;
-    double phase_ms[5];        /* sort, intersect, schedule, shade, miss (with phase timing) */
+    double phase_ms[5];        /* sort, intersect, schedule, shade, miss (with phase timing;
+                                * intersect holds the previous bounce's shadow rays) */
 } wf_engine_stats;
 
 /* Device bytes an engine of this capacity, pixel budget and sample lanes (0 = 64) needs
