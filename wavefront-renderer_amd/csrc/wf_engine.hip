@@ -496,7 +496,7 @@ struct wf_engine {
     std::vector<void*> allocations;
     size_t bytes = 0;
     // device state
-    RayPacked r0{}, r1{};               // live and sorted rays (packed)
+    RayPacked r0{}, r1{};               // live rays (packed), sorted origins and directions
     uint64_t* keys = nullptr;           // sort word of each live ray (written by its producer)
     uint64_t* keys2 = nullptr;          // radix sort scratch
     // coherence metrics of every iteration's shading queue (set by wf_render; 0 = off)
@@ -504,7 +504,7 @@ struct wf_engine {
     double* coherence = nullptr;        // 9 doubles, see coherence_accumulate
     wf_ray_soa miss{};
     wf_shade_soa s0{}, s1{};
-    int32_t *t0 = nullptr, *t1 = nullptr, *st0 = nullptr, *st1 = nullptr, *mt = nullptr;
+    int32_t *t0 = nullptr, *st0 = nullptr, *st1 = nullptr, *mt = nullptr;
     // the last shaded bounce: its shadow walks run in the next iteration's closest-hit launch,
     // after which its paths that ended at a hit are finished (pending_tags: their job tags)
     DeferredShadow pending;
@@ -569,13 +569,14 @@ wf_status carve_rays(wf_engine* e, size_t cap, wf_ray_soa* r) {
     return WF_OK;
 }
 
-wf_status carve_packed(wf_engine* e, size_t cap, RayPacked* r) {
+// columns = 3: a whole ray per slot; 2: origin and direction only (c = NULL)
+wf_status carve_packed(wf_engine* e, size_t cap, RayPacked* r, int columns = 3) {
     void* p = nullptr;
-    WF_TRY(engine_alloc(e, cap * 48, &p));
+    WF_TRY(engine_alloc(e, cap * 16 * (size_t)columns, &p));
     float4* f = (float4*)p;
     r->a = f;
     r->b = f + cap;
-    r->c = f + 2 * cap;
+    r->c = columns > 2 ? f + 2 * cap : nullptr;
     return WF_OK;
 }
 
@@ -603,7 +604,8 @@ wf_status engine_array(wf_engine* e, size_t count, T** out) {
 
 // Bytes of device memory per ray slot, per framebuffer pixel and per further sample lane of a
 // pixel (admission accounting).
-constexpr size_t kBytesPerSlot = 48 * 3 + 104 * 2 + 4 * 5 + 400;   // queues + tags + workspaces
+constexpr size_t kBytesPerSlot = 48 * 2 + 32 + 104 * 2 + 4 * 4 + 400;   // queues + tags +
+                                                                        // workspaces
 constexpr size_t kBytesPerPixel = 12 + 12 + 4 + 4 + 4 + 4 + 8;   // accum, start copy, mean, M2,
                                                                   // count, lanes done, queue
 constexpr size_t kBytesPerLane = 12 + 8;                          // accum, queue
@@ -658,13 +660,15 @@ wf_status enqueue_iteration(wf_engine* e) {
     }
     phase_mark(e, 1);
     const RayPacked* rq = &e->r0;
-    const int32_t* rt = e->t0;
+    // sorted rays: only origin and direction (what traversal reads) are gathered; the split
+    // reads the third column and the job tags through the sort order
+    RayPacked sorted_od{e->r1.a, e->r1.b, nullptr};
+    const RayPackedOrdered ordered{e->r1.a, e->r1.b, e->r0.c, e->idx};
     if (sort) {
         WF_TRY(radix_sort_packed(ctx, e->keys, e->keys2, e->idx, live, kf.idx_bits, 33,
                                  "ray_sort"));
-        WF_TRY(gather_rays(ctx, &e->r0, e->idx, live, &e->r1, e->t0, e->t1));
-        rq = &e->r1;
-        rt = e->t1;
+        WF_TRY(gather_rays(ctx, &e->r0, e->idx, live, &sorted_od, nullptr, nullptr));
+        rq = &sorted_od;
     }
     phase_mark(e, 2);
     const unsigned finish_grid = grid_for(cap, kFinishBlock * kFinishItems);
@@ -684,8 +688,12 @@ wf_status enqueue_iteration(wf_engine* e) {
         } else {
             WF_TRY(trace_closest(ctx, s, rq, live, 0.001f, 1e9f, hl, ht, hu, hv, nullptr));
         }
-        WF_TRY(split_traced(ctx, s, rq, live, hl, ht, hu, hv, &e->s0, &e->miss, c + C_HITS, rt,
-                            e->st0, e->mt));
+        if (sort)
+            WF_TRY(split_traced(ctx, s, &ordered, live, hl, ht, hu, hv, &e->s0, &e->miss,
+                                c + C_HITS, e->t0, e->st0, e->mt));
+        else
+            WF_TRY(split_traced(ctx, s, rq, live, hl, ht, hu, hv, &e->s0, &e->miss, c + C_HITS,
+                                e->t0, e->st0, e->mt));
     }
     phase_mark(e, 3);
     const wf_shade_soa* q = &e->s0;
@@ -853,7 +861,7 @@ wf_status wf_engine_create(wf_context* ctx, wf_scene* scene, const wf_engine_con
         if (s_ != WF_OK) return fail(s_); \
     } while (0)
     ETRY(carve_packed(e, cap, &e->r0));
-    ETRY(carve_packed(e, cap, &e->r1));
+    ETRY(carve_packed(e, cap, &e->r1, 2));   // sorted rays: origin and direction
     ETRY(carve_rays(e, cap, &e->miss));
     ETRY(carve_shading(e, cap, &e->s0));
     ETRY(carve_shading(e, cap, &e->s1));
@@ -866,7 +874,6 @@ wf_status wf_engine_create(wf_context* ctx, wf_scene* scene, const wf_engine_con
     }
     e->miss.ox = e->miss.oy = e->miss.oz = nullptr;
     ETRY(engine_array(e, cap, &e->t0));
-    ETRY(engine_array(e, cap, &e->t1));
     ETRY(engine_array(e, cap, &e->st0));
     ETRY(engine_array(e, cap, &e->st1));
     ETRY(engine_array(e, cap, &e->mt));

@@ -413,6 +413,14 @@ struct RayPacked {
     float4* b = nullptr;
     float4* c = nullptr;
 };
+// Packed rays in a sorted order of which only the origin and direction columns were gathered:
+// ray i is (a[i], b[i], c[order[i]]) and its job tag is read at order[i] too.
+struct RayPackedOrdered {
+    const float4* a = nullptr;
+    const float4* b = nullptr;
+    const float4* c = nullptr;
+    const int32_t* order = nullptr;
+};
 wf_status trace_closest(wf_context* ctx, const wf_scene* s, const RayPacked* rays, Count n,
                         float tnear, float tfar, int32_t* hit_leaf, float* hit_t, float* hit_bu,
                         float* hit_bv, uint32_t* flags);
@@ -426,6 +434,12 @@ wf_status trace_closest_deferred(wf_context* ctx, const wf_scene* s, const RayPa
 wf_status hit_buffers(wf_context* ctx, Count n, int32_t** leaf, float** t, float** u, float** v);
 // The hit/miss split of traced packed rays (hits and misses appended at device cursors).
 wf_status split_traced(wf_context* ctx, const wf_scene* s, const RayPacked* rays, Count n,
+                       const int32_t* hit_leaf, const float* hit_t, const float* hit_bu,
+                       const float* hit_bv, wf_shade_soa* hits_out, wf_ray_soa* misses_out,
+                       int32_t* counts_dev, const int32_t* tags_in, int32_t* hit_tags,
+                       int32_t* miss_tags);
+// The same over rays in a sorted order (tags_in in the unsorted order, read through it).
+wf_status split_traced(wf_context* ctx, const wf_scene* s, const RayPackedOrdered* rays, Count n,
                        const int32_t* hit_leaf, const float* hit_t, const float* hit_bu,
                        const float* hit_bv, wf_shade_soa* hits_out, wf_ray_soa* misses_out,
                        int32_t* counts_dev, const int32_t* tags_in, int32_t* hit_tags,

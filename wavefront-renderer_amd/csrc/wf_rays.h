@@ -47,6 +47,23 @@ __device__ __forceinline__ RayVal ld_ray(const RayPacked& r, int i) {
     return v;
 }
 
+__device__ __forceinline__ RayVal ld_ray(const RayPackedOrdered& r, int i) {
+    const float4 a = r.a[i], b = r.b[i], c = r.c[r.order[i]];
+    RayVal v;
+    v.o = V3(a.x, a.y, a.z);
+    v.pixel = __float_as_int(a.w);
+    v.d = V3(b.x, b.y, b.z);
+    v.pdf = b.w;
+    v.t = V3(c.x, c.y, c.z);
+    v.depth = __float_as_int(c.w);
+    return v;
+}
+
+// Slot of ray i's per-ray side data (job tags): i, or its source slot in an ordered view.
+__device__ __forceinline__ int ray_source(const wf_ray_soa&, int i) { return i; }
+__device__ __forceinline__ int ray_source(const RayPacked&, int i) { return i; }
+__device__ __forceinline__ int ray_source(const RayPackedOrdered& r, int i) { return r.order[i]; }
+
 // A column queue without origin columns (ox NULL) stores the rest of the ray.
 __device__ __forceinline__ void st_ray(const wf_ray_soa& r, int i, const RayVal& v) {
     if (r.ox) {

@@ -141,13 +141,13 @@ __global__ void k_gather_packed(RayPacked s, const int32_t* __restrict__ idx, in
     if (tags_out) tags_out[i] = tags_in[j];
     d.a[i] = s.a[j];
     d.b[i] = s.b[j];
-    d.c[i] = s.c[j];
+    if (d.c) d.c[i] = s.c[j];   // NULL: the consumer reads the third column through idx
 }
 
 wf_status gather_rays(wf_context* ctx, const RayPacked* src, const int32_t* idx, Count n,
                       RayPacked* dst, const int32_t* tags_in, int32_t* tags_out) {
     if (n.n <= 0) return WF_OK;
-    Timed tm(ctx, "gather_rays", tags_out ? 104.0 : 96.0, n);
+    Timed tm(ctx, "gather_rays", (dst->c ? 96.0 : 64.0) + (tags_out ? 8.0 : 0.0), n);
     k_gather_packed<<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(*src, idx, n.n, n.dev, *dst,
                                                                 tags_in, tags_out);
     WF_HIP(hipGetLastError());

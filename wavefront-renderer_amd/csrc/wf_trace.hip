@@ -175,10 +175,10 @@ __global__ __launch_bounds__(kAppendBlock) void k_split_append(
         const int i = base + j * kAppendBlock + (int)threadIdx.x;
         if (pos[j] >= 0) {
             map_hit(s, in, i, L[j], ht[i], hu[i], hv[i], out, pos[j]);
-            if (hit_tags) hit_tags[pos[j]] = tags_in[i];
+            if (hit_tags) hit_tags[pos[j]] = tags_in[ray_source(in, i)];
         } else if (m[j] >= 0) {
             st_ray(miss, m[j], ld_ray(in, i));
-            if (miss_tags) miss_tags[m[j]] = tags_in[i];
+            if (miss_tags) miss_tags[m[j]] = tags_in[ray_source(in, i)];
         }
     }
 }
@@ -309,6 +309,15 @@ static wf_status split_append(wf_context* ctx, const wf_scene* s, const R& rays,
 }
 
 wf_status split_traced(wf_context* ctx, const wf_scene* s, const RayPacked* rays, Count n,
+                       const int32_t* hit_leaf, const float* hit_t, const float* hit_bu,
+                       const float* hit_bv, wf_shade_soa* hits_out, wf_ray_soa* misses_out,
+                       int32_t* counts_dev, const int32_t* tags_in, int32_t* hit_tags,
+                       int32_t* miss_tags) {
+    return split_append(ctx, s, *rays, n, hit_leaf, hit_t, hit_bu, hit_bv, hits_out, misses_out,
+                        counts_dev, tags_in, hit_tags, miss_tags);
+}
+
+wf_status split_traced(wf_context* ctx, const wf_scene* s, const RayPackedOrdered* rays, Count n,
                        const int32_t* hit_leaf, const float* hit_t, const float* hit_bu,
                        const float* hit_bv, wf_shade_soa* hits_out, wf_ray_soa* misses_out,
                        int32_t* counts_dev, const int32_t* tags_in, int32_t* hit_tags,
