@@ -444,10 +444,6 @@ wf_status split_traced(wf_context* ctx, const wf_scene* s, const RayPackedOrdere
                        const float* hit_bv, wf_shade_soa* hits_out, wf_ray_soa* misses_out,
                        int32_t* counts_dev, const int32_t* tags_in, int32_t* hit_tags,
                        int32_t* miss_tags);
-// Unordered closest hit + split of packed rays (hits and misses appended at device cursors).
-wf_status intersect_split(wf_context* ctx, const wf_scene* s, const RayPacked* rays, Count n,
-                          wf_shade_soa* hits_out, wf_ray_soa* misses_out, int32_t* counts_dev,
-                          const int32_t* tags_in, int32_t* hit_tags, int32_t* miss_tags);
 wf_status ray_schedule(wf_context* ctx, const RayPacked* rays, Count n, const float bmin[3],
                        const float bmax[3], int32_t* sorted, uint64_t* keys_out);
 wf_status gather_rays(wf_context* ctx, const RayPacked* src, const int32_t* idx, Count n,

@@ -340,13 +340,6 @@ static wf_status split_unordered(wf_context* ctx, const wf_scene* s, const R& ra
                         tags_in, hit_tags, miss_tags);
 }
 
-wf_status intersect_split(wf_context* ctx, const wf_scene* s, const RayPacked* rays, Count n,
-                          wf_shade_soa* hits_out, wf_ray_soa* misses_out, int32_t* counts_dev,
-                          const int32_t* tags_in, int32_t* hit_tags, int32_t* miss_tags) {
-    return split_unordered(ctx, s, *rays, n, hits_out, misses_out, counts_dev, tags_in, hit_tags,
-                           miss_tags);
-}
-
 // Closest hit + stable split; counts_dev[0] = hits, counts_dev[1] = misses (device int32).
 wf_status intersect_split(wf_context* ctx, const wf_scene* s, const wf_ray_soa* rays, Count n,
                           wf_shade_soa* hits_out, wf_ray_soa* misses_out, int32_t* counts_dev,
