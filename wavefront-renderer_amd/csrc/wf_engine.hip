@@ -496,7 +496,7 @@ struct wf_engine {
     std::vector<void*> allocations;
     size_t bytes = 0;
     // device state
-    RayPacked r0{}, r1{};               // live rays (packed), sorted origins and directions
+    RayPacked r0{};                     // live rays (packed; visited in sorted order in place)
     uint64_t* keys = nullptr;           // sort word of each live ray (written by its producer)
     uint64_t* keys2 = nullptr;          // radix sort scratch
     // coherence metrics of every iteration's shading queue (set by wf_render; 0 = off)
@@ -569,14 +569,13 @@ wf_status carve_rays(wf_engine* e, size_t cap, wf_ray_soa* r) {
     return WF_OK;
 }
 
-// columns = 3: a whole ray per slot; 2: origin and direction only (c = NULL)
-wf_status carve_packed(wf_engine* e, size_t cap, RayPacked* r, int columns = 3) {
+wf_status carve_packed(wf_engine* e, size_t cap, RayPacked* r) {
     void* p = nullptr;
-    WF_TRY(engine_alloc(e, cap * 16 * (size_t)columns, &p));
+    WF_TRY(engine_alloc(e, cap * 48, &p));
     float4* f = (float4*)p;
     r->a = f;
     r->b = f + cap;
-    r->c = columns > 2 ? f + 2 * cap : nullptr;
+    r->c = f + 2 * cap;
     return WF_OK;
 }
 
@@ -604,8 +603,7 @@ wf_status engine_array(wf_engine* e, size_t count, T** out) {
 
 // Bytes of device memory per ray slot, per framebuffer pixel and per further sample lane of a
 // pixel (admission accounting).
-constexpr size_t kBytesPerSlot = 48 * 2 + 32 + 104 * 2 + 4 * 4 + 400;   // queues + tags +
-                                                                        // workspaces
+constexpr size_t kBytesPerSlot = 48 * 2 + 104 * 2 + 4 * 4 + 400;   // queues + tags + workspaces
 constexpr size_t kBytesPerPixel = 12 + 12 + 4 + 4 + 4 + 4 + 8;   // accum, start copy, mean, M2,
                                                                   // count, lanes done, queue
 constexpr size_t kBytesPerLane = 12 + 8;                          // accum, queue
@@ -662,14 +660,12 @@ wf_status enqueue_iteration(wf_engine* e) {
     const RayPacked* rq = &e->r0;
     // sorted rays: only origin and direction (what traversal reads) are gathered; the split
     // reads the third column and the job tags through the sort order
-    RayPacked sorted_od{e->r1.a, e->r1.b, nullptr};
-    const RayPackedOrdered ordered{e->r1.a, e->r1.b, e->r0.c, e->idx};
-    if (sort) {
+    // sorted rays are visited in place through the sort order (traversal and split read each
+    // ray once, so moving them first would only add a pass)
+    const RayPackedOrdered ordered{e->r0.a, e->r0.b, e->r0.c, e->idx};
+    if (sort)
         WF_TRY(radix_sort_packed(ctx, e->keys, e->keys2, e->idx, live, kf.idx_bits, 33,
                                  "ray_sort"));
-        WF_TRY(gather_rays(ctx, &e->r0, e->idx, live, &sorted_od, nullptr, nullptr));
-        rq = &sorted_od;
-    }
     phase_mark(e, 2);
     const unsigned finish_grid = grid_for(cap, kFinishBlock * kFinishItems);
     {
@@ -678,13 +674,18 @@ wf_status enqueue_iteration(wf_engine* e) {
         WF_TRY(hit_buffers(ctx, live, &hl, &ht, &hu, &hv));
         const DeferredShadow& d = e->pending;
         if (d.armed) {
-            WF_TRY(trace_closest_deferred(ctx, s, rq, live, hl, ht, hu, hv, d));
+            if (sort)
+                WF_TRY(trace_closest_deferred(ctx, s, &ordered, live, hl, ht, hu, hv, d));
+            else
+                WF_TRY(trace_closest_deferred(ctx, s, rq, live, hl, ht, hu, hv, d));
             // the rest of the previous bounce's paths that ended at a hit, now that their shadow
             // rays are in (its queue, tags and hit count are still those of that bounce)
             Timed tm(ctx, "finish", 12.0, hits);
             k_finish_hits<<<finish_grid, kFinishBlock, 0, ctx->stream>>>(
                 v, d.q.pixel, e->pending_tags, d.o.alive, kFinishRest);
             WF_HIP(hipGetLastError());
+        } else if (sort) {
+            WF_TRY(trace_closest(ctx, s, &ordered, live, 0.001f, 1e9f, hl, ht, hu, hv, nullptr));
         } else {
             WF_TRY(trace_closest(ctx, s, rq, live, 0.001f, 1e9f, hl, ht, hu, hv, nullptr));
         }
@@ -861,7 +862,6 @@ wf_status wf_engine_create(wf_context* ctx, wf_scene* scene, const wf_engine_con
         if (s_ != WF_OK) return fail(s_); \
     } while (0)
     ETRY(carve_packed(e, cap, &e->r0));
-    ETRY(carve_packed(e, cap, &e->r1, 2));   // sorted rays: origin and direction
     ETRY(carve_rays(e, cap, &e->miss));
     ETRY(carve_shading(e, cap, &e->s0));
     ETRY(carve_shading(e, cap, &e->s1));

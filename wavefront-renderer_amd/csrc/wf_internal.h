@@ -413,8 +413,8 @@ struct RayPacked {
     float4* b = nullptr;
     float4* c = nullptr;
 };
-// Packed rays in a sorted order of which only the origin and direction columns were gathered:
-// ray i is (a[i], b[i], c[order[i]]) and its job tag is read at order[i] too.
+// Packed rays visited in a sorted order without moving them: ray i is (a, b, c)[order[i]], and
+// its job tag is read at order[i] too.
 struct RayPackedOrdered {
     const float4* a = nullptr;
     const float4* b = nullptr;
@@ -430,6 +430,12 @@ struct DeferredShadow;
 wf_status trace_closest_deferred(wf_context* ctx, const wf_scene* s, const RayPacked* rays,
                                  Count n, int32_t* hit_leaf, float* hit_t, float* hit_bu,
                                  float* hit_bv, const DeferredShadow& d);
+wf_status trace_closest_deferred(wf_context* ctx, const wf_scene* s, const RayPackedOrdered* rays,
+                                 Count n, int32_t* hit_leaf, float* hit_t, float* hit_bu,
+                                 float* hit_bv, const DeferredShadow& d);
+wf_status trace_closest(wf_context* ctx, const wf_scene* s, const RayPackedOrdered* rays, Count n,
+                        float tnear, float tfar, int32_t* hit_leaf, float* hit_t, float* hit_bu,
+                        float* hit_bv, uint32_t* flags);
 // The context's closest-hit buffers for n rays.
 wf_status hit_buffers(wf_context* ctx, Count n, int32_t** leaf, float** t, float** u, float** v);
 // The hit/miss split of traced packed rays (hits and misses appended at device cursors).
@@ -446,8 +452,6 @@ wf_status split_traced(wf_context* ctx, const wf_scene* s, const RayPackedOrdere
                        int32_t* miss_tags);
 wf_status ray_schedule(wf_context* ctx, const RayPacked* rays, Count n, const float bmin[3],
                        const float bmax[3], int32_t* sorted, uint64_t* keys_out);
-wf_status gather_rays(wf_context* ctx, const RayPacked* src, const int32_t* idx, Count n,
-                      RayPacked* dst, const int32_t* tags_in, int32_t* tags_out);
 
 // Per-entry outputs of k_shade consumed by the shadow and compaction kernels.
 struct ShadeOut {

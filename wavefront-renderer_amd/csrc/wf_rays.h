@@ -47,8 +47,15 @@ __device__ __forceinline__ RayVal ld_ray(const RayPacked& r, int i) {
     return v;
 }
 
+__device__ __forceinline__ void ld_od(const RayPackedOrdered& r, int i, V3& o, V3& d) {
+    const int j = r.order[i];
+    const float4 a = r.a[j], b = r.b[j];
+    o = V3(a.x, a.y, a.z);
+    d = V3(b.x, b.y, b.z);
+}
 __device__ __forceinline__ RayVal ld_ray(const RayPackedOrdered& r, int i) {
-    const float4 a = r.a[i], b = r.b[i], c = r.c[r.order[i]];
+    const int j = r.order[i];
+    const float4 a = r.a[j], b = r.b[j], c = r.c[j];
     RayVal v;
     v.o = V3(a.x, a.y, a.z);
     v.pixel = __float_as_int(a.w);

@@ -131,28 +131,7 @@ wf_status gather_rays(wf_context* ctx, const wf_ray_soa* src, const int32_t* idx
     return WF_OK;
 }
 
-__global__ void k_gather_packed(RayPacked s, const int32_t* __restrict__ idx, int n_host,
-                                const int32_t* __restrict__ nd, RayPacked d,
-                                const int32_t* __restrict__ tags_in,
-                                int32_t* __restrict__ tags_out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= load_count(n_host, nd)) return;
-    const int j = idx[i];
-    if (tags_out) tags_out[i] = tags_in[j];
-    d.a[i] = s.a[j];
-    d.b[i] = s.b[j];
-    if (d.c) d.c[i] = s.c[j];   // NULL: the consumer reads the third column through idx
-}
 
-wf_status gather_rays(wf_context* ctx, const RayPacked* src, const int32_t* idx, Count n,
-                      RayPacked* dst, const int32_t* tags_in, int32_t* tags_out) {
-    if (n.n <= 0) return WF_OK;
-    Timed tm(ctx, "gather_rays", (dst->c ? 96.0 : 64.0) + (tags_out ? 8.0 : 0.0), n);
-    k_gather_packed<<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(*src, idx, n.n, n.dev, *dst,
-                                                                tags_in, tags_out);
-    WF_HIP(hipGetLastError());
-    return WF_OK;
-}
 
 wf_status gather_shading(wf_context* ctx, const wf_shade_soa* src, const int32_t* idx, Count n,
                          wf_shade_soa* dst, const int32_t* tags_in, int32_t* tags_out) {

@@ -780,9 +780,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(const SceneDev s, const 
 // one persistent launch: a wave that finds the ray queue drained moves on to shadow entries
 // while the slowest rays of other waves finish. The two queues are independent (the shadow
 // entries only add to the accumulators of paths shaded the bounce before).
-template <class NODE, int STACK>
+template <class NODE, int STACK, class R>
 __global__ __launch_bounds__(kTraceBlock) void k_closest_shadow(
-    const SceneDev s, const NODE* __restrict__ nodes, RayPacked rays, int n_host,
+    const SceneDev s, const NODE* __restrict__ nodes, R rays, int n_host,
     const int32_t* __restrict__ nd, int32_t* __restrict__ out_leaf, float* __restrict__ out_t,
     float* __restrict__ out_u, float* __restrict__ out_v, int32_t* __restrict__ ray_counter,
     int refill_lanes, wf_shade_soa q, ShadeOut o, const int32_t* __restrict__ list,
@@ -842,9 +842,10 @@ static int shadow_refill(const wf_context* ctx) {
                                                : ctx->tuning.refill_lanes;
 }
 
-wf_status trace_closest_deferred(wf_context* ctx, const wf_scene* s, const RayPacked* rays,
-                                 Count n, int32_t* hit_leaf, float* hit_t, float* hit_bu,
-                                 float* hit_bv, const DeferredShadow& d) {
+template <class R>
+static wf_status closest_deferred(wf_context* ctx, const wf_scene* s, const R* rays, Count n,
+                                  int32_t* hit_leaf, float* hit_t, float* hit_bu, float* hit_bv,
+                                  const DeferredShadow& d) {
     WF_TRY(scene_ready(s));
     if (n.n <= 0) return WF_OK;
     void *prc, *psc;
@@ -856,7 +857,7 @@ wf_status trace_closest_deferred(wf_context* ctx, const wf_scene* s, const RayPa
     Timed tm(ctx, "traverse_closest_shadow", 44.0, n, 41.0, d.n);
     with_bvh(s->dev, [&](auto nodes, auto S) {
         k_closest_shadow<std::remove_cv_t<std::remove_pointer_t<decltype(nodes)>>,
-                         decltype(S)::value>
+                         decltype(S)::value, R>
             <<<persistent_grid(n.n, ctx), kTraceBlock, 0, ctx->stream>>>(
                 s->dev, nodes, *rays, n.n, n.dev, hit_leaf, hit_t, hit_bu, hit_bv,
                 (int32_t*)prc, ctx->tuning.refill_lanes, d.q, d.o, d.list, d.n.n, d.n.dev,
@@ -864,6 +865,18 @@ wf_status trace_closest_deferred(wf_context* ctx, const wf_scene* s, const RayPa
     });
     WF_HIP(hipGetLastError());
     return WF_OK;
+}
+
+wf_status trace_closest_deferred(wf_context* ctx, const wf_scene* s, const RayPacked* rays,
+                                 Count n, int32_t* hit_leaf, float* hit_t, float* hit_bu,
+                                 float* hit_bv, const DeferredShadow& d) {
+    return closest_deferred(ctx, s, rays, n, hit_leaf, hit_t, hit_bu, hit_bv, d);
+}
+
+wf_status trace_closest_deferred(wf_context* ctx, const wf_scene* s, const RayPackedOrdered* rays,
+                                 Count n, int32_t* hit_leaf, float* hit_t, float* hit_bu,
+                                 float* hit_bv, const DeferredShadow& d) {
+    return closest_deferred(ctx, s, rays, n, hit_leaf, hit_t, hit_bu, hit_bv, d);
 }
 
 wf_status shade_bounce(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q, Count n,

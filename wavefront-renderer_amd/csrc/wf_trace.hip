@@ -258,6 +258,12 @@ wf_status trace_closest(wf_context* ctx, const wf_scene* s, const RayPacked* r, 
     return trace_closest_any(ctx, s, *r, n, tnear, tfar, hit_leaf, hit_t, hit_bu, hit_bv, flags);
 }
 
+wf_status trace_closest(wf_context* ctx, const wf_scene* s, const RayPackedOrdered* r, Count n,
+                        float tnear, float tfar, int32_t* hit_leaf, float* hit_t, float* hit_bu,
+                        float* hit_bv, uint32_t* flags) {
+    return trace_closest_any(ctx, s, *r, n, tnear, tfar, hit_leaf, hit_t, hit_bu, hit_bv, flags);
+}
+
 wf_status trace_occluded(const wf_scene* s, const float* ox, const float* oy, const float* oz,
                          const float* dx, const float* dy, const float* dz, const float* maxd,
                          int32_t n, uint8_t* out) {
