@@ -18,6 +18,7 @@
 #include <thread>
 
 #include "orc_core.h"
+#include "wf_fmath.h"
 
 namespace orc {
 void hit_record(const Scene& s, const V3& o, const V3& d, float t, int32_t g, float bu, float bv,
@@ -33,6 +34,24 @@ using namespace orc;
 namespace {
 
 const float PI = 3.14159265f;  // Material.cpp:6 and WavefrontRenderer.h:22
+
+// Transcendentals. The restatement calls the single-precision sin/cos/asin/acos/atan2/x^5 that
+// the device kernels call (wf_fmath.h), so a path's directions and weights are the same bits on
+// both sides. While a replay entry point pins a sampling map against the reference's compiled
+// Material.cpp, it calls the C library's functions instead, as the reference does
+// (std::cos/std::sin/std::pow/std::acos, Material.cpp:26-37, 40-53, 268-297).
+thread_local bool t_libm = false;
+struct ReferenceLibm {
+    bool saved;
+    ReferenceLibm() : saved(t_libm) { t_libm = true; }
+    ~ReferenceLibm() { t_libm = saved; }
+};
+float m_cos(float x) { return t_libm ? std::cos(x) : wf::fm::cos(x); }
+float m_sin(float x) { return t_libm ? std::sin(x) : wf::fm::sin(x); }
+float m_pow5(float x) { return t_libm ? std::pow(x, 5.0f) : wf::fm::pow5(x); }
+float m_acos(float x) { return t_libm ? std::acos(x) : wf::fm::acos(x); }
+float m_asin(float x) { return t_libm ? std::asin(x) : wf::fm::asin(x); }
+float m_atan2(float y, float x) { return t_libm ? std::atan2(y, x) : wf::fm::atan2(y, x); }
 
 // Draw source: the counter-based stream, or (for pinning against the compiled reference) a
 // replayed sequence consumed in request order. Every call site below requests its draws in the
@@ -65,7 +84,7 @@ V3 refract(const V3& d, const V3& n, float eta) {
 float schlick(float cos_i, float eta) {
     float r0 = (1.0f - eta) / (1.0f + eta);
     r0 = r0 * r0;
-    return r0 + (1.0f - r0) * std::pow((1.0f - cos_i), 5.0f);
+    return r0 + (1.0f - r0) * m_pow5(1.0f - cos_i);
 }
 
 void frame(const V3& n, V3& t, V3& b) {
@@ -77,8 +96,8 @@ void frame(const V3& n, V3& t, V3& b) {
 V3 cosine_hemisphere(const V3& n, float u1, float u2) {
     float r = std::sqrt(u1);
     float ang = 2.0f * PI * u2;
-    float x = r * std::cos(ang);
-    float z = r * std::sin(ang);
+    float x = r * m_cos(ang);
+    float z = r * m_sin(ang);
     float y = std::sqrt(std::max(0.0f, 1.0f - r * r));
     V3 t, b;
     frame(n, t, b);
@@ -135,7 +154,7 @@ V3 emitted(const wf_material& m, const V3& ln, const V3& dir_from_light) {
     if (m.type == WF_MAT_SPOTLIGHT) {
         V3 radiance = alb * m.emission;
         float c = ln.dot(dir_from_light);
-        float deg = std::acos(std::max(-1.0f, std::min(1.0f, c))) * (180.0f / PI);
+        float deg = m_acos(std::max(-1.0f, std::min(1.0f, c))) * (180.0f / PI);
         if (deg >= m.spot_outer_deg) return V3(0, 0, 0);
         if (deg > m.spot_falloff_deg) {
             float t = (deg - m.spot_falloff_deg) / (m.spot_outer_deg - m.spot_falloff_deg);
@@ -181,7 +200,7 @@ float ggx_g1(const V3& w, float a) {
     return 2.0f * w.z / (w.z + std::sqrt(s2 + w.z * w.z));
 }
 V3 schlick_rgb(const V3& f0, float c) {
-    float k = std::pow(1.0f - std::max(0.0f, std::min(1.0f, c)), 5.0f);
+    float k = m_pow5(1.0f - std::max(0.0f, std::min(1.0f, c)));
     return V3(f0.x + (1.0f - f0.x) * k, f0.y + (1.0f - f0.y) * k, f0.z + (1.0f - f0.z) * k);
 }
 V3 to_local(const V3& w, const V3& t, const V3& b, const V3& n) {
@@ -220,8 +239,8 @@ bool ggx_sample(const V3& f0, float a, const V3& n, const V3& wo, float u1, floa
     V3 t2 = vh.cross(t1);
     float r = std::sqrt(u1);
     float phi = 2.0f * PI * u2;
-    float p1 = r * std::cos(phi);
-    float p2 = r * std::sin(phi);
+    float p1 = r * m_cos(phi);
+    float p2 = r * m_sin(phi);
     float sh = 0.5f * (1.0f + vh.z);
     p2 = (1.0f - sh) * std::sqrt(std::max(0.0f, 1.0f - p1 * p1)) + sh * p2;
     V3 nh = t1 * p1 + t2 * p2 + vh * std::sqrt(std::max(0.0f, 1.0f - p1 * p1 - p2 * p2));
@@ -341,8 +360,8 @@ bool scatter(const Scene& s, const wf_material& m, const V3& d, const V3& n, con
 // getSkyColor, cpp:879-889, with EnvironmentMap::sample, EnvironmentMap.cpp:30-48.
 int env_texel(const Scene& s, const V3& dir, float& lat_cos) {
     V3 d = dir.normalized();
-    float u = 0.5f + std::atan2(d.z, d.x) / (2.0f * static_cast<float>(M_PI));
-    float v = 0.5f - std::asin(std::max(-1.0f, std::min(1.0f, d.y))) / static_cast<float>(M_PI);
+    float u = 0.5f + m_atan2(d.z, d.x) / (2.0f * static_cast<float>(M_PI));
+    float v = 0.5f - m_asin(std::max(-1.0f, std::min(1.0f, d.y))) / static_cast<float>(M_PI);
     int x = std::min(s.env_w - 1, std::max(0, static_cast<int>(u * s.env_w)));
     int y = std::min(s.env_h - 1, std::max(0, static_cast<int>(v * s.env_h)));
     lat_cos = std::sqrt(std::max(0.0f, 1.0f - d.y * d.y));
@@ -465,8 +484,8 @@ bool shade_one(const Scene& s, const wf_shade_params& sp, const double* mat_cost
             float ev = ((float)ty + rng(DIM_ENV_JY)) / (float)s.env_h;
             float phi = (eu - 0.5f) * (2.0f * PI);
             float lat = (0.5f - ev) * PI;
-            float cl = std::cos(lat);
-            V3 wi(cl * std::cos(phi), std::sin(lat), cl * std::sin(phi));
+            float cl = m_cos(lat);
+            V3 wi(cl * m_cos(phi), m_sin(lat), cl * m_sin(phi));
             float cos_s = N.dot(wi);
             if (cos_s > 0.0f) {
                 float lc;
@@ -645,6 +664,7 @@ int32_t orc_scatter_replay(void* sp, const wf_material* m, const float* d, const
                            int32_t n_draws, float* att, float* dir, float* pdf,
                            int32_t* consumed) {
     const Scene& s = *(Scene*)sp;
+    ReferenceLibm libm;
     Rng rng{0, 0, 0, 0, 0, draws, n_draws, 0};
     V3 a, o;
     float pd = 0.0f;
@@ -721,6 +741,7 @@ int32_t orc_env_tables(void* sp, float* prob, int32_t* alias, float* pdf, float*
 
 // Material::emitted (Material.cpp:268-297).
 void orc_emitted(const wf_material* m, const float* ln, const float* dfl, float* out) {
+    ReferenceLibm libm;
     V3 e = emitted(*m, V3(ln[0], ln[1], ln[2]), V3(dfl[0], dfl[1], dfl[2]));
     out[0] = e.x; out[1] = e.y; out[2] = e.z;
 }
@@ -733,6 +754,18 @@ void orc_surface_color(void* sp, const wf_material* m, const float* p, float u, 
 }
 
 uint32_t orc_hash_grid_cell(int32_t x, int32_t y, int32_t z) { return hash_grid_cell(x, y, z); }
+
+// The shared single-precision functions, exported for their accuracy tests.
+void orc_fm_sincos(int32_t n, const float* x, float* s, float* c) {
+    for (int32_t i = 0; i < n; ++i) wf::fm::sincos(x[i], s[i], c[i]);
+}
+void orc_fm_inverse(int32_t n, const float* x, const float* y, float* as, float* ac, float* at) {
+    for (int32_t i = 0; i < n; ++i) {
+        as[i] = wf::fm::asin(x[i]);
+        ac[i] = wf::fm::acos(x[i]);
+        at[i] = wf::fm::atan2(y[i], x[i]);
+    }
+}
 
 float orc_uniform(uint32_t seed, uint32_t job, uint32_t pixel, uint32_t sample, uint32_t depth,
                   uint32_t dim) {

@@ -72,6 +72,8 @@ class Oracle:
         L.orc_run_length.argtypes = [C.c_int32, P, P]
         L.orc_line_homogeneity.restype = C.c_double
         L.orc_line_homogeneity.argtypes = [C.c_int32, P]
+        L.orc_fm_sincos.argtypes = [C.c_int32, P, P, P]
+        L.orc_fm_inverse.argtypes = [C.c_int32, P, P, P, P, P]
 
     # -- RNG
     def philox(self, ctr, key):
@@ -80,6 +82,21 @@ class Oracle:
         out = np.zeros(4, np.uint32)
         self.lib.orc_philox(fptr(c), fptr(k), fptr(out))
         return out
+
+    def fm_sincos(self, x):
+        """The shared single-precision sin/cos (wf_fmath.h) over an array."""
+        x = np.ascontiguousarray(x, np.float32)
+        s, c = np.zeros_like(x), np.zeros_like(x)
+        self.lib.orc_fm_sincos(len(x), fptr(x), fptr(s), fptr(c))
+        return s, c
+
+    def fm_inverse(self, x, y):
+        """The shared asin(x), acos(x) and atan2(y, x) (wf_fmath.h) over arrays."""
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        a, b, t = np.zeros_like(x), np.zeros_like(x), np.zeros_like(x)
+        self.lib.orc_fm_inverse(len(x), fptr(x), fptr(y), fptr(a), fptr(b), fptr(t))
+        return a, b, t
 
     def uniform(self, seed, job, pixel, sample, depth, dim):
         return self.lib.orc_uniform(seed, job, pixel, sample, depth, dim)

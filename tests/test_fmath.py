@@ -1,0 +1,87 @@
+"""The single-precision sin/cos, asin/acos, atan2 and x^5 that the shading kernels and the oracle
+share (wavefront-renderer_amd/csrc/wf_fmath.h), against float64 evaluations of the functions.
+
+Errors are counted in units in the last place of the float nearest the exact value. The bounds
+are what the method gives (Cody-Waite reduction + near-minimax polynomials, one rounding per
+fma): on the angles the samplers pass (|x| <= 2 pi) sin and cos stay within 2 ulp; further out
+the reduction keeps them within 2 ulp up to |x| = 200, near the zeros too.
+A wrong coefficient, quadrant or branch constant shows up as thousands of ulp."""
+import numpy as np
+import pytest
+
+f32 = np.float32
+
+
+def ulp(exact):
+    """Spacing of float32 at the float nearest each exact value (the subnormal step at 0)."""
+    a = np.abs(exact).astype(f32)
+    return np.spacing(a).astype(np.float64)
+
+
+def err_ulp(got, exact):
+    return np.abs(got.astype(np.float64) - exact) / ulp(exact)
+
+
+@pytest.fixture(scope="module")
+def rng():
+    return np.random.default_rng(2024)
+
+
+def test_sincos_within_two_ulp_on_sampler_angles(orc, rng):
+    x = np.concatenate([rng.uniform(0, 2 * np.pi, 400_000), rng.uniform(-np.pi, np.pi, 400_000),
+                        np.linspace(-2 * np.pi, 2 * np.pi, 200_001)]).astype(f32)
+    s, c = orc.fm_sincos(x)
+    xd = x.astype(np.float64)
+    assert err_ulp(s, np.sin(xd)).max() <= 2.0
+    assert err_ulp(c, np.cos(xd)).max() <= 2.0
+
+
+def test_sincos_quadrants_and_far_arguments(orc, rng):
+    x = rng.uniform(-200, 200, 400_000).astype(f32)
+    s, c = orc.fm_sincos(x)
+    xd = x.astype(np.float64)
+    assert err_ulp(s, np.sin(xd)).max() <= 2.0
+    assert err_ulp(c, np.cos(xd)).max() <= 2.0
+    # arguments next to multiples of pi/2, where the result is tiny
+    k = np.arange(-100, 101)
+    near = np.concatenate([np.nextafter((k * np.pi / 2).astype(f32), f32(np.inf)),
+                           (k * np.pi / 2).astype(f32),
+                           np.nextafter((k * np.pi / 2).astype(f32), f32(-np.inf))])
+    s, c = orc.fm_sincos(near)
+    nd = near.astype(np.float64)
+    assert err_ulp(s, np.sin(nd)).max() <= 2.0
+    assert err_ulp(c, np.cos(nd)).max() <= 2.0
+    # exact values at the quadrant boundaries and at zero
+    s, c = orc.fm_sincos(np.array([0.0, -0.0], f32))
+    assert s.tolist() == [0.0, 0.0] and c.tolist() == [1.0, 1.0]   # sin(-0) is +0 here
+    tiny = np.ldexp(f32(1), -np.arange(20, 120)).astype(f32)
+    s, c = orc.fm_sincos(tiny)
+    assert np.array_equal(s, tiny) and (c == 1.0).all()
+
+
+def test_asin_acos_within_two_ulp(orc, rng):
+    x = np.concatenate([rng.uniform(-1, 1, 500_000), np.linspace(-1, 1, 200_001),
+                        1 - rng.uniform(0, 1e-3, 50_000), -1 + rng.uniform(0, 1e-3, 50_000),
+                        np.ldexp(1.0, -rng.integers(1, 60, 20_000))]).astype(f32)
+    x = np.clip(x, -1, 1)
+    a, b, _ = orc.fm_inverse(x, np.zeros_like(x))
+    xd = x.astype(np.float64)
+    assert err_ulp(a, np.arcsin(xd)).max() <= 2.0
+    assert err_ulp(b, np.arccos(xd)).max() <= 2.0
+    a, b, _ = orc.fm_inverse(np.array([1, -1, 0.5, -0.5], f32), np.zeros(4, f32))
+    assert a[0] == f32(np.pi / 2) and a[1] == -f32(np.pi / 2)
+    assert b[0] == 0.0 and b[1] == f32(np.pi)
+    assert a[2] == -a[3] and abs(float(b[2]) - np.pi / 3) <= np.spacing(f32(1.0))
+
+
+def test_atan2_within_three_ulp_and_signed_zeros(orc, rng):
+    y = np.concatenate([rng.uniform(-1, 1, 500_000), rng.normal(size=100_000) * 1e-5])
+    x = np.concatenate([rng.uniform(-1, 1, 500_000), rng.normal(size=100_000)])
+    y, x = y.astype(f32), x.astype(f32)
+    _, _, t = orc.fm_inverse(x, y)
+    assert err_ulp(t, np.arctan2(y.astype(np.float64), x.astype(np.float64))).max() <= 3.0
+    z = np.array([0.0, -0.0, 0.0, -0.0, 1.0, -1.0, 0.0, 0.0], f32)
+    w = np.array([0.0, 0.0, -0.0, -0.0, 0.0, 0.0, 1.0, -1.0], f32)     # atan2(z, w)
+    _, _, t = orc.fm_inverse(w, z)
+    want = np.arctan2(z, w).astype(f32)
+    assert np.array_equal(t, want) and np.array_equal(np.signbit(t), np.signbit(want))

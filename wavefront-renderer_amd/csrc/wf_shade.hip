@@ -8,6 +8,7 @@
 // atomics, and the per-pixel order is the reference's (emission, area NEE, directional NEE,
 // miss). Shadow rays are resolved by a second kernel over the compacted list of entries that
 // emitted one; surviving next rays are compacted in shading order.
+#include "wf_fmath.h"
 #include "wf_rays.h"
 #include "wf_traverse.h"
 
@@ -67,7 +68,7 @@ __device__ __forceinline__ V3 emitted(const DevMaterial& m, const V3& ln, const 
     if (m.type == WF_MAT_SPOTLIGHT) {
         V3 radiance = alb * m.emission;
         const float c = ln.dot(dfl);
-        const float deg = acosf(smax(-1.0f, smin(1.0f, c))) * (180.0f / kPi);
+        const float deg = fm::acos(smax(-1.0f, smin(1.0f, c))) * (180.0f / kPi);
         if (deg >= m.spot_outer_deg) return V3(0.0f, 0.0f, 0.0f);
         if (deg > m.spot_falloff_deg) {
             const float t = (deg - m.spot_falloff_deg) / (m.spot_outer_deg - m.spot_falloff_deg);
@@ -105,7 +106,7 @@ __device__ __forceinline__ V3 refract(const V3& d, const V3& n, float eta) {
 __device__ __forceinline__ float schlick(float ci, float eta) {
     float r0 = (1.0f - eta) / (1.0f + eta);
     r0 = r0 * r0;
-    return r0 + (1.0f - r0) * powf((1.0f - ci), 5.0f);
+    return r0 + (1.0f - r0) * fm::pow5(1.0f - ci);
 }
 __device__ __forceinline__ void frame(const V3& n, V3& t, V3& b) {
     const V3 up = fabsf(n.x) > 0.9f ? V3(0.0f, 1.0f, 0.0f) : V3(1.0f, 0.0f, 0.0f);
@@ -115,8 +116,10 @@ __device__ __forceinline__ void frame(const V3& n, V3& t, V3& b) {
 __device__ __forceinline__ V3 cosine_hemisphere(const V3& n, float u1, float u2) {
     const float r = __builtin_sqrtf(u1);
     const float ang = 2.0f * kPi * u2;
-    const float x = r * cosf(ang);
-    const float z = r * sinf(ang);
+    float sa, ca;
+    fm::sincos(ang, sa, ca);
+    const float x = r * ca;
+    const float z = r * sa;
     const float y = __builtin_sqrtf(smax(0.0f, 1.0f - r * r));
     V3 t, b;
     frame(n, t, b);
@@ -148,7 +151,7 @@ __device__ __forceinline__ float ggx_g1(const V3& w, float a) {
     return 2.0f * w.z / (w.z + __builtin_sqrtf(s2 + w.z * w.z));
 }
 __device__ __forceinline__ V3 schlick_rgb(const V3& f0, float c) {
-    const float k = powf(1.0f - smax(0.0f, smin(1.0f, c)), 5.0f);
+    const float k = fm::pow5(1.0f - smax(0.0f, smin(1.0f, c)));
     return V3(f0.x + (1.0f - f0.x) * k, f0.y + (1.0f - f0.y) * k, f0.z + (1.0f - f0.z) * k);
 }
 __device__ __forceinline__ V3 to_local(const V3& w, const V3& t, const V3& b, const V3& n) {
@@ -183,8 +186,10 @@ __device__ __forceinline__ bool ggx_sample(const V3& f0, float a, const V3& n, c
     const V3 t2 = vh.cross(t1);
     const float r = __builtin_sqrtf(u1);
     const float phi = 2.0f * kPi * u2;
-    const float p1 = r * cosf(phi);
-    float p2 = r * sinf(phi);
+    float sp, cp;
+    fm::sincos(phi, sp, cp);
+    const float p1 = r * cp;
+    float p2 = r * sp;
     const float sh = 0.5f * (1.0f + vh.z);
     p2 = (1.0f - sh) * __builtin_sqrtf(smax(0.0f, 1.0f - p1 * p1)) + sh * p2;
     const V3 nh = t1 * p1 + t2 * p2 + vh * __builtin_sqrtf(smax(0.0f, 1.0f - p1 * p1 - p2 * p2));
@@ -292,8 +297,8 @@ __device__ __forceinline__ bool scatter(const SceneDev& s, const DevMaterial& m,
 // EnvironmentMap::sample mapping (EnvironmentMap.cpp:30-48): nearest texel of the direction.
 __device__ __forceinline__ int env_texel(const SceneDev& s, const V3& dir, float& lat_cos) {
     const V3 d = dir.normalized();
-    const float u = 0.5f + atan2f(d.z, d.x) / (2.0f * 3.14159274f);
-    const float v = 0.5f - asinf(smax(-1.0f, smin(1.0f, d.y))) / 3.14159274f;
+    const float u = 0.5f + fm::atan2(d.z, d.x) / (2.0f * 3.14159274f);
+    const float v = 0.5f - fm::asin(smax(-1.0f, smin(1.0f, d.y))) / 3.14159274f;
     int x = (int)(u * s.env_w);
     int y = (int)(v * s.env_h);
     x = x < 0 ? 0 : (x > s.env_w - 1 ? s.env_w - 1 : x);
@@ -440,8 +445,10 @@ __global__ void k_shade(const SceneDev s, wf_shade_soa q, int n_host,
             const float ev = ((float)ty + rng(D_ENV_JY)) / (float)s.env_h;
             const float phi = (eu - 0.5f) * (2.0f * kPi);
             const float lat = (0.5f - ev) * kPi;
-            const float cl = cosf(lat);
-            wi = V3(cl * cosf(phi), sinf(lat), cl * sinf(phi));
+            float sl, cl, sp, cp;
+            fm::sincos(lat, sl, cl);
+            fm::sincos(phi, sp, cp);
+            wi = V3(cl * cp, sl, cl * sp);
             if (N.dot(wi) > 0.0f) {
                 float lc;
                 const int tx2 = env_texel(s, wi, lc);

@@ -7,6 +7,8 @@ every parameter the workload description leaves open in `SceneData.params`.
   sorting_stress()     1.3M-triangle noise-displaced icosphere(8), 64 materials, 32 checker textures
   sphere_cloud()       4096 displaced UV spheres (10.5M triangles), 256 materials, 1024 emitters, HDRI
   floor_and_light()    the floor + light-quad scene of the reference's renderer tests
+  material_variants()  the reference's material set (fuzz Metal, Schlick Glass, Plastic, SpotLight,
+                       spatial checker, textures) with a directional light and an environment map
 """
 from __future__ import annotations
 
@@ -444,3 +446,85 @@ def floor_and_light() -> SceneData:
     cam = {"eye": (0.0, 0.5, 2.0), "target": (0.0, 0.0, -1.0), "up": (0.0, 1.0, 0.0), "vfov": 45.0}
     return SceneData([Mesh(p0, i0, 0, name="floor"), Mesh(p1, i1, 1, name="light")], mats,
                      camera=cam, name="floor_and_light")
+
+
+def material_variants(seed: int = SEED_BASE + 5, subdiv: int = 2, env_size=(32, 16),
+                      directional: bool = True, environment: bool = True) -> SceneData:
+    """Every material of the reference's own material set on the reference model, in one small
+    scene, after scenes/materialVariants.usda and cornellBoxDragon.usda: a box [-1,1]x[0,2]x[-1,1]
+    open at the camera side (z = +1) with a UV-textured RGBA8 checker floor, a fuzz Metal left
+    wall (roughness 0.3), a Plastic back wall (roughness 0.15), a spatial-checker Diffuse right
+    wall (cell 0.3 on a 64^2 palette), a Schlick Glass icosphere (IOR 1.5; refracted paths hit
+    its faces from inside), a mirror Metal icosphere (roughness 0), a Plastic icosphere with no
+    fuzz and a textured Metal panel; a SpotLight quad on the ceiling (outer 60 deg, falloff 30
+    deg), a small Emissive quad, a directional light and an environment map seen through the
+    open side."""
+    from .usd import spatial_palette
+
+    rng = np.random.default_rng(seed)
+    floor_tex = checker_texture(rng, 64)
+    palette = spatial_palette(64, 0.4)
+    textures = [floor_tex, palette]
+    R = _abi.MODEL_REFERENCE
+    mats = [
+        {"type": _abi.MAT_DIFFUSE, "albedo": (0.8, 0.8, 0.8), "texture_id": 0, "model": R},
+        {"type": _abi.MAT_METAL, "albedo": (0.9, 0.75, 0.5), "roughness": 0.3, "model": R},
+        {"type": _abi.MAT_PLASTIC, "albedo": (0.85, 0.08, 0.08), "roughness": 0.15, "model": R},
+        {"type": _abi.MAT_DIFFUSE, "albedo": (0.5, 0.7, 0.85), "texture_id": 1, "model": R,
+         "use_spatial_checker": 1, "checker_cell_size": 0.3},
+        {"type": _abi.MAT_DIFFUSE, "albedo": (0.73, 0.73, 0.73), "model": R},
+        {"type": _abi.MAT_GLASS, "albedo": (1.0, 1.0, 1.0), "ior": 1.5, "model": R},
+        {"type": _abi.MAT_METAL, "albedo": (0.95, 0.95, 0.95), "roughness": 0.0, "model": R},
+        {"type": _abi.MAT_PLASTIC, "albedo": (0.12, 0.45, 0.15), "roughness": 0.0, "model": R},
+        {"type": _abi.MAT_METAL, "albedo": (0.8, 0.8, 0.8), "roughness": 0.05, "texture_id": 0,
+         "model": R},
+        {"type": _abi.MAT_SPOTLIGHT, "albedo": (30.0, 28.0, 24.0), "emission": 1.0,
+         "spot_outer_deg": 60.0, "spot_falloff_deg": 30.0, "model": R},
+        {"type": _abi.MAT_EMISSIVE, "albedo": (6.0, 5.5, 7.5), "emission": 1.0, "model": R},
+    ]
+    a, b, top = -1.0, 1.0, 2.0
+    meshes = []
+    # floor with face-varying UVs: two checker periods across the floor
+    p, i = quad((a, 0, a), (a, 0, b), (b, 0, b), (b, 0, a))
+    uvc = np.array([[0.0, 0.0], [0.0, 2.0], [2.0, 2.0], [2.0, 0.0]], np.float32)
+    fl = Mesh(p, i, 0, name="/Variants/Floor")
+    fl.uv_u = uvc[i, 0].astype(np.float32)
+    fl.uv_v = uvc[i, 1].astype(np.float32)
+    meshes.append(fl)
+    for name, corners, mid in [
+        ("Ceiling", [(a, top, a), (b, top, a), (b, top, b), (a, top, b)], 4),
+        ("BackWall", [(a, 0, a), (b, 0, a), (b, top, a), (a, top, a)], 2),
+        ("LeftWall", [(a, 0, a), (a, top, a), (a, top, b), (a, 0, b)], 1),
+        ("RightWall", [(b, 0, a), (b, 0, b), (b, top, b), (b, top, a)], 3),
+    ]:
+        p, i = quad(*corners)
+        meshes.append(Mesh(p, i, mid, name=f"/Variants/{name}"))
+    v, f = icosphere(subdiv)
+    for name, centre, radius, mid in [("GlassBall", (-0.45, 0.4, 0.1), 0.4, 5),
+                                      ("MirrorBall", (0.5, 0.35, -0.35), 0.35, 6),
+                                      ("PlasticBall", (0.35, 0.25, 0.45), 0.25, 7)]:
+        meshes.append(Mesh((v * radius + np.array(centre)).astype(np.float32), f, mid,
+                           name=f"/Variants/{name}"))
+    p, i = quad((-0.2, 0.05, -0.95), (0.3, 0.05, -0.95), (0.3, 0.9, -0.95), (-0.2, 0.9, -0.95))
+    panel = Mesh(p, i, 8, name="/Variants/TexturedMetalPanel")
+    panel.uv_u = uvc[i, 0].astype(np.float32)
+    panel.uv_v = uvc[i, 1].astype(np.float32)
+    meshes.append(panel)
+    # SpotLight facing down (normal (p1-p0)x(p2-p0) = -y), Emissive accent quad on the back wall
+    s, y = 0.3, top - 0.001
+    p, i = quad((-s, y, -s), (-s, y, s), (s, y, s), (s, y, -s))
+    meshes.append(Mesh(p, i, 9, name="/Variants/Spot"))
+    p, i = quad((-0.9, 1.2, -0.999), (-0.6, 1.2, -0.999), (-0.6, 1.5, -0.999),
+                (-0.9, 1.5, -0.999))
+    meshes.append(Mesh(p, i, 10, name="/Variants/Accent"))
+    dl = None
+    if directional:
+        d = np.array([-0.3, -1.0, -0.45])
+        dl = {"direction": tuple((d / np.linalg.norm(d)).tolist()), "color": (1.0, 0.95, 0.85),
+              "intensity": 1.5}
+    env = hdri_sky(*env_size, peak_over_mean=20.0) if environment else None
+    cam = {"eye": (0.0, 1.0, 3.4), "target": (0.0, 0.8, 0.0), "up": (0.0, 1.0, 0.0), "vfov": 45.0}
+    return SceneData(meshes, mats, textures, directional=dl, environment=env, camera=cam,
+                     name="material_variants",
+                     params={"seed": seed, "subdiv": subdiv, "env": list(env_size),
+                             "box": [[a, 0, a], [b, top, b]], "open_side": "+z"})
