@@ -256,6 +256,21 @@ wf_status wf_intersect_split(const wf_scene* scene, const wf_ray_soa* rays, int3
 wf_status wf_ray_schedule(wf_context* ctx, const wf_ray_soa* rays, int32_t n,
                           const float bounds_min[3], const float bounds_max[3],
                           int32_t* sorted_indices, uint64_t* keys_out);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): wf_ray_schedule with all of
+ * its scratch (sort words, the sort's ping-pong buffer, tile counters, digit totals, look-back
+ * states) in one fixed array per device of WF_NOALLOC_MAX_RAYS rays (about 36 MB, held by each
+ * device the library runs on until the process ends). It calls no hipMalloc, hipFree, pinned
+ * host allocation or event creation, and rejects more than WF_NOALLOC_MAX_RAYS rays and contexts
+ * with kernel timing on. It only enqueues work on the context's stream: the permutation (the same
+ * as wf_ray_schedule's, bit for bit) is ready, and an exhausted look-back wait is reported, at the
+ * next synchronising call (wf_synchronize). The array keeps the last call's contents; every call
+ * zero-fills what must start at zero. Calls that overlap on one device (another context's stream,
+ * another thread) share the array and race: serialise them. */
+#define WF_NOALLOC_MAX_RAYS (1 << 21)
+wf_status wf_ray_schedule_noalloc(wf_context* ctx, const wf_ray_soa* rays, int32_t n,
+                                  const float bounds_min[3], const float bounds_max[3],
+                                  int32_t* sorted_indices, uint64_t* keys_out);
 /* RayQueue::materialize (RayQueue.h:182-215), out of place: dst[i] = src[idx[i]]. */
 wf_status wf_materialize_rays(wf_context* ctx, const wf_ray_soa* src, const int32_t* idx,
                               int32_t n, wf_ray_soa* dst);

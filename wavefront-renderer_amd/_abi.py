@@ -29,6 +29,7 @@ POLICIES = {"none": POLICY_NONE, "material": POLICY_MATERIAL, "texture": POLICY_
 PHYSICS_REFERENCE, PHYSICS_UNBIASED = 0, 1
 SCHEDULE_REGENERATE, SCHEDULE_PASSES = 0, 1
 TEX_RGBA8_SRGB, TEX_RGB_F32 = 0, 1
+NOALLOC_MAX_RAYS = 1 << 21   # WF_NOALLOC_MAX_RAYS
 
 f3 = C.c_float * 3
 P = C.c_void_p
@@ -199,6 +200,8 @@ _PROTOS = {
                                 C.POINTER(C.c_int32), C.POINTER(RaySoA), C.POINTER(C.c_int32)]),
     "wf_ray_schedule": (_S, [P, C.POINTER(RaySoA), C.c_int32, C.POINTER(C.c_float),
                              C.POINTER(C.c_float), P, P]),
+    "wf_ray_schedule_noalloc": (_S, [P, C.POINTER(RaySoA), C.c_int32, C.POINTER(C.c_float),
+                                     C.POINTER(C.c_float), P, P]),
     "wf_materialize_rays": (_S, [P, C.POINTER(RaySoA), P, C.c_int32, C.POINTER(RaySoA)]),
     "wf_scene_set_costs": (_S, [P, P, P]),
     "wf_shading_schedule": (_S, [P, C.POINTER(ShadeSoA), C.c_int32, C.c_int32, P]),

@@ -192,6 +192,13 @@ wf_status wf_context_create(int32_t device, wf_context** out) {
         return set_error(WF_ERR_HIP, "wf_context_create: fault word: %s", hipGetErrorString(fe));
     }
     ctx->fault_owned = true;
+    const wf_status ss = wf::fixed_scratch_init(ctx);
+    if (ss != WF_OK) {
+        (void)hipFree(ctx->fault);
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return ss;
+    }
     *out = ctx;
     return WF_OK;
 }

@@ -83,6 +83,9 @@ struct wf_context {
     // slot of an engine's status block (copied back with every iteration's status).
     int32_t* fault = nullptr;
     bool fault_owned = false;
+    // This device's copy of the allocation-free ray schedule's scratch array (its address as
+    // hipGetSymbolAddress returns it on this device).
+    void* fixed_scratch = nullptr;
 };
 
 namespace wf {
@@ -407,6 +410,24 @@ struct RayKeyFrame {
 RayKeyFrame ray_key_frame(const float bmin[3], const float bmax[3], int32_t idx_bits);
 // Smallest b >= 1 with 2^b >= n: the index bits of packed sort words over n elements.
 int32_t index_bits(int64_t n);
+
+// Fixed buffers for a radix sort that may not allocate: the second buffer of packed words and a
+// meta block (tile counters, digit totals, look-back states) of meta_words 32-bit words.
+struct SortArena {
+    uint64_t* keys_alt = nullptr;
+    uint32_t* meta = nullptr;
+    size_t meta_words = 0;
+};
+// Stable sort of packed words (key_bits above idx_bits) by all key bits with 8-bit digits into
+// the arena: no allocation and no readback. words is overwritten (it is the other ping-pong
+// buffer); order receives the low idx_bits of each sorted word. A look-back wait that runs out
+// sets the context's fault word, which the next synchronising call reports.
+wf_status radix_sort_packed_fixed(wf_context* ctx, uint64_t* words, int32_t* order, Count n,
+                                  int32_t idx_bits, int32_t key_bits, const SortArena& arena,
+                                  const char* tag);
+// Looks up this device's copy of the fixed scratch array (at context creation; the context's
+// device is current).
+wf_status fixed_scratch_init(wf_context* ctx);
 
 struct RayPacked {
     float4* a = nullptr;

@@ -7,11 +7,47 @@
 //   costBenefit : type(3 bits) | priorityRank | hit-point Morton (ShadingQueue.h:437-510)
 // typeRank is computed on the device from per-type hit counts / texture bytes; cost and size
 // ranks are dense descending ranks of the host tables; ties keep queue order.
+#include <mutex>
+
 #include "wf_device.h"
 #include "wf_internal.h"
 #include "wf_rays.h"
 
 namespace wf {
+
+// ---- scratch of the allocation-free ray schedule (wf_ray_schedule_noalloc) ----------------------
+// Two buffers of WF_NOALLOC_MAX_RAYS packed sort words (8 bytes each: the words, then the sort's
+// other ping-pong buffer) and the sort's meta block for 8-bit digits over the 33-bit key: a tile
+// counter per pass (64 words), the digit totals (4 passes x 512 slots, 2048 words) and the
+// look-back states (512 slots x 512 tiles of 4096 words x 4 passes). The sort checks its need
+// against kFixedMetaWords and reports an error rather than overrun. Each device holds its own copy
+// until the process ends.
+constexpr size_t kFixedWords = (size_t)WF_NOALLOC_MAX_RAYS;
+constexpr size_t kFixedMetaWords = 64 + 2048 + (size_t)512 * (kFixedWords / 4096) * 4;
+constexpr size_t kFixedScratchBytes = 2 * sizeof(uint64_t) * kFixedWords +
+                                      sizeof(uint32_t) * kFixedMetaWords;
+static __device__ __align__(256) unsigned char scratch[kFixedScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+wf_status fixed_scratch_init(wf_context* ctx) {
+    if (ctx->device < 0 || ctx->device >= kMaxDevices)
+        return set_error(WF_ERR_INVALID_ARGUMENT, "device %d beyond the scratch table",
+                         ctx->device);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[ctx->device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return set_error(WF_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s",
+                             ctx->device, hipGetErrorString(e));
+        g_scratch_ptr[ctx->device] = ptr;
+    }
+    ctx->fixed_scratch = g_scratch_ptr[ctx->device];
+    return WF_OK;
+}
 
 RayKeyFrame ray_key_frame(const float bmin[3], const float bmax[3], int32_t idx_bits) {
     RayKeyFrame f;
@@ -527,6 +563,36 @@ wf_status wf_ray_schedule(wf_context* ctx, const wf_ray_soa* rays, int32_t n,
     WF_TRY(ray_schedule(ctx, rays, host_count(n), bounds_min, bounds_max, sorted_indices,
                         keys_out));
     return check_fault(ctx);
+}
+
+wf_status wf_ray_schedule_noalloc(wf_context* ctx, const wf_ray_soa* rays, int32_t n,
+                                  const float bounds_min[3], const float bounds_max[3],
+                                  int32_t* sorted_indices, uint64_t* keys_out) {
+    clear_error();
+    WF_REQUIRE(ctx && rays && bounds_min && bounds_max && sorted_indices,
+               "wf_ray_schedule_noalloc: NULL argument");
+    WF_REQUIRE(n >= 0, "wf_ray_schedule_noalloc: negative ray count");
+    WF_REQUIRE(n <= WF_NOALLOC_MAX_RAYS,
+               "wf_ray_schedule_noalloc: %d rays exceed the fixed scratch of %d rays", n,
+               WF_NOALLOC_MAX_RAYS);
+    WF_REQUIRE(!ctx->profile, "wf_ray_schedule_noalloc: kernel timing is on (it records events)");
+    WF_REQUIRE(ctx->fixed_scratch, "wf_ray_schedule_noalloc: no scratch on device %d",
+               ctx->device);
+    if (n == 0) return WF_OK;
+    unsigned char* base = (unsigned char*)ctx->fixed_scratch;
+    uint64_t* words = (uint64_t*)base;
+    SortArena arena;
+    arena.keys_alt = (uint64_t*)(base + sizeof(uint64_t) * kFixedWords);
+    arena.meta = (uint32_t*)(base + 2 * sizeof(uint64_t) * kFixedWords);
+    arena.meta_words = kFixedMetaWords;
+    const Count cnt = host_count(n);
+    const int idx_bits = index_bits(n);
+    k_ray_keys<wf_ray_soa><<<grid_for(n, 256), 256, 0, ctx->stream>>>(
+        *rays, n, nullptr, ray_key_frame(bounds_min, bounds_max, idx_bits), words, keys_out);
+    WF_HIP(hipGetLastError());
+    WF_TRY(radix_sort_packed_fixed(ctx, words, sorted_indices, cnt, idx_bits, 33, arena,
+                                   "ray_sort"));
+    return WF_OK;
 }
 
 wf_status wf_materialize_rays(wf_context* ctx, const wf_ray_soa* src, const int32_t* idx,

@@ -569,7 +569,7 @@ static void launch_pass(wf_context* ctx, bool packed, int tiles, const uint64_t*
 template <int BITS>
 static wf_status sweep_sort(wf_context* ctx, const uint64_t* keys_in, const int32_t* vals_in,
                             uint64_t* keys_out, int32_t* vals_out, Count n, int lo_bit, int hi_bit,
-                            const char* tag, int idx_bits = 0) {
+                            const char* tag, int idx_bits = 0, const SortArena* arena = nullptr) {
     constexpr int R = Digits<BITS>::R;
     const int span = hi_bit - lo_bit + 1;
     const bool packed = idx_bits > 0;
@@ -584,8 +584,13 @@ static wf_status sweep_sort(wf_context* ctx, const uint64_t* keys_in, const int3
     if (passes > Digits<BITS>::MaxPasses || passes > kMaxPasses)
         return set_error(WF_ERR_INVALID_ARGUMENT, "sort key too wide");
     void *pk = nullptr, *pv = nullptr, *pm = nullptr;
-    WF_TRY(workspace(ctx, "sort_keys_alt", sizeof(uint64_t) * (size_t)n.n, &pk));
-    if (!packed) WF_TRY(workspace(ctx, "sort_vals_alt", sizeof(int32_t) * (size_t)n.n, &pv));
+    if (arena) {
+        if (!packed) return set_error(WF_ERR_INVALID_ARGUMENT, "fixed sort buffers hold packed words");
+        pk = arena->keys_alt;
+    } else {
+        WF_TRY(workspace(ctx, "sort_keys_alt", sizeof(uint64_t) * (size_t)n.n, &pk));
+        if (!packed) WF_TRY(workspace(ctx, "sort_vals_alt", sizeof(int32_t) * (size_t)n.n, &pv));
+    }
     // one zero-filled block: a tile counter per pass (an exhausted look-back is reported through
     // ctx->fault), the digit totals of every pass, the look-back states of every pass
     // (each part starts on a 256-byte boundary: a tile's state row must not share cache lines
@@ -593,7 +598,14 @@ static wf_status sweep_sort(wf_context* ctx, const uint64_t* keys_in, const int3
     auto up = [](size_t words) { return (words + 63) & ~(size_t)63; };
     const size_t off_hist = up(kMaxPasses), off_state = off_hist + up((size_t)stride * passes);
     const size_t meta = off_state + (size_t)stride * tiles * passes;
-    WF_TRY(workspace(ctx, "sort_meta", sizeof(uint32_t) * meta, &pm));
+    if (arena) {
+        if (meta > arena->meta_words)
+            return set_error(WF_ERR_INVALID_ARGUMENT, "sort of %d words needs %zu meta words, the "
+                             "fixed buffers hold %zu", n.n, meta, arena->meta_words);
+        pm = arena->meta;
+    } else {
+        WF_TRY(workspace(ctx, "sort_meta", sizeof(uint32_t) * meta, &pm));
+    }
     int32_t* counters = (int32_t*)pm;
     uint32_t* hist = (uint32_t*)pm + off_hist;
     uint32_t* state = (uint32_t*)pm + off_state;
@@ -682,6 +694,18 @@ wf_status radix_sort_packed(wf_context* ctx, const uint64_t* words, uint64_t* sc
         case 11: return sweep_sort<11>(ctx, words, nullptr, scratch, order, n, lo_bit, hi_bit, tag, idx_bits);
         default: return sweep_sort<8>(ctx, words, nullptr, scratch, order, n, lo_bit, hi_bit, tag, idx_bits);
     }
+}
+
+wf_status radix_sort_packed_fixed(wf_context* ctx, uint64_t* words, int32_t* order, Count n,
+                                  int32_t idx_bits, int32_t key_bits, const SortArena& arena,
+                                  const char* tag) {
+    if (n.n <= 0) return WF_OK;
+    if (idx_bits < 1 || idx_bits > 31 || key_bits < 1 || idx_bits + key_bits > 64)
+        return set_error(WF_ERR_INVALID_ARGUMENT, "radix_sort_packed_fixed: %d key bits above %d "
+                         "index bits do not fit a 64-bit word", key_bits, idx_bits);
+    // every key bit is sorted (no readback of the varying range): the same stable permutation
+    return sweep_sort<8>(ctx, words, nullptr, words, order, n, idx_bits, idx_bits + key_bits - 1,
+                         tag, idx_bits, &arena);
 }
 
 wf_status radix_sort_pairs(wf_context* ctx, const uint64_t* keys_in, const int32_t* vals_in,
