@@ -706,6 +706,7 @@ float orc_fresnel_dielectric(float cos_i, float eta) {
 }
 
 float orc_mis_power(float a, float b) { return mis_power(a, b); }
+float orc_area_to_angle(float area, float dist, float c) { return area_to_angle(area, dist, c); }
 float orc_mis_balance(float a, float b) { return mis_balance(a, b); }
 
 void orc_cosine_hemisphere(const float* n, float u1, float u2, float* out) {
