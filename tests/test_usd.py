@@ -162,6 +162,150 @@ def test_reference_cornell_box():
     assert cam == {"eye": (0, 5, 15), "target": (0, 5, -2), "up": (0, 1, 0), "vfov": 53.0}
 
 
+@pytest.mark.skipif(not REF.is_dir(), reason="reference tree absent")
+def test_reference_material_variants():
+    """scenes/materialVariants.usda under UsdSceneLoader.cpp:441-616: emissive colour with
+    spotOuterAngle -> SpotLight with both angles; metallic > 0.5 -> Metal (keeping a connected
+    UsdUVTexture); plastic -> Plastic; spatialChecker -> the 512^2 palette at the loader's fixed
+    contrast 0.25 (the file's spatialCheckerReduceContrast is not read) and the cell size;
+    roughness alone leaves a Diffuse. The dragon meshes are references to absent assets."""
+    warnings = []
+    s = usd.read_usda(REF / "materialVariants.usda", warn=warnings.append)
+    by = {m.name.split("/")[-1]: s.materials[m.material_id] for m in s.meshes}
+    spot = by["LampDisc"]
+    assert spot["type"] == abi.MAT_SPOTLIGHT and spot["albedo"] == (120.0, 120.0, 120.0)
+    assert (spot["spot_outer_deg"], spot["spot_falloff_deg"], spot["emission"]) == (120.0, 90.0, 1.0)
+    assert by["LampDisc"] is by["LampDisc"] and s.meshes[5].indices.shape[0] > 2
+    mirror = by["MirrorLeft"]
+    assert mirror["type"] == abi.MAT_METAL and mirror["roughness"] == 0.0
+    assert mirror["albedo"] == (1.0, 1.0, 1.0) and mirror["texture_id"] == -1
+    floor = by["Floor"]
+    assert floor["type"] == abi.MAT_METAL and floor["texture_id"] >= 0
+    tex = s.textures[floor["texture_id"]]
+    assert tex["format"] == abi.TEX_RGBA8_SRGB and tex["width"] > 1 and tex["height"] > 1
+    assert by["AccentLight_A"]["type"] == abi.MAT_EMISSIVE
+    assert by["AccentLight_A"]["albedo"] == (6.0, 5.5, 7.5)
+    assert by["AccentLightBacking_B"] == {"type": abi.MAT_DIFFUSE, "albedo": (0.02, 0.02, 0.02),
+                                          "texture_id": -1, "model": abi.MODEL_REFERENCE}
+    # Material prims in traversal order; the four dragon materials have no mesh here
+    none, plastic, checker, cb = s.materials[6:10]
+    assert none["type"] == abi.MAT_DIFFUSE and none["albedo"] == (0.75, 0.75, 0.75)
+    assert plastic["type"] == abi.MAT_PLASTIC and plastic["albedo"] == (0.85, 0.08, 0.08)
+    assert plastic["roughness"] == 0.15
+    assert checker["type"] == abi.MAT_DIFFUSE and checker["use_spatial_checker"] == 1
+    assert checker["checker_cell_size"] == 0.3
+    pal = s.textures[checker["texture_id"]]
+    assert pal["width"] == pal["height"] == 512 and pal["format"] == abi.TEX_RGB_F32
+    assert np.asarray(pal["data"]).reshape(512, 512, 4)[0, 0, :3].tolist() == [0.25] * 3
+    assert cb["type"] == abi.MAT_METAL and cb["albedo"] == (0.72, 0.35, 0.15)
+    assert cb["roughness"] == 0.25
+    assert sum("xyzrgb_dragon" in w for w in warnings) == 4
+
+
+@pytest.mark.skipif(not REF.is_dir(), reason="reference tree absent")
+def test_reference_cornell_box_dragon_composition():
+    """scenes/cornellBoxDragon.usda: the Cornell box comes in through a reference with its own
+    paths re-rooted; `over` prims deactivate the tall box, re-bind and move the short box and
+    lower the light; the teapot asset resolves from the repository root and binds Glass; the
+    dragon asset is absent."""
+    warnings = []
+    s = usd.read_usda(REF / "cornellBoxDragon.usda", warn=warnings.append)
+    names = [m.name.split("/")[-1] for m in s.meshes]
+    assert not any(n.startswith("TallBox") for n in names)
+    assert names[:6] == ["Floor", "Ceiling", "BackWall", "LeftWall", "RightWall", "CeilingLight"]
+    mat = lambda n: s.materials[s.meshes[names.index(n)].material_id]
+    assert mat("CeilingLight")["type"] == abi.MAT_EMISSIVE
+    assert mat("CeilingLight")["albedo"] == (18.0, 18.0, 18.0)
+    short = [n for n in names if n.startswith("ShortBox")]
+    assert len(short) == 6
+    for n in short:
+        m = mat(n)
+        assert m["type"] == abi.MAT_METAL and m["albedo"] == (0.75, 0.6, 0.15)
+        assert m["roughness"] == 0.9
+    top = s.meshes[names.index("ShortBox_Top")].positions
+    assert np.allclose(top[:, 1], top[0, 1]) and abs(float(top[0, 1]) - 5.13) < 1e-5
+    teapot = s.meshes[names.index("Teapot")]
+    assert teapot.indices.shape[0] > 10_000 and mat("Teapot")["type"] == abi.MAT_GLASS
+    assert abs(float(teapot.positions[:, 1].min()) - 5.13) < 1e-3     # sits on the short box
+    checker = next(m for m in s.materials if m.get("use_spatial_checker"))
+    assert checker["checker_cell_size"] == 0.5
+    assert any("dragon.usda" in w for w in warnings)
+
+
+def test_composition_rules_closed_form(tmp_path):
+    """A reference with an over that deactivates a child, overrides an attribute and re-binds a
+    mesh; the referenced layer's own binding targets follow it to its new path."""
+    (tmp_path / "sub").mkdir()
+    (tmp_path / "sub" / "part.usda").write_text('''#usda 1.0
+(
+    defaultPrim = "Part"
+)
+def Xform "Part"
+{
+    def Scope "Looks"
+    {
+        def Material "Red"
+        {
+            token outputs:surface.connect = </Part/Looks/Red/S.outputs:surface>
+            def Shader "S"
+            {
+                uniform token info:id = "UsdPreviewSurface"
+                color3f inputs:diffuseColor = (1, 0, 0)
+            }
+        }
+    }
+    def Mesh "Keep" (apiSchemas = ["MaterialBindingAPI"])
+    {
+        point3f[] points = [(0, 0, 0), (1, 0, 0), (0, 1, 0)]
+        int[] faceVertexCounts = [3]
+        int[] faceVertexIndices = [0, 1, 2]
+        rel material:binding = </Part/Looks/Red>
+    }
+    def Mesh "Drop"
+    {
+        point3f[] points = [(0, 0, 0), (1, 0, 0), (0, 1, 0)]
+        int[] faceVertexCounts = [3]
+        int[] faceVertexIndices = [0, 1, 2]
+    }
+}
+''')
+    (tmp_path / "top.usda").write_text('''#usda 1.0
+def Xform "World"
+{
+    def "A" (
+        prepend references = @sub/part.usda@
+    )
+    {
+        double3 xformOp:translate = (10, 0, 0)
+        uniform token[] xformOpOrder = ["xformOp:translate"]
+        over "Drop" (active = false) {}
+        over "Looks"
+        {
+            over "Red"
+            {
+                over "S"
+                {
+                    color3f inputs:diffuseColor = (0, 0, 1)
+                }
+            }
+        }
+    }
+    def "B" (
+        prepend references = @sub/part.usda@
+    )
+    {
+    }
+}
+''')
+    s = usd.read_usda(tmp_path / "top.usda")
+    assert [m.name for m in s.meshes] == ["/World/A/Keep", "/World/B/Keep", "/World/B/Drop"]
+    a, b = s.meshes[0], s.meshes[1]
+    assert s.materials[a.material_id]["albedo"] == (0.0, 0.0, 1.0)    # A's over wins
+    assert s.materials[b.material_id]["albedo"] == (1.0, 0.0, 0.0)    # B keeps the layer's value
+    assert a.positions[:, 0].tolist() == [10.0, 11.0, 10.0]
+    assert b.positions[:, 0].tolist() == [0.0, 1.0, 0.0]
+
+
 @pytest.mark.gpu
 def test_round_trip_renders_identically(tmp_path, ctx):
     wf = importlib.import_module("wavefront-renderer_amd")
@@ -177,3 +321,24 @@ def test_round_trip_renders_identically(tmp_path, ctx):
     assert np.array_equal(ia, ib)
     a.close()
     b.close()
+
+
+@pytest.mark.gpu
+def test_read_back_scene_renders_as_the_oracle(tmp_path, ctx, orc):
+    """A scene written to .usda and read back, rendered on the device, against the oracle's
+    render of the scene as generated: the same image bits (one sample lane)."""
+    wf = importlib.import_module("wavefront-renderer_amd")
+    data = scenes.material_variants()
+    data.textures[1] = usd.spatial_palette(512, 0.25)   # what the loader builds for a checker
+    usd.write_usda(data, tmp_path / "v.usda")
+    back = usd.read_usda(tmp_path / "v.usda")
+    back.directional, back.environment = data.directional, data.environment
+    for physics in (wf.PHYSICS_REFERENCE, wf.PHYSICS_UNBIASED):
+        p = wf.render_params(48, 40, 2, 8, data.camera, firefly=1e30, seed=8, physics=physics,
+                             sample_lanes=1)
+        sc = wf.Scene(ctx, back)
+        img, cnt, st = wf.render(sc, p)
+        oimg, ocnt, _, ost = orc.scene(data).render(p)
+        assert np.array_equal(cnt, ocnt) and st.rays_traced == ost.rays_traced
+        assert np.array_equal(img, oimg)
+        sc.close()

@@ -19,8 +19,12 @@ palette (TextureGenerator::generateSpatialPalette, include/shading/Texture.h:76-
 fanned from their first vertex (triangulateFace, UsdSceneLoader.cpp:58-78); vertices go through
 the prim's local-to-world transform (translate, scale, rotateX/Y/Z/XYZ, transform ops); RectLight
 and DiskLight prims become emissive meshes; `upAxis = "Z"` is turned to Y up; an unbound mesh
-gets a diffuse material of its displayColor (default 0.8 grey). Not read: references, payloads,
-variants, instancing, Cube/Sphere/Cylinder gprims and sphere/cylinder lights.
+gets a diffuse material of its displayColor (default 0.8 grey). Internal references
+(`references = @file.usda@`) are composed: the referenced layer's default prim lands under the
+referencing prim with its paths re-rooted there, weaker than the referencing prim's own opinions
+and `over` prims, and prims with `active = false` are dropped; an asset path is resolved against
+the referencing layer's directory, then its parents. Not read: payloads, variants, instancing,
+sublayers, Cube/Sphere/Cylinder gprims and sphere/cylinder lights.
 """
 from __future__ import annotations
 
@@ -391,6 +395,87 @@ def parse_usda(text: str) -> Prim:
     return _Parser(text).stage()
 
 
+# ---- composition: internal references and overs ---------------------------------------------------
+def _resolve_asset(layer_dir: Path, asset: str) -> Optional[Path]:
+    """An asset path relative to the referencing layer's directory, else to one of its parents (the
+    reference's scenes name some assets from the repository root, e.g. @scenes/objects/x.usda@)."""
+    for base in (layer_dir, *layer_dir.parents):
+        f = base / asset
+        if f.is_file():
+            return f
+    return None
+
+
+def _fix_path(v, old: str, new: str):
+    if isinstance(v, tuple) and len(v) == 2 and v[0] == "path":
+        t = v[1]
+        if t == old or t.startswith(old + "/") or t.startswith(old + "."):
+            return ("path", new + t[len(old):])
+        return v
+    if isinstance(v, list):
+        return [_fix_path(x, old, new) for x in v]
+    return v
+
+
+def _reroot(p: Prim, old: str, new: str, parent: Optional[Prim], name: str) -> Prim:
+    """A copy of p's subtree moved from namespace path `old` to `new` (targets inside follow)."""
+    q = Prim(p.kind, p.type, name, new + p.path[len(old):], dict(p.meta),
+             {k: (t, _fix_path(v, old, new), m) for k, (t, v, m) in p.attrs.items()},
+             {k: _fix_path(v, old, new) for k, v in p.rels.items()}, [], parent)
+    q.children = [_reroot(c, old, new, q, c.name) for c in p.children]
+    return q
+
+
+def _merge(strong: Prim, weak: Prim) -> None:
+    """strong's opinions over weak's, in place: attributes, relationships and metadata of strong win;
+    children are matched by name (weak's order, then strong's new children)."""
+    strong.attrs = {**weak.attrs, **strong.attrs}
+    strong.rels = {**weak.rels, **strong.rels}
+    strong.meta = {**weak.meta, **strong.meta}
+    if not strong.type:
+        strong.type = weak.type
+    if strong.kind == "over" and weak.kind == "def":
+        strong.kind = "def"
+    local = {c.name: c for c in strong.children}
+    merged = []
+    for wc in weak.children:
+        sc = local.pop(wc.name, None)
+        if sc is None:
+            wc.parent = strong
+            merged.append(wc)
+        else:
+            _merge(sc, wc)
+            merged.append(sc)
+    merged += [c for c in strong.children if c.name in local]
+    strong.children = merged
+
+
+def _compose(p: Prim, layer_dir: Path, warn, depth: int) -> None:
+    """Composes internal references (`references = @file@`: the file's default prim, or its first
+    root prim, under p, weaker than p's own opinions and overs) and drops inactive prims."""
+    ref = p.meta.pop("references", None)
+    for item in (ref if isinstance(ref, list) else [ref] if ref is not None else []):
+        if not (isinstance(item, tuple) and item[0] == "asset"):
+            continue
+        f = _resolve_asset(layer_dir, item[1])
+        if f is None or depth > 8:
+            if warn:
+                warn(f"{p.path}: reference {item[1]} not found")
+            continue
+        sub = parse_usda(f.read_text())
+        for c in sub.children:
+            _compose(c, f.parent, warn, depth + 1)
+        name = sub.meta.get("defaultPrim")
+        target = next((c for c in sub.children if c.name == name), None) if name else None
+        target = target or (sub.children[0] if sub.children else None)
+        if target is None:
+            continue
+        _merge(p, _reroot(target, target.path, p.path, p.parent, p.name))
+    for c in p.children:
+        _compose(c, layer_dir, warn, depth)
+    p.children = [c for c in p.children if c.meta.get("active", True) is not False]
+
+
 def _traverse(p: Prim):
     for c in p.children:
         if c.kind == "def":
@@ -494,6 +579,9 @@ def read_usda(path, warn=None) -> SceneData:
     """The scene of a .usda file under the reference loader's rules (module docstring)."""
     path = Path(path)
     root = parse_usda(path.read_text())
+    for c in root.children:
+        _compose(c, path.parent, warn, 0)
+    root.children = [c for c in root.children if c.meta.get("active", True) is not False]
     z_up = root.meta.get("upAxis") == "Z"
     fix = _rot(0, -90.0) if z_up else np.eye(4)
     materials, textures, mat_index = [], [], {}
