@@ -489,6 +489,7 @@ class Engine:
         check(self._lib.wf_engine_create(ctx.handle, scene.handle, C.byref(cfg), C.byref(h)))
         self.handle = h.value
         self.jobs = {}
+        self.lanes = sample_lanes or 64     # the lanes a lone render needs to match its jobs
         ctx._engines.add(self)
 
     def admit(self, job: _abi.JobDesc) -> int:

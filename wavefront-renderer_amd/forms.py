@@ -107,12 +107,15 @@ def make_engines(ctx, scenes, physics, capacity, pixel_budget, max_jobs=64):
                    physics=ph, firefly=1e30, use_graph=True) for sc, ph in zip(scenes, physics)]
 
 
-def serve(engines, datas, jobs, iterations: int = 2, realtime: bool = True) -> dict:
+def serve(engines, datas, jobs, iterations: int = 2, realtime: bool = True,
+          collect: bool = False) -> dict:
     """Admits jobs at their arrival times (all at once when realtime is False), runs every engine
-    with work, collects finished jobs. Returns makespan, rays and per-job completion times."""
+    with work, collects finished jobs. Returns makespan, rays and per-job completion times (and,
+    with collect, every job's (image, counts, result) on the host, in job order)."""
     order = sorted(range(len(jobs)), key=lambda k: jobs[k].arrival)
     pending = []           # arrived, not admitted yet (FIFO)
     slot = [dict() for _ in engines]
+    images = {}
     done = 0
     nxt = 0
     r0 = sum(e.stats().rays_traced + e.stats().shadow_rays for e in engines)
@@ -143,12 +146,17 @@ def serve(engines, datas, jobs, iterations: int = 2, realtime: bool = True) -> d
                 continue
             for jid in e.wait():
                 k = slot[i].pop(jid)
-                e.result(jid, keep_on_device=True)
+                out = e.result(jid, keep_on_device=not collect)
+                if collect:
+                    images[k] = out
                 jobs[k].done = time.perf_counter() - t0
                 done += 1
     makespan = time.perf_counter() - t0
     rays = sum(e.stats().rays_traced + e.stats().shadow_rays for e in engines) - r0
-    return {"makespan_s": makespan, "rays": rays}
+    res = {"makespan_s": makespan, "rays": rays}
+    if collect:
+        res["images"] = [images[k] for k in range(len(jobs))]
+    return res
 
 
 def fcfs_latency(arrivals, service) -> np.ndarray:
