@@ -347,6 +347,11 @@ def main() -> None:
                          "accumulation order bit for bit)")
     ap.add_argument("--wavefront", type=int, default=0,
                     help="regenerate: ray slots of the wavefront (0 = automatic)")
+    ap.add_argument("--ray-key-bits", type=int, default=0,
+                    help="regenerate: order each bounce's rays by the top N bits of "
+                         "RayQueue::sortKey (1..33, 33 = the reference's full key); 0 = no "
+                         "ordering (the queue order there changes no result; the frame is also "
+                         "timed with the full key for comparison)")
     ap.add_argument("--quick", action="store_true", help="reduced sizes (CPU-side checks)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-policies", action="store_true")
@@ -412,10 +417,14 @@ def main() -> None:
     adaptive = {k: ws[k] for k in ("adaptive", "adaptive_threshold", "adaptive_min_samples",
                                    "adaptive_interval") if k in ws}
 
-    def params(policy, seed=1, w=W, h=H, s=spp, schedule=args.schedule):
+    if args.ray_key_bits:
+        ctx.set_tuning(ray_key_bits=args.ray_key_bits)
+
+    def params(policy, seed=1, w=W, h=H, s=spp, schedule=args.schedule, ray_sort=None):
+        rs = args.ray_key_bits > 0 if ray_sort is None else ray_sort
         return wf.render_params(width=w, height=h, spp=s, max_depth=depth,
                                 camera=data.camera, policy=policy, physics=physics,
-                                firefly=1e30, seed=seed, schedule=schedule,
+                                firefly=1e30, seed=seed, schedule=schedule, ray_sort=rs,
                                 sample_lanes=args.lanes, wavefront=args.wavefront, **adaptive)
 
     tuned = None
@@ -434,9 +443,9 @@ def main() -> None:
         log(f"[bench] autotune ({time.perf_counter() - t0:.1f}s, {tuned['trials']} trials): "
             f"{tuned['best']}")
 
-    def frame(policy, seed=1, schedule=args.schedule):
-        img, cnt, st = wf.render(scene, params(policy, seed, schedule=schedule),
-                                 keep_on_device=True)
+    def frame(policy, seed=1, schedule=args.schedule, ray_sort=None):
+        img, cnt, st = wf.render(scene, params(policy, seed, schedule=schedule,
+                                               ray_sort=ray_sort), keep_on_device=True)
         return st
 
     for w in range(args.warmup):
@@ -504,7 +513,7 @@ def main() -> None:
         if roofline is not None:
             roofline["stream_copy_GB_s"] = copy
 
-    tstats, other_sched = None, None
+    tstats, other_sched, full_key = None, None, None
     if not args.profile_run:
         tstats = traversal_stats(wf, ctx, scene, data, W, H, np.random.default_rng(5))
         log(f"[bench] traversal: {json.dumps(tstats)}")
@@ -518,6 +527,19 @@ def main() -> None:
         other_sched = {"schedule": other, "ms": round(dt * 1e3, 1),
                        "mrays_s": round((st.rays_traced + st.shadow_rays) / dt / 1e6, 2)}
         log(f"[bench] schedule {other}: {other_sched}")
+        # the same frame with every bounce's rays ordered by the reference's full 33-bit key
+        saved_bits = ctx.tuning()["ray_key_bits"]
+        ctx.set_tuning(ray_key_bits=33)
+        frame(args.policy, seed=6, ray_sort=True)
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        st = frame(args.policy, seed=5, ray_sort=True)
+        ctx.synchronize()
+        dt = time.perf_counter() - t1
+        ctx.set_tuning(ray_key_bits=saved_bits)
+        full_key = {"ray_key_bits": 33, "ms": round(dt * 1e3, 1),
+                    "mrays_s": round((st.rays_traced + st.shadow_rays) / dt / 1e6, 2)}
+        log(f"[bench] full ray key: {full_key}")
 
     policies = {}
     if not args.no_policies and not args.profile_run:
@@ -583,6 +605,9 @@ def main() -> None:
         "data": "synthetic (seeded generator)",
         "config": {"workload": desc, "policy": args.policy, "schedule": args.schedule,
                    "sample_lanes": args.lanes, "wavefront": args.wavefront or "auto",
+                   "ray_order": (f"top {args.ray_key_bits} bits of RayQueue::sortKey"
+                                 if args.ray_key_bits else "none (queue order)"),
+                   "full_ray_key": full_key,
                    "tuning": ctx.tuning(), "autotuned": tuned is not None,
                    "bvh_width": scene.stats.bvh_width,
                    "other_schedule": other_sched, "physics":

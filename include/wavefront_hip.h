@@ -541,6 +541,11 @@ typedef struct wf_tuning {
     int32_t shadow_blocks_per_cu;   /* the shadow-ray kernel's grid (0 = trace_blocks_per_cu): it
                                      * traces few rays per lane, so its shape is tuned apart */
     int32_t shadow_refill_lanes;    /* the shadow-ray kernel's refill threshold (0 = refill_lanes) */
+    int32_t ray_key_bits;        /* regenerating wavefront: how many top bits of RayQueue::sortKey's
+                                  * 33-bit key (octant, then Morton from the top) order the rays of
+                                  * each bounce (1..33); the queue order there is free, so a
+                                  * shorter key trades coherence for fewer sort passes.
+                                  * wf_ray_schedule always sorts by the full key */
 } wf_tuning;
 /* Engines created on the context take its tuning at every run. */
 wf_status wf_context_set_tuning(wf_context* ctx, const wf_tuning* tuning);

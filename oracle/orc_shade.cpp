@@ -14,6 +14,8 @@
 #include <atomic>
 #include <cfloat>
 #include <chrono>
+#include <cmath>
+#include <cstring>
 #include <functional>
 #include <thread>
 
@@ -756,6 +758,37 @@ void orc_surface_color(void* sp, const wf_material* m, const float* p, float u, 
 
 uint32_t orc_hash_grid_cell(int32_t x, int32_t y, int32_t z) { return hash_grid_cell(x, y, z); }
 
+// Markstein's reciprocal correction y + y (1 - b y) (two fmas) over every significand of the
+// binade [2^e, 2^(e+1)) and every faithful approximation y of 1/b (|y - 1/b| < 1 ulp): the number
+// of (b, y) pairs whose corrected value differs from the correctly rounded 1/b, and the last such
+// b's bits. The device traversal corrects the hardware reciprocal (faithful) this way.
+int64_t orc_reciprocal_correction_failures(int32_t e, uint32_t* last_bad_bits) {
+    int64_t bad = 0;
+    for (uint32_t m = 0; m < (1u << 23); ++m) {
+        const uint32_t bits = ((uint32_t)(e + 127) << 23) | m;
+        float b;
+        std::memcpy(&b, &bits, 4);
+        const float cr = 1.0f / b;
+        const double exact = 1.0 / (double)b;
+        uint32_t cb;
+        std::memcpy(&cb, &cr, 4);
+        for (int k = -2; k <= 2; ++k) {
+            const uint32_t yb = cb + (uint32_t)k;
+            float y, y1;
+            std::memcpy(&y, &yb, 4);
+            const uint32_t yb1 = yb + 1u;
+            std::memcpy(&y1, &yb1, 4);
+            if (std::fabs((double)y - exact) >= (double)(y1 - y)) continue;   // not faithful
+            const float z = std::fma(std::fma(-b, y, 1.0f), y, y);
+            if (z != cr) {
+                ++bad;
+                *last_bad_bits = bits;
+            }
+        }
+    }
+    return bad;
+}
+
 // The shared single-precision functions, exported for their accuracy tests.
 void orc_fm_sincos(int32_t n, const float* x, float* s, float* c) {
     for (int32_t i = 0; i < n; ++i) wf::fm::sincos(x[i], s[i], c[i]);
@@ -1030,9 +1063,36 @@ void orc_adaptive_update(int32_t n, const float* mean, const float* m2, const in
 // WavefrontRenderer::renderScene (cpp:94-399) on the CPU with the same queues and policies.
 // image: float RGB out; counts: per-pixel samples out (may be NULL); image64: FP64-accumulated
 // image out (may be NULL); stats: ray counts and phase wall times (milliseconds).
+static int32_t render_masked(void* sp, const wf_render_params* rp, const uint8_t* pixel_mask,
+                             const double* mat_cost, const double* tex_cost, float* image,
+                             int32_t* counts, double* image64, wf_render_stats* stats,
+                             int32_t threads);
+
 int32_t orc_render(void* sp, const wf_render_params* rp, const double* mat_cost,
                    const double* tex_cost, float* image, int32_t* counts, double* image64,
                    wf_render_stats* stats, int32_t threads) {
+    return render_masked(sp, rp, nullptr, mat_cost, tex_cost, image, counts, image64, stats,
+                         threads);
+}
+
+// The frame of orc_render restricted to the pixels of a window [x0, x0 + w) x [y0, y0 + h) of the
+// full image: every other pixel is never traced. A pixel's paths are those of the full frame
+// (their draws are addressed by the full-frame pixel index). Outputs are full-frame sized.
+int32_t orc_render_window(void* sp, const wf_render_params* rp, int32_t x0, int32_t y0,
+                          int32_t w, int32_t h, float* image, int32_t* counts, double* image64,
+                          wf_render_stats* stats, int32_t threads) {
+    std::vector<uint8_t> mask((size_t)rp->width * rp->height, 0);
+    for (int y = std::max(0, y0); y < std::min(rp->height, y0 + h); ++y)
+        for (int x = std::max(0, x0); x < std::min(rp->width, x0 + w); ++x)
+            mask[(size_t)y * rp->width + x] = 1;
+    return render_masked(sp, rp, mask.data(), nullptr, nullptr, image, counts, image64, stats,
+                         threads);
+}
+
+static int32_t render_masked(void* sp, const wf_render_params* rp, const uint8_t* pixel_mask,
+                             const double* mat_cost, const double* tex_cost, float* image,
+                             int32_t* counts, double* image64, wf_render_stats* stats,
+                             int32_t threads) {
     const Scene& s = *(Scene*)sp;
     const int W = rp->width, H = rp->height, P = W * H;
     std::vector<float> accum((size_t)P * 3, 0.0f), before;
@@ -1060,7 +1120,7 @@ int32_t orc_render(void* sp, const wf_render_params* rp, const double* mat_cost,
         if (rp->adaptive) before = accum;
         std::vector<int32_t> pix;
         for (int p = 0; p < P; ++p)
-            if (!rp->adaptive || active[p]) pix.push_back(p);
+            if ((!pixel_mask || pixel_mask[p]) && (!rp->adaptive || active[p])) pix.push_back(p);
         RayQ q;
         q.resize(pix.size());
         orc_raygen(&rp->camera, W, H, (int)pix.size(), pix.data(), rp->shade.seed, rp->shade.job,

@@ -62,6 +62,9 @@ class Oracle:
         L.orc_render.restype = C.c_int32
         L.orc_render.argtypes = [P, C.POINTER(abi.RenderParams), P, P, P, P, P,
                                  C.POINTER(abi.RenderStats), C.c_int32]
+        L.orc_render_window.restype = C.c_int32
+        L.orc_render_window.argtypes = [P, C.POINTER(abi.RenderParams)] + [C.c_int32] * 4 + \
+            [P, P, P, C.POINTER(abi.RenderStats), C.c_int32]
         L.orc_scatter_replay.restype = C.c_int32
         L.orc_scatter_replay.argtypes = [P, C.POINTER(abi.Material), P, P, P, C.c_float,
                                          C.c_float, C.c_int32, P, C.c_int32, P, P, P, P]
@@ -78,6 +81,8 @@ class Oracle:
         for name in ("orc_mis_balance", "orc_mis_power"):
             getattr(L, name).restype = C.c_float
             getattr(L, name).argtypes = [C.c_float, C.c_float]
+        L.orc_reciprocal_correction_failures.restype = C.c_int64
+        L.orc_reciprocal_correction_failures.argtypes = [C.c_int32, P]
         L.orc_area_to_angle.restype = C.c_float
         L.orc_area_to_angle.argtypes = [C.c_float] * 3
         L.orc_fm_inverse.argtypes = [C.c_int32, P, P, P, P, P]
@@ -308,6 +313,27 @@ class OracleScene:
         shape = (params.height, params.width)
         return (img.reshape(*shape, 3), cnt.reshape(shape),
                 None if img64 is None else img64.reshape(*shape, 3), st)
+
+
+def _render_window(osc, params, window, threads=8):
+    """orc_render_window: (float image, counts, FP64 image, stats) of the frame's pixels inside
+    window = (x0, y0, w, h), cropped to the window."""
+    x0, y0, w, h = window
+    P = params.width * params.height
+    img = np.zeros(P * 3, np.float32)
+    cnt = np.zeros(P, np.int32)
+    img64 = np.zeros(P * 3, np.float64)
+    st = abi.RenderStats()
+    osc.orc.lib.orc_render_window(osc.h, C.byref(params), x0, y0, w, h, fptr(img), fptr(cnt),
+                                  fptr(img64), C.byref(st), threads)
+    shape = (params.height, params.width)
+    crop = (slice(y0, y0 + h), slice(x0, x0 + w))
+    return (img.reshape(*shape, 3)[crop], cnt.reshape(shape)[crop],
+            img64.reshape(*shape, 3)[crop], st)
+
+
+OracleScene.render_window = lambda self, params, window, threads=8: _render_window(
+    self, params, window, threads)
 
 
 class Reference:

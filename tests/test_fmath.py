@@ -85,3 +85,14 @@ def test_atan2_within_three_ulp_and_signed_zeros(orc, rng):
     _, _, t = orc.fm_inverse(w, z)
     want = np.arctan2(z, w).astype(f32)
     assert np.array_equal(t, want) and np.array_equal(np.signbit(t), np.signbit(want))
+
+
+@pytest.mark.parametrize("e", [-20, 0, 30])
+def test_reciprocal_correction_is_correctly_rounded(orc, e):
+    """The traversal's 1/det (wf_traverse.h rcp_cr): one fma correction of a faithful reciprocal is
+    the correctly rounded 1/b for every significand of a binade and every faithful approximation,
+    except the all-ones significand, which the device routes to the division."""
+    import ctypes as C
+    last = C.c_uint32(0)
+    bad = orc.lib.orc_reciprocal_correction_failures(e, C.byref(last))
+    assert bad == 1 and (last.value & 0x7FFFFF) == 0x7FFFFF, (bad, hex(last.value))

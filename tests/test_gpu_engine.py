@@ -67,6 +67,25 @@ def test_regenerate_equals_passes_textured_unbiased(wf, stress, policy):
     assert_same(a, b)
 
 
+@pytest.mark.parametrize("key_bits", [33, 9, 1])
+def test_ray_order_changes_no_result(wf, ctx, cornell, key_bits):
+    """Renders of one scene and size switch the per-bounce ray order off and on (the library keeps
+    one engine per scene and turns its sort on when asked): the same image and statistics with
+    the reference's full 33-bit key, a shorter key and no ordering."""
+    data, scene = cornell
+    saved = ctx.tuning()
+    kw = dict(camera=data.camera, width=48, height=40, spp=4, max_depth=8, seed=12, firefly=1e30)
+    try:
+        a = wf.render(scene, wf.render_params(ray_sort=False, **kw))
+        ctx.set_tuning(ray_key_bits=key_bits)
+        b = wf.render(scene, wf.render_params(ray_sort=True, **kw))
+        c = wf.render(scene, wf.render_params(ray_sort=False, **kw))
+    finally:
+        ctx.set_tuning(**saved)
+    assert_same(a, b)
+    assert_same(a, c)
+
+
 @pytest.mark.parametrize("radix_bits", [6, 11])
 def test_regenerate_equals_passes_radix_widths(wf, ctx, cornell, radix_bits):
     """Device-count sorts (ray keys and shading keys inside the engine) with other digit widths:

@@ -84,14 +84,15 @@ class Context:
                 "bvh_builder": t.bvh_builder, "ploc_radius": t.ploc_radius,
                 "bvh_width": t.bvh_width, "radix_bits": t.radix_bits,
                 "shadow_blocks_per_cu": t.shadow_blocks_per_cu,
-                "shadow_refill_lanes": t.shadow_refill_lanes}
+                "shadow_refill_lanes": t.shadow_refill_lanes, "ray_key_bits": t.ray_key_bits}
 
     def set_tuning(self, **kw):
         cur = self.tuning()
         cur.update(kw)
         t = _abi.Tuning(cur["refill_lanes"], cur["trace_blocks_per_cu"], cur["bvh_builder"],
                         cur["ploc_radius"], cur["bvh_width"], cur["radix_bits"],
-                        cur["shadow_blocks_per_cu"], cur["shadow_refill_lanes"])
+                        cur["shadow_blocks_per_cu"], cur["shadow_refill_lanes"],
+                        cur["ray_key_bits"])
         check(self._lib.wf_context_set_tuning(self.handle, C.byref(t)))
 
     def stream_copy(self, dst: int, src: int, nbytes: int):

@@ -167,7 +167,8 @@ class Tuning(C.Structure):
     _fields_ = [("refill_lanes", C.c_int32), ("trace_blocks_per_cu", C.c_int32),
                 ("bvh_builder", C.c_int32), ("ploc_radius", C.c_int32),
                 ("bvh_width", C.c_int32), ("radix_bits", C.c_int32),
-                ("shadow_blocks_per_cu", C.c_int32), ("shadow_refill_lanes", C.c_int32)]
+                ("shadow_blocks_per_cu", C.c_int32), ("shadow_refill_lanes", C.c_int32),
+                ("ray_key_bits", C.c_int32)]
 
 
 _S = C.c_int32

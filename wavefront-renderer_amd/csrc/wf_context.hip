@@ -319,6 +319,8 @@ wf_status wf_context_set_tuning(wf_context* ctx, const wf_tuning* t) {
     WF_REQUIRE(t->shadow_refill_lanes >= 0 && t->shadow_refill_lanes <= 64,
                "wf_context_set_tuning: shadow_refill_lanes %d not in [0, 64]",
                t->shadow_refill_lanes);
+    WF_REQUIRE(t->ray_key_bits >= 1 && t->ray_key_bits <= 33,
+               "wf_context_set_tuning: ray_key_bits %d not in [1, 33]", t->ray_key_bits);
     ctx->tuning = *t;
     return WF_OK;
 }

@@ -665,7 +665,7 @@ wf_status enqueue_iteration(wf_engine* e) {
     const RayPackedOrdered ordered{e->r0.a, e->r0.b, e->r0.c, e->idx};
     if (sort)
         WF_TRY(radix_sort_packed(ctx, e->keys, e->keys2, e->idx, live, kf.idx_bits, 33,
-                                 "ray_sort"));
+                                 "ray_sort", 33 - ctx->tuning.ray_key_bits));
     phase_mark(e, 2);
     const unsigned finish_grid = grid_for(cap, kFinishBlock * kFinishItems);
     {
@@ -1290,6 +1290,11 @@ wf_status render_regenerate(wf_scene* s, const wf_render_params* rp, float* imag
         if (std::memcmp(&cfg, &e->cfg, sizeof(cfg)) != 0) {
             // the captured iteration bakes the policy and shading settings in
             drop_graph(e);
+            // an engine created without ray ordering has no sort-word buffers yet
+            if (cfg.ray_sort && !e->keys) {
+                WF_TRY(engine_array(e, e->cap, &e->keys));
+                WF_TRY(engine_array(e, e->cap, &e->keys2));
+            }
             e->cfg = cfg;
         }
     }

@@ -75,7 +75,7 @@ struct wf_context {
     std::vector<wf::PendingTiming> pending;
     std::vector<hipEvent_t> event_pool;
     int32_t* count_slots = nullptr;    // pinned, kCountSlots entries
-    wf_tuning tuning{48, 12, WF_BVH_PLOC, 16, 4, 8, 0, 0};
+    wf_tuning tuning{48, 12, WF_BVH_PLOC, 16, 4, 8, 0, 0, 33};
     int next_count_slot = 0;
     int cu_count = 256;
     // Device word that kernels set when a bounded device-wide wait (the radix sort's look-back)
@@ -349,10 +349,11 @@ wf_status radix_sort_pairs(wf_context* ctx, const uint64_t* keys_in, const int32
 // key_bits) and its own position i in bits [0, idx_bits). Sorting on the key bits alone is the
 // same permutation as sorting (key, i) pairs, with 8 bytes moved per element instead of 12;
 // order receives the low idx_bits bits of the sorted words. scratch (n words) is overwritten;
-// words is not. A host count sorts only the key bits where the words differ.
+// words is not. A host count sorts only the key bits where the words differ. key_lo > 0 leaves the
+// low key_lo key bits out of the order (a stable sort by the key's top key_bits - key_lo bits).
 wf_status radix_sort_packed(wf_context* ctx, const uint64_t* words, uint64_t* scratch,
                             int32_t* order, Count n, int32_t idx_bits, int32_t key_bits,
-                            const char* tag);
+                            const char* tag, int32_t key_lo = 0);
 // Coherence metrics of one shading batch (wf_coherence.hip): adds the batch's material/texture
 // run lengths (and, with lines, cache-line homogeneities) times its size into acc[0..3], its
 // size into acc[4], and stores the batch's own values in acc[5..8] (9 device doubles).

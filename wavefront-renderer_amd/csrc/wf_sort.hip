@@ -661,7 +661,7 @@ static wf_status sweep_sort(wf_context* ctx, const uint64_t* keys_in, const int3
 
 wf_status radix_sort_packed(wf_context* ctx, const uint64_t* words, uint64_t* scratch,
                             int32_t* order, Count n, int32_t idx_bits, int32_t key_bits,
-                            const char* tag) {
+                            const char* tag, int32_t key_lo) {
     if (n.n <= 0) return WF_OK;
     if (idx_bits < 1 || idx_bits > 31 || key_bits < 0 || idx_bits + key_bits > 64)
         return set_error(WF_ERR_INVALID_ARGUMENT, "radix_sort_packed: %d key bits above %d "
@@ -669,8 +669,11 @@ wf_status radix_sort_packed(wf_context* ctx, const uint64_t* words, uint64_t* sc
     if ((int64_t)n.n > (1ll << idx_bits))
         return set_error(WF_ERR_INVALID_ARGUMENT, "radix_sort_packed: %d elements need more than "
                          "%d index bits", n.n, idx_bits);
-    if (key_bits == 0) return iota_count(ctx, order, n);
-    int lo_bit = idx_bits, hi_bit = idx_bits + key_bits - 1;
+    if (key_lo < 0 || key_lo > key_bits)
+        return set_error(WF_ERR_INVALID_ARGUMENT, "radix_sort_packed: low key bit %d of %d", key_lo,
+                         key_bits);
+    if (key_bits == key_lo) return iota_count(ctx, order, n);
+    int lo_bit = idx_bits + key_lo, hi_bit = idx_bits + key_bits - 1;
     if (!n.dev) {
         // as for pairs: sort only the key bits where the words differ
         void* pr = nullptr;
@@ -685,6 +688,7 @@ wf_status radix_sort_packed(wf_context* ctx, const uint64_t* words, uint64_t* sc
         WF_TRY(readback(ctx, r, pr, sizeof(r)));
         unsigned long long diff = (r[0] ^ r[1]) >> idx_bits;
         if (key_bits < 64) diff &= (1ull << key_bits) - 1ull;
+        diff &= ~((1ull << key_lo) - 1ull);
         if (diff == 0) return iota_count(ctx, order, n);
         lo_bit = idx_bits + __builtin_ctzll(diff);
         hi_bit = idx_bits + 63 - __builtin_clzll(diff);

@@ -64,6 +64,26 @@ __device__ __forceinline__ bool mt_test(const V3& o, const V3& d, const V3& v0, 
     return true;
 }
 
+// 1/b correctly rounded, as the reference's `1.0f / det` (Triangle.h:20). The hardware
+// reciprocal is faithful (error below one ulp); one fma correction y + y (1 - b y) of any faithful
+// reciprocal is the correctly rounded one (Markstein), except when b's significand is all ones,
+// which takes the division, as do results outside the normal range (checked for every
+// significand and every faithful approximation on the host, and for every significand on the
+// device against the division).
+__device__ __forceinline__ float rcp_cr(float b) {
+    const float y = __builtin_amdgcn_rcpf(b);
+    float z = __builtin_fmaf(__builtin_fmaf(-b, y, 1.0f), y, y);
+    const float a = fabsf(b);
+    if ((__float_as_uint(b) & 0x7FFFFFu) == 0x7FFFFFu || !(a >= 0x1p-125f && a <= 0x1p125f)) {
+        // an empty volatile statement keeps the division inside this branch (the compiler would
+        // otherwise evaluate it for every lane and select)
+        float bb = b;
+        asm volatile("" : "+v"(bb));
+        z = 1.0f / bb;
+    }
+    return z;
+}
+
 __device__ __forceinline__ float safe_rcp(float x) {
     float a = fabsf(x) < 1e-20f ? copysignf(1e-20f, x) : x;
     return 1.0f / a;
@@ -100,7 +120,7 @@ __device__ __forceinline__ bool mt_all(const V3& o, const V3& d, const V3& v0, c
     const float EPSILON = 0.0000001f;
     const V3 pvec = d.cross(e2);
     const float det = e1.dot(pvec);
-    const float inv = 1.0f / det;
+    const float inv = rcp_cr(det);
     const V3 tvec = o - v0;
     u = inv * tvec.dot(pvec);
     const V3 qvec = tvec.cross(e1);
