@@ -15,8 +15,9 @@
 // different sizes share one wavefront.
 //
 // One iteration (all counts stay on the device, so the sequence can be captured in a graph):
-//   regenerate -> [radix sort of the rays' sort words + gather] -> closest hit + split ->
-//   [shading policy sort + gather] -> shade (emission, NEE, scatter, RR, depth limit) -> shadow
+//   regenerate -> [radix sort of the rays' sort words (rays then visited through the sort
+//   index)] -> closest hit + split -> [shading policy sort + gather] -> shade (emission, NEE,
+//   scatter, RR, depth limit) -> shadow
 //   rays -> next-ray compaction -> misses -> per-pixel completion (adaptive sampling, sample
 //   budget, requeue).
 #include <algorithm>
@@ -526,6 +527,7 @@ struct wf_engine {
     int32_t* status_host = nullptr;     // pinned: counters + job_left per slot (two buffers)
     hipEvent_t status_ev[2] = {nullptr, nullptr};
     hipEvent_t phase_ev[2][8] = {};    // per status buffer: marks between the phases
+    int phase_set = 0;                 // event set of the iteration being enqueued
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     bool warm = false;                  // a direct iteration has sized every workspace since the
@@ -618,10 +620,8 @@ bool lanes_ok(int32_t sample_lanes) {
     return sample_lanes >= 0 && sample_lanes <= kMaxLanes && (sample_lanes & (sample_lanes - 1)) == 0;
 }
 
-int g_phase_set = 0;   // event set of the iteration being enqueued
-
 void phase_mark(wf_engine* e, int k) {
-    if (e->phase_timing) (void)hipEventRecord(e->phase_ev[g_phase_set][k], e->ctx->stream);
+    if (e->phase_timing) (void)hipEventRecord(e->phase_ev[e->phase_set][k], e->ctx->stream);
 }
 
 // Adds the phase times of the iteration recorded in event set b (already complete).
@@ -658,8 +658,6 @@ wf_status enqueue_iteration(wf_engine* e) {
     }
     phase_mark(e, 1);
     const RayPacked* rq = &e->r0;
-    // sorted rays: only origin and direction (what traversal reads) are gathered; the split
-    // reads the third column and the job tags through the sort order
     // sorted rays are visited in place through the sort order (traversal and split read each
     // ray once, so moving them first would only add a pass)
     const RayPackedOrdered ordered{e->r0.a, e->r0.b, e->r0.c, e->idx};
@@ -782,7 +780,7 @@ void merge_profile(wf_engine* e) {
 }
 
 wf_status launch_iteration(wf_engine* e, int b) {
-    g_phase_set = b;
+    e->phase_set = b;
     const bool graph_ok = e->cfg.use_graph && !e->ctx->profile && !e->phase_timing;
     if (e->graph_exec && e->graph_reuse != (e->reuse_jobs > 0)) {
         // the captured iteration has (or lacks) the requeue pass: capture again, once the
