@@ -67,6 +67,7 @@ def workload_settings(name: str, wf) -> dict:
 
 # Kernels whose bound is HBM bandwidth (sort, compaction, gathers, shading); traversal and the
 # shadow any-hit walks are latency- and divergence-bound and are reported as rays/s.
+POLICY_KERNELS = ("policy_keys", "shading_sort", "gather_shading")
 HBM_KERNELS = ("ray_keys", "ray_sort", "gather_rays", "split", "compact", "policy_keys",
                "shading_sort", "gather_shading", "shade", "next_rays", "miss", "raygen",
                "regen", "finish")
@@ -209,6 +210,25 @@ def measured_traffic(workload: str, kernel: str):
     return None
 
 
+# MI355X: 256 CUs x 4 SIMDs; a wave64 FP32 VALU instruction occupies a 16-lane SIMD for 4 cycles
+# (packed FP32 is not used), at the 2.4 GHz peak engine clock
+VALU_PEAK_GINST = 256 * 4 * 2.4 / 4.0
+
+
+def measured_valu(workload: str, kernel: str):
+    """VALU wave-instructions per launch of `kernel` from the committed rocprofv3 SQ_INSTS_VALU
+    summary of this workload (profiles/valu_<workload>_1080p_64spp.json), or None."""
+    path = ROOT / "profiles" / f"valu_{workload}_1080p_64spp.json"
+    sym = KERNEL_SYMBOLS.get(kernel)
+    if not sym or not path.exists():
+        return None
+    t = json.loads(path.read_text())
+    for name, v in t["kernels"].items():
+        if name.split("<")[0] == sym and "SQ_INSTS_VALU" in v:
+            return {"valu_per_launch": v["SQ_INSTS_VALU"], "source": str(path.relative_to(ROOT))}
+    return None
+
+
 def host_cpu() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -244,7 +264,8 @@ def autotune_batched(wf, ctx, scs, datas, physics, jobs, args) -> list:
                 space.pop(knob)
         res = autotune.tune_engine(
             ctx, scs[k], lambda: [forms._desc(j, datas[k].camera) for j in calib], budget,
-            len(calib), {"physics": physics[k], "firefly": 1e30}, space=space, rounds=1)
+            len(calib), {"physics": physics[k], "firefly": 1e30, "ray_sort": False}, space=space,
+            rounds=1)
         out[k] = {"capacity": res["best"]["capacity"],
                   "sample_lanes": res["best"]["sample_lanes"]}
         log(f"[bench] autotune scene {k}: {res['best']} ({res['trials']} trials)")
@@ -273,7 +294,7 @@ def run_batched(wf, ctx, scenes, args) -> dict:
         else [{"capacity": min(pix[k], 4 << 20), "sample_lanes": 0} for k in range(2)]
     engines = [wf.Engine(ctx, scs[k], capacity=tuned[k]["capacity"], pixel_budget=pix[k],
                          max_jobs=max(1, len(jobs)), physics=physics[k], firefly=1e30,
-                         use_graph=True, sample_lanes=tuned[k]["sample_lanes"])
+                         use_graph=True, sample_lanes=tuned[k]["sample_lanes"], ray_sort=False)
                for k in range(2)]
     res = forms.serve(engines, datas, jobs, realtime=False)
     done = [j.done for j in jobs]
@@ -311,10 +332,12 @@ def run_served(wf, ctx, scenes, args) -> dict:
     capacity = len(calib) / res["makespan_s"]
     rate = 0.7 * capacity
     jobs = forms.served_jobs(args.jobs, rate=rate, max_spp=args.served_spp)
-    res = forms.serve(engines, datas, jobs, realtime=True)
+    res = forms.serve(engines, datas, jobs, realtime=True, collect=True)
     lat = [j.done - j.arrival for j in jobs]
     for e in engines:
         e.close()
+    conv = forms.check_converged(scs, datas, physics, jobs, res["images"])
+    del res["images"]
     secs, rays_seq = forms.sequential(ctx, scs, datas, physics, jobs)
     seq_lat = forms.fcfs_latency([j.arrival for j in jobs], secs)
     out = {"requests": len(jobs), "scenes": ["cornell_box", "sorting_stress", "sphere_cloud"],
@@ -326,7 +349,8 @@ def run_served(wf, ctx, scenes, args) -> dict:
            "sequential_capacity_jobs_s": round(len(jobs) / sum(secs), 2),
            "sequential_p50_s": round(forms.pct(seq_lat, 50), 3),
            "sequential_p99_s": round(forms.pct(seq_lat, 99), 3),
-           "sequential_mrays_s": round(rays_seq / sum(secs) / 1e6, 1)}
+           "sequential_mrays_s": round(rays_seq / sum(secs) / 1e6, 1),
+           "converged": conv}
     log(f"[bench] served: {json.dumps(out)}")
     return out
 
@@ -373,6 +397,11 @@ def main() -> None:
                     help="batched leg: fixed engine settings instead of the autotuner's")
     ap.add_argument("--profile-run", action="store_true",
                     help="only the timed frames (no comparison legs): for profiler runs")
+    ap.add_argument("--ref-spp", type=int, default=1024,
+                    help="samples per pixel of the FP64 oracle image the timed frame's relMSE is "
+                         "measured against (a 128x72 window at the image centre)")
+    ap.add_argument("--no-image-check", action="store_true",
+                    help="skip the relMSE of the timed frame against the oracle")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -487,13 +516,22 @@ def main() -> None:
     if dom:
         ms, launches, nbytes = ctx.profile_query(dom)
         gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-        roofline = {"kernel": dom, "bound": "hbm", "achieved": round(gbs, 1),
-                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
-                    "traffic": None, "avg_launch_us": round(ms * 1e3 / max(launches, 1), 2),
-                    "launches": launches, "bytes_per_launch": round(nbytes / max(launches, 1))}
-        if dom.startswith("traverse") or dom == "shadow":
-            roofline["note"] = ("BVH traversal is latency/divergence-bound; achieved counts only "
-                                "its algorithmic ray in/hit out bytes")
+        avg_us = ms * 1e3 / max(launches, 1)
+        hbm_view = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        roofline = {"kernel": dom, **hbm_view, "traffic": None,
+                    "avg_launch_us": round(avg_us, 2), "launches": launches,
+                    "bytes_per_launch": round(nbytes / max(launches, 1))}
+        valu = measured_valu(args.workload, dom)
+        if valu:
+            # the traversal issues VALU instructions near the rate the SIMDs can take them: that
+            # roof bounds it, not HBM (its algorithmic bytes are rays in and hits out)
+            ginst = valu["valu_per_launch"] / (avg_us * 1e-6) / 1e9
+            roofline.update({"bound": "valu", "achieved": round(ginst, 1),
+                             "peak": VALU_PEAK_GINST, "unit": "G VALU wave-instructions/s",
+                             "frac": round(ginst / VALU_PEAK_GINST, 4),
+                             "valu_per_launch": round(valu["valu_per_launch"]),
+                             "valu_source": valu["source"], "hbm": hbm_view})
         tr = measured_traffic(args.workload, dom)
         if tr:
             roofline["traffic"] = tr["bytes_per_launch"]
@@ -558,6 +596,24 @@ def main() -> None:
                              "shade_ms": round(st.shade_ms, 1)}
             log(f"[bench] policy {pol}: {policies[pol]}")
         ctx.profile(False)
+        # one more frame per ordering policy with events around every kernel: the policy's key,
+        # sort and gather kernels against the HBM roof
+        for pol in ("material", "texture", "costBenefit"):
+            ctx.profile_filter(None)
+            ctx.profile(True)
+            frame(pol, seed=8)
+            rows = {}
+            for name in ctx.profile_names():
+                ms, launches, nbytes = ctx.profile_query(name)
+                if launches == 0 or name not in POLICY_KERNELS:
+                    continue
+                gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+                rows[name] = {"ms_per_frame": round(ms, 3), "launches_per_frame": launches,
+                              "avg_us": round(ms * 1e3 / launches, 2), "GB_s": round(gbs, 1),
+                              "frac": round(gbs / HBM_PEAK_GBS, 4)}
+            policies[pol]["kernels"] = rows
+            ctx.profile(False)
+            log(f"[bench] policy {pol} kernels: {json.dumps(rows)}")
         ctx.profile_filter(None)
 
     cpu = None
@@ -583,6 +639,32 @@ def main() -> None:
                          f"{build_s:.1f}s excluded); host {host_cpu()}, "
                          f"{os.cpu_count()} logical CPUs visible"}
         log(f"[bench] cpu baseline: {cpu}")
+
+    image = None
+    if not args.no_image_check and not args.profile_run:
+        sys.path.insert(0, str(ROOT / "tests"))
+        import oracle_lib  # the checker
+
+        t1 = time.perf_counter()
+        osc_img = oracle_lib.Oracle().scene(data)
+        ww, wh = min(128, W), min(72, H)
+        win = ((W - ww) // 2, (H - wh) // 2, ww, wh)
+        ref_p = wf.render_params(width=W, height=H, spp=args.ref_spp, max_depth=depth,
+                                 camera=data.camera, policy="none", physics=physics,
+                                 firefly=1e30, seed=4242)
+        _, _, ref, _ = osc_img.render_window(ref_p, win, threads=max(1, args.cpu_threads))
+        # the first timed frame again (seed 1), its window
+        img, cnt, _ = wf.render(scene, params(args.policy, seed=1))
+        x0, y0 = win[0], win[1]
+        got = img[y0:y0 + wh, x0:x0 + ww].astype(np.float64)
+        n_spp = float(np.mean(cnt[y0:y0 + wh, x0:x0 + ww]))
+        relmse = float(np.mean((got - ref) ** 2 / (ref ** 2 + 1e-2)))
+        image = {"window": list(win), "reference": f"oracle, FP64 accumulation, {args.ref_spp} spp",
+                 "frame_spp": round(n_spp, 1), "relmse": round(relmse, 6),
+                 "reference_noise_share": round(n_spp / args.ref_spp, 4),
+                 "spp_for_relmse_1e-3": round(n_spp * relmse / 1e-3),
+                 "seconds": round(time.perf_counter() - t1, 1)}
+        log(f"[bench] image check: {image}")
 
     forms_out = None
     if not args.no_forms and not args.profile_run and not args.quick:
@@ -615,6 +697,7 @@ def main() -> None:
                    "bvh_build_ms": round(scene.stats.build_ms, 1), "policies": policies},
         "roofline": roofline,
         "traversal": tstats,
+        "image": image,
         "forms": forms_out,
         "kernels": kernels,
         "cpu_baseline": cpu,

@@ -440,11 +440,13 @@ __global__ void k_fill_u32(uint32_t* dst, uint32_t value, size_t n) {
 
 __global__ void k_set_i32(int32_t* dst, int32_t value) { *dst = value; }
 
+// Streaming copy: one 16-byte element per thread over a grid that covers the buffer (measured on
+// MI355X at 1 GiB: 6.2 TB/s, against 4.3-4.9 TB/s for grid-stride loops of 2048-16384 blocks with
+// one to eight elements per thread, nontemporal stores or not, and 5.2 TB/s for hipMemcpy).
 __global__ __launch_bounds__(256) void k_stream_copy(const float4* __restrict__ src,
                                                      float4* __restrict__ dst, size_t n) {
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-        dst[i] = src[i];
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
 }
 
 // AND and OR of all keys: bits where they differ are the only ones a sort needs to look at.
@@ -752,8 +754,8 @@ extern "C" wf_status wf_stream_copy(wf_context* ctx, void* dst, const void* src,
     WF_REQUIRE(bytes == 0 || (dst && src), "wf_stream_copy: NULL buffer");
     if (bytes == 0) return WF_OK;
     const size_t n = bytes / 16;
-    // 8 resident 256-lane blocks per CU over 256 CUs, or fewer for small copies
-    const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 2048);
+    WF_REQUIRE((n + 255) / 256 <= 0x7FFFFFFFu, "wf_stream_copy: %zu bytes is too large", bytes);
+    const unsigned grid = (unsigned)((n + 255) / 256);
     wf::Timed tm(ctx, "stream_copy", 2.0 * (double)bytes);
     wf::k_stream_copy<<<grid, 256, 0, ctx->stream>>>((const float4*)src, (float4*)dst, n);
     WF_HIP(hipGetLastError());

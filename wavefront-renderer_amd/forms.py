@@ -81,20 +81,21 @@ def _desc(job: Job, camera):
                     adaptive_min_samples=job.min_samples, adaptive_interval=job.interval)
 
 
-def _params(job: Job, camera, physics):
+def _params(job: Job, camera, physics, ray_sort=False):
     return render_params(job.width, job.height, job.spp, job.max_depth, camera, seed=job.seed,
                          job=job.tag, adaptive=job.adaptive, adaptive_threshold=job.threshold,
                          adaptive_min_samples=job.min_samples, adaptive_interval=job.interval,
-                         physics=physics, firefly=1e30)
+                         physics=physics, firefly=1e30, ray_sort=ray_sort)
 
 
-def sequential(ctx, scenes, datas, physics, jobs) -> tuple:
+def sequential(ctx, scenes, datas, physics, jobs, ray_sort=False) -> tuple:
     """wf_render of every job in turn: (per-job seconds, rays)."""
     secs, rays = [], 0
     for j in jobs:
         ctx.synchronize()
         t0 = time.perf_counter()
-        img, cnt, st = render(scenes[j.scene], _params(j, datas[j.scene].camera, physics[j.scene]),
+        img, cnt, st = render(scenes[j.scene],
+                              _params(j, datas[j.scene].camera, physics[j.scene], ray_sort),
                               keep_on_device=True)
         ctx.synchronize()
         secs.append(time.perf_counter() - t0)
@@ -102,9 +103,42 @@ def sequential(ctx, scenes, datas, physics, jobs) -> tuple:
     return secs, rays
 
 
-def make_engines(ctx, scenes, physics, capacity, pixel_budget, max_jobs=64):
+def make_engines(ctx, scenes, physics, capacity, pixel_budget, max_jobs=64, ray_sort=False):
+    """One engine per resident scene. Without ray ordering by default: the queue order of the
+    wavefront changes no result, and ordering each bounce costs more than it saves."""
     return [Engine(ctx, sc, capacity=capacity, pixel_budget=pixel_budget, max_jobs=max_jobs,
-                   physics=ph, firefly=1e30, use_graph=True) for sc, ph in zip(scenes, physics)]
+                   physics=ph, firefly=1e30, use_graph=True, ray_sort=ray_sort)
+            for sc, ph in zip(scenes, physics)]
+
+
+def rel_mse(img, ref) -> float:
+    """mean over pixels and channels of (I - R)^2 / (R^2 + 0.01)."""
+    a = np.asarray(img, np.float64)
+    r = np.asarray(ref, np.float64)
+    return float(np.mean((a - r) ** 2 / (r * r + 1e-2)))
+
+
+def check_converged(scenes, datas, physics, jobs, images, per_scene: int = 2,
+                    ref_spp: int = 1024) -> dict:
+    """relMSE of served results against a fixed-budget render of the same request with another
+    seed at ref_spp samples per pixel (whose own noise adds about spp/ref_spp of the value)."""
+    picked, seen = [], {}
+    for k, j in enumerate(jobs):
+        if seen.get(j.scene, 0) < per_scene:
+            seen[j.scene] = seen.get(j.scene, 0) + 1
+            picked.append(k)
+    vals, spps = [], []
+    for k in picked:
+        j = jobs[k]
+        p = render_params(j.width, j.height, ref_spp, j.max_depth, datas[j.scene].camera,
+                          seed=j.seed + 7919, job=j.tag, physics=physics[j.scene], firefly=1e30)
+        ref, _, _ = render(scenes[j.scene], p)
+        img, cnt, _ = images[k]
+        vals.append(rel_mse(img, ref))
+        spps.append(float(np.mean(cnt)))
+    return {"checked": len(vals), "reference_spp": ref_spp,
+            "relmse": [round(v, 6) for v in vals], "mean_spp": [round(s, 1) for s in spps],
+            "within_target": sum(v <= 1e-3 for v in vals)}
 
 
 def serve(engines, datas, jobs, iterations: int = 2, realtime: bool = True,
