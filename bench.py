@@ -67,8 +67,12 @@ def workload_settings(name: str, wf) -> dict:
 
 # Kernels whose bound is HBM bandwidth (sort, compaction, gathers, shading); traversal and the
 # shadow any-hit walks are latency- and divergence-bound and are reported as rays/s.
-POLICY_KERNELS = ("policy_keys", "shading_sort", "gather_shading")
-HBM_KERNELS = ("ray_keys", "ray_sort", "gather_rays", "split", "compact", "policy_keys",
+# the kernels of a shading policy in the engine (hit counts per material, the range plan, and
+# the split that lays the order out), and those of the sort-and-gather form used by the C-ABI
+POLICY_KERNELS = ("policy_count", "policy_plan", "split", "policy_keys", "shading_sort",
+                  "gather_shading")
+HBM_KERNELS = ("ray_keys", "ray_sort", "gather_rays", "split", "compact", "policy_count",
+               "policy_keys",
                "shading_sort", "gather_shading", "shade", "next_rays", "miss", "raygen",
                "regen", "finish")
 
@@ -596,8 +600,8 @@ def main() -> None:
                              "shade_ms": round(st.shade_ms, 1)}
             log(f"[bench] policy {pol}: {policies[pol]}")
         ctx.profile(False)
-        # one more frame per ordering policy with events around every kernel: the policy's key,
-        # sort and gather kernels against the HBM roof
+        # one more frame per ordering policy with events around every kernel: the policy's
+        # kernels against the HBM roof
         for pol in ("material", "texture", "costBenefit"):
             ctx.profile_filter(None)
             ctx.profile(True)

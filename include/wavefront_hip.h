@@ -524,6 +524,15 @@ wf_status wf_engine_result(wf_engine* engine, int32_t job_id, float* image_rgb,
 wf_status wf_engine_query(wf_engine* engine, wf_engine_stats* out);
 /* Per-phase event timing of every iteration (disables graph replay while on). */
 wf_status wf_engine_phase_timing(wf_engine* engine, int32_t enable);
+/* The shading queue of the engine's last iteration, in the order it was shaded: the order a
+ * shading policy produced (ShadingQueue::schedule + materialize, include/scheduling/
+ * ShadingQueue.h:233-275, applied inside the engine), and the queue the reference's coherence
+ * statistics are computed from (src/render/WavefrontRenderer.cpp:187-246). Copies
+ * min(capacity, length) entries into the non-NULL device columns of dst (columns the engine does
+ * not keep -- ray origin, tri, geom -- are left alone); *n receives the length. Waits for the
+ * engine's stream. */
+wf_status wf_engine_shading_queue(wf_engine* engine, wf_shade_soa* dst, int32_t capacity,
+                                  int32_t* n);
 
 /* ---- launch tuning (the autotuner's knobs; results never depend on them) ------------------ */
 #define WF_BVH_PLOC 0   /* locally-ordered clustering of the Morton-sorted triangles */

@@ -104,7 +104,9 @@ int32_t orc_shading_schedule(int32_t policy, int32_t n, const int32_t* mat, cons
     std::iota(sorted, sorted + n, 0);
     if (policy == WF_POLICY_NONE) return 0;
     auto rel_mat = [&](int32_t m) { return mat_cost ? mat_cost[m] : 1.0; };
-    auto rel_tex = [&](int32_t t) { return tex_cost ? tex_cost[t] : 1.0; };
+    // an id beyond the table is an untracked texture (relativeCost 1.0); the reference indexes
+    // its table with any id >= 0
+    auto rel_tex = [&](int32_t t) { return tex_cost && t < n_tex ? tex_cost[t] : 1.0; };
 
     if (policy == WF_POLICY_COST_BENEFIT) {
         if (n == 0) return 0;

@@ -930,7 +930,7 @@ void orc_shade(void* sp, const wf_shade_soa* q, int32_t n, const wf_shade_params
             float* a = accum_rgb + (size_t)pix * 3;
             double* a64 = accum64 ? accum64 + (size_t)pix * 3 : nullptr;
             V3 o, d, t;
-            float pd;
+            float pd = 0.0f;   // set when the path continues
             alive[i] = shade_one(
                 s, *prm, mat_cost, V3(q->px[i], q->py[i], q->pz[i]),
                 V3(q->nx[i], q->ny[i], q->nz[i]), q->t[i], q->u[i], q->v[i], q->mat[i],

@@ -88,8 +88,8 @@ def test_ray_order_changes_no_result(wf, ctx, cornell, key_bits):
 
 @pytest.mark.parametrize("radix_bits", [6, 11])
 def test_regenerate_equals_passes_radix_widths(wf, ctx, cornell, radix_bits):
-    """Device-count sorts (ray keys and shading keys inside the engine) with other digit widths:
-    the same permutations, so the same image bit for bit."""
+    """Device-count sorts (the ray keys inside the engine) with other digit widths: the same
+    permutations, so the same image bit for bit."""
     data, scene = cornell
     saved = ctx.tuning()
     ctx.set_tuning(radix_bits=radix_bits)
@@ -99,6 +99,71 @@ def test_regenerate_equals_passes_radix_widths(wf, ctx, cornell, radix_bits):
     finally:
         ctx.set_tuning(**saved)
     assert_same(a, b)
+
+
+def luminance32(q):
+    """BT.709 luminance of the throughput columns in the reference's float evaluation order."""
+    return (f32(0.2126) * q["tr"] + f32(0.7152) * q["tg"]) + f32(0.0722) * q["tb"]
+
+
+@pytest.mark.parametrize("policy", ["material", "texture", "costBenefit"])
+def test_engine_policy_order(wf, ctx, scenes, policy):
+    """The engine does not sort for a shading policy: it counts each iteration's hits per
+    material and appends every hit into the range of its group key, the ranges laid out in key
+    order. The queue an iteration shades must therefore already be in the order of the exact
+    stable sort of ShadingQueue::schedule: under material and texture that sort
+    (wf_shading_schedule, on the same entries) returns the identity; under costBenefit the
+    entries follow (type, descending priority), the priority being the queue's own
+    throughput-luminance sum over the material's relative cost (ShadingQueue.h:437-510, the
+    engine's key without the spatial term), and materials of one key share one run."""
+    data = scenes.sorting_stress(subdiv=5, tex_size=64)
+    scene = wf.Scene(ctx, data)
+    rng = np.random.default_rng(17)
+    mat_cost = rng.uniform(0.2, 3.0, len(data.materials))
+    tex_cost = rng.uniform(0.2, 3.0, len(data.textures))
+    scene.set_costs(mat_cost, tex_cost)
+    cap = 1 << 16
+    eng = wf.Engine(ctx, scene, capacity=cap, pixel_budget=1 << 16, policy=policy,
+                    physics=wf.PHYSICS_UNBIASED, firefly=1e30, ray_sort=False)
+    eng.admit(wf.job_desc(96, 72, 64, 12, data.camera, seed=5))
+    mtype = np.array([m["type"] & 7 for m in data.materials])
+    checked = 0
+    for it in range(4):
+        eng.run(max_iterations=1)
+        q = eng.shading_queue(cap)
+        n = len(q["mat"])
+        print(f"{policy}: iteration {it}, {n} shaded hits", flush=True)
+        if n == 0:
+            continue
+        mats = q["mat"]
+        if policy in ("material", "texture"):
+            sq = wf.ShadingQueue(ctx, n)
+            sq.upload(q)
+            assert np.array_equal(wf.shading_schedule(scene, sq, n, policy), np.arange(n))
+        else:
+            lum = luminance32(q).astype(np.float64)
+            benefit = np.bincount(mats, weights=lum, minlength=len(data.materials))
+            tex = np.full(len(data.materials), -1)
+            tex[mats] = q["tex"]
+            cost = mat_cost + np.where(tex >= 0, tex_cost[np.maximum(tex, 0)], 0.0)
+            prio = benefit / np.maximum(cost, 1e-6)
+            ch = np.flatnonzero(mats[1:] != mats[:-1])
+            a, b = mats[ch], mats[ch + 1]
+            # type ascending, then priority descending (sums may differ in the last bits)
+            assert np.all(mtype[a] <= mtype[b])
+            same = mtype[a] == mtype[b]
+            assert np.all(prio[a][same] >= prio[b][same] * (1 - 1e-9))
+            # a material of distinct priority is one run
+            runs = mats[np.r_[0, ch + 1]]
+            uniq, k = np.unique(runs, return_counts=True)
+            for m in uniq[k > 1]:
+                peers = np.flatnonzero((mtype == mtype[m]) & (np.abs(prio - prio[m]) <=
+                                                              1e-9 * prio[m]))
+                assert len(peers) > 1, m
+        checked += 1
+    assert checked >= 3
+    eng.close()
+    scene.close()
 
 
 @pytest.mark.parametrize("lanes", [2, 4, 8, 16])

@@ -547,6 +547,15 @@ class Engine:
     def phase_timing(self, enable: bool):
         check(self._lib.wf_engine_phase_timing(self.handle, 1 if enable else 0))
 
+    def shading_queue(self, capacity: int) -> Dict[str, np.ndarray]:
+        """The last iteration's shading queue in shading order (wf_engine_shading_queue)."""
+        q = ShadingQueue(self.ctx, max(1, capacity))
+        n = C.c_int32()
+        soa = q.soa()
+        check(self._lib.wf_engine_shading_queue(self.handle, C.byref(soa), capacity,
+                                                C.byref(n)))
+        return q.download(min(n.value, capacity))
+
     def close(self):
         if getattr(self, "handle", None):
             self._lib.wf_engine_destroy(self.handle)

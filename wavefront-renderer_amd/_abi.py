@@ -229,6 +229,7 @@ _PROTOS = {
     "wf_engine_result": (_S, [P, C.c_int32, P, P, C.POINTER(JobResult)]),
     "wf_engine_query": (_S, [P, C.POINTER(EngineStats)]),
     "wf_engine_phase_timing": (_S, [P, C.c_int32]),
+    "wf_engine_shading_queue": (_S, [P, C.POINTER(ShadeSoA), C.c_int32, C.POINTER(C.c_int32)]),
     "wf_context_set_tuning": (_S, [P, C.POINTER(Tuning)]),
     "wf_context_get_tuning": (_S, [P, C.POINTER(Tuning)]),
     "wf_profile_enable": (_S, [P, C.c_int32]),

@@ -16,10 +16,12 @@
 //
 // One iteration (all counts stay on the device, so the sequence can be captured in a graph):
 //   regenerate -> [radix sort of the rays' sort words (rays then visited through the sort
-//   index)] -> closest hit + split -> [shading policy sort + gather] -> shade (emission, NEE,
-//   scatter, RR, depth limit) -> shadow
-//   rays -> next-ray compaction -> misses -> per-pixel completion (adaptive sampling, sample
-//   budget, requeue).
+//   index)] -> closest hit, in the same launch as the previous bounce's shadow rays ->
+//   [shading policy: hits counted per material, one range per group key, in key order] ->
+//   split (hits into the shading queue, each into its key's range under a policy; misses into
+//   the miss queue) -> shade (emission, NEE, scatter, RR, depth limit; next
+//   rays appended to the ray queue, shadow rays left for the next closest-hit launch) -> misses
+//   -> per-pixel completion (adaptive sampling, sample budget, requeue).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -510,6 +512,7 @@ struct wf_engine {
     // after which its paths that ended at a hit are finished (pending_tags: their job tags)
     DeferredShadow pending;
     const int32_t* pending_tags = nullptr;
+    const wf_shade_soa* last_q = nullptr;   // the queue the last enqueued iteration shaded
     int32_t* idx = nullptr;
     EngineView view{};
     JobDev* jobs_dev = nullptr;
@@ -666,6 +669,7 @@ wf_status enqueue_iteration(wf_engine* e) {
                                  "ray_sort", 33 - ctx->tuning.ray_key_bits));
     phase_mark(e, 2);
     const unsigned finish_grid = grid_for(cap, kFinishBlock * kFinishItems);
+    bool bucketed = false;
     {
         int32_t* hl;
         float *ht, *hu, *hv;
@@ -687,22 +691,34 @@ wf_status enqueue_iteration(wf_engine* e) {
         } else {
             WF_TRY(trace_closest(ctx, s, rq, live, 0.001f, 1e9f, hl, ht, hu, hv, nullptr));
         }
+        // a shading policy's order: hits appended into the ranges of their group keys
+        PolicyRanges ranges;
+        if (e->cfg.policy != WF_POLICY_NONE) {
+            if (sort)
+                WF_TRY(policy_buckets(ctx, s, &ordered, live, hl, e->cfg.policy, c + C_HITS,
+                                      &ranges, &bucketed));
+            else
+                WF_TRY(policy_buckets(ctx, s, rq, live, hl, e->cfg.policy, c + C_HITS, &ranges,
+                                      &bucketed));
+        }
         if (sort)
             WF_TRY(split_traced(ctx, s, &ordered, live, hl, ht, hu, hv, &e->s0, &e->miss,
-                                c + C_HITS, e->t0, e->st0, e->mt));
+                                c + C_HITS, e->t0, e->st0, e->mt, bucketed ? &ranges : nullptr));
         else
             WF_TRY(split_traced(ctx, s, rq, live, hl, ht, hu, hv, &e->s0, &e->miss, c + C_HITS,
-                                e->t0, e->st0, e->mt));
+                                e->t0, e->st0, e->mt, bucketed ? &ranges : nullptr));
     }
     phase_mark(e, 3);
     const wf_shade_soa* q = &e->s0;
     const int32_t* qt = e->st0;
-    if (e->cfg.policy != WF_POLICY_NONE) {
+    // (a scene with more materials than the ranges support sorts and gathers instead)
+    if (e->cfg.policy != WF_POLICY_NONE && !bucketed) {
         WF_TRY(shading_schedule(ctx, s, &e->s0, hits, e->cfg.policy, e->idx, false));
         WF_TRY(gather_shading(ctx, &e->s0, e->idx, hits, &e->s1, e->st0, e->st1));
         q = &e->s1;
         qt = e->st1;
     }
+    e->last_q = q;
     if (e->coherence_mode)
         WF_TRY(coherence_accumulate(ctx, q->mat, q->tex, hits,
                                     e->coherence_mode >= WF_COHERENCE_CACHE_LINES, e->coherence));
@@ -1236,6 +1252,41 @@ wf_status wf_engine_query(wf_engine* e, wf_engine_stats* out) {
     out->free_pixels = free_px;
     for (int k = 0; k < 5; ++k) out->phase_ms[k] = e->phase_ms[k];
     return WF_OK;
+}
+
+wf_status wf_engine_shading_queue(wf_engine* e, wf_shade_soa* dst, int32_t capacity,
+                                  int32_t* n) {
+    clear_error();
+    WF_REQUIRE(e && dst && n, "wf_engine_shading_queue: NULL argument");
+    WF_REQUIRE(capacity >= 0, "wf_engine_shading_queue: negative capacity");
+    WF_REQUIRE(!e->launched, "wf_engine_shading_queue: a launch is pending (wf_engine_wait first)");
+    hipStream_t st = e->ctx->stream;
+    int32_t len = 0;
+    if (e->last_q) {
+        WF_HIP(hipMemcpyAsync(&len, e->view.counters + C_HITS, sizeof(int32_t),
+                              hipMemcpyDeviceToHost, st));
+        WF_HIP(hipStreamSynchronize(st));
+    }
+    *n = len;
+    const size_t k = (size_t)std::min(len, capacity);
+    if (k > 0) {
+        const wf_shade_soa& q = *e->last_q;
+        auto col = [&](void* to, const void* from, size_t elem) -> wf_status {
+            if (to && from) WF_HIP(hipMemcpyAsync(to, from, elem * k, hipMemcpyDeviceToDevice, st));
+            return WF_OK;
+        };
+        const void* src[] = {q.px, q.py, q.pz, q.nx, q.ny, q.nz, q.t, q.u, q.v, q.tri, q.mat,
+                             q.tex, q.ox, q.oy, q.oz, q.dx, q.dy, q.dz, q.tr, q.tg, q.tb,
+                             q.pixel, q.depth, q.pdf, q.geom};
+        void* to[] = {dst->px, dst->py, dst->pz, dst->nx, dst->ny, dst->nz, dst->t, dst->u,
+                      dst->v, dst->tri, dst->mat, dst->tex, dst->ox, dst->oy, dst->oz, dst->dx,
+                      dst->dy, dst->dz, dst->tr, dst->tg, dst->tb, dst->pixel, dst->depth,
+                      dst->pdf, dst->geom};
+        for (size_t c = 0; c < sizeof(src) / sizeof(src[0]); ++c) WF_TRY(col(to[c], src[c], 4));
+        WF_TRY(col(dst->front, q.front, 1));
+        WF_HIP(hipStreamSynchronize(st));
+    }
+    return check_fault(e->ctx);
 }
 
 wf_status wf_engine_phase_timing(wf_engine* e, int32_t enable) {

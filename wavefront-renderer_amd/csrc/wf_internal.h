@@ -179,6 +179,7 @@ struct SceneDev {
     // their vertex UVs in leaf order, two records per leaf: (u0, v0, u1, v1), (u2, v2, 0, 0), so
     // that a hit reads its shading data next to its triangle
     float4* tri_uv_leaf = nullptr;
+    int32_t* leaf_mat = nullptr;       // material of each leaf triangle (shading-order buckets)
     // per global triangle
     int32_t* tri_geom = nullptr;
     float* tri_uv = nullptr;           // 6 per triangle (zeros without UVs)
@@ -461,17 +462,25 @@ wf_status trace_closest(wf_context* ctx, const wf_scene* s, const RayPackedOrder
 // The context's closest-hit buffers for n rays.
 wf_status hit_buffers(wf_context* ctx, Count n, int32_t** leaf, float** t, float** u, float** v);
 // The hit/miss split of traced packed rays (hits and misses appended at device cursors).
+// Ranges of the shading queue per group key of a shading policy (policy_buckets below).
+constexpr int kPolicyBucketMax = 1024;
+struct PolicyRanges {
+    const int32_t* bucket = nullptr;   // range of each material
+    int32_t* cursor = nullptr;         // next free slot of each range (advanced by the split)
+};
 wf_status split_traced(wf_context* ctx, const wf_scene* s, const RayPacked* rays, Count n,
                        const int32_t* hit_leaf, const float* hit_t, const float* hit_bu,
                        const float* hit_bv, wf_shade_soa* hits_out, wf_ray_soa* misses_out,
                        int32_t* counts_dev, const int32_t* tags_in, int32_t* hit_tags,
-                       int32_t* miss_tags);
+                       int32_t* miss_tags,
+                       const PolicyRanges* ranges = nullptr);
 // The same over rays in a sorted order (tags_in in the unsorted order, read through it).
 wf_status split_traced(wf_context* ctx, const wf_scene* s, const RayPackedOrdered* rays, Count n,
                        const int32_t* hit_leaf, const float* hit_t, const float* hit_bu,
                        const float* hit_bv, wf_shade_soa* hits_out, wf_ray_soa* misses_out,
                        int32_t* counts_dev, const int32_t* tags_in, int32_t* hit_tags,
-                       int32_t* miss_tags);
+                       int32_t* miss_tags,
+                       const PolicyRanges* ranges = nullptr);
 wf_status ray_schedule(wf_context* ctx, const RayPacked* rays, Count n, const float bmin[3],
                        const float bmax[3], int32_t* sorted, uint64_t* keys_out);
 
@@ -562,5 +571,19 @@ wf_status gather_rays(wf_context* ctx, const wf_ray_soa* src, const int32_t* idx
 wf_status gather_shading(wf_context* ctx, const wf_shade_soa* src, const int32_t* idx, Count n,
                          wf_shade_soa* dst, const int32_t* tags_in = nullptr,
                          int32_t* tags_out = nullptr);
+
+// Shading order of the regenerating wavefront without a sort: the engine's queue order changes
+// no result, so a policy's order is produced by appending every hit into the range of its group
+// key, the ranges laid out in key order. Counts the materials of the traced rays' hits (and, for
+// costBenefit, their throughput luminance), computes each material's group key and leaves in
+// `out` the range of every material (bucket, n_materials entries) and the first slot of every
+// range (cursor); hits_dev receives the hit count. False in *used when the scene has no
+// materials or more than kPolicyBucketMax of them (then the caller sorts and gathers instead).
+wf_status policy_buckets(wf_context* ctx, const wf_scene* s, const RayPacked* rays, Count n,
+                         const int32_t* hit_leaf, int32_t policy, int32_t* hits_dev,
+                         PolicyRanges* out, bool* used);
+wf_status policy_buckets(wf_context* ctx, const wf_scene* s, const RayPackedOrdered* rays, Count n,
+                         const int32_t* hit_leaf, int32_t policy, int32_t* hits_dev,
+                         PolicyRanges* out, bool* used);
 
 }  // namespace wf

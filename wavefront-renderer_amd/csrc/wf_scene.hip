@@ -187,14 +187,18 @@ __global__ void k_bvh_gather_tris(const int32_t* __restrict__ order, int n,
                                   const int32_t* __restrict__ tri_geom,
                                   const float* __restrict__ tri_uv, float4* __restrict__ tri,
                                   float4* __restrict__ uv_leaf, float4* __restrict__ slo,
-                                  float4* __restrict__ shi) {
+                                  float4* __restrict__ shi,
+                                  const int32_t* __restrict__ mesh_material,
+                                  int32_t* __restrict__ leaf_mat) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     int g = order[j];
     float4 a = v0[g];
     a.w = __int_as_float(g);
     float4 b = e1[g];
-    b.w = __int_as_float(tri_geom[g]);
+    const int32_t geom = tri_geom[g];
+    b.w = __int_as_float(geom);
+    leaf_mat[j] = mesh_material[geom];
     tri[3 * (size_t)j] = a;
     tri[3 * (size_t)j + 1] = b;
     tri[3 * (size_t)j + 2] = e2[g];
@@ -661,10 +665,13 @@ wf_status wf_accel_build(wf_scene* s, wf_build_stats* stats) {
             d.tri = (float4*)pt;
             WF_TRY(scene_alloc(s, sizeof(float4) * 2 * (size_t)n, &pt));
             d.tri_uv_leaf = (float4*)pt;
+            WF_TRY(scene_alloc(s, sizeof(int32_t) * (size_t)n, &pt));
+            d.leaf_mat = (int32_t*)pt;
         }
         k_bvh_gather_tris<<<grid_for(n, 256), 256, 0, ctx->stream>>>(
             (int32_t*)porder, n, d.tri_v0, d.tri_e1, d.tri_e2, (float4*)plo, (float4*)phi,
-            d.tri_geom, d.tri_uv, d.tri, d.tri_uv_leaf, (float4*)pslo, (float4*)pshi);
+            d.tri_geom, d.tri_uv, d.tri, d.tri_uv_leaf, (float4*)pslo, (float4*)pshi,
+            d.mesh_material, d.leaf_mat);
         WF_HIP(hipGetLastError());
         int32_t depth = 0;
         // a single triangle always gets the four-wide root

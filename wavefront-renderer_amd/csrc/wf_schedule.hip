@@ -7,6 +7,7 @@
 //   costBenefit : type(3 bits) | priorityRank | hit-point Morton (ShadingQueue.h:437-510)
 // typeRank is computed on the device from per-type hit counts / texture bytes; cost and size
 // ranks are dense descending ranks of the host tables; ties keep queue order.
+#include <algorithm>
 #include <mutex>
 
 #include "wf_device.h"
@@ -214,8 +215,7 @@ __global__ void k_type_stats(const int32_t* __restrict__ mat, const int32_t* __r
 
 // computeTypeOrder (ShadingQueue.h:277-329): material = count desc, type desc;
 // texture = bytes desc (equal sums: type desc).
-__global__ void k_type_rank(int policy, TypeStats* st) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ void type_order(int policy, TypeStats* st) {
     int order[8];
     int m = 0;
     for (int t = 0; t < 8; ++t)
@@ -238,6 +238,10 @@ __global__ void k_type_rank(int policy, TypeStats* st) {
     }
     for (int t = 0; t < 8; ++t) st->rank[t] = 7;
     for (int r = 0; r < m; ++r) st->rank[order[r]] = r;
+}
+
+__global__ void k_type_rank(int policy, TypeStats* st) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) type_order(policy, st);
 }
 
 __global__ void k_policy_keys(int policy, const int32_t* __restrict__ mat,
@@ -364,6 +368,14 @@ __global__ __launch_bounds__(kCbBlock) void k_cb_partials(
     }
 }
 
+// Relative cost of texture t in costBenefit's combined cost: 0 without a texture; an id beyond
+// the cost table is an untracked texture, which CostTracker::relativeCost rates 1.0 (the
+// reference indexes its table with any id >= 0).
+__device__ __forceinline__ double texture_cost(const double* __restrict__ tex_cost, int t,
+                                               int n_tex) {
+    return t < 0 ? 0.0 : (t < n_tex ? tex_cost[t] : 1.0);
+}
+
 // priority = benefit / max(relCost(mat) + relCost(tex of mat), 1e-6)   (ShadingQueue.h:458-471)
 // One block per material: thread t sums tiles t, t + 256, ... in order, then a fixed pairwise
 // tree over the threads (the same association on every run).
@@ -373,7 +385,7 @@ __global__ __launch_bounds__(256) void k_cb_priority(const double* __restrict__ 
                                                      const DevMaterial* __restrict__ mats,
                                                      const double* __restrict__ mat_cost,
                                                      const double* __restrict__ tex_cost,
-                                                     double* __restrict__ prio) {
+                                                     int n_tex, double* __restrict__ prio) {
     __shared__ double red[256];
     const int m = blockIdx.x;
     const int used = (load_count(n_host, nd) + kCbTile - 1) / kCbTile;   // tiles with entries
@@ -388,9 +400,7 @@ __global__ __launch_bounds__(256) void k_cb_priority(const double* __restrict__ 
     if (threadIdx.x != 0) return;
     const double s = red[0];
     const double sc = mat_cost[m];
-    const int tid = mats[m].texture_id;
-    const double tc = tid >= 0 ? tex_cost[tid] : 0.0;
-    const double c = sc + tc;
+    const double c = sc + texture_cost(tex_cost, mats[m].texture_id, n_tex);
     prio[m] = s / (c > 1e-6 ? c : 1e-6);
 }
 
@@ -433,6 +443,156 @@ __global__ void k_cb_keys(const int32_t* __restrict__ mat, const float* __restri
     const int m = mat[i];
     keys[i] = ((uint64_t)(mats[m].type & 7) << (rank_bits + 30)) | ((uint64_t)rank[m] << 30) |
               (uint64_t)morton;
+}
+
+// ---- shading order as per-material ranges (regenerating wavefront) ------------------------------
+// Every policy's group key is a function of the hit's material (the engine's costBenefit has no
+// spatial term), so the engine does not sort: it counts the hits per material, gives each
+// distinct group key one range of the shading queue, the ranges in key order, and appends each
+// hit into its key's range (k_split_buckets). Inside a range the order is the append order; the
+// queue order of the wavefront changes no result.
+constexpr int kPolicyCountBlock = 256;
+// copies of the per-material sums: block b adds into copy b % kCountCopies, so that each address
+// takes few atomics (device-scope atomics on one address serialise)
+constexpr int kCountCopies = 16;
+
+// Hits per material of the traced rays (leaf >= 0) and, with benefit_on, the FP64 sums of their
+// throughput luminance (benefit of ShadingQueue.h:458-471); LDS histograms flushed with one atomic
+// per material and block into the block's copy. Dynamic LDS: n_mat doubles (benefit_on) then
+// n_mat counts.
+template <class R>
+__global__ __launch_bounds__(kPolicyCountBlock) void k_policy_count(
+    R rays, int n_host, const int32_t* __restrict__ nd, const int32_t* __restrict__ leaf,
+    const int32_t* __restrict__ leaf_mat, int n_mat, int benefit_on,
+    uint32_t* __restrict__ counts, double* __restrict__ benefit) {
+    extern __shared__ double pc_lds[];
+    double* ben = pc_lds;
+    uint32_t* cnt = (uint32_t*)(pc_lds + (benefit_on ? n_mat : 0));
+    for (int k = threadIdx.x; k < n_mat; k += kPolicyCountBlock) {
+        cnt[k] = 0u;
+        if (benefit_on) ben[k] = 0.0;
+    }
+    __syncthreads();
+    const int n = load_count(n_host, nd);
+    for (int i = blockIdx.x * kPolicyCountBlock + threadIdx.x; i < n;
+         i += gridDim.x * kPolicyCountBlock) {
+        const int L = leaf[i];
+        if (L < 0) continue;
+        const int m = leaf_mat[L];
+        atomicAdd(&cnt[m], 1u);
+        if (benefit_on) atomicAdd(&ben[m], (double)luminance(ld_ray(rays, i).t));
+    }
+    __syncthreads();
+    const size_t copy = (size_t)(blockIdx.x % kCountCopies) * n_mat;
+    for (int k = threadIdx.x; k < n_mat; k += kPolicyCountBlock) {
+        if (cnt[k]) atomicAdd(&counts[copy + k], cnt[k]);
+        if (benefit_on && ben[k] != 0.0) atomicAdd(&benefit[copy + k], ben[k]);
+    }
+}
+
+// Group key of every material under the policy -- the sort keys above without their per-entry
+// part: typeRank | costRank (material), typeRank | sizeRank(texture of the material) (texture),
+// type | priority rank (costBenefit) -- then one range per distinct key among the present
+// materials, in key order: bucket[m] = the range of material m, cursor[b] = the first slot of
+// range b. One block, thread m = material m.
+__global__ __launch_bounds__(kPolicyBucketMax) void k_policy_plan(
+    int policy, int n_mat, const uint32_t* __restrict__ counts, const double* __restrict__ benefit,
+    const DevMaterial* __restrict__ mats, const int32_t* __restrict__ cost_rank, int cost_bits,
+    const int32_t* __restrict__ size_rank, int size_bits, int n_tex,
+    const uint64_t* __restrict__ tex_mem, const double* __restrict__ mat_cost,
+    const double* __restrict__ tex_cost, int rank_bits, int32_t* __restrict__ bucket,
+    int32_t* __restrict__ cursor, int32_t* __restrict__ hits_dev) {
+    __shared__ uint32_t s_cnt[kPolicyBucketMax];
+    __shared__ uint64_t s_key[kPolicyBucketMax];
+    __shared__ double s_prio[kPolicyBucketMax];
+    __shared__ uint8_t s_first[kPolicyBucketMax];
+    __shared__ TypeStats st;
+    __shared__ uint32_t s_total;
+    const int m = threadIdx.x;
+    const bool in = m < n_mat;
+    const bool cb = policy == WF_POLICY_COST_BENEFIT;
+    // every global read up front (independent loads overlap)
+    uint32_t cc[kCountCopies];
+    double bb[kCountCopies];
+#pragma unroll
+    for (int k = 0; k < kCountCopies; ++k) {
+        cc[k] = in ? counts[(size_t)k * n_mat + m] : 0u;
+        bb[k] = in && cb ? benefit[(size_t)k * n_mat + m] : 0.0;
+    }
+    const DevMaterial* mp = mats + (in ? m : 0);
+    const int ty = in ? (mp->type & 7) : 0;
+    const int tex = in ? mp->texture_id : -1;
+    const bool tex_ok = tex >= 0 && tex < n_tex;
+    const int32_t crank = in && policy == WF_POLICY_MATERIAL ? cost_rank[m] : 0;
+    const int32_t srank = in && policy == WF_POLICY_TEXTURE ? size_rank[tex_ok ? tex : n_tex] : 0;
+    const uint64_t tbytes = in && tex_ok && !cb ? tex_mem[tex] : 0ull;
+    const double mcost = in && cb ? mat_cost[m] : 0.0;
+    const double tcost = in && cb ? texture_cost(tex_cost, tex, n_tex) : 0.0;
+    uint32_t c = 0u;
+    double ben = 0.0;
+#pragma unroll
+    for (int k = 0; k < kCountCopies; ++k) {
+        c += cc[k];
+        ben += bb[k];
+    }
+    s_cnt[m] = c;
+    if (m < 8) {
+        st.count[m] = 0u;
+        st.bytes[m] = 0ull;
+    }
+    if (m == 0) s_total = 0u;
+    __syncthreads();
+    if (c) atomicAdd(&s_total, c);
+    if (cb) {
+        if (in) {
+            const double cost = mcost + tcost;
+            s_prio[m] = ben / (cost > 1e-6 ? cost : 1e-6);
+        }
+    } else if (c) {
+        // per-type hit counts and texture bytes (k_type_stats over the hits, grouped by material)
+        atomicAdd(&st.count[ty], c);
+        if (tex_ok) atomicAdd(&st.bytes[ty], (unsigned long long)c * tbytes);
+    }
+    __syncthreads();
+    if (!cb && m == 0) type_order(policy, &st);
+    __syncthreads();
+    uint64_t key = 0;
+    if (in) {
+        if (policy == WF_POLICY_MATERIAL) {
+            key = ((uint64_t)st.rank[ty] << cost_bits) | (uint64_t)crank;
+        } else if (policy == WF_POLICY_TEXTURE) {
+            key = ((uint64_t)st.rank[ty] << size_bits) | (uint64_t)srank;
+        } else {
+            int r = 0;   // competition rank by descending priority among present materials
+            for (int k = 0; k < n_mat; ++k)
+                if (s_cnt[k] && s_prio[k] > s_prio[m]) ++r;
+            key = ((uint64_t)ty << rank_bits) | (uint64_t)r;
+        }
+    }
+    s_key[m] = key;
+    __syncthreads();
+    // the first present material of each key stands for its range
+    bool first = c != 0;
+    if (first)
+        for (int k = 0; k < m; ++k)
+            if (s_cnt[k] && s_key[k] == key) {
+                first = false;
+                break;
+            }
+    s_first[m] = first ? 1 : 0;
+    __syncthreads();
+    if (c) {
+        int b = 0;
+        uint32_t slot = 0;
+        for (int k = 0; k < n_mat; ++k) {
+            if (!s_cnt[k] || s_key[k] >= key) continue;
+            b += s_first[k];
+            slot += s_cnt[k];
+        }
+        bucket[m] = b;
+        if (first) cursor[b] = (int32_t)slot;
+    }
+    if (m == 0) *hits_dev = (int32_t)s_total;
 }
 
 static int bits_for(int max_value) {
@@ -492,7 +652,7 @@ wf_status shading_keys(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q
         WF_HIP(hipGetLastError());
         k_cb_priority<<<nm, 256, 0, ctx->stream>>>((double*)pp, tiles, n.n, n.dev, nm,
                                                     d.materials, d.mat_rel_cost, d.tex_rel_cost,
-                                                    (double*)pprio);
+                                                    d.n_textures, (double*)pprio);
         WF_HIP(hipGetLastError());
         k_cb_rank<<<grid_for(nm, 256), 256, 0, ctx->stream>>>((double*)pprio, (uint32_t*)ppres,
                                                               nm, (int32_t*)prank);
@@ -521,6 +681,60 @@ wf_status shading_schedule(wf_context* ctx, const wf_scene* s, const wf_shade_so
     WF_TRY(shading_keys(ctx, s, q, n, policy, spatial, (uint64_t*)pk, &bits));
     return radix_sort_pairs(ctx, (uint64_t*)pk, nullptr, (uint64_t*)pk2, sorted, n, bits,
                             "shading_sort");
+}
+
+template <class R>
+static wf_status policy_buckets_any(wf_context* ctx, const wf_scene* s, const R& rays, Count n,
+                                    const int32_t* leaf, int32_t policy, int32_t* hits_dev,
+                                    PolicyRanges* out, bool* used) {
+    const SceneDev& d = s->dev;
+    const int nm = d.n_materials;
+    *used = false;
+    if (policy == WF_POLICY_NONE || nm <= 0 || nm > kPolicyBucketMax || !d.leaf_mat) return WF_OK;
+    if (policy < WF_POLICY_NONE || policy > WF_POLICY_COST_BENEFIT)
+        return set_error(WF_ERR_INVALID_ARGUMENT, "unknown scheduling policy %d", policy);
+    const bool cb = policy == WF_POLICY_COST_BENEFIT;
+    void *pc, *pb, *pcur, *pbk;
+    WF_TRY(workspace(ctx, "bucket_counts", sizeof(uint32_t) * kCountCopies * nm, &pc));
+    WF_TRY(workspace(ctx, "bucket_benefit", sizeof(double) * kCountCopies * nm, &pb));
+    WF_TRY(workspace(ctx, "bucket_cursor", sizeof(int32_t) * nm, &pcur));
+    WF_TRY(workspace(ctx, "bucket_of", sizeof(int32_t) * nm, &pbk));
+    WF_TRY(wf_fill_u32(ctx, pc, 0u, (size_t)kCountCopies * nm));
+    if (cb) WF_TRY(wf_fill_u32(ctx, pb, 0u, 2 * (size_t)kCountCopies * nm));
+    {
+        // per ray: its leaf (+ the throughput column for costBenefit)
+        Timed tm(ctx, "policy_count", cb ? 20.0 : 4.0, n);
+        const unsigned grid = std::min(grid_for(n.n, kPolicyCountBlock), (unsigned)ctx->cu_count * 16);
+        const size_t lds = (cb ? sizeof(double) * nm : 0) + sizeof(uint32_t) * nm;
+        k_policy_count<R><<<grid, kPolicyCountBlock, lds, ctx->stream>>>(
+            rays, n.n, n.dev, leaf, d.leaf_mat, nm, cb ? 1 : 0, (uint32_t*)pc, (double*)pb);
+        WF_HIP(hipGetLastError());
+    }
+    {
+        Timed tm(ctx, "policy_plan", 16.0 * nm);
+        k_policy_plan<<<1, kPolicyBucketMax, 0, ctx->stream>>>(
+            policy, nm, (const uint32_t*)pc, (const double*)pb, d.materials, d.mat_cost_rank,
+            d.cost_rank_bits, d.tex_size_rank, d.size_rank_bits, d.n_textures, d.tex_mem_bytes,
+            d.mat_rel_cost, d.tex_rel_cost, nm <= 1 ? 0 : bits_for(nm - 1), (int32_t*)pbk,
+            (int32_t*)pcur, hits_dev);
+        WF_HIP(hipGetLastError());
+    }
+    out->bucket = (const int32_t*)pbk;
+    out->cursor = (int32_t*)pcur;
+    *used = true;
+    return WF_OK;
+}
+
+wf_status policy_buckets(wf_context* ctx, const wf_scene* s, const RayPacked* rays, Count n,
+                         const int32_t* hit_leaf, int32_t policy, int32_t* hits_dev,
+                         PolicyRanges* out, bool* used) {
+    return policy_buckets_any(ctx, s, *rays, n, hit_leaf, policy, hits_dev, out, used);
+}
+
+wf_status policy_buckets(wf_context* ctx, const wf_scene* s, const RayPackedOrdered* rays, Count n,
+                         const int32_t* hit_leaf, int32_t policy, int32_t* hits_dev,
+                         PolicyRanges* out, bool* used) {
+    return policy_buckets_any(ctx, s, *rays, n, hit_leaf, policy, hits_dev, out, used);
 }
 
 template <class R>

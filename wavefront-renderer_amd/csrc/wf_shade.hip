@@ -539,7 +539,7 @@ __global__ void k_shade(const SceneDev s, wf_shade_soa q, int n_host,
         const unsigned long long dt = wall_clock64() - t_start;
         atomicAdd(&o.probe.mat_sum[mat_id], dt);
         atomicAdd(&o.probe.mat_cnt[mat_id], 1u);
-        if (m.texture_id >= 0) {
+        if (m.texture_id >= 0 && m.texture_id < s.n_textures) {   // ids beyond the table: untracked
             atomicAdd(&o.probe.tex_sum[m.texture_id], dt);
             atomicAdd(&o.probe.tex_cnt[m.texture_id], 1u);
         }

@@ -183,6 +183,95 @@ __global__ __launch_bounds__(kAppendBlock) void k_split_append(
     }
 }
 
+// Split with the hits appended into the ranges of their group keys laid out by policy_buckets
+// (the shading order of a policy without a sort). Each block counts its hits per range in LDS,
+// claims one run of slots per range it holds with a single atomic on that range's cursor, and
+// lays its hits out in LDS in range order; thread k then maps the k-th hit of that order, so
+// that consecutive lanes write consecutive slots of one run (coalesced stores) and only the
+// reads of a hit's ray and triangle follow the permutation. Misses are appended as in
+// k_split_append. Dynamic LDS: 3 n_materials words (there are at most n_materials ranges).
+constexpr int kBucketItems = 8;
+constexpr int kBucketTile = kAppendBlock * kBucketItems;
+template <class R>
+__global__ __launch_bounds__(kAppendBlock) void k_split_buckets(
+    const SceneDev s, R in, int n_host, const int32_t* __restrict__ nd,
+    const int32_t* __restrict__ leaf, const float* __restrict__ ht, const float* __restrict__ hu,
+    const float* __restrict__ hv, wf_shade_soa out, wf_ray_soa miss, int32_t* __restrict__ counts,
+    const int32_t* __restrict__ bucket, int32_t* __restrict__ cursor,
+    const int32_t* __restrict__ tags_in, int32_t* __restrict__ hit_tags,
+    int32_t* __restrict__ miss_tags) {
+    extern __shared__ uint32_t sb_lds[];
+    __shared__ uint32_t lds[kBucketItems * kAppendBlock / 64];
+    __shared__ int32_t order[kBucketTile];   // ray of the k-th hit of the block, in range order
+    __shared__ uint16_t order_range[kBucketTile];   // and its range
+    __shared__ uint32_t s_hits;
+    const int n = load_count(n_host, nd);
+    const int base = blockIdx.x * kBucketTile;
+    if (base >= n) return;
+    const int nm = s.n_materials;
+    uint32_t* cnt = sb_lds;              // hits per range in this block
+    uint32_t* local = sb_lds + nm;       // first position of each range in `order`
+    uint32_t* run = sb_lds + 2 * nm;     // first slot of this block's run of each range
+    for (int k = threadIdx.x; k < nm; k += kAppendBlock) cnt[k] = 0u;
+    __syncthreads();
+    bool mis[kBucketItems];
+    int L[kBucketItems], mt[kBucketItems], m[kBucketItems];
+    uint32_t rk[kBucketItems];
+#pragma unroll
+    for (int j = 0; j < kBucketItems; ++j) {
+        const int i = base + j * kAppendBlock + (int)threadIdx.x;
+        L[j] = i < n ? leaf[i] : -1;
+        mis[j] = i < n && L[j] < 0;
+        mt[j] = L[j] >= 0 ? bucket[s.leaf_mat[L[j]]] : 0;
+        rk[j] = L[j] >= 0 ? atomicAdd(&cnt[mt[j]], 1u) : 0u;
+    }
+    __syncthreads();
+    // exclusive scan of the block's counts in range order (wave 0), and the runs
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        uint32_t carry = 0;
+        for (int c0 = 0; c0 < nm; c0 += 64) {
+            const int k = c0 + lane;
+            const uint32_t v = k < nm ? cnt[k] : 0u;
+            uint32_t inc = v;
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(inc, o, 64);
+                if (lane >= o) inc += y;
+            }
+            if (k < nm) local[k] = carry + inc - v;
+            carry += __shfl(inc, 63, 64);
+        }
+        if (lane == 0) s_hits = carry;
+    }
+    for (int k = threadIdx.x; k < nm; k += kAppendBlock)
+        if (cnt[k]) run[k] = (uint32_t)atomicAdd(&cursor[k], (int32_t)cnt[k]);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kBucketItems; ++j)
+        if (L[j] >= 0) {
+            const uint32_t k = local[mt[j]] + rk[j];
+            order[k] = base + j * kAppendBlock + (int)threadIdx.x;
+            order_range[k] = (uint16_t)mt[j];
+        }
+    // (its barriers also publish `order`)
+    block_append_rounds<kBucketItems, kAppendBlock>(&counts[1], mis, m, lds);
+#pragma unroll
+    for (int j = 0; j < kBucketItems; ++j) {
+        if (m[j] < 0) continue;
+        const int i = base + j * kAppendBlock + (int)threadIdx.x;
+        st_ray(miss, m[j], ld_ray(in, i));
+        if (miss_tags) miss_tags[m[j]] = tags_in[ray_source(in, i)];
+    }
+    const int hits = (int)s_hits;
+    for (int k = threadIdx.x; k < hits; k += kAppendBlock) {
+        const int i = order[k];
+        const int mi = order_range[k];
+        const int pos = (int)(run[mi] + ((uint32_t)k - local[mi]));
+        map_hit(s, in, i, leaf[i], ht[i], hu[i], hv[i], out, pos);
+        if (hit_tags) hit_tags[pos] = tags_in[ray_source(in, i)];
+    }
+}
+
 // One HitRecord per input ray (slot i), hit flag per ray.
 __global__ void k_records(const SceneDev s, wf_ray_soa in, int n, const int32_t* __restrict__ leaf,
                           const float* __restrict__ ht, const float* __restrict__ hu,
@@ -295,13 +384,27 @@ wf_status hit_buffers(wf_context* ctx, Count n, int32_t** leaf, float** t, float
     return WF_OK;
 }
 
-// Hits and misses of traced rays appended at device cursors (queue order free).
+// Hits and misses of traced rays appended at device cursors (queue order free). With ranges
+// (from policy_buckets, which has also written the hit count to counts_dev[0]) the hits go into
+// the ranges of their group keys instead.
 template <class R>
 static wf_status split_append(wf_context* ctx, const wf_scene* s, const R& rays, Count n,
                               const int32_t* hl, const float* ht, const float* hu,
                               const float* hv, wf_shade_soa* hits_out, wf_ray_soa* misses_out,
                               int32_t* counts_dev, const int32_t* tags_in, int32_t* hit_tags,
-                              int32_t* miss_tags) {
+                              int32_t* miss_tags, const PolicyRanges* ranges = nullptr) {
+    if (ranges) {
+        WF_TRY(wf_fill_u32(ctx, counts_dev + 1, 0u, 1));
+        if (n.n <= 0) return WF_OK;
+        Timed tm(ctx, "split", 97.0, n);
+        const size_t lds = sizeof(uint32_t) * 3 * (size_t)s->dev.n_materials;
+        k_split_buckets<R><<<grid_for(n.n, kBucketTile), kAppendBlock, lds,
+                             ctx->stream>>>(
+            s->dev, rays, n.n, n.dev, hl, ht, hu, hv, *hits_out, *misses_out, counts_dev,
+            ranges->bucket, ranges->cursor, tags_in, hit_tags, miss_tags);
+        WF_HIP(hipGetLastError());
+        return WF_OK;
+    }
     WF_TRY(wf_fill_u32(ctx, counts_dev, 0u, 2));
     if (n.n > 0) {
         Timed tm(ctx, "split", 97.0, n);
@@ -318,18 +421,18 @@ wf_status split_traced(wf_context* ctx, const wf_scene* s, const RayPacked* rays
                        const int32_t* hit_leaf, const float* hit_t, const float* hit_bu,
                        const float* hit_bv, wf_shade_soa* hits_out, wf_ray_soa* misses_out,
                        int32_t* counts_dev, const int32_t* tags_in, int32_t* hit_tags,
-                       int32_t* miss_tags) {
+                       int32_t* miss_tags, const PolicyRanges* ranges) {
     return split_append(ctx, s, *rays, n, hit_leaf, hit_t, hit_bu, hit_bv, hits_out, misses_out,
-                        counts_dev, tags_in, hit_tags, miss_tags);
+                        counts_dev, tags_in, hit_tags, miss_tags, ranges);
 }
 
 wf_status split_traced(wf_context* ctx, const wf_scene* s, const RayPackedOrdered* rays, Count n,
                        const int32_t* hit_leaf, const float* hit_t, const float* hit_bu,
                        const float* hit_bv, wf_shade_soa* hits_out, wf_ray_soa* misses_out,
                        int32_t* counts_dev, const int32_t* tags_in, int32_t* hit_tags,
-                       int32_t* miss_tags) {
+                       int32_t* miss_tags, const PolicyRanges* ranges) {
     return split_append(ctx, s, *rays, n, hit_leaf, hit_t, hit_bu, hit_bv, hits_out, misses_out,
-                        counts_dev, tags_in, hit_tags, miss_tags);
+                        counts_dev, tags_in, hit_tags, miss_tags, ranges);
 }
 
 // Closest hit, then hits and misses appended at device cursors (queue order free).
