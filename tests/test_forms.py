@@ -2,6 +2,7 @@
 import importlib
 
 import numpy as np
+import pytest
 
 forms = importlib.import_module("wavefront-renderer_amd.forms")
 
@@ -32,3 +33,32 @@ def test_served_jobs_arrivals_and_sizes():
 def test_fcfs_latency():
     lat = forms.fcfs_latency([0.0, 0.5, 0.6, 5.0], [1.0, 1.0, 1.0, 0.5])
     assert np.allclose(lat, [1.0, 1.5, 2.4, 0.5])
+
+
+def test_rel_mse_and_its_trimmed_view():
+    rng = np.random.default_rng(3)
+    ref = rng.uniform(0.1, 2.0, (40, 50, 3))
+    img = ref * (1.0 + 0.01)                              # 1% gain error everywhere
+    want = np.mean((0.01 * ref) ** 2 / (ref * ref + 1e-2))
+    assert np.isclose(forms.rel_mse(img, ref), want, rtol=1e-12)
+    bad = img.copy()
+    bad[3, 7] = 1e4                                       # one firefly pixel
+    assert forms.rel_mse(bad, ref) > 1e3
+    # the trimmed view drops the 0.1% worst pixels (2 of 2000): the firefly is gone
+    assert np.isclose(forms.rel_mse(bad, ref, trim=1e-3), forms.rel_mse(img, ref, trim=1e-3),
+                      rtol=1e-2)
+    assert forms.served_jobs(3, rate=1.0)[0].spp == 1024
+
+
+def test_hdri_sky_has_the_stated_peak_over_mean():
+    """The large scene's sky: the brightest texel is peak_over_mean times the mean texel
+    luminance, at its full size and a reduced one (the sun lobe narrows until the ratio is
+    reachable), with a finite sun scale."""
+    scenes = importlib.import_module("wavefront-renderer_amd.scenes")
+    for size, ratio in (((2048, 1024), 1e4), ((256, 128), 1e4), ((64, 32), 100.0)):
+        env = scenes.hdri_sky(*size, peak_over_mean=ratio)
+        rgb = np.asarray(env["rgb"], np.float64).reshape(-1, 3)
+        lum = 0.2126 * rgb[:, 0] + 0.7152 * rgb[:, 1] + 0.0722 * rgb[:, 2]
+        assert lum.max() / lum.mean() == pytest.approx(ratio, rel=1e-4), size
+        assert 0.0 < env["sun_scale"] < 1e6 and 0.0 < env["sun_sigma"] <= 0.02
+        assert lum.mean() < 1.0                    # sky-level mean, not sun-dominated

@@ -350,9 +350,12 @@ def sorting_stress(seed: int = SEED_BASE + 1, subdiv: int = 8, n_materials: int 
 
 
 def hdri_sky(width: int = 2048, height: int = 1024, peak_over_mean: float = 1e4,
-             sun_dir=(0.45, 0.6, -0.66)) -> dict:
-    """Sky gradient + Gaussian sun lobe, scaled so the brightest texel is peak_over_mean times
-    the mean texel luminance."""
+             sun_dir=(0.45, 0.6, -0.66), sun_sigma: float = 0.02) -> dict:
+    """Sky gradient + Gaussian sun lobe, the sun scaled so that the brightest texel is exactly
+    peak_over_mean times the mean texel luminance. With sky s and lobe l (luminances per texel)
+    and the lobe's brightest texel j, (s_j + k l_j) / mean(s + k l) = R gives
+    k = (R mean(s) - s_j) / (l_j - R mean(l)); a lobe too wide for the ratio (l_j <= R mean(l):
+    its texels carry more than 1/R of the peak on average) is narrowed by halves until it fits."""
     y, x = np.mgrid[0:height, 0:width]
     u = (x + 0.5) / width
     v = (y + 0.5) / height
@@ -364,23 +367,28 @@ def hdri_sky(width: int = 2048, height: int = 1024, peak_over_mean: float = 1e4,
                    np.array([0.12, 0.1, 0.08]) * (1 + d[..., 1:2]))
     s = np.array(sun_dir, np.float64)
     s /= np.linalg.norm(s)
-    cosang = np.clip((d * s).sum(-1), -1, 1)
-    ang = np.arccos(cosang)
-    lobe = np.exp(-0.5 * (ang / 0.02) ** 2)[..., None] * np.array([1.0, 0.95, 0.85])
+    ang = np.arccos(np.clip((d * s).sum(-1), -1, 1))
     lum = lambda c: 0.2126 * c[..., 0] + 0.7152 * c[..., 1] + 0.0722 * c[..., 2]
-    # choose the sun scale k so that max(sky+k*lobe) / mean(sky+k*lobe) = peak_over_mean
-    ls, ll = lum(sky), lum(lobe)
-    lo, hi = 0.0, 1e9
-    for _ in range(200):
-        k = 0.5 * (lo + hi)
-        tot = ls + k * ll
-        if tot.max() / tot.mean() < peak_over_mean:
-            lo = k
-        else:
-            hi = k
-    rgb = (sky + lo * lobe).astype(np.float32)
+    R = float(peak_over_mean)
+    ls = lum(sky)
+    sigma = float(sun_sigma)
+    for _ in range(40):
+        lobe = np.exp(-0.5 * (ang / sigma) ** 2)[..., None] * np.array([1.0, 0.95, 0.85])
+        ll = lum(lobe)
+        j = np.unravel_index(np.argmax(ll), ll.shape)
+        den = ll[j] - R * ll.mean()
+        if den > 0:
+            k = (R * ls.mean() - ls[j]) / den
+            tot = ls + k * ll
+            if k > 0 and abs(tot.max() / tot.mean() - R) <= 1e-6 * R:
+                break
+        sigma *= 0.5
+    else:
+        raise ValueError(f"no sun lobe reaches peak/mean {R} on a {width}x{height} map")
+    rgb = (sky + k * lobe).astype(np.float32)
     return {"width": width, "height": height, "rgb": rgb.reshape(-1), "intensity": 1.0,
-            "tint": (1.0, 1.0, 1.0), "sun_scale": lo, "sun_dir": tuple(s)}
+            "tint": (1.0, 1.0, 1.0), "sun_scale": float(k), "sun_sigma": sigma,
+            "sun_dir": tuple(s)}
 
 
 def sphere_cloud(seed: int = SEED_BASE + 2, copies: int = 4096, n_materials: int = 256,
@@ -432,6 +440,7 @@ def sphere_cloud(seed: int = SEED_BASE + 2, copies: int = 4096, n_materials: int
                      params={"seed": seed, "copies": copies, "uv_sphere": [64, 21],
                              "materials": n_materials, "emitters": n_emitters, "box": box,
                              "env": list(env_size), "sun_scale": env["sun_scale"],
+                             "sun_sigma": env["sun_sigma"],
                              "width": 1920, "height": 1080, "spp_min": 16, "spp_max": 1024,
                              "adaptive_threshold": 0.01, "max_depth": 12,
                              "policy": "costBenefit"})
