@@ -272,6 +272,14 @@ __global__ void k_collapse_write(int count, int next_base, const int32_t* __rest
         }
     }
     if (W == 4) {
+        // an empty slot holds the inverted infinite box (lo = +inf, hi = -inf), which the
+        // four-wide box test rejects for either direction sign without checking the child word
+        const float inf = __int_as_float(0x7F800000);
+        for (int q = 0; q < W; ++q)
+            if (ch[q] == kEmptyChild) {
+                lo[q] = make_float4(inf, inf, inf, 0.0f);
+                hi[q] = make_float4(-inf, -inf, -inf, 0.0f);
+            }
         Bvh4Node nd;
         nd.lox = make_float4(lo[0].x, lo[1].x, lo[2].x, lo[3].x);
         nd.loy = make_float4(lo[0].y, lo[1].y, lo[2].y, lo[3].y);

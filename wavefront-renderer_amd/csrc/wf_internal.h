@@ -150,7 +150,7 @@ struct DevTexture {
 
 // Four-wide node (128 B): child boxes per axis in SoA form, child index >= 0 = wide node,
 // < 0 and != kEmptyChild = ~leaf triangle.
-struct Bvh4Node {
+struct Bvh4Node {   // an empty child slot has lo = +inf, hi = -inf (never entered)
     float4 lox, loy, loz;
     float4 hix, hiy, hiz;
     int4 child;

@@ -320,13 +320,14 @@ __global__ void k_bvh_depth(int n, const int32_t* __restrict__ iparent,
 __global__ void k_bvh4_single(const float4* __restrict__ slo, const float4* __restrict__ shi,
                               Bvh4Node* __restrict__ out) {
     float4 l = slo[0], h = shi[0];
+    const float inf = __int_as_float(0x7F800000);   // empty slots: the inverted infinite box
     Bvh4Node nd;
-    nd.lox = make_float4(l.x, 0, 0, 0);
-    nd.loy = make_float4(l.y, 0, 0, 0);
-    nd.loz = make_float4(l.z, 0, 0, 0);
-    nd.hix = make_float4(h.x, 0, 0, 0);
-    nd.hiy = make_float4(h.y, 0, 0, 0);
-    nd.hiz = make_float4(h.z, 0, 0, 0);
+    nd.lox = make_float4(l.x, inf, inf, inf);
+    nd.loy = make_float4(l.y, inf, inf, inf);
+    nd.loz = make_float4(l.z, inf, inf, inf);
+    nd.hix = make_float4(h.x, -inf, -inf, -inf);
+    nd.hiy = make_float4(h.y, -inf, -inf, -inf);
+    nd.hiz = make_float4(h.z, -inf, -inf, -inf);
     nd.child = make_int4(~0, kEmptyChild, kEmptyChild, kEmptyChild);
     nd.pad = make_int4(0, 0, 0, 0);
     out[0] = nd;
@@ -674,8 +675,10 @@ wf_status wf_accel_build(wf_scene* s, wf_build_stats* stats) {
             d.mesh_material, d.leaf_mat);
         WF_HIP(hipGetLastError());
         int32_t depth = 0;
-        // a single triangle always gets the four-wide root
-        d.width = (n > 1 && ctx->tuning.bvh_width == 8) ? 8 : 4;
+        // a single triangle always gets the four-wide root; four-wide nodes are addressed by
+        // 32-bit byte offsets, so a tree of more than 2^32 / 128 of them is built eight-wide
+        const bool four_fits = (size_t)std::max(1, n - 1) * sizeof(Bvh4Node) <= ((size_t)1 << 32);
+        d.width = (n > 1 && (ctx->tuning.bvh_width == 8 || !four_fits)) ? 8 : 4;
         if (d.width == 4 && !d.nodes4) {
             void* pn4 = nullptr;
             WF_TRY(scene_alloc(s, sizeof(Bvh4Node) * (size_t)std::max(1, n - 1), &pn4));
@@ -740,6 +743,11 @@ wf_status wf_accel_build(wf_scene* s, wf_build_stats* stats) {
             if (d.width == 8 && d.stack_bound > kStackMax) {
                 // an eight-wide node pushes up to seven entries: fall back to four-wide nodes
                 // when the eight-wide stack bound exceeds the largest traversal instantiation
+                if (!four_fits)
+                    return set_error(WF_ERR_UNSUPPORTED,
+                                     "BVH of %d triangles: eight-wide stack bound %d exceeds %d "
+                                     "and four-wide nodes would exceed 4 GiB", n, d.stack_bound,
+                                     kStackMax);
                 d.width = 4;
                 if (!d.nodes4) {
                     void* pn4 = nullptr;

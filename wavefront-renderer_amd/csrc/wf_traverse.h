@@ -102,6 +102,43 @@ __device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, flo
     return t0 <= t1 * 1.00000024f;
 }
 
+// The four child boxes of a four-wide node: entry distance t0[k] (clipped to [0, tmax]) and
+// whether the ray enters the box (2-ulp relative slack, as slab()). Each axis is cut by the plane
+// the ray meets first and the one it meets last: lo then hi for a non-negative inverse direction,
+// hi then lo otherwise. Rounding is monotonic, so these distances are exactly the min and max of
+// slab()'s two, and the result is slab()'s, without the min/max pairs. The node's rows are read
+// at byte offsets from the (uniform) node array: node * 128 + row * 16 with rows lo x, y, z, hi
+// x, y, z; an axis whose inverse direction is negative reads its hi row first. An empty slot
+// holds the inverted infinite box, whose entry distance is +inf and exit distance -inf on every
+// axis.
+__device__ __forceinline__ void boxes4(const char* __restrict__ nb, uint32_t off, const V3& o,
+                                       const V3& inv, float tmax, float (&t0)[4], bool (&in)[4]) {
+    // inv is never zero (safe_rcp), so its sign bit says which plane comes first
+    const uint32_t fxo = (uint32_t)(__float_as_int(inv.x) >> 31) & 48u;
+    const uint32_t fyo = (uint32_t)(__float_as_int(inv.y) >> 31) & 48u;
+    const uint32_t fzo = (uint32_t)(__float_as_int(inv.z) >> 31) & 48u;
+    // (the row constants stay outside the 32-bit sums so that they fold into the loads' offsets;
+    // fxo ^ 48 = 48 - fxo for fxo in {0, 48})
+    const float4 nx = *reinterpret_cast<const float4*>(nb + (size_t)(off + fxo));
+    const float4 ny = *reinterpret_cast<const float4*>(nb + (size_t)(off + fyo) + 16);
+    const float4 nz = *reinterpret_cast<const float4*>(nb + (size_t)(off + fzo) + 32);
+    const float4 ex = *reinterpret_cast<const float4*>(nb + (size_t)(off + (fxo ^ 48u)));
+    const float4 ey = *reinterpret_cast<const float4*>(nb + (size_t)(off + (fyo ^ 48u)) + 16);
+    const float4 ez = *reinterpret_cast<const float4*>(nb + (size_t)(off + (fzo ^ 48u)) + 32);
+    const float nxa[4] = {nx.x, nx.y, nx.z, nx.w}, nya[4] = {ny.x, ny.y, ny.z, ny.w};
+    const float nza[4] = {nz.x, nz.y, nz.z, nz.w}, exa[4] = {ex.x, ex.y, ex.z, ex.w};
+    const float eya[4] = {ey.x, ey.y, ey.z, ey.w}, eza[4] = {ez.x, ez.y, ez.z, ez.w};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float a = fmaxf(fmaxf((nxa[k] - o.x) * inv.x, (nya[k] - o.y) * inv.y),
+                              fmaxf((nza[k] - o.z) * inv.z, 0.0f));
+        const float b = fminf(fminf((exa[k] - o.x) * inv.x, (eya[k] - o.y) * inv.y),
+                              fminf((eza[k] - o.z) * inv.z, tmax));
+        t0[k] = a;
+        in[k] = a <= b * 1.00000024f;
+    }
+}
+
 struct TraceHit {
     float t;
     int32_t leaf;   // leaf (sorted) triangle index, -1 = none
@@ -171,29 +208,22 @@ struct Walker {
         const int lane = threadIdx.x;
         const float kInf = __int_as_float(0x7F800000);
         if (STATS) ++h.nodes;
-        const float4* np = reinterpret_cast<const float4*>(nodes + node);
-        const float4 lox = np[0], loy = np[1], loz = np[2];
-        const float4 hix = np[3], hiy = np[4], hiz = np[5];
-        const int4 ch = *reinterpret_cast<const int4*>(np + 6);
-        const float lx[4] = {lox.x, lox.y, lox.z, lox.w};
-        const float ly[4] = {loy.x, loy.y, loy.z, loy.w};
-        const float lz[4] = {loz.x, loz.y, loz.z, loz.w};
-        const float hx[4] = {hix.x, hix.y, hix.z, hix.w};
-        const float hy[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
-        const float hz[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
+        const char* nb = reinterpret_cast<const char*>(nodes);
+        const uint32_t off = (uint32_t)node * (uint32_t)sizeof(Bvh4Node);
+        const int4 ch = *reinterpret_cast<const int4*>(nb + (size_t)off + 96);
         int c[4] = {ch.x, ch.y, ch.z, ch.w};
         const float tcur = ANY ? tfar : h.t;
+        float t0[4];
+        bool box[4];
+        boxes4(nb, off, o, inv, tcur, t0, box);
         float dist[4];
         bool leaf_hit[4];
         bool any_leaf = false;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            float t0;
-            const bool box = slab(lx[k], ly[k], lz[k], hx[k], hy[k], hz[k], o, inv, tcur, t0) &&
-                             c[k] != kEmptyChild;
-            leaf_hit[k] = box && c[k] < 0;
+            leaf_hit[k] = box[k] && c[k] < 0;
             any_leaf = any_leaf || leaf_hit[k];
-            dist[k] = (box && c[k] >= 0) ? t0 : kInf;
+            dist[k] = (box[k] && c[k] >= 0) ? t0[k] : kInf;
         }
         if (any_leaf) {
             // every candidate triangle of this node in one round trip (non-candidates re-read
@@ -287,25 +317,18 @@ struct Walker {
         uint32_t lm = 0;   // leaf children whose box the ray enters
         if (active) {
             if (STATS) ++h.nodes;
-            const float4* np = reinterpret_cast<const float4*>(nodes + node);
-            const float4 lox = np[0], loy = np[1], loz = np[2];
-            const float4 hix = np[3], hiy = np[4], hiz = np[5];
-            const int4 ch = *reinterpret_cast<const int4*>(np + 6);
-            const float lx[4] = {lox.x, lox.y, lox.z, lox.w};
-            const float ly[4] = {loy.x, loy.y, loy.z, loy.w};
-            const float lz[4] = {loz.x, loz.y, loz.z, loz.w};
-            const float hx[4] = {hix.x, hix.y, hix.z, hix.w};
-            const float hy[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
-            const float hz[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
+            const char* nb = reinterpret_cast<const char*>(nodes);
+            const uint32_t off = (uint32_t)node * (uint32_t)sizeof(Bvh4Node);
+            const int4 ch = *reinterpret_cast<const int4*>(nb + (size_t)off + 96);
             c[0] = ch.x; c[1] = ch.y; c[2] = ch.z; c[3] = ch.w;
             const float tcur = ANY ? tfar : h.t;
+            float t0[4];
+            bool box[4];
+            boxes4(nb, off, o, inv, tcur, t0, box);
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                float t0;
-                const bool box = slab(lx[k], ly[k], lz[k], hx[k], hy[k], hz[k], o, inv, tcur, t0) &&
-                                 c[k] != kEmptyChild;
-                if (box && c[k] < 0) lm |= 1u << k;
-                dist[k] = (box && c[k] >= 0) ? t0 : kInf;
+                if (box[k] && c[k] < 0) lm |= 1u << k;
+                dist[k] = (box[k] && c[k] >= 0) ? t0[k] : kInf;
             }
         }
         // slots of this lane's candidates: [off, off + nc) of T in the wave
@@ -315,21 +338,16 @@ struct Walker {
         const int off = __popcll(b0 & lt) + 2 * __popcll(b1 & lt) + 4 * __popcll(b2 & lt);
         const int T = __popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2);
         if (T > 0) {
-            int cand[4];
-            {
-                uint32_t m = lm;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int k = m ? __ffs(m) - 1 : 0;
-                    cand[q] = ~(k == 0 ? c[0] : (k == 1 ? c[1] : (k == 2 ? c[2] : c[3])));
-                    m &= m - 1u;
-                }
-            }
             for (int base = 0; base < T; base += 64) {
+                // leaf child k of this lane goes to slot off + (leaf children before k)
+                {
+                    int sl = off - base;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int sl = off + q - base;
-                    if (q < nc && sl >= 0 && sl < 64) xchg[sl] = make_int4(wl, cand[q], 0, 0);
+                    for (int k = 0; k < 4; ++k) {
+                        if (!((lm >> k) & 1u)) continue;
+                        if (sl >= 0 && sl < 64) xchg[sl] = make_int4(wl, ~c[k], 0, 0);
+                        ++sl;
+                    }
                 }
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
                 __builtin_amdgcn_wave_barrier();
@@ -353,23 +371,27 @@ struct Walker {
                 if (work) xchg[wl] = res;
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
                 __builtin_amdgcn_wave_barrier();
+                // results in child order (the closest-hit rule is a total order, so the order
+                // only decides which accepted triangle an any-hit walk reports)
+                int sl = off - base;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int sl = off + q - base;
-                    if (!(q < nc && sl >= 0 && sl < 64)) continue;
+                for (int k = 0; k < 4; ++k) {
+                    if (!((lm >> k) & 1u)) continue;
+                    const int slot = sl++;
+                    if (!(slot >= 0 && slot < 64)) continue;
                     if (STATS) ++h.tris;
-                    const int4 r = xchg[sl];
+                    const int4 r = xchg[slot];
                     if (r.x < 0) continue;
                     const float t = __int_as_float(r.y);
                     if (ANY) {
                         if (h.leaf < 0) {
                             h.t = t;
-                            h.leaf = cand[q];
+                            h.leaf = ~c[k];
                             h.g = r.x;
                         }
                     } else if (h.leaf < 0 || t < h.t || (t == h.t && r.x < h.g)) {
                         h.t = t;
-                        h.leaf = cand[q];
+                        h.leaf = ~c[k];
                         h.g = r.x;
                         h.u = __int_as_float(r.z);
                         h.v = __int_as_float(r.w);
