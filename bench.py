@@ -245,8 +245,8 @@ def host_cpu() -> str:
 def autotune_batched(wf, ctx, scs, datas, physics, jobs, args) -> list:
     """Engine capacity and sample lanes per scene, and the context's traversal block shape,
     refill threshold and radix digit width, picked by the autotuner on a calibration subset of
-    each scene's jobs (budgets capped at 64 spp). The context knobs are tuned on the scene with
-    the most samples and kept for both engines."""
+    each scene's jobs (its four largest and two others, budgets capped at 64 spp). The context
+    knobs are tuned on the scene with the most samples and kept for both engines."""
     autotune = importlib.import_module("wavefront-renderer_amd.autotune")
     forms = importlib.import_module("wavefront-renderer_amd.forms")
     from dataclasses import replace
@@ -259,7 +259,12 @@ def autotune_batched(wf, ctx, scs, datas, physics, jobs, args) -> list:
         if not mine:
             out[k] = {"capacity": 1 << 20, "sample_lanes": 0}
             continue
-        calib = [replace(j, spp=min(j.spp, 64)) for j in mine[::max(1, len(mine) // 6)]][:6]
+        # the jobs with the most samples set the makespan: calibrate on the four largest (budgets
+        # capped at 64 spp) and two spread over the rest
+        big = sorted(mine, key=lambda j: -j.pixels * j.spp)
+        rest = big[4:]
+        picked = big[:4] + (rest[::max(1, len(rest) // 2)][:2] if rest else [])
+        calib = [replace(j, spp=min(j.spp, 64)) for j in picked]
         budget = sum(j.pixels for j in calib)
         space = dict(autotune.ENGINE_SPACE)
         if n > 0:
@@ -321,18 +326,21 @@ def run_batched(wf, ctx, scenes, args) -> dict:
 
 def run_served(wf, ctx, scenes, args) -> dict:
     """Config 4: Poisson requests against three resident scenes at ~0.7 of the measured
-    capacity, adaptive sampling to a relMSE target; p50/p99 time-to-converged-image."""
+    capacity (the same requests admitted at once), adaptive sampling to a relMSE target;
+    p50/p99 time-to-converged-image, against the same requests rendered one after another."""
     forms = importlib.import_module("wavefront-renderer_amd.forms")
     datas = [scenes.cornell_box(), scenes.sorting_stress(), scenes.sphere_cloud()]
     scs = [wf.Scene(ctx, d) for d in datas]
     physics = [wf.PHYSICS_REFERENCE, wf.PHYSICS_UNBIASED, wf.PHYSICS_UNBIASED]
-    engines = forms.make_engines(ctx, scs, physics, capacity=8 << 20, pixel_budget=8 << 20,
-                                 max_jobs=64)
-    # capacity: a burst of requests admitted at once
-    calib = forms.served_jobs(args.jobs // 3 or 1, rate=1.0, seed=0x5EED0104,
-                              max_spp=args.served_spp)
-    res = forms.serve(engines, datas, calib, realtime=False)
-    capacity = len(calib) / res["makespan_s"]
+    engines = forms.make_engines(ctx, scs, physics, capacity=args.served_capacity,
+                                 pixel_budget=8 << 20, max_jobs=64)
+    # capacity: the stream's requests admitted at once (a burst); the same requests then arrive
+    # as a Poisson stream at 0.7 of that rate, and are rendered one after another with wf_render
+    burst = forms.served_jobs(args.jobs, rate=1.0, max_spp=args.served_spp)
+    forms.serve(engines, datas, burst[:3], realtime=False)   # warm the engines' graphs
+    burst = forms.served_jobs(args.jobs, rate=1.0, max_spp=args.served_spp)
+    res = forms.serve(engines, datas, burst, realtime=False)
+    capacity = len(burst) / res["makespan_s"]
     rate = 0.7 * capacity
     jobs = forms.served_jobs(args.jobs, rate=rate, max_spp=args.served_spp)
     res = forms.serve(engines, datas, jobs, realtime=True, collect=True)
@@ -392,6 +400,8 @@ def main() -> None:
     ap.add_argument("--served-spp", type=int, default=1024,
                     help="per-pixel sample cap of served requests (adaptive sampling stops each "
                          "pixel at relative standard error sqrt(1e-3) before it)")
+    ap.add_argument("--served-capacity", type=int, default=8 << 20,
+                    help="ray slots of each served scene's engine")
     ap.add_argument("--no-forms", action="store_true",
                     help="skip the batched (config 3) and served (config 4) legs")
     ap.add_argument("--autotune", action="store_true",
