@@ -10,11 +10,10 @@
 // emitted one; surviving next rays are compacted in shading order.
 #include "wf_fmath.h"
 #include "wf_rays.h"
+#include "wf_sky.h"
 #include "wf_traverse.h"
 
 namespace wf {
-
-constexpr float kPi = 3.14159265f;
 
 struct ShadeArgs {
     int32_t physics;
@@ -84,15 +83,6 @@ __device__ __forceinline__ V3 emitted(const DevMaterial& m, const V3& ln, const 
 __device__ __forceinline__ float area_to_angle(float area, float dist, float c) {
     const float safe = smax(fabsf(c), 1e-4f);
     return area * (dist * dist) / safe;
-}
-__device__ __forceinline__ float mis_balance(float a, float b) {
-    const float den = a + b;
-    return den <= 0.0f ? 0.0f : a / den;
-}
-__device__ __forceinline__ float mis_power(float a, float b) {
-    const float a2 = a * a, b2 = b * b;
-    const float den = a2 + b2;
-    return den <= 0.0f ? 0.0f : a2 / den;
 }
 __device__ __forceinline__ V3 vmin(const V3& a, float c) { return V3(smin(a.x, c), smin(a.y, c), smin(a.z, c)); }
 
@@ -292,28 +282,6 @@ __device__ __forceinline__ bool scatter(const SceneDev& s, const DevMaterial& m,
         pdf = -1.0f;
         return false;
     }
-}
-
-// EnvironmentMap::sample mapping (EnvironmentMap.cpp:30-48): nearest texel of the direction.
-__device__ __forceinline__ int env_texel(const SceneDev& s, const V3& dir, float& lat_cos) {
-    const V3 d = dir.normalized();
-    const float u = 0.5f + fm::atan2(d.z, d.x) / (2.0f * 3.14159274f);
-    const float v = 0.5f - fm::asin(smax(-1.0f, smin(1.0f, d.y))) / 3.14159274f;
-    int x = (int)(u * s.env_w);
-    int y = (int)(v * s.env_h);
-    x = x < 0 ? 0 : (x > s.env_w - 1 ? s.env_w - 1 : x);
-    y = y < 0 ? 0 : (y > s.env_h - 1 ? s.env_h - 1 : y);
-    lat_cos = __builtin_sqrtf(smax(0.0f, 1.0f - d.y * d.y));
-    return y * s.env_w + x;
-}
-__device__ __forceinline__ V3 env_radiance(const SceneDev& s, int texel_i) {
-    const float4 r = s.env[texel_i];
-    return V3(r.x, r.y, r.z) * s.env_intensity * V3(s.env_tint[0], s.env_tint[1], s.env_tint[2]);
-}
-__device__ __forceinline__ float env_pdf_sa(const SceneDev& s, int texel_i, float lc) {
-    if (s.env_choice_prob <= 0.0f || lc <= 0.0f) return 0.0f;
-    const float p = s.env_choice_prob * s.env_pdf[texel_i] * (float)s.env_w * (float)s.env_h;
-    return p / (2.0f * kPi * kPi * lc);
 }
 
 // With tags, each entry's job supplies the RNG stream (seed, job id, job-local pixel, sample) and
@@ -659,25 +627,7 @@ __global__ void k_miss(const SceneDev s, wf_ray_soa q, int n_host, const int32_t
                        int physics, float* __restrict__ accum) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= load_count(n_host, nd)) return;
-    const V3 d(q.dx[i], q.dy[i], q.dz[i]);
-    const V3 T(q.tr[i], q.tg[i], q.tb[i]);
-    V3 c;
-    if (s.env_w > 0) {
-        float lc;
-        const int tx = env_texel(s, d, lc);
-        float w = 1.0f;
-        if (physics == WF_PHYSICS_UNBIASED && q.depth[i] > 0 && q.pdf[i] > 0.0f)
-            w = mis_power(q.pdf[i], env_pdf_sa(s, tx, lc));
-        c = physics == WF_PHYSICS_UNBIASED ? T * env_radiance(s, tx) * w : T * env_radiance(s, tx);
-    } else {
-        const float blend = 0.5f * (d.normalized().y + 1.0f);
-        const V3 dark(0.02f, 0.02f, 0.05f), horizon(0.05f, 0.04f, 0.03f);
-        c = T * (horizon * (1.0f - blend) + dark * blend);
-    }
-    float* acc = accum + (size_t)q.pixel[i] * 3;
-    acc[0] = acc[0] + c.x;
-    acc[1] = acc[1] + c.y;
-    acc[2] = acc[2] + c.z;
+    add_miss(s, q, i, physics, accum);
 }
 
 // EmbreeAccelerator::occluded for the NEE rays of each listed entry, then the contributions in
