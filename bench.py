@@ -684,7 +684,8 @@ def main() -> None:
     if not args.no_forms and not args.profile_run and not args.quick:
         forms_out = {"batched": run_batched(wf, ctx, scenes, args),
                      "served": run_served(wf, ctx, scenes, argparse.Namespace(
-                         jobs=max(3, args.jobs * 9 // 16), served_spp=args.served_spp))}
+                         jobs=max(3, args.jobs * 9 // 16), served_spp=args.served_spp,
+                         served_capacity=args.served_capacity))}
 
     out = {
         "metric": "Mrays/s @1080p depth 12",
