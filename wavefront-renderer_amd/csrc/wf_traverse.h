@@ -89,6 +89,13 @@ __device__ __forceinline__ float safe_rcp(float x) {
     return 1.0f / a;
 }
 
+// The float next to f towards +inf (f finite); towards -inf as -next_up(-f).
+__device__ __forceinline__ float next_up(float f) {
+    const uint32_t b = __float_as_uint(f);
+    if ((b & 0x7FFFFFFFu) == 0u) return __uint_as_float(1u);
+    return __uint_as_float((int32_t)b >= 0 ? b + 1u : b - 1u);
+}
+
 // Entry distance of the box [lo, hi] along the ray, clipped to [0, tmax]; true when the
 // interval is non-empty (with a relative slack of 2 ulp so that rounding never drops a box).
 __device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, float hy, float hz,
@@ -103,16 +110,24 @@ __device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, flo
 }
 
 // The four child boxes of a four-wide node: entry distance t0[k] (clipped to [0, tmax]) and
-// whether the ray enters the box (2-ulp relative slack, as slab()). Each axis is cut by the plane
-// the ray meets first and the one it meets last: lo then hi for a non-negative inverse direction,
-// hi then lo otherwise. Rounding is monotonic, so these distances are exactly the min and max of
-// slab()'s two, and the result is slab()'s, without the min/max pairs. The node's rows are read
+// whether the ray enters the box. Each axis is cut by the plane the ray meets first and the one
+// it meets last: lo then hi for a non-negative inverse direction, hi then lo otherwise. A plane
+// at x is met at (x - o) inv = x inv - o inv; with p = o inv rounded, the first plane's distance
+// is fma(x, inv, -next_up(p)) and the last one's fma(x, inv, -next_down(p)) (nup, ndn hold
+// -next_up(p) and -next_down(p)). next_up(p) >= o inv >= next_down(p) exactly and rounding is
+// monotonic, so a computed entry distance is at most the rounded exact (x - o) inv and a computed
+// exit distance at least the rounded exact one: one fma per plane, with no cancellation error.
+// inv is the rounded 1/d (relative error u = 2^-24), so against the true ray o + t d an entry
+// distance is at most (1 + u)^2 and an exit distance at least (1 - u)^2 times the true one; the
+// exit side's factor 1 + 2^-21 >= (1 + u)^2 / (1 - u)^3 (its product rounded once more) makes
+// the test accept every box that the true ray meets inside [0, tmax]. The node's rows are read
 // at byte offsets from the (uniform) node array: node * 128 + row * 16 with rows lo x, y, z, hi
 // x, y, z; an axis whose inverse direction is negative reads its hi row first. An empty slot
 // holds the inverted infinite box, whose entry distance is +inf and exit distance -inf on every
 // axis.
-__device__ __forceinline__ void boxes4(const char* __restrict__ nb, uint32_t off, const V3& o,
-                                       const V3& inv, float tmax, float (&t0)[4], bool (&in)[4]) {
+__device__ __forceinline__ void boxes4(const char* __restrict__ nb, uint32_t off, const V3& inv,
+                                       const V3& nup, const V3& ndn, float tmax, float (&t0)[4],
+                                       bool (&in)[4]) {
     // inv is never zero (safe_rcp), so its sign bit says which plane comes first
     const uint32_t fxo = (uint32_t)(__float_as_int(inv.x) >> 31) & 48u;
     const uint32_t fyo = (uint32_t)(__float_as_int(inv.y) >> 31) & 48u;
@@ -130,12 +145,14 @@ __device__ __forceinline__ void boxes4(const char* __restrict__ nb, uint32_t off
     const float eya[4] = {ey.x, ey.y, ey.z, ey.w}, eza[4] = {ez.x, ez.y, ez.z, ez.w};
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const float a = fmaxf(fmaxf((nxa[k] - o.x) * inv.x, (nya[k] - o.y) * inv.y),
-                              fmaxf((nza[k] - o.z) * inv.z, 0.0f));
-        const float b = fminf(fminf((exa[k] - o.x) * inv.x, (eya[k] - o.y) * inv.y),
-                              fminf((eza[k] - o.z) * inv.z, tmax));
+        const float a = fmaxf(fmaxf(__builtin_fmaf(nxa[k], inv.x, nup.x),
+                                    __builtin_fmaf(nya[k], inv.y, nup.y)),
+                              fmaxf(__builtin_fmaf(nza[k], inv.z, nup.z), 0.0f));
+        const float b = fminf(fminf(__builtin_fmaf(exa[k], inv.x, ndn.x),
+                                    __builtin_fmaf(eya[k], inv.y, ndn.y)),
+                              fminf(__builtin_fmaf(eza[k], inv.z, ndn.z), tmax));
         t0[k] = a;
-        in[k] = a <= b * 1.00000024f;
+        in[k] = a <= b * 1.00000048f;
     }
 }
 
@@ -182,6 +199,7 @@ struct Walker {
     static constexpr int L = lds_entries<STACK>();
     static constexpr int X = STACK - L;
     V3 o, d, inv;
+    V3 nup, ndn;   // -next_up(o inv), -next_down(o inv) per axis (the box test's plane offsets)
     float tnear, tfar;
     TraceHit h;
     int node, sp;
@@ -190,6 +208,9 @@ struct Walker {
         o = ro;
         d = rd;
         inv = V3(safe_rcp(rd.x), safe_rcp(rd.y), safe_rcp(rd.z));
+        const V3 p = ro * inv;
+        nup = V3(-next_up(p.x), -next_up(p.y), -next_up(p.z));
+        ndn = V3(next_up(-p.x), next_up(-p.y), next_up(-p.z));
         tnear = t0;
         tfar = t1;
         h.t = t1;
@@ -215,7 +236,7 @@ struct Walker {
         const float tcur = ANY ? tfar : h.t;
         float t0[4];
         bool box[4];
-        boxes4(nb, off, o, inv, tcur, t0, box);
+        boxes4(nb, off, inv, nup, ndn, tcur, t0, box);
         float dist[4];
         bool leaf_hit[4];
         bool any_leaf = false;
@@ -300,11 +321,14 @@ struct Walker {
 
     // Wave-cooperative step over a four-wide node, called by all 64 lanes of the wave together
     // (active = this lane has a walk in progress). The box tests are per lane as in step(); the
-    // candidate triangles of the whole wave are then compacted into consecutive slots and tested
-    // one (ray, triangle) pair per lane, 64 pairs per round, instead of four masked tests per
-    // lane: a lane with no candidate helps test the others'. xchg: this wave's 64 LDS slots.
-    // Results are the same as step(): every candidate is tested with the same operations and
-    // the closest hit is the (t, triangle) minimum.
+    // hit inner children are ordered and pushed right away, and the candidate triangles of the
+    // whole wave are then compacted into consecutive slots and tested one (ray, triangle) pair
+    // per lane, 64 pairs per round, instead of four masked tests per lane: a lane with no
+    // candidate helps test the others' (the owner's ray comes by lane shuffle). xchg: this wave's
+    // 64 LDS slots. Results are the same as step(): every
+    // candidate is tested with the same operations and the closest hit is the (t, global
+    // triangle index) minimum; a slot's result carries the leaf index, and the global indices
+    // are read from the triangle array only when two accepted t are equal.
     __device__ __forceinline__ bool step_wave(bool active, const Bvh4Node* __restrict__ nodes,
                                               const float4* __restrict__ tri,
                                               int32_t* __restrict__ stack, volatile int32_t* spill,
@@ -313,8 +337,8 @@ struct Walker {
         const int wl = (int)__lane_id();
         const float kInf = __int_as_float(0x7F800000);
         int c[4] = {kEmptyChild, kEmptyChild, kEmptyChild, kEmptyChild};
-        float dist[4] = {kInf, kInf, kInf, kInf};
         uint32_t lm = 0;   // leaf children whose box the ray enters
+        int next = -1;     // the nearest hit inner child (visited next)
         if (active) {
             if (STATS) ++h.nodes;
             const char* nb = reinterpret_cast<const char*>(nodes);
@@ -324,12 +348,46 @@ struct Walker {
             const float tcur = ANY ? tfar : h.t;
             float t0[4];
             bool box[4];
-            boxes4(nb, off, o, inv, tcur, t0, box);
+            boxes4(nb, off, inv, nup, ndn, tcur, t0, box);
+            float dist[4];
+            int ic[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 if (box[k] && c[k] < 0) lm |= 1u << k;
                 dist[k] = (box[k] && c[k] >= 0) ? t0[k] : kInf;
+                ic[k] = c[k];
             }
+            // order the inner children nearest first (misses and leaves sort last as +inf); the
+            // triangle tests below cannot change which of them are visited
+            auto cswap = [&](int x, int y) {
+                if (dist[y] < dist[x]) {
+                    const float td = dist[x];
+                    dist[x] = dist[y];
+                    dist[y] = td;
+                    const int tc = ic[x];
+                    ic[x] = ic[y];
+                    ic[y] = tc;
+                }
+            };
+            if (!ANY) {
+                cswap(0, 1);
+                cswap(2, 3);
+                cswap(0, 2);
+                cswap(1, 3);
+                cswap(1, 2);
+            } else {
+                cswap(0, 1);
+                cswap(0, 2);
+                cswap(0, 3);
+            }
+#pragma unroll
+            for (int k = 3; k >= 1; --k)
+                if (dist[k] < kInf) {
+                    if (X == 0 || sp < L) stack[sp * kTraceBlock + lane] = ic[k];
+                    else spill[sp - L] = ic[k];
+                    ++sp;
+                }
+            if (dist[0] < kInf) next = ic[0];
         }
         // slots of this lane's candidates: [off, off + nc) of T in the wave
         const int nc = __popc(lm);
@@ -363,7 +421,7 @@ struct Walker {
                     const float4 A = tri[li], B = tri[li + 1], C = tri[li + 2];
                     float t, u, v;
                     if (mt_all(ro, rd, xyz(A), xyz(B), xyz(C), r0, r1, t, u, v))
-                        res = make_int4(__float_as_int(A.w), __float_as_int(t), __float_as_int(u),
+                        res = make_int4(job.y, __float_as_int(t), __float_as_int(u),
                                         __float_as_int(v));
                 }
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -371,14 +429,13 @@ struct Walker {
                 if (work) xchg[wl] = res;
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
                 __builtin_amdgcn_wave_barrier();
-                // results in child order (the closest-hit rule is a total order, so the order
+                // results in slot order (the closest-hit rule is a total order, so the order
                 // only decides which accepted triangle an any-hit walk reports)
-                int sl = off - base;
+                const int s0 = off - base;
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
-                    if (!((lm >> k) & 1u)) continue;
-                    const int slot = sl++;
-                    if (!(slot >= 0 && slot < 64)) continue;
+                    const int slot = s0 + k;
+                    if (!(k < nc && slot >= 0 && slot < 64)) continue;
                     if (STATS) ++h.tris;
                     const int4 r = xchg[slot];
                     if (r.x < 0) continue;
@@ -386,13 +443,19 @@ struct Walker {
                     if (ANY) {
                         if (h.leaf < 0) {
                             h.t = t;
-                            h.leaf = ~c[k];
-                            h.g = r.x;
+                            h.leaf = r.x;
                         }
-                    } else if (h.leaf < 0 || t < h.t || (t == h.t && r.x < h.g)) {
+                        continue;
+                    }
+                    bool take = h.leaf < 0 || t < h.t;
+                    if (!take && t == h.t) {
+                        // equal distances: the smaller global triangle index wins
+                        take = __float_as_int(tri[3 * (size_t)r.x].w) <
+                               __float_as_int(tri[3 * (size_t)h.leaf].w);
+                    }
+                    if (take) {
                         h.t = t;
-                        h.leaf = ~c[k];
-                        h.g = r.x;
+                        h.leaf = r.x;
                         h.u = __int_as_float(r.z);
                         h.v = __int_as_float(r.w);
                     }
@@ -403,36 +466,8 @@ struct Walker {
         }
         if (!active) return false;
         if (ANY && h.leaf >= 0) return true;
-        auto cswap = [&](int x, int y) {
-            if (dist[y] < dist[x]) {
-                const float td = dist[x];
-                dist[x] = dist[y];
-                dist[y] = td;
-                const int tc = c[x];
-                c[x] = c[y];
-                c[y] = tc;
-            }
-        };
-        if (!ANY) {
-            cswap(0, 1);
-            cswap(2, 3);
-            cswap(0, 2);
-            cswap(1, 3);
-            cswap(1, 2);
-        } else {
-            cswap(0, 1);
-            cswap(0, 2);
-            cswap(0, 3);
-        }
-#pragma unroll
-        for (int k = 3; k >= 1; --k)
-            if (dist[k] < kInf) {
-                if (X == 0 || sp < L) stack[sp * kTraceBlock + lane] = c[k];
-                else spill[sp - L] = c[k];
-                ++sp;
-            }
-        if (dist[0] < kInf) {
-            node = c[0];
+        if (next >= 0) {
+            node = next;
             return false;
         }
         if (sp == 0) return true;
