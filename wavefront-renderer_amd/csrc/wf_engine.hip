@@ -499,7 +499,13 @@ struct wf_engine {
     std::vector<void*> allocations;
     size_t bytes = 0;
     // device state
-    RayPacked r0{};                     // live rays (packed; visited in sorted order in place)
+    // live rays (packed; visited in sorted order in place) and their job tags, two buffers: an
+    // iteration reads buffer par and appends the next bounce's rays to the other one
+    RayPacked rays[2]{};
+    int32_t* tags[2] = {nullptr, nullptr};
+    int par = 0;
+    int32_t* hit_list = nullptr;        // index mode: traced-ray index of every hit, of every miss
+    int32_t* miss_list = nullptr;
     uint64_t* keys = nullptr;           // sort word of each live ray (written by its producer)
     uint64_t* keys2 = nullptr;          // radix sort scratch
     // coherence metrics of every iteration's shading queue (set by wf_render; 0 = off)
@@ -507,12 +513,14 @@ struct wf_engine {
     double* coherence = nullptr;        // 9 doubles, see coherence_accumulate
     wf_ray_soa miss{};
     wf_shade_soa s0{}, s1{};
-    int32_t *t0 = nullptr, *st0 = nullptr, *st1 = nullptr, *mt = nullptr;
+    int32_t *st0 = nullptr, *st1 = nullptr, *mt = nullptr;
     // the last shaded bounce: its shadow walks run in the next iteration's closest-hit launch,
     // after which its paths that ended at a hit are finished (pending_tags: their job tags)
     DeferredShadow pending;
-    const int32_t* pending_tags = nullptr;
-    const wf_shade_soa* last_q = nullptr;   // the queue the last enqueued iteration shaded
+    // what the last enqueued iteration shaded: its shading queue, or (index mode) its hits
+    const wf_shade_soa* last_q = nullptr;
+    bool last_index = false;
+    HitSource last_hits{};
     int32_t* idx = nullptr;
     EngineView view{};
     JobDev* jobs_dev = nullptr;
@@ -531,8 +539,8 @@ struct wf_engine {
     hipEvent_t status_ev[2] = {nullptr, nullptr};
     hipEvent_t phase_ev[2][8] = {};    // per status buffer: marks between the phases
     int phase_set = 0;                 // event set of the iteration being enqueued
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph[2] = {nullptr, nullptr};          // the captured iteration of each parity
+    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
     bool warm = false;                  // a direct iteration has sized every workspace since the
                                         // last change of the iteration's shape
     bool launched = false;              // wf_engine_launch issued, wf_engine_wait pending
@@ -608,7 +616,9 @@ wf_status engine_array(wf_engine* e, size_t count, T** out) {
 
 // Bytes of device memory per ray slot, per framebuffer pixel and per further sample lane of a
 // pixel (admission accounting).
-constexpr size_t kBytesPerSlot = 48 * 2 + 104 * 2 + 4 * 4 + 400;   // queues + tags + workspaces
+// two ray buffers and the miss queue, two shading queues, tags (two ray buffers, two shading
+// queues, misses), sort index, hit and miss lists, workspaces
+constexpr size_t kBytesPerSlot = 48 * 3 + 104 * 2 + 4 * 5 + 4 + 4 * 2 + 400;
 constexpr size_t kBytesPerPixel = 12 + 12 + 4 + 4 + 4 + 4 + 8;   // accum, start copy, mean, M2,
                                                                   // count, lanes done, queue
 constexpr size_t kBytesPerLane = 12 + 8;                          // accum, queue
@@ -639,7 +649,16 @@ void accumulate_phases(wf_engine* e, int b) {
     }
 }
 
-// One iteration on the engine's stream (no host synchronisation).
+// One iteration on the engine's stream (no host synchronisation), reading the rays of buffer
+// e->par and appending the next bounce's rays to the other buffer.
+//
+// Index mode (the default): the split only lists the traced-ray index of every hit (into the
+// range of its group key under a policy) and of every miss; shading and miss shading read each
+// ray and its closest hit where they are and map the HitRecord in registers, so no shading-queue
+// or miss-queue record is written. The shading-queue mode materialises the queue as the C-ABI
+// does: the engine uses it for coherence statistics (they are taken over the queue) and for
+// policies whose scenes have too many materials for ranges (their order comes from sorting the
+// queue).
 wf_status enqueue_iteration(wf_engine* e) {
     wf_context* ctx = e->ctx;
     const wf_scene* s = e->scene;
@@ -647,33 +666,36 @@ wf_status enqueue_iteration(wf_engine* e) {
     const int cap = e->cap;
     int32_t* c = v.counters;
     const Count live{cap, c + C_LIVE}, hits{cap, c + C_HITS}, misses{cap, c + C_MISSES};
+    const RayPacked cur = e->rays[e->par], nxt = e->rays[e->par ^ 1];
+    int32_t* const ctag = e->tags[e->par];
+    int32_t* const ntag = e->tags[e->par ^ 1];
     // the producers of live rays (regeneration, the next-ray append) write each ray's sort word
     const bool sort = e->cfg.ray_sort != 0;
     const RayKeyFrame kf = ray_key_frame(s->dev.bounds_min, s->dev.bounds_max, index_bits(cap));
     phase_mark(e, 0);
     {
         Timed tm(ctx, "regen", sort ? 64.0 : 56.0, Count{cap, c + C_TAKEN});   // bytes per new path
-        k_regen<<<grid_for(cap, 256), 256, 0, ctx->stream>>>(v, e->r0, e->t0,
+        k_regen<<<grid_for(cap, 256), 256, 0, ctx->stream>>>(v, cur, ctag,
                                                              sort ? e->keys : nullptr, kf);
         WF_HIP(hipGetLastError());
         k_regen_commit<<<1, 1, 0, ctx->stream>>>(v);
         WF_HIP(hipGetLastError());
     }
     phase_mark(e, 1);
-    const RayPacked* rq = &e->r0;
+    const RayPacked* rq = &cur;
     // sorted rays are visited in place through the sort order (traversal and split read each
     // ray once, so moving them first would only add a pass)
-    const RayPackedOrdered ordered{e->r0.a, e->r0.b, e->r0.c, e->idx};
+    const RayPackedOrdered ordered{cur.a, cur.b, cur.c, e->idx};
     if (sort)
         WF_TRY(radix_sort_packed(ctx, e->keys, e->keys2, e->idx, live, kf.idx_bits, 33,
                                  "ray_sort", 33 - ctx->tuning.ray_key_bits));
     phase_mark(e, 2);
     const unsigned finish_grid = grid_for(cap, kFinishBlock * kFinishItems);
     bool bucketed = false;
+    int32_t* hl;
+    float *ht, *hu, *hv;
+    WF_TRY(hit_buffers(ctx, live, &hl, &ht, &hu, &hv));
     {
-        int32_t* hl;
-        float *ht, *hu, *hv;
-        WF_TRY(hit_buffers(ctx, live, &hl, &ht, &hu, &hv));
         const DeferredShadow& d = e->pending;
         if (d.armed) {
             if (sort)
@@ -681,69 +703,96 @@ wf_status enqueue_iteration(wf_engine* e) {
             else
                 WF_TRY(trace_closest_deferred(ctx, s, rq, live, hl, ht, hu, hv, d));
             // the rest of the previous bounce's paths that ended at a hit, now that their shadow
-            // rays are in (its queue, tags and hit count are still those of that bounce)
+            // rays are in (its hit columns and hit count are still those of that bounce)
             Timed tm(ctx, "finish", 12.0, hits);
             k_finish_hits<<<finish_grid, kFinishBlock, 0, ctx->stream>>>(
-                v, d.q.pixel, e->pending_tags, d.o.alive, kFinishRest);
+                v, d.hc.pixel, d.hc.tag, d.o.alive, kFinishRest);
             WF_HIP(hipGetLastError());
         } else if (sort) {
             WF_TRY(trace_closest(ctx, s, &ordered, live, 0.001f, 1e9f, hl, ht, hu, hv, nullptr));
         } else {
             WF_TRY(trace_closest(ctx, s, rq, live, 0.001f, 1e9f, hl, ht, hu, hv, nullptr));
         }
-        // a shading policy's order: hits appended into the ranges of their group keys
-        PolicyRanges ranges;
-        if (e->cfg.policy != WF_POLICY_NONE) {
-            if (sort)
-                WF_TRY(policy_buckets(ctx, s, &ordered, live, hl, e->cfg.policy, c + C_HITS,
-                                      &ranges, &bucketed));
-            else
-                WF_TRY(policy_buckets(ctx, s, rq, live, hl, e->cfg.policy, c + C_HITS, &ranges,
-                                      &bucketed));
-        }
+    }
+    // a shading policy's order: hits appended into the ranges of their group keys
+    PolicyRanges ranges;
+    if (e->cfg.policy != WF_POLICY_NONE) {
         if (sort)
-            WF_TRY(split_traced(ctx, s, &ordered, live, hl, ht, hu, hv, &e->s0, &e->miss,
-                                c + C_HITS, e->t0, e->st0, e->mt, bucketed ? &ranges : nullptr));
+            WF_TRY(policy_buckets(ctx, s, &ordered, live, hl, e->cfg.policy, c + C_HITS,
+                                  &ranges, &bucketed));
         else
-            WF_TRY(split_traced(ctx, s, rq, live, hl, ht, hu, hv, &e->s0, &e->miss, c + C_HITS,
-                                e->t0, e->st0, e->mt, bucketed ? &ranges : nullptr));
+            WF_TRY(policy_buckets(ctx, s, rq, live, hl, e->cfg.policy, c + C_HITS, &ranges,
+                                  &bucketed));
     }
-    phase_mark(e, 3);
-    const wf_shade_soa* q = &e->s0;
-    const int32_t* qt = e->st0;
-    // (a scene with more materials than the ranges support sorts and gathers instead)
-    if (e->cfg.policy != WF_POLICY_NONE && !bucketed) {
-        WF_TRY(shading_schedule(ctx, s, &e->s0, hits, e->cfg.policy, e->idx, false));
-        WF_TRY(gather_shading(ctx, &e->s0, e->idx, hits, &e->s1, e->st0, e->st1));
-        q = &e->s1;
-        qt = e->st1;
-    }
-    e->last_q = q;
-    if (e->coherence_mode)
-        WF_TRY(coherence_accumulate(ctx, q->mat, q->tex, hits,
-                                    e->coherence_mode >= WF_COHERENCE_CACHE_LINES, e->coherence));
-    phase_mark(e, 4);
-    const uint32_t* alive = nullptr;
+    const bool index_mode = !e->coherence_mode && (e->cfg.policy == WF_POLICY_NONE || bucketed);
     ShadeJobs jobs;
-    jobs.tags = qt;
     jobs.jobs = e->jobs_dev;
-    jobs.next_tags = e->t0;
-    jobs.next_packed = e->r0;
+    jobs.next_tags = ntag;
+    jobs.next_packed = nxt;
     jobs.next_keys = sort ? e->keys : nullptr;
     jobs.key_frame = kf;
     jobs.unordered = true;
     jobs.defer = &e->pending;
-    WF_TRY(shade_bounce(ctx, s, q, hits, &e->cfg.shade, v.accum, nullptr, c + C_NEXT, jobs,
-                        &alive));
-    e->pending_tags = qt;
-    if (e->reuse_jobs > 0) {
-        Timed tm(ctx, "finish", 12.0, hits);
-        k_finish_hits<<<finish_grid, kFinishBlock, 0, ctx->stream>>>(v, q->pixel, qt, alive,
-                                                                     kFinishNext);
-        WF_HIP(hipGetLastError());
+    const uint32_t* alive = nullptr;
+    e->last_index = index_mode;
+    if (index_mode) {
+        WF_TRY(split_index(ctx, s, live, hl, e->hit_list, e->miss_list, c + C_HITS,
+                           bucketed ? &ranges : nullptr));
+        phase_mark(e, 3);
+        const HitSource h{cur.a, cur.b, cur.c, sort ? e->idx : nullptr, e->hit_list, hl, ht, hu,
+                          hv, ctag};
+        e->last_hits = h;
+        e->last_q = nullptr;
+        phase_mark(e, 4);
+        HitCols hc;
+        WF_TRY(shade_hits(ctx, s, h, hits, &e->cfg.shade, v.accum, c + C_NEXT, jobs, &alive,
+                          &hc));
+        if (e->reuse_jobs > 0) {
+            Timed tm(ctx, "finish", 12.0, hits);
+            k_finish_hits<<<finish_grid, kFinishBlock, 0, ctx->stream>>>(v, hc.pixel, hc.tag,
+                                                                         alive, kFinishNext);
+            WF_HIP(hipGetLastError());
+        }
+        phase_mark(e, 5);
+        HitSource m = h;
+        m.list = e->miss_list;
+        WF_TRY(shade_misses_index(ctx, s, m, misses, e->cfg.shade.physics, v.accum,
+                                  e->miss.pixel, e->mt));
+    } else {
+        if (sort)
+            WF_TRY(split_traced(ctx, s, &ordered, live, hl, ht, hu, hv, &e->s0, &e->miss,
+                                c + C_HITS, ctag, e->st0, e->mt, bucketed ? &ranges : nullptr));
+        else
+            WF_TRY(split_traced(ctx, s, rq, live, hl, ht, hu, hv, &e->s0, &e->miss, c + C_HITS,
+                                ctag, e->st0, e->mt, bucketed ? &ranges : nullptr));
+        phase_mark(e, 3);
+        const wf_shade_soa* q = &e->s0;
+        const int32_t* qt = e->st0;
+        // (a scene with more materials than the ranges support sorts and gathers instead)
+        if (e->cfg.policy != WF_POLICY_NONE && !bucketed) {
+            WF_TRY(shading_schedule(ctx, s, &e->s0, hits, e->cfg.policy, e->idx, false));
+            WF_TRY(gather_shading(ctx, &e->s0, e->idx, hits, &e->s1, e->st0, e->st1));
+            q = &e->s1;
+            qt = e->st1;
+        }
+        e->last_q = q;
+        if (e->coherence_mode)
+            WF_TRY(coherence_accumulate(ctx, q->mat, q->tex, hits,
+                                        e->coherence_mode >= WF_COHERENCE_CACHE_LINES,
+                                        e->coherence));
+        phase_mark(e, 4);
+        jobs.tags = qt;
+        WF_TRY(shade_bounce(ctx, s, q, hits, &e->cfg.shade, v.accum, nullptr, c + C_NEXT, jobs,
+                            &alive));
+        if (e->reuse_jobs > 0) {
+            Timed tm(ctx, "finish", 12.0, hits);
+            k_finish_hits<<<finish_grid, kFinishBlock, 0, ctx->stream>>>(v, q->pixel, qt, alive,
+                                                                         kFinishNext);
+            WF_HIP(hipGetLastError());
+        }
+        phase_mark(e, 5);
+        WF_TRY(shade_misses(ctx, s, &e->miss, misses, e->cfg.shade.physics, v.accum));
     }
-    phase_mark(e, 5);
-    WF_TRY(shade_misses(ctx, s, &e->miss, misses, e->cfg.shade.physics, v.accum));
     phase_mark(e, 6);
     {
         Timed tm(ctx, "finish", 8.0, misses);
@@ -766,10 +815,12 @@ wf_status enqueue_status(wf_engine* e, int b) {
 }
 
 void drop_graph(wf_engine* e) {
-    if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
-    if (e->graph) (void)hipGraphDestroy(e->graph);
-    e->graph_exec = nullptr;
-    e->graph = nullptr;
+    for (int p = 0; p < 2; ++p) {
+        if (e->graph_exec[p]) (void)hipGraphExecDestroy(e->graph_exec[p]);
+        if (e->graph[p]) (void)hipGraphDestroy(e->graph[p]);
+        e->graph_exec[p] = nullptr;
+        e->graph[p] = nullptr;
+    }
     e->warm = false;
 }
 
@@ -798,15 +849,17 @@ void merge_profile(wf_engine* e) {
 wf_status launch_iteration(wf_engine* e, int b) {
     e->phase_set = b;
     const bool graph_ok = e->cfg.use_graph && !e->ctx->profile && !e->phase_timing;
-    if (e->graph_exec && e->graph_reuse != (e->reuse_jobs > 0)) {
-        // the captured iteration has (or lacks) the requeue pass: capture again, once the
-        // launches of the current graph have completed
+    const int p = e->par;
+    if ((e->graph_exec[0] || e->graph_exec[1]) && e->graph_reuse != (e->reuse_jobs > 0)) {
+        // the captured iterations have (or lack) the requeue pass: capture again, once the
+        // launches of the current graphs have completed
         WF_HIP(hipStreamSynchronize(e->ctx->stream));
         drop_graph(e);
     }
-    if (graph_ok && !e->graph_exec && e->warm) {
-        e->graph_reuse = e->reuse_jobs > 0;
-        // a direct iteration of this shape sized every workspace: capture from here on
+    if (graph_ok && !e->graph_exec[p] && e->warm) {
+        if (!e->graph_exec[p ^ 1]) e->graph_reuse = e->reuse_jobs > 0;
+        // a direct iteration of this shape sized every workspace: capture from here on (one
+        // graph per ray-buffer parity, the buffers being baked into the launches)
         WF_HIP(hipStreamBeginCapture(e->ctx->stream, hipStreamCaptureModeThreadLocal));
         wf_status st = enqueue_iteration(e);
         hipGraph_t g = nullptr;
@@ -816,15 +869,16 @@ wf_status launch_iteration(wf_engine* e, int b) {
             return st;
         }
         if (ce != hipSuccess) return set_error(WF_ERR_HIP, "engine graph capture: %s", hipGetErrorString(ce));
-        e->graph = g;
-        WF_HIP(hipGraphInstantiate(&e->graph_exec, g, nullptr, nullptr, 0));
+        e->graph[p] = g;
+        WF_HIP(hipGraphInstantiate(&e->graph_exec[p], g, nullptr, nullptr, 0));
     }
-    if (graph_ok && e->graph_exec) {
-        WF_HIP(hipGraphLaunch(e->graph_exec, e->ctx->stream));
+    if (graph_ok && e->graph_exec[p]) {
+        WF_HIP(hipGraphLaunch(e->graph_exec[p], e->ctx->stream));
     } else {
         WF_TRY(enqueue_iteration(e));
         e->warm = true;
     }
+    e->par ^= 1;
     ++e->iterations;
     return WF_OK;
 }
@@ -875,7 +929,8 @@ wf_status wf_engine_create(wf_context* ctx, wf_scene* scene, const wf_engine_con
         wf_status s_ = (x);             \
         if (s_ != WF_OK) return fail(s_); \
     } while (0)
-    ETRY(carve_packed(e, cap, &e->r0));
+    ETRY(carve_packed(e, cap, &e->rays[0]));
+    ETRY(carve_packed(e, cap, &e->rays[1]));
     ETRY(carve_rays(e, cap, &e->miss));
     ETRY(carve_shading(e, cap, &e->s0));
     ETRY(carve_shading(e, cap, &e->s1));
@@ -887,7 +942,10 @@ wf_status wf_engine_create(wf_context* ctx, wf_scene* scene, const wf_engine_con
         q->geom = nullptr;
     }
     e->miss.ox = e->miss.oy = e->miss.oz = nullptr;
-    ETRY(engine_array(e, cap, &e->t0));
+    ETRY(engine_array(e, cap, &e->tags[0]));
+    ETRY(engine_array(e, cap, &e->tags[1]));
+    ETRY(engine_array(e, cap, &e->hit_list));
+    ETRY(engine_array(e, cap, &e->miss_list));
     ETRY(engine_array(e, cap, &e->st0));
     ETRY(engine_array(e, cap, &e->st1));
     ETRY(engine_array(e, cap, &e->mt));
@@ -946,8 +1004,10 @@ wf_status wf_engine_create(wf_context* ctx, wf_scene* scene, const wf_engine_con
 wf_status wf_engine_destroy(wf_engine* e) {
     if (!e) return WF_OK;
     if (e->ctx) (void)hipStreamSynchronize(e->ctx->stream);
-    if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
-    if (e->graph) (void)hipGraphDestroy(e->graph);
+    for (int p = 0; p < 2; ++p) {
+        if (e->graph_exec[p]) (void)hipGraphExecDestroy(e->graph_exec[p]);
+        if (e->graph[p]) (void)hipGraphDestroy(e->graph[p]);
+    }
     for (auto& ev : e->status_ev)
         if (ev) (void)hipEventDestroy(ev);
     for (auto& set : e->phase_ev)
@@ -1262,7 +1322,7 @@ wf_status wf_engine_shading_queue(wf_engine* e, wf_shade_soa* dst, int32_t capac
     WF_REQUIRE(!e->launched, "wf_engine_shading_queue: a launch is pending (wf_engine_wait first)");
     hipStream_t st = e->ctx->stream;
     int32_t len = 0;
-    if (e->last_q) {
+    if (e->last_q || e->last_index) {
         WF_HIP(hipMemcpyAsync(&len, e->view.counters + C_HITS, sizeof(int32_t),
                               hipMemcpyDeviceToHost, st));
         WF_HIP(hipStreamSynchronize(st));
@@ -1270,7 +1330,12 @@ wf_status wf_engine_shading_queue(wf_engine* e, wf_shade_soa* dst, int32_t capac
     *n = len;
     const size_t k = (size_t)std::min(len, capacity);
     if (k > 0) {
-        const wf_shade_soa& q = *e->last_q;
+        // an index-mode iteration left its hits where they were traced (its ray buffer and the
+        // hit buffers are intact until the next iteration): map them into a queue first
+        if (e->last_index)
+            WF_TRY(gather_hits(e->ctx, e->scene, e->last_hits, Count{len, nullptr}, &e->s1,
+                               e->st1));
+        const wf_shade_soa& q = e->last_index ? e->s1 : *e->last_q;
         auto col = [&](void* to, const void* from, size_t elem) -> wf_status {
             if (to && from) WF_HIP(hipMemcpyAsync(to, from, elem * k, hipMemcpyDeviceToDevice, st));
             return WF_OK;

@@ -484,6 +484,33 @@ wf_status split_traced(wf_context* ctx, const wf_scene* s, const RayPackedOrdere
 wf_status ray_schedule(wf_context* ctx, const RayPacked* rays, Count n, const float bmin[3],
                        const float bmax[3], int32_t* sorted, uint64_t* keys_out);
 
+// The columns of a shaded entry that the kernels after shading read (next-ray append, shadow
+// walks, path completion): hit point, accumulator index, depth and job tag, by shading position.
+struct HitCols {
+    const float* px = nullptr;
+    const float* py = nullptr;
+    const float* pz = nullptr;
+    const int32_t* pixel = nullptr;
+    const int32_t* depth = nullptr;
+    const int32_t* tag = nullptr;    // NULL without jobs
+};
+
+// Hits shaded in place (the engine's index mode): entry j of the shading order is traced ray
+// list[j] of the packed rays (a, b, c), visited through `order` when it is set, with its closest
+// hit in the hit buffers (leaf, t, u, v by traced-ray index). The same form lists misses.
+struct HitSource {
+    const float4* a = nullptr;
+    const float4* b = nullptr;
+    const float4* c = nullptr;
+    const int32_t* order = nullptr;  // sorted view: traced ray i is slot order[i]; NULL = i
+    const int32_t* list = nullptr;   // traced-ray index of each entry
+    const int32_t* leaf = nullptr;
+    const float* t = nullptr;
+    const float* u = nullptr;
+    const float* v = nullptr;
+    const int32_t* tags = nullptr;   // job tag of each ray slot
+};
+
 // Per-entry outputs of k_shade consumed by the shadow and compaction kernels.
 struct ShadeOut {
     float* ldx; float* ldy; float* ldz; float* lmax;   // area/environment shadow ray
@@ -507,13 +534,16 @@ struct ShadeOut {
     uint64_t* next_keys;                               // sort word of each next ray, when set
     RayKeyFrame key_frame;
     CostProbe probe;                                   // live cost tracking, when set
+    // index-mode shading: the hit columns of every entry (HitCols), for the kernels after it
+    float* hpx; float* hpy; float* hpz;
+    int32_t* hpixel; int32_t* hdepth; int32_t* htag;
 };
 
 // Shadow entries of a shaded bounce whose occlusion walks were handed to the next closest-hit
 // launch instead of a launch of their own: what that launch needs of the bounce.
 struct DeferredShadow {
     bool armed = false;
-    wf_shade_soa q{};             // the bounce's shading queue (hit points, accumulators)
+    HitCols hc{};                 // the bounce's hit points and accumulators
     ShadeOut o{};                 // its shading outputs (shadow rays and contributions)
     const int32_t* list = nullptr;   // shadow entries (shading-queue indices)
     Count n{};                    // their count (device)
@@ -549,6 +579,26 @@ wf_status costs_clear(wf_scene* s);
 wf_status costs_record(wf_scene* s);
 wf_status shade_misses(wf_context* ctx, const wf_scene* s, const wf_ray_soa* q, Count n,
                        int32_t physics, float* accum);
+// Index mode (queue order free, shadow walks deferred): the hits listed by h (h.list) shaded in
+// place; *hc_out receives the hit columns the kernels after shading read.
+wf_status shade_hits(wf_context* ctx, const wf_scene* s, const HitSource& h, Count n,
+                     const wf_shade_params* p, float* accum, int32_t* counts_dev,
+                     const ShadeJobs& jobs, const uint32_t** alive_out, HitCols* hc_out);
+// Index mode: the misses listed by m shaded in place; each entry's accumulator index and job tag
+// are copied to pixel_out / tag_out (by miss position) for path completion.
+wf_status shade_misses_index(wf_context* ctx, const wf_scene* s, const HitSource& m, Count n,
+                             int32_t physics, float* accum, int32_t* pixel_out,
+                             int32_t* tag_out);
+// Index-mode split of traced packed rays: the traced-ray index of every hit appended to
+// hit_list (into the range of its group key with ranges) and of every miss to miss_list;
+// counts_dev[0] = hits, counts_dev[1] = misses.
+wf_status split_index(wf_context* ctx, const wf_scene* s, Count n, const int32_t* hit_leaf,
+                      int32_t* hit_list, int32_t* miss_list, int32_t* counts_dev,
+                      const PolicyRanges* ranges = nullptr);
+// The shading queue of listed hits (HitRecord mapping of each), with their job tags: the
+// engine's diagnostic of an index-mode iteration.
+wf_status gather_hits(wf_context* ctx, const wf_scene* s, const HitSource& h, Count n,
+                      wf_shade_soa* dst, int32_t* tags_out);
 wf_status render_regenerate(wf_scene* s, const wf_render_params* rp, float* image_rgb,
                             int32_t* sample_counts, wf_render_stats* stats);
 // Binary hierarchy by PLOC over n >= 2 Morton-ordered leaf boxes (Karras-layout outputs).

@@ -88,6 +88,57 @@ __device__ __forceinline__ void st_ray(const RayPacked& r, int i, const RayVal& 
     r.c[i] = make_float4(v.t.x, v.t.y, v.t.z, __int_as_float(v.depth));
 }
 
+// HitRecord mapping of EmbreeAccelerator::intersect4 (Embree.cpp:240-252) for a ray (o, d) whose
+// closest hit is leaf triangle L at (t, bu, bv): hit point, the triangle's outward normal turned
+// against the ray (frontFace), UV (Mesh::interpolateUV, src/geometry/Mesh.cpp:46-63, from the
+// UVs stored beside the leaf; the barycentrics without UVs), global triangle, mesh and material.
+struct MappedHit {
+    V3 P, N;
+    float u, v;
+    int32_t g, geom, mat;
+    bool front;
+};
+__device__ __forceinline__ MappedHit map_hit_vals(const SceneDev& s, const V3& o, const V3& d,
+                                                  int L, float t, float bu, float bv) {
+    const float4 A = s.tri[3 * (size_t)L];
+    const float4 B = s.tri[3 * (size_t)L + 1];
+    const float4 C = s.tri[3 * (size_t)L + 2];
+    MappedHit h;
+    h.g = __float_as_int(A.w);
+    h.P = o + d * t;
+    h.geom = __float_as_int(B.w);
+    h.mat = s.mesh_material[h.geom];
+    h.u = bu;
+    h.v = bv;
+    if (s.mesh_has_uv[h.geom]) {
+        const float4 ua = s.tri_uv_leaf[2 * (size_t)L], ub = s.tri_uv_leaf[2 * (size_t)L + 1];
+        const float w0 = 1.0f - bu - bv;
+        h.u = w0 * ua.x + bu * ua.z + bv * ub.x;
+        h.v = w0 * ua.y + bu * ua.w + bv * ub.y;
+    }
+    const V3 outward = xyz(B).cross(xyz(C)).normalized();
+    h.front = d.dot(outward) < 0.0f;
+    h.N = h.front ? outward : outward * -1.0f;
+    return h;
+}
+
+// Ray `e` of a hit source (its list entry) and the slot of its side data.
+__device__ __forceinline__ int hs_traced(const HitSource& h, int e) { return h.list[e]; }
+__device__ __forceinline__ int hs_slot(const HitSource& h, int i) {
+    return h.order ? h.order[i] : i;
+}
+__device__ __forceinline__ RayVal hs_ray(const HitSource& h, int slot) {
+    const float4 a = h.a[slot], b = h.b[slot], c = h.c[slot];
+    RayVal v;
+    v.o = V3(a.x, a.y, a.z);
+    v.pixel = __float_as_int(a.w);
+    v.d = V3(b.x, b.y, b.z);
+    v.pdf = b.w;
+    v.t = V3(c.x, c.y, c.z);
+    v.depth = __float_as_int(c.w);
+    return v;
+}
+
 // spreadBits10 (RayQueue.h:45-55)
 __device__ __forceinline__ uint32_t spread_bits10(uint32_t v) {
     v &= 0x3FFu;

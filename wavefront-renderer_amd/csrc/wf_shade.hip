@@ -284,33 +284,43 @@ __device__ __forceinline__ bool scatter(const SceneDev& s, const DevMaterial& m,
     }
 }
 
-// With tags, each entry's job supplies the RNG stream (seed, job id, job-local pixel, sample) and
-// the depth limit; otherwise they come from ShadeArgs and the caller bounds the depth.
-__global__ void k_shade(const SceneDev s, wf_shade_soa q, int n_host,
-                        const int32_t* __restrict__ nd, ShadeArgs a,
-                        const double* __restrict__ mat_rel_cost, float* __restrict__ accum,
-                        ShadeOut o, const int32_t* __restrict__ tags,
-                        const JobDev* __restrict__ jobs) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= load_count(n_host, nd)) return;
-    const int32_t pixel = q.pixel[i];
-    const int32_t depth = q.depth[i];
-    const float pdf_in = q.pdf[i];
-    const V3 P = ld3(q.px, q.py, q.pz, i);
-    const V3 N = ld3(q.nx, q.ny, q.nz, i);
-    const V3 D = ld3(q.dx, q.dy, q.dz, i);
-    const V3 T(q.tr[i], q.tg[i], q.tb[i]);
-    const float t = q.t[i];
-    const float u = q.u[i], v = q.v[i];
-    const int mat_id = q.mat[i];
-    const bool front = q.front ? q.front[i] != 0 : true;
+// One entry of the shading queue: what shadeAll reads of a hit (ShadingQueue::getHitRecord and
+// the ray's state).
+struct ShadeIn {
+    int32_t pixel, depth;
+    float pdf_in;
+    V3 P, N, D, T;
+    float t, u, v;
+    int32_t mat_id;
+    bool front;
+};
+
+// One entry of shadeAll (cpp:602-797), its outputs at position i of o. With a tag (>= 0) the
+// entry's job supplies the RNG stream (seed, job id, job-local pixel, sample) and the depth
+// limit; otherwise they come from ShadeArgs and the caller bounds the depth.
+__device__ __forceinline__ void shade_entry(const SceneDev& s, const ShadeIn& h, int i,
+                                            const ShadeArgs& a,
+                                            const double* __restrict__ mat_rel_cost,
+                                            float* __restrict__ accum, const ShadeOut& o,
+                                            int32_t tag, const JobDev* __restrict__ jobs) {
+    const int32_t pixel = h.pixel;
+    const int32_t depth = h.depth;
+    const float pdf_in = h.pdf_in;
+    const V3 P = h.P;
+    const V3 N = h.N;
+    const V3 D = h.D;
+    const V3 T = h.T;
+    const float t = h.t;
+    const float u = h.u, v = h.v;
+    const int mat_id = h.mat_id;
+    const bool front = h.front;
     const DevMaterial m = s.materials[mat_id];
     const bool unbiased = a.physics == WF_PHYSICS_UNBIASED;
     uint32_t seed = a.seed, job = (uint32_t)a.job, sample = (uint32_t)a.sample;
     int32_t rng_pixel = pixel;
     int32_t depth_limit = 0x7FFFFFFF;
-    if (tags) {
-        const int32_t tg = tags[i];
+    if (tag >= 0) {
+        const int32_t tg = tag;
         const JobDev& J = jobs[tg >> kTagJobShift];
         seed = J.seed;
         job = (uint32_t)J.tag;
@@ -537,14 +547,71 @@ __global__ void k_shade(const SceneDev s, wf_shade_soa q, int n_host,
     }
 }
 
+// Shading-queue entries (the C-ABI and pass-loop layout).
+__global__ void k_shade(const SceneDev s, wf_shade_soa q, int n_host,
+                        const int32_t* __restrict__ nd, ShadeArgs a,
+                        const double* __restrict__ mat_rel_cost, float* __restrict__ accum,
+                        ShadeOut o, const int32_t* __restrict__ tags,
+                        const JobDev* __restrict__ jobs) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= load_count(n_host, nd)) return;
+    ShadeIn h;
+    h.pixel = q.pixel[i];
+    h.depth = q.depth[i];
+    h.pdf_in = q.pdf[i];
+    h.P = ld3(q.px, q.py, q.pz, i);
+    h.N = ld3(q.nx, q.ny, q.nz, i);
+    h.D = ld3(q.dx, q.dy, q.dz, i);
+    h.T = V3(q.tr[i], q.tg[i], q.tb[i]);
+    h.t = q.t[i];
+    h.u = q.u[i];
+    h.v = q.v[i];
+    h.mat_id = q.mat[i];
+    h.front = q.front ? q.front[i] != 0 : true;
+    shade_entry(s, h, i, a, mat_rel_cost, accum, o, tags ? tags[i] : -1, jobs);
+}
+
+// Index mode: entry j is traced ray h.list[j], mapped (HitRecord) from the ray and its closest
+// hit in registers, then shaded; its hit columns are written for the kernels after shading.
+__global__ void k_shade_hits(const SceneDev s, HitSource src, int n_host,
+                             const int32_t* __restrict__ nd, ShadeArgs a,
+                             const double* __restrict__ mat_rel_cost, float* __restrict__ accum,
+                             ShadeOut o, const JobDev* __restrict__ jobs) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= load_count(n_host, nd)) return;
+    const int i = hs_traced(src, j);
+    const int slot = hs_slot(src, i);
+    const RayVal ray = hs_ray(src, slot);
+    const float t = src.t[i];
+    const MappedHit m = map_hit_vals(s, ray.o, ray.d, src.leaf[i], t, src.u[i], src.v[i]);
+    const int32_t tag = src.tags ? src.tags[slot] : -1;
+    ShadeIn h;
+    h.pixel = ray.pixel;
+    h.depth = ray.depth;
+    h.pdf_in = ray.pdf;
+    h.P = m.P;
+    h.N = m.N;
+    h.D = ray.d;
+    h.T = ray.t;
+    h.t = t;
+    h.u = m.u;
+    h.v = m.v;
+    h.mat_id = m.mat;
+    h.front = m.front;
+    st3(o.hpx, o.hpy, o.hpz, j, m.P);
+    o.hpixel[j] = ray.pixel;
+    o.hdepth[j] = ray.depth;
+    if (o.htag) o.htag[j] = tag;
+    shade_entry(s, h, j, a, mat_rel_cost, accum, o, tag, jobs);
+}
+
 // Unordered compaction after k_shade: shadow entries and next rays appended at device cursors
 // (kAppendItems entries per thread in coalesced rounds, one atomic per block and cursor), shadow
 // rays counted (one atomic per block).
 constexpr int kAppendBlock = 1024;
 constexpr int kAppendItems = 4;
-__global__ __launch_bounds__(kAppendBlock) void k_shade_append(wf_shade_soa q, ShadeOut o, int n_host,
-                                                               const int32_t* __restrict__ nd,
-                                                               const int32_t* __restrict__ tags) {
+__global__ __launch_bounds__(kAppendBlock) void k_shade_append(HitCols q, ShadeOut o, int n_host,
+                                                               const int32_t* __restrict__ nd) {
     __shared__ uint32_t lds[kAppendItems * kAppendBlock / 64];
     __shared__ uint32_t block_rays;
     const int n = load_count(n_host, nd);
@@ -581,7 +648,7 @@ __global__ __launch_bounds__(kAppendBlock) void k_shade_append(wf_shade_soa q, S
             r.pdf = o.npdf[i];
             if (o.next_packed.a) st_ray(o.next_packed, k[j], r);
             else st_ray(o.next, k[j], r);
-            if (o.next_tags) o.next_tags[k[j]] = tags[i];
+            if (o.next_tags) o.next_tags[k[j]] = q.tag[i];
             if (o.next_keys) o.next_keys[k[j]] = ray_sort_word(r.o, r.d, o.key_frame, k[j]);
         }
     }
@@ -627,7 +694,25 @@ __global__ void k_miss(const SceneDev s, wf_ray_soa q, int n_host, const int32_t
                        int physics, float* __restrict__ accum) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= load_count(n_host, nd)) return;
-    add_miss(s, q, i, physics, accum);
+    add_miss(s, V3(q.dx[i], q.dy[i], q.dz[i]), V3(q.tr[i], q.tg[i], q.tb[i]), q.pixel[i],
+             q.depth[i], q.pdf[i], physics, accum);
+}
+
+// Index mode: miss j is traced ray m.list[j], read in place; its accumulator index and job tag
+// are kept for path completion.
+__global__ void k_miss_index(const SceneDev s, HitSource m, int n_host,
+                             const int32_t* __restrict__ nd, int physics,
+                             float* __restrict__ accum, int32_t* __restrict__ pixel_out,
+                             int32_t* __restrict__ tag_out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= load_count(n_host, nd)) return;
+    const int slot = hs_slot(m, hs_traced(m, j));
+    const float4 b = m.b[slot], c = m.c[slot];
+    const int32_t pixel = reinterpret_cast<const int32_t*>(m.a)[4 * (size_t)slot + 3];
+    add_miss(s, V3(b.x, b.y, b.z), V3(c.x, c.y, c.z), pixel, __float_as_int(c.w), b.w, physics,
+             accum);
+    pixel_out[j] = pixel;
+    if (tag_out && m.tags) tag_out[j] = m.tags[slot];
 }
 
 // EmbreeAccelerator::occluded for the NEE rays of each listed entry, then the contributions in
@@ -636,7 +721,7 @@ __global__ void k_miss(const SceneDev s, wf_ray_soa q, int n_host, const int32_t
 // xchg and spill as for closest_walks.
 template <class NODE, int STACK>
 __device__ __forceinline__ void shadow_walks(const SceneDev& s, const NODE* __restrict__ nodes,
-                                             const wf_shade_soa& q, const ShadeOut& o,
+                                             const HitCols& q, const ShadeOut& o,
                                              const int32_t* __restrict__ list, int n,
                                              float* __restrict__ accum,
                                              int32_t* __restrict__ counter, int refill_lanes,
@@ -720,7 +805,7 @@ __device__ __forceinline__ void shadow_walks(const SceneDev& s, const NODE* __re
 
 template <class NODE, int STACK>
 __global__ __launch_bounds__(kTraceBlock) void k_shadow(const SceneDev s, const NODE* __restrict__ nodes,
-                                                        wf_shade_soa q, ShadeOut o,
+                                                        HitCols q, ShadeOut o,
                                                         const int32_t* __restrict__ list, int count_host,
                                                         const int32_t* __restrict__ count_dev,
                                                         float* __restrict__ accum,
@@ -742,7 +827,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_closest_shadow(
     const SceneDev s, const NODE* __restrict__ nodes, R rays, int n_host,
     const int32_t* __restrict__ nd, int32_t* __restrict__ out_leaf, float* __restrict__ out_t,
     float* __restrict__ out_u, float* __restrict__ out_v, int32_t* __restrict__ ray_counter,
-    int refill_lanes, wf_shade_soa q, ShadeOut o, const int32_t* __restrict__ list,
+    int refill_lanes, HitCols q, ShadeOut o, const int32_t* __restrict__ list,
     int shadow_host, const int32_t* __restrict__ shadow_dev, float* __restrict__ accum,
     int32_t* __restrict__ shadow_counter, int shadow_refill_lanes) {
     __shared__ int32_t stack[lds_entries<STACK>() * kTraceBlock];
@@ -771,7 +856,7 @@ __global__ void k_count_shadow_rays(const uint32_t* __restrict__ flag, const int
 static wf_status shade_outputs(wf_context* ctx, int n, ShadeOut& o) {
     void* p = nullptr;
     const size_t nn = (size_t)(n > 0 ? n : 1);
-    WF_TRY(workspace(ctx, "shade_out", nn * 4 * 20, &p));
+    WF_TRY(workspace(ctx, "shade_out", nn * 4 * 25, &p));
     float* f = (float*)p;
     o.ldx = f + 0 * nn; o.ldy = f + 1 * nn; o.ldz = f + 2 * nn; o.lmax = f + 3 * nn;
     o.lr = f + 4 * nn; o.lg = f + 5 * nn; o.lb = f + 6 * nn;
@@ -781,6 +866,10 @@ static wf_status shade_outputs(wf_context* ctx, int n, ShadeOut& o) {
     o.ntr = f + 14 * nn; o.ntg = f + 15 * nn; o.ntb = f + 16 * nn;
     o.npdf = f + 17 * nn;
     o.alive = (uint32_t*)(f + 18 * nn);
+    o.hpx = f + 19 * nn; o.hpy = f + 20 * nn; o.hpz = f + 21 * nn;
+    o.hpixel = (int32_t*)(f + 22 * nn);
+    o.hdepth = (int32_t*)(f + 23 * nn);
+    o.htag = (int32_t*)(f + 24 * nn);
     o.append = 0;
     o.shadow_list = nullptr;
     o.shadow_entries = o.shadow_rays = o.next_count = nullptr;
@@ -817,7 +906,7 @@ static wf_status closest_deferred(wf_context* ctx, const wf_scene* s, const R* r
                          decltype(S)::value, R>
             <<<persistent_grid(n.n, ctx), kTraceBlock, 0, ctx->stream>>>(
                 s->dev, nodes, *rays, n.n, n.dev, hit_leaf, hit_t, hit_bu, hit_bv,
-                (int32_t*)prc, ctx->tuning.refill_lanes, d.q, d.o, d.list, d.n.n, d.n.dev,
+                (int32_t*)prc, ctx->tuning.refill_lanes, d.hc, d.o, d.list, d.n.n, d.n.dev,
                 d.accum, (int32_t*)psc, shadow_refill(ctx));
     });
     WF_HIP(hipGetLastError());
@@ -875,11 +964,11 @@ wf_status shade_bounce(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q
                                                                  jobs.tags, jobs.jobs);
             WF_HIP(hipGetLastError());
         }
+        const HitCols hc{q->px, q->py, q->pz, q->pixel, q->depth, jobs.tags};
         {
             Timed tm(ctx, "next_rays", o.next_keys ? 64.0 : 56.0, n);
             k_shade_append<<<grid_for(n.n, kAppendBlock * kAppendItems), kAppendBlock, 0,
-                             ctx->stream>>>(
-                *q, o, n.n, n.dev, jobs.tags);
+                             ctx->stream>>>(hc, o, n.n, n.dev);
             WF_HIP(hipGetLastError());
         }
         const Count ns{n.n, counts_dev + 2};
@@ -888,7 +977,7 @@ wf_status shade_bounce(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q
             void* pc = nullptr;
             WF_TRY(workspace(ctx, "shadow_counter", sizeof(int32_t) * kWorkInts, &pc));
             jobs.defer->armed = true;
-            jobs.defer->q = *q;
+            jobs.defer->hc = hc;
             jobs.defer->o = o;
             jobs.defer->list = (int32_t*)plist;
             jobs.defer->n = ns;
@@ -904,7 +993,7 @@ wf_status shade_bounce(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q
             k_shadow<std::remove_cv_t<std::remove_pointer_t<decltype(nodes)>>, decltype(S)::value>
                 <<<persistent_grid(ns.n, ctx, ctx->tuning.shadow_blocks_per_cu), kTraceBlock, 0,
                    ctx->stream>>>(
-                    s->dev, nodes, *q, o, (int32_t*)plist, ns.n, ns.dev, accum, (int32_t*)pc,
+                    s->dev, nodes, hc, o, (int32_t*)plist, ns.n, ns.dev, accum, (int32_t*)pc,
                     shadow_refill(ctx));
         });
         WF_HIP(hipGetLastError());
@@ -949,8 +1038,8 @@ wf_status shade_bounce(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q
             k_shadow<std::remove_cv_t<std::remove_pointer_t<decltype(nodes)>>, decltype(S)::value>
                 <<<persistent_grid(ns.n, ctx, ctx->tuning.shadow_blocks_per_cu), kTraceBlock, 0,
                    ctx->stream>>>(
-                    s->dev, nodes, *q, o, (int32_t*)plist, ns.n, ns.dev, accum, (int32_t*)pc,
-                    shadow_refill(ctx));
+                    s->dev, nodes, HitCols{q->px, q->py, q->pz, q->pixel, q->depth, jobs.tags},
+                    o, (int32_t*)plist, ns.n, ns.dev, accum, (int32_t*)pc, shadow_refill(ctx));
         });
         WF_HIP(hipGetLastError());
     }
@@ -971,6 +1060,72 @@ wf_status shade_misses(wf_context* ctx, const wf_scene* s, const wf_ray_soa* q, 
     if (n.n <= 0) return WF_OK;
     Timed tm(ctx, "miss", 52.0, n);
     k_miss<<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(s->dev, *q, n.n, n.dev, physics, accum);
+    WF_HIP(hipGetLastError());
+    return WF_OK;
+}
+
+wf_status shade_hits(wf_context* ctx, const wf_scene* s, const HitSource& h, Count n,
+                     const wf_shade_params* p, float* accum, int32_t* counts_dev,
+                     const ShadeJobs& jobs, const uint32_t** alive_out, HitCols* hc_out) {
+    WF_TRY(scene_ready(s));
+    WF_REQUIRE(jobs.unordered && jobs.defer, "shade_hits: index mode needs deferred shadow walks");
+    ShadeOut o;
+    WF_TRY(shade_outputs(ctx, n.n, o));
+    void* plist;
+    WF_TRY(workspace(ctx, "shade_list", (size_t)(n.n > 0 ? n.n : 1) * 4, &plist));
+    // sized here so that a captured iteration finds the next launch's counters allocated
+    void* pc = nullptr;
+    WF_TRY(workspace(ctx, "shadow_counter", sizeof(int32_t) * kWorkInts, &pc));
+    ShadeArgs a{p->physics, p->rr_min_depth, p->cost_aware_rr, p->firefly_threshold, p->seed,
+                p->sample_index, p->job};
+    o.append = 1;
+    o.shadow_list = (int32_t*)plist;
+    o.next_count = counts_dev;
+    o.shadow_rays = counts_dev + 1;
+    o.shadow_entries = counts_dev + 2;
+    o.next_packed = jobs.next_packed;
+    o.next_tags = jobs.next_tags;
+    o.next_keys = jobs.next_keys;
+    o.key_frame = jobs.key_frame;
+    if (!h.tags) o.htag = nullptr;
+    WF_TRY(wf_fill_u32(ctx, counts_dev, 0u, 3));
+    const HitCols hc{o.hpx, o.hpy, o.hpz, o.hpixel, o.hdepth, o.htag};
+    if (n.n > 0) {
+        {
+            // list 4, hit buffers 16, packed ray 48, tag 4, triangle 48, UVs 32, mesh material
+            // 4 read; hit columns 24, shadow record 44, next-ray state 32 written
+            Timed tm(ctx, "shade", 256.0, n);
+            k_shade_hits<<<grid_for(n.n, 128), 128, 0, ctx->stream>>>(
+                s->dev, h, n.n, n.dev, a, s->dev.mat_rel_cost, accum, o, jobs.jobs);
+            WF_HIP(hipGetLastError());
+        }
+        {
+            Timed tm(ctx, "next_rays", o.next_keys ? 64.0 : 56.0, n);
+            k_shade_append<<<grid_for(n.n, kAppendBlock * kAppendItems), kAppendBlock, 0,
+                             ctx->stream>>>(hc, o, n.n, n.dev);
+            WF_HIP(hipGetLastError());
+        }
+    }
+    jobs.defer->armed = true;
+    jobs.defer->hc = hc;
+    jobs.defer->o = o;
+    jobs.defer->list = (int32_t*)plist;
+    jobs.defer->n = Count{n.n, counts_dev + 2};
+    jobs.defer->accum = accum;
+    if (alive_out) *alive_out = o.alive;
+    if (hc_out) *hc_out = hc;
+    return WF_OK;
+}
+
+wf_status shade_misses_index(wf_context* ctx, const wf_scene* s, const HitSource& m, Count n,
+                             int32_t physics, float* accum, int32_t* pixel_out,
+                             int32_t* tag_out) {
+    if (n.n <= 0) return WF_OK;
+    // list 4, ray direction/pdf and throughput/depth 32, accumulator index 4, tag 4 read;
+    // accumulator 12 read and written; completion columns 8 written
+    Timed tm(ctx, "miss", 76.0, n);
+    k_miss_index<<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(s->dev, m, n.n, n.dev, physics,
+                                                               accum, pixel_out, tag_out);
     WF_HIP(hipGetLastError());
     return WF_OK;
 }

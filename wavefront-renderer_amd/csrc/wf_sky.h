@@ -45,26 +45,25 @@ __device__ __forceinline__ float env_pdf_sa(const SceneDev& s, int texel_i, floa
     return p / (2.0f * kPi * kPi * lc);
 }
 
-// Adds the contribution of miss-queue entry i to its pixel's accumulator (a plain
-// read-modify-write: a path owns its accumulator).
-__device__ __forceinline__ void add_miss(const SceneDev& s, const wf_ray_soa& q, int i, int physics,
+// Adds the contribution of a missed ray (direction d, throughput T, accumulator index, depth and
+// scatter pdf) to its accumulator (a plain read-modify-write: a path owns its accumulator).
+__device__ __forceinline__ void add_miss(const SceneDev& s, const V3& d, const V3& T,
+                                         int32_t pixel, int32_t depth, float pdf, int physics,
                                          float* __restrict__ accum) {
-    const V3 d(q.dx[i], q.dy[i], q.dz[i]);
-    const V3 T(q.tr[i], q.tg[i], q.tb[i]);
     V3 c;
     if (s.env_w > 0) {
         float lc;
         const int tx = env_texel(s, d, lc);
         float w = 1.0f;
-        if (physics == WF_PHYSICS_UNBIASED && q.depth[i] > 0 && q.pdf[i] > 0.0f)
-            w = mis_power(q.pdf[i], env_pdf_sa(s, tx, lc));
+        if (physics == WF_PHYSICS_UNBIASED && depth > 0 && pdf > 0.0f)
+            w = mis_power(pdf, env_pdf_sa(s, tx, lc));
         c = physics == WF_PHYSICS_UNBIASED ? T * env_radiance(s, tx) * w : T * env_radiance(s, tx);
     } else {
         const float blend = 0.5f * (d.normalized().y + 1.0f);
         const V3 dark(0.02f, 0.02f, 0.05f), horizon(0.05f, 0.04f, 0.03f);
         c = T * (horizon * (1.0f - blend) + dark * blend);
     }
-    float* acc = accum + (size_t)q.pixel[i] * 3;
+    float* acc = accum + (size_t)pixel * 3;
     acc[0] = acc[0] + c.x;
     acc[1] = acc[1] + c.y;
     acc[2] = acc[2] + c.z;

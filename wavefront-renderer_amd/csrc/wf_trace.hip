@@ -69,49 +69,30 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_stats(
     out_tris[i] = h.tris;
 }
 
-// HitRecord mapping of EmbreeAccelerator::intersect4 (Embree.cpp:240-252) for ray i into slot j.
+// HitRecord mapping (map_hit_vals) of ray i into shading-queue slot j, with the ray's state.
 template <class R>
 __device__ __forceinline__ void map_hit(const SceneDev& s, const R& in, int i, int L,
                                         float t, float bu, float bv, wf_shade_soa& out, int j) {
     const RayVal ray = ld_ray(in, i);
-    const V3 o = ray.o;
-    const V3 d = ray.d;
-    const float4 A = s.tri[3 * (size_t)L];
-    const float4 B = s.tri[3 * (size_t)L + 1];
-    const float4 C = s.tri[3 * (size_t)L + 2];
-    const int32_t g = __float_as_int(A.w);
-    const V3 P = o + d * t;
-    const int32_t geom = __float_as_int(B.w);
-    const int32_t mat = s.mesh_material[geom];
-    float u = bu, v = bv;
-    if (s.mesh_has_uv[geom]) {
-        // Mesh::interpolateUV (src/geometry/Mesh.cpp:46-63), UVs stored beside the leaf
-        const float4 ua = s.tri_uv_leaf[2 * (size_t)L], ub = s.tri_uv_leaf[2 * (size_t)L + 1];
-        const float w0 = 1.0f - bu - bv;
-        u = w0 * ua.x + bu * ua.z + bv * ub.x;
-        v = w0 * ua.y + bu * ua.w + bv * ub.y;
-    }
-    const V3 outward = xyz(B).cross(xyz(C)).normalized();
-    const bool ff = d.dot(outward) < 0.0f;
-    const V3 N = ff ? outward : outward * -1.0f;
-    st3(out.px, out.py, out.pz, j, P);
-    st3(out.nx, out.ny, out.nz, j, N);
+    const MappedHit h = map_hit_vals(s, ray.o, ray.d, L, t, bu, bv);
+    st3(out.px, out.py, out.pz, j, h.P);
+    st3(out.nx, out.ny, out.nz, j, h.N);
     out.t[j] = t;
-    out.u[j] = u;
-    out.v[j] = v;
-    if (out.tri) out.tri[j] = g - s.mesh_tri_offset[geom];
-    out.mat[j] = mat;
-    out.tex[j] = s.materials[mat].texture_id;
-    if (out.ox) st3(out.ox, out.oy, out.oz, j, o);
-    st3(out.dx, out.dy, out.dz, j, d);
+    out.u[j] = h.u;
+    out.v[j] = h.v;
+    if (out.tri) out.tri[j] = h.g - s.mesh_tri_offset[h.geom];
+    out.mat[j] = h.mat;
+    out.tex[j] = s.materials[h.mat].texture_id;
+    if (out.ox) st3(out.ox, out.oy, out.oz, j, ray.o);
+    st3(out.dx, out.dy, out.dz, j, ray.d);
     out.tr[j] = ray.t.x;
     out.tg[j] = ray.t.y;
     out.tb[j] = ray.t.z;
     out.pixel[j] = ray.pixel;
     out.depth[j] = ray.depth;
     out.pdf[j] = ray.pdf;
-    if (out.geom) out.geom[j] = geom;
-    if (out.front) out.front[j] = ff ? 1 : 0;
+    if (out.geom) out.geom[j] = h.geom;
+    if (out.front) out.front[j] = h.front ? 1 : 0;
 }
 
 // Stable split by the exclusive prefix of hit flags (intersectAll's hit and miss queues).
@@ -148,6 +129,36 @@ __global__ void k_split(const SceneDev s, wf_ray_soa in, int n_host, const int32
 // thread (coalesced rounds), one cursor atomic per block and list.
 constexpr int kAppendBlock = 1024;
 constexpr int kAppendItems = 4;
+
+// The index-mode split: the traced-ray index of each hit appended to hit_list, of each miss to
+// miss_list (the rays stay where they are and are shaded in place).
+__global__ __launch_bounds__(kAppendBlock) void k_split_index(
+    int n_host, const int32_t* __restrict__ nd, const int32_t* __restrict__ leaf,
+    int32_t* __restrict__ hit_list, int32_t* __restrict__ miss_list,
+    int32_t* __restrict__ counts) {
+    __shared__ uint32_t lds[kAppendItems * kAppendBlock / 64];
+    const int n = load_count(n_host, nd);
+    const int base = blockIdx.x * kAppendBlock * kAppendItems;
+    if (base >= n) return;
+    bool hit[kAppendItems], mis[kAppendItems];
+    int pos[kAppendItems], m[kAppendItems];
+#pragma unroll
+    for (int j = 0; j < kAppendItems; ++j) {
+        const int i = base + j * kAppendBlock + (int)threadIdx.x;
+        const int L = i < n ? leaf[i] : -1;
+        hit[j] = i < n && L >= 0;
+        mis[j] = i < n && L < 0;
+    }
+    block_append_rounds<kAppendItems, kAppendBlock>(&counts[0], hit, pos, lds);
+    block_append_rounds<kAppendItems, kAppendBlock>(&counts[1], mis, m, lds);
+#pragma unroll
+    for (int j = 0; j < kAppendItems; ++j) {
+        const int i = base + j * kAppendBlock + (int)threadIdx.x;
+        if (pos[j] >= 0) hit_list[pos[j]] = i;
+        else if (m[j] >= 0) miss_list[m[j]] = i;
+    }
+}
+
 template <class R>
 __global__ __launch_bounds__(kAppendBlock) void k_split_append(
     const SceneDev s, R in, int n_host, const int32_t* __restrict__ nd,
@@ -189,17 +200,20 @@ __global__ __launch_bounds__(kAppendBlock) void k_split_append(
 // lays its hits out in LDS in range order; thread k then maps the k-th hit of that order, so
 // that consecutive lanes write consecutive slots of one run (coalesced stores) and only the
 // reads of a hit's ray and triangle follow the permutation. Misses are appended as in
-// k_split_append. Dynamic LDS: 3 n_materials words (there are at most n_materials ranges).
+// k_split_append. INDEX: the traced-ray indices of the hits go into the ranges of hit_list and
+// those of the misses to miss_list instead (index mode: nothing else is written). Dynamic LDS:
+// 3 n_materials words (there are at most n_materials ranges).
 constexpr int kBucketItems = 8;
 constexpr int kBucketTile = kAppendBlock * kBucketItems;
-template <class R>
+template <class R, bool INDEX>
 __global__ __launch_bounds__(kAppendBlock) void k_split_buckets(
     const SceneDev s, R in, int n_host, const int32_t* __restrict__ nd,
     const int32_t* __restrict__ leaf, const float* __restrict__ ht, const float* __restrict__ hu,
     const float* __restrict__ hv, wf_shade_soa out, wf_ray_soa miss, int32_t* __restrict__ counts,
     const int32_t* __restrict__ bucket, int32_t* __restrict__ cursor,
     const int32_t* __restrict__ tags_in, int32_t* __restrict__ hit_tags,
-    int32_t* __restrict__ miss_tags) {
+    int32_t* __restrict__ miss_tags, int32_t* __restrict__ hit_list,
+    int32_t* __restrict__ miss_list) {
     extern __shared__ uint32_t sb_lds[];
     __shared__ uint32_t lds[kBucketItems * kAppendBlock / 64];
     __shared__ int32_t order[kBucketTile];   // ray of the k-th hit of the block, in range order
@@ -259,6 +273,10 @@ __global__ __launch_bounds__(kAppendBlock) void k_split_buckets(
     for (int j = 0; j < kBucketItems; ++j) {
         if (m[j] < 0) continue;
         const int i = base + j * kAppendBlock + (int)threadIdx.x;
+        if (INDEX) {
+            miss_list[m[j]] = i;
+            continue;
+        }
         st_ray(miss, m[j], ld_ray(in, i));
         if (miss_tags) miss_tags[m[j]] = tags_in[ray_source(in, i)];
     }
@@ -267,6 +285,10 @@ __global__ __launch_bounds__(kAppendBlock) void k_split_buckets(
         const int i = order[k];
         const int mi = order_range[k];
         const int pos = (int)(run[mi] + ((uint32_t)k - local[mi]));
+        if (INDEX) {
+            hit_list[pos] = i;
+            continue;
+        }
         map_hit(s, in, i, leaf[i], ht[i], hu[i], hv[i], out, pos);
         if (hit_tags) hit_tags[pos] = tags_in[ray_source(in, i)];
     }
@@ -398,10 +420,10 @@ static wf_status split_append(wf_context* ctx, const wf_scene* s, const R& rays,
         if (n.n <= 0) return WF_OK;
         Timed tm(ctx, "split", 97.0, n);
         const size_t lds = sizeof(uint32_t) * 3 * (size_t)s->dev.n_materials;
-        k_split_buckets<R><<<grid_for(n.n, kBucketTile), kAppendBlock, lds,
-                             ctx->stream>>>(
+        k_split_buckets<R, false><<<grid_for(n.n, kBucketTile), kAppendBlock, lds,
+                                    ctx->stream>>>(
             s->dev, rays, n.n, n.dev, hl, ht, hu, hv, *hits_out, *misses_out, counts_dev,
-            ranges->bucket, ranges->cursor, tags_in, hit_tags, miss_tags);
+            ranges->bucket, ranges->cursor, tags_in, hit_tags, miss_tags, nullptr, nullptr);
         WF_HIP(hipGetLastError());
         return WF_OK;
     }
@@ -433,6 +455,56 @@ wf_status split_traced(wf_context* ctx, const wf_scene* s, const RayPackedOrdere
                        int32_t* miss_tags, const PolicyRanges* ranges) {
     return split_append(ctx, s, *rays, n, hit_leaf, hit_t, hit_bu, hit_bv, hits_out, misses_out,
                         counts_dev, tags_in, hit_tags, miss_tags, ranges);
+}
+
+wf_status split_index(wf_context* ctx, const wf_scene* s, Count n, const int32_t* hit_leaf,
+                      int32_t* hit_list, int32_t* miss_list, int32_t* counts_dev,
+                      const PolicyRanges* ranges) {
+    if (ranges) {
+        WF_TRY(wf_fill_u32(ctx, counts_dev + 1, 0u, 1));
+        if (n.n <= 0) return WF_OK;
+        // leaf and its material read, one index written per ray
+        Timed tm(ctx, "split", 12.0, n);
+        const size_t lds = sizeof(uint32_t) * 3 * (size_t)s->dev.n_materials;
+        k_split_buckets<RayPacked, true><<<grid_for(n.n, kBucketTile), kAppendBlock, lds,
+                                           ctx->stream>>>(
+            s->dev, RayPacked{}, n.n, n.dev, hit_leaf, nullptr, nullptr, nullptr, wf_shade_soa{},
+            wf_ray_soa{}, counts_dev, ranges->bucket, ranges->cursor, nullptr, nullptr, nullptr,
+            hit_list, miss_list);
+        WF_HIP(hipGetLastError());
+        return WF_OK;
+    }
+    WF_TRY(wf_fill_u32(ctx, counts_dev, 0u, 2));
+    if (n.n > 0) {
+        // leaf read, one index written per ray
+        Timed tm(ctx, "split", 8.0, n);
+        k_split_index<<<grid_for(n.n, kAppendBlock * kAppendItems), kAppendBlock, 0,
+                        ctx->stream>>>(n.n, n.dev, hit_leaf, hit_list, miss_list, counts_dev);
+        WF_HIP(hipGetLastError());
+    }
+    return WF_OK;
+}
+
+// HitRecord mapping of listed hits into a shading queue (the engine's diagnostic).
+__global__ void k_gather_hits(const SceneDev s, HitSource h, int n_host,
+                              const int32_t* __restrict__ nd, wf_shade_soa out,
+                              int32_t* __restrict__ tags_out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= load_count(n_host, nd)) return;
+    const int i = hs_traced(h, j);
+    const int slot = hs_slot(h, i);
+    const RayPacked r{const_cast<float4*>(h.a), const_cast<float4*>(h.b), const_cast<float4*>(h.c)};
+    map_hit(s, r, slot, h.leaf[i], h.t[i], h.u[i], h.v[i], out, j);
+    if (tags_out && h.tags) tags_out[j] = h.tags[slot];
+}
+
+wf_status gather_hits(wf_context* ctx, const wf_scene* s, const HitSource& h, Count n,
+                      wf_shade_soa* dst, int32_t* tags_out) {
+    if (n.n <= 0) return WF_OK;
+    k_gather_hits<<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(s->dev, h, n.n, n.dev, *dst,
+                                                                tags_out);
+    WF_HIP(hipGetLastError());
+    return WF_OK;
 }
 
 // Closest hit, then hits and misses appended at device cursors (queue order free).
