@@ -504,8 +504,7 @@ struct wf_engine {
     RayPacked rays[2]{};
     int32_t* tags[2] = {nullptr, nullptr};
     int par = 0;
-    int32_t* hit_list = nullptr;        // index mode: traced-ray index of every hit, of every miss
-    int32_t* miss_list = nullptr;
+    int32_t* hit_list = nullptr;        // index mode: traced-ray index of every hit
     uint64_t* keys = nullptr;           // sort word of each live ray (written by its producer)
     uint64_t* keys2 = nullptr;          // radix sort scratch
     // coherence metrics of every iteration's shading queue (set by wf_render; 0 = off)
@@ -617,8 +616,8 @@ wf_status engine_array(wf_engine* e, size_t count, T** out) {
 // Bytes of device memory per ray slot, per framebuffer pixel and per further sample lane of a
 // pixel (admission accounting).
 // two ray buffers and the miss queue, two shading queues, tags (two ray buffers, two shading
-// queues, misses), sort index, hit and miss lists, workspaces
-constexpr size_t kBytesPerSlot = 48 * 3 + 104 * 2 + 4 * 5 + 4 + 4 * 2 + 400;
+// queues, misses), sort index, hit list, workspaces
+constexpr size_t kBytesPerSlot = 48 * 3 + 104 * 2 + 4 * 5 + 4 + 4 + 400;
 constexpr size_t kBytesPerPixel = 12 + 12 + 4 + 4 + 4 + 4 + 8;   // accum, start copy, mean, M2,
                                                                   // count, lanes done, queue
 constexpr size_t kBytesPerLane = 12 + 8;                          // accum, queue
@@ -653,9 +652,10 @@ void accumulate_phases(wf_engine* e, int b) {
 // e->par and appending the next bounce's rays to the other buffer.
 //
 // Index mode (the default): the split only lists the traced-ray index of every hit (into the
-// range of its group key under a policy) and of every miss; shading and miss shading read each
-// ray and its closest hit where they are and map the HitRecord in registers, so no shading-queue
-// or miss-queue record is written. The shading-queue mode materialises the queue as the C-ABI
+// range of its group key under a policy) and resolves every miss on the spot (sky or
+// environment, then the accumulator index and job tag kept for completion); shading reads each
+// listed ray and its closest hit where they are and maps the HitRecord in registers, so no
+// shading-queue or miss-queue record is written. The shading-queue mode materialises the queue as the C-ABI
 // does: the engine uses it for coherence statistics (they are taken over the queue) and for
 // policies whose scenes have too many materials for ranges (their order comes from sorting the
 // queue).
@@ -736,11 +736,12 @@ wf_status enqueue_iteration(wf_engine* e) {
     const uint32_t* alive = nullptr;
     e->last_index = index_mode;
     if (index_mode) {
-        WF_TRY(split_index(ctx, s, live, hl, e->hit_list, e->miss_list, c + C_HITS,
-                           bucketed ? &ranges : nullptr));
-        phase_mark(e, 3);
         const HitSource h{cur.a, cur.b, cur.c, sort ? e->idx : nullptr, e->hit_list, hl, ht, hu,
                           hv, ctag};
+        // (the misses are resolved by the split)
+        WF_TRY(split_index(ctx, s, h, live, e->hit_list, e->cfg.shade.physics, v.accum,
+                           e->miss.pixel, e->mt, c + C_HITS, bucketed ? &ranges : nullptr));
+        phase_mark(e, 3);
         e->last_hits = h;
         e->last_q = nullptr;
         phase_mark(e, 4);
@@ -754,10 +755,6 @@ wf_status enqueue_iteration(wf_engine* e) {
             WF_HIP(hipGetLastError());
         }
         phase_mark(e, 5);
-        HitSource m = h;
-        m.list = e->miss_list;
-        WF_TRY(shade_misses_index(ctx, s, m, misses, e->cfg.shade.physics, v.accum,
-                                  e->miss.pixel, e->mt));
     } else {
         if (sort)
             WF_TRY(split_traced(ctx, s, &ordered, live, hl, ht, hu, hv, &e->s0, &e->miss,
@@ -945,7 +942,6 @@ wf_status wf_engine_create(wf_context* ctx, wf_scene* scene, const wf_engine_con
     ETRY(engine_array(e, cap, &e->tags[0]));
     ETRY(engine_array(e, cap, &e->tags[1]));
     ETRY(engine_array(e, cap, &e->hit_list));
-    ETRY(engine_array(e, cap, &e->miss_list));
     ETRY(engine_array(e, cap, &e->st0));
     ETRY(engine_array(e, cap, &e->st1));
     ETRY(engine_array(e, cap, &e->mt));

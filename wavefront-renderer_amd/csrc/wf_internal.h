@@ -584,16 +584,14 @@ wf_status shade_misses(wf_context* ctx, const wf_scene* s, const wf_ray_soa* q, 
 wf_status shade_hits(wf_context* ctx, const wf_scene* s, const HitSource& h, Count n,
                      const wf_shade_params* p, float* accum, int32_t* counts_dev,
                      const ShadeJobs& jobs, const uint32_t** alive_out, HitCols* hc_out);
-// Index mode: the misses listed by m shaded in place; each entry's accumulator index and job tag
-// are copied to pixel_out / tag_out (by miss position) for path completion.
-wf_status shade_misses_index(wf_context* ctx, const wf_scene* s, const HitSource& m, Count n,
-                             int32_t physics, float* accum, int32_t* pixel_out,
-                             int32_t* tag_out);
-// Index-mode split of traced packed rays: the traced-ray index of every hit appended to
-// hit_list (into the range of its group key with ranges) and of every miss to miss_list;
+// Index-mode split of traced packed rays (src: the rays and their hit buffers; its list is not
+// used): the traced-ray index of every hit appended to hit_list (into the range of its group
+// key with ranges); every miss resolved in place (sky or environment into accum) with its
+// accumulator index and job tag appended to miss_pixel / miss_tag for path completion;
 // counts_dev[0] = hits, counts_dev[1] = misses.
-wf_status split_index(wf_context* ctx, const wf_scene* s, Count n, const int32_t* hit_leaf,
-                      int32_t* hit_list, int32_t* miss_list, int32_t* counts_dev,
+wf_status split_index(wf_context* ctx, const wf_scene* s, const HitSource& src, Count n,
+                      int32_t* hit_list, int32_t physics, float* accum, int32_t* miss_pixel,
+                      int32_t* miss_tag, int32_t* counts_dev,
                       const PolicyRanges* ranges = nullptr);
 // The shading queue of listed hits (HitRecord mapping of each), with their job tags: the
 // engine's diagnostic of an index-mode iteration.

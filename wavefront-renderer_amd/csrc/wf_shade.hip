@@ -698,23 +698,6 @@ __global__ void k_miss(const SceneDev s, wf_ray_soa q, int n_host, const int32_t
              q.depth[i], q.pdf[i], physics, accum);
 }
 
-// Index mode: miss j is traced ray m.list[j], read in place; its accumulator index and job tag
-// are kept for path completion.
-__global__ void k_miss_index(const SceneDev s, HitSource m, int n_host,
-                             const int32_t* __restrict__ nd, int physics,
-                             float* __restrict__ accum, int32_t* __restrict__ pixel_out,
-                             int32_t* __restrict__ tag_out) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= load_count(n_host, nd)) return;
-    const int slot = hs_slot(m, hs_traced(m, j));
-    const float4 b = m.b[slot], c = m.c[slot];
-    const int32_t pixel = reinterpret_cast<const int32_t*>(m.a)[4 * (size_t)slot + 3];
-    add_miss(s, V3(b.x, b.y, b.z), V3(c.x, c.y, c.z), pixel, __float_as_int(c.w), b.w, physics,
-             accum);
-    pixel_out[j] = pixel;
-    if (tag_out && m.tags) tag_out[j] = m.tags[slot];
-}
-
 // EmbreeAccelerator::occluded for the NEE rays of each listed entry, then the contributions in
 // the reference order: area/environment light first, directional light second (cpp:678, 720).
 // The body of the shadow kernels, run by a persistent block over entries [0, n) of list; stack,
@@ -1114,19 +1097,6 @@ wf_status shade_hits(wf_context* ctx, const wf_scene* s, const HitSource& h, Cou
     jobs.defer->accum = accum;
     if (alive_out) *alive_out = o.alive;
     if (hc_out) *hc_out = hc;
-    return WF_OK;
-}
-
-wf_status shade_misses_index(wf_context* ctx, const wf_scene* s, const HitSource& m, Count n,
-                             int32_t physics, float* accum, int32_t* pixel_out,
-                             int32_t* tag_out) {
-    if (n.n <= 0) return WF_OK;
-    // list 4, ray direction/pdf and throughput/depth 32, accumulator index 4, tag 4 read;
-    // accumulator 12 read and written; completion columns 8 written
-    Timed tm(ctx, "miss", 76.0, n);
-    k_miss_index<<<grid_for(n.n, 256), 256, 0, ctx->stream>>>(s->dev, m, n.n, n.dev, physics,
-                                                               accum, pixel_out, tag_out);
-    WF_HIP(hipGetLastError());
     return WF_OK;
 }
 

@@ -10,6 +10,7 @@
 // triangle the test accepts in [tnear, tfar], which makes the result independent of traversal
 // order. Box tests are conservative (boxes are padded at build time).
 #include "wf_rays.h"
+#include "wf_sky.h"
 #include "wf_traverse.h"
 
 namespace wf {
@@ -130,32 +131,45 @@ __global__ void k_split(const SceneDev s, wf_ray_soa in, int n_host, const int32
 constexpr int kAppendBlock = 1024;
 constexpr int kAppendItems = 4;
 
-// The index-mode split: the traced-ray index of each hit appended to hit_list, of each miss to
-// miss_list (the rays stay where they are and are shaded in place).
+// The index-mode split: the traced-ray index of each hit appended to hit_list; each miss is
+// resolved right here (the sky or environment added to its accumulator, as k_miss does) and its
+// accumulator index and job tag appended for path completion (the rays stay where they are and
+// the hits are shaded in place). kSplitItems rays per thread (more per thread measured no
+// faster: the cursor atomics are not what bounds it).
+constexpr int kSplitItems = 4;
 __global__ __launch_bounds__(kAppendBlock) void k_split_index(
-    int n_host, const int32_t* __restrict__ nd, const int32_t* __restrict__ leaf,
-    int32_t* __restrict__ hit_list, int32_t* __restrict__ miss_list,
-    int32_t* __restrict__ counts) {
-    __shared__ uint32_t lds[kAppendItems * kAppendBlock / 64];
+    const SceneDev s, HitSource src, int n_host, const int32_t* __restrict__ nd, int physics,
+    float* __restrict__ accum, int32_t* __restrict__ hit_list, int32_t* __restrict__ miss_pixel,
+    int32_t* __restrict__ miss_tag, int32_t* __restrict__ counts) {
+    __shared__ uint32_t lds[kSplitItems * kAppendBlock / 64];
     const int n = load_count(n_host, nd);
-    const int base = blockIdx.x * kAppendBlock * kAppendItems;
+    const int base = blockIdx.x * kAppendBlock * kSplitItems;
     if (base >= n) return;
-    bool hit[kAppendItems], mis[kAppendItems];
-    int pos[kAppendItems], m[kAppendItems];
+    bool hit[kSplitItems], mis[kSplitItems];
+    int pos[kSplitItems], m[kSplitItems];
 #pragma unroll
-    for (int j = 0; j < kAppendItems; ++j) {
+    for (int j = 0; j < kSplitItems; ++j) {
         const int i = base + j * kAppendBlock + (int)threadIdx.x;
-        const int L = i < n ? leaf[i] : -1;
+        const int L = i < n ? src.leaf[i] : -1;
         hit[j] = i < n && L >= 0;
         mis[j] = i < n && L < 0;
     }
-    block_append_rounds<kAppendItems, kAppendBlock>(&counts[0], hit, pos, lds);
-    block_append_rounds<kAppendItems, kAppendBlock>(&counts[1], mis, m, lds);
+    block_append_rounds<kSplitItems, kAppendBlock>(&counts[0], hit, pos, lds);
+    block_append_rounds<kSplitItems, kAppendBlock>(&counts[1], mis, m, lds);
 #pragma unroll
-    for (int j = 0; j < kAppendItems; ++j) {
+    for (int j = 0; j < kSplitItems; ++j) {
         const int i = base + j * kAppendBlock + (int)threadIdx.x;
-        if (pos[j] >= 0) hit_list[pos[j]] = i;
-        else if (m[j] >= 0) miss_list[m[j]] = i;
+        if (pos[j] >= 0) {
+            hit_list[pos[j]] = i;
+        } else if (m[j] >= 0) {
+            const int slot = hs_slot(src, i);
+            const float4 b = src.b[slot], c = src.c[slot];
+            const int32_t pixel = reinterpret_cast<const int32_t*>(src.a)[4 * (size_t)slot + 3];
+            add_miss(s, V3(b.x, b.y, b.z), V3(c.x, c.y, c.z), pixel, __float_as_int(c.w), b.w,
+                     physics, accum);
+            miss_pixel[m[j]] = pixel;
+            if (miss_tag) miss_tag[m[j]] = src.tags[slot];
+        }
     }
 }
 
@@ -201,7 +215,8 @@ __global__ __launch_bounds__(kAppendBlock) void k_split_append(
 // that consecutive lanes write consecutive slots of one run (coalesced stores) and only the
 // reads of a hit's ray and triangle follow the permutation. Misses are appended as in
 // k_split_append. INDEX: the traced-ray indices of the hits go into the ranges of hit_list and
-// those of the misses to miss_list instead (index mode: nothing else is written). Dynamic LDS:
+// the misses are resolved as in k_split_index (index mode: no queue record is written). Dynamic
+// LDS:
 // 3 n_materials words (there are at most n_materials ranges).
 constexpr int kBucketItems = 8;
 constexpr int kBucketTile = kAppendBlock * kBucketItems;
@@ -212,8 +227,8 @@ __global__ __launch_bounds__(kAppendBlock) void k_split_buckets(
     const float* __restrict__ hv, wf_shade_soa out, wf_ray_soa miss, int32_t* __restrict__ counts,
     const int32_t* __restrict__ bucket, int32_t* __restrict__ cursor,
     const int32_t* __restrict__ tags_in, int32_t* __restrict__ hit_tags,
-    int32_t* __restrict__ miss_tags, int32_t* __restrict__ hit_list,
-    int32_t* __restrict__ miss_list) {
+    int32_t* __restrict__ miss_tags, int32_t* __restrict__ hit_list, HitSource src,
+    int physics, float* __restrict__ accum) {
     extern __shared__ uint32_t sb_lds[];
     __shared__ uint32_t lds[kBucketItems * kAppendBlock / 64];
     __shared__ int32_t order[kBucketTile];   // ray of the k-th hit of the block, in range order
@@ -274,7 +289,14 @@ __global__ __launch_bounds__(kAppendBlock) void k_split_buckets(
         if (m[j] < 0) continue;
         const int i = base + j * kAppendBlock + (int)threadIdx.x;
         if (INDEX) {
-            miss_list[m[j]] = i;
+            // resolved here, as in k_split_index
+            const int slot = hs_slot(src, i);
+            const float4 b = src.b[slot], c = src.c[slot];
+            const int32_t pixel = reinterpret_cast<const int32_t*>(src.a)[4 * (size_t)slot + 3];
+            add_miss(s, V3(b.x, b.y, b.z), V3(c.x, c.y, c.z), pixel, __float_as_int(c.w), b.w,
+                     physics, accum);
+            miss.pixel[m[j]] = pixel;
+            if (miss_tags) miss_tags[m[j]] = src.tags[slot];
             continue;
         }
         st_ray(miss, m[j], ld_ray(in, i));
@@ -423,7 +445,8 @@ static wf_status split_append(wf_context* ctx, const wf_scene* s, const R& rays,
         k_split_buckets<R, false><<<grid_for(n.n, kBucketTile), kAppendBlock, lds,
                                     ctx->stream>>>(
             s->dev, rays, n.n, n.dev, hl, ht, hu, hv, *hits_out, *misses_out, counts_dev,
-            ranges->bucket, ranges->cursor, tags_in, hit_tags, miss_tags, nullptr, nullptr);
+            ranges->bucket, ranges->cursor, tags_in, hit_tags, miss_tags, nullptr, HitSource{},
+            0, nullptr);
         WF_HIP(hipGetLastError());
         return WF_OK;
     }
@@ -457,29 +480,34 @@ wf_status split_traced(wf_context* ctx, const wf_scene* s, const RayPackedOrdere
                         counts_dev, tags_in, hit_tags, miss_tags, ranges);
 }
 
-wf_status split_index(wf_context* ctx, const wf_scene* s, Count n, const int32_t* hit_leaf,
-                      int32_t* hit_list, int32_t* miss_list, int32_t* counts_dev,
-                      const PolicyRanges* ranges) {
+wf_status split_index(wf_context* ctx, const wf_scene* s, const HitSource& src, Count n,
+                      int32_t* hit_list, int32_t physics, float* accum, int32_t* miss_pixel,
+                      int32_t* miss_tag, int32_t* counts_dev, const PolicyRanges* ranges) {
+    // per ray: leaf 4 read, hit index 4 written; per miss also direction/pdf and
+    // throughput/depth 32, accumulator index 4, tag 4 read, accumulator 12 read and written,
+    // completion columns 8 written (the misses are about half of the rays)
+    const double bytes = 8.0 + 0.5 * 72.0;
     if (ranges) {
         WF_TRY(wf_fill_u32(ctx, counts_dev + 1, 0u, 1));
         if (n.n <= 0) return WF_OK;
-        // leaf and its material read, one index written per ray
-        Timed tm(ctx, "split", 12.0, n);
+        Timed tm(ctx, "split", bytes + 4.0, n);   // + the leaf's material
         const size_t lds = sizeof(uint32_t) * 3 * (size_t)s->dev.n_materials;
+        wf_ray_soa miss{};
+        miss.pixel = miss_pixel;
         k_split_buckets<RayPacked, true><<<grid_for(n.n, kBucketTile), kAppendBlock, lds,
                                            ctx->stream>>>(
-            s->dev, RayPacked{}, n.n, n.dev, hit_leaf, nullptr, nullptr, nullptr, wf_shade_soa{},
-            wf_ray_soa{}, counts_dev, ranges->bucket, ranges->cursor, nullptr, nullptr, nullptr,
-            hit_list, miss_list);
+            s->dev, RayPacked{}, n.n, n.dev, src.leaf, nullptr, nullptr, nullptr, wf_shade_soa{},
+            miss, counts_dev, ranges->bucket, ranges->cursor, nullptr, nullptr, miss_tag,
+            hit_list, src, physics, accum);
         WF_HIP(hipGetLastError());
         return WF_OK;
     }
     WF_TRY(wf_fill_u32(ctx, counts_dev, 0u, 2));
     if (n.n > 0) {
-        // leaf read, one index written per ray
-        Timed tm(ctx, "split", 8.0, n);
-        k_split_index<<<grid_for(n.n, kAppendBlock * kAppendItems), kAppendBlock, 0,
-                        ctx->stream>>>(n.n, n.dev, hit_leaf, hit_list, miss_list, counts_dev);
+        Timed tm(ctx, "split", bytes, n);
+        k_split_index<<<grid_for(n.n, kAppendBlock * kSplitItems), kAppendBlock, 0,
+                        ctx->stream>>>(s->dev, src, n.n, n.dev, physics, accum, hit_list,
+                                       miss_pixel, miss_tag, counts_dev);
         WF_HIP(hipGetLastError());
     }
     return WF_OK;
