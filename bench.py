@@ -70,7 +70,8 @@ def workload_settings(name: str, wf) -> dict:
 POLICY_KERNELS = ("policy_count", "policy_plan", "split", "policy_keys", "shading_sort",
                   "gather_shading")
 # Kernels whose bound is HBM bandwidth (sort, compaction, gathers, shading); the traversal is
-# bound by VALU issue (see measured_valu).
+# bound by the vector-memory data return of its loads, with VALU issue close behind (see
+# measured_pipes).
 HBM_KERNELS = ("ray_keys", "ray_sort", "gather_rays", "split", "compact", "policy_count",
                "policy_keys", "shading_sort", "gather_shading", "shade", "next_rays", "miss",
                "raygen", "regen", "finish")
@@ -194,8 +195,9 @@ def traversal_stats(wf, ctx, scene, data, W, H, rng) -> dict:
 # Library kernel names (per-launch HIP-event timing) -> device kernel symbols in rocprofv3 output
 KERNEL_SYMBOLS = {"traverse_closest": "k_trace_closest", "shadow": "k_shadow",
                   "traverse_closest_shadow": "k_closest_shadow",
-                  "split": "k_split", "shade": "k_shade", "gather_rays": "k_gather_rays",
-                  "miss": "k_miss", "regen": "k_regen", "ray_keys": "k_ray_keys"}
+                  "split": "k_split_index", "shade": "k_shade_hits",
+                  "gather_rays": "k_gather_rays", "miss": "k_miss", "regen": "k_regen",
+                  "ray_keys": "k_ray_keys"}
 
 
 def measured_traffic(workload: str, kernel: str):
@@ -216,19 +218,23 @@ def measured_traffic(workload: str, kernel: str):
 # MI355X: 256 CUs x 4 SIMDs; a wave64 FP32 VALU instruction occupies a 16-lane SIMD for 4 cycles
 # (packed FP32 is not used), at the 2.4 GHz peak engine clock
 VALU_PEAK_GINST = 256 * 4 * 2.4 / 4.0
+# one texture-data (TD) unit per CU returns vector-memory load data to the waves: busy cycles per
+# second of all 256, at the same clock
+TD_PEAK_GCYCLES = 256 * 2.4
 
 
-def measured_valu(workload: str, kernel: str):
-    """VALU wave-instructions per launch of `kernel` from the committed rocprofv3 SQ_INSTS_VALU
-    summary of this workload (profiles/valu_<workload>_1080p_64spp.json), or None."""
-    path = ROOT / "profiles" / f"valu_{workload}_1080p_64spp.json"
+def measured_pipes(workload: str, kernel: str):
+    """Per-launch hardware counters of `kernel` (VALU instructions, TD busy cycles, ...) from the
+    committed rocprofv3 counter summary of this workload
+    (profiles/pipes_<workload>_1080p_64spp.json), or None."""
+    path = ROOT / "profiles" / f"pipes_{workload}_1080p_64spp.json"
     sym = KERNEL_SYMBOLS.get(kernel)
     if not sym or not path.exists():
         return None
     t = json.loads(path.read_text())
     for name, v in t["kernels"].items():
         if name.split("<")[0] == sym and "SQ_INSTS_VALU" in v:
-            return {"valu_per_launch": v["SQ_INSTS_VALU"], "source": str(path.relative_to(ROOT))}
+            return {**v, "source": str(path.relative_to(ROOT))}
     return None
 
 
@@ -536,16 +542,28 @@ def main() -> None:
         roofline = {"kernel": dom, **hbm_view, "traffic": None,
                     "avg_launch_us": round(avg_us, 2), "launches": launches,
                     "bytes_per_launch": round(nbytes / max(launches, 1))}
-        valu = measured_valu(args.workload, dom)
-        if valu:
-            # the traversal issues VALU instructions near the rate the SIMDs can take them: that
-            # roof bounds it, not HBM (its algorithmic bytes are rays in and hits out)
-            ginst = valu["valu_per_launch"] / (avg_us * 1e-6) / 1e9
-            roofline.update({"bound": "valu", "achieved": round(ginst, 1),
-                             "peak": VALU_PEAK_GINST, "unit": "G VALU wave-instructions/s",
-                             "frac": round(ginst / VALU_PEAK_GINST, 4),
-                             "valu_per_launch": round(valu["valu_per_launch"]),
-                             "valu_source": valu["source"], "hbm": hbm_view})
+        pipes = measured_pipes(args.workload, dom)
+        if pipes:
+            # the traversal is bound by the vector-memory data return (TD) of its node and
+            # triangle loads, with VALU issue close behind; not by HBM (its algorithmic bytes are
+            # rays in and hits out): both views are per launch from the counters over the live
+            # launch time
+            ginst = pipes["SQ_INSTS_VALU"] / (avg_us * 1e-6) / 1e9
+            valu_view = {"bound": "valu", "achieved": round(ginst, 1), "peak": VALU_PEAK_GINST,
+                         "unit": "G VALU wave-instructions/s",
+                         "frac": round(ginst / VALU_PEAK_GINST, 4),
+                         "valu_per_launch": round(pipes["SQ_INSTS_VALU"])}
+            roofline.update({"valu": valu_view, "hbm": hbm_view,
+                             "counters_source": pipes["source"]})
+            if "TD_TD_BUSY_sum" in pipes:
+                gtd = pipes["TD_TD_BUSY_sum"] / (avg_us * 1e-6) / 1e9
+                roofline.update({"bound": "td", "achieved": round(gtd, 1),
+                                 "peak": TD_PEAK_GCYCLES, "unit": "G TD-busy cycles/s",
+                                 "frac": round(gtd / TD_PEAK_GCYCLES, 4),
+                                 "td_busy_per_launch": round(pipes["TD_TD_BUSY_sum"])})
+            else:
+                roofline.update({k: valu_view[k] for k in ("bound", "achieved", "peak", "unit",
+                                                          "frac")})
         tr = measured_traffic(args.workload, dom)
         if tr:
             roofline["traffic"] = tr["bytes_per_launch"]
