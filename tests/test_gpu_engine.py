@@ -67,6 +67,25 @@ def test_regenerate_equals_passes_textured_unbiased(wf, stress, policy):
     assert_same(a, b)
 
 
+@pytest.mark.parametrize("policy", ["none", "texture", "costBenefit"])
+@pytest.mark.parametrize("lanes", [1, 0])
+def test_index_mode_equals_queue_mode(wf, stress, policy, lanes):
+    """The engine shades hits in place from the ray buffer and the hit buffers (index mode)
+    unless coherence statistics are asked for, which materialise the shading queue and shade it
+    (the C-ABI's path). Every path does the same arithmetic on the same values either way, so
+    the images, counts and ray statistics are bit-identical; with one lane both also equal the
+    pass loop (the reference's order)."""
+    data, scene = stress
+    kw = dict(camera=data.camera, width=64, height=48, spp=3, max_depth=12, seed=11,
+              physics=wf.PHYSICS_UNBIASED, firefly=1e30, policy=policy, sample_lanes=lanes)
+    index = wf.render(scene, wf.render_params(**kw))
+    queue = wf.render(scene, wf.render_params(coherence_stats=1, **kw))
+    assert queue[2].material_run_length >= 1.0 and index[2].material_run_length == 0.0
+    assert_same(index, queue)
+    if lanes == 1:
+        assert_same(index, wf.render(scene, wf.render_params(**{**kw, "schedule": "passes"})))
+
+
 @pytest.mark.parametrize("key_bits", [33, 9, 1])
 def test_ray_order_changes_no_result(wf, ctx, cornell, key_bits):
     """Renders of one scene and size switch the per-bounce ray order off and on (the library keeps

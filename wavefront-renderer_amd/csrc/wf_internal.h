@@ -511,6 +511,19 @@ struct HitSource {
     const int32_t* tags = nullptr;   // job tag of each ray slot
 };
 
+// Hits copied into shading order (a policy's ranges) by the split: packed ray, closest hit and
+// job tag of each entry, so that shading reads them sequentially (HitSource with no list).
+struct HitRecords {
+    float4* a = nullptr;
+    float4* b = nullptr;
+    float4* c = nullptr;
+    int32_t* leaf = nullptr;
+    float* t = nullptr;
+    float* u = nullptr;
+    float* v = nullptr;
+    int32_t* tag = nullptr;
+};
+
 // Per-entry outputs of k_shade consumed by the shadow and compaction kernels.
 struct ShadeOut {
     float* ldx; float* ldy; float* ldz; float* lmax;   // area/environment shadow ray
@@ -585,14 +598,17 @@ wf_status shade_hits(wf_context* ctx, const wf_scene* s, const HitSource& h, Cou
                      const wf_shade_params* p, float* accum, int32_t* counts_dev,
                      const ShadeJobs& jobs, const uint32_t** alive_out, HitCols* hc_out);
 // Index-mode split of traced packed rays (src: the rays and their hit buffers; its list is not
-// used): the traced-ray index of every hit appended to hit_list (into the range of its group
-// key with ranges); every miss resolved in place (sky or environment into accum) with its
-// accumulator index and job tag appended to miss_pixel / miss_tag for path completion;
-// counts_dev[0] = hits, counts_dev[1] = misses.
+// used): the traced-ray index of every hit appended to hit_list (with ranges, into the range of
+// its group key), or, with ranges and records, the hit's ray, closest hit and tag copied into
+// its range of `records` (an order of many sparse ranges scatters each range's hits over the
+// whole ray buffer: copied once in ray order, they are read sequentially by the shading);
+// every miss resolved in place (sky or environment into accum)
+// with its accumulator index and job tag appended to miss_pixel / miss_tag for path
+// completion; counts_dev[0] = hits, counts_dev[1] = misses.
 wf_status split_index(wf_context* ctx, const wf_scene* s, const HitSource& src, Count n,
                       int32_t* hit_list, int32_t physics, float* accum, int32_t* miss_pixel,
                       int32_t* miss_tag, int32_t* counts_dev,
-                      const PolicyRanges* ranges = nullptr);
+                      const PolicyRanges* ranges = nullptr, const HitRecords* records = nullptr);
 // The shading queue of listed hits (HitRecord mapping of each), with their job tags: the
 // engine's diagnostic of an index-mode iteration.
 wf_status gather_hits(wf_context* ctx, const wf_scene* s, const HitSource& h, Count n,

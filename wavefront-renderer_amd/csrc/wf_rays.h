@@ -122,8 +122,11 @@ __device__ __forceinline__ MappedHit map_hit_vals(const SceneDev& s, const V3& o
     return h;
 }
 
-// Ray `e` of a hit source (its list entry) and the slot of its side data.
-__device__ __forceinline__ int hs_traced(const HitSource& h, int e) { return h.list[e]; }
+// Ray `e` of a hit source (its list entry; e itself for records already in shading order) and
+// the slot of its side data.
+__device__ __forceinline__ int hs_traced(const HitSource& h, int e) {
+    return h.list ? h.list[e] : e;
+}
 __device__ __forceinline__ int hs_slot(const HitSource& h, int i) {
     return h.order ? h.order[i] : i;
 }

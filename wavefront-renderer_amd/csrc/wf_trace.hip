@@ -214,9 +214,10 @@ __global__ __launch_bounds__(kAppendBlock) void k_split_append(
 // lays its hits out in LDS in range order; thread k then maps the k-th hit of that order, so
 // that consecutive lanes write consecutive slots of one run (coalesced stores) and only the
 // reads of a hit's ray and triangle follow the permutation. Misses are appended as in
-// k_split_append. INDEX: the traced-ray indices of the hits go into the ranges of hit_list and
-// the misses are resolved as in k_split_index (index mode: no queue record is written). Dynamic
-// LDS:
+// k_split_append. INDEX: each hit's traced-ray index goes into its range of hit_list, or, with
+// records (rec.a set), its packed ray, closest hit and tag are copied into its range of the
+// records (read in ray order, written in runs); the misses are resolved as in k_split_index
+// (index mode: no shading-queue record is mapped). Dynamic LDS:
 // 3 n_materials words (there are at most n_materials ranges).
 constexpr int kBucketItems = 8;
 constexpr int kBucketTile = kAppendBlock * kBucketItems;
@@ -227,8 +228,8 @@ __global__ __launch_bounds__(kAppendBlock) void k_split_buckets(
     const float* __restrict__ hv, wf_shade_soa out, wf_ray_soa miss, int32_t* __restrict__ counts,
     const int32_t* __restrict__ bucket, int32_t* __restrict__ cursor,
     const int32_t* __restrict__ tags_in, int32_t* __restrict__ hit_tags,
-    int32_t* __restrict__ miss_tags, int32_t* __restrict__ hit_list, HitSource src,
-    int physics, float* __restrict__ accum) {
+    int32_t* __restrict__ miss_tags, int32_t* __restrict__ hit_list, HitRecords rec,
+    HitSource src, int physics, float* __restrict__ accum) {
     extern __shared__ uint32_t sb_lds[];
     __shared__ uint32_t lds[kBucketItems * kAppendBlock / 64];
     __shared__ int32_t order[kBucketTile];   // ray of the k-th hit of the block, in range order
@@ -307,8 +308,20 @@ __global__ __launch_bounds__(kAppendBlock) void k_split_buckets(
         const int i = order[k];
         const int mi = order_range[k];
         const int pos = (int)(run[mi] + ((uint32_t)k - local[mi]));
-        if (INDEX) {
+        if (INDEX && !rec.a) {
             hit_list[pos] = i;
+            continue;
+        }
+        if (INDEX) {
+            const int slot = hs_slot(src, i);
+            rec.a[pos] = src.a[slot];
+            rec.b[pos] = src.b[slot];
+            rec.c[pos] = src.c[slot];
+            rec.leaf[pos] = src.leaf[i];
+            rec.t[pos] = src.t[i];
+            rec.u[pos] = src.u[i];
+            rec.v[pos] = src.v[i];
+            if (rec.tag) rec.tag[pos] = src.tags[slot];
             continue;
         }
         map_hit(s, in, i, leaf[i], ht[i], hu[i], hv[i], out, pos);
@@ -445,8 +458,8 @@ static wf_status split_append(wf_context* ctx, const wf_scene* s, const R& rays,
         k_split_buckets<R, false><<<grid_for(n.n, kBucketTile), kAppendBlock, lds,
                                     ctx->stream>>>(
             s->dev, rays, n.n, n.dev, hl, ht, hu, hv, *hits_out, *misses_out, counts_dev,
-            ranges->bucket, ranges->cursor, tags_in, hit_tags, miss_tags, nullptr, HitSource{},
-            0, nullptr);
+            ranges->bucket, ranges->cursor, tags_in, hit_tags, miss_tags, nullptr, HitRecords{},
+            HitSource{}, 0, nullptr);
         WF_HIP(hipGetLastError());
         return WF_OK;
     }
@@ -482,7 +495,8 @@ wf_status split_traced(wf_context* ctx, const wf_scene* s, const RayPackedOrdere
 
 wf_status split_index(wf_context* ctx, const wf_scene* s, const HitSource& src, Count n,
                       int32_t* hit_list, int32_t physics, float* accum, int32_t* miss_pixel,
-                      int32_t* miss_tag, int32_t* counts_dev, const PolicyRanges* ranges) {
+                      int32_t* miss_tag, int32_t* counts_dev, const PolicyRanges* ranges,
+                      const HitRecords* records) {
     // per ray: leaf 4 read, hit index 4 written; per miss also direction/pdf and
     // throughput/depth 32, accumulator index 4, tag 4 read, accumulator 12 read and written,
     // completion columns 8 written (the misses are about half of the rays)
@@ -490,7 +504,9 @@ wf_status split_index(wf_context* ctx, const wf_scene* s, const HitSource& src, 
     if (ranges) {
         WF_TRY(wf_fill_u32(ctx, counts_dev + 1, 0u, 1));
         if (n.n <= 0) return WF_OK;
-        Timed tm(ctx, "split", bytes + 4.0, n);   // + the leaf's material
+        // + the leaf's material, and with records per hit (about half) the 68-byte record read
+        // and written
+        Timed tm(ctx, "split", bytes + 4.0 + (records ? 0.5 * 2.0 * 68.0 : 0.0), n);
         const size_t lds = sizeof(uint32_t) * 3 * (size_t)s->dev.n_materials;
         wf_ray_soa miss{};
         miss.pixel = miss_pixel;
@@ -498,7 +514,7 @@ wf_status split_index(wf_context* ctx, const wf_scene* s, const HitSource& src, 
                                            ctx->stream>>>(
             s->dev, RayPacked{}, n.n, n.dev, src.leaf, nullptr, nullptr, nullptr, wf_shade_soa{},
             miss, counts_dev, ranges->bucket, ranges->cursor, nullptr, nullptr, miss_tag,
-            hit_list, src, physics, accum);
+            hit_list, records ? *records : HitRecords{}, src, physics, accum);
         WF_HIP(hipGetLastError());
         return WF_OK;
     }
