@@ -403,9 +403,9 @@ def main() -> None:
                     help="samples per pixel of the CPU baseline sample (same image size)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--jobs", type=int, default=64, help="jobs of the batched/served workloads")
-    ap.add_argument("--served-spp", type=int, default=1024,
+    ap.add_argument("--served-spp", type=int, default=2048,
                     help="per-pixel sample cap of served requests (adaptive sampling stops each "
-                         "pixel at relative standard error sqrt(1e-3) before it)")
+                         "pixel at relative standard error sqrt(1e-3) / 1.25 before it)")
     ap.add_argument("--served-capacity", type=int, default=8 << 20,
                     help="ray slots of each served scene's engine")
     ap.add_argument("--no-forms", action="store_true",

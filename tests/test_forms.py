@@ -27,7 +27,7 @@ def test_served_jobs_arrivals_and_sizes():
     assert all(64 <= j.width <= 1920 and 64 <= j.height <= 1080 for j in jobs)
     side = np.median([j.width for j in jobs])
     assert 400 < side < 650                              # lognormal around 512
-    assert all(j.adaptive and abs(j.threshold ** 2 - 1e-3) < 1e-9 for j in jobs)
+    assert all(j.adaptive and abs((1.25 * j.threshold) ** 2 - 1e-3) < 1e-9 for j in jobs)
 
 
 def test_fcfs_latency():
@@ -47,7 +47,7 @@ def test_rel_mse_and_its_trimmed_view():
     # the trimmed view drops the 0.1% worst pixels (2 of 2000): the firefly is gone
     assert np.isclose(forms.rel_mse(bad, ref, trim=1e-3), forms.rel_mse(img, ref, trim=1e-3),
                       rtol=1e-2)
-    assert forms.served_jobs(3, rate=1.0)[0].spp == 1024
+    assert forms.served_jobs(3, rate=1.0)[0].spp == 2048
 
 
 def test_hdri_sky_has_the_stated_peak_over_mean():

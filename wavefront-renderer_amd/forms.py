@@ -7,7 +7,8 @@ rendered one after another with wf_render.
 
 Served (config 4): a Poisson stream of requests against the three resident scenes (Cornell box,
 sorting stress, sphere cloud), resolution lognormal (median 512^2, clamped to 64^2..1920x1080),
-adaptive sampling towards a per-pixel relative standard error whose square is the target relMSE;
+adaptive sampling (at most 2048 spp) towards a per-pixel relative standard error of
+sqrt(target relMSE) / 1.25;
 requests are admitted as job slots and framebuffer ranges free up and their paths share the
 scene's wavefront while other requests run. Reports jobs/s, Mrays/s and p50/p99
 time-to-converged-image, next to the same requests served by sequential wf_render calls (their
@@ -59,10 +60,14 @@ def batched_jobs(n: int = 64, seed: int = 0x5EED0003) -> list:
 
 
 def served_jobs(n: int, rate: float, n_scenes: int = 3, seed: int = 0x5EED0004,
-                rel_mse: float = 1e-3, max_spp: int = 1024) -> list:
+                rel_mse: float = 1e-3, max_spp: int = 2048) -> list:
     rng = np.random.default_rng(seed)
     t = 0.0
-    thr = float(np.sqrt(rel_mse))
+    # AdaptiveSampler's per-pixel rule (relative standard error below the threshold) at
+    # sqrt(rel_mse) leaves the image at 1.3-1.5 times the target (pixels stopped by the sample
+    # cap, the noisy variance estimate, the +0.01 in the rule's denominator): the threshold is
+    # sqrt(rel_mse) / 1.25, which brings the Cornell-box and sorting-stress requests below it
+    thr = float(np.sqrt(rel_mse)) / 1.25
     jobs = []
     for i in range(n):
         t += rng.exponential(1.0 / rate)
