@@ -160,10 +160,15 @@ def serve(engines, datas, jobs, iterations: int = 2, realtime: bool = True,
           collect: bool = False) -> dict:
     """Admits jobs at their arrival times (all at once when realtime is False), runs every engine
     with work, collects finished jobs. Returns makespan, rays and per-job completion times (and,
-    with collect, every job's (image, counts, result) on the host, in job order)."""
+    with collect, every job's (image, counts, result) on the host, in job order).
+
+    Each engine is driven on its own: once its launch has been waited for, its finished jobs are
+    collected, waiting requests of its scene admitted and its next launch issued, while the other
+    engines' launches keep the GPU busy."""
     order = sorted(range(len(jobs)), key=lambda k: jobs[k].arrival)
-    pending = []           # arrived, not admitted yet (FIFO)
+    pending = [[] for _ in engines]   # arrived, not admitted yet (FIFO per scene)
     slot = [dict() for _ in engines]
+    inflight = [False] * len(engines)
     images = {}
     done = 0
     nxt = 0
@@ -172,34 +177,32 @@ def serve(engines, datas, jobs, iterations: int = 2, realtime: bool = True,
     while done < len(jobs):
         now = time.perf_counter() - t0
         while nxt < len(order) and (not realtime or jobs[order[nxt]].arrival <= now):
-            pending.append(order[nxt])
+            pending[jobs[order[nxt]].scene].append(order[nxt])
             nxt += 1
-        still = []
-        for k in pending:
-            j = jobs[k]
-            jid = engines[j.scene].try_admit(_desc(j, datas[j.scene].camera))
-            if jid is None:
-                still.append(k)
-            else:
-                slot[j.scene][jid] = k
-        pending = still
-        active = [e for i, e in enumerate(engines) if slot[i]]
-        if not active:
-            if nxt < len(order):
-                time.sleep(max(0.0, jobs[order[nxt]].arrival - (time.perf_counter() - t0)))
-            continue
-        for e in active:
-            e.launch(iterations)
         for i, e in enumerate(engines):
-            if not slot[i]:
-                continue
-            for jid in e.wait():
-                k = slot[i].pop(jid)
-                out = e.result(jid, keep_on_device=not collect)
-                if collect:
-                    images[k] = out
-                jobs[k].done = time.perf_counter() - t0
-                done += 1
+            if inflight[i]:
+                for jid in e.wait():
+                    k = slot[i].pop(jid)
+                    out = e.result(jid, keep_on_device=not collect)
+                    if collect:
+                        images[k] = out
+                    jobs[k].done = time.perf_counter() - t0
+                    done += 1
+                inflight[i] = False
+            still = []
+            for k in pending[i]:
+                j = jobs[k]
+                jid = e.try_admit(_desc(j, datas[i].camera))
+                if jid is None:
+                    still.append(k)
+                else:
+                    slot[i][jid] = k
+            pending[i] = still
+            if slot[i]:
+                e.launch(iterations)
+                inflight[i] = True
+        if not any(inflight) and done < len(jobs) and nxt < len(order):
+            time.sleep(max(0.0, jobs[order[nxt]].arrival - (time.perf_counter() - t0)))
     makespan = time.perf_counter() - t0
     rays = sum(e.stats().rays_traced + e.stats().shadow_rays for e in engines) - r0
     res = {"makespan_s": makespan, "rays": rays}
