@@ -494,7 +494,8 @@ __global__ __launch_bounds__(kPolicyCountBlock) void k_policy_count(
 // part: typeRank | costRank (material), typeRank | sizeRank(texture of the material) (texture),
 // type | priority rank (costBenefit) -- then one range per distinct key among the present
 // materials, in key order: bucket[m] = the range of material m, cursor[b] = the first slot of
-// range b. One block, thread m = material m.
+// range b. One block: thread m gathers material m, then a group of lanes per material runs the
+// loops over all materials.
 __global__ __launch_bounds__(kPolicyBucketMax) void k_policy_plan(
     int policy, int n_mat, const uint32_t* __restrict__ counts, const double* __restrict__ benefit,
     const DevMaterial* __restrict__ mats, const int32_t* __restrict__ cost_rank, int cost_bits,
@@ -506,6 +507,7 @@ __global__ __launch_bounds__(kPolicyBucketMax) void k_policy_plan(
     __shared__ uint64_t s_key[kPolicyBucketMax];
     __shared__ double s_prio[kPolicyBucketMax];
     __shared__ uint8_t s_first[kPolicyBucketMax];
+    __shared__ int32_t s_type[kPolicyBucketMax];
     __shared__ TypeStats st;
     __shared__ uint32_t s_total;
     const int m = threadIdx.x;
@@ -556,40 +558,59 @@ __global__ __launch_bounds__(kPolicyBucketMax) void k_policy_plan(
     __syncthreads();
     if (!cb && m == 0) type_order(policy, &st);
     __syncthreads();
-    uint64_t key = 0;
     if (in) {
-        if (policy == WF_POLICY_MATERIAL) {
-            key = ((uint64_t)st.rank[ty] << cost_bits) | (uint64_t)crank;
-        } else if (policy == WF_POLICY_TEXTURE) {
-            key = ((uint64_t)st.rank[ty] << size_bits) | (uint64_t)srank;
-        } else {
-            int r = 0;   // competition rank by descending priority among present materials
-            for (int k = 0; k < n_mat; ++k)
-                if (s_cnt[k] && s_prio[k] > s_prio[m]) ++r;
-            key = ((uint64_t)ty << rank_bits) | (uint64_t)r;
-        }
+        if (policy == WF_POLICY_MATERIAL)
+            s_key[m] = ((uint64_t)st.rank[ty] << cost_bits) | (uint64_t)crank;
+        else if (policy == WF_POLICY_TEXTURE)
+            s_key[m] = ((uint64_t)st.rank[ty] << size_bits) | (uint64_t)srank;
+        s_type[m] = ty;
     }
-    s_key[m] = key;
+    // The loops over all materials below run on a group of G lanes per material (G a power of
+    // two, at most a wave, G * materials <= the block): lane `sub` of group g takes the
+    // materials k = sub, sub + G, ... and the group sums its counts with shuffles (integer sums:
+    // the same result in any order).
+    int gsh = 0;
+    while ((64 >> gsh) * n_mat > kPolicyBucketMax) ++gsh;
+    const int G = 64 >> gsh;
+    const int g = m / G, sub = m % G;
+    const bool gin = g < n_mat;
+    auto group_sum = [&](uint32_t v) {
+        for (int o = 1; o < G; o <<= 1) v += __shfl_xor(v, o, 64);
+        return v;
+    };
+    if (cb) {
+        // competition rank by descending priority among present materials
+        __syncthreads();
+        uint32_t r = 0;
+        if (gin)
+            for (int k = sub; k < n_mat; k += G)
+                if (s_cnt[k] && s_prio[k] > s_prio[g]) ++r;
+        r = group_sum(r);
+        if (gin && sub == 0) s_key[g] = ((uint64_t)s_type[g] << rank_bits) | (uint64_t)r;
+    }
     __syncthreads();
     // the first present material of each key stands for its range
-    bool first = c != 0;
-    if (first)
-        for (int k = 0; k < m; ++k)
-            if (s_cnt[k] && s_key[k] == key) {
-                first = false;
-                break;
-            }
-    s_first[m] = first ? 1 : 0;
+    const uint32_t gc = gin ? s_cnt[g] : 0u;
+    const uint64_t key = gin ? s_key[g] : 0ull;
+    uint32_t before = 0;   // present materials of the same key before g
+    if (gc)
+        for (int k = sub; k < g; k += G)
+            if (s_cnt[k] && s_key[k] == key) ++before;
+    before = group_sum(before);
+    const bool first = gc != 0 && before == 0;
+    if (gin && sub == 0) s_first[g] = first ? 1 : 0;
     __syncthreads();
-    if (c) {
-        int b = 0;
-        uint32_t slot = 0;
-        for (int k = 0; k < n_mat; ++k) {
+    uint32_t b = 0, slot = 0;
+    if (gc)
+        for (int k = sub; k < n_mat; k += G) {
             if (!s_cnt[k] || s_key[k] >= key) continue;
             b += s_first[k];
             slot += s_cnt[k];
         }
-        bucket[m] = b;
+    b = group_sum(b);
+    slot = group_sum(slot);
+    if (gc && sub == 0) {
+        bucket[g] = (int32_t)b;
         if (first) cursor[b] = (int32_t)slot;
     }
     if (m == 0) *hits_dev = (int32_t)s_total;
