@@ -288,7 +288,7 @@ def autotune_batched(wf, ctx, scs, datas, physics, jobs, args) -> list:
 
 
 def run_batched(wf, ctx, scenes, args) -> dict:
-    """Config 3: 64 jobs from the Cornell-box and sorting-stress generators, sequential wf_render
+    """Batched form: 64 jobs from the Cornell-box and sorting-stress generators, sequential wf_render
     against one job-tagged engine per scene (all jobs admitted at once)."""
     from dataclasses import replace
     forms = importlib.import_module("wavefront-renderer_amd.forms")
@@ -331,7 +331,7 @@ def run_batched(wf, ctx, scenes, args) -> dict:
 
 
 def run_served(wf, ctx, scenes, args) -> dict:
-    """Config 4: Poisson requests against three resident scenes at ~0.7 of the measured
+    """Served form: Poisson requests against three resident scenes at ~0.7 of the measured
     capacity (the same requests admitted at once), adaptive sampling to a relMSE target;
     p50/p99 time-to-converged-image, against the same requests rendered one after another."""
     forms = importlib.import_module("wavefront-renderer_amd.forms")
@@ -409,7 +409,7 @@ def main() -> None:
     ap.add_argument("--served-capacity", type=int, default=8 << 20,
                     help="ray slots of each served scene's engine")
     ap.add_argument("--no-forms", action="store_true",
-                    help="skip the batched (config 3) and served (config 4) legs")
+                    help="skip the batched and served legs")
     ap.add_argument("--autotune", action="store_true",
                     help="pick the plain frame's wavefront size, sample lanes, traversal block "
                          "shape and radix digit width with the autotuner before timing")

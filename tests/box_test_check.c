@@ -1,13 +1,16 @@
-// CPU restatement of the traversal's four-wide box test (wavefront-renderer_amd/csrc/wf_traverse.h,
-// boxes4) for one child box, checked against exact arithmetic on the true ray o + t d: the test
-// must accept every box whose exact slab interval meets [0, tmax] (a dropped box could drop the
-// closest hit). Random rays and boxes, most of them built to graze the ray within a few ulps.
+// CPU restatement of the traversal's box test (wavefront-renderer_amd/csrc/wf_traverse.h: boxes4
+// of the four-wide walk; slab of the eight-wide walk applies the same rule to one box) for one
+// box, checked against exact arithmetic on the true ray o + t d: the test must accept every box
+// whose exact slab interval meets [0, tmax] (a dropped box could drop the closest hit). Random
+// rays and boxes, most of them built to graze the ray within a few ulps.
 //
 //   box_test_check <cases> <mode> <seed>
 //   mode 0: the device rule (entry planes at fma(x, inv, -next_up(o inv)), exit planes at
 //           fma(x, inv, -next_down(o inv)), exit distance times 1 + 2^-21)
 //   mode 1: the device rule without the exit-side factor
 //   mode 2: the device rule with o inv rounded to nearest on both sides (no outward step)
+//   mode 3: planes at (x - o) inv, exit distance times 1 + 2^-21 (drops the boxes whose face
+//           holds the origin of a ray with a zero direction component)
 // Prints "<cases> <exact accepts> <test accepts> <misses>".
 #include <math.h>
 #include <stdint.h>
@@ -42,6 +45,7 @@ int main(int argc, char** argv) {
     const int mode = atoi(argv[2]);
     state = 0x9E3779B97F4A7C15ull ^ (uint64_t)atoll(argv[3]);
     const float slack = mode == 1 ? 1.0f : 1.00000048f;
+    const int slab = mode == 3;
     long exact_accepts = 0, test_accepts = 0, misses = 0;
     const float scales[4] = {1000.0f, 1.0f, 0.01f, 100.0f};
     for (long it = 0; it < cases; ++it) {
@@ -83,6 +87,11 @@ int main(int argc, char** argv) {
                 const __float128 tf = ((__float128)xf - (__float128)o[a]) / (__float128)d[a];
                 if (tn > en) en = tn;
                 if (tf < ef) ef = tf;
+            }
+            if (slab) {
+                a_c = fmaxf(a_c, (xn - o[a]) * inv[a]);
+                b_c = fminf(b_c, (xf - o[a]) * inv[a]);
+                continue;
             }
             const float p = o[a] * inv[a];
             const float nup = mode == 2 ? -p : -next_up(p);

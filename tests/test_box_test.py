@@ -1,10 +1,11 @@
 """The traversal's box test is conservative against the true ray (CPU only).
 
-`box_test_check.c` restates the four-wide box test of `csrc/wf_traverse.h` (`boxes4`: one fma per
-plane with the origin term rounded outward, exit distance times 1 + 2^-21) and compares it with
-exact 113-bit arithmetic on the ray o + t d, over rays and boxes built to graze each other within
-a few ulps. The device rule must accept every box the exact interval accepts; the two plausible
-shortcuts (no exit factor, no outward rounding) must not, which shows the check can fail."""
+`box_test_check.c` restates the box test of `csrc/wf_traverse.h` (`boxes4` of the four-wide walk,
+`slab` of the eight-wide walk: one fma per plane with the origin term rounded outward, exit
+distance times 1 + 2^-21) and compares it with exact 113-bit arithmetic on the ray o + t d, over
+rays and boxes built to graze each other within a few ulps. The device rule must accept every box
+the exact interval accepts; the three plausible shortcuts (no exit factor, no outward rounding,
+plain (x - o) inv planes) must not, which shows the check can fail."""
 import shutil
 import subprocess
 from pathlib import Path
@@ -41,7 +42,7 @@ def test_device_rule_never_drops_a_box(checker, seed):
     assert accepted < 3 * exact
 
 
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("mode", [1, 2, 3])
 def test_shortcuts_drop_boxes(checker, mode):
     _, _, misses = run(checker, 2_000_000, mode, 3)
     assert misses > 100

@@ -28,11 +28,11 @@ def rel_close(a, b, tol=2e-5):
 
 
 @pytest.mark.parametrize("lanes", [1, 64])
-def test_config0_cornell_full_size(wf, ctx, scenes, orc, lanes):
+def test_cornell_256_full_size(wf, ctx, scenes, orc, lanes):
     data = scenes.cornell_box()
     P = data.params
     assert (P["width"], P["height"], P["spp"], P["max_depth"]) == (256, 256, 16, 8)
-    print(f"config 0: cornell 256x256x16 depth 8, lanes {lanes}", flush=True)
+    print(f"cornell 256x256x16 depth 8, lanes {lanes}", flush=True)
     sc = wf.Scene(ctx, data)
     p = wf.render_params(256, 256, 16, 8, data.camera, firefly=1e30, seed=1, sample_lanes=lanes)
     img, cnt, st = wf.render(sc, p)
@@ -47,12 +47,12 @@ def test_config0_cornell_full_size(wf, ctx, scenes, orc, lanes):
     sc.close()
 
 
-def test_config2_full_cloud_hits_match_oracle(wf, ctx, scenes, orc):
+def test_full_cloud_hits_match_oracle(wf, ctx, scenes, orc):
     """10.5 M triangles: device BVH (PLOC, four-wide) against the oracle's SAH BVH on camera
     rays of the workload's camera and random rays through the box."""
     data = scenes.sphere_cloud()
     assert data.triangle_count > 10_000_000
-    print(f"config 2: full cloud, {data.triangle_count} triangles", flush=True)
+    print(f"sphere cloud, full, {data.triangle_count} triangles", flush=True)
     sc = wf.Scene(ctx, data)
     osc = orc.scene(data)
     rays = orc.raygen(data.camera, 192, 108, np.arange(192 * 108), 5, 0, 0)
@@ -81,11 +81,11 @@ def small_cloud(scenes):
 
 
 @pytest.mark.parametrize("lanes", [1, 0])
-def test_config2_reduced_cloud_adaptive_costbenefit(wf, ctx, orc, small_cloud, lanes):
+def test_reduced_cloud_adaptive_costbenefit(wf, ctx, orc, small_cloud, lanes):
     """The workload's render settings -- unbiased physics, HDRI importance sampling, costBenefit,
     adaptive 16.. spp to relative standard error 0.01, depth 12 -- at 64x36 and a 256-spp cap."""
     data = small_cloud
-    print(f"config 2: reduced cloud, {data.triangle_count} triangles, lanes {lanes}", flush=True)
+    print(f"sphere cloud, reduced, {data.triangle_count} triangles, lanes {lanes}", flush=True)
     sc = wf.Scene(ctx, data)
     p = wf.render_params(64, 36, 256, 12, data.camera, policy="costBenefit", adaptive=True,
                          adaptive_threshold=0.01, adaptive_min_samples=16, adaptive_interval=16,
@@ -113,7 +113,7 @@ def lone(wf, scene, job, camera, physics, lanes):
     return wf.render(scene, p)
 
 
-def test_config3_batched_mix_equals_lone_renders(wf, ctx, scenes, orc):
+def test_batched_mix_equals_lone_renders(wf, ctx, scenes, orc):
     """A subset of the 64-job mix (the smaller jobs of each scene, budgets as drawn) admitted at
     once into one resident engine per scene, merged into job-tagged wavefronts."""
     jobs = forms.batched_jobs(64)
@@ -121,7 +121,7 @@ def test_config3_batched_mix_equals_lone_renders(wf, ctx, scenes, orc):
     for s in (0, 1):
         mine = sorted([j for j in jobs if j.scene == s], key=lambda j: j.pixels * j.spp)
         pick += mine[:5]
-    print("config 3: jobs", [(j.scene, j.width, j.spp, j.max_depth) for j in pick], flush=True)
+    print("batched jobs", [(j.scene, j.width, j.spp, j.max_depth) for j in pick], flush=True)
     datas = [scenes.cornell_box(), scenes.sorting_stress()]
     physics = [wf.PHYSICS_REFERENCE, wf.PHYSICS_UNBIASED]
     scs = [wf.Scene(ctx, d) for d in datas]
@@ -149,7 +149,7 @@ def test_config3_batched_mix_equals_lone_renders(wf, ctx, scenes, orc):
         s.close()
 
 
-def test_config4_served_stream_equals_lone_renders(wf, ctx, scenes, small_cloud):
+def test_served_stream_equals_lone_renders(wf, ctx, scenes, small_cloud):
     """Poisson arrivals against three resident scenes (Cornell box, a reduced sorting-stress
     sphere, the reduced cloud), adaptive towards relative standard error sqrt(1e-3), admitted as
     slots free up while other requests run."""
@@ -161,7 +161,7 @@ def test_config4_served_stream_equals_lone_renders(wf, ctx, scenes, small_cloud)
     jobs = forms.served_jobs(9, rate=20.0, max_spp=64, seed=0x5EED0044)
     for j in jobs:     # keep the stream short: quarter-size frames of the drawn shapes
         j.width, j.height = max(64, j.width // 4), max(64, j.height // 4)
-    print("config 4: requests", [(j.scene, j.width, j.height) for j in jobs], flush=True)
+    print("served requests", [(j.scene, j.width, j.height) for j in jobs], flush=True)
     res = forms.serve(engines, datas, jobs, realtime=True, collect=True)
     assert all(j.done >= 0 for j in jobs)
     for k, j in enumerate(jobs):

@@ -743,9 +743,10 @@ wf_status enqueue_iteration(wf_engine* e) {
         // (the misses are resolved by the split.) costBenefit ranks every material on its own,
         // so each of its ranges holds one material's hits, spread over the whole ray buffer:
         // the split copies them into their ranges of the records and shading reads them in
-        // order (395 -> ~150 us of shading for +150 us of split per iteration at config 1). The
-        // other policies' ranges are few and dense (type rank with cost or texture-size rank),
-        // and their hits are read where they are (the copy costs more than it saves there).
+        // order (395 -> ~150 us of shading for +150 us of split per iteration on the
+        // sorting-stress frame at 1080p). The other policies' ranges are few and dense (type
+        // rank with cost or texture-size rank), and their hits are read where they are (the copy
+        // costs more than it saves there).
         const bool records = bucketed && e->cfg.policy == WF_POLICY_COST_BENEFIT;
         WF_TRY(split_index(ctx, s, h, live, e->hit_list, e->cfg.shade.physics, v.accum,
                            e->miss.pixel, e->mt, c + C_HITS, bucketed ? &ranges : nullptr,

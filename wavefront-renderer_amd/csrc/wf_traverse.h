@@ -96,17 +96,24 @@ __device__ __forceinline__ float next_up(float f) {
     return __uint_as_float((int32_t)b >= 0 ? b + 1u : b - 1u);
 }
 
-// Entry distance of the box [lo, hi] along the ray, clipped to [0, tmax]; true when the
-// interval is non-empty (with a relative slack of 2 ulp so that rounding never drops a box).
+// Entry distance of the box [lo, hi] along the ray, clipped to [0, tmax]; true when the true ray
+// meets the box inside [0, tmax]. The planes are cut with the rule of boxes4 below (one fma per
+// plane with o inv rounded outward, exit distance times 1 + 2^-21). The plain (x - o) inv would
+// drop a box whose face holds the origin when the direction has a zero component: 0 * inv = 0
+// makes the exit distance 0 where the ray in fact runs inside the slab.
 __device__ __forceinline__ bool slab(float lx, float ly, float lz, float hx, float hy, float hz,
-                                     const V3& o, const V3& inv, float tmax, float& tenter) {
-    float tx0 = (lx - o.x) * inv.x, tx1 = (hx - o.x) * inv.x;
-    float ty0 = (ly - o.y) * inv.y, ty1 = (hy - o.y) * inv.y;
-    float tz0 = (lz - o.z) * inv.z, tz1 = (hz - o.z) * inv.z;
-    float t0 = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.0f));
-    float t1 = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
-    tenter = t0;
-    return t0 <= t1 * 1.00000024f;
+                                     const V3& inv, const V3& nup, const V3& ndn, float tmax,
+                                     float& tenter) {
+    // inv is never zero (safe_rcp), so its sign says which plane comes first
+    const bool fx = inv.x < 0.0f, fy = inv.y < 0.0f, fz = inv.z < 0.0f;
+    const float a = fmaxf(fmaxf(__builtin_fmaf(fx ? hx : lx, inv.x, nup.x),
+                                __builtin_fmaf(fy ? hy : ly, inv.y, nup.y)),
+                          fmaxf(__builtin_fmaf(fz ? hz : lz, inv.z, nup.z), 0.0f));
+    const float b = fminf(fminf(__builtin_fmaf(fx ? lx : hx, inv.x, ndn.x),
+                                __builtin_fmaf(fy ? ly : hy, inv.y, ndn.y)),
+                          fminf(__builtin_fmaf(fz ? lz : hz, inv.z, ndn.z), tmax));
+    tenter = a;
+    return a <= b * 1.00000048f;
 }
 
 // The four child boxes of a four-wide node: entry distance t0[k] (clipped to [0, tmax]) and
@@ -558,7 +565,8 @@ struct Walker {
             const float hy = oy + (float)(((hi4 ? qc.y : qc.x) >> b) & 0xFFu) * sy;
             const float hz = oz + (float)(((hi4 ? qc.w : qc.z) >> b) & 0xFFu) * sz;
             float t0;
-            const bool box = slab(lx, ly, lz, hx, hy, hz, o, inv, tcur, t0) && c[k] != kEmptyChild;
+            const bool box = slab(lx, ly, lz, hx, hy, hz, inv, nup, ndn, tcur, t0) &&
+                              c[k] != kEmptyChild;
             leaf_hit[k] = box && c[k] < 0;
             dist[k] = (box && c[k] >= 0) ? t0 : kInf;
         }

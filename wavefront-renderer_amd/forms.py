@@ -1,11 +1,11 @@
 """Batched and served forms over device-resident scenes (one wavefront engine per scene).
 
-Batched (config 3): independent render jobs drawn by seed from the Cornell-box and sorting-stress
+Batched: independent render jobs drawn by seed from the Cornell-box and sorting-stress
 generators (resolution log-uniform 128^2..1024^2, spp in {4, 16, 64, 256}, depth 8..12), all
 admitted at once and merged into each scene's job-tagged wavefront; compared with the same jobs
 rendered one after another with wf_render.
 
-Served (config 4): a Poisson stream of requests against the three resident scenes (Cornell box,
+Served: a Poisson stream of requests against the three resident scenes (Cornell box,
 sorting stress, sphere cloud), resolution lognormal (median 512^2, clamped to 64^2..1920x1080),
 adaptive sampling (at most 2048 spp) towards a per-pixel relative standard error of
 sqrt(target relMSE) / 1.25;
