@@ -1,6 +1,6 @@
 // Higher-quality accelerator build: PLOC clustering of the Morton-ordered leaves (Meister and
 // Bittner, "Parallel Locally-Ordered Clustering for Bounding Volume Hierarchy Construction",
-// IEEE TVCG 2018) and a greedy surface-area collapse of the binary tree into four-wide nodes.
+// IEEE TVCG 2018) and a surface-area-optimal collapse of the binary tree into wide nodes.
 //
 // PLOC, per iteration over the current clusters (in Morton order):
 //   1. every cluster finds, within `radius` positions, the cluster whose union box has the
@@ -11,11 +11,15 @@
 // Internal nodes are numbered downwards from n - 2, so the last merge (the root) is node 0: the
 // output arrays have the layout of the Karras build (left/right child: node index, or ~leaf).
 //
-// The collapse is top-down, level by level: a wide node (four or eight children) starts from the
-// two children of its binary node and keeps opening the inner child with the largest box surface
-// area until it is full or has only leaves. Nodes of a level are numbered in order after the
-// previous level, and the exact traversal stack bound is carried down the levels. Eight-wide
-// nodes store their child boxes quantised outward to 8 bits on a per-node power-of-two grid.
+// The collapse keeps a subset of the binary nodes as wide nodes (four or eight children). With one
+// triangle per leaf every leaf sits in exactly one wide node whatever the choice, so the surface
+// area heuristic reduces to the sum of the kept nodes' box areas (the expected number of node
+// visits of a random ray). A bottom-up dynamic programme over the binary tree finds the subset of
+// least total area (the slot-distribution recurrence of Ylitie, Karras and Laine, "Efficient
+// Incoherent Ray Traversal on GPUs Through Compressed Wide BVHs", HPG 2017, with the leaf term
+// constant); the top-down pass then writes the wide nodes level by level, numbered in order after
+// the previous level, carrying the exact traversal stack bound down the levels. Eight-wide nodes
+// store their child boxes quantised outward to 8 bits on a per-node power-of-two grid.
 #include <algorithm>
 
 #include "wf_device.h"
@@ -146,43 +150,124 @@ __global__ void k_ploc_write(int m, int n, const int32_t* __restrict__ created,
 
 __global__ void k_ploc_commit(int32_t* created, const int32_t* merges) { *created += *merges; }
 
-// ---- greedy surface-area collapse --------------------------------------------------------------
-__device__ __forceinline__ float ref_area(int ref, const float4* slo, const float4* shi,
-                                          const float4* nlo, const float4* nhi) {
-    return ref < 0 ? half_area(slo[~ref], shi[~ref]) : half_area(nlo[ref], nhi[ref]);
+// ---- surface-area-optimal collapse ------------------------------------------------------------
+// For a binary node m placed in j child slots of the wide node above it (1 <= j <= W):
+//   G(m, j) = min over 1 <= k < j of F(left, k) + F(right, j - k)    (m opened: its two subtrees
+//             share the j slots),
+//   F(m, 1) = A(m) + G(m, W)                                          (m kept as a wide node),
+//   F(m, j) = min(F(m, 1), G(m, j)) for j >= 2,
+// and F(leaf, j) = 0 (a leaf takes one slot wherever it goes). F(m, j) is non-increasing in j.
+// The choice word of m holds, for j = 2..W, the split k of G(m, j) (kSplitBits bits each) and,
+// above them, whether m is kept (F(m, 1) <= G(m, j)).
+template <int W>
+struct CollapseChoice {
+    static constexpr int kSplitBits = W == 4 ? 2 : 3;
+    static constexpr int kKeepBase = kSplitBits * (W - 1);
+    __device__ static int split(uint32_t w, int j) {
+        return (int)((w >> (kSplitBits * (j - 2))) & ((1u << kSplitBits) - 1u));
+    }
+    __device__ static bool keep(uint32_t w, int j) { return (w >> (kKeepBase + j - 2)) & 1u; }
+};
+
+__device__ __forceinline__ float coherent_load_f(const float* p) {
+    return __uint_as_float(__hip_atomic_load(reinterpret_cast<const uint32_t*>(p),
+                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
 }
 
-// Chooses the (up to W) children of each queued binary node; kint = inner children.
+__device__ __forceinline__ void coherent_store_f(float* p, float v) {
+    __hip_atomic_store(reinterpret_cast<uint32_t*>(p), __float_as_uint(v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Bottom-up: every leaf climbs; the second arrival at a node has both subtrees' F and computes
+// the node's (F(m, 1..W) at F[m * W + j - 1], its choice word in choice[m]).
+template <int W>
+__global__ void k_collapse_cost(int n, const int32_t* __restrict__ left,
+                                const int32_t* __restrict__ right,
+                                const int32_t* __restrict__ iparent,
+                                const int32_t* __restrict__ lparent,
+                                const float4* __restrict__ nlo, const float4* __restrict__ nhi,
+                                float* F, uint32_t* __restrict__ choice, uint32_t* visits) {
+    using CC = CollapseChoice<W>;
+    const int leaf = blockIdx.x * blockDim.x + threadIdx.x;
+    if (leaf >= n) return;
+    int p = lparent[leaf];
+    while (p >= 0) {
+        __threadfence();
+        if (atomicAdd(&visits[p], 1u) == 0u) return;   // the sibling subtree is not done yet
+        __threadfence();
+        const int cl = left[p], cr = right[p];
+        float fl[W], fr[W];
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+            fl[j] = cl < 0 ? 0.0f : coherent_load_f(F + (size_t)cl * W + j);
+            fr[j] = cr < 0 ? 0.0f : coherent_load_f(F + (size_t)cr * W + j);
+        }
+        float g[W + 1];
+        uint32_t word = 0;
+#pragma unroll
+        for (int j = 2; j <= W; ++j) {
+            float best = fl[0] + fr[j - 2];
+            int bk = 1;
+#pragma unroll
+            for (int k = 2; k < j; ++k) {
+                const float c = fl[k - 1] + fr[j - k - 1];
+                if (c < best) {
+                    best = c;
+                    bk = k;
+                }
+            }
+            g[j] = best;
+            word |= (uint32_t)bk << (CC::kSplitBits * (j - 2));
+        }
+        const float f1 = half_area(nlo[p], nhi[p]) + g[W];
+        coherent_store_f(F + (size_t)p * W, f1);
+#pragma unroll
+        for (int j = 2; j <= W; ++j) {
+            const bool keep = f1 <= g[j];
+            word |= (uint32_t)keep << (CC::kKeepBase + j - 2);
+            coherent_store_f(F + (size_t)p * W + j - 1, keep ? f1 : g[j]);
+        }
+        choice[p] = word;
+        p = iparent[p];
+    }
+}
+
+// Chooses the (up to W) children of each queued binary node by expanding its W slots along the
+// choice words; kint = inner children.
 template <int W>
 __global__ void k_collapse_pick(int count, const int32_t* __restrict__ qb,
                                 const int32_t* __restrict__ left, const int32_t* __restrict__ right,
-                                const float4* __restrict__ slo, const float4* __restrict__ shi,
-                                const float4* __restrict__ nlo, const float4* __restrict__ nhi,
-                                int32_t* __restrict__ picked, uint32_t* __restrict__ kint) {
+                                const uint32_t* __restrict__ choice, int32_t* __restrict__ picked,
+                                uint32_t* __restrict__ kint) {
+    using CC = CollapseChoice<W>;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= count) return;
     const int b = qb[t];
     int c[W];
-    c[0] = left[b];
-    c[1] = right[b];
-    for (int q = 2; q < W; ++q) c[q] = kEmptyChild;
-    int m = 2;
-    while (m < W) {
-        int best = -1;
-        float ba = -1.0f;
-        for (int k = 0; k < m; ++k)
-            if (c[k] >= 0) {
-                const float a = ref_area(c[k], slo, shi, nlo, nhi);
-                if (a > ba) {
-                    ba = a;
-                    best = k;
-                }
-            }
-        if (best < 0) break;
-        const int open = c[best];
-        c[best] = left[open];
-        c[m++] = right[open];
+    // pending (subtree, slots); each expansion replaces one entry by two whose slots sum to its
+    // own, and every entry holds at least one slot, so at most W are pending
+    int sm[W], sj[W];
+    int top = 0, m = 0;
+    const int kb = CC::split(choice[b], W);
+    sm[top] = right[b];
+    sj[top++] = W - kb;
+    sm[top] = left[b];
+    sj[top++] = kb;
+    while (top > 0) {
+        --top;
+        const int x = sm[top], j = sj[top];
+        if (x < 0 || j == 1 || CC::keep(choice[x], j)) {
+            c[m++] = x;
+            continue;
+        }
+        const int k = CC::split(choice[x], j);
+        sm[top] = right[x];
+        sj[top++] = j - k;
+        sm[top] = left[x];
+        sj[top++] = k;
     }
+    for (int q = m; q < W; ++q) c[q] = kEmptyChild;
     uint32_t k = 0;
     for (int q = 0; q < W; ++q) {
         k += (c[q] >= 0) ? 1u : 0u;
@@ -394,10 +479,19 @@ wf_status ploc_build(wf_context* ctx, int n, int radius, const float4* slo, cons
 
 template <int W>
 static wf_status collapse_levels(wf_context* ctx, int n, const int32_t* left, const int32_t* right,
-                                 int32_t* iparent, const float4* slo, const float4* shi,
-                                 const float4* nlo, const float4* nhi, Bvh4Node* out4,
-                                 Bvh8Node* out8, int32_t* n_nodes, int32_t* stack_bound) {
+                                 int32_t* iparent, const int32_t* lparent, const float4* slo,
+                                 const float4* shi, const float4* nlo, const float4* nhi,
+                                 Bvh4Node* out4, Bvh8Node* out8, int32_t* n_nodes,
+                                 int32_t* stack_bound) {
     const int ni = n - 1;
+    void *pf, *pch, *pvis;
+    WF_TRY(workspace(ctx, "col_cost", sizeof(float) * W * (size_t)ni, &pf));
+    WF_TRY(workspace(ctx, "col_choice", sizeof(uint32_t) * ni, &pch));
+    WF_TRY(workspace(ctx, "col_visits", sizeof(uint32_t) * ni, &pvis));
+    WF_TRY(wf_fill_u32(ctx, pvis, 0u, (size_t)ni));
+    k_collapse_cost<W><<<grid_for(n, 256), 256, 0, ctx->stream>>>(
+        n, left, right, iparent, lparent, nlo, nhi, (float*)pf, (uint32_t*)pch, (uint32_t*)pvis);
+    WF_HIP(hipGetLastError());
     void *pqb[2], *pqw[2], *pqd[2], *ppick, *pk, *pkoff, *pcnt;
     for (int k = 0; k < 2; ++k) {
         char name[32];
@@ -421,7 +515,7 @@ static wf_status collapse_levels(wf_context* ctx, int n, const int32_t* left, co
     while (count > 0) {
         const int nxt = cur ^ 1;
         k_collapse_pick<W><<<grid_for(count, 256), 256, 0, ctx->stream>>>(
-            count, (int32_t*)pqb[cur], left, right, slo, shi, nlo, nhi, (int32_t*)ppick,
+            count, (int32_t*)pqb[cur], left, right, (const uint32_t*)pch, (int32_t*)ppick,
             (uint32_t*)pk);
         WF_HIP(hipGetLastError());
         WF_TRY(exclusive_scan_u32(ctx, (uint32_t*)pk, (uint32_t*)pkoff, host_count(count), cnt,
@@ -445,15 +539,15 @@ static wf_status collapse_levels(wf_context* ctx, int n, const int32_t* left, co
     return WF_OK;
 }
 
-wf_status collapse_greedy(wf_context* ctx, int n, const int32_t* left, const int32_t* right,
-                          int32_t* iparent, const float4* slo, const float4* shi,
-                          const float4* nlo, const float4* nhi, Bvh4Node* out4, Bvh8Node* out8,
-                          int32_t* n_nodes, int32_t* stack_bound) {
+wf_status collapse_wide(wf_context* ctx, int n, const int32_t* left, const int32_t* right,
+                        int32_t* iparent, const int32_t* lparent, const float4* slo,
+                        const float4* shi, const float4* nlo, const float4* nhi, Bvh4Node* out4,
+                        Bvh8Node* out8, int32_t* n_nodes, int32_t* stack_bound) {
     if (out8)
-        return collapse_levels<8>(ctx, n, left, right, iparent, slo, shi, nlo, nhi, nullptr, out8,
-                                  n_nodes, stack_bound);
-    return collapse_levels<4>(ctx, n, left, right, iparent, slo, shi, nlo, nhi, out4, nullptr,
-                              n_nodes, stack_bound);
+        return collapse_levels<8>(ctx, n, left, right, iparent, lparent, slo, shi, nlo, nhi,
+                                  nullptr, out8, n_nodes, stack_bound);
+    return collapse_levels<4>(ctx, n, left, right, iparent, lparent, slo, shi, nlo, nhi, out4,
+                              nullptr, n_nodes, stack_bound);
 }
 
 }  // namespace wf

@@ -619,12 +619,12 @@ wf_status render_regenerate(wf_scene* s, const wf_render_params* rp, float* imag
 wf_status ploc_build(wf_context* ctx, int n, int radius, const float4* slo, const float4* shi,
                      int32_t* left, int32_t* right, int32_t* iparent, int32_t* lparent,
                      float4* nlo, float4* nhi);
-// Greedy surface-area collapse of a binary hierarchy (root 0) into four-wide nodes (out4) or
-// eight-wide quantised nodes (out8), whichever is given.
-wf_status collapse_greedy(wf_context* ctx, int n, const int32_t* left, const int32_t* right,
-                          int32_t* iparent, const float4* slo, const float4* shi,
-                          const float4* nlo, const float4* nhi, Bvh4Node* out4, Bvh8Node* out8,
-                          int32_t* n_nodes, int32_t* stack_bound);
+// Surface-area-optimal collapse of a binary hierarchy (root 0, parent links iparent/lparent)
+// into four-wide nodes (out4) or eight-wide quantised nodes (out8), whichever is given.
+wf_status collapse_wide(wf_context* ctx, int n, const int32_t* left, const int32_t* right,
+                        int32_t* iparent, const int32_t* lparent, const float4* slo,
+                        const float4* shi, const float4* nlo, const float4* nhi, Bvh4Node* out4,
+                        Bvh8Node* out8, int32_t* n_nodes, int32_t* stack_bound);
 wf_status ray_keys(wf_context* ctx, const wf_ray_soa* rays, Count n, const float bmin[3],
                    const float bmax[3], uint64_t* keys);
 wf_status shading_keys(wf_context* ctx, const wf_scene* s, const wf_shade_soa* q, Count n,

@@ -734,12 +734,12 @@ wf_status wf_accel_build(wf_scene* s, wf_build_stats* stats) {
             WF_TRY(readback(ctx, &depth, pdep, sizeof(int32_t)));
             if (depth >= n) return set_error(WF_ERR_HIP, "BVH build produced a malformed hierarchy");
             d.n_nodes = n - 1;
-            // greedy surface-area collapse to four- or eight-wide nodes
-            WF_TRY(collapse_greedy(ctx, n, (int32_t*)pl, (int32_t*)pr, (int32_t*)pip,
-                                   (float4*)pslo, (float4*)pshi, (float4*)pnlo, (float4*)pnhi,
-                                   d.width == 4 ? d.nodes4 : nullptr,
-                                   d.width == 8 ? d.nodes8 : nullptr, &d.n_nodes4,
-                                   &d.stack_bound));
+            // surface-area-optimal collapse to four- or eight-wide nodes
+            WF_TRY(collapse_wide(ctx, n, (int32_t*)pl, (int32_t*)pr, (int32_t*)pip,
+                                 (int32_t*)plp, (float4*)pslo, (float4*)pshi, (float4*)pnlo,
+                                 (float4*)pnhi, d.width == 4 ? d.nodes4 : nullptr,
+                                 d.width == 8 ? d.nodes8 : nullptr, &d.n_nodes4,
+                                 &d.stack_bound));
             if (d.width == 8 && d.stack_bound > kStackMax) {
                 // an eight-wide node pushes up to seven entries: fall back to four-wide nodes
                 // when the eight-wide stack bound exceeds the largest traversal instantiation
@@ -754,10 +754,10 @@ wf_status wf_accel_build(wf_scene* s, wf_build_stats* stats) {
                     WF_TRY(scene_alloc(s, sizeof(Bvh4Node) * (size_t)(n - 1), &pn4));
                     d.nodes4 = (Bvh4Node*)pn4;
                 }
-                WF_TRY(collapse_greedy(ctx, n, (int32_t*)pl, (int32_t*)pr, (int32_t*)pip,
-                                       (float4*)pslo, (float4*)pshi, (float4*)pnlo,
-                                       (float4*)pnhi, d.nodes4, nullptr, &d.n_nodes4,
-                                       &d.stack_bound));
+                WF_TRY(collapse_wide(ctx, n, (int32_t*)pl, (int32_t*)pr, (int32_t*)pip,
+                                     (int32_t*)plp, (float4*)pslo, (float4*)pshi, (float4*)pnlo,
+                                     (float4*)pnhi, d.nodes4, nullptr, &d.n_nodes4,
+                                     &d.stack_bound));
             }
         }
         d.max_depth = depth;
