@@ -711,7 +711,7 @@ __device__ __forceinline__ void shadow_walks(const SceneDev& s, const NODE* __re
                                              int32_t* __restrict__ stack, int4* __restrict__ xchg,
                                              volatile int32_t* spill) {
     using W = Walker<true, STACK>;
-    int4* wx = xchg + (threadIdx.x & ~63u);
+    int4* wx = xchg + (threadIdx.x >> 6) * kXchgPerWave;
     const V3 tl = V3(s.directional.direction[0], s.directional.direction[1],
                      s.directional.direction[2]) * -1.0f;
     W w;
@@ -752,7 +752,7 @@ __device__ __forceinline__ void shadow_walks(const SceneDev& s, const NODE* __re
         }
         // every lane of the wave takes part in the step (idle lanes test others' triangles)
         const bool walked = w.step_wave(item >= 0 && phase != 2 && s.n_tris > 0, nodes, s.tri,
-                                        stack, spill, wx);
+                                        stack, spill, wx, refill_lanes);
         if (item < 0) continue;
         if (phase != 2 && (s.n_tris <= 0 || walked)) {
             const bool occ = w.h.leaf >= 0;
@@ -795,7 +795,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(const SceneDev s, const 
                                                         int32_t* __restrict__ counter,
                                                         int refill_lanes) {
     __shared__ int32_t stack[lds_entries<STACK>() * kTraceBlock];
-    __shared__ int4 xchg[kTraceBlock];   // per wave: 64 candidate-test slots
+    __shared__ int4 xchg[kTraceBlock / 64 * kXchgPerWave];   // candidate tests and stashes
     volatile int32_t spill[Walker<true, STACK>::X > 0 ? Walker<true, STACK>::X : 1];
     shadow_walks<NODE, STACK>(s, nodes, q, o, list, load_count(count_host, count_dev), accum,
                               counter, refill_lanes, stack, xchg, spill);
@@ -814,7 +814,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_closest_shadow(
     int shadow_host, const int32_t* __restrict__ shadow_dev, float* __restrict__ accum,
     int32_t* __restrict__ shadow_counter, int shadow_refill_lanes) {
     __shared__ int32_t stack[lds_entries<STACK>() * kTraceBlock];
-    __shared__ int4 xchg[kTraceBlock];   // per wave: 64 candidate-test slots
+    __shared__ int4 xchg[kTraceBlock / 64 * kXchgPerWave];   // candidate tests and stashes
     volatile int32_t spill[Walker<false, STACK>::X > 0 ? Walker<false, STACK>::X : 1];
     closest_walks<NODE, STACK>(nodes, s.tri, s.n_tris, rays, load_count(n_host, nd), 0.001f,
                                1e9f, out_leaf, out_t, out_u, out_v, nullptr, ray_counter,

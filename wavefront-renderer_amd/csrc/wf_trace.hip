@@ -25,7 +25,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_closest(
     float* __restrict__ out_v, uint32_t* __restrict__ out_flag, int32_t* __restrict__ counter,
     int refill_lanes) {
     __shared__ int32_t stack[lds_entries<STACK>() * kTraceBlock];
-    __shared__ int4 xchg[kTraceBlock];   // per wave: 64 candidate-test slots
+    __shared__ int4 xchg[kTraceBlock / 64 * kXchgPerWave];   // candidate tests and stashes
     volatile int32_t spill[Walker<false, STACK>::X > 0 ? Walker<false, STACK>::X : 1];
     closest_walks<NODE, STACK>(nodes, tri, n_tris, rays, load_count(n_host, nd), tnear, tfar,
                                out_leaf, out_t, out_u, out_v, out_flag, counter, refill_lanes,

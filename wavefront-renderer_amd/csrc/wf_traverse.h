@@ -194,6 +194,14 @@ __device__ __forceinline__ bool mt_all(const V3& o, const V3& d, const V3& v0, c
     return det_ok && u_ok && v_ok && t_ok;
 }
 
+// Candidate triangles a lane of the cooperative walk holds between test rounds, and the wave's
+// LDS for that walk: 64 int4 job/result slots, then kStash int32 per lane.
+constexpr int kStash = 6;
+constexpr int kXchgPerWave = 64 + kStash * 64 / 4;
+// a lane visits a node only with room for its four leaves, and the wave sums the stash counts
+// from three ballot bits
+static_assert(kStash >= 4 && kStash <= 7, "stash counts must fit three bits");
+
 // Resumable walk over the four-wide BVH. At each node the four child boxes are tested; leaf
 // children are tested at once, the hit inner children are visited nearest first (the others are
 // pushed far to near). ANY stops at the first accepted triangle. The stack never holds more than
@@ -210,6 +218,7 @@ struct Walker {
     float tnear, tfar;
     TraceHit h;
     int node, sp;
+    int pc = 0;    // candidates in the stash, not tested yet (cooperative walk)
 
     __device__ __forceinline__ void start(const V3& ro, const V3& rd, float t0, float t1) {
         o = ro;
@@ -227,6 +236,7 @@ struct Walker {
         h.nodes = h.tris = 0;
         node = 0;
         sp = 0;
+        pc = 0;
     }
 
     // Processes one node; returns true when the walk is over (h holds the result).
@@ -328,52 +338,53 @@ struct Walker {
 
     // Wave-cooperative step over a four-wide node, called by all 64 lanes of the wave together
     // (active = this lane has a walk in progress). The box tests are per lane as in step(); the
-    // hit inner children are ordered and pushed right away, and the candidate triangles of the
-    // whole wave are then compacted into consecutive slots and tested one (ray, triangle) pair
-    // per lane, 64 pairs per round, instead of four masked tests per lane: a lane with no
-    // candidate helps test the others' (the owner's ray comes by lane shuffle). xchg: this wave's
-    // 64 LDS slots. Results are the same as step(): every
-    // candidate is tested with the same operations and the closest hit is the (t, global
-    // triangle index) minimum; a slot's result carries the leaf index, and the global indices
-    // are read from the triangle array only when two accepted t are equal.
+    // hit inner children are ordered and pushed right away and the nearest becomes the lane's
+    // next node, and the candidate triangles (leaf children whose box the ray enters) go to the
+    // lane's stash; a lane whose stash could not take another node's four leaves pauses. The
+    // stashed candidates of the whole wave are tested together, one (ray, triangle) pair per
+    // lane and 64 pairs per pass, in a round that runs only when it is due: a full pass is
+    // pending, no lane is walking, or the lanes not walking (paused, waiting for the results of
+    // a finished walk, or idle) reach due_lanes (the refill threshold, so that finished lanes
+    // can be refilled). Between rounds a lane keeps walking with the closest hit it had: the
+    // boxes it enters are a superset of those it would enter with the tested candidates
+    // applied, so the candidates tested are too, and the closest hit, the (t, global triangle
+    // index) minimum over them, is the same as step()'s. A lane with no candidate in a round
+    // helps test the others' (the owner's ray comes by lane shuffle). xchg: this wave's
+    // kXchgPerWave LDS int4 (64 job and result slots, then the stash, kStash entries per lane
+    // in lane columns). An any-hit walk ends at the round that accepts one of its candidates.
     __device__ __forceinline__ bool step_wave(bool active, const Bvh4Node* __restrict__ nodes,
                                               const float4* __restrict__ tri,
                                               int32_t* __restrict__ stack, volatile int32_t* spill,
-                                              int4* __restrict__ xchg) {
+                                              int4* __restrict__ xchg, int due_lanes) {
         const int lane = threadIdx.x;
         const int wl = (int)__lane_id();
         const float kInf = __int_as_float(0x7F800000);
-        int c[4] = {kEmptyChild, kEmptyChild, kEmptyChild, kEmptyChild};
-        uint32_t lm = 0;   // leaf children whose box the ray enters
-        int next = -1;     // the nearest hit inner child (visited next)
-        if (active) {
+        int32_t* stash = reinterpret_cast<int32_t*>(xchg + 64);
+        if (active && node >= 0 && pc <= kStash - 4) {
             if (STATS) ++h.nodes;
             const char* nb = reinterpret_cast<const char*>(nodes);
             const uint32_t off = (uint32_t)node * (uint32_t)sizeof(Bvh4Node);
             const int4 ch = *reinterpret_cast<const int4*>(nb + (size_t)off + 96);
-            c[0] = ch.x; c[1] = ch.y; c[2] = ch.z; c[3] = ch.w;
+            int c[4] = {ch.x, ch.y, ch.z, ch.w};
             const float tcur = ANY ? tfar : h.t;
             float t0[4];
             bool box[4];
             boxes4(nb, off, inv, nup, ndn, tcur, t0, box);
             float dist[4];
-            int ic[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                if (box[k] && c[k] < 0) lm |= 1u << k;
+                if (box[k] && c[k] < 0) stash[(pc++) * 64 + wl] = ~c[k];
                 dist[k] = (box[k] && c[k] >= 0) ? t0[k] : kInf;
-                ic[k] = c[k];
             }
-            // order the inner children nearest first (misses and leaves sort last as +inf); the
-            // triangle tests below cannot change which of them are visited
+            // order the inner children nearest first (misses and leaves sort last as +inf)
             auto cswap = [&](int x, int y) {
                 if (dist[y] < dist[x]) {
                     const float td = dist[x];
                     dist[x] = dist[y];
                     dist[y] = td;
-                    const int tc = ic[x];
-                    ic[x] = ic[y];
-                    ic[y] = tc;
+                    const int tc = c[x];
+                    c[x] = c[y];
+                    c[y] = tc;
                 }
             };
             if (!ANY) {
@@ -390,28 +401,37 @@ struct Walker {
 #pragma unroll
             for (int k = 3; k >= 1; --k)
                 if (dist[k] < kInf) {
-                    if (X == 0 || sp < L) stack[sp * kTraceBlock + lane] = ic[k];
-                    else spill[sp - L] = ic[k];
+                    if (X == 0 || sp < L) stack[sp * kTraceBlock + lane] = c[k];
+                    else spill[sp - L] = c[k];
                     ++sp;
                 }
-            if (dist[0] < kInf) next = ic[0];
+            if (dist[0] < kInf) {
+                node = c[0];
+            } else if (sp > 0) {
+                --sp;
+                node = (X == 0 || sp < L) ? stack[sp * kTraceBlock + lane] : spill[sp - L];
+            } else {
+                node = -1;   // walk over once the stashed candidates are tested
+            }
         }
-        // slots of this lane's candidates: [off, off + nc) of T in the wave
-        const int nc = __popc(lm);
+        // pending candidates: this lane's go to slots [off, off + pc) of T in the wave
         const uint64_t lt = lanemask_lt();
-        const uint64_t b0 = __ballot(nc & 1), b1 = __ballot(nc & 2), b2 = __ballot(nc & 4);
-        const int off = __popcll(b0 & lt) + 2 * __popcll(b1 & lt) + 4 * __popcll(b2 & lt);
+        const uint64_t b0 = __ballot(pc & 1), b1 = __ballot(pc & 2), b2 = __ballot(pc & 4);
         const int T = __popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2);
-        if (T > 0) {
+        const bool walking = active && node >= 0 && pc <= kStash - 4;
+        const int free_lanes = __popcll(__ballot(!walking));
+        const bool due = T > 0 && (T >= 64 || __ballot(walking) == 0 || free_lanes >= due_lanes);
+        if (due) {
+            const int off = __popcll(b0 & lt) + 2 * __popcll(b1 & lt) + 4 * __popcll(b2 & lt);
             for (int base = 0; base < T; base += 64) {
-                // leaf child k of this lane goes to slot off + (leaf children before k)
+                // stash entry k of this lane goes to slot off + k
                 {
-                    int sl = off - base;
+                    const int s0 = off - base;
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        if (!((lm >> k) & 1u)) continue;
-                        if (sl >= 0 && sl < 64) xchg[sl] = make_int4(wl, ~c[k], 0, 0);
-                        ++sl;
+                    for (int k = 0; k < kStash; ++k) {
+                        const int sl = s0 + k;
+                        if (k < pc && sl >= 0 && sl < 64)
+                            xchg[sl] = make_int4(wl, stash[k * 64 + wl], 0, 0);
                     }
                 }
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -440,9 +460,9 @@ struct Walker {
                 // only decides which accepted triangle an any-hit walk reports)
                 const int s0 = off - base;
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
+                for (int k = 0; k < kStash; ++k) {
                     const int slot = s0 + k;
-                    if (!(k < nc && slot >= 0 && slot < 64)) continue;
+                    if (!(k < pc && slot >= 0 && slot < 64)) continue;
                     if (STATS) ++h.tris;
                     const int4 r = xchg[slot];
                     if (r.x < 0) continue;
@@ -470,24 +490,18 @@ struct Walker {
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
                 __builtin_amdgcn_wave_barrier();
             }
+            pc = 0;
         }
         if (!active) return false;
         if (ANY && h.leaf >= 0) return true;
-        if (next >= 0) {
-            node = next;
-            return false;
-        }
-        if (sp == 0) return true;
-        --sp;
-        node = (X == 0 || sp < L) ? stack[sp * kTraceBlock + lane] : spill[sp - L];
-        return false;
+        return node < 0 && pc == 0;
     }
 
     // Eight-wide nodes keep the per-lane step (their leaves are tested in groups of four).
     __device__ __forceinline__ bool step_wave(bool active, const Bvh8Node* __restrict__ nodes,
                                               const float4* __restrict__ tri,
                                               int32_t* __restrict__ stack, volatile int32_t* spill,
-                                              int4* __restrict__) {
+                                              int4* __restrict__, int) {
         return active ? step(nodes, tri, stack, spill) : false;
     }
 
@@ -704,7 +718,7 @@ __device__ __forceinline__ void closest_walks(const NODE* __restrict__ nodes,
                                               int32_t* __restrict__ stack, int4* __restrict__ xchg,
                                               volatile int32_t* spill) {
     using W = Walker<false, STACK>;
-    int4* wx = xchg + (threadIdx.x & ~63u);
+    int4* wx = xchg + (threadIdx.x >> 6) * kXchgPerWave;
     W w;
     int ray = -1;
     WorkQueue wq;
@@ -722,7 +736,8 @@ __device__ __forceinline__ void closest_walks(const NODE* __restrict__ nodes,
             continue;
         }
         // every lane of the wave takes part in the step (idle lanes test others' triangles)
-        const bool done = w.step_wave(ray >= 0 && n_tris > 0, nodes, tri, stack, spill, wx);
+        const bool done = w.step_wave(ray >= 0 && n_tris > 0, nodes, tri, stack, spill, wx,
+                                      refill_lanes);
         if (ray >= 0 && (n_tris <= 0 || done)) {
             out_leaf[ray] = w.h.leaf;
             out_t[ray] = w.h.t;
