@@ -614,7 +614,22 @@ def main() -> None:
         # the same frame with every bounce's rays ordered by the reference's full 33-bit key
         saved_bits = ctx.tuning()["ray_key_bits"]
         ctx.set_tuning(ray_key_bits=33)
+        # its first frame with events around every kernel: the rows of the ray-order kernels
+        # (keys, sort passes), which the frame without ray ordering does not run
+        ctx.synchronize()
+        ctx.profile_filter(None)
+        ctx.profile(True)
         frame(args.policy, seed=6, ray_sort=True)
+        sort_rows = {}
+        for name in ctx.profile_names():
+            ms, launches, nbytes = ctx.profile_query(name)
+            if launches == 0 or name not in ("ray_keys", "ray_sort"):
+                continue
+            gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+            sort_rows[name] = {"ms_per_frame": round(ms, 3), "launches_per_frame": launches,
+                               "avg_us": round(ms * 1e3 / launches, 2), "GB_s": round(gbs, 1),
+                               "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        ctx.profile(False)
         ctx.synchronize()
         t1 = time.perf_counter()
         st = frame(args.policy, seed=5, ray_sort=True)
@@ -622,7 +637,8 @@ def main() -> None:
         dt = time.perf_counter() - t1
         ctx.set_tuning(ray_key_bits=saved_bits)
         full_key = {"ray_key_bits": 33, "ms": round(dt * 1e3, 1),
-                    "mrays_s": round((st.rays_traced + st.shadow_rays) / dt / 1e6, 2)}
+                    "mrays_s": round((st.rays_traced + st.shadow_rays) / dt / 1e6, 2),
+                    "kernels": sort_rows}
         log(f"[bench] full ray key: {full_key}")
 
     policies = {}
