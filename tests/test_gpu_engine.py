@@ -390,3 +390,23 @@ def test_autotune_render_and_engine(wf, ctx, cornell):
         assert eres["best"]["capacity"] in (1024, 4096) and eres["seconds"] > 0
     finally:
         ctx.set_tuning(**saved)
+
+
+def test_context_close_with_live_scene_and_engine(wf, scenes):
+    """Closing a context while a scene and an engine made on it are alive destroys the engine
+    first, then the scene, then the context; their own close() (and the finalisers) are then
+    no-ops, and a render on the closed scene is refused rather than run on freed memory."""
+    c = wf.Context(0)
+    data = scenes.cornell_box()
+    sc = wf.Scene(c, data)
+    eng = wf.Engine(c, sc, capacity=512, pixel_budget=1000, max_jobs=2)
+    j = eng.admit(wf.job_desc(8, 8, 1, 2, data.camera, seed=3))
+    while j not in eng.run():
+        pass
+    c.close()
+    assert c.handle is None and sc.handle is None and eng.handle is None
+    sc.close()
+    eng.close()
+    c.close()
+    with pytest.raises(wf.WavefrontError):
+        wf.render(sc, wf.render_params(8, 8, 1, 2, data.camera))
